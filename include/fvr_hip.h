@@ -1,0 +1,188 @@
+/*
+ * fvr_hip.h -- C ABI of libfvr_hip.so, the MI355X (gfx950) kernels behind the
+ * Diff-FVR rendering hot path.
+ *
+ * Conventions for every entry point:
+ *   - device buffers are caller-owned (passed as plain device addresses);
+ *   - calls are stream-ordered and asynchronous: no host synchronisation and
+ *     no allocation inside a launch entry point (graph-capturable);
+ *   - the return value is an fvr_status; on failure fvr_last_error() returns a
+ *     thread-local message.  No C++ exception or abort crosses the ABI.
+ *
+ * Reference interfaces replaced (pareelamre/diff-fvr, src/fvr/):
+ *   fvr_sncc_eval       diff_fvr.py:106-150 extract_frequency_slice +
+ *                       diff_fvr.py:184-195 compute_spectral_ncc +
+ *                       gradients.py:27-135 (loss and 6-DoF gradient), and
+ *                       torch_engine.py:176-307 (_extract_slice_spectrum,
+ *                       _spectral_ncc_core) with the pose adjoint that
+ *                       triton_slice.py:146-339 provides to autograd.
+ *   fvr_spec_pack       the storage step of diff_fvr.py:22-54 /
+ *                       torch_engine.py:76-88 (centred spectrum -> device
+ *                       layout, optional FP16/BF16 with per-brick scale).
+ *   fvr_slice_fwd/bwd   triton_slice.py:648-661 fourier_slice_3d (forward and
+ *                       the u/v gradient of triton_slice.py:306-339).
+ *   fvr_warp_fwd/bwd    triton_slice.py:787-808 pou_pinhole_warp_sum (forward
+ *                       and the imgs/d_b/cu/cv gradients of :548-591).
+ *   fvr_pou_slice_ifft  cone_beam_fvr.py:480-505 _slice_and_ifft.
+ *   fvr_pou_slice_ifft_adj  its adjoint (d/du_det, d/dv_det).
+ */
+#ifndef FVR_HIP_H
+#define FVR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FVR_ABI_VERSION 1
+
+typedef enum fvr_status {
+  FVR_OK = 0,
+  FVR_ERR_INVALID = 1,     /* bad argument (shape, size, null pointer) */
+  FVR_ERR_HIP = 2,         /* HIP runtime error (launch / copy) */
+  FVR_ERR_UNSUPPORTED = 3  /* valid request this build does not implement */
+} fvr_status;
+
+/* element type of a packed spectrum */
+typedef enum fvr_dtype {
+  FVR_C32 = 0,  /* complex64 interleaved (re, im) */
+  FVR_C16F = 1, /* fp16 (re, im) pairs, one fp32 power-of-two scale per 8^3 brick */
+  FVR_C16B = 2  /* bf16 (re, im) pairs, one fp32 power-of-two scale per 8^3 brick */
+} fvr_dtype;
+
+/* A packed spectrum: logical dims (slow, mid, fast) stored with a one-voxel
+ * zero margin below and zero margins above every axis, so that out-of-range
+ * trilinear corners read 0.  Storage index = logical index + 1.
+ * Storage dims: slow n0+3, mid n1+2, fast n2+2 (zero rows/planes beyond). */
+typedef struct fvr_layout {
+  uint64_t base;        /* device address of storage element (0,0,0) */
+  uint64_t scales;      /* device address of float brick scales (0 for FVR_C32) */
+  int32_t n[3];         /* logical dims (slow, mid, fast) */
+  int32_t dtype;        /* fvr_dtype */
+  int32_t sdim[3];      /* storage dims */
+  int32_t bricks[3];    /* brick counts over storage dims (8^3 bricks) */
+  uint32_t elem_bytes;  /* 8 (C32) or 4 (C16F, C16B) */
+  uint32_t row_bytes;   /* sdim[2] * elem_bytes */
+  uint32_t plane_bytes; /* sdim[1] * row_bytes */
+  uint32_t pad0;
+  uint64_t total_bytes; /* sdim[0] * plane_bytes */
+} fvr_layout;
+
+/* Fill a layout descriptor for dims n (slow, mid, fast) and dtype; returns the
+ * storage bytes and, for scaled dtypes, the scale-array element count. */
+int fvr_layout_init(fvr_layout* lay, const int32_t n[3], int32_t dtype,
+                    uint64_t* storage_bytes, uint64_t* n_scales);
+
+/* Pack a dense complex64 spectrum src[(z*ny + y)*nx + x] (interleaved re, im)
+ * into lay's storage.  transpose_xy = 0: layout axes (z, y, x); 1: (z, x, y).
+ * lay->base (and lay->scales) must point at caller buffers of the sizes
+ * fvr_layout_init returned.  Writes every storage element (margins = 0). */
+int fvr_spec_pack(const fvr_layout* lay, uint64_t src, int32_t nz, int32_t ny, int32_t nx,
+                  int32_t transpose_xy, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Fused Spectral-NCC forward + pose adjoint over a batch of work items.      */
+/* ------------------------------------------------------------------------ */
+
+/* item flags */
+#define FVR_ITEM_ADJOINT 1u     /* accumulate the pose-adjoint moments */
+#define FVR_ITEM_HP 2u          /* apply the separable high-pass weight (convention T) */
+#define FVR_ITEM_WRITE_SLICE 4u /* also store the phase-modulated slice S (complex64) */
+
+/* One (pose, view) evaluation.  Sample (row i, col j) sits at
+ *   p = c + j * ustep + i * vstep        (32.32 fixed point, layout axes,
+ *                                         already shifted by -box_lo)
+ * and is inside iff 0 <= p[a] <= box_hi[a] for every axis a; inside samples
+ * read storage corners floor(p)+lo_int .. floor(p)+lo_int+1.  The phase is
+ *   exp(-2 pi i * (ph_col0 + j*ph_col + ph_row0 + i*ph_row) / 2^32).
+ * Moment weights are cw(j) = cw0 + j*cws and rw(i) = rw0 + i*rws. */
+typedef struct fvr_sncc_item {
+  int64_t c[3];
+  int64_t ustep[3];
+  int64_t vstep[3];
+  uint64_t box_hi[3];
+  uint32_t ph_col0, ph_col, ph_row0, ph_row;
+  float cw0, cws, rw0, rws;
+  float hp_inv2s2;      /* 1/(2 sigma^2) of the high-pass weight */
+  float hp_x0, hp_dx;   /* column coordinate x(j) = hp_x0 + j*hp_dx */
+  float hp_y0, hp_dy;   /* row coordinate y(i) = hp_y0 + i*hp_dy */
+  int32_t h, w;         /* slice rows, cols */
+  int32_t layout;       /* index into the layouts array */
+  int32_t lo_int;       /* storage offset of floor(p) (1 = scipy box, 0 = per-corner) */
+  uint32_t guard_off;   /* byte offset (from layout base) of a zero region */
+  int32_t tile_base;    /* global index of this item's first tile */
+  int32_t tiles_x;      /* tiles per slice row */
+  uint32_t flags;
+  uint64_t target;      /* device address of the complex64 target (h x w, row-major) */
+  uint64_t slice_out;   /* device address for S (FVR_ITEM_WRITE_SLICE), else 0 */
+} fvr_sncc_item;
+
+#define FVR_SNCC_NMOM 16 /* partial sums per tile */
+
+/* Tile geometry of fvr_sncc_eval: 64 columns x fvr_sncc_tile_rows() rows. */
+int fvr_sncc_tile_rows(void);
+
+/* Launch the fused kernel over n_tiles tiles (the sum of items' tile counts).
+ * variant = dtype | (flags << 4); every item of one launch shares it.
+ * partials: float[n_tiles][FVR_SNCC_NMOM] (caller buffer). */
+int fvr_sncc_eval_v(const fvr_layout* layouts_dev, const fvr_sncc_item* items_dev, int32_t n_items,
+                    int32_t n_tiles, float* partials, uint32_t variant, void* stream);
+
+/* Per-item reduction of the tile partials in fixed order (float64):
+ * out[item][FVR_SNCC_NMOM] as double, ordered
+ *   A, E, Mc_im, Mr_im, Mc_g[3], Mr_g[3], Mc_h[3], Mr_h[3]. */
+int fvr_sncc_reduce(const fvr_sncc_item* items_dev, int32_t n_items, const float* partials,
+                    double* out, void* stream);
+
+/* Convention-N pose set-up on the device: builds 2 items per pose (AP, LAT)
+ * from float64 poses (yaw, pitch, roll in degrees; tx, ty, tz in voxels) and
+ * the float64 Jacobians jac[pose][view][2][3][3] (du/dtheta, dv/dtheta in
+ * layout axes, per degree).  tmpl[2] are the AP and LAT item templates whose
+ * geometry fields are overwritten. */
+int fvr_pn_build_items(const double* poses, int32_t n_poses, const fvr_sncc_item* tmpl_dev,
+                       const double* pad_dims, fvr_sncc_item* items_dev, double* jac,
+                       int32_t lat_transposed, void* stream);
+
+/* Convention-N biplanar combine: from reduce output (2 items per pose, AP then
+ * LAT) and the target energies t2[2], writes loss[pose] and grad[pose][6] as
+ * float64 (L = 1/2(1-NCC_AP) + 1/2(1-NCC_LAT)).  slice_hw[view][2] = (H, W). */
+int fvr_pn_combine(const double* red, const double* jac, int32_t n_poses, const double* t2,
+                   const double* slice_hw, double* loss, double* grad, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Generic fourier_slice_3d (grid_sample semantics, planar (B,2,D,H,W)).      */
+/* ------------------------------------------------------------------------ */
+/* vol dtype: 0 float32, 1 float16, 2 bfloat16; out has the vol dtype. */
+int fvr_slice_fwd(uint64_t vol, int32_t vol_dtype, int32_t b_vol, int32_t d, int32_t h, int32_t w,
+                  const float* u_xyz, const float* v_xyz, int32_t n_pose, const float* fx,
+                  const float* fy, int32_t k, float cx, float cy, float cz, uint64_t out,
+                  void* stream);
+/* grad_out has the vol dtype; gu/gv: float[n_pose][3]; part: float workspace of
+ * fvr_slice_bwd_workspace(n_pose, k) floats. */
+int64_t fvr_slice_bwd_workspace(int32_t n_pose, int32_t k);
+int fvr_slice_bwd(uint64_t vol, int32_t vol_dtype, int32_t b_vol, int32_t d, int32_t h, int32_t w,
+                  uint64_t grad_out, const float* u_xyz, const float* v_xyz, int32_t n_pose,
+                  const float* fx, const float* fy, int32_t k, float cx, float cy, float cz,
+                  float* part, float* gu, float* gv, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* misc                                                                      */
+/* ------------------------------------------------------------------------ */
+int fvr_abi_version(void);
+const char* fvr_last_error(void);
+/* out[0] = sizeof(fvr_layout), out[1] = sizeof(fvr_sncc_item), out[2..3] = 0 */
+int fvr_struct_sizes(int32_t out[4]);
+/* *out = sum |x[i]|^2 (float64, fixed order) of a complex64 device array */
+int fvr_c64_norm2(uint64_t x, int64_t n, double* out, void* stream);
+/* out[5] = [Re sum a conj(b), sum |a|^2, sum |b|^2, sum uu Im(a conj b), sum vv Im(a conj b)]
+ * over two complex64 (h x w) row-major device slices, uu = j - w/2, vv = i - h/2
+ * (float64, fixed order).  Replaces diff_fvr.py:184-195 compute_spectral_ncc and
+ * the sums of gradients.py:27-74 compute_spectral_ncc_and_grad_translation. */
+int fvr_c64_pair_moments(uint64_t a, uint64_t b, int32_t h, int32_t w, double* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FVR_HIP_H */

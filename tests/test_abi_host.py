@@ -1,0 +1,146 @@
+"""CPU checks of the C ABI library and of the host-side logic (no GPU compute)."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "fvr_hip.h")
+
+
+def _declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(fvr_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    from fvr import _lib
+    L = _lib.lib()
+    names = _declared_functions()
+    assert len(names) >= 10
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert L.fvr_abi_version() == _lib.ABI_VERSION
+
+
+def test_python_bindings_cover_the_header():
+    from fvr import _lib
+    assert set(_declared_functions()) <= set(_lib._SIGNATURES), \
+        set(_declared_functions()) - set(_lib._SIGNATURES)
+
+
+def test_struct_sizes_match_numpy_mirrors():
+    from fvr import _lib
+    L = _lib.lib()
+    out = (ctypes.c_int32 * 4)()
+    assert L.fvr_struct_sizes(out) == 0
+    assert out[0] == _lib.LAYOUT_DTYPE.itemsize == 80
+    assert out[1] == _lib.ITEM_DTYPE.itemsize == 200
+
+
+def test_layout_init_and_error_reporting():
+    from fvr import _lib
+    L = _lib.lib()
+    lay = np.zeros(1, dtype=_lib.LAYOUT_DTYPE)
+    nb, ns = ctypes.c_uint64(), ctypes.c_uint64()
+    n = np.array([512, 512, 512], dtype=np.int32)
+    assert L.fvr_layout_init(lay.ctypes.data, n.ctypes.data, 0, ctypes.byref(nb), ctypes.byref(ns)) == 0
+    assert tuple(lay["sdim"][0]) == (515, 514, 514)
+    assert lay["row_bytes"][0] == 514 * 8 and lay["plane_bytes"][0] == 514 * 514 * 8
+    assert nb.value == 515 * 514 * 514 * 8 and ns.value == 0
+    assert L.fvr_layout_init(lay.ctypes.data, n.ctypes.data, 1, ctypes.byref(nb), ctypes.byref(ns)) == 0
+    assert ns.value == 65 * 65 * 65 and lay["elem_bytes"][0] == 4
+    # too large for 32-bit offsets -> error with a message, no abort
+    big = np.array([1024, 1024, 1024], dtype=np.int32)
+    assert L.fvr_layout_init(lay.ctypes.data, big.ctypes.data, 0, ctypes.byref(nb), ctypes.byref(ns)) == 1
+    assert b"too large" in L.fvr_last_error()
+    with pytest.raises(_lib.FvrNativeError):
+        _lib.check(1, "probe")
+
+
+def test_entry_points_reject_bad_arguments_without_a_gpu():
+    from fvr import _lib
+    L = _lib.lib()
+    assert L.fvr_sncc_eval_v(None, None, 0, 0, None, 0, None) == 1
+    assert b"null" in L.fvr_last_error()
+    assert L.fvr_slice_fwd(0, 0, 1, 1, 1, 1, None, None, 1, None, None, 1, 0.0, 0.0, 0.0, 0, None) == 1
+
+
+def test_product_path_refuses_cpu():
+    from fvr import _lib
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    with pytest.raises(RuntimeError):
+        _lib.default_device()
+    with pytest.raises(RuntimeError):
+        _lib.require_device(torch.zeros(3), "probe")
+    from fvr import DifferentiableFourierVolumeRenderer
+    with pytest.raises(RuntimeError):
+        DifferentiableFourierVolumeRenderer(np.zeros((8, 8, 8)))
+
+
+def test_item_builder_identity_geometry():
+    """Host item geometry at the identity pose equals the integer slice grid (diff_fvr.py:128-139)."""
+    from fvr import _items
+
+    class FakePacked:
+        shape = (16, 20, 24)
+        index = {"zyx": 0, "zxy": 1}
+
+        def layout_for(self, view):
+            return (1, (0, 2, 1)) if view == "lateral" else (0, (0, 1, 2))
+
+        def desc(self, k):
+            d = np.zeros(1, dtype=__import__("fvr")._lib.LAYOUT_DTYPE)[0]
+            n = [16, 24, 20] if k == 1 else [16, 20, 24]
+            d["n"] = n
+            d["sdim"] = [n[0] + 3, n[1] + 2, n[2] + 2]
+            d["plane_bytes"] = 1000
+            return d
+
+    it = _items.n_item(FakePacked(), "ap", 0, 0, 0, 0, 0, 0, 0, 0, 64)[0]
+    assert (it["h"], it["w"]) == (16, 24)
+    # c = (pz/2, py/2, px/2) + uu0*u + vv0*v with u = x, v = z, uu0 = -12, vv0 = -8
+    np.testing.assert_array_equal(it["c"], np.array([0, 10, 0]) << 32)
+    np.testing.assert_array_equal(it["ustep"], np.array([0, 0, 1]) << 32)
+    np.testing.assert_array_equal(it["vstep"], np.array([1, 0, 0]) << 32)
+    lat = _items.n_item(FakePacked(), "lateral", 0, 0, 0, 0, 0, 0, 0, 0, 64)[0]
+    # lateral uses the (z, x, y) layout: u = y is the fast axis
+    np.testing.assert_array_equal(lat["ustep"], np.array([0, 0, 1]) << 32)
+    np.testing.assert_array_equal(lat["c"], np.array([0, 12, 0]) << 32)
+    assert lat["box_hi"][2] == (20 - 1) << 32
+    # phase turns: a = 3 voxels over W = 24 -> 1/8 turn per column, uu0 * a / W = -1.5 -> 0.5 turn
+    it = _items.n_item(FakePacked(), "ap", 0, 0, 0, 3.0, 0, 0, 0, 0, 64)[0]
+    assert it["ph_col"] == 1 << 29 and it["ph_col0"] == 1 << 31
+
+
+def test_jacobian_matches_finite_difference():
+    from fvr import _items
+    for view, perm in (("ap", (0, 1, 2)), ("lateral", (0, 2, 1))):
+        y, p, r = 3.0, -7.0, 11.0
+        du, dv = _items.n_jacobian(y, p, r, view, perm)
+        h = 1e-6
+        for j in range(3):
+            a = [y, p, r]
+            b = [y, p, r]
+            a[j] += h
+            b[j] -= h
+            ua, va = _items.euler_to_basis_n(*a, view)
+            ub, vb = _items.euler_to_basis_n(*b, view)
+            np.testing.assert_allclose(du[:, j], ((ua - ub) / (2 * h))[list(perm)], atol=1e-9)
+            np.testing.assert_allclose(dv[:, j], ((va - vb) / (2 * h))[list(perm)], atol=1e-9)
+
+
+def test_phantom_is_seeded_and_in_range():
+    from fvr.phantom import ellipsoid_phantom
+    a = ellipsoid_phantom(24, seed=1234)
+    b = ellipsoid_phantom(24, seed=1234)
+    assert a.dtype == np.float32 and a.shape == (24, 24, 24)
+    np.testing.assert_array_equal(a, b)
+    assert a.max() > 0.0 and a.min() == 0.0
+    assert not np.array_equal(a, ellipsoid_phantom(24, seed=1235))

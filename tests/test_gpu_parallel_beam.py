@@ -1,0 +1,148 @@
+"""GPU parity of the convention-N path (HIP kernels) against the FP64 oracle.
+
+Tolerances (complex64 spectrum, FP32 interpolation and sums, float64 final
+reductions): slices max|diff| <= 2e-6 * max|ref|; loss within 1e-5 relative;
+gradients within 1e-4 of max|grad| (the north-star bound is 1e-3 relative).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import diff_fvr_np as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _vol(shape, seed=5):
+    from fvr.phantom import ellipsoid_phantom
+    n = max(shape)
+    v = ellipsoid_phantom(n, n_ellipsoids=6, seed=seed).astype(np.float64)
+    return np.ascontiguousarray(v[: shape[0], : shape[1], : shape[2]])
+
+
+@pytest.fixture(scope="module", params=[(24, 24, 24), (17, 20, 23)], ids=["cube24", "ragged17x20x23"])
+def setup(request):
+    from fvr import DifferentiableFourierVolumeRenderer
+    vol = _vol(request.param)
+    r = DifferentiableFourierVolumeRenderer(vol)
+    f3 = O.make_spectrum(vol)
+    return vol, r, f3
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def test_spectrum_matches_oracle(setup):
+    _, r, f3 = setup
+    assert (r.pz, r.py, r.px) == f3.shape
+    assert _rel(r.f_3d, f3) < 1e-12
+
+
+@pytest.mark.parametrize("view", ["ap", "lateral", "axial"])
+@pytest.mark.parametrize("pose", [(0, 0, 0, 0, 0), (7.5, -4.0, 12.0, 2.3, -1.7), (-15, 15, -15, -20, 20),
+                                  (45, 30, -60, 0.5, 0.25)])
+def test_slice_parity(setup, view, pose):
+    _, r, f3 = setup
+    got = r.extract_frequency_slice(*pose[:3], tx=pose[3], ty=pose[4], view=view)
+    ref = O.extract_frequency_slice(f3, *pose[:3], tx=pose[3], ty=pose[4], view=view)
+    assert got.shape == ref.shape
+    assert _rel(got, ref) < 2e-6
+
+
+def test_known_answer_mm_shift_is_np_roll():
+    # the reference's known answer (tests/test_physical_units.py:28-34) on the GPU path
+    from fvr import RobustDiffFVREngine
+    z, y, x = np.indices((24,) * 3)
+    vol = (np.exp(-((z - 10) ** 2 + (y - 11) ** 2 + (x - 10) ** 2) / 8)
+           + .7 * np.exp(-((z - 15) ** 2 + (y - 15) ** 2 + (x - 15) ** 2) / 4))
+    eng = RobustDiffFVREngine(vol, spacing=(2.0,) * 3)
+    for view in ("ap", "lateral"):
+        img = eng.render_calibrated_drr(view=view, crop_to_roi=False)
+        shifted = eng.render_calibrated_drr(tx=4, ty=-2, view=view, crop_to_roi=False)
+        # FP32 slice: 1e-6 of the image peak instead of the FP64 reference's 1e-9 absolute
+        np.testing.assert_allclose(shifted, np.roll(img, (-1, 2), axis=(0, 1)), atol=1e-6 * np.abs(img).max())
+
+
+def _targets(f3, tp):
+    T_ap = O.extract_frequency_slice(f3, *tp[:3], tx=tp[3], ty=tp[5], view="ap")
+    T_la = O.extract_frequency_slice(f3, *tp[:3], tx=tp[4], ty=tp[5], view="lateral")
+    return T_ap, T_la
+
+
+POSES = [[3.3, 1.1, -2.2, 2.1, 1.5, -1.0], [-6.0, 4.0, 2.0, -3.0, 0.5, 2.5], [0.0, 0.0, 0.0, 0.0, 0.0, 0.0],
+         [14.0, -13.0, 9.0, 18.0, -19.0, 7.0]]
+
+
+@pytest.mark.parametrize("pose", POSES)
+def test_reference_fd_scheme_parity(setup, pose):
+    from fvr import SpectralGradientEngine
+    _, r, f3 = setup
+    T_ap, T_la = _targets(f3, [1.0, -2.0, 1.5, 0.7, -1.2, 0.4])
+    eng = SpectralGradientEngine(r)
+    L, g = eng.compute_biplanar_loss_and_gradients(np.array(pose), T_ap, T_la)
+    Lr, gr = O.biplanar_loss_and_gradients(f3, np.array(pose), T_ap, T_la)
+    assert abs(L - Lr) <= 1e-5 * max(abs(Lr), 1e-3)
+    # rotation entries are FD quotients (difference / 0.4 deg): FP32 noise is amplified 2.5x
+    assert np.max(np.abs(g - gr)) <= 1e-4 * np.max(np.abs(gr)) + 1e-9, (g, gr)
+
+
+@pytest.mark.parametrize("pose", POSES)
+def test_analytic_adjoint_parity(setup, pose):
+    from fvr import SpectralGradientEngine
+    _, r, f3 = setup
+    T_ap, T_la = _targets(f3, [1.0, -2.0, 1.5, 0.7, -1.2, 0.4])
+    eng = SpectralGradientEngine(r)
+    L, g = eng.compute_biplanar_loss_and_gradients(np.array(pose), T_ap, T_la, rotation_grad="analytic")
+    Lr, gr = O.biplanar_loss_and_grad_analytic(f3, np.array(pose), T_ap, T_la)
+    assert abs(L - Lr) <= 1e-5 * max(abs(Lr), 1e-3)
+    assert np.max(np.abs(g - gr)) <= 1e-4 * np.max(np.abs(gr)) + 1e-9, (g, gr)
+
+
+def test_batched_equals_single_and_is_deterministic(setup):
+    from fvr import SpectralGradientEngine
+    from fvr.phantom import random_poses
+    _, r, f3 = setup
+    T_ap, T_la = _targets(f3, [1.0, -2.0, 1.5, 0.7, -1.2, 0.4])
+    eng = SpectralGradientEngine(r)
+    poses = random_poses(9, 15.0, 5.0, seed=3456)
+    P = torch.from_numpy(poses).to(r.device)
+    L1, g1 = [t.cpu().numpy().copy() for t in eng.compute_biplanar_loss_and_gradients_batch(P, T_ap, T_la)]
+    L2, g2 = [t.cpu().numpy().copy() for t in eng.compute_biplanar_loss_and_gradients_batch(P, T_ap, T_la)]
+    np.testing.assert_array_equal(L1, L2)
+    np.testing.assert_array_equal(g1, g2)
+    for k in range(len(poses)):
+        Ls, gs = eng.compute_biplanar_loss_and_gradients(poses[k], T_ap, T_la, rotation_grad="analytic")
+        assert Ls == L1[k]
+        np.testing.assert_array_equal(gs, g1[k])
+        Lr, gr = O.biplanar_loss_and_grad_analytic(f3, poses[k], T_ap, T_la)
+        assert abs(L1[k] - Lr) <= 1e-5 * max(abs(Lr), 1e-3)
+        assert np.max(np.abs(g1[k] - gr)) <= 1e-4 * np.max(np.abs(gr)) + 1e-9
+
+
+def test_zero_target_and_zero_volume_give_reference_zero_convention():
+    from fvr import DifferentiableFourierVolumeRenderer, SpectralGradientEngine
+    vol = _vol((16, 16, 16))
+    r = DifferentiableFourierVolumeRenderer(vol)
+    eng = SpectralGradientEngine(r)
+    z = np.zeros((32, 32), dtype=np.complex128)
+    L, g = eng.compute_biplanar_loss_and_gradients(np.ones(6), z, z)
+    assert L == 1.0 and np.all(g == 0.0)
+    L, g = eng.compute_biplanar_loss_and_gradients(np.ones(6), z, z, rotation_grad="analytic")
+    assert L == 1.0 and np.all(g == 0.0)
+    r0 = DifferentiableFourierVolumeRenderer(np.zeros((8, 8, 8)))
+    s = r0.extract_frequency_slice(5, 5, 5, view="ap")
+    assert np.all(s == 0)
+    assert r0.compute_spectral_ncc(s, np.ones_like(s)) == 0.0
+
+
+def test_pair_moment_helpers_match_oracle(setup):
+    from fvr import SpectralGradientEngine
+    _, r, f3 = setup
+    a = O.extract_frequency_slice(f3, 2.0, 1.0, -1.0, tx=1.2, ty=-0.3, view="ap")
+    b = O.extract_frequency_slice(f3, 1.0, 0.0, 0.5, tx=0.2, ty=0.3, view="ap")
+    assert abs(r.compute_spectral_ncc(a, b) - O.compute_spectral_ncc(a, b)) < 1e-6
+    got = SpectralGradientEngine(r).compute_spectral_ncc_and_grad_translation(a, b, view="ap")
+    ref = O.ncc_and_grad_translation(a, b)
+    np.testing.assert_allclose(got, ref, rtol=1e-5, atol=1e-7)
