@@ -76,24 +76,6 @@ __global__ __launch_bounds__(256) void pack_kernel(PackArgs a) {
   }
 }
 
-// ||x||^2 of a complex64 array in float64, fixed-order (one workgroup)
-__global__ __launch_bounds__(1024) void c64_norm2_kernel(const float2* __restrict__ x, int64_t n,
-                                                         double* __restrict__ out) {
-  double acc = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += 1024) {
-    const float2 v = x[i];
-    acc += double(v.x) * double(v.x) + double(v.y) * double(v.y);
-  }
-  __shared__ double red[1024];
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int st = 512; st > 0; st >>= 1) {
-    if (int(threadIdx.x) < st) red[threadIdx.x] += red[threadIdx.x + st];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *out = red[0];
-}
-
 // Moments of two complex64 (h x w) slices a, b with uu = j - w/2, vv = i - h/2:
 //   out = [Re sum a b*, sum |a|^2, sum |b|^2, sum uu Im(a b*), sum vv Im(a b*)]
 // (float64, fixed order; one workgroup).  Serves compute_spectral_ncc
@@ -144,14 +126,6 @@ int fvr_struct_sizes(int32_t* out) {
   out[1] = int32_t(sizeof(fvr_sncc_item));
   out[2] = 0;
   out[3] = 0;
-  return FVR_OK;
-}
-
-int fvr_c64_norm2(uint64_t x, int64_t n, double* out, void* stream) {
-  FVR_CHECK_ARG(x && out && n > 0, "fvr_c64_norm2: bad arguments");
-  hipLaunchKernelGGL(c64_norm2_kernel, dim3(1), dim3(1024), 0, reinterpret_cast<hipStream_t>(stream),
-                     reinterpret_cast<const float2*>(x), n, out);
-  FVR_CHECK_LAUNCH("c64_norm2_kernel");
   return FVR_OK;
 }
 

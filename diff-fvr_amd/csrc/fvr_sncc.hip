@@ -11,7 +11,7 @@
 //   Mc_im = sum cw(j) Im(I W)   Mr_im = sum rw(i) Im(I W)
 //   Mc_g  = sum cw(j) Re(dI W)  Mr_g  = sum rw(i) Re(dI W)      (3 axes)
 //   Mc_h  = sum cw(j) Re(dI I*) Mr_h  = sum rw(i) Re(dI I*)     (3 axes)
-// from which the host or fvr_pn_combine forms NCC and its exact derivative
+// from which the host forms NCC and its exact derivative
 // w.r.t. the slice axes and the in-plane translations:
 //   dNCC = Re<dS,T>/(|S||T|) - NCC Re<dS,S>/|S|^2.
 // Reference behaviour reproduced: diff_fvr.py:106-150 (slice + ramp),
@@ -124,8 +124,9 @@ __device__ __forceinline__ cpx phasor(uint32_t t) {
 
 template <int DT, bool ADJ, bool HP, bool WS>
 __global__ __launch_bounds__(256) void sncc_kernel(const fvr_layout* __restrict__ layouts,
-                                                   const fvr_sncc_item* __restrict__ items, int n_items,
-                                                   int n_tiles, int per_xcd, float* __restrict__ partials) {
+                                                   int n_layouts, const fvr_sncc_item* __restrict__ items,
+                                                   int n_items, int n_tiles, int per_xcd,
+                                                   float* __restrict__ partials) {
   // XCD-aware remap: workgroup b runs on XCD b % 8, so each XCD takes a
   // contiguous range of tiles (whole items) and keeps their lines in its L2.
   const int b = blockIdx.x;
@@ -139,6 +140,10 @@ __global__ __launch_bounds__(256) void sncc_kernel(const fvr_layout* __restrict_
     else hi = mid - 1;
   }
   const fvr_sncc_item& it = items[lo];
+  if (uint32_t(it.layout) >= uint32_t(n_layouts)) {  // malformed descriptor: contribute nothing
+    if (threadIdx.x < FVR_SNCC_NMOM) partials[size_t(tile) * FVR_SNCC_NMOM + threadIdx.x] = 0.f;
+    return;
+  }
   const fvr_layout& L = layouts[it.layout];
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -329,135 +334,21 @@ __global__ __launch_bounds__(256) void sncc_reduce_kernel(const fvr_sncc_item* _
   }
 }
 
-// ---------------------------------------------------------------------------
-// Convention N: pose -> AP / LAT items on the device (float64 set-up).
-// ---------------------------------------------------------------------------
-__device__ inline void rot_zyx(double y, double p, double r, double R[3][3], double dR[3][3][3]) {
-  // rot = Rz(yaw) Ry(roll) Rx(pitch) in (Z, Y, X) index space (diff_fvr.py:64-87)
-  const double cy = cos(y), sy = sin(y), cp = cos(p), sp = sin(p), cr = cos(r), sr = sin(r);
-  const double Rz[3][3] = {{1, 0, 0}, {0, cy, -sy}, {0, sy, cy}};
-  const double Ry[3][3] = {{cr, 0, sr}, {0, 1, 0}, {-sr, 0, cr}};
-  const double Rx[3][3] = {{cp, -sp, 0}, {sp, cp, 0}, {0, 0, 1}};
-  const double dRz[3][3] = {{0, 0, 0}, {0, -sy, -cy}, {0, cy, -sy}};
-  const double dRy[3][3] = {{-sr, 0, cr}, {0, 0, 0}, {-cr, 0, -sr}};
-  const double dRx[3][3] = {{-sp, -cp, 0}, {cp, -sp, 0}, {0, 0, 0}};
-  auto mm = [](const double a[3][3], const double b[3][3], double o[3][3]) {
-    for (int i = 0; i < 3; ++i)
-      for (int k = 0; k < 3; ++k) o[i][k] = a[i][0] * b[0][k] + a[i][1] * b[1][k] + a[i][2] * b[2][k];
-  };
-  double t[3][3];
-  mm(Ry, Rx, t);
-  mm(Rz, t, R);
-  mm(dRz, t, dR[0]);  // d/dyaw
-  double t2[3][3];
-  mm(Ry, dRx, t2);
-  mm(Rz, t2, dR[1]);  // d/dpitch
-  mm(dRy, Rx, t2);
-  mm(Rz, t2, dR[2]);  // d/droll
-}
-
-__device__ inline int64_t to_fx(double x) { return llrint(x * 4294967296.0); }
-__device__ inline uint32_t turns_fx(double x) {
-  x -= floor(x);
-  return uint32_t(uint64_t(llrint(x * 4294967296.0)) & 0xFFFFFFFFull);
-}
-
-__global__ void pn_build_kernel(const double* __restrict__ poses, int n_poses, const fvr_sncc_item* __restrict__ tmpl,
-                                const double* __restrict__ pad, fvr_sncc_item* __restrict__ items,
-                                double* __restrict__ jac, int lat_transposed) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= 2 * n_poses) return;
-  const int pose = idx >> 1, view = idx & 1;  // 0 = AP, 1 = LAT
-  const double* P = poses + 6 * pose;
-  const double deg = 3.14159265358979323846 / 180.0;
-  double R[3][3], dR[3][3][3];
-  rot_zyx(P[0] * deg, P[1] * deg, P[2] * deg, R, dR);
-  // u_base: AP x (index 2), LAT y (index 1); v_base: z (index 0)
-  const int ub = view == 0 ? 2 : 1;
-  const double pz = pad[0], py = pad[1], px = pad[2];
-  const double H = pz, W = view == 0 ? px : py;
-  const double uu0 = -W / 2.0, vv0 = -H / 2.0;
-  const double cz[3] = {pz / 2.0, py / 2.0, px / 2.0};
-  int perm[3] = {0, 1, 2};
-  if (view == 1 && lat_transposed) { perm[1] = 2; perm[2] = 1; }
-  fvr_sncc_item it = tmpl[view];
-  const int tiles_ap = tmpl[0].tiles_x * ((tmpl[0].h + kSnccTileH - 1) / kSnccTileH);
-  const int tiles_lat = tmpl[1].tiles_x * ((tmpl[1].h + kSnccTileH - 1) / kSnccTileH);
-  double* J = jac + size_t(idx) * 18;
-  for (int a = 0; a < 3; ++a) {
-    const int z = perm[a];
-    const double u = R[z][ub], v = R[z][0];
-    it.c[a] = to_fx(cz[z] + uu0 * u + vv0 * v);
-    it.ustep[a] = to_fx(u);
-    it.vstep[a] = to_fx(v);
-    for (int jj = 0; jj < 3; ++jj) {
-      J[a * 3 + jj] = dR[jj][z][ub] * deg;      // du/dtheta_j (per degree)
-      J[9 + a * 3 + jj] = dR[jj][z][0] * deg;   // dv/dtheta_j
-    }
-  }
-  const double aa = view == 0 ? P[3] : P[4];  // AP: tx, LAT: ty
-  const double bb = P[5];                     // tz
-  it.ph_col = turns_fx(aa / W);
-  it.ph_col0 = turns_fx(uu0 * aa / W);
-  it.ph_row = turns_fx(bb / H);
-  it.ph_row0 = turns_fx(vv0 * bb / H);
-  it.tile_base = pose * (tiles_ap + tiles_lat) + (view == 1 ? tiles_ap : 0);
-  items[idx] = it;
-}
-
-// Biplanar combine: L = 1/2(1 - NCC_AP) + 1/2(1 - NCC_LAT) (gradients.py:95-108)
-__global__ void pn_combine_kernel(const double* __restrict__ red, const double* __restrict__ jac, int n_poses,
-                                  const double* __restrict__ t2, const double* __restrict__ hw,
-                                  double* __restrict__ loss, double* __restrict__ grad) {
-  const int pose = blockIdx.x * blockDim.x + threadIdx.x;
-  if (pose >= n_poses) return;
-  const double two_pi = 6.283185307179586476925286766559;
-  double g[6] = {0, 0, 0, 0, 0, 0};
-  double L = 0.0;
-  for (int view = 0; view < 2; ++view) {
-    const double* m = red + size_t(2 * pose + view) * FVR_SNCC_NMOM;
-    const double* J = jac + size_t(2 * pose + view) * 18;
-    const double nt = sqrt(t2[view]), ns = sqrt(m[1]);
-    double ncc = 0.0, d[5] = {0, 0, 0, 0, 0};
-    if (ns * nt != 0.0) {
-      const double inv = 1.0 / (ns * nt);
-      ncc = m[0] * inv;
-      for (int jj = 0; jj < 3; ++jj) {
-        double G = 0.0, Hh = 0.0;
-        for (int a = 0; a < 3; ++a) {
-          G += m[4 + a] * J[a * 3 + jj] + m[7 + a] * J[9 + a * 3 + jj];
-          Hh += m[10 + a] * J[a * 3 + jj] + m[13 + a] * J[9 + a * 3 + jj];
-        }
-        d[jj] = G * inv - ncc * Hh / m[1];
-      }
-      d[3] = two_pi / hw[2 * view + 1] * m[2] * inv;  // d/da: 2 pi / W * sum uu Im(S T*)
-      d[4] = two_pi / hw[2 * view + 0] * m[3] * inv;  // d/db: 2 pi / H * sum vv Im(S T*)
-    }
-    L += 0.5 * (1.0 - ncc);
-    for (int jj = 0; jj < 3; ++jj) g[jj] -= 0.5 * d[jj];
-    if (view == 0) g[3] -= 0.5 * d[3];
-    else g[4] -= 0.5 * d[3];
-    g[5] -= 0.5 * d[4];
-  }
-  loss[pose] = L;
-  for (int a = 0; a < 6; ++a) grad[6 * pose + a] = g[a];
-}
-
 template <int DT, bool ADJ, bool HP, bool WS>
-static void launch_sncc(dim3 g, hipStream_t st, const fvr_layout* lay, const fvr_sncc_item* items, int n_items,
-                        int n_tiles, int per_xcd, float* partials) {
-  hipLaunchKernelGGL((sncc_kernel<DT, ADJ, HP, WS>), g, dim3(256), 0, st, lay, items, n_items, n_tiles, per_xcd,
-                     partials);
+static void launch_sncc(dim3 g, hipStream_t st, const fvr_layout* lay, int n_lay, const fvr_sncc_item* items,
+                        int n_items, int n_tiles, int per_xcd, float* partials) {
+  hipLaunchKernelGGL((sncc_kernel<DT, ADJ, HP, WS>), g, dim3(256), 0, st, lay, n_lay, items, n_items, n_tiles,
+                     per_xcd, partials);
 }
 
 template <int DT>
-static void dispatch_flags(uint32_t flags, dim3 g, hipStream_t st, const fvr_layout* lay,
+static bool dispatch_flags(uint32_t flags, dim3 g, hipStream_t st, const fvr_layout* lay, int n_lay,
                            const fvr_sncc_item* items, int n_items, int n_tiles, int per_xcd, float* partials) {
   const bool adj = flags & FVR_ITEM_ADJOINT, hp = flags & FVR_ITEM_HP, ws = flags & FVR_ITEM_WRITE_SLICE;
 #define FVR_L(a, h, w)                                                                     \
   if (adj == a && hp == h && ws == w) {                                                    \
-    launch_sncc<DT, a, h, w>(g, st, lay, items, n_items, n_tiles, per_xcd, partials);     \
-    return;                                                                                \
+    launch_sncc<DT, a, h, w>(g, st, lay, n_lay, items, n_items, n_tiles, per_xcd, partials); \
+    return true;                                                                           \
   }
   FVR_L(true, false, false)
   FVR_L(false, false, false)
@@ -468,6 +359,7 @@ static void dispatch_flags(uint32_t flags, dim3 g, hipStream_t st, const fvr_lay
   FVR_L(false, true, true)
   FVR_L(true, true, true)
 #undef FVR_L
+  return false;
 }
 
 }  // namespace fvr
@@ -478,20 +370,28 @@ extern "C" {
 
 int fvr_sncc_tile_rows(void) { return kSnccTileH; }
 
-int fvr_sncc_eval_v(const fvr_layout* layouts_dev, const fvr_sncc_item* items_dev, int32_t n_items, int32_t n_tiles,
-                    float* partials, uint32_t variant, void* stream) {
+int fvr_sncc_eval_v(const fvr_layout* layouts_dev, int32_t n_layouts, const fvr_sncc_item* items_dev, int32_t n_items,
+                    int32_t n_tiles, float* partials, uint32_t variant, void* stream) {
   FVR_CHECK_ARG(layouts_dev && items_dev && partials, "fvr_sncc_eval: null pointer");
-  FVR_CHECK_ARG(n_items > 0 && n_tiles > 0, "fvr_sncc_eval: empty batch");
+  FVR_CHECK_ARG(n_items > 0 && n_tiles > 0 && n_layouts > 0, "fvr_sncc_eval: empty batch");
+  FVR_CHECK_ARG((variant >> 4) < 8u, "fvr_sncc_eval: bad flags %u", variant >> 4);
   const int dt = int(variant & 0xF);
   FVR_CHECK_ARG(dt == FVR_C32 || dt == FVR_C16F || dt == FVR_C16B, "fvr_sncc_eval: bad dtype %d", dt);
   const int per_xcd = (n_tiles + 7) / 8;
   dim3 g(per_xcd * 8);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   const uint32_t flags = variant >> 4;
-  if (dt == FVR_C32) dispatch_flags<FVR_C32>(flags, g, st, layouts_dev, items_dev, n_items, n_tiles, per_xcd, partials);
+  bool launched;
+  if (dt == FVR_C32)
+    launched = dispatch_flags<FVR_C32>(flags, g, st, layouts_dev, n_layouts, items_dev, n_items, n_tiles, per_xcd,
+                                       partials);
   else if (dt == FVR_C16F)
-    dispatch_flags<FVR_C16F>(flags, g, st, layouts_dev, items_dev, n_items, n_tiles, per_xcd, partials);
-  else dispatch_flags<FVR_C16B>(flags, g, st, layouts_dev, items_dev, n_items, n_tiles, per_xcd, partials);
+    launched = dispatch_flags<FVR_C16F>(flags, g, st, layouts_dev, n_layouts, items_dev, n_items, n_tiles, per_xcd,
+                                        partials);
+  else
+    launched = dispatch_flags<FVR_C16B>(flags, g, st, layouts_dev, n_layouts, items_dev, n_items, n_tiles, per_xcd,
+                                        partials);
+  FVR_CHECK_ARG(launched, "fvr_sncc_eval: no kernel for variant %u", variant);
   FVR_CHECK_LAUNCH("sncc_kernel");
   return FVR_OK;
 }
@@ -502,25 +402,6 @@ int fvr_sncc_reduce(const fvr_sncc_item* items_dev, int32_t n_items, const float
   hipLaunchKernelGGL(sncc_reduce_kernel, dim3(n_items), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
                      items_dev, partials, out);
   FVR_CHECK_LAUNCH("sncc_reduce_kernel");
-  return FVR_OK;
-}
-
-int fvr_pn_build_items(const double* poses, int32_t n_poses, const fvr_sncc_item* tmpl_dev, const double* pad_dims,
-                       fvr_sncc_item* items_dev, double* jac, int32_t lat_transposed, void* stream) {
-  FVR_CHECK_ARG(poses && tmpl_dev && pad_dims && items_dev && jac && n_poses > 0, "fvr_pn_build_items: bad arguments");
-  const int n = 2 * n_poses;
-  hipLaunchKernelGGL(pn_build_kernel, dim3((n + 63) / 64), dim3(64), 0, reinterpret_cast<hipStream_t>(stream), poses,
-                     n_poses, tmpl_dev, pad_dims, items_dev, jac, lat_transposed);
-  FVR_CHECK_LAUNCH("pn_build_kernel");
-  return FVR_OK;
-}
-
-int fvr_pn_combine(const double* red, const double* jac, int32_t n_poses, const double* t2, const double* slice_hw,
-                   double* loss, double* grad, void* stream) {
-  FVR_CHECK_ARG(red && jac && t2 && slice_hw && loss && grad && n_poses > 0, "fvr_pn_combine: bad arguments");
-  hipLaunchKernelGGL(pn_combine_kernel, dim3((n_poses + 63) / 64), dim3(64), 0, reinterpret_cast<hipStream_t>(stream),
-                     red, jac, n_poses, t2, slice_hw, loss, grad);
-  FVR_CHECK_LAUNCH("pn_combine_kernel");
   return FVR_OK;
 }
 

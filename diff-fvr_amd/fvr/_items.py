@@ -123,7 +123,7 @@ def n_item(packed, view: str, yaw, pitch, roll, a, b, target_ptr: int, flags: in
 
 
 def n_template(packed, view: str, target_ptr: int, flags: int, tile_rows: int) -> np.ndarray:
-    """Static fields of a convention-N item (geometry filled on the device)."""
+    """Static fields of a convention-N item (geometry filled per pose by n_biplanar_items)."""
     return n_item(packed, view, 0.0, 0.0, 0.0, 0.0, 0.0, target_ptr, flags, 0, tile_rows)
 
 
@@ -147,3 +147,107 @@ def n_jacobian(yaw, pitch, roll, view, perm) -> Tuple[np.ndarray, np.ndarray]:
     du = np.stack([d @ ub for d in ds], axis=1) * DEG
     dv = np.stack([d @ vb for d in ds], axis=1) * DEG
     return du[list(perm)], dv[list(perm)]
+
+
+# ---------------------------------------------------------------------------
+# Vectorised convention-N set-up for batches of poses (AP + LAT per pose)
+# ---------------------------------------------------------------------------
+def _rot_and_derivs(poses: np.ndarray):
+    """R (N,3,3) = Rz(yaw) Ry(roll) Rx(pitch) and dR/dtheta (N,3,3,3) per DEGREE."""
+    y, p, r = (np.asarray(poses[:, k], dtype=np.float64) * DEG for k in range(3))
+    N = len(y)
+    z0, o1 = np.zeros(N), np.ones(N)
+
+    def m(rows):
+        return np.stack([np.stack(row, axis=-1) for row in rows], axis=-2)
+
+    cy, sy, cp, sp, cr, sr = np.cos(y), np.sin(y), np.cos(p), np.sin(p), np.cos(r), np.sin(r)
+    Rz = m([[o1, z0, z0], [z0, cy, -sy], [z0, sy, cy]])
+    Ry = m([[cr, z0, sr], [z0, o1, z0], [-sr, z0, cr]])
+    Rx = m([[cp, -sp, z0], [sp, cp, z0], [z0, z0, o1]])
+    dRz = m([[z0, z0, z0], [z0, -sy, -cy], [z0, cy, -sy]])
+    dRy = m([[-sr, z0, cr], [z0, z0, z0], [-cr, z0, -sr]])
+    dRx = m([[-sp, -cp, z0], [cp, -sp, z0], [z0, z0, z0]])
+    R = Rz @ Ry @ Rx
+    dR = np.stack([dRz @ Ry @ Rx, Rz @ Ry @ dRx, Rz @ dRy @ Rx], axis=1) * DEG  # (N, 3, 3, 3)
+    return R, dR
+
+
+def n_biplanar_items(templates: np.ndarray, pad_shape, poses: np.ndarray, lat_perm, tile_rows: int):
+    """Items (2N, AP then LAT per pose) and Jacobians (2N, 2, 3, 3) for convention N.
+
+    ``templates`` are the AP / LAT items whose static fields are kept.  The
+    geometry restates diff_fvr.py:56-150 with tx on AP, ty on LAT and tz as the
+    vertical in-plane translation of both views (gradients.py:89-95).
+    """
+    poses = np.asarray(poses, dtype=np.float64).reshape(-1, 6)
+    N = len(poses)
+    R, dR = _rot_and_derivs(poses)
+    pz, py, px = pad_shape
+    c = np.array([pz / 2.0, py / 2.0, px / 2.0])
+    items = np.empty(2 * N, dtype=templates.dtype)
+    jac = np.empty((2 * N, 2, 3, 3))
+    tiles = [int(tiles_for(int(t["h"]), int(t["w"]), tile_rows)[0] * tiles_for(int(t["h"]), int(t["w"]),
+                                                                               tile_rows)[1]) for t in templates]
+    for view in (0, 1):
+        t = templates[view]
+        perm = [0, 1, 2] if view == 0 else list(lat_perm)
+        ub = 2 if view == 0 else 1
+        H, W = float(pz), float(px if view == 0 else py)
+        uu0, vv0 = -W / 2.0, -H / 2.0
+        u = R[:, :, ub]  # rot @ u_base, (N, 3) in (z, y, x)
+        v = R[:, :, 0]
+        cg = c[None, :] + uu0 * u + vv0 * v
+        sl = items[view::2]
+        sl[:] = t
+        sl["c"] = to_fx(cg[:, perm])
+        sl["ustep"] = to_fx(u[:, perm])
+        sl["vstep"] = to_fx(v[:, perm])
+        a = poses[:, 3] if view == 0 else poses[:, 4]
+        b = poses[:, 5]
+        sl["ph_col"] = turns_fx(a / W)
+        sl["ph_col0"] = turns_fx(uu0 * a / W)
+        sl["ph_row"] = turns_fx(b / H)
+        sl["ph_row0"] = turns_fx(vv0 * b / H)
+        sl["tile_base"] = np.arange(N) * (tiles[0] + tiles[1]) + (tiles[0] if view == 1 else 0)
+        # du/dtheta_j = dR_j @ u_base, dv/dtheta_j = dR_j @ v_base, layout axes
+        jac[view::2, 0] = np.transpose(dR[:, :, :, ub], (0, 2, 1))[:, perm, :]
+        jac[view::2, 1] = np.transpose(dR[:, :, :, 0], (0, 2, 1))[:, perm, :]
+    return items, jac, N * (tiles[0] + tiles[1])
+
+
+def validate_items(items: np.ndarray, n_layouts: int, descs: np.ndarray, tile_rows: int) -> int:
+    """Host-side checks of a launch's descriptors; returns the tile count.
+
+    Every field the kernel uses to form an address is checked here, so a
+    malformed descriptor raises before anything reaches the device.
+    """
+    if len(items) == 0:
+        raise ValueError("empty item batch")
+    lay = items["layout"]
+    if np.any(lay < 0) or np.any(lay >= n_layouts):
+        raise ValueError("item layout index out of range")
+    if np.any(items["h"] <= 0) or np.any(items["w"] <= 0):
+        raise ValueError("item slice shape must be positive")
+    if np.any((items["lo_int"] != 0) & (items["lo_int"] != 1)):
+        raise ValueError("item lo_int must be 0 or 1")
+    if np.any(items["target"] == 0):
+        raise ValueError("item target pointer is null")
+    if np.any(((items["flags"] & _lib.ITEM_WRITE_SLICE) != 0) & (items["slice_out"] == 0)):
+        raise ValueError("slice output requested without a buffer")
+    ntile = ((items["w"] + 63) // 64) * ((items["h"] + tile_rows - 1) // tile_rows)
+    if np.any(items["tiles_x"] != (items["w"] + 63) // 64):
+        raise ValueError("item tiles_x inconsistent with its width")
+    base = np.concatenate([[0], np.cumsum(ntile)[:-1]])
+    if not np.array_equal(items["tile_base"].astype(np.int64), base):
+        raise ValueError("item tile_base not the running tile count")
+    for k in range(len(items)):
+        d = descs[int(lay[k])]
+        lo = int(items["lo_int"][k])
+        hi = np.array(items["box_hi"][k], dtype=np.uint64)
+        # inside samples have floor(p) <= hi>>32; their corners (+lo, +1) must stay in storage
+        if np.any((hi >> np.uint64(32)).astype(np.int64) + lo + 1 > np.array(d["sdim"], dtype=np.int64) - 1):
+            raise ValueError("item box exceeds the layout's storage")
+        if int(items["guard_off"][k]) != (int(d["sdim"][0]) - 2) * int(d["plane_bytes"]):
+            raise ValueError("item guard offset is not the layout's zero plane")
+    return int(ntile.sum())

@@ -64,11 +64,8 @@ _SIGNATURES = {
     "fvr_layout_init": ([_P, _P, _I, _P, _P], _I),
     "fvr_spec_pack": ([_P, _U64, _I, _I, _I, _I, _P], _I),
     "fvr_sncc_tile_rows": ([], _I),
-    "fvr_sncc_eval_v": ([_P, _P, _I, _I, _P, _U, _P], _I),
+    "fvr_sncc_eval_v": ([_P, _I, _P, _I, _I, _P, _U, _P], _I),
     "fvr_sncc_reduce": ([_P, _I, _P, _P, _P], _I),
-    "fvr_pn_build_items": ([_P, _I, _P, _P, _P, _P, _I, _P], _I),
-    "fvr_pn_combine": ([_P, _P, _I, _P, _P, _P, _P, _P], _I),
-    "fvr_c64_norm2": ([_U64, ctypes.c_int64, _P, _P], _I),
     "fvr_c64_pair_moments": ([_U64, _U64, _I, _I, _P, _P], _I),
     "fvr_slice_fwd": ([_U64, _I, _I, _I, _I, _I, _P, _P, _I, _P, _P, _I, _F, _F, _F, _U64, _P], _I),
     "fvr_slice_bwd_workspace": ([_I, _I], ctypes.c_int64),

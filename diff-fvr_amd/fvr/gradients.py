@@ -68,11 +68,10 @@ class SpectralGradientEngine:
                                             rotation_grad: str = "fd") -> Tuple[float, np.ndarray]:
         """L = 1/2(1 - NCC_AP) + 1/2(1 - NCC_LAT) and dL/d(yaw, pitch, roll, tx, ty, tz)."""
         pose = np.asarray(pose, dtype=np.float64).reshape(6)
-        dev = self.renderer.device
         if rotation_grad == "analytic":
             ev = self._evaluator(f_target_ap, f_target_lat, adjoint=True)
-            loss, grad = ev.evaluate(torch.from_numpy(pose.reshape(1, 6).copy()).to(dev))
-            return float(loss[0].item()), grad[0].cpu().numpy().copy()
+            loss, grad = ev.evaluate(pose.reshape(1, 6))
+            return float(loss[0]), grad[0].copy()
         if rotation_grad != "fd":
             raise ValueError("rotation_grad must be 'fd' or 'analytic'")
         # base pose + (+delta, -delta) per Euler angle, evaluated in one launch
@@ -81,9 +80,9 @@ class SpectralGradientEngine:
             variants[1 + 2 * i, i] += delta_rot_deg
             variants[2 + 2 * i, i] -= delta_rot_deg
         ev = self._evaluator(f_target_ap, f_target_lat, adjoint=False)
-        ev.evaluate(torch.from_numpy(variants).to(dev))
-        red = ev.moments(7).cpu().numpy()
-        t2 = ev.t2.cpu().numpy()
+        ev.evaluate(variants)
+        red = ev.last_moments
+        t2 = ev.t2
         (h_ap, w_ap), (h_lat, w_lat) = ev.hw["ap"], ev.hw["lateral"]
 
         def ncc(k, view):
@@ -108,8 +107,8 @@ class SpectralGradientEngine:
                          -0.5 * (d_ap_dtz + d_lat_dtz)], dtype=np.float64)
         return float(loss), grad
 
-    def compute_biplanar_loss_and_gradients_batch(self, poses: torch.Tensor, f_target_ap, f_target_lat
-                                                  ) -> Tuple[torch.Tensor, torch.Tensor]:
-        """Batched exact loss + gradient: poses (N, 6) float64 device tensor -> (N,), (N, 6)."""
+    def compute_biplanar_loss_and_gradients_batch(self, poses, f_target_ap, f_target_lat
+                                                  ) -> Tuple[np.ndarray, np.ndarray]:
+        """Batched exact loss + gradient: poses (N, 6) -> float64 (N,), (N, 6), one fused launch."""
         ev = self._evaluator(f_target_ap, f_target_lat, adjoint=True)
-        return ev.evaluate(poses.to(torch.float64).contiguous())
+        return ev.evaluate(poses)

@@ -125,31 +125,17 @@ typedef struct fvr_sncc_item {
 int fvr_sncc_tile_rows(void);
 
 /* Launch the fused kernel over n_tiles tiles (the sum of items' tile counts).
- * variant = dtype | (flags << 4); every item of one launch shares it.
+ * variant = dtype | (flags << 4); every item of one launch shares it.  Items
+ * whose layout index is not below n_layouts contribute zero partials.
  * partials: float[n_tiles][FVR_SNCC_NMOM] (caller buffer). */
-int fvr_sncc_eval_v(const fvr_layout* layouts_dev, const fvr_sncc_item* items_dev, int32_t n_items,
-                    int32_t n_tiles, float* partials, uint32_t variant, void* stream);
+int fvr_sncc_eval_v(const fvr_layout* layouts_dev, int32_t n_layouts, const fvr_sncc_item* items_dev,
+                    int32_t n_items, int32_t n_tiles, float* partials, uint32_t variant, void* stream);
 
 /* Per-item reduction of the tile partials in fixed order (float64):
  * out[item][FVR_SNCC_NMOM] as double, ordered
  *   A, E, Mc_im, Mr_im, Mc_g[3], Mr_g[3], Mc_h[3], Mr_h[3]. */
 int fvr_sncc_reduce(const fvr_sncc_item* items_dev, int32_t n_items, const float* partials,
                     double* out, void* stream);
-
-/* Convention-N pose set-up on the device: builds 2 items per pose (AP, LAT)
- * from float64 poses (yaw, pitch, roll in degrees; tx, ty, tz in voxels) and
- * the float64 Jacobians jac[pose][view][2][3][3] (du/dtheta, dv/dtheta in
- * layout axes, per degree).  tmpl[2] are the AP and LAT item templates whose
- * geometry fields are overwritten. */
-int fvr_pn_build_items(const double* poses, int32_t n_poses, const fvr_sncc_item* tmpl_dev,
-                       const double* pad_dims, fvr_sncc_item* items_dev, double* jac,
-                       int32_t lat_transposed, void* stream);
-
-/* Convention-N biplanar combine: from reduce output (2 items per pose, AP then
- * LAT) and the target energies t2[2], writes loss[pose] and grad[pose][6] as
- * float64 (L = 1/2(1-NCC_AP) + 1/2(1-NCC_LAT)).  slice_hw[view][2] = (H, W). */
-int fvr_pn_combine(const double* red, const double* jac, int32_t n_poses, const double* t2,
-                   const double* slice_hw, double* loss, double* grad, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Generic fourier_slice_3d (grid_sample semantics, planar (B,2,D,H,W)).      */
@@ -174,8 +160,6 @@ int fvr_abi_version(void);
 const char* fvr_last_error(void);
 /* out[0] = sizeof(fvr_layout), out[1] = sizeof(fvr_sncc_item), out[2..3] = 0 */
 int fvr_struct_sizes(int32_t out[4]);
-/* *out = sum |x[i]|^2 (float64, fixed order) of a complex64 device array */
-int fvr_c64_norm2(uint64_t x, int64_t n, double* out, void* stream);
 /* out[5] = [Re sum a conj(b), sum |a|^2, sum |b|^2, sum uu Im(a conj b), sum vv Im(a conj b)]
  * over two complex64 (h x w) row-major device slices, uu = j - w/2, vv = i - h/2
  * (float64, fixed order).  Replaces diff_fvr.py:184-195 compute_spectral_ncc and

@@ -66,7 +66,7 @@ def test_layout_init_and_error_reporting():
 def test_entry_points_reject_bad_arguments_without_a_gpu():
     from fvr import _lib
     L = _lib.lib()
-    assert L.fvr_sncc_eval_v(None, None, 0, 0, None, 0, None) == 1
+    assert L.fvr_sncc_eval_v(None, 0, None, 0, 0, None, 0, None) == 1
     assert b"null" in L.fvr_last_error()
     assert L.fvr_slice_fwd(0, 0, 1, 1, 1, 1, None, None, 1, None, None, 1, 0.0, 0.0, 0.0, 0, None) == 1
 

@@ -1,0 +1,133 @@
+"""CPU checks of the host-side set-up around the fused kernel (no GPU compute):
+the vectorised item builder, the descriptor validator and the biplanar combine.
+"""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import diff_fvr_np as O
+
+
+class _Packed:
+    """A PackedSpectrum stand-in with real layout descriptors (no device storage)."""
+
+    def __init__(self, shape):
+        from fvr import _lib
+        L = _lib.lib()
+        self.shape = shape
+        Z, Y, X = shape
+        descs = np.zeros(2, dtype=_lib.LAYOUT_DTYPE)
+        for k, n in enumerate(([Z, Y, X], [Z, X, Y])):
+            one = np.zeros(1, dtype=_lib.LAYOUT_DTYPE)
+            nb, ns = ctypes.c_uint64(), ctypes.c_uint64()
+            assert L.fvr_layout_init(one.ctypes.data, np.array(n, np.int32).ctypes.data, 0, ctypes.byref(nb),
+                                     ctypes.byref(ns)) == 0
+            descs[k] = one[0]
+        self.descs = descs
+        self.index = {"zyx": 0, "zxy": 1}
+
+    def layout_for(self, view):
+        return (1, (0, 2, 1)) if view == "lateral" else (0, (0, 1, 2))
+
+    def desc(self, k):
+        return self.descs[k]
+
+
+POSES = np.array([[3.3, 1.1, -2.2, 2.1, 1.5, -1.0], [-14.0, 13.0, 9.0, -18.0, 19.0, 7.0], [0.0] * 6,
+                  [45.0, -30.0, 60.0, 0.25, -0.5, 3.0]])
+
+
+@pytest.mark.parametrize("shape", [(48, 48, 48), (34, 40, 46)])
+def test_vectorised_builder_equals_per_item_builder(shape):
+    from fvr import _items
+    pk = _Packed(shape)
+    tm = np.concatenate([_items.n_template(pk, "ap", 1234, 1, 64), _items.n_template(pk, "lateral", 5678, 1, 64)])
+    items, jac, n_tiles = _items.n_biplanar_items(tm, shape, POSES, (0, 2, 1), 64)
+    assert len(items) == 2 * len(POSES)
+    base = 0
+    for k, p in enumerate(POSES):
+        for v, view in enumerate(("ap", "lateral")):
+            a = p[3] if view == "ap" else p[4]
+            ref = _items.n_item(pk, view, p[0], p[1], p[2], a, p[5], 1234 if v == 0 else 5678, 1, base, 64)[0]
+            got = items[2 * k + v]
+            for f in ref.dtype.names:
+                np.testing.assert_array_equal(got[f], ref[f], err_msg=f"{view} pose {k} field {f}")
+            du, dv = _items.n_jacobian(p[0], p[1], p[2], view, pk.layout_for(view)[1])
+            np.testing.assert_allclose(jac[2 * k + v, 0], du, atol=1e-15)
+            np.testing.assert_allclose(jac[2 * k + v, 1], dv, atol=1e-15)
+            base += int(np.prod(_items.tiles_for(int(ref["h"]), int(ref["w"]), 64)))
+    assert n_tiles == base
+    assert _items.validate_items(items, 2, pk.descs, 64) == n_tiles
+
+
+def test_validator_rejects_malformed_descriptors():
+    from fvr import _items
+    pk = _Packed((32, 32, 32))
+    tm = np.concatenate([_items.n_template(pk, "ap", 1, 1, 64), _items.n_template(pk, "lateral", 2, 1, 64)])
+    items, _, _ = _items.n_biplanar_items(tm, pk.shape, POSES[:2], (0, 2, 1), 64)
+    for field, value in (("layout", 2), ("layout", -1), ("lo_int", 2), ("target", 0), ("h", 0),
+                         ("guard_off", 0), ("tile_base", 7), ("tiles_x", 5)):
+        bad = items.copy()
+        bad[1][field] = value
+        with pytest.raises(ValueError):
+            _items.validate_items(bad, 2, pk.descs, 64)
+    bad = items.copy()
+    bad[0]["box_hi"][0] = np.uint64(40) << np.uint64(32)
+    with pytest.raises(ValueError):
+        _items.validate_items(bad, 2, pk.descs, 64)
+    bad = items.copy()
+    bad[0]["flags"] = 4  # slice output without a buffer
+    with pytest.raises(ValueError):
+        _items.validate_items(bad, 2, pk.descs, 64)
+
+
+def _moments_from_oracle(f3, pose, view, T, perm):
+    """The kernel's 16 moments computed directly in float64 from oracle I, dI (layout axes)."""
+    yaw, pitch, roll, tx, ty, tz = pose
+    a = tx if view == "ap" else ty
+    h, w = O.slice_dims(f3.shape, view)
+    u, v = O.euler_to_basis(yaw, pitch, roll, view)
+    coords, uu, vv = O.slice_coords(f3.shape, u, v, view)
+    I, dI = O.trilinear_value_and_grad(f3, coords)
+    uu, vv = uu.ravel(), vv.ravel()
+    ph = np.exp(-2j * np.pi * (uu * a / w + vv * tz / h))
+    W = ph * np.conj(T.ravel())
+    IW = I * W
+    g = np.real(dI * W[None])[list(perm)]
+    hh = np.real(dI * np.conj(I)[None])[list(perm)]
+    m = np.zeros(16)
+    m[0], m[1] = IW.real.sum(), np.sum(np.abs(I) ** 2)
+    m[2], m[3] = np.sum(uu * IW.imag), np.sum(vv * IW.imag)
+    m[4:7], m[7:10] = (g * uu).sum(1), (g * vv).sum(1)
+    m[10:13], m[13:16] = (hh * uu).sum(1), (hh * vv).sum(1)
+    return m
+
+
+def test_combine_reproduces_oracle_gradient():
+    from fvr import _items
+    from fvr.sncc import combine_biplanar
+    vol = np.zeros((16, 16, 16))
+    vol[4:11, 5:12, 3:9] = 1.0
+    vol[9:13, 2:6, 8:14] = 0.5
+    f3 = O.make_spectrum(vol)
+    tp = [1.0, -2.0, 1.5, 0.7, -1.2, 0.4]
+    T_ap = O.extract_frequency_slice(f3, *tp[:3], tx=tp[3], ty=tp[5], view="ap")
+    T_la = O.extract_frequency_slice(f3, *tp[:3], tx=tp[4], ty=tp[5], view="lateral")
+    poses = np.array([[3.3, 1.1, -2.2, 2.1, 1.5, -1.0], [-6.0, 4.0, 2.0, -3.0, 0.5, 2.5]])
+    pk = _Packed(f3.shape)
+    mom = np.stack([_moments_from_oracle(f3, p, view, T, perm)
+                    for p in poses for view, T, perm in (("ap", T_ap, (0, 1, 2)), ("lateral", T_la, (0, 2, 1)))])
+    tm = np.concatenate([_items.n_template(pk, "ap", 1, 1, 64), _items.n_template(pk, "lateral", 2, 1, 64)])
+    _, jac, _ = _items.n_biplanar_items(tm, f3.shape, poses, (0, 2, 1), 64)
+    t2 = [np.sum(np.abs(T_ap) ** 2), np.sum(np.abs(T_la) ** 2)]
+    hw = {"ap": O.slice_dims(f3.shape, "ap"), "lateral": O.slice_dims(f3.shape, "lateral")}
+    loss, grad = combine_biplanar(mom, jac, t2, hw)
+    for k, p in enumerate(poses):
+        Lr, gr = O.biplanar_loss_and_grad_analytic(f3, p, T_ap, T_la)
+        assert abs(loss[k] - Lr) < 1e-12
+        np.testing.assert_allclose(grad[k], gr, rtol=1e-9, atol=1e-14)
+    # zero target -> NCC 0, loss 1, gradient 0 (diff_fvr.py:193-194)
+    loss0, grad0 = combine_biplanar(mom, jac, [0.0, 0.0], hw)
+    assert np.all(loss0 == 1.0) and np.all(grad0 == 0.0)
