@@ -146,3 +146,27 @@ def test_pair_moment_helpers_match_oracle(setup):
     got = SpectralGradientEngine(r).compute_spectral_ncc_and_grad_translation(a, b, view="ap")
     ref = O.ncc_and_grad_translation(a, b)
     np.testing.assert_allclose(got, ref, rtol=1e-5, atol=1e-7)
+
+
+@pytest.mark.parametrize("storage,tol_slice,tol_grad", [("fp16", 2e-3, 2e-2), ("bf16", 2e-2, 1e-1)])
+def test_reduced_precision_storage(storage, tol_slice, tol_grad):
+    """FP16 / BF16 spectrum storage with per-8^3-brick power-of-two scale, FP32 interpolation.
+
+    Bounds (relative to the largest magnitude): slices 2e-3 (fp16, 11-bit significand)
+    / 2e-2 (bf16, 8-bit); loss 1e-3 / 1e-2 absolute; gradient 2e-2 / 1e-1 of max|grad|.
+    """
+    from fvr import DifferentiableFourierVolumeRenderer, SpectralGradientEngine
+    vol = _vol((24, 24, 24))
+    r = DifferentiableFourierVolumeRenderer(vol, storage=storage)
+    f3 = O.make_spectrum(vol)
+    for view in ("ap", "lateral"):
+        got = r.extract_frequency_slice(7.5, -4.0, 12.0, tx=2.3, ty=-1.7, view=view)
+        ref = O.extract_frequency_slice(f3, 7.5, -4.0, 12.0, tx=2.3, ty=-1.7, view=view)
+        assert _rel(got, ref) < tol_slice
+    T_ap, T_la = _targets(f3, [1.0, -2.0, 1.5, 0.7, -1.2, 0.4])
+    eng = SpectralGradientEngine(r)
+    for pose in POSES[:2]:
+        L, g = eng.compute_biplanar_loss_and_gradients(np.array(pose), T_ap, T_la, rotation_grad="analytic")
+        Lr, gr = O.biplanar_loss_and_grad_analytic(f3, np.array(pose), T_ap, T_la)
+        assert abs(L - Lr) <= (1e-3 if storage == "fp16" else 1e-2)
+        assert np.max(np.abs(g - gr)) <= tol_grad * np.max(np.abs(gr))

@@ -131,3 +131,25 @@ def test_combine_reproduces_oracle_gradient():
     # zero target -> NCC 0, loss 1, gradient 0 (diff_fvr.py:193-194)
     loss0, grad0 = combine_biplanar(mom, jac, [0.0, 0.0], hw)
     assert np.all(loss0 == 1.0) and np.all(grad0 == 0.0)
+
+
+def test_footprint_counter_equals_direct_count():
+    from fvr.footprint import slice_footprint
+    shape, pose = (40, 40, 40), (7.0, -4.0, 3.0)
+    U, B = slice_footprint(shape, pose, "lateral")
+    u, v = O.euler_to_basis(*pose, "lateral")
+    c, _, _ = O.slice_coords(shape, u, v, "lateral")
+    n = np.array(shape)[:, None]
+    c = c[:, np.all((c >= 0) & (c <= n - 1), 0)]
+    i0 = np.floor(c).astype(int)
+    t = c - i0
+    keys, bricks = set(), set()
+    for dz in (0, 1):
+        for dy in (0, 1):
+            for dx in (0, 1):
+                w = (t[0] if dz else 1 - t[0]) * (t[1] if dy else 1 - t[1]) * (t[2] if dx else 1 - t[2])
+                idx = i0[:, w > 0] + np.array([dz, dy, dx])[:, None]
+                keys |= set(map(tuple, idx.T))
+                bricks |= set(map(tuple, ((idx + 1) // 8).T))
+    assert U == len(keys) and B == len(bricks)
+    assert 1.9 < U / (shape[0] * shape[1]) < 2.6
