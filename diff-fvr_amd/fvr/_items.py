@@ -185,7 +185,8 @@ def n_biplanar_items(templates: np.ndarray, pad_shape, poses: np.ndarray, lat_pe
     R, dR = _rot_and_derivs(poses)
     pz, py, px = pad_shape
     c = np.array([pz / 2.0, py / 2.0, px / 2.0])
-    items = np.empty(2 * N, dtype=templates.dtype)
+    templates = _lib.as_struct(templates, _lib.ITEM_DTYPE)
+    items = np.empty(2 * N, dtype=_lib.ITEM_DTYPE)
     jac = np.empty((2 * N, 2, 3, 3))
     tiles = [int(tiles_for(int(t["h"]), int(t["w"]), tile_rows)[0] * tiles_for(int(t["h"]), int(t["w"]),
                                                                                tile_rows)[1]) for t in templates]
@@ -224,6 +225,8 @@ def validate_items(items: np.ndarray, n_layouts: int, descs: np.ndarray, tile_ro
     """
     if len(items) == 0:
         raise ValueError("empty item batch")
+    if items.dtype != _lib.ITEM_DTYPE or items.dtype.itemsize != _lib.ITEM_DTYPE.itemsize:
+        raise ValueError("items must use the C item layout (fvr._lib.ITEM_DTYPE)")
     lay = items["layout"]
     if np.any(lay < 0) or np.any(lay >= n_layouts):
         raise ValueError("item layout index out of range")

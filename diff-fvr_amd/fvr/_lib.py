@@ -35,12 +35,12 @@ LAYOUT_DTYPE = np.dtype({
 ITEM_DTYPE = np.dtype({
     "names": ["c", "ustep", "vstep", "box_hi", "ph_col0", "ph_col", "ph_row0", "ph_row", "cw0", "cws", "rw0",
               "rws", "hp_inv2s2", "hp_x0", "hp_dx", "hp_y0", "hp_dy", "h", "w", "layout", "lo_int", "guard_off",
-              "tile_base", "tiles_x", "flags", "target", "slice_out"],
+              "tile_base", "tiles_x", "flags", "pad1", "target", "slice_out"],
     "formats": [("<i8", 3), ("<i8", 3), ("<i8", 3), ("<u8", 3), "<u4", "<u4", "<u4", "<u4", "<f4", "<f4", "<f4",
                 "<f4", "<f4", "<f4", "<f4", "<f4", "<f4", "<i4", "<i4", "<i4", "<i4", "<u4", "<i4", "<i4", "<u4",
-                "<u8", "<u8"],
+                "<u4", "<u8", "<u8"],
     "offsets": [0, 24, 48, 72, 96, 100, 104, 108, 112, 116, 120, 124, 128, 132, 136, 140, 144, 148, 152, 156, 160,
-                164, 168, 172, 176, 184, 192],
+                164, 168, 172, 176, 180, 184, 192],
     "itemsize": 200,
 })
 
@@ -127,7 +127,29 @@ def ptr(t: torch.Tensor) -> ctypes.c_void_p:
     return ctypes.c_void_p(t.data_ptr())
 
 
-def struct_to_device(arr: np.ndarray, device: torch.device) -> torch.Tensor:
-    """Upload a numpy structured array as raw bytes (uint8 tensor)."""
-    raw = np.ascontiguousarray(arr).view(np.uint8).reshape(-1)
+def as_struct(arr, dtype: np.dtype) -> np.ndarray:
+    """``arr`` as a contiguous array of exactly ``dtype`` (the C layout).
+
+    NumPy operations such as ``np.concatenate`` return structured arrays with a
+    *packed* dtype (alignment holes dropped, later fields shifted); converting
+    by field name restores the C offsets and item size.
+    """
+    a = np.asarray(arr)
+    if a.dtype != dtype or a.dtype.itemsize != dtype.itemsize:
+        a = a.astype(dtype)
+    return np.ascontiguousarray(a)
+
+
+def struct_bytes(arr, dtype: np.dtype) -> np.ndarray:
+    """Raw bytes of ``arr`` in the C layout of ``dtype`` (size checked)."""
+    a = as_struct(arr, dtype)
+    raw = a.view(np.uint8).reshape(-1)
+    if raw.size != a.size * dtype.itemsize:
+        raise AssertionError("struct byte image does not match the C item size")
+    return raw
+
+
+def struct_to_device(arr: np.ndarray, device: torch.device, dtype: np.dtype = None) -> torch.Tensor:
+    """Upload a numpy structured array in its C layout as raw bytes (uint8 tensor)."""
+    raw = struct_bytes(arr, dtype if dtype is not None else np.asarray(arr).dtype)
     return torch.from_numpy(raw.copy()).to(device, non_blocking=False)

@@ -58,7 +58,7 @@ def run_items(packed: PackedSpectrum, items: np.ndarray, flags: int,
     same ``flags`` (one kernel variant per launch).
     """
     rows = tile_rows()
-    items = np.array(items, copy=True)
+    items = _lib.as_struct(items, _lib.ITEM_DTYPE).copy()
     ntile = item_tiles(items, rows)
     items["tile_base"] = np.concatenate([[0], np.cumsum(ntile)[:-1]]).astype(np.int32)
     if np.any(items["flags"] != flags):
@@ -76,8 +76,7 @@ def launch_items(packed: PackedSpectrum, items: np.ndarray, n_tiles: int, flags:
     L = _lib.lib()
     n_items = len(items)
     dev_items = ws.get("items", n_items * _lib.ITEM_DTYPE.itemsize, torch.uint8)
-    dev_items[: n_items * _lib.ITEM_DTYPE.itemsize].copy_(
-        torch.from_numpy(np.ascontiguousarray(items).view(np.uint8).reshape(-1)))
+    dev_items[: n_items * _lib.ITEM_DTYPE.itemsize].copy_(torch.from_numpy(_lib.struct_bytes(items, _lib.ITEM_DTYPE)))
     partials = ws.get("partials", n_tiles * _lib.NMOM, torch.float32)
     st = _lib.stream_ptr(packed.device)
     variant = packed.code | (flags << 4)
@@ -136,9 +135,10 @@ class BiplanarEvaluator:
         self.set_targets(target_ap, target_lat)
         self.rows = tile_rows()
         self.flags = _lib.ITEM_ADJOINT if adjoint else 0
-        self.templates = np.concatenate([
+        self.templates = _lib.as_struct(np.concatenate([
             _items.n_template(packed, "ap", self.targets[0].data_ptr(), self.flags, self.rows),
-            _items.n_template(packed, "lateral", self.targets[1].data_ptr(), self.flags, self.rows)])
+            _items.n_template(packed, "lateral", self.targets[1].data_ptr(), self.flags, self.rows)]),
+            _lib.ITEM_DTYPE)
         self.lat_perm = packed.layout_for("lateral")[1]
         self.ws = _Workspace(self.device)
         self.last_moments: Optional[np.ndarray] = None
@@ -169,7 +169,7 @@ class BiplanarEvaluator:
             L = _lib.lib()
             dev_items = self.ws.get("items", len(items) * _lib.ITEM_DTYPE.itemsize, torch.uint8)
             dev_items[: len(items) * _lib.ITEM_DTYPE.itemsize].copy_(
-                torch.from_numpy(np.ascontiguousarray(items).view(np.uint8).reshape(-1)))
+                torch.from_numpy(_lib.struct_bytes(items, _lib.ITEM_DTYPE)))
             partials = self.ws.get("partials", n_tiles * _lib.NMOM, torch.float32)
             st = _lib.stream_ptr(self.device)
             kernel_events[0].record()

@@ -95,7 +95,7 @@ class PackedSpectrum:
             descs[k] = one[0]
         torch.cuda.current_stream(self.device).synchronize()  # src is freed on return
         self.descs = descs
-        self.table = _lib.struct_to_device(descs, self.device)
+        self.table = _lib.struct_to_device(descs, self.device, _lib.LAYOUT_DTYPE)
 
     def layout_for(self, view: str) -> Tuple[int, Tuple[int, int, int]]:
         """(layout index, permutation layout-axis -> (z, y, x) axis) for a view."""

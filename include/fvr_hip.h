@@ -115,6 +115,7 @@ typedef struct fvr_sncc_item {
   int32_t tile_base;    /* global index of this item's first tile */
   int32_t tiles_x;      /* tiles per slice row */
   uint32_t flags;
+  uint32_t pad1;        /* explicit: keeps the layout free of implicit padding */
   uint64_t target;      /* device address of the complex64 target (h x w, row-major) */
   uint64_t slice_out;   /* device address for S (FVR_ITEM_WRITE_SLICE), else 0 */
 } fvr_sncc_item;

@@ -144,3 +144,28 @@ def test_phantom_is_seeded_and_in_range():
     np.testing.assert_array_equal(a, b)
     assert a.max() > 0.0 and a.min() == 0.0
     assert not np.array_equal(a, ellipsoid_phantom(24, seed=1235))
+
+
+def test_every_struct_field_offset_matches_the_header(tmp_path):
+    """Compile a probe against include/fvr_hip.h and compare offsetof() of every field."""
+    import shutil
+    import subprocess
+    from fvr import _lib
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
+    for cname, dt in (("fvr_sncc_item", _lib.ITEM_DTYPE), ("fvr_layout", _lib.LAYOUT_DTYPE)):
+        for f in dt.names:
+            lines.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+        lines.append(f'printf("{cname}.__size__ %zu\\n", sizeof({cname}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-o", str(exe), str(src)], check=True)
+    out = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                          text=True).stdout.split("\n") if l)
+    for cname, dt in (("fvr_sncc_item", _lib.ITEM_DTYPE), ("fvr_layout", _lib.LAYOUT_DTYPE)):
+        for f in dt.names:
+            assert int(out[f"{cname}.{f}"]) == dt.fields[f][1], (cname, f)
+        assert int(out[f"{cname}.__size__"]) == dt.itemsize

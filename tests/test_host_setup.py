@@ -153,3 +153,30 @@ def test_footprint_counter_equals_direct_count():
                 bricks |= set(map(tuple, ((idx + 1) // 8).T))
     assert U == len(keys) and B == len(bricks)
     assert 1.9 < U / (shape[0] * shape[1]) < 2.6
+
+
+def test_item_byte_image_keeps_the_c_layout():
+    """Regression: NumPy concatenation of structured items must not shift fields.
+
+    A packed item dtype (alignment hole dropped) once moved ``target`` from byte
+    184 to 180, so the kernel read its target address from the wrong bytes.
+    """
+    from fvr import _items, _lib
+    a = np.zeros(1, dtype=_lib.ITEM_DTYPE)
+    c = np.concatenate([a, a])
+    assert c.dtype.itemsize == 200 and c.dtype.fields["target"][1] == 184
+    pk = _Packed((32, 32, 32))
+    tm = np.concatenate([_items.n_template(pk, "ap", 0x1111222233334444, 1, 64),
+                         _items.n_template(pk, "lateral", 0x5555666677778888, 1, 64)])
+    items, _, _ = _items.n_biplanar_items(tm, pk.shape, POSES[:3], (0, 2, 1), 64)
+    raw = _lib.struct_bytes(items, _lib.ITEM_DTYPE)
+    assert raw.size == 200 * len(items)
+    for k in range(len(items)):
+        tgt = int.from_bytes(raw[200 * k + 184: 200 * k + 192].tobytes(), "little")
+        assert tgt == (0x1111222233334444 if k % 2 == 0 else 0x5555666677778888)
+        assert int.from_bytes(raw[200 * k + 156: 200 * k + 160].tobytes(), "little") == (0 if k % 2 == 0 else 1)
+    # a packed (196-byte) copy is converted back by field name
+    packed = np.zeros(2, dtype=np.dtype([(n, _lib.ITEM_DTYPE.fields[n][0]) for n in _lib.ITEM_DTYPE.names]))
+    packed["target"] = [7, 9]
+    assert _lib.as_struct(packed, _lib.ITEM_DTYPE)["target"].tolist() == [7, 9]
+    assert _lib.struct_bytes(packed, _lib.ITEM_DTYPE).size == 400
