@@ -130,6 +130,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="headline storage only (no f32 / plain forms)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-footprint", action="store_true",
+                    help="skip the distinct-entry count (roofline fields null; for profiler runs)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -174,9 +176,13 @@ def main(argv=None) -> int:
         del ev
     head = results[args.storage]
     elem = 8 if args.storage == "f32" else 4
-    log("counting the algorithmic footprint (distinct spectrum entries per slice)")
-    fp = batch_algorithmic_bytes((2 * args.size,) * 3, cands, elem, scaled=args.storage != "f32", device=dev)
-    achieved = fp["bytes"] / (head["kernel_ms"] * 1e-3) / 1e9
+    if args.no_footprint:
+        fp = {"bytes": None, "unique_entries": None, "bricks": None}
+        achieved = None
+    else:
+        log("counting the algorithmic footprint (distinct spectrum entries per slice)")
+        fp = batch_algorithmic_bytes((2 * args.size,) * 3, cands, elem, scaled=args.storage != "f32", device=dev)
+        achieved = fp["bytes"] / (head["kernel_ms"] * 1e-3) / 1e9
     cpu = None
     if not args.no_cpu:
         log("CPU baseline (oracle port, one core)")
@@ -209,8 +215,9 @@ def main(argv=None) -> int:
                    "volume": args.size, "spectrum": 2 * args.size, "views": ["ap", "lateral"],
                    "slice_samples_per_view": (2 * args.size) ** 2, "candidates_per_launch": args.n_cand,
                    "spectrum_storage": args.storage},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "roofline": {"bound": "hbm", "achieved": None if achieved is None else round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": None if achieved is None else round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "sncc_kernel", "kernel_ms": round(head["kernel_ms"], 4),
                      "algorithmic_bytes_per_launch": fp["bytes"], "unique_entries": fp["unique_entries"],
                      "bricks": fp["bricks"]},

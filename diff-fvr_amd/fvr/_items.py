@@ -244,13 +244,12 @@ def validate_items(items: np.ndarray, n_layouts: int, descs: np.ndarray, tile_ro
     base = np.concatenate([[0], np.cumsum(ntile)[:-1]])
     if not np.array_equal(items["tile_base"].astype(np.int64), base):
         raise ValueError("item tile_base not the running tile count")
-    for k in range(len(items)):
-        d = descs[int(lay[k])]
-        lo = int(items["lo_int"][k])
-        hi = np.array(items["box_hi"][k], dtype=np.uint64)
-        # inside samples have floor(p) <= hi>>32; their corners (+lo, +1) must stay in storage
-        if np.any((hi >> np.uint64(32)).astype(np.int64) + lo + 1 > np.array(d["sdim"], dtype=np.int64) - 1):
-            raise ValueError("item box exceeds the layout's storage")
-        if int(items["guard_off"][k]) != (int(d["sdim"][0]) - 2) * int(d["plane_bytes"]):
-            raise ValueError("item guard offset is not the layout's zero plane")
+    d = descs[lay]
+    lo = items["lo_int"].astype(np.int64)
+    hi = (items["box_hi"] >> np.uint64(32)).astype(np.int64)  # (n_items, 3)
+    # inside samples have floor(p) <= hi; their corners (+lo, +1) must stay inside the storage
+    if np.any(hi + lo[:, None] + 1 > d["sdim"].astype(np.int64) - 1):
+        raise ValueError("item box exceeds the layout's storage")
+    if np.any(items["guard_off"].astype(np.int64) != (d["sdim"][:, 0].astype(np.int64) - 2) * d["plane_bytes"]):
+        raise ValueError("item guard offset is not the layout's zero plane")
     return int(ntile.sum())

@@ -84,8 +84,9 @@ def test_reference_fd_scheme_parity(setup, pose):
     L, g = eng.compute_biplanar_loss_and_gradients(np.array(pose), T_ap, T_la)
     Lr, gr = O.biplanar_loss_and_gradients(f3, np.array(pose), T_ap, T_la)
     assert abs(L - Lr) <= 1e-5 * max(abs(Lr), 1e-3)
-    # rotation entries are FD quotients (difference / 0.4 deg): FP32 noise is amplified 2.5x
-    assert np.max(np.abs(g - gr)) <= 1e-4 * np.max(np.abs(gr)) + 1e-9, (g, gr)
+    # rotation entries are FD quotients (NCC difference / 0.4 deg): an FP32 NCC error of
+    # up to 1e-7 becomes 2.5e-7 in the gradient, independent of the gradient's size
+    assert np.max(np.abs(g - gr)) <= 1e-4 * np.max(np.abs(gr)) + 2.5e-7, (g, gr)
 
 
 @pytest.mark.parametrize("pose", POSES)
