@@ -183,6 +183,12 @@ def main(argv=None) -> int:
         log("counting the algorithmic footprint (distinct spectrum entries per slice)")
         fp = batch_algorithmic_bytes((2 * args.size,) * 3, cands, elem, scaled=args.storage != "f32", device=dev)
         achieved = fp["bytes"] / (head["kernel_ms"] * 1e-3) / 1e9
+    traffic = None
+    pmc_path = os.path.join(ROOT, "profiles", f"sncc_kernel_{args.storage}_{args.size}cube_{args.n_cand}cand.json")
+    if os.path.exists(pmc_path):
+        pmc = json.load(open(pmc_path))
+        traffic = {"bytes": pmc["traffic_bytes_per_launch"], "l2_hit_rate": pmc["l2_hit_rate"],
+                   "source": os.path.relpath(pmc_path, ROOT), "pmc_avg_duration_us": pmc["avg_duration_us"]}
     cpu = None
     if not args.no_cpu:
         log("CPU baseline (oracle port, one core)")
@@ -217,7 +223,8 @@ def main(argv=None) -> int:
                    "spectrum_storage": args.storage},
         "roofline": {"bound": "hbm", "achieved": None if achieved is None else round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": None if achieved is None else round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": None if achieved is None else round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None if traffic is None else traffic["bytes"], "traffic_detail": traffic,
                      "kernel": "sncc_kernel", "kernel_ms": round(head["kernel_ms"], 4),
                      "algorithmic_bytes_per_launch": fp["bytes"], "unique_entries": fp["unique_entries"],
                      "bricks": fp["bricks"]},
