@@ -8,10 +8,11 @@ views (512^2 slice samples each), 64 candidate poses (rotations U(-15,15) deg,
 translations U(-20,20) mm) per launch, against AP/LAT target spectra of 256^2
 DRRs rendered at a pose with rotations and translations U(-5,5).
 
-One step = float64 pose set-up on the host + one fused HIP launch over all
-2x64 (pose, view) items + fixed-order device reduction + float64 biplanar
-combine on the host, for the whole batch (spectrum and targets resident in
-HBM; the step includes the item upload and the moment download).  `value` is
+One step = three stream-ordered launches for the whole batch: float64 pose
+set-up on the device (items + Jacobians), one fused HIP launch over all 2x64
+(pose, view) items, and the per-pose fixed-order float64 reduction + biplanar
+combine (spectrum, targets and poses resident in HBM; loss and gradient left
+in HBM, no host sync inside a step).  `value` is
 pose evaluations per second of the batched FP16-spectrum form (FP32
 interpolation and sums); the FP32-spectrum batched form and the plain
 one-pose-per-launch FP32 form are reported beside it.
@@ -64,8 +65,9 @@ def make_targets(renderer, true_pose):
 
 
 def time_batched(ev, P, steps: int, warmup: int):
-    """Wall time per step (host set-up + fused kernel + reduction + host combine) and the
-    fused kernel's mean duration from HIP events on its launch stream."""
+    """Wall time per step (device pose set-up + fused kernel + per-pose reduction and
+    combine; poses resident in HBM, loss and gradient left in HBM) and the fused kernel's
+    mean duration from HIP events on its launch stream."""
     import torch
     for _ in range(warmup):
         ev.evaluate(P)
@@ -81,10 +83,10 @@ def time_batched(ev, P, steps: int, warmup: int):
 
 
 def time_plain(ev, P, reps: int = 2):
-    """One pose per launch (set-up + kernel + reduce + combine per pose), FP32 spectrum."""
+    """One pose per launch sequence (set-up + kernel + reduce + combine per pose), FP32 spectrum."""
     import torch
     n = P.shape[0]
-    singles = [P[k:k + 1] for k in range(n)]
+    singles = [P[k:k + 1].contiguous() for k in range(n)]
     for s in singles[:4]:
         ev.evaluate(s)
     torch.cuda.synchronize()
@@ -152,7 +154,7 @@ def main(argv=None) -> int:
     log("spectrum precompute (float64, device) + packing")
     r = DifferentiableFourierVolumeRenderer(vol, pad_factor=2.0, storage="f32", device=dev)
     T_ap, T_lat = make_targets(r, true_pose)
-    P = cands
+    P = torch.from_numpy(np.ascontiguousarray(cands, dtype=np.float64)).to(dev)
     results = {}
     storages = [args.storage] if args.no_extra else sorted({args.storage, "f32"}, key=lambda s: s != args.storage)
     for st in storages:
@@ -160,7 +162,7 @@ def main(argv=None) -> int:
             r.repack(st)
         ev = BiplanarEvaluator(r.packed, T_ap, T_lat, adjoint=True)
         step_ms, kern_ms, _ = time_batched(ev, P, args.steps, args.warmup)
-        loss, grad = ev.evaluate(P)
+        loss, grad = ev.evaluate(cands)
         results[st] = {"ms_per_step": step_ms, "kernel_ms": kern_ms, "value": args.n_cand / (step_ms * 1e-3),
                        "loss_mean": float(loss.mean()), "loss": loss, "grad": grad}
         log(f"{st}: {step_ms:.3f} ms/step, fused kernel {kern_ms:.3f} ms, "

@@ -31,6 +31,9 @@ void set_error(const char* fmt, ...);
 
 constexpr int kWave = 64;
 
+// rows of one fused-kernel tile (4 waves x 16 rows); tiles are 64 columns wide
+constexpr int kSnccTileH = 64;
+
 // fp16 / bf16 storage element -> float
 __device__ __forceinline__ float h2f(uint16_t h) { return __half2float(__ushort_as_half(h)); }
 __device__ __forceinline__ float b2f(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }

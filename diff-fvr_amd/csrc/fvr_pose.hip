@@ -1,0 +1,220 @@
+// Convention-N pose set-up and biplanar combine on the device, so that a batched
+// loss + gradient evaluation is three stream-ordered launches with no host sync:
+//   fvr_pn_build_items      poses -> (AP, LAT) work items + du/dtheta, dv/dtheta
+//   fvr_sncc_eval_v         fused slice / Spectral-NCC / adjoint (fvr_sncc.hip)
+//   fvr_pn_reduce_combine   per-pose fixed-order reduction of the tile partials
+//                           and L = 1/2(1-NCC_AP) + 1/2(1-NCC_LAT) with its
+//                           exact 6-DoF gradient (gradients.py:76-108).
+// Geometry follows diff_fvr.py:56-150: rot = Rz(yaw) Ry(roll) Rx(pitch) in
+// (Z, Y, X) index space, AP u = rot x_hat, LAT u = rot y_hat, v = rot z_hat,
+// sample (i, j) at c + (j - W/2) u + (i - H/2) v with c = (pz, py, px)/2.
+// Written as straight-line code: no dynamically indexed local arrays.
+#include <cmath>
+
+#include "fvr_common.h"
+
+namespace fvr {
+
+struct V3 {
+  double a0, a1, a2;  // (z, y, x)
+};
+
+__device__ __forceinline__ V3 v3(double a, double b, double c) { return {a, b, c}; }
+
+__device__ __forceinline__ int64_t to_fx(double x) { return llrint(x * 4294967296.0); }
+
+__device__ __forceinline__ uint32_t turns_fx(double x) {
+  x -= floor(x);
+  return uint32_t(uint64_t(llrint(x * 4294967296.0)) & 0xFFFFFFFFull);
+}
+
+// layout axes: (z, y, x) or, for the transposed lateral copy, (z, x, y)
+__device__ __forceinline__ void put_fx(int64_t dst[3], const V3& p, bool swap) {
+  dst[0] = to_fx(p.a0);
+  dst[1] = to_fx(swap ? p.a2 : p.a1);
+  dst[2] = to_fx(swap ? p.a1 : p.a2);
+}
+
+__device__ __forceinline__ void put_jac(double* J, const V3& d0, const V3& d1, const V3& d2, bool swap) {
+  // J[a*3 + j]: layout axis a, angle j
+  J[0] = d0.a0;
+  J[1] = d1.a0;
+  J[2] = d2.a0;
+  const double m0 = swap ? d0.a2 : d0.a1, m1 = swap ? d1.a2 : d1.a1, m2 = swap ? d2.a2 : d2.a1;
+  const double f0 = swap ? d0.a1 : d0.a2, f1 = swap ? d1.a1 : d1.a2, f2 = swap ? d2.a1 : d2.a2;
+  J[3] = m0;
+  J[4] = m1;
+  J[5] = m2;
+  J[6] = f0;
+  J[7] = f1;
+  J[8] = f2;
+}
+
+__global__ void pn_build_kernel(const double* __restrict__ poses, int n_poses, fvr_sncc_item tap, fvr_sncc_item tlat,
+                                double pz, double py, double px, int lat_swap, fvr_sncc_item* __restrict__ items,
+                                double* __restrict__ jac) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= 2 * n_poses) return;
+  const int pose = idx >> 1;
+  const bool lat = idx & 1;
+  const double deg = 3.14159265358979323846 / 180.0;
+  const double* P = poses + 6 * pose;
+  const double y = P[0] * deg, p = P[1] * deg, r = P[2] * deg;
+  const double cy = cos(y), sy = sin(y), cp = cos(p), sp = sin(p), cr = cos(r), sr = sin(r);
+  // columns of R = Rz(y) Ry(r) Rx(p): R e_x (col 2), R e_y (col 1), R e_z (col 0)
+  const V3 col0 = v3(cr * cp, cy * sp + sy * sr * cp, sy * sp - cy * sr * cp);
+  const V3 col1 = v3(-cr * sp, cy * cp - sy * sr * sp, sy * cp + cy * sr * sp);
+  const V3 col2 = v3(sr, -sy * cr, cy * cr);
+  // d/dyaw:  dRz Ry Rx   d/dpitch: Rz Ry dRx   d/droll: Rz dRy Rx  (columns)
+  const V3 dy0 = v3(0.0, -sy * sp + cy * sr * cp, cy * sp + sy * sr * cp);
+  const V3 dy1 = v3(0.0, -sy * cp - cy * sr * sp, cy * cp - sy * sr * sp);
+  const V3 dy2 = v3(0.0, -cy * cr, -sy * cr);
+  const V3 dp0 = v3(-cr * sp, cy * cp - sy * sr * sp, sy * cp + cy * sr * sp);
+  const V3 dp1 = v3(-cr * cp, -cy * sp - sy * sr * cp, -sy * sp + cy * sr * cp);
+  const V3 dp2 = v3(0.0, 0.0, 0.0);
+  const V3 dr0 = v3(-sr * cp, sy * cr * cp, -cy * cr * cp);
+  const V3 dr1 = v3(sr * sp, -sy * cr * sp, cy * cr * sp);
+  const V3 dr2 = v3(cr, sy * sr, -cy * sr);
+
+  const V3 u = lat ? col1 : col2;
+  const V3 v = col0;
+  const double H = pz, W = lat ? py : px;
+  const double uu0 = -W / 2.0, vv0 = -H / 2.0;
+  const V3 cg = v3(pz / 2.0 + uu0 * u.a0 + vv0 * v.a0, py / 2.0 + uu0 * u.a1 + vv0 * v.a1,
+                   px / 2.0 + uu0 * u.a2 + vv0 * v.a2);
+  const bool swap = lat && lat_swap;
+  fvr_sncc_item it = lat ? tlat : tap;
+  put_fx(it.c, cg, swap);
+  put_fx(it.ustep, u, swap);
+  put_fx(it.vstep, v, swap);
+  const double a = lat ? P[4] : P[3];  // AP: tx, LAT: ty (gradients.py:90-94)
+  const double b = P[5];               // tz
+  it.ph_col = turns_fx(a / W);
+  it.ph_col0 = turns_fx(uu0 * a / W);
+  it.ph_row = turns_fx(b / H);
+  it.ph_row0 = turns_fx(vv0 * b / H);
+  const int t_ap = tap.tiles_x * ((tap.h + kSnccTileH - 1) / kSnccTileH);
+  const int t_lat = tlat.tiles_x * ((tlat.h + kSnccTileH - 1) / kSnccTileH);
+  it.tile_base = pose * (t_ap + t_lat) + (lat ? t_ap : 0);
+  items[idx] = it;
+  double* J = jac + size_t(idx) * 18;
+  const double k = deg;
+  if (lat) {
+    put_jac(J, v3(dy1.a0 * k, dy1.a1 * k, dy1.a2 * k), v3(dp1.a0 * k, dp1.a1 * k, dp1.a2 * k),
+            v3(dr1.a0 * k, dr1.a1 * k, dr1.a2 * k), swap);
+  } else {
+    put_jac(J, v3(dy2.a0 * k, dy2.a1 * k, dy2.a2 * k), v3(dp2.a0 * k, dp2.a1 * k, dp2.a2 * k),
+            v3(dr2.a0 * k, dr2.a1 * k, dr2.a2 * k), swap);
+  }
+  put_jac(J + 9, v3(dy0.a0 * k, dy0.a1 * k, dy0.a2 * k), v3(dp0.a0 * k, dp0.a1 * k, dp0.a2 * k),
+          v3(dr0.a0 * k, dr0.a1 * k, dr0.a2 * k), swap);
+}
+
+// One wave per pose: fixed-order float64 reduction of each view's tile
+// partials (lane-strided partial sums, then an XOR butterfly that every lane
+// evaluates in the same order), then the biplanar loss and exact gradient.
+__global__ __launch_bounds__(64) void pn_reduce_combine_kernel(const fvr_sncc_item* __restrict__ items, int n_poses,
+                                                               const float* __restrict__ partials,
+                                                               const double* __restrict__ jac, double t2_ap,
+                                                               double t2_lat, double* __restrict__ moments,
+                                                               double* __restrict__ loss, double* __restrict__ grad) {
+  const int pose = blockIdx.x;
+  const int lane = threadIdx.x;
+  const double two_pi = 6.283185307179586476925286766559;
+  double g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double L = 0.0;
+#pragma unroll
+  for (int view = 0; view < 2; ++view) {
+    const fvr_sncc_item& it = items[2 * pose + view];
+    const int ntile = it.tiles_x * ((it.h + kSnccTileH - 1) / kSnccTileH);
+    double m[FVR_SNCC_NMOM];
+#pragma unroll
+    for (int q = 0; q < FVR_SNCC_NMOM; ++q) m[q] = 0.0;
+    for (int t = lane; t < ntile; t += 64) {
+      const float* pp = partials + size_t(it.tile_base + t) * FVR_SNCC_NMOM;
+#pragma unroll
+      for (int q = 0; q < FVR_SNCC_NMOM; ++q) m[q] += double(pp[q]);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+      for (int q = 0; q < FVR_SNCC_NMOM; ++q) m[q] += __shfl_xor(m[q], o);
+    if (moments && lane < FVR_SNCC_NMOM) {
+      double mv = 0.0;
+#pragma unroll
+      for (int q = 0; q < FVR_SNCC_NMOM; ++q) mv = q == lane ? m[q] : mv;
+      moments[size_t(2 * pose + view) * FVR_SNCC_NMOM + lane] = mv;
+    }
+    const double* J = jac + size_t(2 * pose + view) * 18;
+    const double nt = sqrt(view ? t2_lat : t2_ap), ns = sqrt(fmax(m[1], 0.0));
+    double ncc = 0.0, d0 = 0.0, d1 = 0.0, d2 = 0.0, da = 0.0, db = 0.0;
+    if (ns * nt != 0.0) {
+      const double inv = 1.0 / (ns * nt);
+      ncc = m[0] * inv;
+      double G0 = 0.0, G1 = 0.0, G2 = 0.0, H0 = 0.0, H1 = 0.0, H2 = 0.0;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        G0 += m[4 + a] * J[a * 3 + 0] + m[7 + a] * J[9 + a * 3 + 0];
+        G1 += m[4 + a] * J[a * 3 + 1] + m[7 + a] * J[9 + a * 3 + 1];
+        G2 += m[4 + a] * J[a * 3 + 2] + m[7 + a] * J[9 + a * 3 + 2];
+        H0 += m[10 + a] * J[a * 3 + 0] + m[13 + a] * J[9 + a * 3 + 0];
+        H1 += m[10 + a] * J[a * 3 + 1] + m[13 + a] * J[9 + a * 3 + 1];
+        H2 += m[10 + a] * J[a * 3 + 2] + m[13 + a] * J[9 + a * 3 + 2];
+      }
+      d0 = G0 * inv - ncc * H0 / m[1];
+      d1 = G1 * inv - ncc * H1 / m[1];
+      d2 = G2 * inv - ncc * H2 / m[1];
+      da = two_pi / double(it.w) * m[2] * inv;  // d/da: 2 pi / W sum uu Im(S T*)
+      db = two_pi / double(it.h) * m[3] * inv;  // d/db: 2 pi / H sum vv Im(S T*)
+    }
+    L += 0.5 * (1.0 - ncc);
+    g[0] -= 0.5 * d0;
+    g[1] -= 0.5 * d1;
+    g[2] -= 0.5 * d2;
+    if (view == 0) g[3] -= 0.5 * da;
+    else g[4] -= 0.5 * da;
+    g[5] -= 0.5 * db;
+  }
+  if (lane == 0) loss[pose] = L;
+  if (lane < 6) {
+    double gv = 0.0;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) gv = a == lane ? g[a] : gv;
+    grad[6 * pose + lane] = gv;
+  }
+}
+
+}  // namespace fvr
+
+using namespace fvr;
+
+extern "C" {
+
+int fvr_pn_build_items(const double* poses, int32_t n_poses, const fvr_sncc_item* tmpl_host, double pz, double py,
+                       double px, int32_t lat_swap, fvr_sncc_item* items_dev, double* jac_dev, void* stream) {
+  FVR_CHECK_ARG(poses && tmpl_host && items_dev && jac_dev && n_poses > 0, "fvr_pn_build_items: bad arguments");
+  FVR_CHECK_ARG(tmpl_host[0].w == int(px) && tmpl_host[0].h == int(pz) && tmpl_host[1].h == int(pz) &&
+                    tmpl_host[1].w == int(py),
+                "fvr_pn_build_items: templates do not match the padded dims");
+  const int64_t t_pose = int64_t(tmpl_host[0].tiles_x) * ((tmpl_host[0].h + kSnccTileH - 1) / kSnccTileH) +
+                         int64_t(tmpl_host[1].tiles_x) * ((tmpl_host[1].h + kSnccTileH - 1) / kSnccTileH);
+  FVR_CHECK_ARG(t_pose > 0 && int64_t(n_poses) * t_pose < (int64_t(1) << 31),
+                "fvr_pn_build_items: %d poses x %lld tiles overflow the 32-bit tile index", n_poses,
+                (long long)t_pose);
+  const int n = 2 * n_poses;
+  hipLaunchKernelGGL(pn_build_kernel, dim3((n + 63) / 64), dim3(64), 0, reinterpret_cast<hipStream_t>(stream), poses,
+                     n_poses, tmpl_host[0], tmpl_host[1], pz, py, px, lat_swap, items_dev, jac_dev);
+  FVR_CHECK_LAUNCH("pn_build_kernel");
+  return FVR_OK;
+}
+
+int fvr_pn_reduce_combine(const fvr_sncc_item* items_dev, int32_t n_poses, const float* partials, const double* jac,
+                          double t2_ap, double t2_lat, double* moments, double* loss, double* grad, void* stream) {
+  FVR_CHECK_ARG(items_dev && partials && jac && loss && grad && n_poses > 0, "fvr_pn_reduce_combine: bad arguments");
+  hipLaunchKernelGGL(pn_reduce_combine_kernel, dim3(n_poses), dim3(64), 0, reinterpret_cast<hipStream_t>(stream),
+                     items_dev, n_poses, partials, jac, t2_ap, t2_lat, moments, loss, grad);
+  FVR_CHECK_LAUNCH("pn_reduce_combine_kernel");
+  return FVR_OK;
+}
+
+}  // extern "C"

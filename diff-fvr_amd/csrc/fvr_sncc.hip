@@ -25,7 +25,7 @@ namespace fvr {
 
 constexpr int kSnccWaves = 4;
 constexpr int kSnccRows = 16;  // rows per wave
-constexpr int kSnccTileH = kSnccWaves * kSnccRows;
+static_assert(kSnccWaves * kSnccRows == kSnccTileH, "tile height");
 constexpr float kTwoM32 = 2.3283064365386963e-10f;  // 2^-32
 
 typedef float f4a8 __attribute__((ext_vector_type(4), aligned(8)));
@@ -154,7 +154,7 @@ __global__ __launch_bounds__(256) void sncc_kernel(const fvr_layout* __restrict_
   const int tyi = lt / it.tiles_x, txi = lt - tyi * it.tiles_x;
   const int j = txi * 64 + lane;
   const int i0 = tyi * kSnccTileH + wave * kSnccRows;
-  const bool colok = j < iw;
+  const bool colok = uint32_t(j) < uint32_t(iw);  // unsigned: a malformed tile index reads nothing
 
   constexpr uint32_t esz = Gather<DT>::kElem;
   const uint32_t rowb = L.row_bytes, planeb = L.plane_bytes;
@@ -194,7 +194,7 @@ __global__ __launch_bounds__(256) void sncc_kernel(const fvr_layout* __restrict_
 #pragma unroll 4
   for (int k = 0; k < kSnccRows; ++k) {
     const int i = i0 + k;
-    const bool ok = colok & (i < ih);
+    const bool ok = colok & (uint32_t(i) < uint32_t(ih));
     const bool inb = ok & (uint64_t(p0) <= h0) & (uint64_t(p1) <= h1) & (uint64_t(p2) <= h2);
     const uint32_t fs = inb ? uint32_t(uint64_t(p0) >> 32) : 0u;
     const uint32_t fm = inb ? uint32_t(uint64_t(p1) >> 32) : 0u;

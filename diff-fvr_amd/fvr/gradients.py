@@ -107,8 +107,14 @@ class SpectralGradientEngine:
                          -0.5 * (d_ap_dtz + d_lat_dtz)], dtype=np.float64)
         return float(loss), grad
 
-    def compute_biplanar_loss_and_gradients_batch(self, poses, f_target_ap, f_target_lat
-                                                  ) -> Tuple[np.ndarray, np.ndarray]:
-        """Batched exact loss + gradient: poses (N, 6) -> float64 (N,), (N, 6), one fused launch."""
+    def compute_biplanar_loss_and_gradients_batch(self, poses, f_target_ap, f_target_lat):
+        """Batched exact loss + gradient: poses (N, 6) -> float64 (N,), (N, 6).
+
+        Device-set-up, one fused launch, device combine.  A device tensor of poses
+        gives device tensors (owned by the caller); a NumPy array gives NumPy arrays.
+        """
         ev = self._evaluator(f_target_ap, f_target_lat, adjoint=True)
+        if isinstance(poses, torch.Tensor) and poses.is_cuda:
+            loss, grad = ev.evaluate(poses)
+            return loss.clone(), grad.clone()
         return ev.evaluate(poses)

@@ -118,10 +118,15 @@ class BiplanarEvaluator:
     """Batched convention-N biplanar loss + exact 6-DoF gradient.
 
     Loss per pose: L = 1/2 (1 - NCC_AP) + 1/2 (1 - NCC_LAT) with AP in-plane
-    translation (tx, tz) and LAT (ty, tz) (gradients.py:76-108).  Per call:
-    float64 pose set-up on the host (vectorised), one fused HIP launch over all
-    2N items, the fixed-order float64 reduction on the device, and the float64
-    biplanar combine on the host from the 16 moments per item.
+    translation (tx, tz) and LAT (ty, tz) (gradients.py:76-108).
+
+    ``evaluate`` runs three stream-ordered launches with no host sync and no
+    allocation once the workspace is sized (``prepare``): float64 pose set-up
+    (``fvr_pn_build_items``), the fused kernel, and the per-pose fixed-order
+    reduction + combine (``fvr_pn_reduce_combine``).  Tensor poses give device
+    results (views of the workspace, overwritten by the next call); NumPy poses
+    give NumPy results.  ``evaluate_host`` builds the items on the host and
+    combines there (the cross-check of the device set-up).
     """
 
     def __init__(self, packed: PackedSpectrum, target_ap: torch.Tensor, target_lat: torch.Tensor,
@@ -140,6 +145,12 @@ class BiplanarEvaluator:
             _items.n_template(packed, "lateral", self.targets[1].data_ptr(), self.flags, self.rows)]),
             _lib.ITEM_DTYPE)
         self.lat_perm = packed.layout_for("lateral")[1]
+        self.lat_swap = int(tuple(self.lat_perm) == (0, 2, 1))
+        # every address-forming field the device builder copies comes from these templates
+        _items.validate_items(self.build(np.zeros((1, 6)))[0], len(self.packed.descs), self.packed.descs, self.rows)
+        ta = _items.tiles_for(*self.hw["ap"], self.rows)
+        tl = _items.tiles_for(*self.hw["lateral"], self.rows)
+        self.tiles_per_pose = ta[0] * ta[1] + tl[0] * tl[1]
         self.ws = _Workspace(self.device)
         self.last_moments: Optional[np.ndarray] = None
 
@@ -152,6 +163,75 @@ class BiplanarEvaluator:
             self.targets[k].copy_(t)
             self.t2[k] = target_energy(self.targets[k])
 
+    # -- device path ---------------------------------------------------------
+    def prepare(self, n: int):
+        """Size the workspace for n poses (no allocation in later calls of size <= n)."""
+        ws = self.ws
+        return (ws.get("poses", 6 * n, torch.float64),
+                ws.get("items", 2 * n * _lib.ITEM_DTYPE.itemsize, torch.uint8),
+                ws.get("jac", 2 * n * 18, torch.float64),
+                ws.get("partials", n * self.tiles_per_pose * _lib.NMOM, torch.float32),
+                ws.get("moments", 2 * n * _lib.NMOM, torch.float64),
+                ws.get("loss", n, torch.float64),
+                ws.get("grad", 6 * n, torch.float64))
+
+    def build_device(self, poses: torch.Tensor) -> int:
+        """Device pose set-up only: items + Jacobians into the workspace; returns N."""
+        _lib.require_device(poses, "BiplanarEvaluator.build_device")
+        if poses.dtype != torch.float64 or poses.dim() != 2 or poses.shape[1] != 6 or not poses.is_contiguous():
+            raise ValueError("poses must be a contiguous (N, 6) float64 tensor")
+        n = int(poses.shape[0])
+        if n < 1:
+            raise ValueError("empty pose batch")
+        _, items, jac = self.prepare(n)[:3]
+        pz, py, px = self.packed.shape
+        _lib.check(_lib.lib().fvr_pn_build_items(_lib.ptr(poses), n, self.templates.ctypes.data, float(pz),
+                                                 float(py), float(px), self.lat_swap, _lib.ptr(items),
+                                                 _lib.ptr(jac), _lib.stream_ptr(self.device)),
+                   "fvr_pn_build_items")
+        return n
+
+    def evaluate_device(self, poses: torch.Tensor, kernel_events=None, want_moments: bool = True):
+        """poses: contiguous (N, 6) float64 device tensor -> (loss (N,), grad (N, 6)) device views."""
+        n = self.build_device(poses)
+        _, items, jac, partials, moments, loss, grad = self.prepare(n)
+        L = _lib.lib()
+        st = _lib.stream_ptr(self.device)
+        if kernel_events is not None:
+            kernel_events[0].record()
+        _lib.check(L.fvr_sncc_eval_v(_lib.ptr(self.packed.table), len(self.packed.descs), _lib.ptr(items), 2 * n,
+                                     n * self.tiles_per_pose, _lib.ptr(partials),
+                                     self.packed.code | (self.flags << 4), st), "fvr_sncc_eval")
+        if kernel_events is not None:
+            kernel_events[1].record()
+        _lib.check(L.fvr_pn_reduce_combine(_lib.ptr(items), n, _lib.ptr(partials), _lib.ptr(jac), float(self.t2[0]),
+                                           float(self.t2[1]), _lib.ptr(moments) if want_moments else None,
+                                           _lib.ptr(loss), _lib.ptr(grad), st), "fvr_pn_reduce_combine")
+        return loss[:n], grad[: 6 * n].view(n, 6)
+
+    def device_items(self, n: int) -> np.ndarray:
+        """The items of the last device build (for cross-checks)."""
+        raw = self.ws.bufs["items"][: 2 * n * _lib.ITEM_DTYPE.itemsize].cpu().numpy()
+        return raw.view(_lib.ITEM_DTYPE).copy()
+
+    def device_jacobians(self, n: int) -> np.ndarray:
+        return self.ws.bufs["jac"][: 2 * n * 18].cpu().numpy().reshape(2 * n, 2, 3, 3).copy()
+
+    def evaluate(self, poses, kernel_events=None):
+        """poses (N, 6) (deg, voxels) -> (loss (N,), grad (N, 6)) float64.
+
+        Device tensors in -> device tensors out (workspace views); NumPy in -> NumPy out.
+        """
+        if isinstance(poses, torch.Tensor) and poses.is_cuda:
+            return self.evaluate_device(poses.to(torch.float64).contiguous(), kernel_events)
+        P = np.ascontiguousarray(np.asarray(poses, dtype=np.float64).reshape(-1, 6))
+        dev_poses = self.prepare(len(P))[0][: P.size].view(len(P), 6)
+        dev_poses.copy_(torch.from_numpy(P))
+        loss, grad = self.evaluate_device(dev_poses, kernel_events)
+        self.last_moments = self.ws.bufs["moments"][: 2 * len(P) * _lib.NMOM].view(2 * len(P), _lib.NMOM).cpu().numpy()
+        return loss.cpu().numpy(), grad.cpu().numpy()
+
+    # -- host-built path (cross-check) -----------------------------------------
     def build(self, poses: np.ndarray):
         """Validated items and Jacobians for a (N, 6) pose batch (host, float64)."""
         items, jac, n_tiles = _items.n_biplanar_items(self.templates, self.packed.shape, poses, self.lat_perm,
@@ -160,43 +240,17 @@ class BiplanarEvaluator:
         assert n_chk == n_tiles
         return items, jac, n_tiles
 
-    def launch(self, items: np.ndarray, n_tiles: int, kernel_events=None) -> torch.Tensor:
-        """Fused kernel + reduction for prebuilt items; returns the device moments (2N, 16)."""
-        red = self.ws.get("red", len(items) * _lib.NMOM, torch.float64)[: len(items) * _lib.NMOM]
-        red = red.view(len(items), _lib.NMOM)
-        if kernel_events is not None:
-            # time the fused kernel alone: wrap eval between events, reduction after
-            L = _lib.lib()
-            dev_items = self.ws.get("items", len(items) * _lib.ITEM_DTYPE.itemsize, torch.uint8)
-            dev_items[: len(items) * _lib.ITEM_DTYPE.itemsize].copy_(
-                torch.from_numpy(_lib.struct_bytes(items, _lib.ITEM_DTYPE)))
-            partials = self.ws.get("partials", n_tiles * _lib.NMOM, torch.float32)
-            st = _lib.stream_ptr(self.device)
-            kernel_events[0].record()
-            _lib.check(L.fvr_sncc_eval_v(_lib.ptr(self.packed.table), len(self.packed.descs), _lib.ptr(dev_items),
-                                         len(items), n_tiles, _lib.ptr(partials),
-                                         self.packed.code | (self.flags << 4), st), "fvr_sncc_eval")
-            kernel_events[1].record()
-            _lib.check(L.fvr_sncc_reduce(_lib.ptr(dev_items), len(items), _lib.ptr(partials), _lib.ptr(red), st),
-                       "fvr_sncc_reduce")
-        else:
-            launch_items(self.packed, items, n_tiles, self.flags, self.ws, red)
-        return red
-
-    def combine(self, mom: np.ndarray, jac: np.ndarray):
-        """float64 loss (N,) and gradient (N, 6) from item moments (host)."""
-        return combine_biplanar(mom, jac, self.t2, self.hw)
-
-    def evaluate(self, poses, kernel_events=None):
-        """poses (N, 6) (deg, voxels; NumPy or tensor) -> (loss (N,), grad (N, 6)) float64 NumPy."""
+    def evaluate_host(self, poses):
+        """Host-built items, fused kernel + per-item reduction, host float64 combine (NumPy out)."""
         if isinstance(poses, torch.Tensor):
             poses = poses.detach().to("cpu").numpy()
         poses = np.asarray(poses, dtype=np.float64).reshape(-1, 6)
         items, jac, n_tiles = self.build(poses)
-        red = self.launch(items, n_tiles, kernel_events)
+        red = torch.empty((len(items), _lib.NMOM), dtype=torch.float64, device=self.device)
+        hws = _Workspace(self.device)
+        launch_items(self.packed, items, n_tiles, self.flags, hws, red)
         mom = red.cpu().numpy()
-        self.last_moments = mom
-        return self.combine(mom, jac)
+        return combine_biplanar(mom, jac, self.t2, self.hw)
 
 
 def combine_biplanar(mom: np.ndarray, jac: np.ndarray, t2, hw):

@@ -138,6 +138,24 @@ int fvr_sncc_eval_v(const fvr_layout* layouts_dev, int32_t n_layouts, const fvr_
 int fvr_sncc_reduce(const fvr_sncc_item* items_dev, int32_t n_items, const float* partials,
                     double* out, void* stream);
 
+/* Convention-N pose set-up on the device (float64): 2 items per pose (AP then
+ * LAT) from poses[n][6] (yaw, pitch, roll in degrees; tx, ty, tz in voxels),
+ * copying the static fields of tmpl_host[2] (host structs: AP, LAT, already
+ * validated) and writing jac[2n][2][3][3] (du/dtheta, dv/dtheta per degree in
+ * layout axes).  lat_swap = 1 when the LAT template uses the (z, x, y) copy.
+ * Replaces the per-pose set-up of diff_fvr.py:56-150 / gradients.py:89-95. */
+int fvr_pn_build_items(const double* poses, int32_t n_poses, const fvr_sncc_item* tmpl_host, double pz,
+                       double py, double px, int32_t lat_swap, fvr_sncc_item* items_dev, double* jac_dev,
+                       void* stream);
+
+/* Per-pose fixed-order float64 reduction of the fused kernel's tile partials
+ * and the biplanar loss L = 1/2(1-NCC_AP) + 1/2(1-NCC_LAT) with its exact
+ * 6-DoF gradient (gradients.py:76-108): loss[n], grad[n][6]; optionally the
+ * item moments[2n][FVR_SNCC_NMOM] (may be NULL). */
+int fvr_pn_reduce_combine(const fvr_sncc_item* items_dev, int32_t n_poses, const float* partials,
+                          const double* jac, double t2_ap, double t2_lat, double* moments, double* loss,
+                          double* grad, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Generic fourier_slice_3d (grid_sample semantics, planar (B,2,D,H,W)).      */
 /* ------------------------------------------------------------------------ */

@@ -109,7 +109,9 @@ def test_batched_equals_single_and_is_deterministic(setup):
     eng = SpectralGradientEngine(r)
     poses = random_poses(9, 15.0, 5.0, seed=3456)
     P = torch.from_numpy(poses).to(r.device)
-    L1, g1 = [t.copy() for t in eng.compute_biplanar_loss_and_gradients_batch(P, T_ap, T_la)]
+    out_dev = eng.compute_biplanar_loss_and_gradients_batch(P, T_ap, T_la)
+    assert all(isinstance(t, torch.Tensor) and t.is_cuda for t in out_dev)  # device in -> device out
+    L1, g1 = [t.cpu().numpy().copy() for t in out_dev]
     L2, g2 = [t.copy() for t in eng.compute_biplanar_loss_and_gradients_batch(poses, T_ap, T_la)]
     np.testing.assert_array_equal(L1, L2)
     np.testing.assert_array_equal(g1, g2)
@@ -171,3 +173,35 @@ def test_reduced_precision_storage(storage, tol_slice, tol_grad):
         Lr, gr = O.biplanar_loss_and_grad_analytic(f3, np.array(pose), T_ap, T_la)
         assert abs(L - Lr) <= (1e-3 if storage == "fp16" else 1e-2)
         assert np.max(np.abs(g - gr)) <= tol_grad * np.max(np.abs(gr))
+
+
+def test_device_setup_matches_host_setup(setup):
+    """The device pose builder reproduces the host builder's items and Jacobians, and the
+    device reduction + combine reproduces the host float64 combine."""
+    from fvr import _lib
+    from fvr.phantom import random_poses
+    from fvr.sncc import BiplanarEvaluator
+    _, r, f3 = setup
+    T_ap, T_la = _targets(f3, [1.0, -2.0, 1.5, 0.7, -1.2, 0.4])
+    dev = r.device
+    ev = BiplanarEvaluator(r.packed, torch.from_numpy(T_ap.astype(np.complex64)).to(dev),
+                           torch.from_numpy(T_la.astype(np.complex64)).to(dev))
+    poses = random_poses(6, 15.0, 5.0, seed=77)
+    P = torch.from_numpy(poses).to(dev)
+    ev.build_device(P)  # set-up only: the items are checked before any gather reads them
+    got = ev.device_items(len(poses))
+    ref, jac_ref, _ = ev.build(poses)
+    for f in _lib.ITEM_DTYPE.names:
+        if f in ("c", "ustep", "vstep"):
+            # cos/sin of the two libms may differ in the last ulp: a few units of 2^-32 voxel
+            assert np.max(np.abs(got[f].astype(np.int64) - ref[f].astype(np.int64))) <= 4, f
+        elif f.startswith("ph_"):
+            d = (got[f].astype(np.int64) - ref[f].astype(np.int64)) % (1 << 32)
+            assert np.all((d <= 4) | (d >= (1 << 32) - 4)), f
+        else:
+            np.testing.assert_array_equal(got[f], ref[f], err_msg=f)
+    np.testing.assert_allclose(ev.device_jacobians(len(poses)), jac_ref, atol=1e-15)
+    L_d, g_d = [t.cpu().numpy() for t in ev.evaluate_device(P)]
+    L_h, g_h = ev.evaluate_host(poses)
+    np.testing.assert_allclose(L_d, L_h, rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(g_d, g_h, rtol=1e-6, atol=1e-9 * np.abs(g_h).max())
