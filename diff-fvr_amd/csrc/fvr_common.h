@@ -31,8 +31,12 @@ void set_error(const char* fmt, ...);
 
 constexpr int kWave = 64;
 
-// rows of one fused-kernel tile (4 waves x 16 rows); tiles are 64 columns wide
-constexpr int kSnccTileH = 64;
+// rows of one fused-kernel tile (4 waves x FVR_SNCC_ROWS rows each); tiles are 64 columns wide
+#ifndef FVR_SNCC_ROWS
+#define FVR_SNCC_ROWS 16
+#endif
+static_assert(FVR_SNCC_ROWS >= 1 && FVR_SNCC_ROWS <= 64, "rows per wave: 1..64 (row phasors live in lanes)");
+constexpr int kSnccTileH = 4 * FVR_SNCC_ROWS;
 
 // fp16 / bf16 storage element -> float
 __device__ __forceinline__ float h2f(uint16_t h) { return __half2float(__ushort_as_half(h)); }

@@ -24,7 +24,7 @@
 namespace fvr {
 
 constexpr int kSnccWaves = 4;
-constexpr int kSnccRows = 16;  // rows per wave
+constexpr int kSnccRows = FVR_SNCC_ROWS;  // rows per wave
 static_assert(kSnccWaves * kSnccRows == kSnccTileH, "tile height");
 constexpr float kTwoM32 = 2.3283064365386963e-10f;  // 2^-32
 
@@ -40,30 +40,30 @@ struct cpx {
   float r, i;
 };
 
-// trilinear value + gradient from 4 fast-axis pairs (already dequantised)
-struct Pair {
-  float r0, i0, r1, i1;
-};
-
+// Raw loads of one sample: the four fast-axis corner pairs (slow, mid) =
+// (0,0), (0,1), (1,0), (1,1) and, for scaled storage, the eight corner scales.
+// They are issued one sample ahead of their use (see the row loop).
 template <int DT>
-struct Gather;
+struct Raw;
 
 template <>
-struct Gather<FVR_C32> {
+struct Raw<FVR_C32> {
   static constexpr uint32_t kElem = 8;
-  __device__ __forceinline__ static void load(gcptr b0, gcptr b1, uint32_t off, uint32_t row, gfptr, int, int, int,
-                                              int, int, int, Pair q[4]) {
+  f4a8 q[4];
+  __device__ __forceinline__ void load(gcptr b0, gcptr b1, uint32_t off, uint32_t row, gcptr, uint32_t, uint32_t,
+                                       uint32_t, uint32_t, uint32_t) {
     typedef const f4a8 __attribute__((address_space(1)))* gp;
     const uint32_t offr = off + row;
-    f4a8 a = *reinterpret_cast<gp>(b0 + off);
-    f4a8 b = *reinterpret_cast<gp>(b0 + offr);
-    f4a8 c = *reinterpret_cast<gp>(b1 + off);
-    f4a8 d = *reinterpret_cast<gp>(b1 + offr);
-    q[0] = {a.x, a.y, a.z, a.w};
-    q[1] = {b.x, b.y, b.z, b.w};
-    q[2] = {c.x, c.y, c.z, c.w};
-    q[3] = {d.x, d.y, d.z, d.w};
+    q[0] = *reinterpret_cast<gp>(b0 + off);
+    q[1] = *reinterpret_cast<gp>(b0 + offr);
+    q[2] = *reinterpret_cast<gp>(b1 + off);
+    q[3] = *reinterpret_cast<gp>(b1 + offr);
   }
+  // corner pair k as (re0, im0, re1, im1)
+  __device__ __forceinline__ float r0(int k) const { return q[k].x; }
+  __device__ __forceinline__ float i0(int k) const { return q[k].y; }
+  __device__ __forceinline__ float r1(int k) const { return q[k].z; }
+  __device__ __forceinline__ float i1(int k) const { return q[k].w; }
 };
 
 template <int DT>
@@ -72,43 +72,58 @@ __device__ __forceinline__ float deq(uint32_t half) {
 }
 
 template <int DT>
-struct GatherQ {
-  // S, M, F: storage indices of the low corner; Bm, Bf: brick counts
+struct RawQ {
   static constexpr uint32_t kElem = 4;
-  __device__ __forceinline__ static void load(gcptr b0, gcptr b1, uint32_t off, uint32_t row, gfptr scales, int S,
-                                              int M, int F, int Bm, int Bf, int, Pair q[4]) {
+  u2a4 q[4];
+  float s[8];  // scale of corner (slow, mid, fast) at index 4 slow + 2 mid + fast
+  // S, M, F: storage indices of the low corner; Bm, Bf: brick counts along mid / fast
+  __device__ __forceinline__ void load(gcptr b0, gcptr b1, uint32_t off, uint32_t row, gcptr scales, uint32_t S,
+                                       uint32_t M, uint32_t F, uint32_t Bm, uint32_t Bf) {
     typedef const u2a4 __attribute__((address_space(1)))* gp;
     const uint32_t offr = off + row;
-    u2a4 a = *reinterpret_cast<gp>(b0 + off);
-    u2a4 b = *reinterpret_cast<gp>(b0 + offr);
-    u2a4 c = *reinterpret_cast<gp>(b1 + off);
-    u2a4 d = *reinterpret_cast<gp>(b1 + offr);
-    const uint32_t bs0 = uint32_t(S) >> 3, bs1 = uint32_t(S + 1) >> 3;
-    const uint32_t bm0 = uint32_t(M) >> 3, bm1 = uint32_t(M + 1) >> 3;
-    const uint32_t bf0 = uint32_t(F) >> 3, bf1 = uint32_t(F + 1) >> 3;
-    const uint32_t r00 = __umul24(__umul24(bs0, Bm) + bm0, Bf);
-    const uint32_t r01 = __umul24(__umul24(bs0, Bm) + bm1, Bf);
-    const uint32_t r10 = __umul24(__umul24(bs1, Bm) + bm0, Bf);
-    const uint32_t r11 = __umul24(__umul24(bs1, Bm) + bm1, Bf);
-    const float s000 = scales[r00 + bf0], s001 = scales[r00 + bf1];
-    const float s010 = scales[r01 + bf0], s011 = scales[r01 + bf1];
-    const float s100 = scales[r10 + bf0], s101 = scales[r10 + bf1];
-    const float s110 = scales[r11 + bf0], s111 = scales[r11 + bf1];
-    q[0] = {deq<DT>(a.x & 0xFFFFu) * s000, deq<DT>(a.x >> 16) * s000, deq<DT>(a.y & 0xFFFFu) * s001,
-            deq<DT>(a.y >> 16) * s001};
-    q[1] = {deq<DT>(b.x & 0xFFFFu) * s010, deq<DT>(b.x >> 16) * s010, deq<DT>(b.y & 0xFFFFu) * s011,
-            deq<DT>(b.y >> 16) * s011};
-    q[2] = {deq<DT>(c.x & 0xFFFFu) * s100, deq<DT>(c.x >> 16) * s100, deq<DT>(c.y & 0xFFFFu) * s101,
-            deq<DT>(c.y >> 16) * s101};
-    q[3] = {deq<DT>(d.x & 0xFFFFu) * s110, deq<DT>(d.x >> 16) * s110, deq<DT>(d.y & 0xFFFFu) * s111,
-            deq<DT>(d.y >> 16) * s111};
+    q[0] = *reinterpret_cast<gp>(b0 + off);
+    q[1] = *reinterpret_cast<gp>(b0 + offr);
+    q[2] = *reinterpret_cast<gp>(b1 + off);
+    q[3] = *reinterpret_cast<gp>(b1 + offr);
+    // byte offsets of the corner bricks' scales (32-bit, from the scale table base)
+    const uint32_t z0 = __umul24(S >> 3, Bm), z1 = __umul24((S + 1) >> 3, Bm);
+    const uint32_t m0 = M >> 3, m1 = (M + 1) >> 3, f0 = (F >> 3) << 2, f1 = ((F + 1) >> 3) << 2;
+    const uint32_t r00 = __umul24(z0 + m0, Bf) << 2, r01 = __umul24(z0 + m1, Bf) << 2;
+    const uint32_t r10 = __umul24(z1 + m0, Bf) << 2, r11 = __umul24(z1 + m1, Bf) << 2;
+    s[0] = *reinterpret_cast<gfptr>(scales + (r00 + f0));
+    s[1] = *reinterpret_cast<gfptr>(scales + (r00 + f1));
+    s[2] = *reinterpret_cast<gfptr>(scales + (r01 + f0));
+    s[3] = *reinterpret_cast<gfptr>(scales + (r01 + f1));
+    s[4] = *reinterpret_cast<gfptr>(scales + (r10 + f0));
+    s[5] = *reinterpret_cast<gfptr>(scales + (r10 + f1));
+    s[6] = *reinterpret_cast<gfptr>(scales + (r11 + f0));
+    s[7] = *reinterpret_cast<gfptr>(scales + (r11 + f1));
   }
+  __device__ __forceinline__ float r0(int k) const { return deq<DT>(q[k].x & 0xFFFFu) * s[2 * k]; }
+  __device__ __forceinline__ float i0(int k) const { return deq<DT>(q[k].x >> 16) * s[2 * k]; }
+  __device__ __forceinline__ float r1(int k) const { return deq<DT>(q[k].y & 0xFFFFu) * s[2 * k + 1]; }
+  __device__ __forceinline__ float i1(int k) const { return deq<DT>(q[k].y >> 16) * s[2 * k + 1]; }
 };
 
 template <>
-struct Gather<FVR_C16F> : GatherQ<FVR_C16F> {};
+struct Raw<FVR_C16F> : RawQ<FVR_C16F> {};
 template <>
-struct Gather<FVR_C16B> : GatherQ<FVR_C16B> {};
+struct Raw<FVR_C16B> : RawQ<FVR_C16B> {};
+
+// one sample in flight: its raw loads, target value and fractional coordinates
+template <int DT>
+struct Smp {
+  Raw<DT> raw;
+  f2v t;
+  float ts, tm, tf;
+  bool ok;
+};
+
+// the 16 moment accumulators of one lane
+struct Acc {
+  float A, E, sim, mrim;
+  float gs[3], gr[3], hs[3], hr[3];
+};
 
 __device__ __forceinline__ float readlane_f(float v, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
@@ -156,17 +171,18 @@ __global__ __launch_bounds__(256) void sncc_kernel(const fvr_layout* __restrict_
   const int i0 = tyi * kSnccTileH + wave * kSnccRows;
   const bool colok = uint32_t(j) < uint32_t(iw);  // unsigned: a malformed tile index reads nothing
 
-  constexpr uint32_t esz = Gather<DT>::kElem;
+  constexpr uint32_t esz = Raw<DT>::kElem;
   const uint32_t rowb = L.row_bytes, planeb = L.plane_bytes;
   const uint32_t lo_off = uint32_t(lo_int) * (planeb + rowb + esz);
   const gcptr b0 = reinterpret_cast<gcptr>(L.base) + lo_off;
   const gcptr b1 = b0 + planeb;
   const uint32_t guard = it.guard_off - lo_off;
-  const gfptr scales = reinterpret_cast<gfptr>(L.scales);
-  const int Bm = L.bricks[1], Bf = L.bricks[2];
+  const gcptr scales = reinterpret_cast<gcptr>(L.scales);
+  const uint32_t Bm = uint32_t(L.bricks[1]), Bf = uint32_t(L.bricks[2]);
   const gcptr Tb = reinterpret_cast<gcptr>(it.target);
   float2* __restrict__ Sout = reinterpret_cast<float2*>(it.slice_out);
 
+  // position of the next sample to issue (row i0 first), 32.32 fixed point
   int64_t p0 = it.c[0] + int64_t(j) * it.ustep[0] + int64_t(i0) * it.vstep[0];
   int64_t p1 = it.c[1] + int64_t(j) * it.ustep[1] + int64_t(i0) * it.vstep[1];
   int64_t p2 = it.c[2] + int64_t(j) * it.ustep[2] + int64_t(i0) * it.vstep[2];
@@ -183,41 +199,46 @@ __global__ __launch_bounds__(256) void sncc_kernel(const fvr_layout* __restrict_
     hy = __expf(-y * y * it.hp_inv2s2);
   }
   const float cw = it.cw0 + float(j) * it.cws;
-  int tidx = i0 * iw + j;  // row-major index of (i, j) in the target / slice
   const uint32_t tstep = uint32_t(iw) * 8u;
-  uint32_t toff = uint32_t(tidx) * 8u;  // its byte offset (targets < 4 GiB)
+  uint32_t toff = uint32_t(i0 * iw + j) * 8u;  // byte offset of the next issued sample's target (< 4 GiB)
 
-  float A = 0.f, E = 0.f, sim = 0.f, mrim = 0.f;
-  float gs[3] = {0.f, 0.f, 0.f}, gr[3] = {0.f, 0.f, 0.f};
-  float hs[3] = {0.f, 0.f, 0.f}, hr[3] = {0.f, 0.f, 0.f};
-
-#pragma unroll 4
-  for (int k = 0; k < kSnccRows; ++k) {
-    const int i = i0 + k;
+  // stage 1: box test, corner addresses and the sample's loads (target included)
+  auto issue = [&](Smp<DT>& sm, int i) {
     const bool ok = colok & (uint32_t(i) < uint32_t(ih));
     const bool inb = ok & (uint64_t(p0) <= h0) & (uint64_t(p1) <= h1) & (uint64_t(p2) <= h2);
     const uint32_t fs = inb ? uint32_t(uint64_t(p0) >> 32) : 0u;
     const uint32_t fm = inb ? uint32_t(uint64_t(p1) >> 32) : 0u;
     const uint32_t ff = inb ? uint32_t(uint64_t(p2) >> 32) : 0u;
-    const float ts = float(uint32_t(p0)) * kTwoM32;
-    const float tm = float(uint32_t(p1)) * kTwoM32;
-    const float tf = float(uint32_t(p2)) * kTwoM32;
+    sm.ts = float(uint32_t(p0)) * kTwoM32;
+    sm.tm = float(uint32_t(p1)) * kTwoM32;
+    sm.tf = float(uint32_t(p2)) * kTwoM32;
     const uint32_t offi = __umul24(fs, planeb) + __umul24(fm, rowb) + ff * esz;
-    const uint32_t off = inb ? offi : guard;
+    sm.raw.load(b0, b1, inb ? offi : guard, rowb, scales, fs + lo_int, fm + lo_int, ff + lo_int, Bm, Bf);
+    sm.t = *reinterpret_cast<gf2ptr>(Tb + (ok ? toff : 0u));
+    sm.ok = ok;
+    p0 += v0;
+    p1 += v1;
+    p2 += v2;
+    toff += tstep;
+  };
 
-    Pair q[4];
-    Gather<DT>::load(b0, b1, off, rowb, scales, int(fs) + lo_int, int(fm) + lo_int, int(ff) + lo_int, Bm, Bf, 0,
-                     q);
-
+  Acc ac = {};
+  // stage 2: interpolation, phase, target product and the moment updates of row i (k = i - i0)
+  auto consume = [&](const Smp<DT>& sm, int k) {
+    const int i = i0 + k;
+    const float ts = sm.ts, tm = sm.tm, tf = sm.tf;
+    const Raw<DT>& q = sm.raw;
     // fast-axis lerps (d = V1 - V0, e = V0 + tf d) for the four (slow, mid) pairs
-    const float d00r = q[0].r1 - q[0].r0, d00i = q[0].i1 - q[0].i0;
-    const float d01r = q[1].r1 - q[1].r0, d01i = q[1].i1 - q[1].i0;
-    const float d10r = q[2].r1 - q[2].r0, d10i = q[2].i1 - q[2].i0;
-    const float d11r = q[3].r1 - q[3].r0, d11i = q[3].i1 - q[3].i0;
-    const float e00r = fmaf(tf, d00r, q[0].r0), e00i = fmaf(tf, d00i, q[0].i0);
-    const float e01r = fmaf(tf, d01r, q[1].r0), e01i = fmaf(tf, d01i, q[1].i0);
-    const float e10r = fmaf(tf, d10r, q[2].r0), e10i = fmaf(tf, d10i, q[2].i0);
-    const float e11r = fmaf(tf, d11r, q[3].r0), e11i = fmaf(tf, d11i, q[3].i0);
+    const float a0r = q.r0(0), a0i = q.i0(0), b0r = q.r0(1), b0i = q.i0(1);
+    const float c0r = q.r0(2), c0i = q.i0(2), e0r_ = q.r0(3), e0i_ = q.i0(3);
+    const float d00r = q.r1(0) - a0r, d00i = q.i1(0) - a0i;
+    const float d01r = q.r1(1) - b0r, d01i = q.i1(1) - b0i;
+    const float d10r = q.r1(2) - c0r, d10i = q.i1(2) - c0i;
+    const float d11r = q.r1(3) - e0r_, d11i = q.i1(3) - e0i_;
+    const float e00r = fmaf(tf, d00r, a0r), e00i = fmaf(tf, d00i, a0i);
+    const float e01r = fmaf(tf, d01r, b0r), e01i = fmaf(tf, d01i, b0i);
+    const float e10r = fmaf(tf, d10r, c0r), e10i = fmaf(tf, d10i, c0i);
+    const float e11r = fmaf(tf, d11r, e0r_), e11i = fmaf(tf, d11i, e0i_);
     // mid-axis
     const float y0r = e01r - e00r, y0i = e01i - e00i;
     const float y1r = e11r - e10r, y1i = e11i - e10i;
@@ -227,12 +248,12 @@ __global__ __launch_bounds__(256) void sncc_kernel(const fvr_layout* __restrict_
     const float zr = f1r - f0r, zi = f1i - f0i;
     const float Ir = fmaf(ts, zr, f0r), Ii = fmaf(ts, zi, f0i);
 
-    // target and phase
-    const f2v t = *reinterpret_cast<gf2ptr>(Tb + (ok ? toff : 0u));
+    // phase and target
+    const f2v t = sm.t;
     const float prr = readlane_f(rowph.r, k), pri = readlane_f(rowph.i, k);
     const float phr = colph.r * prr - colph.i * pri;
     const float phi = fmaf(colph.r, pri, colph.i * prr);
-    if (WS && ok) Sout[tidx] = make_float2(Ir * phr - Ii * phi, fmaf(Ir, phi, Ii * phr));
+    if (WS && sm.ok) Sout[i * iw + j] = make_float2(Ir * phr - Ii * phi, fmaf(Ir, phi, Ii * phr));
     // W = phase * conj(T)
     float Wr = fmaf(phr, t.x, phi * t.y);
     float Wi = fmaf(phi, t.x, -phr * t.y);
@@ -248,10 +269,10 @@ __global__ __launch_bounds__(256) void sncc_kernel(const fvr_layout* __restrict_
     const float rw = rw0 + float(i) * rws;
     const float IWr = fmaf(Ir, Wr, -Ii * Wi);
     const float IWi = fmaf(Ir, Wi, Ii * Wr);
-    A += IWr;
-    sim += IWi;
-    mrim = fmaf(rw, IWi, mrim);
-    E = fmaf(Ir, Jr, fmaf(Ii, Ji, E));
+    ac.A += IWr;
+    ac.sim += IWi;
+    ac.mrim = fmaf(rw, IWi, ac.mrim);
+    ac.E = fmaf(Ir, Jr, fmaf(Ii, Ji, ac.E));
 
     if (ADJ) {
       // gradient: slow = z, mid = y0 + ts (y1 - y0), fast = bilerp of d
@@ -261,39 +282,53 @@ __global__ __launch_bounds__(256) void sncc_kernel(const fvr_layout* __restrict_
       const float gfr = fmaf(ts, x1r - x0r, x0r), gfi = fmaf(ts, x1i - x0i, x0i);
       const float g0 = fmaf(zr, Wr, -zi * Wi), g1 = fmaf(gmr, Wr, -gmi * Wi), g2 = fmaf(gfr, Wr, -gfi * Wi);
       const float k0 = fmaf(zr, Jr, zi * Ji), k1 = fmaf(gmr, Jr, gmi * Ji), k2 = fmaf(gfr, Jr, gfi * Ji);
-      gs[0] += g0;
-      gs[1] += g1;
-      gs[2] += g2;
-      gr[0] = fmaf(rw, g0, gr[0]);
-      gr[1] = fmaf(rw, g1, gr[1]);
-      gr[2] = fmaf(rw, g2, gr[2]);
-      hs[0] += k0;
-      hs[1] += k1;
-      hs[2] += k2;
-      hr[0] = fmaf(rw, k0, hr[0]);
-      hr[1] = fmaf(rw, k1, hr[1]);
-      hr[2] = fmaf(rw, k2, hr[2]);
+      ac.gs[0] += g0;
+      ac.gs[1] += g1;
+      ac.gs[2] += g2;
+      ac.gr[0] = fmaf(rw, g0, ac.gr[0]);
+      ac.gr[1] = fmaf(rw, g1, ac.gr[1]);
+      ac.gr[2] = fmaf(rw, g2, ac.gr[2]);
+      ac.hs[0] += k0;
+      ac.hs[1] += k1;
+      ac.hs[2] += k2;
+      ac.hr[0] = fmaf(rw, k0, ac.hr[0]);
+      ac.hr[1] = fmaf(rw, k1, ac.hr[1]);
+      ac.hr[2] = fmaf(rw, k2, ac.hr[2]);
     }
-    p0 += v0;
-    p1 += v1;
-    p2 += v2;
-    tidx += iw;
-    toff += tstep;
+  };
+
+  // two samples in flight per lane: the loads of row k+1 are issued before row k is consumed
+  static_assert(kSnccRows % 2 == 0, "rows per wave must be even");
+  Smp<DT> sa, sb;
+  issue(sa, i0);
+  // (the scheduling barriers keep each issue after the consume that frees its
+  // registers, so the loop carries no register copies)
+#pragma unroll 1
+  for (int k = 0; k < kSnccRows - 2; k += 2) {
+    issue(sb, i0 + k + 1);
+    consume(sa, k);
+    __builtin_amdgcn_sched_barrier(0);
+    issue(sa, i0 + k + 2);
+    consume(sb, k + 1);
+    __builtin_amdgcn_sched_barrier(0);
   }
+  issue(sb, i0 + kSnccRows - 1);
+  consume(sa, kSnccRows - 2);
+  consume(sb, kSnccRows - 1);
 
   // ---- deterministic workgroup reduction of the 16 moments ----
   __shared__ float red[FVR_SNCC_NMOM][257];
   const int tid = threadIdx.x;
-  red[0][tid] = A;
-  red[1][tid] = E;
-  red[2][tid] = cw * sim;
-  red[3][tid] = mrim;
+  red[0][tid] = ac.A;
+  red[1][tid] = ac.E;
+  red[2][tid] = cw * ac.sim;
+  red[3][tid] = ac.mrim;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    red[4 + a][tid] = cw * gs[a];
-    red[7 + a][tid] = gr[a];
-    red[10 + a][tid] = cw * hs[a];
-    red[13 + a][tid] = hr[a];
+    red[4 + a][tid] = cw * ac.gs[a];
+    red[7 + a][tid] = ac.gr[a];
+    red[10 + a][tid] = cw * ac.hs[a];
+    red[13 + a][tid] = ac.hr[a];
   }
   __syncthreads();
   const int qn = tid >> 4, s = tid & 15;
