@@ -23,6 +23,10 @@ __device__ __forceinline__ V3 v3(double a, double b, double c) { return {a, b, c
 
 __device__ __forceinline__ int64_t to_fx(double x) { return llrint(x * 4294967296.0); }
 
+// +2^-22 voxel on every sample coordinate: a coordinate that float64 puts exactly
+// on a lattice plane stays on it after rounded fixed-point stepping
+constexpr int64_t kSnapFx = 1 << 10;
+
 __device__ __forceinline__ uint32_t turns_fx(double x) {
   x -= floor(x);
   return uint32_t(uint64_t(llrint(x * 4294967296.0)) & 0xFFFFFFFFull);
@@ -85,6 +89,7 @@ __global__ void pn_build_kernel(const double* __restrict__ poses, int n_poses, f
   const bool swap = lat && lat_swap;
   fvr_sncc_item it = lat ? tlat : tap;
   put_fx(it.c, cg, swap);
+  for (int a = 0; a < 3; ++a) it.c[a] += kSnapFx;  // lattice-plane bias, as the host builder
   put_fx(it.ustep, u, swap);
   put_fx(it.vstep, v, swap);
   const double a = lat ? P[4] : P[3];  // AP: tx, LAT: ty (gradients.py:90-94)

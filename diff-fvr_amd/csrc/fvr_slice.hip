@@ -158,7 +158,8 @@ int fvr_slice_fwd(uint64_t vol, int32_t vol_dtype, int32_t b_vol, int32_t d, int
                   int32_t k, float cx, float cy, float cz, uint64_t out, void* stream) {
   FVR_CHECK_ARG(vol && u_xyz && v_xyz && fx && fy && out, "fvr_slice_fwd: null pointer");
   FVR_CHECK_ARG(b_vol > 0 && d > 0 && h > 0 && w > 0 && n_pose > 0 && k > 0, "fvr_slice_fwd: empty shape");
-  FVR_CHECK_ARG(size_t(b_vol) * n_pose < 65536u * 1024u, "fvr_slice_fwd: too many (volume, pose) pairs");
+  FVR_CHECK_ARG(int64_t(b_vol) * n_pose <= 65535, "fvr_slice_fwd: %lld (volume, pose) pairs exceed the grid's y limit",
+                (long long)b_vol * n_pose);
   SliceGeom g{b_vol, d, h, w, n_pose, k, cx, cy, cz};
   dim3 grid((k + 255) / 256, b_vol * n_pose);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
@@ -181,6 +182,7 @@ int fvr_slice_bwd(uint64_t vol, int32_t vol_dtype, int32_t b_vol, int32_t d, int
                   int32_t k, float cx, float cy, float cz, float* part, float* gu, float* gv, void* stream) {
   FVR_CHECK_ARG(vol && grad_out && u_xyz && v_xyz && fx && fy && part && gu && gv, "fvr_slice_bwd: null pointer");
   FVR_CHECK_ARG(b_vol > 0 && d > 0 && h > 0 && w > 0 && n_pose > 0 && k > 0, "fvr_slice_bwd: empty shape");
+  FVR_CHECK_ARG(n_pose <= 65535, "fvr_slice_bwd: %d poses exceed the grid's y limit", n_pose);
   SliceGeom g{b_vol, d, h, w, n_pose, k, cx, cy, cz};
   const int nq = (k + 255) / 256;
   dim3 grid(nq, n_pose);

@@ -14,6 +14,8 @@ from .gradient_optimizer import GradientRegistrationOptimizer
 from .core_engine import RobustDiffFVREngine, detect_radio_opaque_metal
 from .registration_3d2d import compute_mtre_6dof
 from .phantom import ellipsoid_phantom
+from .slice_ops import fourier_slice_3d
+from .torch_engine import TorchDiffFVREngine
 
 __version__ = "0.1.0"
 
@@ -25,4 +27,6 @@ __all__ = [
     "detect_radio_opaque_metal",
     "compute_mtre_6dof",
     "ellipsoid_phantom",
+    "fourier_slice_3d",
+    "TorchDiffFVREngine",
 ]

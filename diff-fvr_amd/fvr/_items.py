@@ -22,6 +22,13 @@ from . import _lib
 
 FX = float(2 ** 32)
 DEG = math.pi / 180.0
+# Every sample coordinate is biased by +2^-22 voxel (and the convention-N box
+# edge with it).  Stepping with rounded 32.32 steps can leave a coordinate
+# that is exactly on a lattice plane in float64 (identity poses, grid end
+# points) up to ~2^-23 voxel below it; the bias puts such samples on the plane,
+# so floor(), the box test and the one-sided derivative there agree with the
+# float64 / float32 computations of the reference.
+SNAP_FX = 1 << 10
 
 
 def to_fx(x) -> np.ndarray:
@@ -100,11 +107,11 @@ def n_item(packed, view: str, yaw, pitch, roll, a, b, target_ptr: int, flags: in
     c = np.array(pad_shape, dtype=np.float64) / 2.0
     cg = c + uu0 * u + vv0 * v
     it = np.zeros(1, dtype=_lib.ITEM_DTYPE)
-    it["c"] = to_fx(cg[list(perm)])
+    it["c"] = to_fx(cg[list(perm)]) + SNAP_FX
     it["ustep"] = to_fx(u[list(perm)])
     it["vstep"] = to_fx(v[list(perm)])
     n = np.array(desc["n"], dtype=np.int64)
-    it["box_hi"] = ((n - 1) << 32).astype(np.uint64)
+    it["box_hi"] = ((n - 1) << 32).astype(np.uint64) + np.uint64(SNAP_FX)
     it["ph_col"] = turns_fx(a / w)
     it["ph_col0"] = turns_fx(uu0 * a / w)
     it["ph_row"] = turns_fx(b / h)
@@ -201,7 +208,7 @@ def n_biplanar_items(templates: np.ndarray, pad_shape, poses: np.ndarray, lat_pe
         cg = c[None, :] + uu0 * u + vv0 * v
         sl = items[view::2]
         sl[:] = t
-        sl["c"] = to_fx(cg[:, perm])
+        sl["c"] = to_fx(cg[:, perm]) + SNAP_FX
         sl["ustep"] = to_fx(u[:, perm])
         sl["vstep"] = to_fx(v[:, perm])
         a = poses[:, 3] if view == 0 else poses[:, 4]
@@ -253,3 +260,130 @@ def validate_items(items: np.ndarray, n_layouts: int, descs: np.ndarray, tile_ro
     if np.any(items["guard_off"].astype(np.int64) != (d["sdim"][:, 0].astype(np.int64) - 2) * d["plane_bytes"]):
         raise ValueError("item guard offset is not the layout's zero plane")
     return int(ntile.sum())
+
+
+# ---------------------------------------------------------------------------
+# Convention T (torch_engine.py:138-234): standard xyz Euler angles, a
+# half-bin grid, per-corner zero padding, optional high-pass weight
+# ---------------------------------------------------------------------------
+T_VIEW_LAYOUT = {"ap": "zxy", "lateral": "zyx"}  # lanes (slice columns) walk the fast axis
+
+
+def t_rotation_and_derivs(poses: np.ndarray):
+    """R (N,3,3) = Rz(yaw) Ry(pitch) Rx(roll) (standard xyz, torch_engine.py:138-174) and
+    dR/dtheta (N,3,3,3) per DEGREE for theta = (yaw, pitch, roll)."""
+    yz, py_, rx = (np.asarray(poses[:, k], dtype=np.float64) * DEG for k in range(3))
+    N = len(yz)
+    z0, o1 = np.zeros(N), np.ones(N)
+
+    def m(rows):
+        return np.stack([np.stack(row, axis=-1) for row in rows], axis=-2)
+
+    cz, sz, cy, sy, cx, sx = np.cos(yz), np.sin(yz), np.cos(py_), np.sin(py_), np.cos(rx), np.sin(rx)
+    Rz = m([[cz, -sz, z0], [sz, cz, z0], [z0, z0, o1]])
+    Ry = m([[cy, z0, sy], [z0, o1, z0], [-sy, z0, cy]])
+    Rx = m([[o1, z0, z0], [z0, cx, -sx], [z0, sx, cx]])
+    dRz = m([[-sz, -cz, z0], [cz, -sz, z0], [z0, z0, z0]])
+    dRy = m([[-sy, z0, cy], [z0, z0, z0], [-cy, z0, -sy]])
+    dRx = m([[z0, z0, z0], [z0, -sx, -cx], [z0, cx, -sx]])
+    R = Rz @ Ry @ Rx
+    dR = np.stack([dRz @ Ry @ Rx, Rz @ dRy @ Rx, Rz @ Ry @ dRx], axis=1) * DEG
+    return R, dR
+
+
+_RZ90 = np.array([[0.0, -1.0, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, 1.0]])
+
+
+def t_center_scale(pad_shape) -> np.ndarray:
+    """(cx, cy, cz) = ((W-1)/2, (H-1)/2, (D-1)/2) of the padded volume (torch_engine.py:200-205)."""
+    D, H, W = pad_shape
+    return np.array([0.5 * max(W - 1, 1), 0.5 * max(H - 1, 1), 0.5 * max(D - 1, 1)], dtype=np.float64)
+
+
+def t_view_axes(poses: np.ndarray, view: str, pad_shape):
+    """u_xyz, v_xyz (N,3) and their per-degree derivatives (N,3 angles,3) for a view."""
+    R, dR = t_rotation_and_derivs(np.asarray(poses, dtype=np.float64).reshape(-1, 6))
+    if view == "lateral":
+        R = R @ _RZ90
+        dR = dR @ _RZ90
+    s = t_center_scale(pad_shape)
+    return R[:, 0, :] * s, R[:, 1, :] * s, dR[:, :, 0, :] * s, dR[:, :, 1, :] * s
+
+
+def t_grid(out_res: int) -> Tuple[float, float]:
+    """First value and step of linspace(-1, 1, out_res) (torch_engine.py:115-118)."""
+    return -1.0, (2.0 / (out_res - 1) if out_res > 1 else 0.0)
+
+
+def t_phase_span(out_res: int, spacing0: float) -> float:
+    """The phase-ramp denominator out_res * spacing + 1e-6 (torch_engine.py:227-234)."""
+    return out_res * float(spacing0) + 1e-6
+
+
+def t_template(packed, view: str, out_res: int, target_ptr: int, flags: int, tile_rows: int,
+               hp_sigma: float | None = None) -> np.ndarray:
+    """Static fields of a convention-T item: layout, box, guard plane, grid sizes, high-pass weights."""
+    lay_name = T_VIEW_LAYOUT[view]
+    lay_idx = packed.index[lay_name]
+    desc = packed.desc(lay_idx)
+    it = np.zeros(1, dtype=_lib.ITEM_DTYPE)
+    n = np.array(desc["n"], dtype=np.int64)
+    # per-corner zero padding: coordinates shifted by +1 (lo_int = 0), inside when -1 < p < n
+    # (the edge is not moved with the bias: floor() must stay <= n)
+    it["box_hi"] = (((n + 1) << 32) - 1).astype(np.uint64)
+    f0, df = t_grid(out_res)
+    it["cw0"], it["cws"], it["rw0"], it["rws"] = f0, df, f0, df
+    if hp_sigma is not None:
+        flags |= _lib.ITEM_HP
+        it["hp_inv2s2"] = 1.0 / (2.0 * float(hp_sigma) ** 2)
+        step = out_res / (out_res - 1) if out_res > 1 else 0.0
+        it["hp_x0"], it["hp_dx"], it["hp_y0"], it["hp_dy"] = -out_res / 2.0, step, -out_res / 2.0, step
+    it["h"], it["w"] = out_res, out_res
+    it["layout"] = lay_idx
+    it["lo_int"] = 0
+    it["guard_off"] = (int(desc["sdim"][0]) - 2) * int(desc["plane_bytes"])
+    it["tiles_x"] = tiles_for(out_res, out_res, tile_rows)[0]
+    it["flags"] = flags
+    it["target"] = target_ptr
+    return it
+
+
+def t_items(template: np.ndarray, pad_shape, poses: np.ndarray, view: str, out_res: int, spacing0: float,
+            tile_rows: int, target_ptrs=None):
+    """Items (N,) and Jacobians (N, 2, 3, 3) of convention T for one view.
+
+    Sample (row i, col j) sits at ``center + fx_i u + fy_j v`` with
+    ``fx = fy = linspace(-1, 1, out_res)`` (rows follow u, columns follow v:
+    torch_engine.py:115-118, 196-211); phase ``exp(-2 pi i (fx_i tx + fy_j ty) / span)``.
+    ``jac[:, 0]`` = dv/dtheta (paired with the column moments), ``jac[:, 1]`` =
+    du/dtheta (row moments), layout axes x angles, per degree.
+    """
+    poses = np.asarray(poses, dtype=np.float64).reshape(-1, 6)
+    N = len(poses)
+    t = _lib.as_struct(template, _lib.ITEM_DTYPE)[0]
+    perm = [0, 2, 1] if T_VIEW_LAYOUT[view] == "zxy" else [0, 1, 2]
+    u, v, du, dv = t_view_axes(poses, view, pad_shape)
+    cen = t_center_scale(pad_shape)
+    f0, df = t_grid(out_res)
+    zyx = [2, 1, 0]  # xyz -> (z, y, x)
+    order = [zyx[a] for a in perm]  # layout axis -> xyz component
+    c0 = cen[None, :] + f0 * u + f0 * v + 1.0  # +1: per-corner coordinates (lo_int = 0)
+    items = np.empty(N, dtype=_lib.ITEM_DTYPE)
+    items[:] = t
+    items["c"] = to_fx(c0[:, order]) + SNAP_FX
+    items["ustep"] = to_fx(df * v[:, order])  # along a row: next column
+    items["vstep"] = to_fx(df * u[:, order])  # down a column: next row
+    span = t_phase_span(out_res, spacing0)
+    tx, ty = poses[:, 3], poses[:, 4]
+    items["ph_row0"] = turns_fx(f0 * tx / span)
+    items["ph_row"] = turns_fx(df * tx / span)
+    items["ph_col0"] = turns_fx(f0 * ty / span)
+    items["ph_col"] = turns_fx(df * ty / span)
+    ntile = tiles_for(out_res, out_res, tile_rows)
+    items["tile_base"] = np.arange(N) * (ntile[0] * ntile[1])
+    if target_ptrs is not None:
+        items["target"] = np.asarray(target_ptrs, dtype=np.uint64)
+    jac = np.empty((N, 2, 3, 3))
+    jac[:, 0] = np.transpose(dv[:, :, order], (0, 2, 1))
+    jac[:, 1] = np.transpose(du[:, :, order], (0, 2, 1))
+    return items, jac, N * ntile[0] * ntile[1]

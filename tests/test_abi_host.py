@@ -106,14 +106,15 @@ def test_item_builder_identity_geometry():
     it = _items.n_item(FakePacked(), "ap", 0, 0, 0, 0, 0, 0, 0, 0, 64)[0]
     assert (it["h"], it["w"]) == (16, 24)
     # c = (pz/2, py/2, px/2) + uu0*u + vv0*v with u = x, v = z, uu0 = -12, vv0 = -8
-    np.testing.assert_array_equal(it["c"], np.array([0, 10, 0]) << 32)
+    # (every coordinate carries the +SNAP_FX lattice bias, and the box edge with it)
+    np.testing.assert_array_equal(it["c"], (np.array([0, 10, 0]) << 32) + _items.SNAP_FX)
     np.testing.assert_array_equal(it["ustep"], np.array([0, 0, 1]) << 32)
     np.testing.assert_array_equal(it["vstep"], np.array([1, 0, 0]) << 32)
     lat = _items.n_item(FakePacked(), "lateral", 0, 0, 0, 0, 0, 0, 0, 0, 64)[0]
     # lateral uses the (z, x, y) layout: u = y is the fast axis
     np.testing.assert_array_equal(lat["ustep"], np.array([0, 0, 1]) << 32)
-    np.testing.assert_array_equal(lat["c"], np.array([0, 12, 0]) << 32)
-    assert lat["box_hi"][2] == (20 - 1) << 32
+    np.testing.assert_array_equal(lat["c"], (np.array([0, 12, 0]) << 32) + _items.SNAP_FX)
+    assert lat["box_hi"][2] == ((20 - 1) << 32) + _items.SNAP_FX
     # phase turns: a = 3 voxels over W = 24 -> 1/8 turn per column, uu0 * a / W = -1.5 -> 0.5 turn
     it = _items.n_item(FakePacked(), "ap", 0, 0, 0, 3.0, 0, 0, 0, 0, 64)[0]
     assert it["ph_col"] == 1 << 29 and it["ph_col0"] == 1 << 31
