@@ -1,0 +1,579 @@
+// PoU-FVR perspective path (cone_beam_fvr.py, triton_slice.py:349-808).
+//
+//   fvr_pou_slice_ifft   one workgroup per (block, pose): the P x P central slice
+//                        of the block spectrum (per-corner trilinear, planar
+//                        (B, 2, P, P, P) storage) is gathered straight into LDS,
+//                        inverse-transformed there (2D radix-2, unnormalised,
+//                        centred: fftshift(ifft2(ifftshift(.)))) and its real part
+//                        / P^2 written as the slab image (B, N, P, P) float.
+//   fvr_pou_warp_fwd     detector pixel (n, q): sum_b bilinear(img_bn, p_bn(q)) / cos,
+//                        p = (d_b r - c) / ds + P/2 (triton_slice.py:349-407).
+//   fvr_pou_warp_bwd     one workgroup per (block, pose): d/d(d_b, cu, cv) and,
+//                        optionally, d/d img (scatter into 64-bit fixed point:
+//                        integer adds commute, so the result does not depend on
+//                        the order in which waves arrive).
+//   fvr_pou_render_bwd   the fused adjoint of slice -> iFFT -> warp for one
+//                        (block, pose): warp adjoint scattered into an LDS
+//                        fixed-point image, forward 2D FFT in LDS, then every
+//                        slice sample is recomputed and contracted with its
+//                        gradient: d/d(d_b, cu, cv, u, v) per (block, pose).
+//   fvr_pou_ray_grads    d/d(rx, ry, inv_cos) per (pose, pixel) for per-pose rays.
+//   fvr_pou_gscale       per-pose power-of-two fixed-point scale of |grad * inv_cos|.
+//   fvr_pou_sum_blocks   fixed-order sum over blocks of per-(block, pose) partials.
+#include <cmath>
+
+#include "fvr_common.h"
+
+namespace fvr {
+
+constexpr int kPouThreads = 512;
+constexpr int kPouMaxLogP = 7;  // LDS-resident transforms up to 128 x 128
+
+template <int VT>
+__device__ __forceinline__ float pld(const void* p, size_t i) {
+  if (VT == 0) return reinterpret_cast<const float*>(p)[i];
+  const uint16_t h = reinterpret_cast<const uint16_t*>(p)[i];
+  return VT == 1 ? h2f(h) : b2f(h);
+}
+
+__device__ __forceinline__ int brev(int x, int logp) {
+  return int(__builtin_bitreverse32(uint32_t(x)) >> (32 - logp));
+}
+
+// 2D radix-2 decimation-in-frequency FFT of a P x P complex array in LDS (row
+// pitch P + 1): natural-order input, bit-reversed output in both dimensions,
+// unnormalised; SIGN = +1 inverse, -1 forward.  tw[k] = exp(-2 pi i k / P).
+template <int SIGN>
+__device__ void fft2d_dif(float2* a, int P, int logP, int pitch, const float2* tw) {
+  const int nb = P * P / 2;
+  const int hp = P / 2;
+#pragma unroll 1
+  for (int dim = 0; dim < 2; ++dim) {
+#pragma unroll 1
+    for (int s = logP; s >= 1; --s) {
+      const int half = 1 << (s - 1);
+      const int tstep = P >> s;
+      for (int idx = threadIdx.x; idx < nb; idx += blockDim.x) {
+        int line, k;
+        if (dim == 0) {  // along rows: consecutive threads walk one row
+          line = idx >> (logP - 1);
+          k = idx & (hp - 1);
+        } else {  // along columns: consecutive threads take consecutive columns
+          line = idx & (P - 1);
+          k = idx >> logP;
+        }
+        const int pos = k & (half - 1);
+        const int i0 = ((k >> (s - 1)) << s) + pos;
+        const int i1 = i0 + half;
+        const int a0 = dim == 0 ? line * pitch + i0 : i0 * pitch + line;
+        const int a1 = dim == 0 ? line * pitch + i1 : i1 * pitch + line;
+        float2 w = tw[pos * tstep];
+        if (SIGN > 0) w.y = -w.y;
+        const float2 x0 = a[a0], x1 = a[a1];
+        const float dr = x0.x - x1.x, di = x0.y - x1.y;
+        a[a0] = make_float2(x0.x + x1.x, x0.y + x1.y);
+        a[a1] = make_float2(dr * w.x - di * w.y, dr * w.y + di * w.x);
+      }
+      __syncthreads();
+    }
+  }
+}
+
+__device__ __forceinline__ void make_twiddles(float2* tw, int P) {
+  for (int k = threadIdx.x; k < P / 2; k += blockDim.x) {
+    float s, c;
+    sincospif(-2.f * float(k) / float(P), &s, &c);
+    tw[k] = make_float2(c, s);
+  }
+}
+
+// per-corner trilinear value (and gradient) of one block's two planes at p (x, y, z)
+template <int VT, bool GRAD>
+__device__ __forceinline__ void sample_block(const void* F, size_t base_re, size_t plane, int P, float px, float py,
+                                             float pz, float& re, float& im, float* gre, float* gim) {
+  const float x0f = floorf(px), y0f = floorf(py), z0f = floorf(pz);
+  const float tx = px - x0f, ty = py - y0f, tz = pz - z0f;
+  const int x0 = int(x0f), y0 = int(y0f), z0 = int(z0f);
+  float cr[8], ci[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int x = x0 + (c & 1), y = y0 + ((c >> 1) & 1), z = z0 + (c >> 2);
+    const bool in = (unsigned)x < (unsigned)P && (unsigned)y < (unsigned)P && (unsigned)z < (unsigned)P;
+    const size_t o = (size_t(z) * P + y) * P + x;
+    cr[c] = in ? pld<VT>(F, base_re + o) : 0.f;
+    ci[c] = in ? pld<VT>(F, base_re + plane + o) : 0.f;
+  }
+  const float ox = 1.f - tx, oy = 1.f - ty, oz = 1.f - tz;
+  // corner c = x + 2 y + 4 z
+  re = ((cr[0] * ox + cr[1] * tx) * oy + (cr[2] * ox + cr[3] * tx) * ty) * oz +
+       ((cr[4] * ox + cr[5] * tx) * oy + (cr[6] * ox + cr[7] * tx) * ty) * tz;
+  im = ((ci[0] * ox + ci[1] * tx) * oy + (ci[2] * ox + ci[3] * tx) * ty) * oz +
+       ((ci[4] * ox + ci[5] * tx) * oy + (ci[6] * ox + ci[7] * tx) * ty) * tz;
+  if (GRAD) {
+    gre[0] = ((cr[1] - cr[0]) * oy + (cr[3] - cr[2]) * ty) * oz + ((cr[5] - cr[4]) * oy + (cr[7] - cr[6]) * ty) * tz;
+    gre[1] = ((cr[2] - cr[0]) * ox + (cr[3] - cr[1]) * tx) * oz + ((cr[6] - cr[4]) * ox + (cr[7] - cr[5]) * tx) * tz;
+    gre[2] = ((cr[4] - cr[0]) * ox + (cr[5] - cr[1]) * tx) * oy + ((cr[6] - cr[2]) * ox + (cr[7] - cr[3]) * tx) * ty;
+    gim[0] = ((ci[1] - ci[0]) * oy + (ci[3] - ci[2]) * ty) * oz + ((ci[5] - ci[4]) * oy + (ci[7] - ci[6]) * ty) * tz;
+    gim[1] = ((ci[2] - ci[0]) * ox + (ci[3] - ci[1]) * tx) * oz + ((ci[6] - ci[4]) * ox + (ci[7] - ci[5]) * tx) * tz;
+    gim[2] = ((ci[4] - ci[0]) * ox + (ci[5] - ci[1]) * tx) * oy + ((ci[6] - ci[2]) * ox + (ci[7] - ci[3]) * tx) * ty;
+  }
+}
+
+struct PouSlice {
+  int B, N, P, logP;
+  float c;  // P // 2, the centre voxel of every axis
+};
+
+template <int VT>
+__global__ __launch_bounds__(kPouThreads) void pou_slice_ifft_kernel(const void* __restrict__ F, PouSlice g,
+                                                                     const float* __restrict__ u,
+                                                                     const float* __restrict__ v,
+                                                                     float* __restrict__ imgs) {
+  extern __shared__ float2 smem[];
+  const int P = g.P, logP = g.logP, pitch = P + 1, hp = P / 2;
+  float2* a = smem;
+  float2* tw = smem + P * pitch;
+  const int bn = blockIdx.x;
+  const int b = bn / g.N, n = bn - b * g.N;
+  make_twiddles(tw, P);
+  const float ux = u[3 * n], uy = u[3 * n + 1], uz = u[3 * n + 2];
+  const float vx = v[3 * n], vy = v[3 * n + 1], vz = v[3 * n + 2];
+  const size_t plane = size_t(P) * P * P;
+  const size_t base = size_t(b) * 2 * plane;
+  for (int q = threadIdx.x; q < P * P; q += blockDim.x) {
+    const int i = q >> logP, j = q & (P - 1);
+    const float kx = float(j - hp), ky = float(i - hp);  // columns follow u, rows follow v
+    float re, im;
+    sample_block<VT, false>(F, base, plane, P, fmaf(ky, vx, fmaf(kx, ux, g.c)), fmaf(ky, vy, fmaf(kx, uy, g.c)),
+                            fmaf(ky, vz, fmaf(kx, uz, g.c)), re, im, nullptr, nullptr);
+    // ifftshift: frequency row i goes to natural index (i + P/2) mod P
+    a[((i + hp) & (P - 1)) * pitch + ((j + hp) & (P - 1))] = make_float2(re, im);
+  }
+  __syncthreads();
+  fft2d_dif<+1>(a, P, logP, pitch, tw);
+  const float inv = 1.f / float(P * P);
+  float* out = imgs + size_t(bn) * P * P;
+  for (int q = threadIdx.x; q < P * P; q += blockDim.x) {
+    const int i = q >> logP, j = q & (P - 1);
+    // fftshift of the natural-order result, read at its bit-reversed position
+    out[q] = a[brev((i + hp) & (P - 1), logP) * pitch + brev((j + hp) & (P - 1), logP)].x * inv;
+  }
+}
+
+struct PouWarp {
+  int B, N, P, K;
+  long long ray_stride;  // 0: rays shared by all poses, K: per pose
+  float inv_ds, c_idx;
+};
+
+__device__ __forceinline__ float img_at(const float* img, int P, int x, int y) {
+  return ((unsigned)x < (unsigned)P && (unsigned)y < (unsigned)P) ? img[y * P + x] : 0.f;
+}
+
+// XCD-aware flat grid: each XCD takes a contiguous range of (pose, pixel-tile)s
+__global__ __launch_bounds__(256) void pou_warp_fwd_kernel(const float* __restrict__ imgs,
+                                                           const float* __restrict__ d_b,
+                                                           const float* __restrict__ cu,
+                                                           const float* __restrict__ cv,
+                                                           const float* __restrict__ rx,
+                                                           const float* __restrict__ ry,
+                                                           const float* __restrict__ ic, PouWarp g, int qtiles,
+                                                           int per_xcd, float* __restrict__ out) {
+  const int t = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+  if (t >= g.N * qtiles) return;
+  const int n = t / qtiles, q = (t - n * qtiles) * 256 + threadIdx.x;
+  if (q >= g.K) return;
+  const size_t ro = size_t(n) * g.ray_stride + q;
+  const float r_x = rx[ro], r_y = ry[ro];
+  const size_t pp = size_t(g.P) * g.P;
+  float acc = 0.f;
+  for (int b = 0; b < g.B; ++b) {
+    const int bn = b * g.N + n;
+    const float px = (d_b[bn] * r_x - cu[bn]) * g.inv_ds + g.c_idx;
+    const float py = (d_b[bn] * r_y - cv[bn]) * g.inv_ds + g.c_idx;
+    const float x0f = floorf(px), y0f = floorf(py);
+    const float tx = px - x0f, ty = py - y0f;
+    const int x0 = int(x0f), y0 = int(y0f);
+    const float* img = imgs + size_t(bn) * pp;
+    const float v00 = img_at(img, g.P, x0, y0), v01 = img_at(img, g.P, x0 + 1, y0);
+    const float v10 = img_at(img, g.P, x0, y0 + 1), v11 = img_at(img, g.P, x0 + 1, y0 + 1);
+    acc += (v00 * (1.f - tx) + v01 * tx) * (1.f - ty) + (v10 * (1.f - tx) + v11 * tx) * ty;
+  }
+  out[size_t(n) * g.K + q] = acc * ic[ro];
+}
+
+// fixed-order workgroup sum of NV per-thread values (kPouThreads threads)
+template <int NV>
+__device__ __forceinline__ void block_sum(float (&v)[NV], float* red /* NV * kPouThreads */) {
+  for (int k = 0; k < NV; ++k) red[k * kPouThreads + threadIdx.x] = v[k];
+  __syncthreads();
+  for (int st = kPouThreads / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st)
+      for (int k = 0; k < NV; ++k) red[k * kPouThreads + threadIdx.x] += red[k * kPouThreads + threadIdx.x + st];
+    __syncthreads();
+  }
+  for (int k = 0; k < NV; ++k) v[k] = red[k * kPouThreads];
+  __syncthreads();
+}
+
+// warp adjoint of one (block, pose): geometry sums and (optionally) the image
+// gradient as 64-bit fixed point, at index map(y, x) of `acc` (LDS or global)
+template <bool IMG>
+__device__ __forceinline__ void warp_adjoint_scan(const float* img, const float* go, const float* rx,
+                                                  const float* ry, const float* ic, const PouWarp& g, int n, float db,
+                                                  float c_u, float c_v, float scale, unsigned long long* acc,
+                                                  int acc_pitch, bool roll, float (&s)[3]) {
+  const int P = g.P, hp = P / 2;
+  for (int q = threadIdx.x; q < g.K; q += blockDim.x) {
+    const size_t ro = size_t(n) * g.ray_stride + q;
+    const float r_x = rx[ro], r_y = ry[ro];
+    const float gg = go[size_t(n) * g.K + q] * ic[ro];
+    const float px = (db * r_x - c_u) * g.inv_ds + g.c_idx;
+    const float py = (db * r_y - c_v) * g.inv_ds + g.c_idx;
+    const float x0f = floorf(px), y0f = floorf(py);
+    const float tx = px - x0f, ty = py - y0f, ox = 1.f - tx, oy = 1.f - ty;
+    const int x0 = int(x0f), y0 = int(y0f);
+    const float v00 = img_at(img, P, x0, y0), v01 = img_at(img, P, x0 + 1, y0);
+    const float v10 = img_at(img, P, x0, y0 + 1), v11 = img_at(img, P, x0 + 1, y0 + 1);
+    const float dsx = ((v01 - v00) * oy + (v11 - v10) * ty) * g.inv_ds;
+    const float dsy = ((v10 - v00) * ox + (v11 - v01) * tx) * g.inv_ds;
+    s[0] = fmaf(gg, dsx * r_x + dsy * r_y, s[0]);
+    s[1] = fmaf(-gg, dsx, s[1]);
+    s[2] = fmaf(-gg, dsy, s[2]);
+    if (IMG && gg != 0.f) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int x = x0 + (c & 1), y = y0 + (c >> 1);
+        if ((unsigned)x < (unsigned)P && (unsigned)y < (unsigned)P) {
+          const float w = ((c & 1) ? tx : ox) * ((c >> 1) ? ty : oy);
+          const long long fx = llrintf(gg * w * scale);
+          const int yy = roll ? ((y + hp) & (P - 1)) : y, xx = roll ? ((x + hp) & (P - 1)) : x;
+          atomicAdd(acc + size_t(yy) * acc_pitch + xx, (unsigned long long)fx);
+        }
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(kPouThreads) void pou_warp_bwd_kernel(
+    const float* __restrict__ imgs, const float* __restrict__ go, const float* __restrict__ d_b,
+    const float* __restrict__ cu, const float* __restrict__ cv, const float* __restrict__ rx,
+    const float* __restrict__ ry, const float* __restrict__ ic, PouWarp g, const float* __restrict__ gscale,
+    unsigned long long* __restrict__ gacc, float* __restrict__ part) {
+  __shared__ float red[3 * kPouThreads];
+  const int bn = blockIdx.x;
+  const int n = bn % g.N;
+  const size_t pp = size_t(g.P) * g.P;
+  float s[3] = {0.f, 0.f, 0.f};
+  if (gacc)
+    warp_adjoint_scan<true>(imgs + bn * pp, go, rx, ry, ic, g, n, d_b[bn], cu[bn], cv[bn], gscale[n],
+                            gacc + bn * pp, g.P, false, s);
+  else
+    warp_adjoint_scan<false>(imgs + bn * pp, go, rx, ry, ic, g, n, d_b[bn], cu[bn], cv[bn], 1.f, nullptr, g.P,
+                             false, s);
+  block_sum<3>(s, red);
+  if (threadIdx.x < 3) part[size_t(bn) * 3 + threadIdx.x] = s[threadIdx.x];
+}
+
+// int64 fixed point -> float, and reset of the accumulator to zero for the next call
+__global__ void pou_fix_to_float_kernel(unsigned long long* __restrict__ acc, const float* __restrict__ gscale,
+                                        int N, long long pp, long long total, float* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int n = int((i / pp) % N);
+  out[i] = float(double((long long)acc[i]) / double(gscale[n]));
+}
+
+template <int VT>
+__global__ __launch_bounds__(kPouThreads) void pou_render_bwd_kernel(
+    const void* __restrict__ F, PouSlice sg, const float* __restrict__ u, const float* __restrict__ v,
+    const float* __restrict__ imgs, const float* __restrict__ go, const float* __restrict__ d_b,
+    const float* __restrict__ cu, const float* __restrict__ cv, const float* __restrict__ rx,
+    const float* __restrict__ ry, const float* __restrict__ ic, PouWarp g, const float* __restrict__ gscale,
+    float* __restrict__ part) {
+  extern __shared__ float2 smem[];
+  const int P = sg.P, logP = sg.logP, pitch = P + 1, hp = P / 2;
+  float2* a = smem;  // P x pitch complex, first used as 64-bit fixed-point accumulators
+  float2* tw = smem + P * pitch;
+  float* red = reinterpret_cast<float*>(tw + P / 2);  // 9 * kPouThreads floats
+  unsigned long long* acc = reinterpret_cast<unsigned long long*>(a);
+  const int bn = blockIdx.x;
+  const int b = bn / sg.N, n = bn - b * sg.N;
+  for (int i = threadIdx.x; i < P * pitch; i += blockDim.x) acc[i] = 0ull;
+  make_twiddles(tw, P);
+  __syncthreads();
+  // 1) warp adjoint: geometry sums + image gradient (ifftshifted) in LDS fixed point
+  float s3[3] = {0.f, 0.f, 0.f};
+  const float scale = gscale[n];
+  warp_adjoint_scan<true>(imgs + size_t(bn) * P * P, go, rx, ry, ic, g, n, d_b[bn], cu[bn], cv[bn], scale, acc,
+                          pitch, true, s3);
+  __syncthreads();
+  // 2) fixed point -> complex (real part), in place
+  const float inv_scale = 1.f / scale;
+  for (int i = threadIdx.x; i < P * pitch; i += blockDim.x) {
+    const long long x = (long long)acc[i];
+    a[i] = make_float2(float(double(x)) * inv_scale, 0.f);
+  }
+  __syncthreads();
+  // 3) adjoint of fftshift(ifft2(ifftshift(.))).real: fftshift(fft2(ifftshift(g))) / P^2
+  fft2d_dif<-1>(a, P, logP, pitch, tw);
+  // 4) recompute every slice sample and contract its spatial gradient with the slice gradient
+  const float ux = u[3 * n], uy = u[3 * n + 1], uz = u[3 * n + 2];
+  const float vx = v[3 * n], vy = v[3 * n + 1], vz = v[3 * n + 2];
+  const size_t plane = size_t(P) * P * P;
+  const size_t base = size_t(b) * 2 * plane;
+  const float inv = 1.f / float(P * P);
+  float gu[3] = {0.f, 0.f, 0.f}, gv[3] = {0.f, 0.f, 0.f};
+  for (int q = threadIdx.x; q < P * P; q += blockDim.x) {
+    const int i = q >> logP, j = q & (P - 1);
+    const float kx = float(j - hp), ky = float(i - hp);
+    const float2 gs = a[brev((i + hp) & (P - 1), logP) * pitch + brev((j + hp) & (P - 1), logP)];
+    const float gr = gs.x * inv, gi = gs.y * inv;
+    float re, im, dre[3], dim[3];
+    sample_block<VT, true>(F, base, plane, P, fmaf(ky, vx, fmaf(kx, ux, sg.c)), fmaf(ky, vy, fmaf(kx, uy, sg.c)),
+                           fmaf(ky, vz, fmaf(kx, uz, sg.c)), re, im, dre, dim);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float d = gr * dre[c] + gi * dim[c];
+      gu[c] = fmaf(kx, d, gu[c]);
+      gv[c] = fmaf(ky, d, gv[c]);
+    }
+  }
+  float all[9] = {s3[0], s3[1], s3[2], gu[0], gu[1], gu[2], gv[0], gv[1], gv[2]};
+  __syncthreads();
+  block_sum<9>(all, red);
+  if (threadIdx.x < 9) part[size_t(bn) * 9 + threadIdx.x] = all[threadIdx.x];
+}
+
+// d out / d(rx, ry, inv_cos) for per-pose ray arrays (explicit-ray projections)
+__global__ __launch_bounds__(256) void pou_ray_grads_kernel(const float* __restrict__ imgs,
+                                                            const float* __restrict__ go,
+                                                            const float* __restrict__ d_b,
+                                                            const float* __restrict__ cu,
+                                                            const float* __restrict__ cv,
+                                                            const float* __restrict__ rx,
+                                                            const float* __restrict__ ry,
+                                                            const float* __restrict__ ic, PouWarp g,
+                                                            float* __restrict__ grx, float* __restrict__ gry,
+                                                            float* __restrict__ gic) {
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  const int n = blockIdx.y;
+  if (q >= g.K) return;
+  const size_t ro = size_t(n) * g.ray_stride + q;
+  const float r_x = rx[ro], r_y = ry[ro], o = go[size_t(n) * g.K + q], icq = ic[ro];
+  const size_t pp = size_t(g.P) * g.P;
+  float acc = 0.f, sx = 0.f, sy = 0.f;
+  for (int b = 0; b < g.B; ++b) {
+    const int bn = b * g.N + n;
+    const float db = d_b[bn];
+    const float px = (db * r_x - cu[bn]) * g.inv_ds + g.c_idx;
+    const float py = (db * r_y - cv[bn]) * g.inv_ds + g.c_idx;
+    const float x0f = floorf(px), y0f = floorf(py);
+    const float tx = px - x0f, ty = py - y0f, ox = 1.f - tx, oy = 1.f - ty;
+    const int x0 = int(x0f), y0 = int(y0f);
+    const float* img = imgs + size_t(bn) * pp;
+    const float v00 = img_at(img, g.P, x0, y0), v01 = img_at(img, g.P, x0 + 1, y0);
+    const float v10 = img_at(img, g.P, x0, y0 + 1), v11 = img_at(img, g.P, x0 + 1, y0 + 1);
+    acc += (v00 * ox + v01 * tx) * oy + (v10 * ox + v11 * tx) * ty;
+    sx = fmaf(((v01 - v00) * oy + (v11 - v10) * ty) * g.inv_ds, db, sx);
+    sy = fmaf(((v10 - v00) * ox + (v11 - v01) * tx) * g.inv_ds, db, sy);
+  }
+  const size_t oq = size_t(n) * g.K + q;
+  grx[oq] = o * icq * sx;
+  gry[oq] = o * icq * sy;
+  gic[oq] = o * acc;
+}
+
+// per pose: scale = 2^(40 - e) with max |go * ic| = m * 2^e (m in [0.5, 1)), 1 when all zero
+__global__ __launch_bounds__(256) void pou_gscale_kernel(const float* __restrict__ go, const float* __restrict__ ic,
+                                                         long long ray_stride, int K, float* __restrict__ scale) {
+  __shared__ float red[256];
+  const int n = blockIdx.x;
+  float m = 0.f;
+  for (int q = threadIdx.x; q < K; q += 256) m = fmaxf(m, fabsf(go[size_t(n) * K + q] * ic[size_t(n) * ray_stride + q]));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + st]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const float mx = red[0];
+    int e = 0;
+    if (mx > 0.f && isfinite(mx)) frexpf(mx, &e);
+    scale[n] = mx > 0.f && isfinite(mx) ? ldexpf(1.f, 40 - e) : 1.f;
+  }
+}
+
+__global__ void pou_sum_blocks_kernel(const float* __restrict__ part, int B, int N, int C, int c0, int nc,
+                                      float* __restrict__ out) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= N * nc) return;
+  const int n = idx / nc, c = idx - n * nc;
+  double s = 0.0;
+  for (int b = 0; b < B; ++b) s += double(part[(size_t(b) * N + n) * C + c0 + c]);
+  out[idx] = float(s);
+}
+
+static size_t slice_lds_bytes(int P) { return (size_t(P) * (P + 1) + P / 2) * sizeof(float2); }
+
+__global__ void pou_zero_u64_kernel(unsigned long long* __restrict__ p, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+// dynamic LDS above the default limit has to be allowed per kernel
+template <typename K>
+static bool allow_lds(K kernel, size_t bytes) {
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                             int(bytes)) == hipSuccess;
+}
+
+}  // namespace fvr
+
+using namespace fvr;
+
+extern "C" {
+
+int fvr_pou_lds_ok(int32_t P) {
+  if (P < 4 || P > (1 << kPouMaxLogP) || (P & (P - 1))) return 0;
+  return slice_lds_bytes(P) + 9 * kPouThreads * sizeof(float) <= 160 * 1024 ? 1 : 0;
+}
+
+int fvr_pou_slice_ifft(uint64_t F, int32_t dtype, int32_t B, int32_t P, const float* u_xyz, const float* v_xyz,
+                       int32_t N, float* imgs, void* stream) {
+  FVR_CHECK_ARG(F && u_xyz && v_xyz && imgs && B > 0 && N > 0, "fvr_pou_slice_ifft: bad arguments");
+  FVR_CHECK_ARG(fvr_pou_lds_ok(P), "fvr_pou_slice_ifft: P = %d must be a power of two in [4, 128]", P);
+  FVR_CHECK_ARG(int64_t(B) * N < (int64_t(1) << 31), "fvr_pou_slice_ifft: too many (block, pose) pairs");
+  PouSlice g{B, N, P, 31 - __builtin_clz(uint32_t(P)), float(P / 2)};
+  const size_t lds = slice_lds_bytes(P);
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const void* fp = reinterpret_cast<const void*>(F);
+#define FVR_SI(VT)                                                                                            \
+  FVR_CHECK_ARG(allow_lds(pou_slice_ifft_kernel<VT>, lds), "fvr_pou_slice_ifft: %zu B of LDS refused", lds); \
+  hipLaunchKernelGGL(pou_slice_ifft_kernel<VT>, dim3(B * N), dim3(kPouThreads), lds, st, fp, g, u_xyz, v_xyz, imgs)
+  switch (dtype) {
+    case 0: FVR_SI(0); break;
+    case 1: FVR_SI(1); break;
+    case 2: FVR_SI(2); break;
+    default: set_error("fvr_pou_slice_ifft: bad dtype %d", dtype); return FVR_ERR_INVALID;
+  }
+#undef FVR_SI
+  FVR_CHECK_LAUNCH("pou_slice_ifft_kernel");
+  return FVR_OK;
+}
+
+static int warp_geom(PouWarp& g, int32_t B, int32_t N, int32_t P, int32_t K, int32_t per_pose_rays, float inv_ds,
+                     float c_idx) {
+  FVR_CHECK_ARG(B > 0 && N > 0 && P > 1 && K > 0, "pou warp: empty shape");
+  FVR_CHECK_ARG(int64_t(B) * N * P * P < (int64_t(1) << 40), "pou warp: image stack too large");
+  g = PouWarp{B, N, P, K, per_pose_rays ? (long long)K : 0ll, inv_ds, c_idx};
+  return FVR_OK;
+}
+
+int fvr_pou_warp_fwd(const float* imgs, int32_t B, int32_t N, int32_t P, const float* d_b, const float* cu,
+                     const float* cv, const float* rx, const float* ry, const float* ic, int32_t per_pose_rays,
+                     int32_t K, float inv_ds, float c_idx, float* out, void* stream) {
+  FVR_CHECK_ARG(imgs && d_b && cu && cv && rx && ry && ic && out, "fvr_pou_warp_fwd: null pointer");
+  PouWarp g;
+  if (warp_geom(g, B, N, P, K, per_pose_rays, inv_ds, c_idx) != FVR_OK) return FVR_ERR_INVALID;
+  const int qtiles = (K + 255) / 256;
+  const int64_t tiles = int64_t(N) * qtiles;
+  FVR_CHECK_ARG(tiles < (int64_t(1) << 30), "fvr_pou_warp_fwd: too many pixel tiles");
+  const int per_xcd = int((tiles + 7) / 8);
+  hipLaunchKernelGGL(pou_warp_fwd_kernel, dim3(per_xcd * 8), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
+                     imgs, d_b, cu, cv, rx, ry, ic, g, qtiles, per_xcd, out);
+  FVR_CHECK_LAUNCH("pou_warp_fwd_kernel");
+  return FVR_OK;
+}
+
+int fvr_pou_gscale(const float* go, const float* ic, int32_t N, int32_t K, int32_t per_pose_rays, float* scale,
+                   void* stream) {
+  FVR_CHECK_ARG(go && ic && scale && N > 0 && K > 0, "fvr_pou_gscale: bad arguments");
+  hipLaunchKernelGGL(pou_gscale_kernel, dim3(N), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), go, ic,
+                     per_pose_rays ? (long long)K : 0ll, K, scale);
+  FVR_CHECK_LAUNCH("pou_gscale_kernel");
+  return FVR_OK;
+}
+
+int64_t fvr_pou_warp_bwd_workspace(int32_t B, int32_t N, int32_t P) { return int64_t(B) * N * P * P * 8; }
+
+int fvr_pou_warp_bwd(const float* imgs, int32_t B, int32_t N, int32_t P, const float* go, const float* d_b,
+                     const float* cu, const float* cv, const float* rx, const float* ry, const float* ic,
+                     int32_t per_pose_rays, int32_t K, float inv_ds, float c_idx, const float* gscale,
+                     void* workspace, float* grad_imgs, float* part, void* stream) {
+  FVR_CHECK_ARG(imgs && go && d_b && cu && cv && rx && ry && ic && part, "fvr_pou_warp_bwd: null pointer");
+  FVR_CHECK_ARG(!grad_imgs || (workspace && gscale), "fvr_pou_warp_bwd: image gradient needs workspace + scale");
+  PouWarp g;
+  if (warp_geom(g, B, N, P, K, per_pose_rays, inv_ds, c_idx) != FVR_OK) return FVR_ERR_INVALID;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  unsigned long long* acc = grad_imgs ? reinterpret_cast<unsigned long long*>(workspace) : nullptr;
+  const int64_t total = int64_t(B) * N * P * P;
+  if (acc) {  // a fill kernel, not a memset node, so that the call can be graph-captured
+    hipLaunchKernelGGL(pou_zero_u64_kernel, dim3(unsigned((total + 255) / 256)), dim3(256), 0, st, acc,
+                       (long long)total);
+    FVR_CHECK_LAUNCH("pou_zero_u64_kernel");
+  }
+  hipLaunchKernelGGL(pou_warp_bwd_kernel, dim3(B * N), dim3(kPouThreads), 0, st, imgs, go, d_b, cu, cv, rx, ry, ic,
+                     g, gscale, acc, part);
+  FVR_CHECK_LAUNCH("pou_warp_bwd_kernel");
+  if (acc) {
+    hipLaunchKernelGGL(pou_fix_to_float_kernel, dim3(unsigned((total + 255) / 256)), dim3(256), 0, st, acc, gscale,
+                       N, (long long)P * P, (long long)total, grad_imgs);
+    FVR_CHECK_LAUNCH("pou_fix_to_float_kernel");
+  }
+  return FVR_OK;
+}
+
+int fvr_pou_render_bwd(uint64_t F, int32_t dtype, int32_t B, int32_t P, const float* u_xyz, const float* v_xyz,
+                       int32_t N, const float* imgs, const float* go, const float* d_b, const float* cu,
+                       const float* cv, const float* rx, const float* ry, const float* ic, int32_t per_pose_rays,
+                       int32_t K, float inv_ds, float c_idx, const float* gscale, float* part, void* stream) {
+  FVR_CHECK_ARG(F && u_xyz && v_xyz && imgs && go && d_b && cu && cv && rx && ry && ic && gscale && part,
+                "fvr_pou_render_bwd: null pointer");
+  FVR_CHECK_ARG(fvr_pou_lds_ok(P), "fvr_pou_render_bwd: P = %d must be a power of two in [4, 128]", P);
+  PouWarp g;
+  if (warp_geom(g, B, N, P, K, per_pose_rays, inv_ds, c_idx) != FVR_OK) return FVR_ERR_INVALID;
+  PouSlice sg{B, N, P, 31 - __builtin_clz(uint32_t(P)), float(P / 2)};
+  const size_t lds = slice_lds_bytes(P) + 9 * kPouThreads * sizeof(float);
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const void* fp = reinterpret_cast<const void*>(F);
+#define FVR_RB(VT)                                                                                            \
+  FVR_CHECK_ARG(allow_lds(pou_render_bwd_kernel<VT>, lds), "fvr_pou_render_bwd: %zu B of LDS refused", lds);  \
+  hipLaunchKernelGGL(pou_render_bwd_kernel<VT>, dim3(B * N), dim3(kPouThreads), lds, st, fp, sg, u_xyz, v_xyz, \
+                     imgs, go, d_b, cu, cv, rx, ry, ic, g, gscale, part)
+  switch (dtype) {
+    case 0: FVR_RB(0); break;
+    case 1: FVR_RB(1); break;
+    case 2: FVR_RB(2); break;
+    default: set_error("fvr_pou_render_bwd: bad dtype %d", dtype); return FVR_ERR_INVALID;
+  }
+#undef FVR_RB
+  FVR_CHECK_LAUNCH("pou_render_bwd_kernel");
+  return FVR_OK;
+}
+
+int fvr_pou_ray_grads(const float* imgs, int32_t B, int32_t N, int32_t P, const float* go, const float* d_b,
+                      const float* cu, const float* cv, const float* rx, const float* ry, const float* ic,
+                      int32_t per_pose_rays, int32_t K, float inv_ds, float c_idx, float* grx, float* gry,
+                      float* gic, void* stream) {
+  FVR_CHECK_ARG(imgs && go && d_b && cu && cv && rx && ry && ic && grx && gry && gic, "fvr_pou_ray_grads: null");
+  FVR_CHECK_ARG(N <= 65535, "fvr_pou_ray_grads: %d poses exceed the grid's y limit", N);
+  PouWarp g;
+  if (warp_geom(g, B, N, P, K, per_pose_rays, inv_ds, c_idx) != FVR_OK) return FVR_ERR_INVALID;
+  hipLaunchKernelGGL(pou_ray_grads_kernel, dim3((K + 255) / 256, N), dim3(256), 0,
+                     reinterpret_cast<hipStream_t>(stream), imgs, go, d_b, cu, cv, rx, ry, ic, g, grx, gry, gic);
+  FVR_CHECK_LAUNCH("pou_ray_grads_kernel");
+  return FVR_OK;
+}
+
+int fvr_pou_sum_blocks(const float* part, int32_t B, int32_t N, int32_t C, int32_t c0, int32_t nc, float* out,
+                       void* stream) {
+  FVR_CHECK_ARG(part && out && B > 0 && N > 0 && nc > 0 && c0 >= 0 && c0 + nc <= C, "fvr_pou_sum_blocks: bad args");
+  hipLaunchKernelGGL(pou_sum_blocks_kernel, dim3((N * nc + 255) / 256), dim3(256), 0,
+                     reinterpret_cast<hipStream_t>(stream), part, B, N, C, c0, nc, out);
+  FVR_CHECK_LAUNCH("pou_sum_blocks_kernel");
+  return FVR_OK;
+}
+
+}  // extern "C"

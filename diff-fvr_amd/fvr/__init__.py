@@ -16,6 +16,8 @@ from .registration_3d2d import compute_mtre_6dof
 from .phantom import ellipsoid_phantom
 from .slice_ops import fourier_slice_3d
 from .torch_engine import TorchDiffFVREngine
+from .cone_beam_fvr import (PerspectiveSlabFVR, SLAB_PRESETS, edge_dominant_ncc, pou_pinhole_warp_sum,
+                            sample_isocenter_parity_offsets)
 
 __version__ = "0.1.0"
 
@@ -29,4 +31,9 @@ __all__ = [
     "ellipsoid_phantom",
     "fourier_slice_3d",
     "TorchDiffFVREngine",
+    "PerspectiveSlabFVR",
+    "SLAB_PRESETS",
+    "sample_isocenter_parity_offsets",
+    "edge_dominant_ncc",
+    "pou_pinhole_warp_sum",
 ]

@@ -72,6 +72,16 @@ _SIGNATURES = {
     "fvr_slice_fwd": ([_U64, _I, _I, _I, _I, _I, _P, _P, _I, _P, _P, _I, _F, _F, _F, _U64, _P], _I),
     "fvr_slice_bwd_workspace": ([_I, _I], ctypes.c_int64),
     "fvr_slice_bwd": ([_U64, _I, _I, _I, _I, _I, _U64, _P, _P, _I, _P, _P, _I, _F, _F, _F, _P, _P, _P, _P], _I),
+    "fvr_pou_lds_ok": ([_I], _I),
+    "fvr_pou_slice_ifft": ([_U64, _I, _I, _I, _P, _P, _I, _P, _P], _I),
+    "fvr_pou_warp_fwd": ([_P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _I, _I, _F, _F, _P, _P], _I),
+    "fvr_pou_gscale": ([_P, _P, _I, _I, _I, _P, _P], _I),
+    "fvr_pou_warp_bwd_workspace": ([_I, _I, _I], ctypes.c_int64),
+    "fvr_pou_warp_bwd": ([_P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _I, _I, _F, _F, _P, _P, _P, _P, _P], _I),
+    "fvr_pou_render_bwd": ([_U64, _I, _I, _I, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _F, _F, _P, _P,
+                            _P], _I),
+    "fvr_pou_ray_grads": ([_P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _I, _I, _F, _F, _P, _P, _P, _P], _I),
+    "fvr_pou_sum_blocks": ([_P, _I, _I, _I, _I, _I, _P, _P], _I),
 }
 
 

@@ -21,10 +21,16 @@
  *                       layout, optional FP16/BF16 with per-brick scale).
  *   fvr_slice_fwd/bwd   triton_slice.py:648-661 fourier_slice_3d (forward and
  *                       the u/v gradient of triton_slice.py:306-339).
- *   fvr_warp_fwd/bwd    triton_slice.py:787-808 pou_pinhole_warp_sum (forward
- *                       and the imgs/d_b/cu/cv gradients of :548-591).
- *   fvr_pou_slice_ifft  cone_beam_fvr.py:480-505 _slice_and_ifft.
- *   fvr_pou_slice_ifft_adj  its adjoint (d/du_det, d/dv_det).
+ *   fvr_pou_warp_fwd/bwd  triton_slice.py:787-808 pou_pinhole_warp_sum (forward
+ *                       K3 :349-407 and the imgs/d_b/cu/cv gradients of K4
+ *                       :410-496, :548-591).
+ *   fvr_pou_slice_ifft  cone_beam_fvr.py:480-505 _slice_and_ifft (slice +
+ *                       centred inverse 2D FFT, fused in LDS).
+ *   fvr_pou_render_bwd  the adjoint of cone_beam_fvr.py:547-573
+ *                       _project_core_invariant (slice -> iFFT -> warp) w.r.t.
+ *                       u_det, v_det, d_b, cu, cv, fused per (block, pose).
+ *   fvr_pou_ray_grads   the ray-array gradients grid_sample gives autograd in
+ *                       cone_beam_fvr.py:423-478 _project_core.
  */
 #ifndef FVR_HIP_H
 #define FVR_HIP_H
@@ -171,6 +177,54 @@ int fvr_slice_bwd(uint64_t vol, int32_t vol_dtype, int32_t b_vol, int32_t d, int
                   uint64_t grad_out, const float* u_xyz, const float* v_xyz, int32_t n_pose,
                   const float* fx, const float* fy, int32_t k, float cx, float cy, float cz,
                   float* part, float* gu, float* gv, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* PoU-FVR perspective path (cone_beam_fvr.py, triton_slice.py:349-808).      */
+/* Slab images imgs: float (B, N, P, P); d_b, cu, cv: float (B, N); rays rx,   */
+/* ry, ic: float (K) shared by every pose (per_pose_rays = 0) or (N, K);       */
+/* detector output / its gradient: float (N, K); K = H * W.                    */
+/* ------------------------------------------------------------------------ */
+/* 1 when P is a power of two in [4, 128] (the LDS-resident transforms). */
+int fvr_pou_lds_ok(int32_t P);
+/* F: planar block spectra (B, 2, P, P, P) in dtype 0 f32 / 1 f16 / 2 bf16; u, v:
+ * (N, 3) unit detector axes in (x, y, z) voxel order; sample (i, j) of slice
+ * (b, n) sits at P/2 + (j - P/2) u + (i - P/2) v; imgs = real(fftshift(ifft2(
+ * ifftshift(slice)))). */
+int fvr_pou_slice_ifft(uint64_t F, int32_t dtype, int32_t B, int32_t P, const float* u_xyz,
+                       const float* v_xyz, int32_t N, float* imgs, void* stream);
+/* out[n][q] = ic[q] * sum_b bilinear(imgs[b][n], (d_b r_x - cu) inv_ds + c_idx,
+ * (d_b r_y - cv) inv_ds + c_idx), zero outside the image. */
+int fvr_pou_warp_fwd(const float* imgs, int32_t B, int32_t N, int32_t P, const float* d_b,
+                     const float* cu, const float* cv, const float* rx, const float* ry,
+                     const float* ic, int32_t per_pose_rays, int32_t K, float inv_ds, float c_idx,
+                     float* out, void* stream);
+/* scale[n]: power of two with max_q |go[n][q] ic[q]| * scale <= 2^40 (1 when zero). */
+int fvr_pou_gscale(const float* go, const float* ic, int32_t N, int32_t K, int32_t per_pose_rays,
+                   float* scale, void* stream);
+/* Bytes of the 64-bit accumulator workspace of fvr_pou_warp_bwd. */
+int64_t fvr_pou_warp_bwd_workspace(int32_t B, int32_t N, int32_t P);
+/* part[b][n][3] = d/d(d_b, cu, cv); grad_imgs (B, N, P, P) when non-NULL (needs
+ * gscale and the workspace; accumulated in 64-bit fixed point: deterministic). */
+int fvr_pou_warp_bwd(const float* imgs, int32_t B, int32_t N, int32_t P, const float* go,
+                     const float* d_b, const float* cu, const float* cv, const float* rx,
+                     const float* ry, const float* ic, int32_t per_pose_rays, int32_t K, float inv_ds,
+                     float c_idx, const float* gscale, void* workspace, float* grad_imgs, float* part,
+                     void* stream);
+/* part[b][n][9] = d/d(d_b, cu, cv), d/du (3), d/dv (3) of sum go * render, for
+ * imgs produced by fvr_pou_slice_ifft with the same F, u, v (P as fvr_pou_lds_ok). */
+int fvr_pou_render_bwd(uint64_t F, int32_t dtype, int32_t B, int32_t P, const float* u_xyz,
+                       const float* v_xyz, int32_t N, const float* imgs, const float* go,
+                       const float* d_b, const float* cu, const float* cv, const float* rx,
+                       const float* ry, const float* ic, int32_t per_pose_rays, int32_t K,
+                       float inv_ds, float c_idx, const float* gscale, float* part, void* stream);
+/* grx/gry/gic (N, K): d/d(rx, ry, ic) of sum go * out for per-pose rays. */
+int fvr_pou_ray_grads(const float* imgs, int32_t B, int32_t N, int32_t P, const float* go,
+                      const float* d_b, const float* cu, const float* cv, const float* rx,
+                      const float* ry, const float* ic, int32_t per_pose_rays, int32_t K,
+                      float inv_ds, float c_idx, float* grx, float* gry, float* gic, void* stream);
+/* out[n][c] = sum_b part[b][n][c0 + c] in float64, fixed order (c < nc). */
+int fvr_pou_sum_blocks(const float* part, int32_t B, int32_t N, int32_t C, int32_t c0, int32_t nc,
+                       float* out, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* misc                                                                      */

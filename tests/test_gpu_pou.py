@@ -1,0 +1,265 @@
+"""GPU parity of the PoU-FVR perspective path (PerspectiveSlabFVR, pou_pinhole_warp_sum)
+against the FP64 oracle (oracle/cone_beam_np.py) and the reference's own checks
+(tests/test_cone_beam_fvr.py of the reference) on the HIP path.
+
+Tolerances: slab images and projections within 1e-4 of max|ref| (float32
+interpolation, float32 LDS FFT); warp outputs 1e-5; gradients 1e-4 of max
+against float64 torch autograd of the reference's grid_sample warp, and 2e-3
+of max between the fused adjoint and the generic composition.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import cone_beam_np as C
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda", 0)
+
+
+def _vol(n=24):
+    vol = np.zeros((n, n, n), dtype=np.float32)
+    zz, yy, xx = np.ogrid[:n, :n, :n]
+    vol[((zz - 11) ** 2 + (yy - 13) ** 2 + (xx - 12) ** 2) < 7 ** 2] = 0.08
+    vol[9:15, 5:9, 14:19] += 0.05
+    return vol
+
+
+@pytest.fixture(scope="module")
+def model():
+    from fvr import PerspectiveSlabFVR
+    vol = _vol()
+    m = PerspectiveSlabFVR.from_volume(vol, vox_mm=2.0, grid_blocks=(2, 2, 2), block_res=16)
+    o = C.from_volume(vol.astype(np.float64), 2.0, (2, 2, 2), 16)
+    return m, o
+
+
+def _lattice(H=20, W=22):
+    from fvr import PerspectiveSlabFVR
+    k_inv = torch.tensor([[1.0 / 300.0, 0.0, -W / 600.0], [0.0, 1.0 / 300.0, -H / 600.0], [0.0, 0.0, 1.0]])
+    lat = PerspectiveSlabFVR.make_pinhole_lattice(k_inv, 600.0, H, W)
+    lat_np = C.make_pinhole_lattice(k_inv.double().numpy(), 600.0, H, W)
+    return lat, lat_np
+
+
+def _se3(rotvec, t):
+    from scipy.spatial.transform import Rotation
+    M = np.tile(np.eye(4), (len(rotvec), 1, 1))
+    M[:, :3, :3] = Rotation.from_rotvec(rotvec).as_matrix()
+    M[:, :3, 3] = t
+    return M
+
+
+MATS = _se3(np.array([[0.15, -0.10, 0.08], [-0.20, 0.12, -0.05], [0.0, 0.0, 0.0]]),
+            np.array([[5.0, -4.0, -300.0], [-3.0, 6.0, -310.0], [0.0, 0.0, -300.0]]))
+
+
+def test_blocks_match_oracle(model):
+    m, o = model
+    assert (m.B, m.P) == (o["B"], o["P"]) and abs(m.ds_block - o["ds"]) < 1e-12
+    np.testing.assert_allclose(m.centers.cpu().numpy(), o["centers"], atol=1e-4)
+    F = m.F_blk_ri.double().cpu().numpy()
+    ref = o["F_blk"]
+    assert np.max(np.abs(F[:, 0] + 1j * F[:, 1] - ref)) <= 1e-6 * np.abs(ref).max()
+
+
+def test_fused_slab_images_match_oracle(model):
+    m, o = model
+    rng = np.random.default_rng(3)
+    Q = np.linalg.qr(rng.standard_normal((3, 3, 3)))[0]
+    u, v = Q[:, :, 0], Q[:, :, 1]
+    assert m.fused_ok()
+    imgs = m._slice_and_ifft(torch.from_numpy(u).float(), torch.from_numpy(v).float()).cpu().numpy()
+    ref = C.slab_images(o, u, v)
+    assert np.max(np.abs(imgs - ref)) <= 1e-4 * np.abs(ref).max()
+    m.fused_kernels = False
+    try:
+        gen = m._slice_and_ifft(torch.from_numpy(u).float(), torch.from_numpy(v).float()).cpu().numpy()
+    finally:
+        m.fused_kernels = True
+    assert np.max(np.abs(gen - ref)) <= 1e-4 * np.abs(ref).max()
+
+
+def test_warp_forward_and_gradients_match_grid_sample_autograd():
+    from fvr import pou_pinhole_warp_sum
+    from test_oracle_pou import _warp_torch_f64, warp_case
+    imgs, d_b, cu, cv, rx, ry, ic, inv_ds, c_idx = warp_case()
+    ref_in = [t.clone().requires_grad_(True) for t in (imgs, d_b, cu, cv)]
+    ref = _warp_torch_f64(*ref_in, rx, ry, ic, inv_ds, c_idx)
+    w = torch.randn(ref.shape, dtype=torch.float64, generator=torch.Generator().manual_seed(4))
+    (ref * w).sum().backward()
+    got_in = [t.float().to(DEV).requires_grad_(True) for t in (imgs, d_b, cu, cv)]
+    out = pou_pinhole_warp_sum(*got_in, rx.float().to(DEV), ry.float().to(DEV), ic.float().to(DEV),
+                               inv_ds=inv_ds, c_idx=c_idx)
+    assert out.shape == ref.shape
+    assert float((out.detach().double().cpu() - ref.detach()).abs().max()) <= 1e-5 * float(ref.detach().abs().max())
+    (out * w.float().to(DEV)).sum().backward()
+    for a, b in zip(got_in, ref_in):
+        ga, gb = a.grad.double().cpu(), b.grad
+        assert float((ga - gb).abs().max()) <= 1e-4 * float(gb.abs().max()), (ga - gb).abs().max()
+    # deterministic: a second backward gives identical bits
+    got2 = [t.float().to(DEV).requires_grad_(True) for t in (imgs, d_b, cu, cv)]
+    out2 = pou_pinhole_warp_sum(*got2, rx.float().to(DEV), ry.float().to(DEV), ic.float().to(DEV),
+                                inv_ds=inv_ds, c_idx=c_idx)
+    (out2 * w.float().to(DEV)).sum().backward()
+    for a, b in zip(got_in, got2):
+        assert torch.equal(a.grad, b.grad)
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_forward_pinhole_matches_oracle(model, fused):
+    m, o = model
+    lat, lat_np = _lattice()
+    m.fused_kernels = fused
+    try:
+        out = m.forward_pinhole(torch.from_numpy(MATS).float(), lat).cpu().numpy()
+    finally:
+        m.fused_kernels = True
+    ref = C.forward_pinhole(o, MATS, lat_np)
+    assert out.shape == ref.shape == (3, 1, 20, 22)
+    assert np.max(np.abs(out - ref)) <= 1e-4 * np.abs(ref).max()
+
+
+def _loss_grad(m, mats, lat, w):
+    M = torch.from_numpy(mats).float().to(DEV).requires_grad_(True)
+    out = m.forward_pinhole(M, lat)
+    (out * w).sum().backward()
+    return M.grad.double().cpu().numpy()
+
+
+def test_fused_adjoint_matches_generic_composition_and_fd(model):
+    m, o = model
+    lat, lat_np = _lattice()
+    w = torch.randn((3, 1, 20, 22), generator=torch.Generator().manual_seed(9)).to(DEV)
+    g_fused = _loss_grad(m, MATS, lat, w)
+    m.fused_kernels = False
+    try:
+        g_gen = _loss_grad(m, MATS, lat, w)
+    finally:
+        m.fused_kernels = True
+    assert np.max(np.abs(g_fused - g_gen)) <= 2e-3 * np.abs(g_gen).max(), np.abs(g_fused - g_gen).max()
+    # float64 central differences of the oracle on the translation entries
+    wn = w.double().cpu().numpy()
+    h = 1e-2
+    for n in range(2):
+        for k in range(3):
+            mp, mm = MATS.copy(), MATS.copy()
+            mp[n, k, 3] += h
+            mm[n, k, 3] -= h
+            fd = (np.sum(C.forward_pinhole(o, mp, lat_np) * wn) - np.sum(C.forward_pinhole(o, mm, lat_np) * wn)) / (2 * h)
+            assert abs(fd - g_fused[n, k, 3]) <= 2e-3 * max(abs(fd), np.abs(g_fused[:, :3, 3]).max()), (n, k, fd,
+                                                                                                      g_fused[n, k, 3])
+
+
+# --- the reference's PoU checks (tests/test_cone_beam_fvr.py) on the HIP path ------------------
+def test_reference_batched_pinhole_and_half_precision():
+    from fvr import PerspectiveSlabFVR
+    n = 32
+    vol = np.zeros((n, n, n), dtype=np.float32)
+    zz, yy, xx = np.ogrid[:n, :n, :n]
+    vol[((zz - 14) ** 2 + (yy - 17) ** 2 + (xx - 16) ** 2) < 9 ** 2] = 0.08
+    fvr = PerspectiveSlabFVR.from_volume(vol, vox_mm=2.0, grid_blocks=(2, 2, 2), block_res=32)
+    H = W = 48
+    k_inv = torch.tensor([[[1.0 / 300.0, 0.0, -W / 600.0], [0.0, 1.0 / 300.0, -H / 600.0], [0.0, 0.0, 1.0]]])
+    lattice = fvr.make_pinhole_lattice(k_inv, sdd_mm=600.0, height=H, width=W)
+    mats = torch.eye(4).unsqueeze(0).repeat(4, 1, 1)
+    mats[:, 2, 3] = -300.0
+    mats[:, 0, 3] = torch.tensor([0.0, 4.0, -3.0, 6.0])
+    mats[:, 1, 3] = torch.tensor([0.0, -2.0, 5.0, 1.0])
+    mats.requires_grad_(True)
+    batched = fvr.forward_pinhole(mats, lattice)
+    assert batched.shape == (4, 1, H, W)
+    for b in range(4):
+        single = fvr.forward_pinhole(mats[b:b + 1].detach(), lattice)
+        assert torch.allclose(batched[b:b + 1], single, atol=1e-4)
+    batched.sum().backward()
+    assert bool(torch.all(torch.isfinite(mats.grad))) and float(mats.grad.abs().sum()) > 1e-4
+    half = PerspectiveSlabFVR.from_volume(vol, vox_mm=2.0, grid_blocks=(2, 2, 2), block_res=32).to_precision(
+        torch.bfloat16)
+    mh = mats.detach().clone().requires_grad_(True)
+    oh = half.forward_pinhole(mh, lattice)
+    oh.sum().backward()
+    assert oh.dtype == torch.float32
+    assert torch.allclose(oh, batched.detach(), atol=0.15, rtol=0.05)
+    assert float(mh.grad.abs().sum()) > 1e-4
+
+
+def _torch_se3(rot_vec, trans):
+    """SE(3) from rotation vectors via the matrix exponential (as the reference's test builds it)."""
+    skew = torch.zeros((rot_vec.shape[0], 3, 3), dtype=rot_vec.dtype)
+    skew[:, 0, 1], skew[:, 1, 0] = -rot_vec[:, 2], rot_vec[:, 2]
+    skew[:, 0, 2], skew[:, 2, 0] = rot_vec[:, 1], -rot_vec[:, 1]
+    skew[:, 1, 2], skew[:, 2, 1] = -rot_vec[:, 0], rot_vec[:, 0]
+    mats = torch.eye(4, dtype=rot_vec.dtype).unsqueeze(0).repeat(rot_vec.shape[0], 1, 1)
+    mats[:, :3, :3] = torch.matrix_exp(skew)
+    mats[:, :3, 3] = trans
+    return mats
+
+
+def test_reference_invariant_matches_explicit_ray_path(model):
+    """SE(3)-invariant lattice path == explicit per-pose rays, outputs and 6-DoF gradients
+    (tests/test_cone_beam_fvr.py:230-274 of the reference, on-manifold parameters)."""
+    m, _ = model
+    lat, _ = _lattice(28, 28)
+    rot0 = torch.tensor([[0.15, -0.10, 0.08], [-0.20, 0.12, -0.05]])
+    xyz0 = torch.tensor([[5.0, -4.0, -300.0], [-3.0, 6.0, -310.0]])
+    r_a, t_a = rot0.clone().requires_grad_(True), xyz0.clone().requires_grad_(True)
+    r_b, t_b = rot0.clone().requires_grad_(True), xyz0.clone().requires_grad_(True)
+    out_inv = m.forward_pinhole(_torch_se3(r_a, t_a), lat)
+    out_exp = m.forward_pinhole(_torch_se3(r_b, t_b), lat[:5])
+    w = torch.randn(out_inv.shape, generator=torch.Generator().manual_seed(2)).to(DEV)
+    (out_inv * w).sum().backward()
+    (out_exp * w).sum().backward()
+    assert float(out_inv.abs().max()) > 0
+    assert torch.allclose(out_inv, out_exp, atol=1e-4, rtol=1e-4)
+    assert torch.allclose(r_a.grad, r_b.grad, atol=5e-4, rtol=1e-3), (r_a.grad - r_b.grad).abs().max()
+    assert torch.allclose(t_a.grad, t_b.grad, atol=5e-4, rtol=1e-3), (t_a.grad - t_b.grad).abs().max()
+
+
+def test_reference_carm_gradients_and_magnification():
+    from fvr import PerspectiveSlabFVR
+    n = 48
+    vol = np.zeros((n, n, n), dtype=np.float32)
+    zz, yy, xx = np.ogrid[:n, :n, :n]
+    vol[((zz - 24) ** 2 + (yy - 24) ** 2 + (xx - 24) ** 2) < 12 ** 2] = 0.05
+    cb = PerspectiveSlabFVR.from_volume(vol, vox_mm=2.0, grid_blocks=(2, 2, 2), block_res=40)  # P = 80: generic path
+    t = [torch.tensor(x) for x in (3.0, -2.0, 1.5, 1.0, -1.0, 0.5)]
+    target = cb.forward_carm_view(*t, sad_mm=600.0, sdd_mm=900.0, det_px=48, det_fov_mm=96.0).detach()
+    params = [torch.tensor(0.0, requires_grad=True) for _ in range(6)]
+    pred = cb.forward_carm_view(*params, sad_mm=600.0, sdd_mm=900.0, det_px=48, det_fov_mm=96.0)
+    a0, b0 = pred - pred.mean(), target - target.mean()
+    loss = 1.0 - (a0 * b0).sum() / (a0.norm() * b0.norm() + 1e-8)
+    loss.backward()
+    assert all(p.grad is not None and bool(torch.isfinite(p.grad)) for p in params)
+    assert any(abs(float(p.grad)) > 1e-4 for p in params)
+    # front/back magnification (reference sizes)
+    n, vox = 64, 2.0
+    vol = np.zeros((n, n, n), dtype=np.float32)
+    c = (np.arange(n) - (n - 1) / 2.0) * vox
+    zz, yy, xx = np.meshgrid(c, c, c, indexing="ij")
+    vol[((zz - 16.0) ** 2 + (yy + 32.0) ** 2 + (xx - 24.0) ** 2) < 9.0 ** 2] = 1.0
+    vol[((zz + 16.0) ** 2 + (yy - 32.0) ** 2 + (xx - 24.0) ** 2) < 9.0 ** 2] = 1.0
+    f = PerspectiveSlabFVR.from_volume(vol, vox_mm=vox, grid_blocks=(2, 4, 2), block_res=48)
+    z0 = torch.tensor(0.0)
+    img = f.forward_carm_view(z0, z0, z0, z0, z0, z0, sad_mm=300.0, sdd_mm=600.0, det_px=96, det_fov_mm=n * vox)
+    u = torch.linspace(-0.5, 0.5, 96, device=img.device) * (n * vox)
+    top, bot = img[48:, :], img[:48, :]
+    x_front = float((top.sum(0) * u).sum() / top.sum())
+    x_back = float((bot.sum(0) * u).sum() / bot.sum())
+    assert x_front > x_back + 7.0
+
+
+def test_reference_presets_and_offsets():
+    from fvr import SLAB_PRESETS, PerspectiveSlabFVR, sample_isocenter_parity_offsets
+    assert SLAB_PRESETS["isotropic"]["grid_blocks"] == (4, 4, 4) and SLAB_PRESETS["isotropic"]["block_res"] == 64
+    vol = np.zeros((32, 32, 32), dtype=np.float32)
+    vol[8:24, 8:24, 8:24] = 0.5
+    iso = PerspectiveSlabFVR.from_volume(vol, vox_mm=2.0, preset="isotropic", block_res=16)
+    assert iso.B == 64 and iso.F_blk_ri.dtype == torch.bfloat16
+    with pytest.raises(ValueError):
+        PerspectiveSlabFVR.resolve_slab_preset("nonexistent")
+    offs = sample_isocenter_parity_offsets((1.0, 2.0, 4.0))
+    assert offs.shape == (27, 3)
+    assert torch.allclose(offs.abs().max(dim=0).values, torch.tensor([0.5, 1.0, 2.0]))
