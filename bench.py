@@ -98,6 +98,70 @@ def time_plain(ev, P, reps: int = 2):
     return (time.perf_counter() - t0) * 1e3 / (reps * n)
 
 
+def pou_config4(steps: int, warmup: int, size: int = 256, n_pose: int = 64):
+    """PoU-FVR perspective workload: 256^3 seeded phantom, 4x4x4 blocks (25% overlap, cos^2
+    windows, 64^3 resampled, P = 128 BF16 spectra), pinhole SDD 1020 mm / SAD 510 mm, 256^2
+    detector at 0.8 mm, 64 candidate poses: forward (slice -> 128^2 LDS iFFT -> warp -> sum)
+    + NCC loss vs a seeded target + adjoint to the 6-DoF pose parameters."""
+    import numpy as np
+    import torch
+    from fvr import PerspectiveSlabFVR
+    from fvr.phantom import ellipsoid_phantom
+    dev = torch.device("cuda", 0)
+    vol = ellipsoid_phantom(size, n_ellipsoids=12, seed=1234).astype(np.float32)
+    t0 = time.perf_counter()
+    model = PerspectiveSlabFVR.from_volume(vol, vox_mm=1.0, preset="isotropic", device=dev)
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - t0
+    det, pix, sdd, sad = 256, 0.8, 1020.0, 510.0
+    f = sdd / pix
+    k_inv = torch.tensor([[1.0 / f, 0.0, -det / (2 * f)], [0.0, 1.0 / f, -det / (2 * f)], [0.0, 0.0, 1.0]], device=dev)
+    lattice = model.make_pinhole_lattice(k_inv, sdd, det, det)
+    rng = np.random.default_rng(3456)
+
+    def mats(rot_deg, trans_mm):
+        from scipy.spatial.transform import Rotation
+        M = np.tile(np.eye(4), (len(rot_deg), 1, 1))
+        M[:, :3, :3] = Rotation.from_euler("zyx", rot_deg, degrees=True).as_matrix()
+        M[:, :3, 3] = np.array([0.0, 0.0, -sad]) + trans_mm
+        return torch.from_numpy(M).float().to(dev)
+
+    true = mats(rng.uniform(-5, 5, (1, 3)), rng.uniform(-5, 5, (1, 3)))
+    cands = mats(rng.uniform(-5, 5, (n_pose, 3)), rng.uniform(-5, 5, (n_pose, 3)))
+    with torch.no_grad():
+        target = model.forward_pinhole(true, lattice)
+    t0c = target - target.mean()
+
+    def step():
+        M = cands.clone().requires_grad_(True)
+        out = model.forward_pinhole(M, lattice)
+        o0 = out - out.mean(dim=(-2, -1), keepdim=True)
+        ncc = (o0 * t0c).sum(dim=(-2, -1)) / (o0.norm(dim=(-2, -1)) * t0c.norm() + 1e-8)
+        loss = (1.0 - ncc).sum()
+        loss.backward()
+        return loss, M.grad
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        loss, g = step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    with torch.no_grad():
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            model.forward_pinhole(cands, lattice)
+        torch.cuda.synchronize()
+        ms_fwd = (time.perf_counter() - t1) * 1e3 / steps
+    return {"workload": "PoU-FVR 256^3, 4x4x4 blocks @64 (P=128, bf16), pinhole SDD 1020/SAD 510, 256^2 det @0.8mm",
+            "poses": n_pose, "blocks": model.B, "P": model.P, "ms_fwd_adj": round(ms, 3), "ms_fwd": round(ms_fwd, 3),
+            "pose_evals_per_s": round(n_pose / (ms * 1e-3), 1), "build_s": round(build_s, 2),
+            "loss": float(loss.detach()), "grad_finite": bool(torch.isfinite(g).all())}
+
+
 def cpu_baseline(renderer, T_ap, T_lat, cands, budget_s: float):
     """The NumPy/SciPy oracle (reference algorithm: 14 map_coordinates slices per
     pose eval) on one host core, on a bounded sample of the candidate poses."""
@@ -132,6 +196,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="headline storage only (no f32 / plain forms)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-pou", action="store_true", help="skip the PoU-FVR perspective workload")
     ap.add_argument("--no-footprint", action="store_true",
                     help="skip the distinct-entry count (roofline fields null; for profiler runs)")
     args = ap.parse_args(argv)
@@ -202,6 +267,12 @@ def main(argv=None) -> int:
         extra["f32_vs_storage_max_abs_dloss"] = float(np.max(np.abs(results["f32"]["loss"] - head["loss"])))
         g32 = results["f32"]["grad"]
         extra["f32_vs_storage_max_rel_dgrad"] = float(np.max(np.abs(g32 - head["grad"])) / np.max(np.abs(g32)))
+    if not args.no_extra and not args.no_pou:
+        log("PoU-FVR perspective workload (config 4)")
+        del r
+        torch.cuda.empty_cache()
+        extra["pou"] = pou_config4(max(3, args.steps // 10), 2)
+        log(f"PoU: {extra['pou']}")
     if plain_ms is not None:
         extra["plain_f32_per_pose_value"] = 1e3 / plain_ms
         extra["speedup_vs_plain_f32"] = head["value"] / (1e3 / plain_ms)
