@@ -229,7 +229,7 @@ def main(argv=None) -> int:
         step_ms, kern_ms, _ = time_batched(ev, P, args.steps, args.warmup)
         loss, grad = ev.evaluate(cands)
         results[st] = {"ms_per_step": step_ms, "kernel_ms": kern_ms, "value": args.n_cand / (step_ms * 1e-3),
-                       "loss_mean": float(loss.mean()), "loss": loss, "grad": grad}
+                       "loss_mean": float(loss.mean()), "loss": loss, "grad": grad, "herm": ev.herm}
         log(f"{st}: {step_ms:.3f} ms/step, fused kernel {kern_ms:.3f} ms, "
             f"{results[st]['value']:.0f} pose evals/s")
         del ev
@@ -237,18 +237,20 @@ def main(argv=None) -> int:
     if not args.no_extra:
         if r.storage != "f32":
             r.repack("f32")
-        ev = BiplanarEvaluator(r.packed, T_ap, T_lat, adjoint=True)
+        ev = BiplanarEvaluator(r.packed, T_ap, T_lat, adjoint=True, herm=False)  # plain: full slice, one pose
         plain_ms = time_plain(ev, P)
         log(f"plain one-pose-per-launch f32: {plain_ms:.3f} ms/pose")
         del ev
     head = results[args.storage]
     elem = 8 if args.storage == "f32" else 4
     if args.no_footprint:
-        fp = {"bytes": None, "unique_entries": None, "bricks": None}
+        fp = {"bytes": None, "unique_entries": None, "bricks": None,
+              "schedule": "hermitian-fold" if head["herm"] else "full-slice"}
         achieved = None
     else:
         log("counting the algorithmic footprint (distinct spectrum entries per slice)")
-        fp = batch_algorithmic_bytes((2 * args.size,) * 3, cands, elem, scaled=args.storage != "f32", device=dev)
+        fp = batch_algorithmic_bytes((2 * args.size,) * 3, cands, elem, scaled=args.storage != "f32", device=dev,
+                                     herm=head["herm"])
         achieved = fp["bytes"] / (head["kernel_ms"] * 1e-3) / 1e9
     traffic = None
     pmc_path = os.path.join(ROOT, "profiles", f"sncc_kernel_{args.storage}_{args.size}cube_{args.n_cand}cand.json")
@@ -300,6 +302,7 @@ def main(argv=None) -> int:
                      "traffic": None if traffic is None else traffic["bytes"], "traffic_detail": traffic,
                      "kernel": "sncc_kernel", "kernel_ms": round(head["kernel_ms"], 4),
                      "algorithmic_bytes_per_launch": fp["bytes"], "unique_entries": fp["unique_entries"],
+                     "schedule": fp.get("schedule"), "samples_per_launch": fp.get("samples"),
                      "bricks": fp["bricks"]},
         "cpu_baseline": cpu,
         "extra": extra,

@@ -30,6 +30,11 @@ struct PackArgs {
 
 __device__ __forceinline__ float2 pack_src(const PackArgs& a, int s, int m, int f) {
   int z = s - 1, y = m - 1, x = f - 1;  // logical (slow, mid, fast)
+  if (a.lay.flags & FVR_LAYOUT_WRAP) {  // logical index n is the periodic copy of index 0
+    if (z == a.lay.n[0]) z = 0;
+    if (y == a.lay.n[1]) y = 0;
+    if (x == a.lay.n[2]) x = 0;
+  }
   if (a.transpose) { int t = y; y = x; x = t; }
   if (z < 0 || z >= a.nz || y < 0 || y >= a.ny || x < 0 || x >= a.nx) return make_float2(0.f, 0.f);
   return a.src[(size_t(z) * a.ny + y) * a.nx + x];
@@ -140,15 +145,20 @@ int fvr_c64_pair_moments(uint64_t a, uint64_t b, int32_t h, int32_t w, double* o
 int fvr_layout_init(fvr_layout* lay, const int32_t n[3], int32_t dtype, uint64_t* storage_bytes,
                     uint64_t* n_scales) {
   FVR_CHECK_ARG(lay && n, "fvr_layout_init: null argument");
+  const uint32_t lflags = uint32_t(dtype) >> 8;
+  dtype &= 0xFF;
   FVR_CHECK_ARG(dtype == FVR_C32 || dtype == FVR_C16F || dtype == FVR_C16B, "fvr_layout_init: bad dtype %d",
                 dtype);
+  FVR_CHECK_ARG((lflags & ~FVR_LAYOUT_WRAP) == 0, "fvr_layout_init: bad layout flags %u", lflags);
   for (int a = 0; a < 3; ++a) FVR_CHECK_ARG(n[a] >= 2 && n[a] < (1 << 14), "fvr_layout_init: bad dim %d", n[a]);
   fvr_layout L = {};
   for (int a = 0; a < 3; ++a) L.n[a] = n[a];
   L.dtype = dtype;
-  L.sdim[0] = n[0] + 3;
-  L.sdim[1] = n[1] + 2;
-  L.sdim[2] = n[2] + 2;
+  L.flags = lflags;
+  const int extra = (lflags & FVR_LAYOUT_WRAP) ? 1 : 0;  // the wrap index n, then the usual zero margins
+  L.sdim[0] = n[0] + 3 + extra;
+  L.sdim[1] = n[1] + 2 + extra;
+  L.sdim[2] = n[2] + 2 + extra;
   for (int a = 0; a < 3; ++a) L.bricks[a] = (L.sdim[a] + 7) / 8;
   L.elem_bytes = dtype == FVR_C32 ? 8u : 4u;
   uint64_t row = uint64_t(L.sdim[2]) * L.elem_bytes;

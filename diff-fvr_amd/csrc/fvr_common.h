@@ -38,6 +38,12 @@ constexpr int kWave = 64;
 static_assert(FVR_SNCC_ROWS >= 1 && FVR_SNCC_ROWS <= 64, "rows per wave: 1..64 (row phasors live in lanes)");
 constexpr int kSnccTileH = 4 * FVR_SNCC_ROWS;
 
+// tile rows of one item (a Hermitian-fold item covers half of its rows)
+__host__ __device__ __forceinline__ int sncc_tiles_y(int h, uint32_t flags) {
+  const int rows = (flags & FVR_ITEM_HERM) ? (h >> 1) : h;
+  return (rows + kSnccTileH - 1) / kSnccTileH;
+}
+
 // fp16 / bf16 storage element -> float
 __device__ __forceinline__ float h2f(uint16_t h) { return __half2float(__ushort_as_half(h)); }
 __device__ __forceinline__ float b2f(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }

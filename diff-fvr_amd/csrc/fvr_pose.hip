@@ -98,8 +98,8 @@ __global__ void pn_build_kernel(const double* __restrict__ poses, int n_poses, f
   it.ph_col0 = turns_fx(uu0 * a / W);
   it.ph_row = turns_fx(b / H);
   it.ph_row0 = turns_fx(vv0 * b / H);
-  const int t_ap = tap.tiles_x * ((tap.h + kSnccTileH - 1) / kSnccTileH);
-  const int t_lat = tlat.tiles_x * ((tlat.h + kSnccTileH - 1) / kSnccTileH);
+  const int t_ap = tap.tiles_x * sncc_tiles_y(tap.h, tap.flags);
+  const int t_lat = tlat.tiles_x * sncc_tiles_y(tlat.h, tlat.flags);
   it.tile_base = pose * (t_ap + t_lat) + (lat ? t_ap : 0);
   items[idx] = it;
   double* J = jac + size_t(idx) * 18;
@@ -131,7 +131,7 @@ __global__ __launch_bounds__(64) void pn_reduce_combine_kernel(const fvr_sncc_it
 #pragma unroll
   for (int view = 0; view < 2; ++view) {
     const fvr_sncc_item& it = items[2 * pose + view];
-    const int ntile = it.tiles_x * ((it.h + kSnccTileH - 1) / kSnccTileH);
+    const int ntile = it.tiles_x * sncc_tiles_y(it.h, it.flags);
     double m[FVR_SNCC_NMOM];
 #pragma unroll
     for (int q = 0; q < FVR_SNCC_NMOM; ++q) m[q] = 0.0;
@@ -201,8 +201,8 @@ int fvr_pn_build_items(const double* poses, int32_t n_poses, const fvr_sncc_item
   FVR_CHECK_ARG(tmpl_host[0].w == int(px) && tmpl_host[0].h == int(pz) && tmpl_host[1].h == int(pz) &&
                     tmpl_host[1].w == int(py),
                 "fvr_pn_build_items: templates do not match the padded dims");
-  const int64_t t_pose = int64_t(tmpl_host[0].tiles_x) * ((tmpl_host[0].h + kSnccTileH - 1) / kSnccTileH) +
-                         int64_t(tmpl_host[1].tiles_x) * ((tmpl_host[1].h + kSnccTileH - 1) / kSnccTileH);
+  const int64_t t_pose = int64_t(tmpl_host[0].tiles_x) * sncc_tiles_y(tmpl_host[0].h, tmpl_host[0].flags) +
+                         int64_t(tmpl_host[1].tiles_x) * sncc_tiles_y(tmpl_host[1].h, tmpl_host[1].flags);
   FVR_CHECK_ARG(t_pose > 0 && int64_t(n_poses) * t_pose < (int64_t(1) << 31),
                 "fvr_pn_build_items: %d poses x %lld tiles overflow the 32-bit tile index", n_poses,
                 (long long)t_pose);

@@ -35,6 +35,8 @@ typedef const char __attribute__((address_space(1)))* gcptr;
 typedef const float __attribute__((address_space(1)))* gfptr;
 typedef float f2v __attribute__((ext_vector_type(2)));
 typedef const f2v __attribute__((address_space(1)))* gf2ptr;
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef const f4v __attribute__((address_space(1)))* gf4ptr;
 
 struct cpx {
   float r, i;
@@ -135,6 +137,32 @@ __device__ __forceinline__ cpx phasor(uint32_t t) {
   float s, c;
   sincospif(2.f * x, &s, &c);
   return {c, -s};
+}
+
+// deterministic workgroup reduction of the 16 moments into partials[tile][16];
+// the column-weighted sums are scaled by cw (1 when accumulated per sample)
+__device__ __forceinline__ void tile_reduce(const Acc& ac, float cw, int tile, float* __restrict__ partials) {
+  __shared__ float red[FVR_SNCC_NMOM][257];
+  const int tid = threadIdx.x;
+  red[0][tid] = ac.A;
+  red[1][tid] = ac.E;
+  red[2][tid] = cw * ac.sim;
+  red[3][tid] = ac.mrim;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    red[4 + a][tid] = cw * ac.gs[a];
+    red[7 + a][tid] = ac.gr[a];
+    red[10 + a][tid] = cw * ac.hs[a];
+    red[13 + a][tid] = ac.hr[a];
+  }
+  __syncthreads();
+  const int qn = tid >> 4, s = tid & 15;
+  float part = 0.f;
+#pragma unroll
+  for (int jj = 0; jj < 16; ++jj) part += red[qn][s * 16 + jj];
+#pragma unroll
+  for (int o = 8; o > 0; o >>= 1) part += __shfl_xor(part, o, 16);
+  if (s == 0) partials[size_t(tile) * FVR_SNCC_NMOM + qn] = part;
 }
 
 template <int DT, bool ADJ, bool HP, bool WS>
@@ -316,28 +344,231 @@ __global__ __launch_bounds__(256) void sncc_kernel(const fvr_layout* __restrict_
   consume(sa, kSnccRows - 2);
   consume(sb, kSnccRows - 1);
 
-  // ---- deterministic workgroup reduction of the 16 moments ----
-  __shared__ float red[FVR_SNCC_NMOM][257];
-  const int tid = threadIdx.x;
-  red[0][tid] = ac.A;
-  red[1][tid] = ac.E;
-  red[2][tid] = cw * ac.sim;
-  red[3][tid] = ac.mrim;
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    red[4 + a][tid] = cw * ac.gs[a];
-    red[7 + a][tid] = ac.gr[a];
-    red[10 + a][tid] = cw * ac.hs[a];
-    red[13 + a][tid] = ac.hr[a];
+  tile_reduce(ac, cw, tile, partials);
+}
+
+// ---------------------------------------------------------------------------
+// Hermitian-fold variant (convention N, wrapped layout).
+//
+// For the spectrum of a real volume the stored (wrapped) lattice satisfies
+// F(n - m) = conj F(m), so the trilinear sample at the mirror position n - p
+// of p is conj I(p) and the slice's phase ramp is conjugated too: the mirror
+// sample (h - i, w - j) of (i, j) is conj S(i, j).  Every moment is linear in
+// S conj(T) or in |S|^2 with weights cw(j), rw(i) that are odd under the
+// mirror exactly where the mirrored derivative is, so a pair contributes
+//   Re(S (bp conj T(i,j) + bq T(h-i, w-j)))  and  (bp + bq) |S|^2
+// (and the same for every moment), with bp / bq the scipy box tests of p and
+// of its mirror -- the box [0, n-1] is not symmetric about n/2, so the two
+// are tested separately and the pair is exact at the edges of the volume.
+// The item covers rows h/2 .. h-1 (row h/2 only right of the centre); the
+// unpaired row 0 is an extra row of the first wave, and the unpaired column 0
+// of rows 1 .. h/2-1 is evaluated by the otherwise idle lanes 1 .. w/2-1 of
+// row h/2.
+// ---------------------------------------------------------------------------
+template <int DT>
+struct SmpH {
+  Raw<DT> raw;
+  f4v t;  // (T(i,j), conj T(mirror))
+  float ts, tm, tf;
+  float wp, wq;  // box flags of the sample and of its mirror
+  float cw, rw;  // moment weights of the sample
+  float phr, phi;  // its phase factor
+};
+
+template <int DT, bool ADJ>
+__global__ __launch_bounds__(256) void sncc_herm_kernel(const fvr_layout* __restrict__ layouts, int n_layouts,
+                                                        const fvr_sncc_item* __restrict__ items, int n_items,
+                                                        int n_tiles, int per_xcd, float* __restrict__ partials) {
+  const int b = blockIdx.x;
+  const int tile = (b & 7) * per_xcd + (b >> 3);
+  if (tile >= n_tiles) return;
+  int lo = 0, hi = n_items - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].tile_base <= tile) lo = mid;
+    else hi = mid - 1;
   }
-  __syncthreads();
-  const int qn = tid >> 4, s = tid & 15;
-  float part = 0.f;
-#pragma unroll
-  for (int jj = 0; jj < 16; ++jj) part += red[qn][s * 16 + jj];
-#pragma unroll
-  for (int o = 8; o > 0; o >>= 1) part += __shfl_xor(part, o, 16);
-  if (s == 0) partials[size_t(tile) * FVR_SNCC_NMOM + qn] = part;
+  const fvr_sncc_item& it = items[lo];
+  if (uint32_t(it.layout) >= uint32_t(n_layouts)) {
+    if (threadIdx.x < FVR_SNCC_NMOM) partials[size_t(tile) * FVR_SNCC_NMOM + threadIdx.x] = 0.f;
+    return;
+  }
+  const fvr_layout& L = layouts[it.layout];
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int H = it.h, W = it.w, hh = H >> 1, hw = W >> 1, lo_int = it.lo_int;
+  const int lt = tile - it.tile_base;
+  const int tyi = lt / it.tiles_x, txi = lt - tyi * it.tiles_x;
+  const int j = txi * 64 + lane;
+  const bool colok = uint32_t(j) < uint32_t(W);
+  const int r0 = tyi * kSnccTileH + wave * kSnccRows;  // first representative row (offset from h/2)
+  int nreg = hh - r0;
+  nreg = nreg < 0 ? 0 : (nreg > kSnccRows ? kSnccRows : nreg);
+  const bool first = r0 == 0;
+  const int nslot = nreg + (first ? 1 : 0);
+
+  constexpr uint32_t esz = Raw<DT>::kElem;
+  const uint32_t rowb = L.row_bytes, planeb = L.plane_bytes;
+  const uint32_t lo_off = uint32_t(lo_int) * (planeb + rowb + esz);
+  const gcptr b0 = reinterpret_cast<gcptr>(L.base) + lo_off;
+  const gcptr b1 = b0 + planeb;
+  const uint32_t guard = it.guard_off - lo_off;
+  const gcptr scales = reinterpret_cast<gcptr>(L.scales);
+  const uint32_t Bm = uint32_t(L.bricks[1]), Bf = uint32_t(L.bricks[2]);
+  const gcptr TT = reinterpret_cast<gcptr>(it.target);
+
+  const uint64_t h0 = it.box_hi[0], h1 = it.box_hi[1], h2 = it.box_hi[2];
+  // mirror of a coordinate: K - p with K = n 2^32 + 2 bias = box_hi + 2^32 + bias
+  const uint64_t K0 = h0 + (1ull << 32) + (h0 & 0xFFFFFFFFull);
+  const uint64_t K1 = h1 + (1ull << 32) + (h1 & 0xFFFFFFFFull);
+  const uint64_t K2 = h2 + (1ull << 32) + (h2 & 0xFFFFFFFFull);
+  // per-lane column part of the position, phase and weight
+  const int64_t bj0 = it.c[0] + int64_t(j) * it.ustep[0];
+  const int64_t bj1 = it.c[1] + int64_t(j) * it.ustep[1];
+  const int64_t bj2 = it.c[2] + int64_t(j) * it.ustep[2];
+  const cpx colph = phasor(it.ph_col0 + uint32_t(j) * it.ph_col);
+  const float cwj = it.cw0 + float(j) * it.cws;
+  // row phasors of this wave's regular rows (lane k <-> row h/2 + r0 + k)
+  const cpx rowph = phasor(it.ph_row0 + uint32_t(hh + r0 + lane) * it.ph_row);
+
+  // stage 1: the sample of slot s -- position, box flags, loads
+  auto issue = [&](SmpH<DT>& sm, int s) {
+    int64_t p0, p1, p2;
+    bool valid, mirror;
+    uint32_t toff;
+    if (s < nreg && !(first && s == 0)) {  // a regular row i = h/2 + r0 + s (> h/2): every column but 0 paired
+      const int i = hh + r0 + s;
+      p0 = bj0 + int64_t(i) * it.vstep[0];
+      p1 = bj1 + int64_t(i) * it.vstep[1];
+      p2 = bj2 + int64_t(i) * it.vstep[2];
+      valid = colok;
+      mirror = j >= 1;
+      const float prr = readlane_f(rowph.r, s), pri = readlane_f(rowph.i, s);
+      sm.phr = colph.r * prr - colph.i * pri;
+      sm.phi = fmaf(colph.r, pri, colph.i * prr);
+      sm.cw = cwj;
+      sm.rw = it.rw0 + float(i) * it.rws;
+      toff = uint32_t(i * W + j) * 16u;
+    } else {  // row h/2 (remapped lanes take column 0 of rows 1 .. h/2-1) or the unpaired row 0
+      int ii, jj;
+      if (s == 0) {
+        const bool remap = j >= 1 && j < hw;
+        ii = remap ? j : hh;
+        jj = remap ? 0 : j;
+        valid = colok && (!remap || j <= hh - 1);
+        mirror = j > hw;
+      } else {
+        ii = 0;
+        jj = j;
+        valid = colok;
+        mirror = false;
+      }
+      p0 = it.c[0] + int64_t(jj) * it.ustep[0] + int64_t(ii) * it.vstep[0];
+      p1 = it.c[1] + int64_t(jj) * it.ustep[1] + int64_t(ii) * it.vstep[1];
+      p2 = it.c[2] + int64_t(jj) * it.ustep[2] + int64_t(ii) * it.vstep[2];
+      const cpx ph = phasor(it.ph_col0 + uint32_t(jj) * it.ph_col + it.ph_row0 + uint32_t(ii) * it.ph_row);
+      sm.phr = ph.r;
+      sm.phi = ph.i;
+      sm.cw = it.cw0 + float(jj) * it.cws;
+      sm.rw = it.rw0 + float(ii) * it.rws;
+      toff = uint32_t(ii * W + jj) * 16u;
+    }
+    const bool bp = valid & (uint64_t(p0) <= h0) & (uint64_t(p1) <= h1) & (uint64_t(p2) <= h2);
+    const bool bq = valid & mirror & (K0 - uint64_t(p0) <= h0) & (K1 - uint64_t(p1) <= h1) &
+                    (K2 - uint64_t(p2) <= h2);
+    const bool inb = bp | bq;
+    const uint32_t fs = inb ? uint32_t(uint64_t(p0) >> 32) : 0u;
+    const uint32_t fm = inb ? uint32_t(uint64_t(p1) >> 32) : 0u;
+    const uint32_t ff = inb ? uint32_t(uint64_t(p2) >> 32) : 0u;
+    sm.ts = float(uint32_t(p0)) * kTwoM32;
+    sm.tm = float(uint32_t(p1)) * kTwoM32;
+    sm.tf = float(uint32_t(p2)) * kTwoM32;
+    const uint32_t offi = __umul24(fs, planeb) + __umul24(fm, rowb) + ff * esz;
+    sm.raw.load(b0, b1, inb ? offi : guard, rowb, scales, fs + lo_int, fm + lo_int, ff + lo_int, Bm, Bf);
+    sm.t = *reinterpret_cast<gf4ptr>(TT + (valid ? toff : 0u));
+    sm.wp = bp ? 1.f : 0.f;
+    sm.wq = bq ? 1.f : 0.f;
+  };
+
+  Acc ac = {};
+  auto consume = [&](const SmpH<DT>& sm) {
+    const float ts = sm.ts, tm = sm.tm, tf = sm.tf;
+    const Raw<DT>& q = sm.raw;
+    const float a0r = q.r0(0), a0i = q.i0(0), b0r = q.r0(1), b0i = q.i0(1);
+    const float c0r = q.r0(2), c0i = q.i0(2), e0r_ = q.r0(3), e0i_ = q.i0(3);
+    const float d00r = q.r1(0) - a0r, d00i = q.i1(0) - a0i;
+    const float d01r = q.r1(1) - b0r, d01i = q.i1(1) - b0i;
+    const float d10r = q.r1(2) - c0r, d10i = q.i1(2) - c0i;
+    const float d11r = q.r1(3) - e0r_, d11i = q.i1(3) - e0i_;
+    const float e00r = fmaf(tf, d00r, a0r), e00i = fmaf(tf, d00i, a0i);
+    const float e01r = fmaf(tf, d01r, b0r), e01i = fmaf(tf, d01i, b0i);
+    const float e10r = fmaf(tf, d10r, c0r), e10i = fmaf(tf, d10i, c0i);
+    const float e11r = fmaf(tf, d11r, e0r_), e11i = fmaf(tf, d11i, e0i_);
+    const float y0r = e01r - e00r, y0i = e01i - e00i;
+    const float y1r = e11r - e10r, y1i = e11i - e10i;
+    const float f0r = fmaf(tm, y0r, e00r), f0i = fmaf(tm, y0i, e00i);
+    const float f1r = fmaf(tm, y1r, e10r), f1i = fmaf(tm, y1i, e10i);
+    const float zr = f1r - f0r, zi = f1i - f0i;
+    const float Ir = fmaf(ts, zr, f0r), Ii = fmaf(ts, zi, f0i);
+
+    // folded target T'' = bp T + bq conj T(mirror); W = phase conj(T'')
+    const float Tr = fmaf(sm.wq, sm.t.z, sm.wp * sm.t.x);
+    const float Ti = fmaf(sm.wq, sm.t.w, sm.wp * sm.t.y);
+    const float Wr = fmaf(sm.phr, Tr, sm.phi * Ti);
+    const float Wi = fmaf(sm.phi, Tr, -sm.phr * Ti);
+    const float wE = sm.wp + sm.wq;
+    const float Jr = Ir * wE, Ji = Ii * wE;
+    const float cw = sm.cw, rw = sm.rw;
+    const float IWr = fmaf(Ir, Wr, -Ii * Wi);
+    const float IWi = fmaf(Ir, Wi, Ii * Wr);
+    ac.A += IWr;
+    ac.sim = fmaf(cw, IWi, ac.sim);
+    ac.mrim = fmaf(rw, IWi, ac.mrim);
+    ac.E = fmaf(Ir, Jr, fmaf(Ii, Ji, ac.E));
+    if (ADJ) {
+      const float gmr = fmaf(ts, y1r - y0r, y0r), gmi = fmaf(ts, y1i - y0i, y0i);
+      const float x0r = fmaf(tm, d01r - d00r, d00r), x0i = fmaf(tm, d01i - d00i, d00i);
+      const float x1r = fmaf(tm, d11r - d10r, d10r), x1i = fmaf(tm, d11i - d10i, d10i);
+      const float gfr = fmaf(ts, x1r - x0r, x0r), gfi = fmaf(ts, x1i - x0i, x0i);
+      const float g0 = fmaf(zr, Wr, -zi * Wi), g1 = fmaf(gmr, Wr, -gmi * Wi), g2 = fmaf(gfr, Wr, -gfi * Wi);
+      const float k0 = fmaf(zr, Jr, zi * Ji), k1 = fmaf(gmr, Jr, gmi * Ji), k2 = fmaf(gfr, Jr, gfi * Ji);
+      ac.gs[0] = fmaf(cw, g0, ac.gs[0]);
+      ac.gs[1] = fmaf(cw, g1, ac.gs[1]);
+      ac.gs[2] = fmaf(cw, g2, ac.gs[2]);
+      ac.gr[0] = fmaf(rw, g0, ac.gr[0]);
+      ac.gr[1] = fmaf(rw, g1, ac.gr[1]);
+      ac.gr[2] = fmaf(rw, g2, ac.gr[2]);
+      ac.hs[0] = fmaf(cw, k0, ac.hs[0]);
+      ac.hs[1] = fmaf(cw, k1, ac.hs[1]);
+      ac.hs[2] = fmaf(cw, k2, ac.hs[2]);
+      ac.hr[0] = fmaf(rw, k0, ac.hr[0]);
+      ac.hr[1] = fmaf(rw, k1, ac.hr[1]);
+      ac.hr[2] = fmaf(rw, k2, ac.hr[2]);
+    }
+  };
+
+  if (nslot > 0) {  // wave-uniform
+    SmpH<DT> sa, sb;
+    issue(sa, 0);
+    int s = 0;
+#pragma unroll 1
+    for (; s + 2 < nslot; s += 2) {
+      issue(sb, s + 1);
+      consume(sa);
+      __builtin_amdgcn_sched_barrier(0);
+      issue(sa, s + 2);
+      consume(sb);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    if (s + 1 < nslot) {
+      issue(sb, s + 1);
+      consume(sa);
+      consume(sb);
+    } else {
+      consume(sa);
+    }
+  }
+  tile_reduce(ac, 1.f, tile, partials);
 }
 
 // ---------------------------------------------------------------------------
@@ -347,7 +578,7 @@ __global__ __launch_bounds__(256) void sncc_reduce_kernel(const fvr_sncc_item* _
                                                           const float* __restrict__ partials,
                                                           double* __restrict__ out) {
   const fvr_sncc_item& it = items[blockIdx.x];
-  const int ntile = it.tiles_x * ((it.h + kSnccTileH - 1) / kSnccTileH);
+  const int ntile = it.tiles_x * sncc_tiles_y(it.h, it.flags);
   double acc[FVR_SNCC_NMOM];
 #pragma unroll
   for (int q = 0; q < FVR_SNCC_NMOM; ++q) acc[q] = 0.0;
@@ -380,6 +611,16 @@ template <int DT>
 static bool dispatch_flags(uint32_t flags, dim3 g, hipStream_t st, const fvr_layout* lay, int n_lay,
                            const fvr_sncc_item* items, int n_items, int n_tiles, int per_xcd, float* partials) {
   const bool adj = flags & FVR_ITEM_ADJOINT, hp = flags & FVR_ITEM_HP, ws = flags & FVR_ITEM_WRITE_SLICE;
+  if (flags & FVR_ITEM_HERM) {
+    if (hp || ws) return false;  // the fold is defined for convention N without slice output
+    if (adj)
+      hipLaunchKernelGGL((sncc_herm_kernel<DT, true>), g, dim3(256), 0, st, lay, n_lay, items, n_items, n_tiles,
+                         per_xcd, partials);
+    else
+      hipLaunchKernelGGL((sncc_herm_kernel<DT, false>), g, dim3(256), 0, st, lay, n_lay, items, n_items, n_tiles,
+                         per_xcd, partials);
+    return true;
+  }
 #define FVR_L(a, h, w)                                                                     \
   if (adj == a && hp == h && ws == w) {                                                    \
     launch_sncc<DT, a, h, w>(g, st, lay, n_lay, items, n_items, n_tiles, per_xcd, partials); \
@@ -409,7 +650,7 @@ int fvr_sncc_eval_v(const fvr_layout* layouts_dev, int32_t n_layouts, const fvr_
                     int32_t n_tiles, float* partials, uint32_t variant, void* stream) {
   FVR_CHECK_ARG(layouts_dev && items_dev && partials, "fvr_sncc_eval: null pointer");
   FVR_CHECK_ARG(n_items > 0 && n_tiles > 0 && n_layouts > 0, "fvr_sncc_eval: empty batch");
-  FVR_CHECK_ARG((variant >> 4) < 8u, "fvr_sncc_eval: bad flags %u", variant >> 4);
+  FVR_CHECK_ARG((variant >> 4) < 16u, "fvr_sncc_eval: bad flags %u", variant >> 4);
   const int dt = int(variant & 0xF);
   FVR_CHECK_ARG(dt == FVR_C32 || dt == FVR_C16F || dt == FVR_C16B, "fvr_sncc_eval: bad dtype %d", dt);
   const int per_xcd = (n_tiles + 7) / 8;

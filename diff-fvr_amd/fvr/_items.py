@@ -91,15 +91,26 @@ def slice_dims_n(pad_shape, view) -> Tuple[int, int]:
     return py, px
 
 
-def tiles_for(h: int, w: int, tile_rows: int) -> Tuple[int, int]:
-    return (w + 63) // 64, (h + tile_rows - 1) // tile_rows
+def tiles_for(h: int, w: int, tile_rows: int, herm: bool = False) -> Tuple[int, int]:
+    """(tiles per row, tile rows) of an item; a Hermitian-fold item covers h/2 rows."""
+    rows = h // 2 if herm else h
+    return (w + 63) // 64, (rows + tile_rows - 1) // tile_rows
+
+
+def herm_ok(h: int, w: int) -> bool:
+    """Slice shapes the Hermitian-fold kernel covers: even sides, h <= w (column 0 of the
+    lower rows is evaluated by the idle lanes of row h/2)."""
+    return h % 2 == 0 and w % 2 == 0 and h <= w and h >= 2
 
 
 def n_item(packed, view: str, yaw, pitch, roll, a, b, target_ptr: int, flags: int, tile_base: int,
            tile_rows: int, slice_out: int = 0) -> np.ndarray:
-    """One convention-N item (numpy structured scalar array of length 1)."""
+    """One convention-N item (numpy structured scalar array of length 1).  With
+    ``ITEM_HERM`` in ``flags`` the item uses the wrapped layout and ``target_ptr`` is
+    the folded (T, conj T(mirror)) float4 array of the view."""
     pad_shape = packed.shape
-    lay_idx, perm = packed.layout_for(view)
+    herm = bool(flags & _lib.ITEM_HERM)
+    lay_idx, perm = packed.layout_for(view, herm)
     desc = packed.desc(lay_idx)
     h, w = slice_dims_n(pad_shape, view)
     u, v = euler_to_basis_n(yaw, pitch, roll, view)
@@ -195,8 +206,8 @@ def n_biplanar_items(templates: np.ndarray, pad_shape, poses: np.ndarray, lat_pe
     templates = _lib.as_struct(templates, _lib.ITEM_DTYPE)
     items = np.empty(2 * N, dtype=_lib.ITEM_DTYPE)
     jac = np.empty((2 * N, 2, 3, 3))
-    tiles = [int(tiles_for(int(t["h"]), int(t["w"]), tile_rows)[0] * tiles_for(int(t["h"]), int(t["w"]),
-                                                                               tile_rows)[1]) for t in templates]
+    tiles = [int(np.prod(tiles_for(int(t["h"]), int(t["w"]), tile_rows, bool(t["flags"] & _lib.ITEM_HERM))))
+             for t in templates]
     for view in (0, 1):
         t = templates[view]
         perm = [0, 1, 2] if view == 0 else list(lat_perm)
@@ -245,7 +256,21 @@ def validate_items(items: np.ndarray, n_layouts: int, descs: np.ndarray, tile_ro
         raise ValueError("item target pointer is null")
     if np.any(((items["flags"] & _lib.ITEM_WRITE_SLICE) != 0) & (items["slice_out"] == 0)):
         raise ValueError("slice output requested without a buffer")
-    ntile = ((items["w"] + 63) // 64) * ((items["h"] + tile_rows - 1) // tile_rows)
+    herm = (items["flags"] & _lib.ITEM_HERM) != 0
+    rows = np.where(herm, items["h"] // 2, items["h"])
+    ntile = ((items["w"] + 63) // 64) * ((rows + tile_rows - 1) // tile_rows)
+    if np.any(herm):
+        hi_ = items[herm]
+        if np.any((hi_["flags"] & (_lib.ITEM_HP | _lib.ITEM_WRITE_SLICE)) != 0) or np.any(hi_["lo_int"] != 1):
+            raise ValueError("Hermitian-fold items are convention N without high-pass or slice output")
+        if np.any(hi_["h"] % 2) or np.any(hi_["w"] % 2) or np.any(hi_["h"] > hi_["w"]):
+            raise ValueError("Hermitian-fold items need even h <= w")
+        if np.any((descs[hi_["layout"]]["flags"] & _lib.LAYOUT_WRAP) == 0):
+            raise ValueError("Hermitian-fold items need a wrapped layout")
+        # the fold samples up to floor(p) = n: its corners (+lo, +1) must stay inside the storage
+        dh = descs[hi_["layout"]]
+        if np.any((hi_["box_hi"] >> np.uint64(32)).astype(np.int64) + 3 > dh["sdim"].astype(np.int64) - 1):
+            raise ValueError("Hermitian-fold item box exceeds the wrapped layout's storage")
     if np.any(items["tiles_x"] != (items["w"] + 63) // 64):
         raise ValueError("item tiles_x inconsistent with its width")
     base = np.concatenate([[0], np.cumsum(ntile)[:-1]])

@@ -20,6 +20,9 @@ from . import _items, _lib
 from .sncc import _Workspace, pair_moments, run_items, tile_rows
 from .spectrum import PackedSpectrum, padded_shape, padded_spectrum
 
+# zero-margin copies (single slices, FD scheme) and wrapped copies (Hermitian-fold batches)
+PACK_LAYOUTS = ("zyx", "zxy", "zyx_h", "zxy_h")
+
 
 class DifferentiableFourierVolumeRenderer:
     """Differentiable & spectral Fourier volume renderer (parallel beam).
@@ -40,7 +43,7 @@ class DifferentiableFourierVolumeRenderer:
         spec = padded_spectrum(vol, pad_factor, apply_premultiplication, self.device)
         self._spec = spec  # complex128, centred, (pz, py, px)
         self.storage = storage
-        self.packed = PackedSpectrum(spec, dtype=storage)
+        self.packed = PackedSpectrum(spec, dtype=storage, layouts=PACK_LAYOUTS)
         self._ws = _Workspace(self.device)
         self._zero_targets: Dict[Tuple[int, int], torch.Tensor] = {}
 
@@ -53,7 +56,7 @@ class DifferentiableFourierVolumeRenderer:
     def repack(self, storage: str) -> "DifferentiableFourierVolumeRenderer":
         """Re-store the spectrum as "f32", "fp16" or "bf16" (per-8^3-brick scale)."""
         self.storage = storage
-        self.packed = PackedSpectrum(self._spec, dtype=storage)
+        self.packed = PackedSpectrum(self._spec, dtype=storage, layouts=PACK_LAYOUTS)
         return self
 
     # -- geometry (diff_fvr.py:56-104) -------------------------------------

@@ -19,18 +19,45 @@ import torch
 from ._items import euler_to_basis_n, slice_dims_n
 
 
-def slice_footprint(pad_shape, pose, view: str, device="cpu") -> Tuple[int, int]:
-    """(U, bricks) of one convention-N slice at pose (yaw, pitch, roll in degrees)."""
+def _herm_slots(h: int, w: int, device):
+    """(ii, jj, mirror) of the Hermitian-fold schedule (sncc_herm_kernel): rows h/2+1..h-1,
+    row h/2 right of the centre plus columns 0 and w/2, column 0 of rows 1..h/2-1, row 0."""
+    hh, hw = h // 2, w // 2
+    I, J = torch.meshgrid(torch.arange(hh + 1, h, device=device), torch.arange(w, device=device), indexing="ij")
+    ii, jj, mm = [I.reshape(-1)], [J.reshape(-1)], [(J >= 1).reshape(-1)]
+    jr = torch.arange(w, device=device)
+    keep = (jr == 0) | (jr >= hw)
+    ii.append(torch.full_like(jr[keep], hh)), jj.append(jr[keep]), mm.append(jr[keep] > hw)
+    rm = torch.arange(1, min(hw, hh), device=device)
+    ii.append(rm), jj.append(torch.zeros_like(rm)), mm.append(torch.zeros_like(rm, dtype=torch.bool))
+    ii.append(torch.zeros_like(jr)), jj.append(jr), mm.append(torch.zeros_like(jr, dtype=torch.bool))
+    return torch.cat(ii), torch.cat(jj), torch.cat(mm)
+
+
+def slice_footprint(pad_shape, pose, view: str, device="cpu", herm: bool = False) -> Tuple[int, int, int]:
+    """(U, bricks, samples) of one convention-N slice at pose (yaw, pitch, roll in degrees).
+
+    ``herm``: the samples the Hermitian-fold kernel evaluates (about half of the slice,
+    each inside if it or its mirror is inside the scipy box) on the wrapped lattice.
+    """
     pz, py, px = pad_shape
     h, w = slice_dims_n(pad_shape, view)
     u, v = euler_to_basis_n(pose[0], pose[1], pose[2], view)
-    uu = torch.arange(w, dtype=torch.float64, device=device) - w / 2.0
-    vv = torch.arange(h, dtype=torch.float64, device=device) - h / 2.0
-    V, U_ = torch.meshgrid(vv, uu, indexing="ij")
     c = np.array(pad_shape, dtype=np.float64) / 2.0
-    p = torch.stack([U_ * u[a] + V * v[a] + c[a] for a in range(3)], dim=-1).reshape(-1, 3)
     n = torch.tensor([pz, py, px], dtype=torch.float64, device=device)
-    inb = ((p >= 0) & (p <= n - 1)).all(dim=1)
+    if herm:
+        ii, jj, mirror = _herm_slots(h, w, device)
+        U_, V = jj.to(torch.float64) - w / 2.0, ii.to(torch.float64) - h / 2.0
+        p = torch.stack([U_ * u[a] + V * v[a] + c[a] for a in range(3)], dim=-1)
+        q = n - p
+        inb = ((p >= 0) & (p <= n - 1)).all(dim=1) | (mirror & ((q >= 0) & (q <= n - 1)).all(dim=1))
+    else:
+        uu = torch.arange(w, dtype=torch.float64, device=device) - w / 2.0
+        vv = torch.arange(h, dtype=torch.float64, device=device) - h / 2.0
+        V, U_ = torch.meshgrid(vv, uu, indexing="ij")
+        p = torch.stack([U_ * u[a] + V * v[a] + c[a] for a in range(3)], dim=-1).reshape(-1, 3)
+        inb = ((p >= 0) & (p <= n - 1)).all(dim=1)
+    samples = int(p.shape[0])
     p = p[inb]
     i0 = torch.floor(p).to(torch.int64)
     t = p - i0
@@ -43,27 +70,31 @@ def slice_footprint(pad_shape, pose, view: str, device="cpu") -> Tuple[int, int]
                        * (t[:, 2] if dx else 1 - t[:, 2]))
                 idx = i0 + torch.tensor([dz, dy, dx], device=device)
                 idx = idx[wgt > 0]
-                keys.append((idx[:, 0] * py + idx[:, 1]) * px + idx[:, 2])
+                keys.append((idx[:, 0] * (py + 2) + idx[:, 1]) * (px + 2) + idx[:, 2])
                 # storage bricks: storage index = logical + 1, 8^3 bricks
                 b = (idx + 1) // 8
                 bkeys.append((b[:, 0] * 4096 + b[:, 1]) * 4096 + b[:, 2])
     U = int(torch.unique(torch.cat(keys)).numel())
     B = int(torch.unique(torch.cat(bkeys)).numel())
-    return U, B
+    return U, B, samples
 
 
-def batch_algorithmic_bytes(pad_shape, poses: np.ndarray, elem_bytes: int, scaled: bool, device="cpu") -> dict:
-    """Algorithmic bytes of one biplanar launch over ``poses`` (AP + LAT per pose)."""
-    total_u = 0
-    total_b = 0
+def batch_algorithmic_bytes(pad_shape, poses: np.ndarray, elem_bytes: int, scaled: bool, device="cpu",
+                            herm: bool = False) -> dict:
+    """Algorithmic bytes of one biplanar launch over ``poses`` (AP + LAT per pose).
+
+    ``herm``: counted for the Hermitian-fold schedule (its half-plane entries and its
+    16-byte folded target entries)."""
+    total_u = total_b = total_s = 0
     for p in poses:
         for view in ("ap", "lateral"):
-            U, B = slice_footprint(pad_shape, p, view, device)
+            U, B, S = slice_footprint(pad_shape, p, view, device, herm)
             total_u += U
             total_b += B
+            total_s += S
     h_ap, w_ap = slice_dims_n(pad_shape, "ap")
     h_la, w_la = slice_dims_n(pad_shape, "lateral")
-    targ = (h_ap * w_ap + h_la * w_la) * 8
+    targ = (total_s // max(len(poses), 1)) * 16 if herm else (h_ap * w_ap + h_la * w_la) * 8
     spec = total_u * elem_bytes + (total_b * 4 if scaled else 0)
     return {"unique_entries": total_u, "bricks": total_b, "spectrum_bytes": spec, "target_bytes": targ,
-            "bytes": spec + targ, "samples": len(poses) * (h_ap * w_ap + h_la * w_la)}
+            "bytes": spec + targ, "samples": total_s, "schedule": "hermitian-fold" if herm else "full-slice"}

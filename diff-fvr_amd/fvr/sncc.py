@@ -46,8 +46,19 @@ class _Workspace:
 
 def item_tiles(items: np.ndarray, rows: int) -> np.ndarray:
     tx = (items["w"] + 63) // 64
-    ty = (items["h"] + rows - 1) // rows
+    r = np.where((items["flags"] & _lib.ITEM_HERM) != 0, items["h"] // 2, items["h"])
+    ty = (r + rows - 1) // rows
     return (tx * ty).astype(np.int64)
+
+
+def fold_target(t: torch.Tensor) -> torch.Tensor:
+    """(h, w, 4) float32 (Re T, Im T, Re, Im of conj T(h-i, w-j)) for the Hermitian-fold
+    kernel; the mirror part is 0 in row 0 and column 0 (they have no mirror in the grid)."""
+    h, w = t.shape
+    tc = t.to(torch.complex64)
+    m = torch.zeros_like(tc)
+    m[1:, 1:] = torch.conj(torch.flip(tc[1:, 1:], dims=(0, 1)))
+    return torch.stack([tc.real, tc.imag, m.real, m.imag], dim=-1).contiguous()
 
 
 def run_items(packed: PackedSpectrum, items: np.ndarray, flags: int,
@@ -130,26 +141,34 @@ class BiplanarEvaluator:
     """
 
     def __init__(self, packed: PackedSpectrum, target_ap: torch.Tensor, target_lat: torch.Tensor,
-                 adjoint: bool = True):
+                 adjoint: bool = True, herm: Optional[bool] = None):
         self.packed = packed
         self.device = packed.device
         self.hw = {"ap": _items.slice_dims_n(packed.shape, "ap"), "lateral": _items.slice_dims_n(packed.shape, "lateral")}
+        fold_ok = packed.has_herm() and all(_items.herm_ok(*self.hw[v]) for v in ("ap", "lateral"))
+        # Hermitian fold: half the samples, same sums (the default whenever the layouts and shapes allow it)
+        self.herm = fold_ok if herm is None else bool(herm)
+        if self.herm and not fold_ok:
+            raise ValueError("Hermitian fold needs the wrapped layouts and even slice sides with h <= w")
         self.targets = tuple(torch.zeros(self.hw[v], dtype=torch.complex64, device=self.device)
                              for v in ("ap", "lateral"))
+        self.folded = tuple(torch.zeros(self.hw[v] + (4,), dtype=torch.float32, device=self.device)
+                            for v in ("ap", "lateral")) if self.herm else None
         self.t2 = np.zeros(2)
         self.set_targets(target_ap, target_lat)
         self.rows = tile_rows()
-        self.flags = _lib.ITEM_ADJOINT if adjoint else 0
+        self.flags = (_lib.ITEM_ADJOINT if adjoint else 0) | (_lib.ITEM_HERM if self.herm else 0)
+        tgt = self.folded if self.herm else self.targets
         self.templates = _lib.as_struct(np.concatenate([
-            _items.n_template(packed, "ap", self.targets[0].data_ptr(), self.flags, self.rows),
-            _items.n_template(packed, "lateral", self.targets[1].data_ptr(), self.flags, self.rows)]),
+            _items.n_template(packed, "ap", tgt[0].data_ptr(), self.flags, self.rows),
+            _items.n_template(packed, "lateral", tgt[1].data_ptr(), self.flags, self.rows)]),
             _lib.ITEM_DTYPE)
-        self.lat_perm = packed.layout_for("lateral")[1]
+        self.lat_perm = packed.layout_for("lateral", self.herm)[1]
         self.lat_swap = int(tuple(self.lat_perm) == (0, 2, 1))
         # every address-forming field the device builder copies comes from these templates
         _items.validate_items(self.build(np.zeros((1, 6)))[0], len(self.packed.descs), self.packed.descs, self.rows)
-        ta = _items.tiles_for(*self.hw["ap"], self.rows)
-        tl = _items.tiles_for(*self.hw["lateral"], self.rows)
+        ta = _items.tiles_for(*self.hw["ap"], self.rows, self.herm)
+        tl = _items.tiles_for(*self.hw["lateral"], self.rows, self.herm)
         self.tiles_per_pose = ta[0] * ta[1] + tl[0] * tl[1]
         self.ws = _Workspace(self.device)
         self.last_moments: Optional[np.ndarray] = None
@@ -162,6 +181,8 @@ class BiplanarEvaluator:
                 raise ValueError(f"{name} target shape {tuple(t.shape)} != slice shape {self.hw[name]}")
             self.targets[k].copy_(t)
             self.t2[k] = target_energy(self.targets[k])
+            if self.folded is not None:
+                self.folded[k].copy_(fold_target(self.targets[k]))
 
     # -- device path ---------------------------------------------------------
     def prepare(self, n: int):

@@ -54,7 +54,9 @@ class PackedSpectrum:
     """Device-resident packed layouts of one centred complex spectrum.
 
     ``layouts`` names which copies to build: ``"zyx"`` (fast axis x) and/or
-    ``"zxy"`` (fast axis y).  ``table`` is the device array of ``fvr_layout``
+    ``"zxy"`` (fast axis y), and their wrapped forms ``"zyx_h"`` / ``"zxy_h"``
+    (index n of every axis holds the periodic copy of index 0, for the
+    Hermitian-fold kernel).  ``table`` is the device array of ``fvr_layout``
     descriptors in that order; ``index[name]`` its position.
     """
 
@@ -74,13 +76,15 @@ class PackedSpectrum:
         descs = np.zeros(len(layouts), dtype=_lib.LAYOUT_DTYPE)
         Z, Y, X = self.shape
         for k, name in enumerate(layouts):
-            if name not in ("zyx", "zxy"):
+            if name not in ("zyx", "zxy", "zyx_h", "zxy_h"):
                 raise ValueError(f"unknown layout {name!r}")
-            transpose = name == "zxy"
+            transpose = name.startswith("zxy")
+            wrap = name.endswith("_h")
             n = np.array([Z, X, Y] if transpose else [Z, Y, X], dtype=np.int32)
             one = np.zeros(1, dtype=_lib.LAYOUT_DTYPE)
             nbytes, nsc = ctypes.c_uint64(0), ctypes.c_uint64(0)
-            _lib.check(L.fvr_layout_init(one.ctypes.data, n.ctypes.data, self.code, ctypes.byref(nbytes),
+            code = self.code | ((_lib.LAYOUT_WRAP << 8) if wrap else 0)
+            _lib.check(L.fvr_layout_init(one.ctypes.data, n.ctypes.data, code, ctypes.byref(nbytes),
                                          ctypes.byref(nsc)), "fvr_layout_init")
             store = torch.empty(int(nbytes.value), dtype=torch.uint8, device=self.device)
             one["base"] = store.data_ptr()
@@ -97,11 +101,16 @@ class PackedSpectrum:
         self.descs = descs
         self.table = _lib.struct_to_device(descs, self.device, _lib.LAYOUT_DTYPE)
 
-    def layout_for(self, view: str) -> Tuple[int, Tuple[int, int, int]]:
-        """(layout index, permutation layout-axis -> (z, y, x) axis) for a view."""
-        if view == "lateral" and "zxy" in self.index:
-            return self.index["zxy"], (0, 2, 1)
-        return self.index["zyx"], (0, 1, 2)
+    def layout_for(self, view: str, herm: bool = False) -> Tuple[int, Tuple[int, int, int]]:
+        """(layout index, permutation layout-axis -> (z, y, x) axis) for a view
+        (the wrapped copy when ``herm``)."""
+        sfx = "_h" if herm else ""
+        if view == "lateral" and "zxy" + sfx in self.index:
+            return self.index["zxy" + sfx], (0, 2, 1)
+        return self.index["zyx" + sfx], (0, 1, 2)
+
+    def has_herm(self) -> bool:
+        return "zyx_h" in self.index
 
     def desc(self, k: int):
         return self.descs[k]

@@ -61,7 +61,12 @@ typedef enum fvr_dtype {
 /* A packed spectrum: logical dims (slow, mid, fast) stored with a one-voxel
  * zero margin below and zero margins above every axis, so that out-of-range
  * trilinear corners read 0.  Storage index = logical index + 1.
- * Storage dims: slow n0+3, mid n1+2, fast n2+2 (zero rows/planes beyond). */
+ * Storage dims: slow n0+3, mid n1+2, fast n2+2 (zero rows/planes beyond).
+ * With FVR_LAYOUT_WRAP the logical index n of every axis holds the periodic
+ * copy of index 0 (the DFT's own wrap, so that the stored lattice is
+ * Hermitian-symmetric about n/2 for the spectrum of a real volume) and the
+ * storage dims are slow n0+4, mid n1+3, fast n2+3. */
+#define FVR_LAYOUT_WRAP 1u
 typedef struct fvr_layout {
   uint64_t base;        /* device address of storage element (0,0,0) */
   uint64_t scales;      /* device address of float brick scales (0 for FVR_C32) */
@@ -72,12 +77,13 @@ typedef struct fvr_layout {
   uint32_t elem_bytes;  /* 8 (C32) or 4 (C16F, C16B) */
   uint32_t row_bytes;   /* sdim[2] * elem_bytes */
   uint32_t plane_bytes; /* sdim[1] * row_bytes */
-  uint32_t pad0;
+  uint32_t flags;       /* FVR_LAYOUT_* */
   uint64_t total_bytes; /* sdim[0] * plane_bytes */
 } fvr_layout;
 
-/* Fill a layout descriptor for dims n (slow, mid, fast) and dtype; returns the
- * storage bytes and, for scaled dtypes, the scale-array element count. */
+/* Fill a layout descriptor for dims n (slow, mid, fast) and dtype (| FVR_LAYOUT_WRAP << 8
+ * for the wrapped layout); returns the storage bytes and, for scaled dtypes, the
+ * scale-array element count. */
 int fvr_layout_init(fvr_layout* lay, const int32_t n[3], int32_t dtype,
                     uint64_t* storage_bytes, uint64_t* n_scales);
 
@@ -96,6 +102,12 @@ int fvr_spec_pack(const fvr_layout* lay, uint64_t src, int32_t nz, int32_t ny, i
 #define FVR_ITEM_ADJOINT 1u     /* accumulate the pose-adjoint moments */
 #define FVR_ITEM_HP 2u          /* apply the separable high-pass weight (convention T) */
 #define FVR_ITEM_WRITE_SLICE 4u /* also store the phase-modulated slice S (complex64) */
+/* Hermitian fold (convention N, wrapped layout, even h <= w): the item covers the
+ * rows h/2 .. h-1, each sample also standing for its mirror (h - i, w - j), plus
+ * the unpaired row 0 and column 0; `target` then points at float4 (T(i,j),
+ * conj T(h-i, w-j)) per sample (zero mirror part in row 0 / column 0).  Sums are
+ * those of the full slice; the tile count covers h/2 rows. */
+#define FVR_ITEM_HERM 8u
 
 /* One (pose, view) evaluation.  Sample (row i, col j) sits at
  *   p = c + j * ustep + i * vstep        (32.32 fixed point, layout axes,

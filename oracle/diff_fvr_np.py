@@ -298,7 +298,7 @@ def biplanar_loss(f_3d, pose, f_target_ap, f_target_lat) -> float:
 # ---------------------------------------------------------------------------
 # analytic: trilinear value + spatial gradient with scipy's constant-mode rule
 # ---------------------------------------------------------------------------
-def trilinear_value_and_grad(f_3d, coords):
+def trilinear_value_and_grad(f_3d, coords, left=None, wrap=False):
     """Value and d/d(z, y, x) of scipy's order-1 'constant' interpolant.
 
     scipy rule (pinned by brute force in the tests): a sample whose coordinate
@@ -307,11 +307,22 @@ def trilinear_value_and_grad(f_3d, coords):
     build's): forward difference across the cell [floor(p), floor(p)+1], with
     a corner index of n (reached only when p == n-1 exactly) reading 0.
     Returns I (K,) complex and dI (3, K) complex.
+
+    The two options matter only where a coordinate is exactly an integer (the
+    interpolant has a kink there; the value is unaffected):
+    ``left`` (bool (K,)) takes the backward difference, cell [ceil(p)-1, ceil(p)],
+    for those samples; ``wrap`` reads corner index n as index 0 (the periodic
+    spectrum).  Together they give the one-sided derivatives of the Hermitian-fold
+    kernel: forward on the wrapped lattice for the samples it evaluates,
+    backward for the mirrors it infers from them.
     """
     shp = np.array(f_3d.shape)
     p = np.asarray(coords, dtype=np.float64)
     inb = np.all((p >= 0.0) & (p <= (shp - 1)[:, None]), axis=0)
     i0 = np.floor(p).astype(np.int64)
+    if left is not None:
+        on = (p == np.floor(p)) & np.asarray(left, bool)[None, :]
+        i0 = np.where(on, i0 - 1, i0)
     t = p - i0
     K = p.shape[1]
     vals = {}
@@ -319,6 +330,8 @@ def trilinear_value_and_grad(f_3d, coords):
         for dy in (0, 1):
             for dx in (0, 1):
                 iz, iy, ix = i0[0] + dz, i0[1] + dy, i0[2] + dx
+                if wrap:
+                    iz, iy, ix = (np.where(q == n, 0, q) for q, n in zip((iz, iy, ix), shp))
                 ok = inb & (iz >= 0) & (iz < shp[0]) & (iy >= 0) & (iy < shp[1]) & (ix >= 0) & (ix < shp[2])
                 v = np.zeros(K, dtype=np.complex128)
                 v[ok] = f_3d[iz[ok], iy[ok], ix[ok]]
@@ -343,11 +356,21 @@ def trilinear_value_and_grad(f_3d, coords):
     return I, dI
 
 
-def view_ncc_and_grad_analytic(f_3d, yaw, pitch, roll, a, b, view, f_targ):
+def fold_mirror_mask(slice_h: int, slice_w: int) -> np.ndarray:
+    """(h*w,) True for the samples the Hermitian-fold kernel infers as mirrors:
+    rows 1 .. h/2-1 (columns >= 1) and row h/2 left of the centre (columns >= 1)."""
+    i, j = np.meshgrid(np.arange(slice_h), np.arange(slice_w), indexing="ij")
+    hh, hw = slice_h // 2, slice_w // 2
+    return (((i >= 1) & (i < hh) & (j >= 1)) | ((i == hh) & (j >= 1) & (j < hw))).ravel()
+
+
+def view_ncc_and_grad_analytic(f_3d, yaw, pitch, roll, a, b, view, f_targ, fold=False):
     """Exact NCC and its derivative w.r.t. (yaw, pitch, roll, a, b) for one view.
 
     S = I(c + uu u + vv v) * exp(-2 pi i (uu a/W + vv b/H));
     dNCC = Re<dS,T>/(|S||T|) - NCC Re<dS,S>/|S|^2.
+    ``fold``: the one-sided derivative convention of the Hermitian-fold kernel at
+    lattice-exact coordinates (see ``trilinear_value_and_grad``).
     Returns (ncc, d (5,), aux) where aux holds the raw sums.
     """
     pad_shape = f_3d.shape
@@ -355,7 +378,10 @@ def view_ncc_and_grad_analytic(f_3d, yaw, pitch, roll, a, b, view, f_targ):
     u_vec, v_vec = euler_to_basis(yaw, pitch, roll, view)
     du, dv = euler_basis_jacobian(yaw, pitch, roll, view)
     coords, uu, vv = slice_coords(pad_shape, u_vec, v_vec, view)
-    I, dI = trilinear_value_and_grad(f_3d, coords)
+    if fold:
+        I, dI = trilinear_value_and_grad(f_3d, coords, left=fold_mirror_mask(slice_h, slice_w), wrap=True)
+    else:
+        I, dI = trilinear_value_and_grad(f_3d, coords)
     uu, vv = uu.ravel(), vv.ravel()
     ph = np.exp(-2j * np.pi * (uu * a / slice_w + vv * b / slice_h))
     S = I * ph
@@ -379,15 +405,16 @@ def view_ncc_and_grad_analytic(f_3d, yaw, pitch, roll, a, b, view, f_targ):
     return ncc, d, {"A": A, "E": ns ** 2, "T2": nt ** 2, "S": S.reshape(slice_h, slice_w)}
 
 
-def biplanar_loss_and_grad_analytic(f_3d, pose, f_target_ap, f_target_lat):
+def biplanar_loss_and_grad_analytic(f_3d, pose, f_target_ap, f_target_lat, fold=False):
     """Exact loss and 6-DoF gradient of L = 1/2(1-NCC_AP) + 1/2(1-NCC_LAT).
 
     Same loss as gradients.py:95-102; rotation derivatives exact instead of the
-    reference's central differences (gradients.py:112-132).
+    reference's central differences (gradients.py:112-132).  ``fold`` selects the
+    Hermitian-fold kernel's one-sided derivatives at lattice-exact poses.
     """
     yaw, pitch, roll, tx, ty, tz = [float(x) for x in pose]
-    n_ap, d_ap, _ = view_ncc_and_grad_analytic(f_3d, yaw, pitch, roll, tx, tz, "ap", f_target_ap)
-    n_la, d_la, _ = view_ncc_and_grad_analytic(f_3d, yaw, pitch, roll, ty, tz, "lateral", f_target_lat)
+    n_ap, d_ap, _ = view_ncc_and_grad_analytic(f_3d, yaw, pitch, roll, tx, tz, "ap", f_target_ap, fold)
+    n_la, d_la, _ = view_ncc_and_grad_analytic(f_3d, yaw, pitch, roll, ty, tz, "lateral", f_target_lat, fold)
     loss = 0.5 * (1.0 - n_ap) + 0.5 * (1.0 - n_la)
     grad = np.zeros(6)
     grad[:3] = -0.5 * (d_ap[:3] + d_la[:3])

@@ -92,7 +92,7 @@ def test_item_builder_identity_geometry():
         shape = (16, 20, 24)
         index = {"zyx": 0, "zxy": 1}
 
-        def layout_for(self, view):
+        def layout_for(self, view, herm=False):
             return (1, (0, 2, 1)) if view == "lateral" else (0, (0, 1, 2))
 
         def desc(self, k):

@@ -96,7 +96,10 @@ def test_analytic_adjoint_parity(setup, pose):
     T_ap, T_la = _targets(f3, [1.0, -2.0, 1.5, 0.7, -1.2, 0.4])
     eng = SpectralGradientEngine(r)
     L, g = eng.compute_biplanar_loss_and_gradients(np.array(pose), T_ap, T_la, rotation_grad="analytic")
-    Lr, gr = O.biplanar_loss_and_grad_analytic(f3, np.array(pose), T_ap, T_la)
+    # the identity pose puts samples on lattice planes, where the interpolant has a kink: the
+    # Hermitian-fold kernel's one-sided derivatives there are the oracle's fold convention
+    fold = eng._evaluator(T_ap, T_la, True).herm
+    Lr, gr = O.biplanar_loss_and_grad_analytic(f3, np.array(pose), T_ap, T_la, fold=fold)
     assert abs(L - Lr) <= 1e-5 * max(abs(Lr), 1e-3)
     assert np.max(np.abs(g - gr)) <= 1e-4 * np.max(np.abs(gr)) + 1e-9, (g, gr)
 
@@ -119,7 +122,8 @@ def test_batched_equals_single_and_is_deterministic(setup):
         Ls, gs = eng.compute_biplanar_loss_and_gradients(poses[k], T_ap, T_la, rotation_grad="analytic")
         assert Ls == L1[k]
         np.testing.assert_array_equal(gs, g1[k])
-        Lr, gr = O.biplanar_loss_and_grad_analytic(f3, poses[k], T_ap, T_la)
+        Lr, gr = O.biplanar_loss_and_grad_analytic(f3, poses[k], T_ap, T_la,
+                                                   fold=eng._evaluator(T_ap, T_la, True).herm)
         assert abs(L1[k] - Lr) <= 1e-5 * max(abs(Lr), 1e-3)
         assert np.max(np.abs(g1[k] - gr)) <= 1e-4 * np.max(np.abs(gr)) + 1e-9
 
@@ -205,3 +209,34 @@ def test_device_setup_matches_host_setup(setup):
     L_h, g_h = ev.evaluate_host(poses)
     np.testing.assert_allclose(L_d, L_h, rtol=1e-9, atol=1e-12)
     np.testing.assert_allclose(g_d, g_h, rtol=1e-6, atol=1e-9 * np.abs(g_h).max())
+
+
+@pytest.mark.parametrize("storage", ["f32", "fp16"])
+def test_hermitian_fold_equals_full_slice_kernel(setup, storage):
+    """The Hermitian-fold kernel (half the samples) against the full-slice kernel on the
+    same spectrum and targets: same sums up to float32 summation order."""
+    from fvr.phantom import random_poses
+    from fvr.sncc import BiplanarEvaluator
+    _, r, f3 = setup
+    if r.packed.dtype != storage:
+        r.repack(storage)
+    try:
+        T_ap, T_la = _targets(f3, [1.0, -2.0, 1.5, 0.7, -1.2, 0.4])
+        dev = r.device
+        ta = torch.from_numpy(T_ap.astype(np.complex64)).to(dev)
+        tl = torch.from_numpy(T_la.astype(np.complex64)).to(dev)
+        full = BiplanarEvaluator(r.packed, ta, tl, herm=False)
+        fold = BiplanarEvaluator(r.packed, ta, tl, herm=True)
+        assert fold.herm and not full.herm
+        # generic poses: off the lattice planes both kernels differentiate the same function
+        poses = random_poses(8, 15.0, 5.0, seed=91)
+        L1, g1 = full.evaluate(poses)
+        L2, g2 = fold.evaluate(poses)
+        np.testing.assert_allclose(L2, L1, rtol=2e-6, atol=1e-7)
+        assert np.max(np.abs(g2 - g1)) <= 2e-5 * np.abs(g1).max(), np.abs(g2 - g1).max()
+        # the identity pose: equal losses (values are not affected by the kink)
+        np.testing.assert_allclose(fold.evaluate(np.zeros((1, 6)))[0], full.evaluate(np.zeros((1, 6)))[0],
+                                   rtol=2e-6, atol=1e-7)
+    finally:
+        if r.packed.dtype != "f32":
+            r.repack("f32")

@@ -28,7 +28,7 @@ class _Packed:
         self.descs = descs
         self.index = {"zyx": 0, "zxy": 1}
 
-    def layout_for(self, view):
+    def layout_for(self, view, herm=False):
         return (1, (0, 2, 1)) if view == "lateral" else (0, (0, 1, 2))
 
     def desc(self, k):
@@ -136,7 +136,8 @@ def test_combine_reproduces_oracle_gradient():
 def test_footprint_counter_equals_direct_count():
     from fvr.footprint import slice_footprint
     shape, pose = (40, 40, 40), (7.0, -4.0, 3.0)
-    U, B = slice_footprint(shape, pose, "lateral")
+    U, B, S = slice_footprint(shape, pose, "lateral")
+    assert S == shape[0] * shape[1]
     u, v = O.euler_to_basis(*pose, "lateral")
     c, _, _ = O.slice_coords(shape, u, v, "lateral")
     n = np.array(shape)[:, None]
@@ -180,3 +181,15 @@ def test_item_byte_image_keeps_the_c_layout():
     packed["target"] = [7, 9]
     assert _lib.as_struct(packed, _lib.ITEM_DTYPE)["target"].tolist() == [7, 9]
     assert _lib.struct_bytes(packed, _lib.ITEM_DTYPE).size == 400
+
+
+def test_hermitian_footprint_is_about_half():
+    """The fold schedule touches about half of the full slice's distinct entries and
+    evaluates about half of its samples."""
+    from fvr.footprint import slice_footprint
+    shape, pose = (40, 40, 40), (7.0, -4.0, 3.0)
+    U, B, S = slice_footprint(shape, pose, "ap")
+    Uh, Bh, Sh = slice_footprint(shape, pose, "ap", herm=True)
+    h, w = 40, 40
+    assert Sh == (h // 2 - 1) * w + (1 + w // 2) + (min(h, w) // 2 - 1) + w
+    assert 0.45 < Uh / U < 0.6 and 0.45 < Sh / S < 0.6
