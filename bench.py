@@ -253,7 +253,8 @@ def main(argv=None) -> int:
                                      herm=head["herm"])
         achieved = fp["bytes"] / (head["kernel_ms"] * 1e-3) / 1e9
     traffic = None
-    pmc_path = os.path.join(ROOT, "profiles", f"sncc_kernel_{args.storage}_{args.size}cube_{args.n_cand}cand.json")
+    kname = "sncc_herm_kernel" if head["herm"] else "sncc_kernel"
+    pmc_path = os.path.join(ROOT, "profiles", f"{kname}_{args.storage}_{args.size}cube_{args.n_cand}cand.json")
     if os.path.exists(pmc_path):
         pmc = json.load(open(pmc_path))
         traffic = {"bytes": pmc["traffic_bytes_per_launch"], "l2_hit_rate": pmc["l2_hit_rate"],
@@ -300,7 +301,7 @@ def main(argv=None) -> int:
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": None if achieved is None else round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None if traffic is None else traffic["bytes"], "traffic_detail": traffic,
-                     "kernel": "sncc_kernel", "kernel_ms": round(head["kernel_ms"], 4),
+                     "kernel": kname, "kernel_ms": round(head["kernel_ms"], 4),
                      "algorithmic_bytes_per_launch": fp["bytes"], "unique_entries": fp["unique_entries"],
                      "schedule": fp.get("schedule"), "samples_per_launch": fp.get("samples"),
                      "bricks": fp["bricks"]},
