@@ -114,6 +114,7 @@ __global__ __launch_bounds__(1024) void c64_pair_kernel(const float2* __restrict
 
 static_assert(sizeof(fvr_layout) == 80, "fvr_layout ABI size");
 static_assert(sizeof(fvr_sncc_item) == 200, "fvr_sncc_item ABI size");
+static_assert(sizeof(fvr_adam_cfg) == 232, "fvr_adam_cfg ABI size");
 
 }  // namespace fvr
 
@@ -129,7 +130,7 @@ int fvr_struct_sizes(int32_t* out) {
   if (!out) return FVR_ERR_INVALID;
   out[0] = int32_t(sizeof(fvr_layout));
   out[1] = int32_t(sizeof(fvr_sncc_item));
-  out[2] = 0;
+  out[2] = int32_t(sizeof(fvr_adam_cfg));
   out[3] = 0;
   return FVR_OK;
 }

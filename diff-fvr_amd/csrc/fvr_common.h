@@ -31,17 +31,19 @@ void set_error(const char* fmt, ...);
 
 constexpr int kWave = 64;
 
-// rows of one fused-kernel tile (4 waves x FVR_SNCC_ROWS rows each); tiles are 64 columns wide
-#ifndef FVR_SNCC_ROWS
-#define FVR_SNCC_ROWS 16
-#endif
-static_assert(FVR_SNCC_ROWS >= 1 && FVR_SNCC_ROWS <= 64, "rows per wave: 1..64 (row phasors live in lanes)");
-constexpr int kSnccTileH = 4 * FVR_SNCC_ROWS;
+// default rows of one fused-kernel tile (4 waves x 16 rows); tiles are 64 columns wide.
+// A launch may use tile_h = 32 or 128 instead (item field tile_h).
+constexpr int kSnccTileH = 64;
 
-// tile rows of one item (a Hermitian-fold item covers half of its rows)
-__host__ __device__ __forceinline__ int sncc_tiles_y(int h, uint32_t flags) {
+__host__ __device__ __forceinline__ bool sncc_tile_h_ok(int tile_h) {
+  return tile_h == 32 || tile_h == 64 || tile_h == 128;
+}
+
+// tile rows of one item (a Hermitian-fold item covers half of its rows); 0 for a bad tile_h
+__host__ __device__ __forceinline__ int sncc_tiles_y(int h, uint32_t flags, int tile_h) {
+  if (!sncc_tile_h_ok(tile_h)) return 0;
   const int rows = (flags & FVR_ITEM_HERM) ? (h >> 1) : h;
-  return (rows + kSnccTileH - 1) / kSnccTileH;
+  return (rows + tile_h - 1) / tile_h;
 }
 
 // fp16 / bf16 storage element -> float

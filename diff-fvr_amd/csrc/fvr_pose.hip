@@ -55,8 +55,8 @@ __device__ __forceinline__ void put_jac(double* J, const V3& d0, const V3& d1, c
 }
 
 __global__ void pn_build_kernel(const double* __restrict__ poses, int n_poses, fvr_sncc_item tap, fvr_sncc_item tlat,
-                                double pz, double py, double px, int lat_swap, fvr_sncc_item* __restrict__ items,
-                                double* __restrict__ jac) {
+                                double pz, double py, double px, int lat_swap, int tile0,
+                                fvr_sncc_item* __restrict__ items, double* __restrict__ jac) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= 2 * n_poses) return;
   const int pose = idx >> 1;
@@ -98,9 +98,9 @@ __global__ void pn_build_kernel(const double* __restrict__ poses, int n_poses, f
   it.ph_col0 = turns_fx(uu0 * a / W);
   it.ph_row = turns_fx(b / H);
   it.ph_row0 = turns_fx(vv0 * b / H);
-  const int t_ap = tap.tiles_x * sncc_tiles_y(tap.h, tap.flags);
-  const int t_lat = tlat.tiles_x * sncc_tiles_y(tlat.h, tlat.flags);
-  it.tile_base = pose * (t_ap + t_lat) + (lat ? t_ap : 0);
+  const int t_ap = tap.tiles_x * sncc_tiles_y(tap.h, tap.flags, tap.tile_h);
+  const int t_lat = tlat.tiles_x * sncc_tiles_y(tlat.h, tlat.flags, tlat.tile_h);
+  it.tile_base = tile0 + pose * (t_ap + t_lat) + (lat ? t_ap : 0);
   items[idx] = it;
   double* J = jac + size_t(idx) * 18;
   const double k = deg;
@@ -131,7 +131,7 @@ __global__ __launch_bounds__(64) void pn_reduce_combine_kernel(const fvr_sncc_it
 #pragma unroll
   for (int view = 0; view < 2; ++view) {
     const fvr_sncc_item& it = items[2 * pose + view];
-    const int ntile = it.tiles_x * sncc_tiles_y(it.h, it.flags);
+    const int ntile = it.tiles_x * sncc_tiles_y(it.h, it.flags, it.tile_h);
     double m[FVR_SNCC_NMOM];
 #pragma unroll
     for (int q = 0; q < FVR_SNCC_NMOM; ++q) m[q] = 0.0;
@@ -189,6 +189,44 @@ __global__ __launch_bounds__(64) void pn_reduce_combine_kernel(const fvr_sncc_it
   }
 }
 
+// Batched Adam, one thread per pose (gradient_optimizer.py:45-83 per pose, with
+// the gradient clamp and cosine learning-rate factor of the multistart driver
+// of experiments/exp30_gpu_multistart_registration.py:140-183).
+__global__ void adam_step_kernel(double* __restrict__ poses, const double* __restrict__ loss,
+                                 const double* __restrict__ grad, double* __restrict__ m, double* __restrict__ v,
+                                 int32_t* __restrict__ t, int32_t* __restrict__ done, double* __restrict__ best_loss,
+                                 double* __restrict__ best_pose, int n, fvr_adam_cfg c) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n || done[k]) return;
+  const int step = t[k] + 1;
+  t[k] = step;
+  double* P = poses + 6 * k;
+  const double L = loss[k];
+  if (L < best_loss[k]) {
+    best_loss[k] = L;
+    for (int a = 0; a < 6; ++a) best_pose[6 * k + a] = P[a];
+  }
+  double f = 1.0;
+  if (c.cosine_t > 0) {
+    const double s = double(step - 1 < c.cosine_t ? step - 1 : c.cosine_t);
+    f = 0.1 + 0.45 * (1.0 + cos(3.14159265358979323846 * s / double(c.cosine_t)));
+  }
+  const double bc1 = 1.0 - pow(c.beta1, double(step)), bc2 = 1.0 - pow(c.beta2, double(step));
+  double mx = 0.0;
+  for (int a = 0; a < 6; ++a) {
+    double g = grad[6 * k + a];
+    if (c.gclip[a] > 0.0) g = fmin(fmax(g, -c.gclip[a]), c.gclip[a]);
+    const double mm = c.beta1 * m[6 * k + a] + (1.0 - c.beta1) * g;
+    const double vv = c.beta2 * v[6 * k + a] + (1.0 - c.beta2) * g * g;
+    m[6 * k + a] = mm;
+    v[6 * k + a] = vv;
+    const double upd = f * c.lr[a] * (mm / bc1) / (sqrt(vv / bc2) + c.eps);
+    mx = fmax(mx, fabs(upd));
+    P[a] = fmin(fmax(P[a] - upd, c.lo[a]), c.hi[a]);
+  }
+  if (mx < c.tol) done[k] = 1;
+}
+
 }  // namespace fvr
 
 using namespace fvr;
@@ -196,19 +234,24 @@ using namespace fvr;
 extern "C" {
 
 int fvr_pn_build_items(const double* poses, int32_t n_poses, const fvr_sncc_item* tmpl_host, double pz, double py,
-                       double px, int32_t lat_swap, fvr_sncc_item* items_dev, double* jac_dev, void* stream) {
-  FVR_CHECK_ARG(poses && tmpl_host && items_dev && jac_dev && n_poses > 0, "fvr_pn_build_items: bad arguments");
+                       double px, int32_t lat_swap, int32_t tile0, fvr_sncc_item* items_dev, double* jac_dev,
+                       void* stream) {
+  FVR_CHECK_ARG(poses && tmpl_host && items_dev && jac_dev && n_poses > 0 && tile0 >= 0,
+                "fvr_pn_build_items: bad arguments");
   FVR_CHECK_ARG(tmpl_host[0].w == int(px) && tmpl_host[0].h == int(pz) && tmpl_host[1].h == int(pz) &&
                     tmpl_host[1].w == int(py),
                 "fvr_pn_build_items: templates do not match the padded dims");
-  const int64_t t_pose = int64_t(tmpl_host[0].tiles_x) * sncc_tiles_y(tmpl_host[0].h, tmpl_host[0].flags) +
-                         int64_t(tmpl_host[1].tiles_x) * sncc_tiles_y(tmpl_host[1].h, tmpl_host[1].flags);
-  FVR_CHECK_ARG(t_pose > 0 && int64_t(n_poses) * t_pose < (int64_t(1) << 31),
+  FVR_CHECK_ARG(sncc_tile_h_ok(tmpl_host[0].tile_h) && tmpl_host[1].tile_h == tmpl_host[0].tile_h,
+                "fvr_pn_build_items: bad template tile_h %d / %d", tmpl_host[0].tile_h, tmpl_host[1].tile_h);
+  const int64_t t_pose =
+      int64_t(tmpl_host[0].tiles_x) * sncc_tiles_y(tmpl_host[0].h, tmpl_host[0].flags, tmpl_host[0].tile_h) +
+      int64_t(tmpl_host[1].tiles_x) * sncc_tiles_y(tmpl_host[1].h, tmpl_host[1].flags, tmpl_host[1].tile_h);
+  FVR_CHECK_ARG(t_pose > 0 && int64_t(tile0) + int64_t(n_poses) * t_pose < (int64_t(1) << 31),
                 "fvr_pn_build_items: %d poses x %lld tiles overflow the 32-bit tile index", n_poses,
                 (long long)t_pose);
   const int n = 2 * n_poses;
   hipLaunchKernelGGL(pn_build_kernel, dim3((n + 63) / 64), dim3(64), 0, reinterpret_cast<hipStream_t>(stream), poses,
-                     n_poses, tmpl_host[0], tmpl_host[1], pz, py, px, lat_swap, items_dev, jac_dev);
+                     n_poses, tmpl_host[0], tmpl_host[1], pz, py, px, lat_swap, int(tile0), items_dev, jac_dev);
   FVR_CHECK_LAUNCH("pn_build_kernel");
   return FVR_OK;
 }
@@ -219,6 +262,21 @@ int fvr_pn_reduce_combine(const fvr_sncc_item* items_dev, int32_t n_poses, const
   hipLaunchKernelGGL(pn_reduce_combine_kernel, dim3(n_poses), dim3(64), 0, reinterpret_cast<hipStream_t>(stream),
                      items_dev, n_poses, partials, jac, t2_ap, t2_lat, moments, loss, grad);
   FVR_CHECK_LAUNCH("pn_reduce_combine_kernel");
+  return FVR_OK;
+}
+
+int fvr_adam_step(double* poses, const double* loss, const double* grad, double* m, double* v, int32_t* t,
+                  int32_t* done, double* best_loss, double* best_pose, int32_t n, const fvr_adam_cfg* cfg_host,
+                  void* stream) {
+  FVR_CHECK_ARG(poses && loss && grad && m && v && t && done && best_loss && best_pose && cfg_host && n > 0,
+                "fvr_adam_step: bad arguments");
+  const fvr_adam_cfg c = *cfg_host;
+  FVR_CHECK_ARG(c.beta1 >= 0.0 && c.beta1 < 1.0 && c.beta2 >= 0.0 && c.beta2 < 1.0 && c.eps >= 0.0 &&
+                    c.cosine_t >= 0,
+                "fvr_adam_step: bad hyper-parameters");
+  hipLaunchKernelGGL(adam_step_kernel, dim3((n + 63) / 64), dim3(64), 0, reinterpret_cast<hipStream_t>(stream), poses,
+                     loss, grad, m, v, t, done, best_loss, best_pose, int(n), c);
+  FVR_CHECK_LAUNCH("adam_step_kernel");
   return FVR_OK;
 }
 

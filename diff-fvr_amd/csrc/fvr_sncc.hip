@@ -1,8 +1,9 @@
 // Fused Fourier central-slice sampling + phase ramp + Spectral-NCC partial sums
 // + pose-adjoint moments, over a batch of (pose, view) work items.
 //
-// One workgroup (4 waves) evaluates a tile of 64 slice columns x 4*ROWS rows of
-// one item; each lane owns one column and walks ROWS rows.  Per sample:
+// One workgroup (4 waves) evaluates a tile of 64 slice columns x 4*R rows of
+// one item (R = 8, 16 or 32 rows per wave, chosen per launch); each lane owns
+// one column and walks R rows.  Per sample:
 //   p      = c + j*ustep + i*vstep            (32.32 fixed point, exact steps)
 //   I, dI  = trilinear value and spatial gradient of the packed spectrum
 //   W      = exp(-2 pi i turns(i,j)) * conj(T(i,j)) [* high-pass weight]
@@ -23,9 +24,6 @@
 
 namespace fvr {
 
-constexpr int kSnccWaves = 4;
-constexpr int kSnccRows = FVR_SNCC_ROWS;  // rows per wave
-static_assert(kSnccWaves * kSnccRows == kSnccTileH, "tile height");
 constexpr float kTwoM32 = 2.3283064365386963e-10f;  // 2^-32
 
 typedef float f4a8 __attribute__((ext_vector_type(4), aligned(8)));
@@ -163,40 +161,59 @@ __device__ __forceinline__ void tile_reduce(const Acc& ac, float cw, int tile, f
 #pragma unroll
   for (int o = 8; o > 0; o >>= 1) part += __shfl_xor(part, o, 16);
   if (s == 0) partials[size_t(tile) * FVR_SNCC_NMOM + qn] = part;
+  __syncthreads();  // red is reused by the workgroup's next tile
 }
 
-template <int DT, bool ADJ, bool HP, bool WS>
-__global__ __launch_bounds__(256) void sncc_kernel(const fvr_layout* __restrict__ layouts,
-                                                   int n_layouts, const fvr_sncc_item* __restrict__ items,
-                                                   int n_items, int n_tiles, int per_xcd,
-                                                   float* __restrict__ partials) {
-  // XCD-aware remap: workgroup b runs on XCD b % 8, so each XCD takes a
-  // contiguous range of tiles (whole items) and keeps their lines in its L2.
-  const int b = blockIdx.x;
-  const int tile = (b & 7) * per_xcd + (b >> 3);
-  if (tile >= n_tiles) return;
-
+// Workgroup slot w -> (item, tile within the item, global tile index).  With a
+// map the host chose the order (entries with an out-of-range item are idle
+// slots); without one, each XCD (slot % 8) takes a contiguous range of tiles
+// and the item is found by binary search over the items' tile_base.
+template <int R>
+__device__ __forceinline__ bool resolve_tile(int w, const int2* __restrict__ map,
+                                             const fvr_sncc_item* __restrict__ items, int n_items, int n_tiles,
+                                             int n_slots, int& item, int& lt, int& tile) {
+  if (map) {
+    const int2 e = map[w];
+    if (uint32_t(e.x) >= uint32_t(n_items)) return false;
+    const fvr_sncc_item& it = items[e.x];
+    const int nt = it.tiles_x * sncc_tiles_y(it.h, it.flags, 4 * R);
+    tile = it.tile_base + e.y;
+    if (uint32_t(e.y) >= uint32_t(nt) || uint32_t(tile) >= uint32_t(n_tiles)) return false;
+    item = e.x;
+    lt = e.y;
+    return true;
+  }
+  tile = (w & 7) * (n_slots >> 3) + (w >> 3);
+  if (tile >= n_tiles) return false;
   int lo = 0, hi = n_items - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if (items[mid].tile_base <= tile) lo = mid;
     else hi = mid - 1;
   }
-  const fvr_sncc_item& it = items[lo];
-  if (uint32_t(it.layout) >= uint32_t(n_layouts)) {  // malformed descriptor: contribute nothing
+  item = lo;
+  lt = tile - items[lo].tile_base;  // a malformed tile_base gives rows / columns outside the item: read nothing
+  return true;
+}
+
+template <int DT, bool ADJ, bool HP, bool WS, int R>
+__device__ __forceinline__ void sncc_tile(const fvr_layout* __restrict__ layouts, int n_layouts,
+                                          const fvr_sncc_item& it, int lt, int tile, float* __restrict__ partials) {
+  static_assert(R % 2 == 0 && R >= 4 && R <= 64, "rows per wave: even, <= 64 (row phasors live in lanes)");
+  if (uint32_t(it.layout) >= uint32_t(n_layouts) || it.tile_h != 4 * R) {  // malformed: contribute nothing
     if (threadIdx.x < FVR_SNCC_NMOM) partials[size_t(tile) * FVR_SNCC_NMOM + threadIdx.x] = 0.f;
     return;
   }
   const fvr_layout& L = layouts[it.layout];
 
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   // item fields -> registers once (uniform)
   const int ih = it.h, iw = it.w, lo_int = it.lo_int;
   const float rw0 = it.rw0, rws = it.rws;
-  const int lt = tile - it.tile_base;
   const int tyi = lt / it.tiles_x, txi = lt - tyi * it.tiles_x;
   const int j = txi * 64 + lane;
-  const int i0 = tyi * kSnccTileH + wave * kSnccRows;
+  const int i0 = tyi * (4 * R) + wave * R;
   const bool colok = uint32_t(j) < uint32_t(iw);  // unsigned: a malformed tile index reads nothing
 
   constexpr uint32_t esz = Raw<DT>::kElem;
@@ -218,7 +235,7 @@ __global__ __launch_bounds__(256) void sncc_kernel(const fvr_layout* __restrict_
   const uint64_t h0 = it.box_hi[0], h1 = it.box_hi[1], h2 = it.box_hi[2];
 
   const cpx colph = phasor(it.ph_col0 + uint32_t(j) * it.ph_col);
-  const cpx rowph = phasor(it.ph_row0 + uint32_t(i0 + lane) * it.ph_row);  // lanes < kSnccRows used
+  const cpx rowph = phasor(it.ph_row0 + uint32_t(i0 + lane) * it.ph_row);  // lanes < R used
   float hx = 0.f, hy = 0.f;
   if (HP) {
     const float x = it.hp_x0 + float(j) * it.hp_dx;
@@ -326,13 +343,12 @@ __global__ __launch_bounds__(256) void sncc_kernel(const fvr_layout* __restrict_
   };
 
   // two samples in flight per lane: the loads of row k+1 are issued before row k is consumed
-  static_assert(kSnccRows % 2 == 0, "rows per wave must be even");
   Smp<DT> sa, sb;
   issue(sa, i0);
   // (the scheduling barriers keep each issue after the consume that frees its
   // registers, so the loop carries no register copies)
 #pragma unroll 1
-  for (int k = 0; k < kSnccRows - 2; k += 2) {
+  for (int k = 0; k < R - 2; k += 2) {
     issue(sb, i0 + k + 1);
     consume(sa, k);
     __builtin_amdgcn_sched_barrier(0);
@@ -340,11 +356,25 @@ __global__ __launch_bounds__(256) void sncc_kernel(const fvr_layout* __restrict_
     consume(sb, k + 1);
     __builtin_amdgcn_sched_barrier(0);
   }
-  issue(sb, i0 + kSnccRows - 1);
-  consume(sa, kSnccRows - 2);
-  consume(sb, kSnccRows - 1);
+  issue(sb, i0 + R - 1);
+  consume(sa, R - 2);
+  consume(sb, R - 1);
 
   tile_reduce(ac, cw, tile, partials);
+}
+
+// Slot loop: one tile per slot, slots walked with stride gridDim.x (a grid of
+// n_slots workgroups takes one slot each).
+template <int DT, bool ADJ, bool HP, bool WS, int R>
+__global__ __launch_bounds__(256) void sncc_kernel(const fvr_layout* __restrict__ layouts, int n_layouts,
+                                                   const fvr_sncc_item* __restrict__ items, int n_items,
+                                                   int n_tiles, const int2* __restrict__ map, int n_slots,
+                                                   float* __restrict__ partials) {
+  for (int w = blockIdx.x; w < n_slots; w += gridDim.x) {
+    int item, lt, tile;
+    if (!resolve_tile<R>(w, map, items, n_items, n_tiles, n_slots, item, lt, tile)) continue;
+    sncc_tile<DT, ADJ, HP, WS, R>(layouts, n_layouts, items[item], lt, tile, partials);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -375,35 +405,26 @@ struct SmpH {
   float phr, phi;  // its phase factor
 };
 
-template <int DT, bool ADJ>
-__global__ __launch_bounds__(256) void sncc_herm_kernel(const fvr_layout* __restrict__ layouts, int n_layouts,
-                                                        const fvr_sncc_item* __restrict__ items, int n_items,
-                                                        int n_tiles, int per_xcd, float* __restrict__ partials) {
-  const int b = blockIdx.x;
-  const int tile = (b & 7) * per_xcd + (b >> 3);
-  if (tile >= n_tiles) return;
-  int lo = 0, hi = n_items - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (items[mid].tile_base <= tile) lo = mid;
-    else hi = mid - 1;
-  }
-  const fvr_sncc_item& it = items[lo];
-  if (uint32_t(it.layout) >= uint32_t(n_layouts)) {
+template <int DT, bool ADJ, int R>
+__device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ layouts, int n_layouts,
+                                               const fvr_sncc_item& it, int lt, int tile,
+                                               float* __restrict__ partials) {
+  static_assert(R >= 1 && R <= 64, "rows per wave: <= 64 (row phasors live in lanes)");
+  if (uint32_t(it.layout) >= uint32_t(n_layouts) || it.tile_h != 4 * R) {
     if (threadIdx.x < FVR_SNCC_NMOM) partials[size_t(tile) * FVR_SNCC_NMOM + threadIdx.x] = 0.f;
     return;
   }
   const fvr_layout& L = layouts[it.layout];
 
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int H = it.h, W = it.w, hh = H >> 1, hw = W >> 1, lo_int = it.lo_int;
-  const int lt = tile - it.tile_base;
   const int tyi = lt / it.tiles_x, txi = lt - tyi * it.tiles_x;
   const int j = txi * 64 + lane;
   const bool colok = uint32_t(j) < uint32_t(W);
-  const int r0 = tyi * kSnccTileH + wave * kSnccRows;  // first representative row (offset from h/2)
+  const int r0 = tyi * (4 * R) + wave * R;  // first representative row (offset from h/2)
   int nreg = hh - r0;
-  nreg = nreg < 0 ? 0 : (nreg > kSnccRows ? kSnccRows : nreg);
+  nreg = nreg < 0 ? 0 : (nreg > R ? R : nreg);
   const bool first = r0 == 0;
   const int nslot = nreg + (first ? 1 : 0);
 
@@ -571,6 +592,18 @@ __global__ __launch_bounds__(256) void sncc_herm_kernel(const fvr_layout* __rest
   tile_reduce(ac, 1.f, tile, partials);
 }
 
+template <int DT, bool ADJ, int R>
+__global__ __launch_bounds__(256) void sncc_herm_kernel(const fvr_layout* __restrict__ layouts, int n_layouts,
+                                                        const fvr_sncc_item* __restrict__ items, int n_items,
+                                                        int n_tiles, const int2* __restrict__ map, int n_slots,
+                                                        float* __restrict__ partials) {
+  for (int w = blockIdx.x; w < n_slots; w += gridDim.x) {
+    int item, lt, tile;
+    if (!resolve_tile<R>(w, map, items, n_items, n_tiles, n_slots, item, lt, tile)) continue;
+    sncc_herm_tile<DT, ADJ, R>(layouts, n_layouts, items[item], lt, tile, partials);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Per-item fixed-order float64 reduction of the tile partials.
 // ---------------------------------------------------------------------------
@@ -578,7 +611,7 @@ __global__ __launch_bounds__(256) void sncc_reduce_kernel(const fvr_sncc_item* _
                                                           const float* __restrict__ partials,
                                                           double* __restrict__ out) {
   const fvr_sncc_item& it = items[blockIdx.x];
-  const int ntile = it.tiles_x * sncc_tiles_y(it.h, it.flags);
+  const int ntile = it.tiles_x * sncc_tiles_y(it.h, it.flags, it.tile_h);
   double acc[FVR_SNCC_NMOM];
 #pragma unroll
   for (int q = 0; q < FVR_SNCC_NMOM; ++q) acc[q] = 0.0;
@@ -600,31 +633,56 @@ __global__ __launch_bounds__(256) void sncc_reduce_kernel(const fvr_sncc_item* _
   }
 }
 
-template <int DT, bool ADJ, bool HP, bool WS>
-static void launch_sncc(dim3 g, hipStream_t st, const fvr_layout* lay, int n_lay, const fvr_sncc_item* items,
-                        int n_items, int n_tiles, int per_xcd, float* partials) {
-  hipLaunchKernelGGL((sncc_kernel<DT, ADJ, HP, WS>), g, dim3(256), 0, st, lay, n_lay, items, n_items, n_tiles,
-                     per_xcd, partials);
+struct Launch {
+  dim3 grid;
+  hipStream_t st;
+  const fvr_layout* lay;
+  int n_lay;
+  const fvr_sncc_item* items;
+  int n_items, n_tiles;
+  const int2* map;
+  int n_slots;
+  float* partials;
+};
+
+template <int DT, bool ADJ, bool HP, bool WS, int R>
+static void launch_sncc(const Launch& a) {
+  hipLaunchKernelGGL((sncc_kernel<DT, ADJ, HP, WS, R>), a.grid, dim3(256), 0, a.st, a.lay, a.n_lay, a.items,
+                     a.n_items, a.n_tiles, a.map, a.n_slots, a.partials);
 }
 
-template <int DT>
-static bool dispatch_flags(uint32_t flags, dim3 g, hipStream_t st, const fvr_layout* lay, int n_lay,
-                           const fvr_sncc_item* items, int n_items, int n_tiles, int per_xcd, float* partials) {
+template <int DT, bool ADJ, int R>
+static void launch_herm(const Launch& a) {
+  hipLaunchKernelGGL((sncc_herm_kernel<DT, ADJ, R>), a.grid, dim3(256), 0, a.st, a.lay, a.n_lay, a.items,
+                     a.n_items, a.n_tiles, a.map, a.n_slots, a.partials);
+}
+
+// Variants: every flag combination at tile_h 64; tile_h 32 / 128 for the batched
+// adjoint forms (convention N plain or folded, convention T with high-pass).
+static bool has_variant(uint32_t flags, int tile_h) {
+  if (flags >= 16u || !sncc_tile_h_ok(tile_h)) return false;
+  const bool adj = flags & FVR_ITEM_ADJOINT, hp = flags & FVR_ITEM_HP, ws = flags & FVR_ITEM_WRITE_SLICE;
+  if (flags & FVR_ITEM_HERM) return !hp && !ws;  // the fold is convention N without slice output
+  return tile_h == 64 || (adj && !ws);
+}
+
+template <int DT, int R>
+static void dispatch_rows(uint32_t flags, const Launch& a) {
   const bool adj = flags & FVR_ITEM_ADJOINT, hp = flags & FVR_ITEM_HP, ws = flags & FVR_ITEM_WRITE_SLICE;
   if (flags & FVR_ITEM_HERM) {
-    if (hp || ws) return false;  // the fold is defined for convention N without slice output
-    if (adj)
-      hipLaunchKernelGGL((sncc_herm_kernel<DT, true>), g, dim3(256), 0, st, lay, n_lay, items, n_items, n_tiles,
-                         per_xcd, partials);
-    else
-      hipLaunchKernelGGL((sncc_herm_kernel<DT, false>), g, dim3(256), 0, st, lay, n_lay, items, n_items, n_tiles,
-                         per_xcd, partials);
-    return true;
+    if (adj) launch_herm<DT, true, R>(a);
+    else launch_herm<DT, false, R>(a);
+    return;
   }
-#define FVR_L(a, h, w)                                                                     \
-  if (adj == a && hp == h && ws == w) {                                                    \
-    launch_sncc<DT, a, h, w>(g, st, lay, n_lay, items, n_items, n_tiles, per_xcd, partials); \
-    return true;                                                                           \
+  if (R != 16) {  // has_variant: adjoint without slice output
+    if (hp) launch_sncc<DT, true, true, false, R>(a);
+    else launch_sncc<DT, true, false, false, R>(a);
+    return;
+  }
+#define FVR_L(x, h, w)                                   \
+  if (adj == x && hp == h && ws == w) {                  \
+    launch_sncc<DT, x, h, w, 16>(a);                     \
+    return;                                              \
   }
   FVR_L(true, false, false)
   FVR_L(false, false, false)
@@ -635,7 +693,13 @@ static bool dispatch_flags(uint32_t flags, dim3 g, hipStream_t st, const fvr_lay
   FVR_L(false, true, true)
   FVR_L(true, true, true)
 #undef FVR_L
-  return false;
+}
+
+template <int DT>
+static void dispatch_dt(uint32_t flags, int tile_h, const Launch& a) {
+  if (tile_h == 32) dispatch_rows<DT, 8>(flags, a);
+  else if (tile_h == 128) dispatch_rows<DT, 32>(flags, a);
+  else dispatch_rows<DT, 16>(flags, a);
 }
 
 }  // namespace fvr
@@ -646,30 +710,50 @@ extern "C" {
 
 int fvr_sncc_tile_rows(void) { return kSnccTileH; }
 
+int fvr_sncc_has_tile_h(uint32_t flags, int32_t tile_h) { return has_variant(flags, tile_h) ? 1 : 0; }
+
+int fvr_sncc_eval_plan(const fvr_layout* layouts_dev, int32_t n_layouts, const fvr_sncc_item* items_dev,
+                       int32_t n_items, int32_t n_tiles, float* partials, uint32_t variant,
+                       const int32_t* tile_map, int32_t n_slots, int32_t grid, void* stream) {
+  FVR_CHECK_ARG(layouts_dev && items_dev && partials, "fvr_sncc_eval: null pointer");
+  FVR_CHECK_ARG(n_items > 0 && n_tiles > 0 && n_layouts > 0, "fvr_sncc_eval: empty batch");
+  const int dt = int(variant & 0xF);
+  const uint32_t flags = (variant >> 4) & 0xF;
+  int tile_h = int(variant >> 8);
+  if (tile_h == 0) tile_h = kSnccTileH;
+  FVR_CHECK_ARG(dt == FVR_C32 || dt == FVR_C16F || dt == FVR_C16B, "fvr_sncc_eval: bad dtype %d", dt);
+  FVR_CHECK_ARG(has_variant(flags, tile_h), "fvr_sncc_eval: no kernel for flags %u with tile_h %d", flags, tile_h);
+  if (tile_map) {
+    FVR_CHECK_ARG(n_slots > 0, "fvr_sncc_eval: empty tile map");
+  } else {
+    FVR_CHECK_ARG(n_slots == 8 * ((n_tiles + 7) / 8), "fvr_sncc_eval: %d slots for %d tiles without a map",
+                  n_slots, n_tiles);
+  }
+  FVR_CHECK_ARG(grid >= 0 && grid <= n_slots, "fvr_sncc_eval: grid %d for %d slots", grid, n_slots);
+  Launch a;
+  a.grid = dim3(grid > 0 ? grid : n_slots);
+  a.st = reinterpret_cast<hipStream_t>(stream);
+  a.lay = layouts_dev;
+  a.n_lay = n_layouts;
+  a.items = items_dev;
+  a.n_items = n_items;
+  a.n_tiles = n_tiles;
+  a.map = reinterpret_cast<const int2*>(tile_map);
+  a.n_slots = n_slots;
+  a.partials = partials;
+  if (dt == FVR_C32) dispatch_dt<FVR_C32>(flags, tile_h, a);
+  else if (dt == FVR_C16F) dispatch_dt<FVR_C16F>(flags, tile_h, a);
+  else dispatch_dt<FVR_C16B>(flags, tile_h, a);
+  FVR_CHECK_LAUNCH("sncc_kernel");
+  return FVR_OK;
+}
+
 int fvr_sncc_eval_v(const fvr_layout* layouts_dev, int32_t n_layouts, const fvr_sncc_item* items_dev, int32_t n_items,
                     int32_t n_tiles, float* partials, uint32_t variant, void* stream) {
   FVR_CHECK_ARG(layouts_dev && items_dev && partials, "fvr_sncc_eval: null pointer");
-  FVR_CHECK_ARG(n_items > 0 && n_tiles > 0 && n_layouts > 0, "fvr_sncc_eval: empty batch");
-  FVR_CHECK_ARG((variant >> 4) < 16u, "fvr_sncc_eval: bad flags %u", variant >> 4);
-  const int dt = int(variant & 0xF);
-  FVR_CHECK_ARG(dt == FVR_C32 || dt == FVR_C16F || dt == FVR_C16B, "fvr_sncc_eval: bad dtype %d", dt);
-  const int per_xcd = (n_tiles + 7) / 8;
-  dim3 g(per_xcd * 8);
-  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  const uint32_t flags = variant >> 4;
-  bool launched;
-  if (dt == FVR_C32)
-    launched = dispatch_flags<FVR_C32>(flags, g, st, layouts_dev, n_layouts, items_dev, n_items, n_tiles, per_xcd,
-                                       partials);
-  else if (dt == FVR_C16F)
-    launched = dispatch_flags<FVR_C16F>(flags, g, st, layouts_dev, n_layouts, items_dev, n_items, n_tiles, per_xcd,
-                                        partials);
-  else
-    launched = dispatch_flags<FVR_C16B>(flags, g, st, layouts_dev, n_layouts, items_dev, n_items, n_tiles, per_xcd,
-                                        partials);
-  FVR_CHECK_ARG(launched, "fvr_sncc_eval: no kernel for variant %u", variant);
-  FVR_CHECK_LAUNCH("sncc_kernel");
-  return FVR_OK;
+  FVR_CHECK_ARG(n_tiles > 0 && n_tiles < (1 << 30), "fvr_sncc_eval: bad tile count %d", n_tiles);
+  return fvr_sncc_eval_plan(layouts_dev, n_layouts, items_dev, n_items, n_tiles, partials, variant, nullptr,
+                            8 * ((n_tiles + 7) / 8), 0, stream);
 }
 
 int fvr_sncc_reduce(const fvr_sncc_item* items_dev, int32_t n_items, const float* partials, double* out,

@@ -134,6 +134,7 @@ def n_item(packed, view: str, yaw, pitch, roll, a, b, target_ptr: int, flags: in
     it["guard_off"] = (int(desc["sdim"][0]) - 2) * int(desc["plane_bytes"])
     it["tile_base"] = tile_base
     it["tiles_x"] = tiles_for(h, w, tile_rows)[0]
+    it["tile_h"] = tile_rows
     it["flags"] = flags
     it["target"] = target_ptr
     it["slice_out"] = slice_out
@@ -191,8 +192,10 @@ def _rot_and_derivs(poses: np.ndarray):
     return R, dR
 
 
-def n_biplanar_items(templates: np.ndarray, pad_shape, poses: np.ndarray, lat_perm, tile_rows: int):
-    """Items (2N, AP then LAT per pose) and Jacobians (2N, 2, 3, 3) for convention N.
+def n_biplanar_items(templates: np.ndarray, pad_shape, poses: np.ndarray, lat_perm, tile_rows: int,
+                     tile0: int = 0):
+    """Items (2N, AP then LAT per pose) and Jacobians (2N, 2, 3, 3) for convention N
+    (tile_base counted from ``tile0``).
 
     ``templates`` are the AP / LAT items whose static fields are kept.  The
     geometry restates diff_fvr.py:56-150 with tx on AP, ty on LAT and tz as the
@@ -228,15 +231,16 @@ def n_biplanar_items(templates: np.ndarray, pad_shape, poses: np.ndarray, lat_pe
         sl["ph_col0"] = turns_fx(uu0 * a / W)
         sl["ph_row"] = turns_fx(b / H)
         sl["ph_row0"] = turns_fx(vv0 * b / H)
-        sl["tile_base"] = np.arange(N) * (tiles[0] + tiles[1]) + (tiles[0] if view == 1 else 0)
+        sl["tile_base"] = tile0 + np.arange(N) * (tiles[0] + tiles[1]) + (tiles[0] if view == 1 else 0)
         # du/dtheta_j = dR_j @ u_base, dv/dtheta_j = dR_j @ v_base, layout axes
         jac[view::2, 0] = np.transpose(dR[:, :, :, ub], (0, 2, 1))[:, perm, :]
         jac[view::2, 1] = np.transpose(dR[:, :, :, 0], (0, 2, 1))[:, perm, :]
     return items, jac, N * (tiles[0] + tiles[1])
 
 
-def validate_items(items: np.ndarray, n_layouts: int, descs: np.ndarray, tile_rows: int) -> int:
-    """Host-side checks of a launch's descriptors; returns the tile count.
+def validate_items(items: np.ndarray, n_layouts: int, descs: np.ndarray, tile_rows: int, tile0: int = 0) -> int:
+    """Host-side checks of a launch's descriptors; returns the tile count
+    (``tile0``: tile_base of the first item, for a segment of a larger launch).
 
     Every field the kernel uses to form an address is checked here, so a
     malformed descriptor raises before anything reaches the device.
@@ -250,6 +254,8 @@ def validate_items(items: np.ndarray, n_layouts: int, descs: np.ndarray, tile_ro
         raise ValueError("item layout index out of range")
     if np.any(items["h"] <= 0) or np.any(items["w"] <= 0):
         raise ValueError("item slice shape must be positive")
+    if tile_rows not in (32, 64, 128) or np.any(items["tile_h"] != tile_rows):
+        raise ValueError(f"item tile_h must be the launch's tile height ({tile_rows}: 32, 64 or 128)")
     if np.any((items["lo_int"] != 0) & (items["lo_int"] != 1)):
         raise ValueError("item lo_int must be 0 or 1")
     if np.any(items["target"] == 0):
@@ -273,7 +279,7 @@ def validate_items(items: np.ndarray, n_layouts: int, descs: np.ndarray, tile_ro
             raise ValueError("Hermitian-fold item box exceeds the wrapped layout's storage")
     if np.any(items["tiles_x"] != (items["w"] + 63) // 64):
         raise ValueError("item tiles_x inconsistent with its width")
-    base = np.concatenate([[0], np.cumsum(ntile)[:-1]])
+    base = int(tile0) + np.concatenate([[0], np.cumsum(ntile)[:-1]])
     if not np.array_equal(items["tile_base"].astype(np.int64), base):
         raise ValueError("item tile_base not the running tile count")
     d = descs[lay]
@@ -368,6 +374,7 @@ def t_template(packed, view: str, out_res: int, target_ptr: int, flags: int, til
     it["lo_int"] = 0
     it["guard_off"] = (int(desc["sdim"][0]) - 2) * int(desc["plane_bytes"])
     it["tiles_x"] = tiles_for(out_res, out_res, tile_rows)[0]
+    it["tile_h"] = tile_rows
     it["flags"] = flags
     it["target"] = target_ptr
     return it

@@ -15,7 +15,7 @@ import torch  # noqa: F401  (must be loaded before the HIP library)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libfvr_hip.so")
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 FVR_C32, FVR_C16F, FVR_C16B = 0, 1, 2
 DTYPE_CODES = {"f32": FVR_C32, "fp16": FVR_C16F, "bf16": FVR_C16B}
@@ -36,13 +36,20 @@ LAYOUT_DTYPE = np.dtype({
 ITEM_DTYPE = np.dtype({
     "names": ["c", "ustep", "vstep", "box_hi", "ph_col0", "ph_col", "ph_row0", "ph_row", "cw0", "cws", "rw0",
               "rws", "hp_inv2s2", "hp_x0", "hp_dx", "hp_y0", "hp_dy", "h", "w", "layout", "lo_int", "guard_off",
-              "tile_base", "tiles_x", "flags", "pad1", "target", "slice_out"],
+              "tile_base", "tiles_x", "flags", "tile_h", "target", "slice_out"],
     "formats": [("<i8", 3), ("<i8", 3), ("<i8", 3), ("<u8", 3), "<u4", "<u4", "<u4", "<u4", "<f4", "<f4", "<f4",
                 "<f4", "<f4", "<f4", "<f4", "<f4", "<f4", "<i4", "<i4", "<i4", "<i4", "<u4", "<i4", "<i4", "<u4",
-                "<u4", "<u8", "<u8"],
+                "<i4", "<u8", "<u8"],
     "offsets": [0, 24, 48, 72, 96, 100, 104, 108, 112, 116, 120, 124, 128, 132, 136, 140, 144, 148, 152, 156, 160,
                 164, 168, 172, 176, 180, 184, 192],
     "itemsize": 200,
+})
+
+ADAM_DTYPE = np.dtype({
+    "names": ["lr", "gclip", "lo", "hi", "beta1", "beta2", "eps", "tol", "cosine_t", "pad0"],
+    "formats": [("<f8", 6), ("<f8", 6), ("<f8", 6), ("<f8", 6), "<f8", "<f8", "<f8", "<f8", "<i4", "<i4"],
+    "offsets": [0, 48, 96, 144, 192, 200, 208, 216, 224, 228],
+    "itemsize": 232,
 })
 
 
@@ -65,9 +72,13 @@ _SIGNATURES = {
     "fvr_layout_init": ([_P, _P, _I, _P, _P], _I),
     "fvr_spec_pack": ([_P, _U64, _I, _I, _I, _I, _P], _I),
     "fvr_sncc_tile_rows": ([], _I),
+    "fvr_sncc_has_tile_h": ([_U, _I], _I),
     "fvr_sncc_eval_v": ([_P, _I, _P, _I, _I, _P, _U, _P], _I),
+    "fvr_sncc_eval_plan": ([_P, _I, _P, _I, _I, _P, _U, _P, _I, _I, _P], _I),
     "fvr_sncc_reduce": ([_P, _I, _P, _P, _P], _I),
-    "fvr_pn_build_items": ([_P, _I, _P, ctypes.c_double, ctypes.c_double, ctypes.c_double, _I, _P, _P, _P], _I),
+    "fvr_pn_build_items": ([_P, _I, _P, ctypes.c_double, ctypes.c_double, ctypes.c_double, _I, _I, _P, _P, _P],
+                           _I),
+    "fvr_adam_step": ([_P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P], _I),
     "fvr_pn_reduce_combine": ([_P, _I, _P, _P, ctypes.c_double, ctypes.c_double, _P, _P, _P, _P], _I),
     "fvr_c64_pair_moments": ([_U64, _U64, _I, _I, _P, _P], _I),
     "fvr_slice_fwd": ([_U64, _I, _I, _I, _I, _I, _P, _P, _I, _P, _P, _I, _F, _F, _F, _U64, _P], _I),
@@ -105,9 +116,9 @@ def lib():
         raise ImportError(f"libfvr_hip.so ABI {ver} != expected {ABI_VERSION}; rebuild the library")
     sizes = (ctypes.c_int32 * 4)()
     handle.fvr_struct_sizes(sizes)
-    if sizes[0] != LAYOUT_DTYPE.itemsize or sizes[1] != ITEM_DTYPE.itemsize:
-        raise ImportError(f"struct size mismatch: C ({sizes[0]}, {sizes[1]}) vs Python "
-                          f"({LAYOUT_DTYPE.itemsize}, {ITEM_DTYPE.itemsize})")
+    if (sizes[0], sizes[1], sizes[2]) != (LAYOUT_DTYPE.itemsize, ITEM_DTYPE.itemsize, ADAM_DTYPE.itemsize):
+        raise ImportError(f"struct size mismatch: C ({sizes[0]}, {sizes[1]}, {sizes[2]}) vs Python "
+                          f"({LAYOUT_DTYPE.itemsize}, {ITEM_DTYPE.itemsize}, {ADAM_DTYPE.itemsize})")
     _lib = handle
     return _lib
 

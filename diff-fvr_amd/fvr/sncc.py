@@ -13,8 +13,10 @@ Moment order (per item): A, E, Mc_im, Mr_im, Mc_g[3], Mr_g[3], Mc_h[3], Mr_h[3]
 
 from __future__ import annotations
 
+import ctypes
 import math
-from typing import Dict, Optional, Tuple
+from dataclasses import dataclass
+from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
 import torch
@@ -26,7 +28,65 @@ TWO_PI = 2.0 * math.pi
 
 
 def tile_rows() -> int:
+    """Default tile height (rows) of the fused kernel."""
     return int(_lib.lib().fvr_sncc_tile_rows())
+
+
+@dataclass(frozen=True)
+class LaunchPlan:
+    """How a batch is laid out on the fused kernel.
+
+    tile_h: rows per tile (32, 64, 128; 4 waves x 8 / 16 / 32 rows, 64 columns).
+    order: "xcd" -- each XCD takes a contiguous range of tiles in item order
+    (pose-major); "target" -- tiles sorted by (view, tile row, tile column, pose)
+    before the contiguous XCD split, so that an XCD's L2 holds one band of one
+    view's target while it walks every pose.
+    persist: workgroups per CU (0: one workgroup per tile slot; k: k * 256
+    persistent workgroups walking the slots).
+    """
+    tile_h: int = 64
+    order: str = "xcd"
+    persist: int = 0
+
+    def variant(self, code: int, flags: int) -> int:
+        return code | (flags << 4) | (self.tile_h << 8)
+
+
+N_CU = 256  # MI355X compute units (8 XCDs x 32)
+
+
+def slot_map(entries: np.ndarray) -> np.ndarray:
+    """(n_slots, 2) int32 map: slot w (XCD w % 8) takes entry (w % 8) * ceil(n/8) + w // 8
+    of ``entries`` ((n, 2) = (item, tile within the item)); the padding slots are idle (-1)."""
+    n = len(entries)
+    per = (n + 7) // 8
+    w = np.arange(8 * per)
+    seq = (w % 8) * per + w // 8
+    out = np.full((8 * per, 2), -1, dtype=np.int32)
+    ok = seq < n
+    out[ok] = entries[seq[ok]]
+    return out
+
+
+def target_major_entries(n_poses: int, tiles: Sequence[Tuple[int, int]], item0: int = 0) -> np.ndarray:
+    """(item, tile) pairs of a biplanar segment in (view, tile row, tile column, pose) order;
+    ``tiles[v]`` = (tiles_x, tiles_y) of view v, items 2 pose + view from ``item0``."""
+    out = []
+    for v, (tx, ty) in enumerate(tiles):
+        lt = np.arange(tx * ty)
+        p = np.arange(n_poses)
+        item = item0 + 2 * p[None, :] + v
+        out.append(np.stack(np.broadcast_arrays(item, lt[:, None]), axis=-1).reshape(-1, 2))
+    return np.concatenate(out).astype(np.int32)
+
+
+def item_major_entries(n_poses: int, tiles: Sequence[Tuple[int, int]], item0: int = 0) -> np.ndarray:
+    out = []
+    for p in range(n_poses):
+        for v, (tx, ty) in enumerate(tiles):
+            lt = np.arange(tx * ty)
+            out.append(np.stack([np.full_like(lt, item0 + 2 * p + v), lt], axis=-1))
+    return np.concatenate(out).astype(np.int32)
 
 
 class _Workspace:
@@ -68,8 +128,8 @@ def run_items(packed: PackedSpectrum, items: np.ndarray, flags: int,
     ``items['tile_base']`` is (re)assigned here.  Every item must carry the
     same ``flags`` (one kernel variant per launch).
     """
-    rows = tile_rows()
     items = _lib.as_struct(items, _lib.ITEM_DTYPE).copy()
+    rows = int(items["tile_h"][0]) if len(items) else tile_rows()
     ntile = item_tiles(items, rows)
     items["tile_base"] = np.concatenate([[0], np.cumsum(ntile)[:-1]]).astype(np.int32)
     if np.any(items["flags"] != flags):
@@ -90,7 +150,7 @@ def launch_items(packed: PackedSpectrum, items: np.ndarray, n_tiles: int, flags:
     dev_items[: n_items * _lib.ITEM_DTYPE.itemsize].copy_(torch.from_numpy(_lib.struct_bytes(items, _lib.ITEM_DTYPE)))
     partials = ws.get("partials", n_tiles * _lib.NMOM, torch.float32)
     st = _lib.stream_ptr(packed.device)
-    variant = packed.code | (flags << 4)
+    variant = packed.code | (flags << 4) | (int(items["tile_h"][0]) << 8)
     _lib.check(L.fvr_sncc_eval_v(_lib.ptr(packed.table), len(packed.descs), _lib.ptr(dev_items), n_items,
                                  n_tiles, _lib.ptr(partials), variant, st), "fvr_sncc_eval")
     _lib.check(L.fvr_sncc_reduce(_lib.ptr(dev_items), n_items, _lib.ptr(partials), _lib.ptr(red), st),
@@ -137,11 +197,12 @@ class BiplanarEvaluator:
     reduction + combine (``fvr_pn_reduce_combine``).  Tensor poses give device
     results (views of the workspace, overwritten by the next call); NumPy poses
     give NumPy results.  ``evaluate_host`` builds the items on the host and
-    combines there (the cross-check of the device set-up).
+    combines there (the cross-check of the device set-up).  ``plan`` (a
+    ``LaunchPlan``) sets the tile height, tile order and persistence.
     """
 
     def __init__(self, packed: PackedSpectrum, target_ap: torch.Tensor, target_lat: torch.Tensor,
-                 adjoint: bool = True, herm: Optional[bool] = None):
+                 adjoint: bool = True, herm: Optional[bool] = None, plan: Optional[LaunchPlan] = None):
         self.packed = packed
         self.device = packed.device
         self.hw = {"ap": _items.slice_dims_n(packed.shape, "ap"), "lateral": _items.slice_dims_n(packed.shape, "lateral")}
@@ -156,25 +217,37 @@ class BiplanarEvaluator:
                             for v in ("ap", "lateral")) if self.herm else None
         self.t2 = np.zeros(2)
         self.set_targets(target_ap, target_lat)
-        self.rows = tile_rows()
         self.flags = (_lib.ITEM_ADJOINT if adjoint else 0) | (_lib.ITEM_HERM if self.herm else 0)
-        tgt = self.folded if self.herm else self.targets
-        self.templates = _lib.as_struct(np.concatenate([
-            _items.n_template(packed, "ap", tgt[0].data_ptr(), self.flags, self.rows),
-            _items.n_template(packed, "lateral", tgt[1].data_ptr(), self.flags, self.rows)]),
-            _lib.ITEM_DTYPE)
         self.lat_perm = packed.layout_for("lateral", self.herm)[1]
         self.lat_swap = int(tuple(self.lat_perm) == (0, 2, 1))
-        # every address-forming field the device builder copies comes from these templates
-        _items.validate_items(self.build(np.zeros((1, 6)))[0], len(self.packed.descs), self.packed.descs, self.rows)
-        ta = _items.tiles_for(*self.hw["ap"], self.rows, self.herm)
-        tl = _items.tiles_for(*self.hw["lateral"], self.rows, self.herm)
-        self.tiles_per_pose = ta[0] * ta[1] + tl[0] * tl[1]
         self.ws = _Workspace(self.device)
         self.last_moments: Optional[np.ndarray] = None
+        self.set_plan(plan or LaunchPlan())
+
+    def set_plan(self, plan: LaunchPlan) -> None:
+        """Choose tile height / order / persistence (rebuilds the item templates)."""
+        if plan.order not in ("xcd", "target") or plan.persist < 0:
+            raise ValueError(f"bad launch plan {plan}")
+        if not _lib.lib().fvr_sncc_has_tile_h(self.flags, plan.tile_h):
+            raise ValueError(f"no fused-kernel variant for flags {self.flags} with tile height {plan.tile_h}")
+        self.plan = plan
+        self.rows = plan.tile_h
+        tgt = self.folded if self.herm else self.targets
+        self.templates = _lib.as_struct(np.concatenate([
+            _items.n_template(self.packed, "ap", tgt[0].data_ptr(), self.flags, self.rows),
+            _items.n_template(self.packed, "lateral", tgt[1].data_ptr(), self.flags, self.rows)]),
+            _lib.ITEM_DTYPE)
+        # every address-forming field the device builder copies comes from these templates
+        _items.validate_items(self.build(np.zeros((1, 6)))[0], len(self.packed.descs), self.packed.descs, self.rows)
+        self.view_tiles = (_items.tiles_for(*self.hw["ap"], self.rows, self.herm),
+                           _items.tiles_for(*self.hw["lateral"], self.rows, self.herm))
+        ta, tl = self.view_tiles
+        self.tiles_per_pose = ta[0] * ta[1] + tl[0] * tl[1]
+        self._maps: Dict[int, Tuple[Optional[torch.Tensor], int, int]] = {}
 
     def set_targets(self, target_ap: torch.Tensor, target_lat: torch.Tensor) -> None:
-        """Copy new AP / LAT target spectra into the evaluator's buffers."""
+        """Copy new AP / LAT target spectra into the evaluator's buffers (a captured graph
+        keeps the target energies it was captured with: capture again after this)."""
         for k, (name, t) in enumerate((("ap", target_ap), ("lateral", target_lat))):
             _lib.require_device(t, "BiplanarEvaluator target")
             if tuple(t.shape) != self.hw[name]:
@@ -196,7 +269,23 @@ class BiplanarEvaluator:
                 ws.get("loss", n, torch.float64),
                 ws.get("grad", 6 * n, torch.float64))
 
-    def build_device(self, poses: torch.Tensor) -> int:
+    def launch_map(self, n: int):
+        """(device tile map or None, n_slots, grid) of the plan for n poses (cached per n)."""
+        got = self._maps.get(n)
+        if got is None:
+            n_tiles = n * self.tiles_per_pose
+            if self.plan.order == "target":
+                m = slot_map(target_major_entries(n, self.view_tiles))
+                dev = torch.from_numpy(m.reshape(-1).copy()).to(self.device)
+                n_slots = len(m)
+            else:
+                dev, n_slots = None, 8 * ((n_tiles + 7) // 8)
+            grid = min(n_slots, 8 * ((self.plan.persist * N_CU + 7) // 8)) if self.plan.persist else 0
+            got = (dev, n_slots, grid)
+            self._maps[n] = got
+        return got
+
+    def build_device(self, poses: torch.Tensor, tile0: int = 0, items=None, jac=None) -> int:
         """Device pose set-up only: items + Jacobians into the workspace; returns N."""
         _lib.require_device(poses, "BiplanarEvaluator.build_device")
         if poses.dtype != torch.float64 or poses.dim() != 2 or poses.shape[1] != 6 or not poses.is_contiguous():
@@ -204,11 +293,13 @@ class BiplanarEvaluator:
         n = int(poses.shape[0])
         if n < 1:
             raise ValueError("empty pose batch")
-        _, items, jac = self.prepare(n)[:3]
+        if items is None:
+            _, items, jac = self.prepare(n)[:3]
+            items, jac = _lib.ptr(items), _lib.ptr(jac)
         pz, py, px = self.packed.shape
         _lib.check(_lib.lib().fvr_pn_build_items(_lib.ptr(poses), n, self.templates.ctypes.data, float(pz),
-                                                 float(py), float(px), self.lat_swap, _lib.ptr(items),
-                                                 _lib.ptr(jac), _lib.stream_ptr(self.device)),
+                                                 float(py), float(px), self.lat_swap, int(tile0), items, jac,
+                                                 _lib.stream_ptr(self.device)),
                    "fvr_pn_build_items")
         return n
 
@@ -218,11 +309,13 @@ class BiplanarEvaluator:
         _, items, jac, partials, moments, loss, grad = self.prepare(n)
         L = _lib.lib()
         st = _lib.stream_ptr(self.device)
+        tmap, n_slots, grid = self.launch_map(n)
         if kernel_events is not None:
             kernel_events[0].record()
-        _lib.check(L.fvr_sncc_eval_v(_lib.ptr(self.packed.table), len(self.packed.descs), _lib.ptr(items), 2 * n,
-                                     n * self.tiles_per_pose, _lib.ptr(partials),
-                                     self.packed.code | (self.flags << 4), st), "fvr_sncc_eval")
+        _lib.check(L.fvr_sncc_eval_plan(_lib.ptr(self.packed.table), len(self.packed.descs), _lib.ptr(items), 2 * n,
+                                        n * self.tiles_per_pose, _lib.ptr(partials),
+                                        self.plan.variant(self.packed.code, self.flags),
+                                        None if tmap is None else _lib.ptr(tmap), n_slots, grid, st), "fvr_sncc_eval")
         if kernel_events is not None:
             kernel_events[1].record()
         _lib.check(L.fvr_pn_reduce_combine(_lib.ptr(items), n, _lib.ptr(partials), _lib.ptr(jac), float(self.t2[0]),
@@ -272,6 +365,119 @@ class BiplanarEvaluator:
         launch_items(self.packed, items, n_tiles, self.flags, hws, red)
         mom = red.cpu().numpy()
         return combine_biplanar(mom, jac, self.t2, self.hw)
+
+
+def _off(t: torch.Tensor, elems: int) -> ctypes.c_void_p:
+    """Device address of element ``elems`` of a contiguous tensor."""
+    return ctypes.c_void_p(t.data_ptr() + elems * t.element_size())
+
+
+class RaggedBiplanarEvaluator:
+    """Several volumes (each with its own targets and pose count) in ONE fused launch.
+
+    ``segments``: BiplanarEvaluator per volume (same device, storage type and
+    kernel flags); ``counts``: poses per volume.  Poses are one (sum counts, 6)
+    float64 device tensor, volume k owning rows [offsets[k], offsets[k] + counts[k]).
+    A step is: one device pose set-up per volume (its templates re-indexed into a
+    combined layout table and its tiles counted from the running total), one
+    fused launch over every item of every volume (ragged item sizes: the kernel
+    maps each tile to its item), and one reduce/combine per volume.  No host
+    sync and no allocation after ``prepare`` (graph-capturable).
+    """
+
+    def __init__(self, segments: Sequence[BiplanarEvaluator], counts: Sequence[int],
+                 plan: Optional[LaunchPlan] = None):
+        if len(segments) == 0 or len(segments) != len(counts) or any(int(c) < 1 for c in counts):
+            raise ValueError("one positive pose count per segment")
+        self.segments = list(segments)
+        self.counts = [int(c) for c in counts]
+        self.offsets = np.concatenate([[0], np.cumsum(self.counts)[:-1]]).astype(int).tolist()
+        self.n = int(sum(self.counts))
+        s0 = self.segments[0]
+        self.device = s0.device
+        for s in self.segments:
+            if s.device != self.device or s.packed.code != s0.packed.code or s.flags != s0.flags:
+                raise ValueError("segments must share the device, the storage type and the kernel flags")
+        self.flags = s0.flags
+        self.code = s0.packed.code
+        self.descs = np.concatenate([s.packed.descs for s in self.segments]).astype(_lib.LAYOUT_DTYPE)
+        self.table = _lib.struct_to_device(self.descs, self.device, _lib.LAYOUT_DTYPE)
+        self.lay0 = np.concatenate([[0], np.cumsum([len(s.packed.descs) for s in self.segments])[:-1]]).astype(int)
+        self.ws = _Workspace(self.device)
+        self.set_plan(plan or s0.plan)
+
+    def set_plan(self, plan: LaunchPlan) -> None:
+        for s in self.segments:
+            s.set_plan(plan)
+        self.plan = plan
+        self.rows = plan.tile_h
+        self.templates = []
+        self.tile0 = []
+        t = 0
+        for k, s in enumerate(self.segments):
+            tm = s.templates.copy()
+            tm["layout"] += self.lay0[k]
+            # the re-indexed templates against the combined table (the device builder copies them)
+            probe, _, _ = _items.n_biplanar_items(tm, s.packed.shape, np.zeros((1, 6)), s.lat_perm, self.rows, t)
+            _items.validate_items(probe, len(self.descs), self.descs, self.rows, tile0=t)
+            self.templates.append(_lib.as_struct(tm, _lib.ITEM_DTYPE))
+            self.tile0.append(t)
+            t += self.counts[k] * s.tiles_per_pose
+        self.n_tiles = t
+        if t >= 2 ** 31:
+            raise ValueError("ragged batch exceeds the 32-bit tile index")
+        if plan.order == "target":
+            ent = np.concatenate([target_major_entries(c, s.view_tiles, 2 * o)
+                                  for s, c, o in zip(self.segments, self.counts, self.offsets)])
+            m = slot_map(ent)
+            self.map = torch.from_numpy(m.reshape(-1).copy()).to(self.device)
+            self.n_slots = len(m)
+        else:
+            self.map, self.n_slots = None, 8 * ((t + 7) // 8)
+        self.grid = min(self.n_slots, 8 * ((plan.persist * N_CU + 7) // 8)) if plan.persist else 0
+        self.prepare()
+
+    def prepare(self):
+        ws, n = self.ws, self.n
+        return (ws.get("items", 2 * n * _lib.ITEM_DTYPE.itemsize, torch.uint8),
+                ws.get("jac", 2 * n * 18, torch.float64),
+                ws.get("partials", self.n_tiles * _lib.NMOM, torch.float32),
+                ws.get("loss", n, torch.float64),
+                ws.get("grad", 6 * n, torch.float64))
+
+    def evaluate_device(self, poses: torch.Tensor, kernel_events=None):
+        """poses (sum counts, 6) contiguous float64 device tensor -> (loss (N,), grad (N, 6)) device views."""
+        _lib.require_device(poses, "RaggedBiplanarEvaluator")
+        if poses.dtype != torch.float64 or tuple(poses.shape) != (self.n, 6) or not poses.is_contiguous():
+            raise ValueError(f"poses must be a contiguous ({self.n}, 6) float64 tensor")
+        items, jac, partials, loss, grad = self.prepare()
+        L = _lib.lib()
+        st = _lib.stream_ptr(self.device)
+        isz = _lib.ITEM_DTYPE.itemsize
+        for k, s in enumerate(self.segments):
+            o, c = self.offsets[k], self.counts[k]
+            pz, py, px = s.packed.shape
+            _lib.check(L.fvr_pn_build_items(_off(poses, 6 * o), c, self.templates[k].ctypes.data, float(pz),
+                                            float(py), float(px), s.lat_swap, self.tile0[k], _off(items, 2 * o * isz),
+                                            _off(jac, 2 * o * 18), st), "fvr_pn_build_items")
+        if kernel_events is not None:
+            kernel_events[0].record()
+        _lib.check(L.fvr_sncc_eval_plan(_lib.ptr(self.table), len(self.descs), _lib.ptr(items), 2 * self.n,
+                                        self.n_tiles, _lib.ptr(partials), self.plan.variant(self.code, self.flags),
+                                        None if self.map is None else _lib.ptr(self.map), self.n_slots, self.grid,
+                                        st), "fvr_sncc_eval")
+        if kernel_events is not None:
+            kernel_events[1].record()
+        for k, s in enumerate(self.segments):
+            o, c = self.offsets[k], self.counts[k]
+            _lib.check(L.fvr_pn_reduce_combine(_off(items, 2 * o * isz), c, _lib.ptr(partials), _off(jac, 2 * o * 18),
+                                               float(s.t2[0]), float(s.t2[1]), None, _off(loss, o),
+                                               _off(grad, 6 * o), st), "fvr_pn_reduce_combine")
+        return loss[: self.n], grad[: 6 * self.n].view(self.n, 6)
+
+    def device_items(self) -> np.ndarray:
+        raw = self.ws.bufs["items"][: 2 * self.n * _lib.ITEM_DTYPE.itemsize].cpu().numpy()
+        return raw.view(_lib.ITEM_DTYPE).copy()
 
 
 def combine_biplanar(mom: np.ndarray, jac: np.ndarray, t2, hw):

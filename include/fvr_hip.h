@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define FVR_ABI_VERSION 1
+#define FVR_ABI_VERSION 2
 
 typedef enum fvr_status {
   FVR_OK = 0,
@@ -133,22 +133,41 @@ typedef struct fvr_sncc_item {
   int32_t tile_base;    /* global index of this item's first tile */
   int32_t tiles_x;      /* tiles per slice row */
   uint32_t flags;
-  uint32_t pad1;        /* explicit: keeps the layout free of implicit padding */
+  int32_t tile_h;       /* rows per tile: 32, 64 or 128 (4 waves x 8, 16 or 32 rows); one value per launch */
   uint64_t target;      /* device address of the complex64 target (h x w, row-major) */
   uint64_t slice_out;   /* device address for S (FVR_ITEM_WRITE_SLICE), else 0 */
 } fvr_sncc_item;
 
 #define FVR_SNCC_NMOM 16 /* partial sums per tile */
 
-/* Tile geometry of fvr_sncc_eval: 64 columns x fvr_sncc_tile_rows() rows. */
+/* Default tile geometry of fvr_sncc_eval: 64 columns x fvr_sncc_tile_rows() rows. */
 int fvr_sncc_tile_rows(void);
+/* 1 when the fused kernel has a variant for (flags, tile_h): every flag set with
+ * tile_h = 64; 32 and 128 for the batched forms (adjoint, without slice output). */
+int fvr_sncc_has_tile_h(uint32_t flags, int32_t tile_h);
 
 /* Launch the fused kernel over n_tiles tiles (the sum of items' tile counts).
- * variant = dtype | (flags << 4); every item of one launch shares it.  Items
- * whose layout index is not below n_layouts contribute zero partials.
- * partials: float[n_tiles][FVR_SNCC_NMOM] (caller buffer). */
+ * variant = dtype | (flags << 4) | (tile_h << 8) (tile_h 0 = 64); every item of
+ * one launch shares flags and tile_h.  Items whose layout index is not below
+ * n_layouts, or whose tile_h differs, contribute zero partials.
+ * partials: float[n_tiles][FVR_SNCC_NMOM] (caller buffer).
+ * Tile order: workgroup slot w (XCD w % 8) takes tile (w % 8) * ceil(n_tiles/8) + w / 8,
+ * i.e. each XCD a contiguous range of tiles.  Replaces the per-slice extraction
+ * and NCC of diff_fvr.py:106-195 / gradients.py:76-135 for a batch of items. */
 int fvr_sncc_eval_v(const fvr_layout* layouts_dev, int32_t n_layouts, const fvr_sncc_item* items_dev,
                     int32_t n_items, int32_t n_tiles, float* partials, uint32_t variant, void* stream);
+
+/* fvr_sncc_eval_v with a launch plan (no counterpart in the reference: the
+ * tuned batched launch of the ragged multistart).  tile_map (device, may be
+ * NULL): int32 pairs (item, tile within the item) per workgroup slot, n_slots
+ * entries; entries whose item is out of range are idle slots.  Without a map,
+ * n_slots must be 8 * ceil(n_tiles / 8) (the XCD-contiguous order above).
+ * grid: workgroups launched (0 = n_slots); a smaller grid walks the slots with
+ * stride grid (persistent workgroups; keep grid a multiple of 8 so that a
+ * workgroup stays on its XCD's slots). */
+int fvr_sncc_eval_plan(const fvr_layout* layouts_dev, int32_t n_layouts, const fvr_sncc_item* items_dev,
+                       int32_t n_items, int32_t n_tiles, float* partials, uint32_t variant,
+                       const int32_t* tile_map, int32_t n_slots, int32_t grid, void* stream);
 
 /* Per-item reduction of the tile partials in fixed order (float64):
  * out[item][FVR_SNCC_NMOM] as double, ordered
@@ -160,11 +179,12 @@ int fvr_sncc_reduce(const fvr_sncc_item* items_dev, int32_t n_items, const float
  * LAT) from poses[n][6] (yaw, pitch, roll in degrees; tx, ty, tz in voxels),
  * copying the static fields of tmpl_host[2] (host structs: AP, LAT, already
  * validated) and writing jac[2n][2][3][3] (du/dtheta, dv/dtheta per degree in
- * layout axes).  lat_swap = 1 when the LAT template uses the (z, x, y) copy.
+ * layout axes).  lat_swap = 1 when the LAT template uses the (z, x, y) copy;
+ * tile0 = tile_base of the first item (several volumes share one launch).
  * Replaces the per-pose set-up of diff_fvr.py:56-150 / gradients.py:89-95. */
 int fvr_pn_build_items(const double* poses, int32_t n_poses, const fvr_sncc_item* tmpl_host, double pz,
-                       double py, double px, int32_t lat_swap, fvr_sncc_item* items_dev, double* jac_dev,
-                       void* stream);
+                       double py, double px, int32_t lat_swap, int32_t tile0, fvr_sncc_item* items_dev,
+                       double* jac_dev, void* stream);
 
 /* Per-pose fixed-order float64 reduction of the fused kernel's tile partials
  * and the biplanar loss L = 1/2(1-NCC_AP) + 1/2(1-NCC_LAT) with its exact
@@ -173,6 +193,29 @@ int fvr_pn_build_items(const double* poses, int32_t n_poses, const fvr_sncc_item
 int fvr_pn_reduce_combine(const fvr_sncc_item* items_dev, int32_t n_poses, const float* partials,
                           const double* jac, double t2_ap, double t2_lat, double* moments, double* loss,
                           double* grad, void* stream);
+
+/* Batched Adam over 6-DoF poses on the device, one thread per pose, so that a
+ * registration loop is a fixed sequence of launches (graph-capturable).  Per
+ * pose, with loss / grad evaluated at the current pose (gradient_optimizer.py:45-83;
+ * experiments/exp30_gpu_multistart_registration.py:140-183 for the gradient
+ * clamp and the cosine schedule):
+ *   if done: nothing;  t += 1;  if loss < best_loss: best = (loss, pose)
+ *   g = clamp(grad, -gclip, gclip) (gclip <= 0: no clamp)
+ *   m = b1 m + (1-b1) g;  v = b2 v + (1-b2) g^2
+ *   f = cosine_T > 0 ? 0.1 + 0.45 (1 + cos(pi min(t-1, T) / T)) : 1
+ *   step = f lr (m / (1 - b1^t)) / (sqrt(v / (1 - b2^t)) + eps)
+ *   pose = clip(pose - step, lo, hi);  done = max|step| < tol. */
+typedef struct fvr_adam_cfg {
+  double lr[6];
+  double gclip[6];
+  double lo[6], hi[6];
+  double beta1, beta2, eps, tol;
+  int32_t cosine_t;
+  int32_t pad0;
+} fvr_adam_cfg;
+int fvr_adam_step(double* poses, const double* loss, const double* grad, double* m, double* v,
+                  int32_t* t, int32_t* done, double* best_loss, double* best_pose, int32_t n,
+                  const fvr_adam_cfg* cfg_host, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Generic fourier_slice_3d (grid_sample semantics, planar (B,2,D,H,W)).      */
@@ -243,7 +286,7 @@ int fvr_pou_sum_blocks(const float* part, int32_t B, int32_t N, int32_t C, int32
 /* ------------------------------------------------------------------------ */
 int fvr_abi_version(void);
 const char* fvr_last_error(void);
-/* out[0] = sizeof(fvr_layout), out[1] = sizeof(fvr_sncc_item), out[2..3] = 0 */
+/* out[0] = sizeof(fvr_layout), out[1] = sizeof(fvr_sncc_item), out[2] = sizeof(fvr_adam_cfg), out[3] = 0 */
 int fvr_struct_sizes(int32_t out[4]);
 /* out[5] = [Re sum a conj(b), sum |a|^2, sum |b|^2, sum uu Im(a conj b), sum vv Im(a conj b)]
  * over two complex64 (h x w) row-major device slices, uu = j - w/2, vv = i - h/2

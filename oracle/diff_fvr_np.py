@@ -38,7 +38,11 @@ How each function is pinned (see ``tests/test_oracle_n.py``):
 * ``biplanar_loss_and_grad_analytic`` -- FP64 central differences (h=1e-3 deg /
                                   voxel) and float64 torch autograd of an
                                   independent stencil restatement.
-* ``adam_register``            -- the Adam closed form on a quadratic.
+* ``adam_register``            -- the Adam closed form on a quadratic; with the
+                                  multistart options (betas, gradient clamp,
+                                  cosine factor, no pose clip) it equals
+                                  ``torch.optim.Adam`` + ``LambdaLR``, the calls
+                                  of ``experiments/exp30_gpu_multistart_registration.py:140-183``.
 * ``compute_mtre_6dof``        -- the reference's closed forms 6.0, 3.0, 16.0
                                   (``tests/test_physical_units.py:52-59``).
 * ``prepare_spectrum``         -- a target rendered at pose p matches the slice
@@ -439,13 +443,19 @@ def prepare_spectrum(img, slice_h, slice_w):
 # ---------------------------------------------------------------------------
 # a8: NumPy Adam (gradient_optimizer.py:24-90)
 # ---------------------------------------------------------------------------
-def adam_register(loss_grad_fn, init_pose, lr_rot=0.25, lr_trans=0.35, max_iter=40, tol=1e-4) -> Dict:
-    """Adam with decoupled lr, clipping to +-20 deg / +-25, early stop on max|step| < tol."""
+def adam_register(loss_grad_fn, init_pose, lr_rot=0.25, lr_trans=0.35, max_iter=40, tol=1e-4, beta1=0.9,
+                  beta2=0.999, eps=1e-8, limits=(20.0, 25.0), gclip=None, cosine_t=0) -> Dict:
+    """Adam with decoupled lr, clipping to +-20 deg / +-25, early stop on max|step| < tol
+    (gradient_optimizer.py:24-90).  Multistart options (exp30:140-183): ``gclip`` =
+    (rot, trans) gradient clamp, ``cosine_t`` > 0 scales lr by 0.1 + 0.9 (1 + cos(pi s/T))/2
+    at scheduler step s = min(step - 1, T); ``limits=None`` leaves the pose unclipped."""
     pose = np.array(init_pose, dtype=np.float64)
     m = np.zeros(6)
     v = np.zeros(6)
-    b1, b2, eps = 0.9, 0.999, 1e-8
+    b1, b2 = beta1, beta2
     lr = np.array([lr_rot] * 3 + [lr_trans] * 3)
+    lim = None if limits is None else np.array([limits[0]] * 3 + [limits[1]] * 3)
+    gc = None if gclip is None else np.array([gclip[0]] * 3 + [gclip[1]] * 3)
     best_loss, best_pose = float("inf"), pose.copy()
     traj = []
     step = 0
@@ -454,12 +464,17 @@ def adam_register(loss_grad_fn, init_pose, lr_rot=0.25, lr_trans=0.35, max_iter=
         traj.append((step, loss, pose.copy()))
         if loss < best_loss:
             best_loss, best_pose = loss, pose.copy()
+        if gc is not None:
+            grad = np.clip(grad, -gc, gc)
         m = b1 * m + (1 - b1) * grad
         v = b2 * v + (1 - b2) * grad ** 2
-        upd = lr * ((m / (1 - b1 ** step)) / (np.sqrt(v / (1 - b2 ** step)) + eps))
+        f = 1.0
+        if cosine_t > 0:
+            f = 0.1 + 0.9 * 0.5 * (1.0 + math.cos(math.pi * min(step - 1, cosine_t) / cosine_t))
+        upd = f * lr * ((m / (1 - b1 ** step)) / (np.sqrt(v / (1 - b2 ** step)) + eps))
         pose = pose - upd
-        pose[:3] = np.clip(pose[:3], -20.0, 20.0)
-        pose[3:] = np.clip(pose[3:], -25.0, 25.0)
+        if lim is not None:
+            pose = np.clip(pose, -lim, lim)
         if np.max(np.abs(upd)) < tol:
             break
     return {"success": bool(best_loss < 0.05), "final_pose": best_pose, "final_loss": float(best_loss),

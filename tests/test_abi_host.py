@@ -41,6 +41,7 @@ def test_struct_sizes_match_numpy_mirrors():
     assert L.fvr_struct_sizes(out) == 0
     assert out[0] == _lib.LAYOUT_DTYPE.itemsize == 80
     assert out[1] == _lib.ITEM_DTYPE.itemsize == 200
+    assert out[2] == _lib.ADAM_DTYPE.itemsize == 232
 
 
 def test_layout_init_and_error_reporting():
@@ -69,6 +70,59 @@ def test_entry_points_reject_bad_arguments_without_a_gpu():
     assert L.fvr_sncc_eval_v(None, 0, None, 0, 0, None, 0, None) == 1
     assert b"null" in L.fvr_last_error()
     assert L.fvr_slice_fwd(0, 0, 1, 1, 1, 1, None, None, 1, None, None, 1, 0.0, 0.0, 0.0, 0, None) == 1
+    # a plan with a tile height the library has no kernel for, and a map-less slot count that
+    # does not cover the tiles, are refused before any launch (fake non-null pointers)
+    p = ctypes.c_void_p(16)
+    assert L.fvr_sncc_eval_plan(p, 1, p, 1, 8, p, 0 | (9 << 4) | (96 << 8), None, 8, 0, None) == 1
+    assert b"tile_h 96" in L.fvr_last_error()
+    assert L.fvr_sncc_eval_plan(p, 1, p, 1, 9, p, 0 | (9 << 4) | (64 << 8), None, 8, 0, None) == 1
+    assert b"slots" in L.fvr_last_error()
+    assert L.fvr_sncc_eval_plan(p, 1, p, 1, 8, p, 0 | (9 << 4) | (64 << 8), None, 8, 9, None) == 1
+    assert b"grid" in L.fvr_last_error()
+    cfg = np.zeros(1, dtype=_lib.ADAM_DTYPE)
+    assert L.fvr_adam_step(None, None, None, None, None, None, None, None, None, 1, cfg.ctypes.data, None) == 1
+    cfg["beta1"] = 1.0
+    assert L.fvr_adam_step(p, p, p, p, p, p, p, p, p, 1, cfg.ctypes.data, None) == 1
+    assert b"hyper" in L.fvr_last_error()
+
+
+def test_kernel_variant_table():
+    """Tile heights 32 / 128 exist for the batched adjoint forms; every flag set has 64."""
+    from fvr import _lib
+    L = _lib.lib()
+    A, HP, WS, HERM = _lib.ITEM_ADJOINT, _lib.ITEM_HP, _lib.ITEM_WRITE_SLICE, _lib.ITEM_HERM
+    for f in range(16):
+        folded_ok = not (f & HERM and f & (HP | WS))
+        assert L.fvr_sncc_has_tile_h(f, 64) == int(folded_ok)
+        for th in (32, 128):
+            want = folded_ok and (bool(f & HERM) or (bool(f & A) and not f & WS))
+            assert L.fvr_sncc_has_tile_h(f, th) == int(want), (f, th)
+        assert L.fvr_sncc_has_tile_h(f, 48) == 0
+    assert L.fvr_sncc_has_tile_h(16, 64) == 0
+
+
+def test_tile_maps_cover_every_tile_once_and_split_contiguously_over_xcds():
+    from fvr.sncc import item_major_entries, slot_map, target_major_entries
+    tiles = [(8, 4), (8, 4)]
+    for n in (1, 5, 64):
+        for ent in (target_major_entries(n, tiles, 6), item_major_entries(n, tiles, 6)):
+            assert len(ent) == n * 64
+            got = {tuple(e) for e in ent}
+            assert got == {(6 + 2 * p + v, t) for p in range(n) for v in range(2) for t in range(32)}
+            m = slot_map(ent)
+            assert len(m) % 8 == 0 and len(m) - len(ent) < 8
+            live = m[m[:, 0] >= 0]
+            assert sorted(map(tuple, live)) == sorted(map(tuple, ent))
+            per = len(m) // 8
+            for x in range(8):  # XCD x's slots hold one contiguous run of the entry sequence
+                run = m[x::8]
+                run = run[run[:, 0] >= 0]
+                np.testing.assert_array_equal(run, ent[x * per: x * per + len(run)])
+    # target-major order: one XCD's run of a 64-pose batch is one (view, tile row) band
+    m = slot_map(target_major_entries(64, tiles))
+    for x in range(8):
+        run = m[x::8]
+        assert len({int(i) % 2 for i in run[:, 0]}) == 1 and len({int(t) // 8 for t in run[:, 1]}) == 1
 
 
 def test_product_path_refuses_cpu():
@@ -104,7 +158,7 @@ def test_item_builder_identity_geometry():
             return d
 
     it = _items.n_item(FakePacked(), "ap", 0, 0, 0, 0, 0, 0, 0, 0, 64)[0]
-    assert (it["h"], it["w"]) == (16, 24)
+    assert (it["h"], it["w"]) == (16, 24) and it["tile_h"] == 64
     # c = (pz/2, py/2, px/2) + uu0*u + vv0*v with u = x, v = z, uu0 = -12, vv0 = -8
     # (every coordinate carries the +SNAP_FX lattice bias, and the box edge with it)
     np.testing.assert_array_equal(it["c"], (np.array([0, 10, 0]) << 32) + _items.SNAP_FX)

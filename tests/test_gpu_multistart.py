@@ -1,0 +1,189 @@
+"""GPU tests of the batched multistart machinery: launch plans of the fused kernel,
+several volumes in one launch, the device Adam step, the graph-captured loop
+and the autotuner.
+
+Tolerances: for one tile height, changing the tile order or the persistence
+changes only which workgroup computes a tile, so results are bit-identical;
+another tile height regroups the float32 tile sums (loss within 1e-5 relative,
+gradient within 1e-4 of max|grad|).  A volume inside a ragged launch computes
+exactly the tiles it computes alone: bit-identical.  The Adam kernel against
+the FP64 oracle on the same loss / gradient sequence: 1e-12.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import diff_fvr_np as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _renderer(n, seed, storage="fp16"):
+    from fvr import DifferentiableFourierVolumeRenderer
+    from fvr.phantom import ellipsoid_phantom
+    vol = ellipsoid_phantom(n, n_ellipsoids=6, seed=seed).astype(np.float64)
+    return DifferentiableFourierVolumeRenderer(vol, storage=storage, device=torch.device("cuda", 0))
+
+
+def _targets(r, tp):
+    a = torch.from_numpy(r.extract_frequency_slice(*tp[:3], tx=tp[3], ty=tp[5], view="ap")).to("cuda")
+    b = torch.from_numpy(r.extract_frequency_slice(*tp[:3], tx=tp[4], ty=tp[5], view="lateral")).to("cuda")
+    return a.to(torch.complex64), b.to(torch.complex64)
+
+
+@pytest.fixture(scope="module")
+def vols():
+    out = []
+    for n, seed in ((32, 11), (48, 12), (40, 13)):
+        r = _renderer(n, seed)
+        tp = np.random.default_rng(seed).uniform(-4, 4, 6)
+        out.append((r, _targets(r, tp)))
+    return out
+
+
+def _poses(n, seed):
+    from fvr.phantom import random_poses
+    return torch.from_numpy(random_poses(n, 12.0, 8.0, seed)).to("cuda")
+
+
+@pytest.mark.parametrize("herm", [True, False], ids=["fold", "full"])
+def test_launch_plans_agree(vols, herm):
+    from fvr.multistart import candidate_plans
+    from fvr.sncc import BiplanarEvaluator, LaunchPlan
+    r, (ta, tl) = vols[1]
+    ev = BiplanarEvaluator(r.packed, ta, tl, adjoint=True, herm=herm)
+    P = _poses(19, 3)
+    ref = {}
+    for th in (32, 64, 128):
+        if not __import__("fvr")._lib.lib().fvr_sncc_has_tile_h(ev.flags, th):
+            continue
+        ev.set_plan(LaunchPlan(th))
+        L, G = ev.evaluate_device(P)
+        ref[th] = (L.cpu().numpy().copy(), G.cpu().numpy().copy())
+    assert set(ref) == {32, 64, 128}
+    for th in (32, 128):
+        np.testing.assert_allclose(ref[th][0], ref[64][0], rtol=1e-5)
+        assert np.max(np.abs(ref[th][1] - ref[64][1])) <= 1e-4 * np.max(np.abs(ref[64][1]))
+    plans = candidate_plans(ev.flags)
+    assert len(plans) == 18
+    for plan in plans:
+        ev.set_plan(plan)
+        L, G = ev.evaluate_device(P)
+        np.testing.assert_array_equal(L.cpu().numpy(), ref[plan.tile_h][0], err_msg=str(plan))
+        np.testing.assert_array_equal(G.cpu().numpy(), ref[plan.tile_h][1], err_msg=str(plan))
+
+
+@pytest.mark.parametrize("order", ["xcd", "target"])
+def test_ragged_launch_equals_separate_launches(vols, order):
+    from fvr.sncc import BiplanarEvaluator, LaunchPlan, RaggedBiplanarEvaluator
+    plan = LaunchPlan(64, order, 2 if order == "target" else 0)
+    counts = (5, 7, 3)
+    segs = [BiplanarEvaluator(r.packed, ta, tl, adjoint=True, plan=plan) for r, (ta, tl) in vols]
+    rag = RaggedBiplanarEvaluator(segs, counts, plan)
+    P = _poses(sum(counts), 7)
+    L, G = rag.evaluate_device(P)
+    L, G = L.cpu().numpy().copy(), G.cpu().numpy().copy()
+    off = 0
+    for s, c in zip(segs, counts):
+        l1, g1 = s.evaluate_device(P[off: off + c].contiguous())
+        np.testing.assert_array_equal(L[off: off + c], l1.cpu().numpy())
+        np.testing.assert_array_equal(G[off: off + c], g1.cpu().numpy())
+        off += c
+    # the device-built items of the ragged launch pass the host validator against the combined table
+    from fvr import _items
+    items = rag.device_items()
+    assert _items.validate_items(items, len(rag.descs), rag.descs, rag.rows) == rag.n_tiles
+
+
+def _adam_sequences(n, steps, seed):
+    rng = np.random.default_rng(seed)
+    losses = rng.uniform(0.1, 1.0, (steps, n))
+    losses[3] = 0.01  # a new best at step 4
+    grads = rng.standard_normal((steps, n, 6)) * np.array([0.3] * 3 + [3.0] * 3)
+    grads[:, 0] *= 1e-12  # pose 0: |grad| << eps, so its first step is below tol (early stop)
+    return losses, grads
+
+
+@pytest.mark.parametrize("opts", ["reference", "exp30"])
+def test_device_adam_equals_oracle(opts):
+    from fvr.multistart import DeviceAdam
+    n, steps = 5, 25
+    losses, grads = _adam_sequences(n, steps, 3 if opts == "reference" else 4)
+    p0 = np.random.default_rng(9).uniform(-15, 15, (n, 6))
+    if opts == "reference":
+        kw = dict(tol=1e-4)
+        okw = dict(tol=1e-4)
+    else:
+        kw = dict(lr_rot=0.02, lr_trans=2.0, beta1=0.8, beta2=0.99, tol=0.0, rot_limit=1e9, trans_limit=1e9,
+                  gclip_rot=0.25, gclip_trans=2.5, cosine_t=20)
+        okw = dict(lr_rot=0.02, lr_trans=2.0, beta1=0.8, beta2=0.99, tol=0.0, limits=None, gclip=(0.25, 2.5),
+                   cosine_t=20)
+    dev = torch.device("cuda", 0)
+    adam = DeviceAdam(n, dev, **kw)
+    P = torch.from_numpy(p0.copy()).to(dev)
+    adam.reset(P)
+    for s in range(steps):
+        adam.step(P, torch.from_numpy(losses[s]).to(dev), torch.from_numpy(grads[s]).to(dev))
+    torch.cuda.synchronize()
+    for k in range(n):
+        it = iter(range(steps))
+
+        def f(p, k=k, it=it):
+            s = next(it)
+            return float(losses[s, k]), grads[s, k]
+
+        res = O.adam_register(f, p0[k], max_iter=steps, **okw)
+        iters = res["iterations"]
+        assert int(adam.t[k]) == iters
+        np.testing.assert_allclose(adam.best_pose[k].cpu().numpy(), res["final_pose"], rtol=1e-12, atol=1e-12)
+        assert float(adam.best_loss[k]) == res["final_loss"]
+        if iters < steps:  # stopped early: the device pose froze after the same step
+            assert int(adam.done[k]) == 1
+        else:
+            assert int(adam.done[k]) == 0
+    assert int(adam.done[0]) == (1 if opts == "reference" else 0)
+
+
+def test_graph_loop_equals_eager_on_every_replay(vols):
+    from fvr.multistart import AdamLoop
+    from fvr.sncc import BiplanarEvaluator
+    r, (ta, tl) = vols[1]
+    ev_g = BiplanarEvaluator(r.packed, ta, tl)
+    ev_e = BiplanarEvaluator(r.packed, ta, tl)
+    P0 = _poses(8, 21)
+    eager = AdamLoop(ev_e, 8, 12, graph=False, tol=0.0)
+    bp, bl, last = (t.cpu().numpy().copy() for t in eager.run(P0))
+    loop = AdamLoop(ev_g, 8, 12, graph=True, tol=0.0)
+    for rep in range(3):
+        gp, gl, glast = (t.cpu().numpy().copy() for t in loop.run(P0))
+        np.testing.assert_array_equal(gp, bp, err_msg=f"replay {rep}")
+        np.testing.assert_array_equal(gl, bl, err_msg=f"replay {rep}")
+        np.testing.assert_array_equal(glast, last, err_msg=f"replay {rep}")
+        ev_e.evaluate_device(_poses(3, rep))  # unrelated eager work between replays
+        torch.cuda.synchronize()
+    assert loop.graph is not None
+    assert np.all(bl < 1.0)
+
+
+def test_ragged_graph_loop_and_autotune(vols):
+    from fvr.multistart import AdamLoop, autotune, candidate_plans
+    from fvr.sncc import BiplanarEvaluator, RaggedBiplanarEvaluator
+    counts = (6, 9, 4)
+    segs = [BiplanarEvaluator(r.packed, ta, tl) for r, (ta, tl) in vols]
+    rag = RaggedBiplanarEvaluator(segs, counts)
+    P0 = _poses(sum(counts), 5)
+    best, timings = autotune(rag, P0, reps=2, use_cache=False)
+    assert best in candidate_plans(rag.flags) and len(timings) == 18 and rag.plan == best
+    assert all(t > 0 for t in timings.values())
+    again, t2 = autotune(rag, P0)  # cached
+    assert again == best and t2 == {}
+    eager = AdamLoop(rag, sum(counts), 6, graph=False, tol=0.0)
+    bp, bl, _ = (t.cpu().numpy().copy() for t in eager.run(P0))
+    loop = AdamLoop(rag, sum(counts), 6, graph=True, tol=0.0)
+    for rep in range(2):
+        gp, gl, _ = (t.cpu().numpy().copy() for t in loop.run(P0))
+        np.testing.assert_array_equal(gp, bp)
+        np.testing.assert_array_equal(gl, bl)

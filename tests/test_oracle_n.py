@@ -8,6 +8,7 @@ finite differences.
 import math
 
 import numpy as np
+import torch
 import pytest
 from scipy import fft
 from scipy.ndimage import map_coordinates
@@ -203,6 +204,35 @@ def test_adam_closed_form_on_quadratic():
     p1 = res2["trajectory"][1][2]
     lr = np.array([0.25] * 3 + [0.35] * 3)
     np.testing.assert_allclose(p1, lr * np.sign(target), rtol=1e-6)
+
+
+def test_adam_multistart_options_equal_torch_adam_with_lambdalr():
+    """exp30's optimiser: torch.optim.Adam(betas=(0.8, 0.99)) per parameter group (rotations,
+    translations), gradient clamp, LambdaLR cosine factor -- in float64 on a fixed sequence."""
+    rng = np.random.default_rng(5)
+    G = rng.standard_normal((30, 6)) * np.array([0.4] * 3 + [4.0] * 3)
+    p0 = rng.standard_normal(6)
+    k = {"n": 0}
+
+    def f(p):
+        g = G[k["n"]]
+        k["n"] += 1
+        return 1.0, g
+
+    T = 20
+    res = O.adam_register(f, p0, lr_rot=0.02, lr_trans=2.0, max_iter=30, tol=0.0, beta1=0.8, beta2=0.99,
+                          limits=None, gclip=(0.25, 2.5), cosine_t=T)
+    rot = torch.tensor(p0[:3], dtype=torch.float64, requires_grad=True)
+    tr = torch.tensor(p0[3:], dtype=torch.float64, requires_grad=True)
+    opt = torch.optim.Adam([{"params": [rot], "lr": 0.02}, {"params": [tr], "lr": 2.0}], betas=(0.8, 0.99))
+    sched = torch.optim.lr_scheduler.LambdaLR(opt, lambda s: 0.1 + 0.9 * 0.5 * (1.0 + math.cos(math.pi * min(s, T) / T)))
+    for i in range(30):
+        np.testing.assert_allclose(np.concatenate([rot.detach().numpy(), tr.detach().numpy()]),
+                                   res["trajectory"][i][2], rtol=1e-12, atol=1e-12)
+        rot.grad = torch.tensor(G[i, :3]).clamp(-0.25, 0.25)
+        tr.grad = torch.tensor(G[i, 3:]).clamp(-2.5, 2.5)
+        opt.step()
+        sched.step()
 
 
 def test_prepare_spectrum_identity_roundtrip(cube):
