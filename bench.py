@@ -162,6 +162,89 @@ def pou_config4(steps: int, warmup: int, size: int = 256, n_pose: int = 64):
             "loss": float(loss.detach()), "grad_finite": bool(torch.isfinite(g).all())}
 
 
+def multistart_config5(n_steps: int = 100, lat_reps: int = 100):
+    """Ragged batched multistart: seeded ellipsoid phantoms of 128^3, 192^3 and 256^3
+    (spectra 256^3 / 384^3 / 512^3, FP16 with per-brick scales), AP + LAT targets of
+    128^2 / 192^2 / 256^2 DRRs at a seeded pose each, 512 starting poses split 96 / 160 /
+    256 (the truth plus rotations U(-8, 8) deg, translations U(-8, 8) voxels).  All
+    volumes go into ONE fused launch per step (autotuned launch plan); the 100-step Adam
+    loop (reference hyper-parameters, tol 0) is captured once as a HIP graph and replayed.
+    Reports pose evaluations per second of the replayed loop, the fused step's p50/p99
+    latency (one-step graph replays timed with HIP events) and each volume's p50/p99 step
+    latency when it runs alone, and the best-start mTRE per volume."""
+    import numpy as np
+    import torch
+    from fvr import DifferentiableFourierVolumeRenderer
+    from fvr.multistart import AdamLoop, autotune
+    from fvr.phantom import ellipsoid_phantom, random_poses
+    from fvr.registration_3d2d import compute_mtre_6dof
+    from fvr.sncc import BiplanarEvaluator, RaggedBiplanarEvaluator
+    dev = torch.device("cuda", 0)
+    sizes, counts = (128, 192, 256), (96, 160, 256)
+    segs, truths, starts = [], [], []
+    t0 = time.perf_counter()
+    for k, (n, c) in enumerate(zip(sizes, counts)):
+        vol = ellipsoid_phantom(n, n_ellipsoids=12, seed=1234 + k)
+        rng = np.random.default_rng(2345 + k)
+        tp = np.concatenate([rng.uniform(-5, 5, 3), rng.uniform(-5, 5, 3)])
+        r = DifferentiableFourierVolumeRenderer(vol, pad_factor=2.0, storage="fp16", device=dev)
+        T_ap, T_lat = make_targets(r, tp)
+        segs.append(BiplanarEvaluator(r.packed, T_ap, T_lat, adjoint=True))
+        truths.append(tp)
+        starts.append(tp + random_poses(c, 8.0, 8.0, seed=3456 + k))
+        del vol
+    build_s = time.perf_counter() - t0
+    N = sum(counts)
+    rag = RaggedBiplanarEvaluator(segs, counts)
+    P0 = torch.from_numpy(np.ascontiguousarray(np.vstack(starts))).to(dev)
+    plan, timings = autotune(rag, P0, reps=5)
+
+    def lat_pcts(ev, n, P):
+        one = AdamLoop(ev, n, 1, graph=True, tol=0.0)
+        one.run(P)
+        ev_s = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(lat_reps)]
+        for a, b in ev_s:
+            a.record()
+            one.graph.replay()
+            b.record()
+        torch.cuda.synchronize()
+        t = np.array([a.elapsed_time(b) for a, b in ev_s])
+        return {"p50_ms": round(float(np.percentile(t, 50)), 4), "p99_ms": round(float(np.percentile(t, 99)), 4)}
+
+    loop = AdamLoop(rag, N, n_steps, graph=True, tol=0.0)
+    loop.run(P0)  # capture + first run
+    torch.cuda.synchronize()
+    reps = 3
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t_host = time.perf_counter()
+    e0.record()
+    for _ in range(reps):
+        best, best_loss, _ = loop.run(P0)
+    e1.record()
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t_host) * 1e3 / reps
+    loop_ms = e0.elapsed_time(e1) / reps
+    fused = lat_pcts(rag, N, P0)
+    per_volume, mtre = {}, {}
+    best, best_loss = best.cpu().numpy(), best_loss.cpu().numpy()
+    off = 0
+    for k, (n, c) in enumerate(zip(sizes, counts)):
+        per_volume[f"{n}^3"] = dict(poses=c, **lat_pcts(segs[k], c, P0[off:off + c].contiguous()))
+        kb = off + int(np.argmin(best_loss[off:off + c]))
+        mtre[f"{n}^3"] = round(float(compute_mtre_6dof(truths[k], best[kb], (n, n, n))), 4)
+        off += c
+    return {"workload": "ragged multistart: 128^3/192^3/256^3 phantoms (256^3/384^3/512^3 fp16 spectra), "
+                        "AP+LAT targets, 512 starts split 96/160/256, one fused launch per step, "
+                        f"{n_steps}-step Adam captured as one HIP graph",
+            "poses": N, "steps": n_steps, "loop_ms": round(loop_ms, 3), "loop_wall_ms": round(wall_ms, 3),
+            "pose_evals_per_s": round(N * n_steps / (loop_ms * 1e-3), 1),
+            "fused_step_latency": fused, "per_volume_step_latency_alone": per_volume,
+            "launch_plan": {"tile_h": plan.tile_h, "order": plan.order, "persist": plan.persist},
+            "autotune_ms": {f"{p.tile_h}/{p.order}/{p.persist}": round(t, 4) for p, t in
+                            sorted(timings.items(), key=lambda kv: kv[1])},
+            "best_start_mtre_mm": mtre, "build_s": round(build_s, 2)}
+
+
 def cpu_baseline(renderer, T_ap, T_lat, cands, budget_s: float):
     """The NumPy/SciPy oracle (reference algorithm: 14 map_coordinates slices per
     pose eval) on one host core, on a bounded sample of the candidate poses."""
@@ -197,6 +280,9 @@ def main(argv=None) -> int:
     ap.add_argument("--no-extra", action="store_true", help="headline storage only (no f32 / plain forms)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-pou", action="store_true", help="skip the PoU-FVR perspective workload")
+    ap.add_argument("--no-multistart", action="store_true", help="skip the ragged multistart workload")
+    ap.add_argument("--plan", default="auto",
+                    help="fused-kernel launch plan 'tile_h/order/persist' (e.g. 64/target/0) or 'auto' (autotune)")
     ap.add_argument("--no-footprint", action="store_true",
                     help="skip the distinct-entry count (roofline fields null; for profiler runs)")
     args = ap.parse_args(argv)
@@ -210,7 +296,8 @@ def main(argv=None) -> int:
     import torch
     from fvr import DifferentiableFourierVolumeRenderer
     from fvr.footprint import batch_algorithmic_bytes
-    from fvr.sncc import BiplanarEvaluator
+    from fvr.multistart import autotune
+    from fvr.sncc import BiplanarEvaluator, LaunchPlan
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
@@ -226,11 +313,19 @@ def main(argv=None) -> int:
         if r.storage != st:
             r.repack(st)
         ev = BiplanarEvaluator(r.packed, T_ap, T_lat, adjoint=True)
+        timings = {}
+        if args.plan == "auto":
+            plan, timings = autotune(ev, P, reps=5)
+        else:
+            th, order, persist = args.plan.split("/")
+            plan = LaunchPlan(int(th), order, int(persist))
+            ev.set_plan(plan)
         step_ms, kern_ms, _ = time_batched(ev, P, args.steps, args.warmup)
         loss, grad = ev.evaluate(cands)
         results[st] = {"ms_per_step": step_ms, "kernel_ms": kern_ms, "value": args.n_cand / (step_ms * 1e-3),
-                       "loss_mean": float(loss.mean()), "loss": loss, "grad": grad, "herm": ev.herm}
-        log(f"{st}: {step_ms:.3f} ms/step, fused kernel {kern_ms:.3f} ms, "
+                       "loss_mean": float(loss.mean()), "loss": loss, "grad": grad, "herm": ev.herm,
+                       "plan": plan, "autotune": timings}
+        log(f"{st}: plan {plan}: {step_ms:.3f} ms/step, fused kernel {kern_ms:.3f} ms, "
             f"{results[st]['value']:.0f} pose evals/s")
         del ev
     plain_ms = None
@@ -276,6 +371,11 @@ def main(argv=None) -> int:
         torch.cuda.empty_cache()
         extra["pou"] = pou_config4(max(3, args.steps // 10), 2)
         log(f"PoU: {extra['pou']}")
+    if not args.no_extra and not args.no_multistart:
+        log("ragged multistart workload (config 5)")
+        torch.cuda.empty_cache()
+        extra["multistart"] = multistart_config5()
+        log(f"multistart: {extra['multistart']}")
     if plain_ms is not None:
         extra["plain_f32_per_pose_value"] = 1e3 / plain_ms
         extra["speedup_vs_plain_f32"] = head["value"] / (1e3 / plain_ms)
@@ -296,7 +396,10 @@ def main(argv=None) -> int:
         "config": {"workload": "parallel-beam biplanar Spectral-NCC fwd + exact 6-DoF adjoint",
                    "volume": args.size, "spectrum": 2 * args.size, "views": ["ap", "lateral"],
                    "slice_samples_per_view": (2 * args.size) ** 2, "candidates_per_launch": args.n_cand,
-                   "spectrum_storage": args.storage},
+                   "spectrum_storage": args.storage,
+                   "launch_plan": {"tile_h": head["plan"].tile_h, "order": head["plan"].order,
+                                   "persist": head["plan"].persist,
+                                   "autotuned": args.plan == "auto"}},
         "roofline": {"bound": "hbm", "achieved": None if achieved is None else round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": None if achieved is None else round(achieved / HBM_PEAK_GBS, 4),
@@ -306,6 +409,8 @@ def main(argv=None) -> int:
                      "schedule": fp.get("schedule"), "samples_per_launch": fp.get("samples"),
                      "bricks": fp["bricks"]},
         "cpu_baseline": cpu,
+        "autotune_ms": {f"{p.tile_h}/{p.order}/{p.persist}": round(t, 4)
+                        for p, t in sorted(head["autotune"].items(), key=lambda kv: kv[1])},
         "extra": extra,
     }
     print(json.dumps(out), flush=True)
