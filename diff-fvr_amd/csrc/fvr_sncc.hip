@@ -41,8 +41,9 @@ struct cpx {
 };
 
 // Raw loads of one sample: the four fast-axis corner pairs (slow, mid) =
-// (0,0), (0,1), (1,0), (1,1) and, for scaled storage, the eight corner scales.
-// They are issued one sample ahead of their use (see the row loop).
+// (0,0), (0,1), (1,0), (1,1) and, for scaled storage, the one scale of the
+// brick that holds all eight corners (apron layout).  They are issued one
+// sample ahead of their use (see the row loop).
 template <int DT>
 struct Raw;
 
@@ -50,6 +51,12 @@ template <>
 struct Raw<FVR_C32> {
   static constexpr uint32_t kElem = 8;
   f4a8 q[4];
+  // byte offset of the storage corner (S, M, F) from the layout base
+  static __device__ __forceinline__ uint32_t offset(uint32_t S, uint32_t M, uint32_t F, uint32_t planeb,
+                                                    uint32_t rowb) {
+    return __umul24(S, planeb) + __umul24(M, rowb) + F * 8u;
+  }
+  __device__ __forceinline__ float scale() const { return 1.f; }
   __device__ __forceinline__ void load(gcptr b0, gcptr b1, uint32_t off, uint32_t row, gcptr, uint32_t, uint32_t,
                                        uint32_t, uint32_t, uint32_t) {
     typedef const f4a8 __attribute__((address_space(1)))* gp;
@@ -75,7 +82,12 @@ template <int DT>
 struct RawQ {
   static constexpr uint32_t kElem = 4;
   u2a4 q[4];
-  float s[8];  // scale of corner (slow, mid, fast) at index 4 slow + 2 mid + fast
+  float s;  // scale of the brick holding the cell (every corner was quantised with it)
+  // apron layout: storage index x of an axis sits at physical index x + (x >> 3)
+  static __device__ __forceinline__ uint32_t offset(uint32_t S, uint32_t M, uint32_t F, uint32_t planeb,
+                                                    uint32_t rowb) {
+    return __umul24(S + (S >> 3), planeb) + __umul24(M + (M >> 3), rowb) + (F + (F >> 3)) * 4u;
+  }
   // S, M, F: storage indices of the low corner; Bm, Bf: brick counts along mid / fast
   __device__ __forceinline__ void load(gcptr b0, gcptr b1, uint32_t off, uint32_t row, gcptr scales, uint32_t S,
                                        uint32_t M, uint32_t F, uint32_t Bm, uint32_t Bf) {
@@ -85,24 +97,15 @@ struct RawQ {
     q[1] = *reinterpret_cast<gp>(b0 + offr);
     q[2] = *reinterpret_cast<gp>(b1 + off);
     q[3] = *reinterpret_cast<gp>(b1 + offr);
-    // byte offsets of the corner bricks' scales (32-bit, from the scale table base)
-    const uint32_t z0 = __umul24(S >> 3, Bm), z1 = __umul24((S + 1) >> 3, Bm);
-    const uint32_t m0 = M >> 3, m1 = (M + 1) >> 3, f0 = (F >> 3) << 2, f1 = ((F + 1) >> 3) << 2;
-    const uint32_t r00 = __umul24(z0 + m0, Bf) << 2, r01 = __umul24(z0 + m1, Bf) << 2;
-    const uint32_t r10 = __umul24(z1 + m0, Bf) << 2, r11 = __umul24(z1 + m1, Bf) << 2;
-    s[0] = *reinterpret_cast<gfptr>(scales + (r00 + f0));
-    s[1] = *reinterpret_cast<gfptr>(scales + (r00 + f1));
-    s[2] = *reinterpret_cast<gfptr>(scales + (r01 + f0));
-    s[3] = *reinterpret_cast<gfptr>(scales + (r01 + f1));
-    s[4] = *reinterpret_cast<gfptr>(scales + (r10 + f0));
-    s[5] = *reinterpret_cast<gfptr>(scales + (r10 + f1));
-    s[6] = *reinterpret_cast<gfptr>(scales + (r11 + f0));
-    s[7] = *reinterpret_cast<gfptr>(scales + (r11 + f1));
+    const uint32_t r = __umul24(__umul24(S >> 3, Bm) + (M >> 3), Bf) + (F >> 3);
+    s = *reinterpret_cast<gfptr>(scales + (r << 2));
   }
-  __device__ __forceinline__ float r0(int k) const { return deq<DT>(q[k].x & 0xFFFFu) * s[2 * k]; }
-  __device__ __forceinline__ float i0(int k) const { return deq<DT>(q[k].x >> 16) * s[2 * k]; }
-  __device__ __forceinline__ float r1(int k) const { return deq<DT>(q[k].y & 0xFFFFu) * s[2 * k + 1]; }
-  __device__ __forceinline__ float i1(int k) const { return deq<DT>(q[k].y >> 16) * s[2 * k + 1]; }
+  __device__ __forceinline__ float scale() const { return s; }
+  // corner values in units of the scale
+  __device__ __forceinline__ float r0(int k) const { return deq<DT>(q[k].x & 0xFFFFu); }
+  __device__ __forceinline__ float i0(int k) const { return deq<DT>(q[k].x >> 16); }
+  __device__ __forceinline__ float r1(int k) const { return deq<DT>(q[k].y & 0xFFFFu); }
+  __device__ __forceinline__ float i1(int k) const { return deq<DT>(q[k].y >> 16); }
 };
 
 template <>
@@ -216,12 +219,10 @@ __device__ __forceinline__ void sncc_tile(const fvr_layout* __restrict__ layouts
   const int i0 = tyi * (4 * R) + wave * R;
   const bool colok = uint32_t(j) < uint32_t(iw);  // unsigned: a malformed tile index reads nothing
 
-  constexpr uint32_t esz = Raw<DT>::kElem;
   const uint32_t rowb = L.row_bytes, planeb = L.plane_bytes;
-  const uint32_t lo_off = uint32_t(lo_int) * (planeb + rowb + esz);
-  const gcptr b0 = reinterpret_cast<gcptr>(L.base) + lo_off;
+  const gcptr b0 = reinterpret_cast<gcptr>(L.base);
   const gcptr b1 = b0 + planeb;
-  const uint32_t guard = it.guard_off - lo_off;
+  const uint32_t guard = it.guard_off;
   const gcptr scales = reinterpret_cast<gcptr>(L.scales);
   const uint32_t Bm = uint32_t(L.bricks[1]), Bf = uint32_t(L.bricks[2]);
   const gcptr Tb = reinterpret_cast<gcptr>(it.target);
@@ -257,8 +258,8 @@ __device__ __forceinline__ void sncc_tile(const fvr_layout* __restrict__ layouts
     sm.ts = float(uint32_t(p0)) * kTwoM32;
     sm.tm = float(uint32_t(p1)) * kTwoM32;
     sm.tf = float(uint32_t(p2)) * kTwoM32;
-    const uint32_t offi = __umul24(fs, planeb) + __umul24(fm, rowb) + ff * esz;
-    sm.raw.load(b0, b1, inb ? offi : guard, rowb, scales, fs + lo_int, fm + lo_int, ff + lo_int, Bm, Bf);
+    const uint32_t S = fs + lo_int, M = fm + lo_int, F = ff + lo_int;
+    sm.raw.load(b0, b1, inb ? Raw<DT>::offset(S, M, F, planeb, rowb) : guard, rowb, scales, S, M, F, Bm, Bf);
     sm.t = *reinterpret_cast<gf2ptr>(Tb + (ok ? toff : 0u));
     sm.ok = ok;
     p0 += v0;
@@ -298,19 +299,20 @@ __device__ __forceinline__ void sncc_tile(const fvr_layout* __restrict__ layouts
     const float prr = readlane_f(rowph.r, k), pri = readlane_f(rowph.i, k);
     const float phr = colph.r * prr - colph.i * pri;
     const float phi = fmaf(colph.r, pri, colph.i * prr);
-    if (WS && sm.ok) Sout[i * iw + j] = make_float2(Ir * phr - Ii * phi, fmaf(Ir, phi, Ii * phr));
-    // W = phase * conj(T)
-    float Wr = fmaf(phr, t.x, phi * t.y);
-    float Wi = fmaf(phi, t.x, -phr * t.y);
-    float Jr = Ir, Ji = Ii;  // the factor carried by E and the H moments
+    // I, dI above are in units of the brick scale sc: the scale is carried by W (sc) and by
+    // the factor J of E and of the H moments (sc^2)
+    const float sc = q.scale();
+    if (WS && sm.ok) Sout[i * iw + j] = make_float2(sc * (Ir * phr - Ii * phi), sc * fmaf(Ir, phi, Ii * phr));
+    float wsc = sc, esc = sc * sc;
     if (HP) {
       const float hw = fmaf(-hx, readlane_f(hy, k), 1.f);
-      Wr *= hw;
-      Wi *= hw;
-      const float h2 = hw * hw;
-      Jr *= h2;
-      Ji *= h2;
+      wsc *= hw;
+      esc *= hw * hw;
     }
+    // W = phase * conj(T)
+    const float Wr = wsc * fmaf(phr, t.x, phi * t.y);
+    const float Wi = wsc * fmaf(phi, t.x, -phr * t.y);
+    const float Jr = Ir * esc, Ji = Ii * esc;
     const float rw = rw0 + float(i) * rws;
     const float IWr = fmaf(Ir, Wr, -Ii * Wi);
     const float IWi = fmaf(Ir, Wi, Ii * Wr);
@@ -428,12 +430,10 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
   const bool first = r0 == 0;
   const int nslot = nreg + (first ? 1 : 0);
 
-  constexpr uint32_t esz = Raw<DT>::kElem;
   const uint32_t rowb = L.row_bytes, planeb = L.plane_bytes;
-  const uint32_t lo_off = uint32_t(lo_int) * (planeb + rowb + esz);
-  const gcptr b0 = reinterpret_cast<gcptr>(L.base) + lo_off;
+  const gcptr b0 = reinterpret_cast<gcptr>(L.base);
   const gcptr b1 = b0 + planeb;
-  const uint32_t guard = it.guard_off - lo_off;
+  const uint32_t guard = it.guard_off;
   const gcptr scales = reinterpret_cast<gcptr>(L.scales);
   const uint32_t Bm = uint32_t(L.bricks[1]), Bf = uint32_t(L.bricks[2]);
   const gcptr TT = reinterpret_cast<gcptr>(it.target);
@@ -451,6 +451,17 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
   const float cwj = it.cw0 + float(j) * it.cws;
   // row phasors of this wave's regular rows (lane k <-> row h/2 + r0 + k)
   const cpx rowph = phasor(it.ph_row0 + uint32_t(hh + r0 + lane) * it.ph_row);
+  // regular rows are consecutive slots: their position, target offset and row weight
+  // are stepped from the wave's first regular row (slot 1 in the first wave)
+  const int ireg = hh + r0 + (first ? 1 : 0);
+  int64_t q0 = bj0 + int64_t(ireg) * it.vstep[0];
+  int64_t q1 = bj1 + int64_t(ireg) * it.vstep[1];
+  int64_t q2 = bj2 + int64_t(ireg) * it.vstep[2];
+  const int64_t v0 = it.vstep[0], v1 = it.vstep[1], v2 = it.vstep[2];
+  uint32_t toffr = uint32_t(ireg * W + j) * 16u;
+  const uint32_t tstep = uint32_t(W) * 16u;
+  float rwr = it.rw0 + float(ireg) * it.rws;
+  const float rws = it.rws;
 
   // stage 1: the sample of slot s -- position, box flags, loads
   auto issue = [&](SmpH<DT>& sm, int s) {
@@ -458,18 +469,22 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
     bool valid, mirror;
     uint32_t toff;
     if (s < nreg && !(first && s == 0)) {  // a regular row i = h/2 + r0 + s (> h/2): every column but 0 paired
-      const int i = hh + r0 + s;
-      p0 = bj0 + int64_t(i) * it.vstep[0];
-      p1 = bj1 + int64_t(i) * it.vstep[1];
-      p2 = bj2 + int64_t(i) * it.vstep[2];
+      p0 = q0;
+      p1 = q1;
+      p2 = q2;
+      q0 += v0;
+      q1 += v1;
+      q2 += v2;
       valid = colok;
       mirror = j >= 1;
       const float prr = readlane_f(rowph.r, s), pri = readlane_f(rowph.i, s);
       sm.phr = colph.r * prr - colph.i * pri;
       sm.phi = fmaf(colph.r, pri, colph.i * prr);
       sm.cw = cwj;
-      sm.rw = it.rw0 + float(i) * it.rws;
-      toff = uint32_t(i * W + j) * 16u;
+      sm.rw = rwr;
+      rwr += rws;
+      toff = toffr;
+      toffr += tstep;
     } else {  // row h/2 (remapped lanes take column 0 of rows 1 .. h/2-1) or the unpaired row 0
       int ii, jj;
       if (s == 0) {
@@ -504,8 +519,8 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
     sm.ts = float(uint32_t(p0)) * kTwoM32;
     sm.tm = float(uint32_t(p1)) * kTwoM32;
     sm.tf = float(uint32_t(p2)) * kTwoM32;
-    const uint32_t offi = __umul24(fs, planeb) + __umul24(fm, rowb) + ff * esz;
-    sm.raw.load(b0, b1, inb ? offi : guard, rowb, scales, fs + lo_int, fm + lo_int, ff + lo_int, Bm, Bf);
+    const uint32_t S = fs + lo_int, M = fm + lo_int, F = ff + lo_int;
+    sm.raw.load(b0, b1, inb ? Raw<DT>::offset(S, M, F, planeb, rowb) : guard, rowb, scales, S, M, F, Bm, Bf);
     sm.t = *reinterpret_cast<gf4ptr>(TT + (valid ? toff : 0u));
     sm.wp = bp ? 1.f : 0.f;
     sm.wq = bq ? 1.f : 0.f;
@@ -532,12 +547,14 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
     const float zr = f1r - f0r, zi = f1i - f0i;
     const float Ir = fmaf(ts, zr, f0r), Ii = fmaf(ts, zi, f0i);
 
-    // folded target T'' = bp T + bq conj T(mirror); W = phase conj(T'')
-    const float Tr = fmaf(sm.wq, sm.t.z, sm.wp * sm.t.x);
-    const float Ti = fmaf(sm.wq, sm.t.w, sm.wp * sm.t.y);
+    // folded target T'' = bp T + bq conj T(mirror); W = phase conj(T'') sc, with I, dI
+    // in units of the brick scale sc (E and the H moments carry sc^2 through J)
+    const float sc = q.scale();
+    const float Tr = sc * fmaf(sm.wq, sm.t.z, sm.wp * sm.t.x);
+    const float Ti = sc * fmaf(sm.wq, sm.t.w, sm.wp * sm.t.y);
     const float Wr = fmaf(sm.phr, Tr, sm.phi * Ti);
     const float Wi = fmaf(sm.phi, Tr, -sm.phr * Ti);
-    const float wE = sm.wp + sm.wq;
+    const float wE = (sm.wp + sm.wq) * (sc * sc);
     const float Jr = Ir * wE, Ji = Ii * wE;
     const float cw = sm.cw, rw = sm.rw;
     const float IWr = fmaf(Ir, Wr, -Ii * Wi);

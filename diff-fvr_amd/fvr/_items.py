@@ -97,6 +97,17 @@ def tiles_for(h: int, w: int, tile_rows: int, herm: bool = False) -> Tuple[int, 
     return (w + 63) // 64, (rows + tile_rows - 1) // tile_rows
 
 
+def physical_index(x, desc):
+    """Physical slow/mid/fast index of storage index ``x`` in a layout: x + (x >> 3) in the
+    apron layout of scaled storage (every 8-element brick run followed by an apron copy)."""
+    return x + (x >> 3) if int(desc["flags"]) & _lib.LAYOUT_APRON else x
+
+
+def guard_offset(desc) -> int:
+    """Byte offset of the all-zero guard plane (storage plane sdim0 - 2) of a layout."""
+    return int(physical_index(int(desc["sdim"][0]) - 2, desc)) * int(desc["plane_bytes"])
+
+
 def herm_ok(h: int, w: int) -> bool:
     """Slice shapes the Hermitian-fold kernel covers: even sides, h <= w (column 0 of the
     lower rows is evaluated by the idle lanes of row h/2)."""
@@ -131,7 +142,7 @@ def n_item(packed, view: str, yaw, pitch, roll, a, b, target_ptr: int, flags: in
     it["h"], it["w"] = h, w
     it["layout"] = lay_idx
     it["lo_int"] = 1
-    it["guard_off"] = (int(desc["sdim"][0]) - 2) * int(desc["plane_bytes"])
+    it["guard_off"] = guard_offset(desc)
     it["tile_base"] = tile_base
     it["tiles_x"] = tiles_for(h, w, tile_rows)[0]
     it["tile_h"] = tile_rows
@@ -288,7 +299,9 @@ def validate_items(items: np.ndarray, n_layouts: int, descs: np.ndarray, tile_ro
     # inside samples have floor(p) <= hi; their corners (+lo, +1) must stay inside the storage
     if np.any(hi + lo[:, None] + 1 > d["sdim"].astype(np.int64) - 1):
         raise ValueError("item box exceeds the layout's storage")
-    if np.any(items["guard_off"].astype(np.int64) != (d["sdim"][:, 0].astype(np.int64) - 2) * d["plane_bytes"]):
+    s_g = d["sdim"][:, 0].astype(np.int64) - 2
+    s_g = np.where((d["flags"] & _lib.LAYOUT_APRON) != 0, s_g + (s_g >> 3), s_g)
+    if np.any(items["guard_off"].astype(np.int64) != s_g * d["plane_bytes"]):
         raise ValueError("item guard offset is not the layout's zero plane")
     return int(ntile.sum())
 
@@ -372,7 +385,7 @@ def t_template(packed, view: str, out_res: int, target_ptr: int, flags: int, til
     it["h"], it["w"] = out_res, out_res
     it["layout"] = lay_idx
     it["lo_int"] = 0
-    it["guard_off"] = (int(desc["sdim"][0]) - 2) * int(desc["plane_bytes"])
+    it["guard_off"] = guard_offset(desc)
     it["tiles_x"] = tiles_for(out_res, out_res, tile_rows)[0]
     it["tile_h"] = tile_rows
     it["flags"] = flags

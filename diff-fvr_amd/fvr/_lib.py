@@ -20,7 +20,7 @@ ABI_VERSION = 2
 FVR_C32, FVR_C16F, FVR_C16B = 0, 1, 2
 DTYPE_CODES = {"f32": FVR_C32, "fp16": FVR_C16F, "bf16": FVR_C16B}
 ITEM_ADJOINT, ITEM_HP, ITEM_WRITE_SLICE, ITEM_HERM = 1, 2, 4, 8
-LAYOUT_WRAP = 1
+LAYOUT_WRAP, LAYOUT_APRON = 1, 2
 NMOM = 16
 
 # numpy mirrors of the C structs (explicit offsets; sizes are checked against

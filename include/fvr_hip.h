@@ -65,8 +65,18 @@ typedef enum fvr_dtype {
  * With FVR_LAYOUT_WRAP the logical index n of every axis holds the periodic
  * copy of index 0 (the DFT's own wrap, so that the stored lattice is
  * Hermitian-symmetric about n/2 for the spectrum of a real volume) and the
- * storage dims are slow n0+4, mid n1+3, fast n2+3. */
+ * storage dims are slow n0+4, mid n1+3, fast n2+3.
+ *
+ * Scaled dtypes (FVR_C16F / FVR_C16B) use the apron layout (FVR_LAYOUT_APRON,
+ * set by fvr_layout_init): storage index s of an axis lives at physical index
+ * s + (s >> 3), i.e. every 8-element brick run is followed by a copy of the
+ * next brick's first element, and every stored element -- apron copies
+ * included -- is quantised with the scale of the brick it is stored in.  The
+ * 2x2x2 corners of a trilinear cell whose low corner is s then sit at physical
+ * indices s + (s >> 3) and that + 1 in one brick, under one scale.  Physical
+ * dims are 9 * bricks; row_bytes and plane_bytes are physical strides. */
 #define FVR_LAYOUT_WRAP 1u
+#define FVR_LAYOUT_APRON 2u
 typedef struct fvr_layout {
   uint64_t base;        /* device address of storage element (0,0,0) */
   uint64_t scales;      /* device address of float brick scales (0 for FVR_C32) */
@@ -75,10 +85,10 @@ typedef struct fvr_layout {
   int32_t sdim[3];      /* storage dims */
   int32_t bricks[3];    /* brick counts over storage dims (8^3 bricks) */
   uint32_t elem_bytes;  /* 8 (C32) or 4 (C16F, C16B) */
-  uint32_t row_bytes;   /* sdim[2] * elem_bytes */
-  uint32_t plane_bytes; /* sdim[1] * row_bytes */
+  uint32_t row_bytes;   /* physical fast-axis length * elem_bytes */
+  uint32_t plane_bytes; /* physical mid-axis length * row_bytes */
   uint32_t flags;       /* FVR_LAYOUT_* */
-  uint64_t total_bytes; /* sdim[0] * plane_bytes */
+  uint64_t total_bytes; /* physical slow-axis length * plane_bytes */
 } fvr_layout;
 
 /* Fill a layout descriptor for dims n (slow, mid, fast) and dtype (| FVR_LAYOUT_WRAP << 8

@@ -240,3 +240,48 @@ def test_hermitian_fold_equals_full_slice_kernel(setup, storage):
     finally:
         if r.packed.dtype != "f32":
             r.repack("f32")
+
+
+@pytest.mark.parametrize("storage", ["fp16", "bf16"])
+def test_apron_layout_storage_image(storage):
+    """The packed FP16/BF16 storage, read back and decoded by the header's definition:
+    physical index g of an axis holds storage index 8 (g // 9) + g % 9 (logical index - 1,
+    zero outside the spectrum), quantised with the scale of brick g // 9 -- a power of two
+    with max|component| / scale in [2^14, 2^15) over the brick's 9^3 stored elements."""
+    from fvr import DifferentiableFourierVolumeRenderer
+    vol = _vol((20, 22, 18))
+    r = DifferentiableFourierVolumeRenderer(vol, storage=storage)
+    f3 = O.make_spectrum(vol)
+    for name, perm in (("zyx", (0, 1, 2)), ("zxy", (0, 2, 1))):
+        k = r.packed.index[name]
+        d = r.packed.descs[k]
+        n = np.array(d["n"])
+        B = np.array(d["bricks"])
+        raw = r.packed.storage[name].cpu().numpy().view(np.uint16).reshape(9 * B[0], 9 * B[1], 9 * B[2], 2)
+        sc = r.packed.scales[name].cpu().numpy().reshape(tuple(B))
+        spec = np.transpose(f3, perm)
+        g = [np.arange(9 * b) for b in B]
+        s = [8 * (x // 9) + x % 9 for x in g]  # storage index held by each physical index
+        # expected values: logical index = storage - 1, zero outside [0, n)
+        ok = [(x - 1 >= 0) & (x - 1 < m) for x, m in zip(s, n)]
+        idx = np.ix_(*[np.clip(x - 1, 0, m - 1) for x, m in zip(s, n)])
+        exp = spec[idx] * (ok[0][:, None, None] & ok[1][None, :, None] & ok[2][None, None, :])
+        if storage == "fp16":
+            dec = raw.view(np.float16).astype(np.float64)
+        else:
+            dec = (raw.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+        scale = sc[np.ix_(*[x // 9 for x in g])]
+        got = (dec[..., 0] + 1j * dec[..., 1]) * scale
+        bits = 11 if storage == "fp16" else 8
+        err = np.abs(got - exp)
+        # round to nearest: half an ulp of the significand, plus fp16's subnormal step (2^-24 x scale)
+        assert np.all(err <= 2.0 ** -bits * np.abs(exp) * (1 + 1e-12) + 2.0 ** -24 * scale)
+        # scales: powers of two with the brick maximum in [2^14, 2^15) x scale
+        comp = np.maximum(np.abs(exp.real), np.abs(exp.imag))
+        bmax = np.zeros(tuple(B))
+        np.maximum.at(bmax, tuple(np.meshgrid(*[x // 9 for x in g], indexing="ij")), comp)
+        nz = bmax > 0
+        assert np.all(np.log2(sc) == np.round(np.log2(sc)))
+        assert np.all(bmax[nz] / sc[nz] >= 2.0 ** 14 * (1 - 1e-6)) and np.all(bmax[nz] / sc[nz] < 2.0 ** 15)
+        # the apron copy at physical 9b + 8 equals storage index 8b + 8 (the next brick's first element)
+        assert np.array_equal(s[2][8::9], 8 * np.arange(B[2]) + 8)
