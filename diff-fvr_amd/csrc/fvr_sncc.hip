@@ -66,11 +66,13 @@ struct Raw<FVR_C32> {
     q[2] = *reinterpret_cast<gp>(b1 + off);
     q[3] = *reinterpret_cast<gp>(b1 + offr);
   }
-  // corner pair k as (re0, im0, re1, im1)
-  __device__ __forceinline__ float r0(int k) const { return q[k].x; }
-  __device__ __forceinline__ float i0(int k) const { return q[k].y; }
-  __device__ __forceinline__ float r1(int k) const { return q[k].z; }
-  __device__ __forceinline__ float i1(int k) const { return q[k].w; }
+  // corner pair k: the low corner's value and the fast-axis difference (high - low)
+  __device__ __forceinline__ void pair(int k, float& ar, float& ai, float& dr, float& di) const {
+    ar = q[k].x;
+    ai = q[k].y;
+    dr = q[k].z - ar;
+    di = q[k].w - ai;
+  }
 };
 
 template <int DT>
@@ -101,11 +103,22 @@ struct RawQ {
     s = *reinterpret_cast<gfptr>(scales + (r << 2));
   }
   __device__ __forceinline__ float scale() const { return s; }
-  // corner values in units of the scale
-  __device__ __forceinline__ float r0(int k) const { return deq<DT>(q[k].x & 0xFFFFu); }
-  __device__ __forceinline__ float i0(int k) const { return deq<DT>(q[k].x >> 16); }
-  __device__ __forceinline__ float r1(int k) const { return deq<DT>(q[k].y & 0xFFFFu); }
-  __device__ __forceinline__ float i1(int k) const { return deq<DT>(q[k].y >> 16); }
+  // corner pair k in units of the scale: the low corner's value and the fast-axis
+  // difference (high - low).  FP16: the difference is one v_fma_mix_f32 each
+  // (f16 operand converted inside the FMA: h * 1 - low).
+  __device__ __forceinline__ void pair(int k, float& ar, float& ai, float& dr, float& di) const {
+    if (DT == FVR_C16F) {
+      ar = h2f(uint16_t(q[k].x & 0xFFFFu));
+      ai = h2f(uint16_t(q[k].x >> 16));
+      asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel_hi:[1,0,0]" : "=v"(dr) : "v"(q[k].y), "v"(ar));
+      asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(di) : "v"(q[k].y), "v"(ai));
+    } else {
+      ar = deq<DT>(q[k].x & 0xFFFFu);
+      ai = deq<DT>(q[k].x >> 16);
+      dr = deq<DT>(q[k].y & 0xFFFFu) - ar;
+      di = deq<DT>(q[k].y >> 16) - ai;
+    }
+  }
 };
 
 template <>
@@ -275,12 +288,11 @@ __device__ __forceinline__ void sncc_tile(const fvr_layout* __restrict__ layouts
     const float ts = sm.ts, tm = sm.tm, tf = sm.tf;
     const Raw<DT>& q = sm.raw;
     // fast-axis lerps (d = V1 - V0, e = V0 + tf d) for the four (slow, mid) pairs
-    const float a0r = q.r0(0), a0i = q.i0(0), b0r = q.r0(1), b0i = q.i0(1);
-    const float c0r = q.r0(2), c0i = q.i0(2), e0r_ = q.r0(3), e0i_ = q.i0(3);
-    const float d00r = q.r1(0) - a0r, d00i = q.i1(0) - a0i;
-    const float d01r = q.r1(1) - b0r, d01i = q.i1(1) - b0i;
-    const float d10r = q.r1(2) - c0r, d10i = q.i1(2) - c0i;
-    const float d11r = q.r1(3) - e0r_, d11i = q.i1(3) - e0i_;
+    float a0r, a0i, b0r, b0i, c0r, c0i, e0r_, e0i_, d00r, d00i, d01r, d01i, d10r, d10i, d11r, d11i;
+    q.pair(0, a0r, a0i, d00r, d00i);
+    q.pair(1, b0r, b0i, d01r, d01i);
+    q.pair(2, c0r, c0i, d10r, d10i);
+    q.pair(3, e0r_, e0i_, d11r, d11i);
     const float e00r = fmaf(tf, d00r, a0r), e00i = fmaf(tf, d00i, a0i);
     const float e01r = fmaf(tf, d01r, b0r), e01i = fmaf(tf, d01i, b0i);
     const float e10r = fmaf(tf, d10r, c0r), e10i = fmaf(tf, d10i, c0i);
@@ -440,9 +452,11 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
 
   const uint64_t h0 = it.box_hi[0], h1 = it.box_hi[1], h2 = it.box_hi[2];
   // mirror of a coordinate: K - p with K = n 2^32 + 2 bias = box_hi + 2^32 + bias
+  // (inside iff K - h <= p <= K: two compares instead of a 64-bit subtract and one)
   const uint64_t K0 = h0 + (1ull << 32) + (h0 & 0xFFFFFFFFull);
   const uint64_t K1 = h1 + (1ull << 32) + (h1 & 0xFFFFFFFFull);
   const uint64_t K2 = h2 + (1ull << 32) + (h2 & 0xFFFFFFFFull);
+  const uint64_t M0 = K0 - h0, M1 = K1 - h1, M2 = K2 - h2;
   // per-lane column part of the position, phase and weight
   const int64_t bj0 = it.c[0] + int64_t(j) * it.ustep[0];
   const int64_t bj1 = it.c[1] + int64_t(j) * it.ustep[1];
@@ -510,8 +524,8 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
       toff = uint32_t(ii * W + jj) * 16u;
     }
     const bool bp = valid & (uint64_t(p0) <= h0) & (uint64_t(p1) <= h1) & (uint64_t(p2) <= h2);
-    const bool bq = valid & mirror & (K0 - uint64_t(p0) <= h0) & (K1 - uint64_t(p1) <= h1) &
-                    (K2 - uint64_t(p2) <= h2);
+    const bool bq = valid & mirror & (uint64_t(p0) >= M0) & (uint64_t(p0) <= K0) & (uint64_t(p1) >= M1) &
+                    (uint64_t(p1) <= K1) & (uint64_t(p2) >= M2) & (uint64_t(p2) <= K2);
     const bool inb = bp | bq;
     const uint32_t fs = inb ? uint32_t(uint64_t(p0) >> 32) : 0u;
     const uint32_t fm = inb ? uint32_t(uint64_t(p1) >> 32) : 0u;
@@ -530,12 +544,11 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
   auto consume = [&](const SmpH<DT>& sm) {
     const float ts = sm.ts, tm = sm.tm, tf = sm.tf;
     const Raw<DT>& q = sm.raw;
-    const float a0r = q.r0(0), a0i = q.i0(0), b0r = q.r0(1), b0i = q.i0(1);
-    const float c0r = q.r0(2), c0i = q.i0(2), e0r_ = q.r0(3), e0i_ = q.i0(3);
-    const float d00r = q.r1(0) - a0r, d00i = q.i1(0) - a0i;
-    const float d01r = q.r1(1) - b0r, d01i = q.i1(1) - b0i;
-    const float d10r = q.r1(2) - c0r, d10i = q.i1(2) - c0i;
-    const float d11r = q.r1(3) - e0r_, d11i = q.i1(3) - e0i_;
+    float a0r, a0i, b0r, b0i, c0r, c0i, e0r_, e0i_, d00r, d00i, d01r, d01i, d10r, d10i, d11r, d11i;
+    q.pair(0, a0r, a0i, d00r, d00i);
+    q.pair(1, b0r, b0i, d01r, d01i);
+    q.pair(2, c0r, c0i, d10r, d10i);
+    q.pair(3, e0r_, e0i_, d11r, d11i);
     const float e00r = fmaf(tf, d00r, a0r), e00i = fmaf(tf, d00i, a0i);
     const float e01r = fmaf(tf, d01r, b0r), e01i = fmaf(tf, d01i, b0i);
     const float e10r = fmaf(tf, d10r, c0r), e10i = fmaf(tf, d10i, c0i);
