@@ -288,7 +288,7 @@ def main(argv=None) -> int:
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
-              "candidate poses would shard across GPUs with the spectrum replicated, see DESIGN.md",
+              "candidate poses would shard across GPUs with the spectrum replicated",
               file=sys.stderr)
         return 2
 
@@ -322,9 +322,15 @@ def main(argv=None) -> int:
             ev.set_plan(plan)
         step_ms, kern_ms, _ = time_batched(ev, P, args.steps, args.warmup)
         loss, grad = ev.evaluate(cands)
+        # NumPy poses in, NumPy loss / gradient / moments out: the host-buffer boundary (PCIe copies included)
+        torch.cuda.synchronize()
+        t_h = time.perf_counter()
+        for _ in range(10):
+            ev.evaluate(cands)
+        host_ms = (time.perf_counter() - t_h) * 1e3 / 10
         results[st] = {"ms_per_step": step_ms, "kernel_ms": kern_ms, "value": args.n_cand / (step_ms * 1e-3),
                        "loss_mean": float(loss.mean()), "loss": loss, "grad": grad, "herm": ev.herm,
-                       "plan": plan, "autotune": timings}
+                       "plan": plan, "autotune": timings, "host_numpy_ms_per_step": host_ms}
         log(f"{st}: plan {plan}: {step_ms:.3f} ms/step, fused kernel {kern_ms:.3f} ms, "
             f"{results[st]['value']:.0f} pose evals/s")
         del ev
@@ -376,6 +382,8 @@ def main(argv=None) -> int:
         torch.cuda.empty_cache()
         extra["multistart"] = multistart_config5()
         log(f"multistart: {extra['multistart']}")
+    extra["host_numpy_ms_per_step"] = round(head["host_numpy_ms_per_step"], 4)
+    extra["host_numpy_pose_evals_per_s"] = round(args.n_cand / (head["host_numpy_ms_per_step"] * 1e-3), 1)
     if plain_ms is not None:
         extra["plain_f32_per_pose_value"] = 1e3 / plain_ms
         extra["speedup_vs_plain_f32"] = head["value"] / (1e3 / plain_ms)

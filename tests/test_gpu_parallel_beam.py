@@ -2,7 +2,7 @@
 
 Tolerances (complex64 spectrum, FP32 interpolation and sums, float64 final
 reductions): slices max|diff| <= 2e-6 * max|ref|; loss within 1e-5 relative;
-gradients within 1e-4 of max|grad| (the north-star bound is 1e-3 relative).
+gradients within 1e-4 of max|grad| (the target bound is 1e-3 relative).
 """
 
 import numpy as np

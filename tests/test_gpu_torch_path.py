@@ -6,7 +6,7 @@ Tolerances: the generic operator computes sample positions in float32 like the
 reference's kernel (triton_slice.py:83-85), so its outputs are compared at
 1e-5 of max|ref| (f32 storage) and its u/v gradients at 1e-4; the fused path
 uses 32.32 fixed-point positions from float64 geometry: loss within 1e-5
-relative, gradient within 1e-4 of max|grad| (north-star bound 1e-3).
+relative, gradient within 1e-4 of max|grad| (the target bound is 1e-3 relative).
 """
 
 import numpy as np
