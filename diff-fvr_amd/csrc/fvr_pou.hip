@@ -1,8 +1,10 @@
 // PoU-FVR perspective path (cone_beam_fvr.py, triton_slice.py:349-808).
 //
+//   fvr_pou_pack_blocks  planar (B, 2, P, P, P) block spectra -> the gather layout
+//                        of the fused kernels (interleaved (re, im), zero margins).
 //   fvr_pou_slice_ifft   one workgroup per (block, pose): the P x P central slice
-//                        of the block spectrum (per-corner trilinear, planar
-//                        (B, 2, P, P, P) storage) is gathered straight into LDS,
+//                        of the block spectrum (per-corner trilinear, packed
+//                        storage) is gathered straight into LDS,
 //                        inverse-transformed there (2D radix-2, unnormalised,
 //                        centred: fftshift(ifft2(ifftshift(.)))) and its real part
 //                        / P^2 written as the slab image (B, N, P, P) float.
@@ -27,13 +29,106 @@
 namespace fvr {
 
 constexpr int kPouThreads = 512;
-constexpr int kPouMaxLogP = 7;  // LDS-resident transforms up to 128 x 128
+constexpr int kPouSliceThreads = 1024;  // the forward slice + iFFT (no reduction buffer in LDS)
+constexpr int kPouBatch = 4;            // slice samples in flight per thread during a gather
+constexpr int kPouMaxLogP = 7;          // LDS-resident transforms up to 128 x 128
+
+typedef float pf4a8 __attribute__((ext_vector_type(4), aligned(8)));
+typedef uint32_t pu2a4 __attribute__((ext_vector_type(2), aligned(4)));
+
+// Gather layout of the fused kernels (fvr_pou_pack_blocks): per block (P+3) x (P+2) x (P+2)
+// elements (z, y, x; storage index = logical index + 1 with zero margins, the slow axis one
+// zero plane longer: the guard), each element the (re, im) pair -- float2 for f32, two
+// 16-bit values in one 32-bit word for f16 / bf16.  Every trilinear corner of a sample with
+// floor(p) in [-1, P-1] is a valid address (out-of-range corners read the zero margins,
+// i.e. grid_sample's per-corner zero padding); a sample with no corner inside reads the
+// guard planes.
+struct PouPk {
+  uint32_t esz, row, plane;  // bytes
+  unsigned long long block;  // bytes per block
+};
+
+__host__ __device__ inline PouPk pou_pk(int P, int vt) {
+  PouPk k;
+  k.esz = vt == 0 ? 8u : 4u;
+  k.row = uint32_t(P + 2) * k.esz;
+  k.plane = k.row * uint32_t(P + 2);
+  k.block = (unsigned long long)k.plane * (unsigned long long)(P + 3);
+  return k;
+}
 
 template <int VT>
-__device__ __forceinline__ float pld(const void* p, size_t i) {
-  if (VT == 0) return reinterpret_cast<const float*>(p)[i];
-  const uint16_t h = reinterpret_cast<const uint16_t*>(p)[i];
-  return VT == 1 ? h2f(h) : b2f(h);
+struct PkRaw;
+template <>
+struct PkRaw<0> {
+  pf4a8 q[4];  // corner pairs (z, y) = (0,0), (0,1), (1,0), (1,1): (re x0, im x0, re x1, im x1)
+};
+template <int VT>
+struct PkRaw {
+  pu2a4 q[4];  // the same pairs, 16-bit (re | im << 16) at x0, x1
+};
+
+template <int VT>
+struct PkSmp {
+  PkRaw<VT> r;
+  float tx, ty, tz;
+};
+
+// issue the four pair loads of the sample at p (x, y, z block index space)
+template <int VT>
+__device__ __forceinline__ void pk_issue(PkSmp<VT>& s, unsigned long long blk, const PouPk& k, int P, float px,
+                                         float py, float pz) {
+  const float fx = floorf(px), fy = floorf(py), fz = floorf(pz);
+  s.tx = px - fx;
+  s.ty = py - fy;
+  s.tz = pz - fz;
+  const int X = int(fx) + 1, Y = int(fy) + 1, Z = int(fz) + 1;  // storage index of the low corner
+  const bool in = unsigned(X) <= unsigned(P) && unsigned(Y) <= unsigned(P) && unsigned(Z) <= unsigned(P);
+  const uint32_t off = in ? __umul24(uint32_t(Z), k.plane) + __umul24(uint32_t(Y), k.row) + uint32_t(X) * k.esz
+                          : uint32_t(P + 1) * k.plane;
+  typedef const char __attribute__((address_space(1)))* gc;
+  const gc b = reinterpret_cast<gc>(blk);
+  if (VT == 0) {
+    typedef const pf4a8 __attribute__((address_space(1)))* gp;
+    PkRaw<0>& r = reinterpret_cast<PkRaw<0>&>(s.r);
+    r.q[0] = *reinterpret_cast<gp>(b + off);
+    r.q[1] = *reinterpret_cast<gp>(b + off + k.row);
+    r.q[2] = *reinterpret_cast<gp>(b + off + k.plane);
+    r.q[3] = *reinterpret_cast<gp>(b + off + k.plane + k.row);
+  } else {
+    typedef const pu2a4 __attribute__((address_space(1)))* gp;
+    PkRaw<1>& r = reinterpret_cast<PkRaw<1>&>(s.r);
+    r.q[0] = *reinterpret_cast<gp>(b + off);
+    r.q[1] = *reinterpret_cast<gp>(b + off + k.row);
+    r.q[2] = *reinterpret_cast<gp>(b + off + k.plane);
+    r.q[3] = *reinterpret_cast<gp>(b + off + k.plane + k.row);
+  }
+}
+
+template <int VT>
+__device__ __forceinline__ float pk_deq(uint32_t h) {
+  return VT == 1 ? h2f(uint16_t(h)) : b2f(uint16_t(h));
+}
+
+// corner values (corner c = x + 2 y + 4 z) of an issued sample
+template <int VT>
+__device__ __forceinline__ void pk_corners(const PkSmp<VT>& s, float (&cr)[8], float (&ci)[8]) {
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    if (VT == 0) {
+      const PkRaw<0>& r = reinterpret_cast<const PkRaw<0>&>(s.r);
+      cr[2 * p] = r.q[p].x;
+      ci[2 * p] = r.q[p].y;
+      cr[2 * p + 1] = r.q[p].z;
+      ci[2 * p + 1] = r.q[p].w;
+    } else {
+      const PkRaw<1>& r = reinterpret_cast<const PkRaw<1>&>(s.r);
+      cr[2 * p] = pk_deq<VT>(r.q[p].x & 0xFFFFu);
+      ci[2 * p] = pk_deq<VT>(r.q[p].x >> 16);
+      cr[2 * p + 1] = pk_deq<VT>(r.q[p].y & 0xFFFFu);
+      ci[2 * p + 1] = pk_deq<VT>(r.q[p].y >> 16);
+    }
+  }
 }
 
 __device__ __forceinline__ int brev(int x, int logp) {
@@ -87,22 +182,10 @@ __device__ __forceinline__ void make_twiddles(float2* tw, int P) {
   }
 }
 
-// per-corner trilinear value (and gradient) of one block's two planes at p (x, y, z)
-template <int VT, bool GRAD>
-__device__ __forceinline__ void sample_block(const void* F, size_t base_re, size_t plane, int P, float px, float py,
-                                             float pz, float& re, float& im, float* gre, float* gim) {
-  const float x0f = floorf(px), y0f = floorf(py), z0f = floorf(pz);
-  const float tx = px - x0f, ty = py - y0f, tz = pz - z0f;
-  const int x0 = int(x0f), y0 = int(y0f), z0 = int(z0f);
-  float cr[8], ci[8];
-#pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    const int x = x0 + (c & 1), y = y0 + ((c >> 1) & 1), z = z0 + (c >> 2);
-    const bool in = (unsigned)x < (unsigned)P && (unsigned)y < (unsigned)P && (unsigned)z < (unsigned)P;
-    const size_t o = (size_t(z) * P + y) * P + x;
-    cr[c] = in ? pld<VT>(F, base_re + o) : 0.f;
-    ci[c] = in ? pld<VT>(F, base_re + plane + o) : 0.f;
-  }
+// trilinear value (and gradient) from the 8 corners (c = x + 2 y + 4 z) and the fractions
+template <bool GRAD>
+__device__ __forceinline__ void interp8(const float (&cr)[8], const float (&ci)[8], float tx, float ty, float tz,
+                                        float& re, float& im, float* gre, float* gim) {
   const float ox = 1.f - tx, oy = 1.f - ty, oz = 1.f - tz;
   // corner c = x + 2 y + 4 z
   re = ((cr[0] * ox + cr[1] * tx) * oy + (cr[2] * ox + cr[3] * tx) * ty) * oz +
@@ -124,30 +207,57 @@ struct PouSlice {
   float c;  // P // 2, the centre voxel of every axis
 };
 
+// (block, pose) of workgroup slot w: each XCD (w % 8) takes a contiguous block-major range,
+// so the poses of one block run on one XCD and share its L2 copy of the block's slices
+__device__ __forceinline__ int pou_bn(int w, int per_xcd) { return (w & 7) * per_xcd + (w >> 3); }
+
+// issue kPouBatch samples (rows i = q >> logP, columns j) of one slice: columns follow u,
+// rows follow v, sample at c + (j - P/2) u + (i - P/2) v
 template <int VT>
-__global__ __launch_bounds__(kPouThreads) void pou_slice_ifft_kernel(const void* __restrict__ F, PouSlice g,
-                                                                     const float* __restrict__ u,
-                                                                     const float* __restrict__ v,
-                                                                     float* __restrict__ imgs) {
+__device__ __forceinline__ void pk_issue_batch(PkSmp<VT> (&s)[kPouBatch], unsigned long long blk, const PouPk& k,
+                                               const PouSlice& g, int q0, int stride, float ux, float uy, float uz,
+                                               float vx, float vy, float vz) {
+  const int P = g.P, hp = P / 2, nq = P * P;
+#pragma unroll
+  for (int t = 0; t < kPouBatch; ++t) {
+    const int q = min(q0 + t * stride, nq - 1);
+    const int i = q >> g.logP, j = q & (P - 1);
+    const float kx = float(j - hp), ky = float(i - hp);
+    pk_issue<VT>(s[t], blk, k, P, fmaf(ky, vx, fmaf(kx, ux, g.c)), fmaf(ky, vy, fmaf(kx, uy, g.c)),
+                 fmaf(ky, vz, fmaf(kx, uz, g.c)));
+  }
+}
+
+template <int VT>
+__global__ __launch_bounds__(kPouSliceThreads) void pou_slice_ifft_kernel(const char* __restrict__ Fp, PouSlice g,
+                                                                          PouPk k, const float* __restrict__ u,
+                                                                          const float* __restrict__ v, int per_xcd,
+                                                                          float* __restrict__ imgs) {
   extern __shared__ float2 smem[];
   const int P = g.P, logP = g.logP, pitch = P + 1, hp = P / 2;
   float2* a = smem;
   float2* tw = smem + P * pitch;
-  const int bn = blockIdx.x;
+  const int bn = pou_bn(blockIdx.x, per_xcd);
+  if (bn >= g.B * g.N) return;  // whole workgroup: padding slot of the XCD split
   const int b = bn / g.N, n = bn - b * g.N;
   make_twiddles(tw, P);
   const float ux = u[3 * n], uy = u[3 * n + 1], uz = u[3 * n + 2];
   const float vx = v[3 * n], vy = v[3 * n + 1], vz = v[3 * n + 2];
-  const size_t plane = size_t(P) * P * P;
-  const size_t base = size_t(b) * 2 * plane;
-  for (int q = threadIdx.x; q < P * P; q += blockDim.x) {
-    const int i = q >> logP, j = q & (P - 1);
-    const float kx = float(j - hp), ky = float(i - hp);  // columns follow u, rows follow v
-    float re, im;
-    sample_block<VT, false>(F, base, plane, P, fmaf(ky, vx, fmaf(kx, ux, g.c)), fmaf(ky, vy, fmaf(kx, uy, g.c)),
-                            fmaf(ky, vz, fmaf(kx, uz, g.c)), re, im, nullptr, nullptr);
-    // ifftshift: frequency row i goes to natural index (i + P/2) mod P
-    a[((i + hp) & (P - 1)) * pitch + ((j + hp) & (P - 1))] = make_float2(re, im);
+  const unsigned long long blk = reinterpret_cast<unsigned long long>(Fp) + (unsigned long long)b * k.block;
+  const int nq = P * P, stride = blockDim.x;
+  for (int q0 = threadIdx.x; q0 < nq; q0 += kPouBatch * stride) {
+    PkSmp<VT> smp[kPouBatch];
+    pk_issue_batch<VT>(smp, blk, k, g, q0, stride, ux, uy, uz, vx, vy, vz);
+#pragma unroll
+    for (int t = 0; t < kPouBatch; ++t) {
+      const int q = q0 + t * stride;
+      float cr[8], ci[8], re, im;
+      pk_corners<VT>(smp[t], cr, ci);
+      interp8<false>(cr, ci, smp[t].tx, smp[t].ty, smp[t].tz, re, im, nullptr, nullptr);
+      const int i = q >> logP, j = q & (P - 1);
+      // ifftshift: frequency row i goes to natural index (i + P/2) mod P
+      if (q < nq) a[((i + hp) & (P - 1)) * pitch + ((j + hp) & (P - 1))] = make_float2(re, im);
+    }
   }
   __syncthreads();
   fft2d_dif<+1>(a, P, logP, pitch, tw);
@@ -286,18 +396,19 @@ __global__ void pou_fix_to_float_kernel(unsigned long long* __restrict__ acc, co
 
 template <int VT>
 __global__ __launch_bounds__(kPouThreads) void pou_render_bwd_kernel(
-    const void* __restrict__ F, PouSlice sg, const float* __restrict__ u, const float* __restrict__ v,
+    const char* __restrict__ Fp, PouSlice sg, PouPk k, const float* __restrict__ u, const float* __restrict__ v,
     const float* __restrict__ imgs, const float* __restrict__ go, const float* __restrict__ d_b,
     const float* __restrict__ cu, const float* __restrict__ cv, const float* __restrict__ rx,
     const float* __restrict__ ry, const float* __restrict__ ic, PouWarp g, const float* __restrict__ gscale,
-    float* __restrict__ part) {
+    int per_xcd, float* __restrict__ part) {
   extern __shared__ float2 smem[];
   const int P = sg.P, logP = sg.logP, pitch = P + 1, hp = P / 2;
   float2* a = smem;  // P x pitch complex, first used as 64-bit fixed-point accumulators
   float2* tw = smem + P * pitch;
   float* red = reinterpret_cast<float*>(tw + P / 2);  // 9 * kPouThreads floats
   unsigned long long* acc = reinterpret_cast<unsigned long long*>(a);
-  const int bn = blockIdx.x;
+  const int bn = pou_bn(blockIdx.x, per_xcd);
+  if (bn >= sg.B * sg.N) return;  // whole workgroup: padding slot of the XCD split
   const int b = bn / sg.N, n = bn - b * sg.N;
   for (int i = threadIdx.x; i < P * pitch; i += blockDim.x) acc[i] = 0ull;
   make_twiddles(tw, P);
@@ -320,23 +431,30 @@ __global__ __launch_bounds__(kPouThreads) void pou_render_bwd_kernel(
   // 4) recompute every slice sample and contract its spatial gradient with the slice gradient
   const float ux = u[3 * n], uy = u[3 * n + 1], uz = u[3 * n + 2];
   const float vx = v[3 * n], vy = v[3 * n + 1], vz = v[3 * n + 2];
-  const size_t plane = size_t(P) * P * P;
-  const size_t base = size_t(b) * 2 * plane;
+  const unsigned long long blk = reinterpret_cast<unsigned long long>(Fp) + (unsigned long long)b * k.block;
   const float inv = 1.f / float(P * P);
   float gu[3] = {0.f, 0.f, 0.f}, gv[3] = {0.f, 0.f, 0.f};
-  for (int q = threadIdx.x; q < P * P; q += blockDim.x) {
-    const int i = q >> logP, j = q & (P - 1);
-    const float kx = float(j - hp), ky = float(i - hp);
-    const float2 gs = a[brev((i + hp) & (P - 1), logP) * pitch + brev((j + hp) & (P - 1), logP)];
-    const float gr = gs.x * inv, gi = gs.y * inv;
-    float re, im, dre[3], dim[3];
-    sample_block<VT, true>(F, base, plane, P, fmaf(ky, vx, fmaf(kx, ux, sg.c)), fmaf(ky, vy, fmaf(kx, uy, sg.c)),
-                           fmaf(ky, vz, fmaf(kx, uz, sg.c)), re, im, dre, dim);
+  const int nq = P * P, stride = blockDim.x;
+  for (int q0 = threadIdx.x; q0 < nq; q0 += kPouBatch * stride) {
+    PkSmp<VT> smp[kPouBatch];
+    pk_issue_batch<VT>(smp, blk, k, sg, q0, stride, ux, uy, uz, vx, vy, vz);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const float d = gr * dre[c] + gi * dim[c];
-      gu[c] = fmaf(kx, d, gu[c]);
-      gv[c] = fmaf(ky, d, gv[c]);
+    for (int t = 0; t < kPouBatch; ++t) {
+      const int q = q0 + t * stride;
+      if (q >= nq) continue;
+      const int i = q >> logP, j = q & (P - 1);
+      const float kx = float(j - hp), ky = float(i - hp);
+      const float2 gs = a[brev((i + hp) & (P - 1), logP) * pitch + brev((j + hp) & (P - 1), logP)];
+      const float gr = gs.x * inv, gi = gs.y * inv;
+      float cr[8], ci[8], re, im, dre[3], dim[3];
+      pk_corners<VT>(smp[t], cr, ci);
+      interp8<true>(cr, ci, smp[t].tx, smp[t].ty, smp[t].tz, re, im, dre, dim);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const float d = gr * dre[c] + gi * dim[c];
+        gu[c] = fmaf(kx, d, gu[c]);
+        gv[c] = fmaf(ky, d, gv[c]);
+      }
     }
   }
   float all[9] = {s3[0], s3[1], s3[2], gu[0], gu[1], gu[2], gv[0], gv[1], gv[2]};
@@ -415,6 +533,27 @@ __global__ void pou_sum_blocks_kernel(const float* __restrict__ part, int B, int
   out[idx] = float(s);
 }
 
+// planar (B, 2, P, P, P) -> gather layout; one thread per stored element (margins written as 0)
+template <int VT>
+__global__ __launch_bounds__(256) void pou_pack_kernel(const void* __restrict__ F, int P, PouPk k,
+                                                       char* __restrict__ out) {
+  const int gx = blockIdx.x * 256 + threadIdx.x, gy = blockIdx.y;
+  if (gx >= P + 2) return;
+  const int b = blockIdx.z / (P + 3), gz = blockIdx.z - b * (P + 3);
+  const int x = gx - 1, y = gy - 1, z = gz - 1;
+  const bool in = unsigned(x) < unsigned(P) && unsigned(y) < unsigned(P) && unsigned(z) < unsigned(P);
+  const size_t plane = size_t(P) * P * P;
+  const size_t o = size_t(b) * 2 * plane + (size_t(in ? z : 0) * P + (in ? y : 0)) * P + (in ? x : 0);
+  char* dst = out + (unsigned long long)b * k.block + size_t(gz) * k.plane + size_t(gy) * k.row + size_t(gx) * k.esz;
+  if (VT == 0) {
+    const float* f = reinterpret_cast<const float*>(F);
+    *reinterpret_cast<float2*>(dst) = in ? make_float2(f[o], f[o + plane]) : make_float2(0.f, 0.f);
+  } else {
+    const uint16_t* h = reinterpret_cast<const uint16_t*>(F);
+    *reinterpret_cast<uint32_t*>(dst) = in ? uint32_t(h[o]) | (uint32_t(h[o + plane]) << 16) : 0u;
+  }
+}
+
 static size_t slice_lds_bytes(int P) { return (size_t(P) * (P + 1) + P / 2) * sizeof(float2); }
 
 __global__ void pou_zero_u64_kernel(unsigned long long* __restrict__ p, long long n) {
@@ -440,18 +579,45 @@ int fvr_pou_lds_ok(int32_t P) {
   return slice_lds_bytes(P) + 9 * kPouThreads * sizeof(float) <= 160 * 1024 ? 1 : 0;
 }
 
+int64_t fvr_pou_packed_bytes(int32_t dtype, int32_t B, int32_t P) {
+  if (B <= 0 || P < 2 || dtype < 0 || dtype > 2) return -1;
+  return int64_t(B) * int64_t(pou_pk(P, dtype).block);
+}
+
+int fvr_pou_pack_blocks(uint64_t F, int32_t dtype, int32_t B, int32_t P, uint64_t packed, void* stream) {
+  FVR_CHECK_ARG(F && packed && B > 0 && P >= 2 && P <= 1024, "fvr_pou_pack_blocks: bad arguments");
+  FVR_CHECK_ARG(int64_t(B) * (P + 3) <= 65535, "fvr_pou_pack_blocks: %d blocks of P = %d exceed the grid", B, P);
+  FVR_CHECK_ARG(pou_pk(P, dtype).plane < (1u << 24), "fvr_pou_pack_blocks: P = %d too large for 24-bit offsets", P);
+  const PouPk k = pou_pk(P, dtype);
+  dim3 grid((P + 2 + 255) / 256, P + 2, B * (P + 3));
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const void* fp = reinterpret_cast<const void*>(F);
+  char* out = reinterpret_cast<char*>(packed);
+  switch (dtype) {
+    case 0: hipLaunchKernelGGL(pou_pack_kernel<0>, grid, dim3(256), 0, st, fp, P, k, out); break;
+    case 1: hipLaunchKernelGGL(pou_pack_kernel<1>, grid, dim3(256), 0, st, fp, P, k, out); break;
+    case 2: hipLaunchKernelGGL(pou_pack_kernel<2>, grid, dim3(256), 0, st, fp, P, k, out); break;
+    default: set_error("fvr_pou_pack_blocks: bad dtype %d", dtype); return FVR_ERR_INVALID;
+  }
+  FVR_CHECK_LAUNCH("pou_pack_kernel");
+  return FVR_OK;
+}
+
 int fvr_pou_slice_ifft(uint64_t F, int32_t dtype, int32_t B, int32_t P, const float* u_xyz, const float* v_xyz,
                        int32_t N, float* imgs, void* stream) {
   FVR_CHECK_ARG(F && u_xyz && v_xyz && imgs && B > 0 && N > 0, "fvr_pou_slice_ifft: bad arguments");
   FVR_CHECK_ARG(fvr_pou_lds_ok(P), "fvr_pou_slice_ifft: P = %d must be a power of two in [4, 128]", P);
-  FVR_CHECK_ARG(int64_t(B) * N < (int64_t(1) << 31), "fvr_pou_slice_ifft: too many (block, pose) pairs");
+  FVR_CHECK_ARG(int64_t(B) * N < (int64_t(1) << 30), "fvr_pou_slice_ifft: too many (block, pose) pairs");
   PouSlice g{B, N, P, 31 - __builtin_clz(uint32_t(P)), float(P / 2)};
+  const PouPk k = pou_pk(P, dtype);
   const size_t lds = slice_lds_bytes(P);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  const void* fp = reinterpret_cast<const void*>(F);
+  const char* fp = reinterpret_cast<const char*>(F);
+  const int per_xcd = int((int64_t(B) * N + 7) / 8);
 #define FVR_SI(VT)                                                                                            \
   FVR_CHECK_ARG(allow_lds(pou_slice_ifft_kernel<VT>, lds), "fvr_pou_slice_ifft: %zu B of LDS refused", lds); \
-  hipLaunchKernelGGL(pou_slice_ifft_kernel<VT>, dim3(B * N), dim3(kPouThreads), lds, st, fp, g, u_xyz, v_xyz, imgs)
+  hipLaunchKernelGGL(pou_slice_ifft_kernel<VT>, dim3(8 * per_xcd), dim3(kPouSliceThreads), lds, st, fp, g, k, \
+                     u_xyz, v_xyz, per_xcd, imgs)
   switch (dtype) {
     case 0: FVR_SI(0); break;
     case 1: FVR_SI(1); break;
@@ -534,14 +700,17 @@ int fvr_pou_render_bwd(uint64_t F, int32_t dtype, int32_t B, int32_t P, const fl
   FVR_CHECK_ARG(fvr_pou_lds_ok(P), "fvr_pou_render_bwd: P = %d must be a power of two in [4, 128]", P);
   PouWarp g;
   if (warp_geom(g, B, N, P, K, per_pose_rays, inv_ds, c_idx) != FVR_OK) return FVR_ERR_INVALID;
+  FVR_CHECK_ARG(int64_t(B) * N < (int64_t(1) << 30), "fvr_pou_render_bwd: too many (block, pose) pairs");
   PouSlice sg{B, N, P, 31 - __builtin_clz(uint32_t(P)), float(P / 2)};
+  const PouPk k = pou_pk(P, dtype);
   const size_t lds = slice_lds_bytes(P) + 9 * kPouThreads * sizeof(float);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  const void* fp = reinterpret_cast<const void*>(F);
+  const char* fp = reinterpret_cast<const char*>(F);
+  const int per_xcd = int((int64_t(B) * N + 7) / 8);
 #define FVR_RB(VT)                                                                                            \
   FVR_CHECK_ARG(allow_lds(pou_render_bwd_kernel<VT>, lds), "fvr_pou_render_bwd: %zu B of LDS refused", lds);  \
-  hipLaunchKernelGGL(pou_render_bwd_kernel<VT>, dim3(B * N), dim3(kPouThreads), lds, st, fp, sg, u_xyz, v_xyz, \
-                     imgs, go, d_b, cu, cv, rx, ry, ic, g, gscale, part)
+  hipLaunchKernelGGL(pou_render_bwd_kernel<VT>, dim3(8 * per_xcd), dim3(kPouThreads), lds, st, fp, sg, k, u_xyz, \
+                     v_xyz, imgs, go, d_b, cu, cv, rx, ry, ic, g, gscale, per_xcd, part)
   switch (dtype) {
     case 0: FVR_RB(0); break;
     case 1: FVR_RB(1); break;

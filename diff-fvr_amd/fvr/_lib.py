@@ -15,7 +15,7 @@ import torch  # noqa: F401  (must be loaded before the HIP library)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libfvr_hip.so")
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 FVR_C32, FVR_C16F, FVR_C16B = 0, 1, 2
 DTYPE_CODES = {"f32": FVR_C32, "fp16": FVR_C16F, "bf16": FVR_C16B}
@@ -85,6 +85,8 @@ _SIGNATURES = {
     "fvr_slice_bwd_workspace": ([_I, _I], ctypes.c_int64),
     "fvr_slice_bwd": ([_U64, _I, _I, _I, _I, _I, _U64, _P, _P, _I, _P, _P, _I, _F, _F, _F, _P, _P, _P, _P], _I),
     "fvr_pou_lds_ok": ([_I], _I),
+    "fvr_pou_packed_bytes": ([_I, _I, _I], ctypes.c_int64),
+    "fvr_pou_pack_blocks": ([_U64, _I, _I, _I, _U64, _P], _I),
     "fvr_pou_slice_ifft": ([_U64, _I, _I, _I, _P, _P, _I, _P, _P], _I),
     "fvr_pou_warp_fwd": ([_P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _I, _I, _F, _F, _P, _P], _I),
     "fvr_pou_gscale": ([_P, _P, _I, _I, _I, _P, _P], _I),

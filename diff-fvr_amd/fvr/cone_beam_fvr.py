@@ -205,10 +205,10 @@ class _PouRender(torch.autograd.Function):
             raise ValueError("PoU render: inconsistent shapes")
         L = _lib.lib()
         st = _lib.stream_ptr(dev)
-        Fb = model.F_blk_ri
+        Fp = model.packed_blocks()
         imgs = torch.empty((B, N, P, P), device=dev, dtype=torch.float32)
-        _lib.check(L.fvr_pou_slice_ifft(Fb.data_ptr(), _VOL_DTYPES[Fb.dtype], B, P, _lib.ptr(u), _lib.ptr(v), N,
-                                        _lib.ptr(imgs), st), "fvr_pou_slice_ifft")
+        _lib.check(L.fvr_pou_slice_ifft(Fp.data_ptr(), _VOL_DTYPES[model.F_blk_ri.dtype], B, P, _lib.ptr(u),
+                                        _lib.ptr(v), N, _lib.ptr(imgs), st), "fvr_pou_slice_ifft")
         out = torch.empty((N, out_h, out_w), device=dev, dtype=torch.float32)
         inv_ds, c_idx = 1.0 / model.ds_block, float(P // 2)
         _lib.check(L.fvr_pou_warp_fwd(_lib.ptr(imgs), B, N, P, _lib.ptr(db), _lib.ptr(c_u), _lib.ptr(c_v),
@@ -232,8 +232,8 @@ class _PouRender(torch.autograd.Function):
         _lib.check(L.fvr_pou_gscale(_lib.ptr(go), _lib.ptr(icf), N, K, per_pose, _lib.ptr(scale), st),
                    "fvr_pou_gscale")
         part = torch.empty((B, N, 9), device=dev, dtype=torch.float32)
-        Fb = model.F_blk_ri
-        _lib.check(L.fvr_pou_render_bwd(Fb.data_ptr(), _VOL_DTYPES[Fb.dtype], B, P, _lib.ptr(u), _lib.ptr(v), N,
+        Fp = model.packed_blocks()
+        _lib.check(L.fvr_pou_render_bwd(Fp.data_ptr(), _VOL_DTYPES[model.F_blk_ri.dtype], B, P, _lib.ptr(u), _lib.ptr(v), N,
                                         _lib.ptr(imgs), _lib.ptr(go), _lib.ptr(db), _lib.ptr(c_u), _lib.ptr(c_v),
                                         _lib.ptr(rxf), _lib.ptr(ryf), _lib.ptr(icf), per_pose, K, inv_ds, c_idx,
                                         _lib.ptr(scale), _lib.ptr(part), st), "fvr_pou_render_bwd")
@@ -401,6 +401,21 @@ class PerspectiveSlabFVR(nn.Module):
     def fused_ok(self) -> bool:
         return bool(self.fused_kernels) and bool(_lib.lib().fvr_pou_lds_ok(self.P))
 
+    def packed_blocks(self) -> torch.Tensor:
+        """The block spectra in the fused kernels' gather layout (interleaved (re, im), zero
+        margins: ``fvr_pou_pack_blocks``), cached and rebuilt whenever ``F_blk_ri`` changes."""
+        Fb = self.F_blk_ri
+        key = (Fb.data_ptr(), Fb.dtype, Fb._version, tuple(Fb.shape))
+        if getattr(self, "_pk_key", None) != key:
+            L = _lib.lib()
+            dt = _VOL_DTYPES[Fb.dtype]
+            Fc = Fb.contiguous()
+            buf = torch.empty(int(L.fvr_pou_packed_bytes(dt, self.B, self.P)), dtype=torch.uint8, device=Fb.device)
+            _lib.check(L.fvr_pou_pack_blocks(Fc.data_ptr(), dt, self.B, self.P, buf.data_ptr(),
+                                             _lib.stream_ptr(Fb.device)), "fvr_pou_pack_blocks")
+            self._pk, self._pk_key = buf, key
+        return self._pk
+
     def _slice_and_ifft(self, u_det: torch.Tensor, v_det: torch.Tensor) -> torch.Tensor:
         """(B, N, P, P) real slab images (cone_beam_fvr.py:480-505)."""
         dev = self.F_blk_ri.device
@@ -408,8 +423,8 @@ class PerspectiveSlabFVR(nn.Module):
         N, B, P = u.shape[0], self.B, self.P
         if self.fused_ok() and not (u.requires_grad or v.requires_grad):
             imgs = torch.empty((B, N, P, P), device=dev, dtype=torch.float32)
-            _lib.check(_lib.lib().fvr_pou_slice_ifft(self.F_blk_ri.data_ptr(), _VOL_DTYPES[self.F_blk_ri.dtype], B,
-                                                     P, _lib.ptr(_f32(u, dev)), _lib.ptr(_f32(v, dev)), N,
+            _lib.check(_lib.lib().fvr_pou_slice_ifft(self.packed_blocks().data_ptr(), _VOL_DTYPES[self.F_blk_ri.dtype],
+                                                     B, P, _lib.ptr(_f32(u, dev)), _lib.ptr(_f32(v, dev)), N,
                                                      _lib.ptr(imgs), _lib.stream_ptr(dev)), "fvr_pou_slice_ifft")
             return imgs
         scale = float(P) * self.ds_block

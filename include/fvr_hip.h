@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define FVR_ABI_VERSION 2
+#define FVR_ABI_VERSION 3
 
 typedef enum fvr_status {
   FVR_OK = 0,
@@ -251,10 +251,20 @@ int fvr_slice_bwd(uint64_t vol, int32_t vol_dtype, int32_t b_vol, int32_t d, int
 /* ------------------------------------------------------------------------ */
 /* 1 when P is a power of two in [4, 128] (the LDS-resident transforms). */
 int fvr_pou_lds_ok(int32_t P);
-/* F: planar block spectra (B, 2, P, P, P) in dtype 0 f32 / 1 f16 / 2 bf16; u, v:
+/* Gather layout of the fused PoU kernels: per block (P+3) x (P+2) x (P+2) elements
+ * (z, y, x; storage index = logical + 1, zero margins, one extra zero plane on the
+ * slow axis), each element the (re, im) pair -- float2 (dtype 0) or two 16-bit
+ * values in a 32-bit word, re in the low half (dtype 1 f16 / 2 bf16).  No
+ * counterpart in the reference (its kernel reads the planar (B, 2, P, P, P) array
+ * of cone_beam_fvr.py:291-294 directly); bytes for B blocks, -1 on bad arguments: */
+int64_t fvr_pou_packed_bytes(int32_t dtype, int32_t B, int32_t P);
+/* planar F (B, 2, P, P, P) of dtype -> packed (caller buffer of fvr_pou_packed_bytes) */
+int fvr_pou_pack_blocks(uint64_t F, int32_t dtype, int32_t B, int32_t P, uint64_t packed, void* stream);
+/* F: packed block spectra (fvr_pou_pack_blocks) of dtype 0 f32 / 1 f16 / 2 bf16; u, v:
  * (N, 3) unit detector axes in (x, y, z) voxel order; sample (i, j) of slice
- * (b, n) sits at P/2 + (j - P/2) u + (i - P/2) v; imgs = real(fftshift(ifft2(
- * ifftshift(slice)))). */
+ * (b, n) sits at P/2 + (j - P/2) u + (i - P/2) v (per-corner zero padding, as
+ * grid_sample); imgs = real(fftshift(ifft2(ifftshift(slice)))).  Replaces
+ * _slice_and_ifft (cone_beam_fvr.py:480-505). */
 int fvr_pou_slice_ifft(uint64_t F, int32_t dtype, int32_t B, int32_t P, const float* u_xyz,
                        const float* v_xyz, int32_t N, float* imgs, void* stream);
 /* out[n][q] = ic[q] * sum_b bilinear(imgs[b][n], (d_b r_x - cu) inv_ds + c_idx,
@@ -276,7 +286,9 @@ int fvr_pou_warp_bwd(const float* imgs, int32_t B, int32_t N, int32_t P, const f
                      float c_idx, const float* gscale, void* workspace, float* grad_imgs, float* part,
                      void* stream);
 /* part[b][n][9] = d/d(d_b, cu, cv), d/du (3), d/dv (3) of sum go * render, for
- * imgs produced by fvr_pou_slice_ifft with the same F, u, v (P as fvr_pou_lds_ok). */
+ * imgs produced by fvr_pou_slice_ifft with the same packed F, u, v (P as
+ * fvr_pou_lds_ok).  Replaces the backward of _PouRender's reference
+ * composition (K4 + FFT adjoint + K2, triton_slice.py:146-271, 410-496). */
 int fvr_pou_render_bwd(uint64_t F, int32_t dtype, int32_t B, int32_t P, const float* u_xyz,
                        const float* v_xyz, int32_t N, const float* imgs, const float* go,
                        const float* d_b, const float* cu, const float* cv, const float* rx,

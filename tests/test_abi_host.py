@@ -93,6 +93,17 @@ def test_entry_points_reject_bad_arguments_without_a_gpu():
     assert b"hyper" in L.fvr_last_error()
 
 
+def test_pou_packed_size_and_argument_checks():
+    from fvr import _lib
+    L = _lib.lib()
+    assert L.fvr_pou_packed_bytes(2, 55, 128) == 55 * 131 * 130 * 130 * 4
+    assert L.fvr_pou_packed_bytes(0, 1, 16) == 19 * 18 * 18 * 8
+    assert L.fvr_pou_packed_bytes(3, 1, 16) == -1 and L.fvr_pou_packed_bytes(1, 0, 16) == -1
+    assert L.fvr_pou_pack_blocks(0, 1, 1, 16, 0, None) == 1
+    p = ctypes.c_void_p(16)
+    assert L.fvr_pou_pack_blocks(16, 1, 600, 128, 16, None) == 1 and b"grid" in L.fvr_last_error()
+
+
 def test_kernel_variant_table():
     """Tile heights 32 / 128 exist for the batched adjoint forms; every flag set has 64."""
     from fvr import _lib

@@ -263,3 +263,33 @@ def test_reference_presets_and_offsets():
     offs = sample_isocenter_parity_offsets((1.0, 2.0, 4.0))
     assert offs.shape == (27, 3)
     assert torch.allclose(offs.abs().max(dim=0).values, torch.tensor([0.5, 1.0, 2.0]))
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float16, torch.bfloat16])
+def test_packed_block_layout_image(dtype):
+    """The fused kernels' gather layout, read back and decoded by the header's definition:
+    per block (P+3) x (P+2) x (P+2) elements, storage index = logical + 1, zero margins,
+    element = (re, im) (float2, or 16-bit re in the low half and im in the high half)."""
+    from fvr import _lib
+    L = _lib.lib()
+    dev = torch.device("cuda", 0)
+    B, P = 3, 8
+    g = torch.Generator().manual_seed(3)
+    F = torch.randn((B, 2, P, P, P), generator=g).to(dtype).to(dev)
+    code = {torch.float32: 0, torch.float16: 1, torch.bfloat16: 2}[dtype]
+    nb = int(L.fvr_pou_packed_bytes(code, B, P))
+    esz = 8 if code == 0 else 4
+    assert nb == B * (P + 3) * (P + 2) * (P + 2) * esz
+    buf = torch.full((nb,), 0x5A, dtype=torch.uint8, device=dev)  # every byte must be written
+    _lib.check(L.fvr_pou_pack_blocks(F.data_ptr(), code, B, P, buf.data_ptr(), _lib.stream_ptr(dev)), "pack")
+    torch.cuda.synchronize()
+    raw = buf.cpu()
+    if code == 0:
+        got = raw.view(torch.float32).view(B, P + 3, P + 2, P + 2, 2)
+    else:
+        got = raw.view(dtype).view(B, P + 3, P + 2, P + 2, 2)
+    exp = torch.zeros((B, P + 3, P + 2, P + 2, 2), dtype=dtype)
+    Fc = F.cpu()
+    exp[:, 1:P + 1, 1:P + 1, 1:P + 1, 0] = Fc[:, 0]
+    exp[:, 1:P + 1, 1:P + 1, 1:P + 1, 1] = Fc[:, 1]
+    assert torch.equal(got, exp)
