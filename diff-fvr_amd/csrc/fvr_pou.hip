@@ -30,7 +30,10 @@ namespace fvr {
 
 constexpr int kPouThreads = 512;
 constexpr int kPouSliceThreads = 1024;  // the forward slice + iFFT (no reduction buffer in LDS)
-constexpr int kPouBatch = 4;            // slice samples in flight per thread during a gather
+#ifndef FVR_POU_BATCH
+#define FVR_POU_BATCH 4
+#endif
+constexpr int kPouBatch = FVR_POU_BATCH;  // slice samples in flight per thread during a gather
 constexpr int kPouMaxLogP = 7;          // LDS-resident transforms up to 128 x 128
 
 typedef float pf4a8 __attribute__((ext_vector_type(4), aligned(8)));
@@ -312,53 +315,93 @@ __global__ __launch_bounds__(256) void pou_warp_fwd_kernel(const float* __restri
   out[size_t(n) * g.K + q] = acc * ic[ro];
 }
 
-// fixed-order workgroup sum of NV per-thread values (kPouThreads threads)
+// fixed-order workgroup sum of NV per-thread values: an XOR butterfly inside every wave
+// (lane 0's result is used), then the per-wave sums added in wave order by every thread;
+// red holds NV x (waves per workgroup) floats
+constexpr int kPouMaxWaves = 16;
 template <int NV>
-__device__ __forceinline__ void block_sum(float (&v)[NV], float* red /* NV * kPouThreads */) {
-  for (int k = 0; k < NV; ++k) red[k * kPouThreads + threadIdx.x] = v[k];
+__device__ __forceinline__ void block_sum(float (&v)[NV], float* red) {
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o);
+  const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < NV; ++k) red[k * nw + wave] = v[k];
   __syncthreads();
-  for (int st = kPouThreads / 2; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st)
-      for (int k = 0; k < NV; ++k) red[k * kPouThreads + threadIdx.x] += red[k * kPouThreads + threadIdx.x + st];
-    __syncthreads();
+  for (int k = 0; k < NV; ++k) {
+    float t = 0.f;
+    for (int w = 0; w < nw; ++w) t += red[k * nw + w];
+    v[k] = t;
   }
-  for (int k = 0; k < NV; ++k) v[k] = red[k * kPouThreads];
   __syncthreads();
 }
 
 // warp adjoint of one (block, pose): geometry sums and (optionally) the image
 // gradient as 64-bit fixed point, at index map(y, x) of `acc` (LDS or global)
+#ifndef FVR_POU_SCAN_BATCH
+#define FVR_POU_SCAN_BATCH 4
+#endif
+constexpr int kScanBatch = FVR_POU_SCAN_BATCH;  // detector pixels in flight per thread
+
 template <bool IMG>
 __device__ __forceinline__ void warp_adjoint_scan(const float* img, const float* go, const float* rx,
                                                   const float* ry, const float* ic, const PouWarp& g, int n, float db,
                                                   float c_u, float c_v, float scale, unsigned long long* acc,
                                                   int acc_pitch, bool roll, float (&s)[3]) {
-  const int P = g.P, hp = P / 2;
-  for (int q = threadIdx.x; q < g.K; q += blockDim.x) {
-    const size_t ro = size_t(n) * g.ray_stride + q;
-    const float r_x = rx[ro], r_y = ry[ro];
-    const float gg = go[size_t(n) * g.K + q] * ic[ro];
-    const float px = (db * r_x - c_u) * g.inv_ds + g.c_idx;
-    const float py = (db * r_y - c_v) * g.inv_ds + g.c_idx;
-    const float x0f = floorf(px), y0f = floorf(py);
-    const float tx = px - x0f, ty = py - y0f, ox = 1.f - tx, oy = 1.f - ty;
-    const int x0 = int(x0f), y0 = int(y0f);
-    const float v00 = img_at(img, P, x0, y0), v01 = img_at(img, P, x0 + 1, y0);
-    const float v10 = img_at(img, P, x0, y0 + 1), v11 = img_at(img, P, x0 + 1, y0 + 1);
-    const float dsx = ((v01 - v00) * oy + (v11 - v10) * ty) * g.inv_ds;
-    const float dsy = ((v10 - v00) * ox + (v11 - v01) * tx) * g.inv_ds;
-    s[0] = fmaf(gg, dsx * r_x + dsy * r_y, s[0]);
-    s[1] = fmaf(-gg, dsx, s[1]);
-    s[2] = fmaf(-gg, dsy, s[2]);
-    if (IMG && gg != 0.f) {
+  const int P = g.P, hp = P / 2, K = g.K, stride = blockDim.x;
+  const int pp1 = P * P - 1;
+  for (int q0 = threadIdx.x; q0 < K; q0 += kScanBatch * stride) {
+    // 1) the pixels' rays and output gradients
+    float r_x[kScanBatch], r_y[kScanBatch], gg[kScanBatch];
+#pragma unroll
+    for (int t = 0; t < kScanBatch; ++t) {
+      const int q = min(q0 + t * stride, K - 1);
+      const size_t ro = size_t(n) * g.ray_stride + q;
+      r_x[t] = rx[ro];
+      r_y[t] = ry[ro];
+      gg[t] = q0 + t * stride < K ? go[size_t(n) * K + q] * ic[ro] : 0.f;
+    }
+    // 2) their four image corners (clamped addresses; out-of-range corners count 0)
+    float tx[kScanBatch], ty[kScanBatch], vv[kScanBatch][4];
+    int x0[kScanBatch], y0[kScanBatch];
+#pragma unroll
+    for (int t = 0; t < kScanBatch; ++t) {
+      const float px = (db * r_x[t] - c_u) * g.inv_ds + g.c_idx;
+      const float py = (db * r_y[t] - c_v) * g.inv_ds + g.c_idx;
+      const float x0f = floorf(px), y0f = floorf(py);
+      tx[t] = px - x0f;
+      ty[t] = py - y0f;
+      x0[t] = int(x0f);
+      y0[t] = int(y0f);
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        const int x = x0 + (c & 1), y = y0 + (c >> 1);
-        if ((unsigned)x < (unsigned)P && (unsigned)y < (unsigned)P) {
-          const float w = ((c & 1) ? tx : ox) * ((c >> 1) ? ty : oy);
-          const long long fx = llrintf(gg * w * scale);
-          const int yy = roll ? ((y + hp) & (P - 1)) : y, xx = roll ? ((x + hp) & (P - 1)) : x;
-          atomicAdd(acc + size_t(yy) * acc_pitch + xx, (unsigned long long)fx);
+        const int x = x0[t] + (c & 1), y = y0[t] + (c >> 1);
+        const bool in = (unsigned)x < (unsigned)P && (unsigned)y < (unsigned)P;
+        const float val = img[min(max(y * P + x, 0), pp1)];
+        vv[t][c] = in ? val : 0.f;
+      }
+    }
+    // 3) geometry sums and the image-gradient scatter
+#pragma unroll
+    for (int t = 0; t < kScanBatch; ++t) {
+      const float ox = 1.f - tx[t], oy = 1.f - ty[t];
+      const float v00 = vv[t][0], v01 = vv[t][1], v10 = vv[t][2], v11 = vv[t][3];
+      const float dsx = ((v01 - v00) * oy + (v11 - v10) * ty[t]) * g.inv_ds;
+      const float dsy = ((v10 - v00) * ox + (v11 - v01) * tx[t]) * g.inv_ds;
+      s[0] = fmaf(gg[t], dsx * r_x[t] + dsy * r_y[t], s[0]);
+      s[1] = fmaf(-gg[t], dsx, s[1]);
+      s[2] = fmaf(-gg[t], dsy, s[2]);
+      if (IMG && gg[t] != 0.f) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int x = x0[t] + (c & 1), y = y0[t] + (c >> 1);
+          if ((unsigned)x < (unsigned)P && (unsigned)y < (unsigned)P) {
+            const float w = ((c & 1) ? tx[t] : ox) * ((c >> 1) ? ty[t] : oy);
+            const long long fx = llrintf(gg[t] * w * scale);
+            const int yy = roll ? ((y + hp) & (P - 1)) : y, xx = roll ? ((x + hp) & (P - 1)) : x;
+            atomicAdd(acc + size_t(yy) * acc_pitch + xx, (unsigned long long)fx);
+          }
         }
       }
     }
@@ -370,7 +413,7 @@ __global__ __launch_bounds__(kPouThreads) void pou_warp_bwd_kernel(
     const float* __restrict__ cu, const float* __restrict__ cv, const float* __restrict__ rx,
     const float* __restrict__ ry, const float* __restrict__ ic, PouWarp g, const float* __restrict__ gscale,
     unsigned long long* __restrict__ gacc, float* __restrict__ part) {
-  __shared__ float red[3 * kPouThreads];
+  __shared__ float red[3 * kPouMaxWaves];
   const int bn = blockIdx.x;
   const int n = bn % g.N;
   const size_t pp = size_t(g.P) * g.P;
@@ -395,7 +438,7 @@ __global__ void pou_fix_to_float_kernel(unsigned long long* __restrict__ acc, co
 }
 
 template <int VT>
-__global__ __launch_bounds__(kPouThreads) void pou_render_bwd_kernel(
+__global__ __launch_bounds__(kPouSliceThreads) void pou_render_bwd_kernel(
     const char* __restrict__ Fp, PouSlice sg, PouPk k, const float* __restrict__ u, const float* __restrict__ v,
     const float* __restrict__ imgs, const float* __restrict__ go, const float* __restrict__ d_b,
     const float* __restrict__ cu, const float* __restrict__ cv, const float* __restrict__ rx,
@@ -405,7 +448,7 @@ __global__ __launch_bounds__(kPouThreads) void pou_render_bwd_kernel(
   const int P = sg.P, logP = sg.logP, pitch = P + 1, hp = P / 2;
   float2* a = smem;  // P x pitch complex, first used as 64-bit fixed-point accumulators
   float2* tw = smem + P * pitch;
-  float* red = reinterpret_cast<float*>(tw + P / 2);  // 9 * kPouThreads floats
+  float* red = reinterpret_cast<float*>(tw + P / 2);  // 9 * kPouMaxWaves floats
   unsigned long long* acc = reinterpret_cast<unsigned long long*>(a);
   const int bn = pou_bn(blockIdx.x, per_xcd);
   if (bn >= sg.B * sg.N) return;  // whole workgroup: padding slot of the XCD split
@@ -576,7 +619,7 @@ extern "C" {
 
 int fvr_pou_lds_ok(int32_t P) {
   if (P < 4 || P > (1 << kPouMaxLogP) || (P & (P - 1))) return 0;
-  return slice_lds_bytes(P) + 9 * kPouThreads * sizeof(float) <= 160 * 1024 ? 1 : 0;
+  return slice_lds_bytes(P) + 9 * kPouMaxWaves * sizeof(float) <= 160 * 1024 ? 1 : 0;
 }
 
 int64_t fvr_pou_packed_bytes(int32_t dtype, int32_t B, int32_t P) {
@@ -703,13 +746,13 @@ int fvr_pou_render_bwd(uint64_t F, int32_t dtype, int32_t B, int32_t P, const fl
   FVR_CHECK_ARG(int64_t(B) * N < (int64_t(1) << 30), "fvr_pou_render_bwd: too many (block, pose) pairs");
   PouSlice sg{B, N, P, 31 - __builtin_clz(uint32_t(P)), float(P / 2)};
   const PouPk k = pou_pk(P, dtype);
-  const size_t lds = slice_lds_bytes(P) + 9 * kPouThreads * sizeof(float);
+  const size_t lds = slice_lds_bytes(P) + 9 * kPouMaxWaves * sizeof(float);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   const char* fp = reinterpret_cast<const char*>(F);
   const int per_xcd = int((int64_t(B) * N + 7) / 8);
 #define FVR_RB(VT)                                                                                            \
   FVR_CHECK_ARG(allow_lds(pou_render_bwd_kernel<VT>, lds), "fvr_pou_render_bwd: %zu B of LDS refused", lds);  \
-  hipLaunchKernelGGL(pou_render_bwd_kernel<VT>, dim3(8 * per_xcd), dim3(kPouThreads), lds, st, fp, sg, k, u_xyz, \
+  hipLaunchKernelGGL(pou_render_bwd_kernel<VT>, dim3(8 * per_xcd), dim3(kPouSliceThreads), lds, st, fp, sg, k, u_xyz, \
                      v_xyz, imgs, go, d_b, cu, cv, rx, ry, ic, g, gscale, per_xcd, part)
   switch (dtype) {
     case 0: FVR_RB(0); break;
