@@ -8,14 +8,18 @@ views (512^2 slice samples each), 64 candidate poses (rotations U(-15,15) deg,
 translations U(-20,20) mm) per launch, against AP/LAT target spectra of 256^2
 DRRs rendered at a pose with rotations and translations U(-5,5).
 
-One step = three stream-ordered launches for the whole batch: float64 pose
-set-up on the device (items + Jacobians), one fused HIP launch over all 2x64
-(pose, view) items, and the per-pose fixed-order float64 reduction + biplanar
-combine (spectrum, targets and poses resident in HBM; loss and gradient left
-in HBM, no host sync inside a step).  `value` is
+One step = two stream-ordered launches for the whole batch: one fused HIP
+launch over all 2x64 (pose, view) items, then one launch that does the
+per-pose fixed-order float64 reduction + biplanar combine (loss and exact
+gradient) and the float64 set-up of the items for the next step (spectrum,
+targets and poses resident in HBM; loss and gradient left in HBM, no host sync
+inside a step); the first step's items are built before the timed region.  The
+three-launch form and a HIP-graph replay of the step are reported beside it.
+The launch plan of the fused kernel is autotuned first.  `value` is
 pose evaluations per second of the batched FP16-spectrum form (FP32
-interpolation and sums); the FP32-spectrum batched form and the plain
-one-pose-per-launch FP32 form are reported beside it.
+interpolation and sums); the FP32-spectrum batched form, the plain
+one-pose-per-launch FP32 form, the PoU-FVR perspective workload, the
+convention-T engine and the ragged multistart loop are reported beside it.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -65,21 +69,100 @@ def make_targets(renderer, true_pose):
 
 
 def time_batched(ev, P, steps: int, warmup: int):
-    """Wall time per step (device pose set-up + fused kernel + per-pose reduction and
-    combine; poses resident in HBM, loss and gradient left in HBM) and the fused kernel's
-    mean duration from HIP events on its launch stream."""
+    """Wall time per step -- the fused kernel over the batch's items, then fvr_pn_step
+    (per-pose reduction + combine -> loss and gradient, and the float64 set-up of the
+    items for the next step); poses resident in HBM, loss and gradient left in HBM --
+    and the fused kernel's mean duration from HIP events on its launch stream."""
     import torch
+    ev.begin(P)
     for _ in range(warmup):
-        ev.evaluate(P)
+        ev.step(P)
     torch.cuda.synchronize()
     ks = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     t0 = time.perf_counter()
     for k in range(steps):
-        ev.evaluate(P, kernel_events=ks[k])
+        ev.step(P, kernel_events=ks[k])
     torch.cuda.synchronize()
     total_ms = (time.perf_counter() - t0) * 1e3
     kern_ms = [a.elapsed_time(b) for a, b in ks]
     return total_ms / steps, statistics.mean(kern_ms), kern_ms
+
+
+def time_three(ev, P, steps: int, warmup: int):
+    """Per-step time of the three-launch form (set-up, fused kernel, reduce + combine)."""
+    import torch
+    for _ in range(warmup):
+        ev.evaluate_device(P, want_moments=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ev.evaluate_device(P, want_moments=False)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def time_graph(ev, P, steps: int, warmup: int):
+    """Per-step time of the same two launches captured once as a HIP graph and replayed;
+    returns ms per step."""
+    import torch
+    ev.begin(P)
+    ev.step(P)  # sizes the workspace and the tile map
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        ev.step(P)
+    for _ in range(warmup):
+        g.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.replay()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def torch_engine_t(reps: int = 30, warmup: int = 5):
+    """The reference's own GPU benchmark workload for TorchDiffFVREngine.forward_spectral_ncc
+    (experiments/exp29_gpu_triton_and_ablation.py:278-286, 391-393: 128^3 volume padded 2x,
+    BF16 spectrum, out_res 256, B = 64 poses from randn, a random 256^2 target spectrum with
+    the high-pass filter), on a seeded phantom: forward only and forward + backward."""
+    import torch
+    from fvr import TorchDiffFVREngine
+    from fvr.phantom import ellipsoid_phantom
+    dev = torch.device("cuda", 0)
+    vol = ellipsoid_phantom(128, n_ellipsoids=12, seed=1234)
+    eng = TorchDiffFVREngine(vol, pad_factor=2, device=dev)
+    eng.to_precision(torch.bfloat16)
+    g = torch.Generator(device="cpu").manual_seed(3456)
+    B, H = 64, 256
+    poses = torch.randn(B, 6, generator=g).to(dev)
+    f_targ = eng.prepare_spectral_target(torch.randn(1, 1, H, H, generator=g).to(dev))
+
+    def fwd():
+        with torch.no_grad():
+            return eng.forward_spectral_ncc(poses, f_targ)
+
+    def fwd_bwd():
+        p = poses.clone().requires_grad_(True)
+        loss = eng.forward_spectral_ncc(p, f_targ)
+        loss.backward()
+        return loss
+
+    out = {}
+    for name, fn in (("fwd", fwd), ("fwd_bwd", fwd_bwd)):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        out[f"ms_{name}"] = round(ms, 4)
+        out[f"pose_views_per_s_{name}"] = round(B / (ms * 1e-3), 1)
+    return {"workload": "TorchDiffFVREngine.forward_spectral_ncc, 128^3 -> 256^3 bf16 spectrum, out_res 256, B=64, "
+                        "high-pass, one view", **out,
+            "reference_a100_fwd_pose_views_per_s": 49454.0}
 
 
 def time_plain(ev, P, reps: int = 2):
@@ -281,6 +364,7 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-pou", action="store_true", help="skip the PoU-FVR perspective workload")
     ap.add_argument("--no-multistart", action="store_true", help="skip the ragged multistart workload")
+    ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replay timing")
     ap.add_argument("--plan", default="auto",
                     help="fused-kernel launch plan 'tile_h/order/persist' (e.g. 64/target/0) or 'auto' (autotune)")
     ap.add_argument("--no-footprint", action="store_true",
@@ -321,6 +405,8 @@ def main(argv=None) -> int:
             plan = LaunchPlan(int(th), order, int(persist))
             ev.set_plan(plan)
         step_ms, kern_ms, _ = time_batched(ev, P, args.steps, args.warmup)
+        graph_ms = time_graph(ev, P, args.steps, args.warmup) if not args.no_graph else None
+        three_ms = time_three(ev, P, args.steps, args.warmup)
         loss, grad = ev.evaluate(cands)
         # NumPy poses in, NumPy loss / gradient / moments out: the host-buffer boundary (PCIe copies included)
         torch.cuda.synchronize()
@@ -330,7 +416,8 @@ def main(argv=None) -> int:
         host_ms = (time.perf_counter() - t_h) * 1e3 / 10
         results[st] = {"ms_per_step": step_ms, "kernel_ms": kern_ms, "value": args.n_cand / (step_ms * 1e-3),
                        "loss_mean": float(loss.mean()), "loss": loss, "grad": grad, "herm": ev.herm,
-                       "plan": plan, "autotune": timings, "host_numpy_ms_per_step": host_ms}
+                       "plan": plan, "autotune": timings, "host_numpy_ms_per_step": host_ms,
+                       "three_launch_ms_per_step": three_ms, "graph_ms_per_step": graph_ms}
         log(f"{st}: plan {plan}: {step_ms:.3f} ms/step, fused kernel {kern_ms:.3f} ms, "
             f"{results[st]['value']:.0f} pose evals/s")
         del ev
@@ -377,11 +464,19 @@ def main(argv=None) -> int:
         torch.cuda.empty_cache()
         extra["pou"] = pou_config4(max(3, args.steps // 10), 2)
         log(f"PoU: {extra['pou']}")
+    if not args.no_extra:
+        log("convention-T engine on the reference's own GPU benchmark workload")
+        torch.cuda.empty_cache()
+        extra["torch_engine_t"] = torch_engine_t()
+        log(f"torch engine: {extra['torch_engine_t']}")
     if not args.no_extra and not args.no_multistart:
         log("ragged multistart workload (config 5)")
         torch.cuda.empty_cache()
         extra["multistart"] = multistart_config5()
         log(f"multistart: {extra['multistart']}")
+    extra["three_launch_ms_per_step"] = round(head["three_launch_ms_per_step"], 4)
+    if head["graph_ms_per_step"] is not None:
+        extra["graph_replay_ms_per_step"] = round(head["graph_ms_per_step"], 4)
     extra["host_numpy_ms_per_step"] = round(head["host_numpy_ms_per_step"], 4)
     extra["host_numpy_pose_evals_per_s"] = round(args.n_cand / (head["host_numpy_ms_per_step"] * 1e-3), 1)
     if plain_ms is not None:

@@ -54,15 +54,12 @@ __device__ __forceinline__ void put_jac(double* J, const V3& d0, const V3& d1, c
   J[8] = f2;
 }
 
-__global__ void pn_build_kernel(const double* __restrict__ poses, int n_poses, fvr_sncc_item tap, fvr_sncc_item tlat,
-                                double pz, double py, double px, int lat_swap, int tile0,
-                                fvr_sncc_item* __restrict__ items, double* __restrict__ jac) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= 2 * n_poses) return;
-  const int pose = idx >> 1;
-  const bool lat = idx & 1;
+// item idx = 2 pose + lat (AP, LAT) of one pose and its Jacobians
+__device__ __forceinline__ void build_item(const double* P, int pose, bool lat, const fvr_sncc_item& tap, const fvr_sncc_item& tlat,
+                           double pz, double py, double px, int lat_swap, int tile0, fvr_sncc_item* items,
+                           double* jac) {
+  const int idx = 2 * pose + (lat ? 1 : 0);
   const double deg = 3.14159265358979323846 / 180.0;
-  const double* P = poses + 6 * pose;
   const double y = P[0] * deg, p = P[1] * deg, r = P[2] * deg;
   const double cy = cos(y), sy = sin(y), cp = cos(p), sp = sin(p), cr = cos(r), sr = sin(r);
   // columns of R = Rz(y) Ry(r) Rx(p): R e_x (col 2), R e_y (col 1), R e_z (col 0)
@@ -115,15 +112,20 @@ __global__ void pn_build_kernel(const double* __restrict__ poses, int n_poses, f
           v3(dr0.a0 * k, dr0.a1 * k, dr0.a2 * k), swap);
 }
 
+__global__ void pn_build_kernel(const double* __restrict__ poses, int n_poses, fvr_sncc_item tap, fvr_sncc_item tlat,
+                                double pz, double py, double px, int lat_swap, int tile0,
+                                fvr_sncc_item* __restrict__ items, double* __restrict__ jac) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= 2 * n_poses) return;
+  build_item(poses + 6 * (idx >> 1), idx >> 1, idx & 1, tap, tlat, pz, py, px, lat_swap, tile0, items, jac);
+}
+
 // One wave per pose: fixed-order float64 reduction of each view's tile
 // partials (lane-strided partial sums, then an XOR butterfly that every lane
 // evaluates in the same order), then the biplanar loss and exact gradient.
-__global__ __launch_bounds__(64) void pn_reduce_combine_kernel(const fvr_sncc_item* __restrict__ items, int n_poses,
-                                                               const float* __restrict__ partials,
-                                                               const double* __restrict__ jac, double t2_ap,
-                                                               double t2_lat, double* __restrict__ moments,
-                                                               double* __restrict__ loss, double* __restrict__ grad) {
-  const int pose = blockIdx.x;
+__device__ void reduce_combine_pose(const fvr_sncc_item* items, int pose, const float* __restrict__ partials,
+                                    const double* jac, double t2_ap, double t2_lat, double* moments, double& L_out,
+                                    double (&g_out)[6]) {
   const int lane = threadIdx.x;
   const double two_pi = 6.283185307179586476925286766559;
   double g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -180,6 +182,13 @@ __global__ __launch_bounds__(64) void pn_reduce_combine_kernel(const fvr_sncc_it
     else g[4] -= 0.5 * da;
     g[5] -= 0.5 * db;
   }
+  L_out = L;
+#pragma unroll
+  for (int a = 0; a < 6; ++a) g_out[a] = g[a];
+}
+
+__device__ __forceinline__ void store_loss_grad(int pose, double L, const double (&g)[6], double* loss, double* grad) {
+  const int lane = threadIdx.x;
   if (lane == 0) loss[pose] = L;
   if (lane < 6) {
     double gv = 0.0;
@@ -189,22 +198,42 @@ __global__ __launch_bounds__(64) void pn_reduce_combine_kernel(const fvr_sncc_it
   }
 }
 
+__global__ __launch_bounds__(64) void pn_reduce_combine_kernel(const fvr_sncc_item* __restrict__ items, int n_poses,
+                                                               const float* __restrict__ partials,
+                                                               const double* __restrict__ jac, double t2_ap,
+                                                               double t2_lat, double* __restrict__ moments,
+                                                               double* __restrict__ loss, double* __restrict__ grad) {
+  const int pose = blockIdx.x;
+  double L, g[6];
+  reduce_combine_pose(items, pose, partials, jac, t2_ap, t2_lat, moments, L, g);
+  store_loss_grad(pose, L, g, loss, grad);
+}
+
 // Batched Adam, one thread per pose (gradient_optimizer.py:45-83 per pose, with
 // the gradient clamp and cosine learning-rate factor of the multistart driver
 // of experiments/exp30_gpu_multistart_registration.py:140-183).
-__global__ void adam_step_kernel(double* __restrict__ poses, const double* __restrict__ loss,
-                                 const double* __restrict__ grad, double* __restrict__ m, double* __restrict__ v,
-                                 int32_t* __restrict__ t, int32_t* __restrict__ done, double* __restrict__ best_loss,
-                                 double* __restrict__ best_pose, int n, fvr_adam_cfg c) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n || done[k]) return;
-  const int step = t[k] + 1;
-  t[k] = step;
-  double* P = poses + 6 * k;
-  const double L = loss[k];
-  if (L < best_loss[k]) {
-    best_loss[k] = L;
-    for (int a = 0; a < 6; ++a) best_pose[6 * k + a] = P[a];
+struct AdamState {
+  double *m, *v;
+  int32_t *t, *done;
+  double *best_loss, *best_pose;
+};
+
+// One Adam step of pose k: P (its 6 values) is updated in place; every calling lane
+// reads the same state and computes the same update, and only a lane with `store`
+// writes the state back (all reads precede all writes).
+__device__ __forceinline__ void adam_pose(int k, double (&P)[6], double L, const double* gk, const AdamState& S,
+                                          const fvr_adam_cfg& c, bool store) {
+  // no implicit FMA contraction: every kernel that inlines this computes the same bits
+#pragma clang fp contract(off)
+  if (S.done[k]) return;
+  const int step = S.t[k] + 1;
+  const bool better = L < S.best_loss[k];
+  double mo[6], vo[6], P0[6];
+#pragma unroll
+  for (int a = 0; a < 6; ++a) {
+    mo[a] = S.m[6 * k + a];
+    vo[a] = S.v[6 * k + a];
+    P0[a] = P[a];
   }
   double f = 1.0;
   if (c.cosine_t > 0) {
@@ -213,18 +242,70 @@ __global__ void adam_step_kernel(double* __restrict__ poses, const double* __res
   }
   const double bc1 = 1.0 - pow(c.beta1, double(step)), bc2 = 1.0 - pow(c.beta2, double(step));
   double mx = 0.0;
+#pragma unroll
   for (int a = 0; a < 6; ++a) {
-    double g = grad[6 * k + a];
+    double g = gk[a];
     if (c.gclip[a] > 0.0) g = fmin(fmax(g, -c.gclip[a]), c.gclip[a]);
-    const double mm = c.beta1 * m[6 * k + a] + (1.0 - c.beta1) * g;
-    const double vv = c.beta2 * v[6 * k + a] + (1.0 - c.beta2) * g * g;
-    m[6 * k + a] = mm;
-    v[6 * k + a] = vv;
-    const double upd = f * c.lr[a] * (mm / bc1) / (sqrt(vv / bc2) + c.eps);
+    mo[a] = fma(c.beta1, mo[a], (1.0 - c.beta1) * g);
+    vo[a] = fma(c.beta2, vo[a], ((1.0 - c.beta2) * g) * g);
+    const double upd = f * c.lr[a] * (mo[a] / bc1) / (sqrt(vo[a] / bc2) + c.eps);
     mx = fmax(mx, fabs(upd));
     P[a] = fmin(fmax(P[a] - upd, c.lo[a]), c.hi[a]);
   }
-  if (mx < c.tol) done[k] = 1;
+  if (store) {
+    S.t[k] = step;
+    if (better) {
+      S.best_loss[k] = L;
+#pragma unroll
+      for (int a = 0; a < 6; ++a) S.best_pose[6 * k + a] = P0[a];
+    }
+#pragma unroll
+    for (int a = 0; a < 6; ++a) {
+      S.m[6 * k + a] = mo[a];
+      S.v[6 * k + a] = vo[a];
+    }
+    if (mx < c.tol) S.done[k] = 1;
+  }
+}
+
+__global__ void adam_step_kernel(double* __restrict__ poses, const double* __restrict__ loss,
+                                 const double* __restrict__ grad, AdamState S, int n, fvr_adam_cfg c) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double P[6];
+#pragma unroll
+  for (int a = 0; a < 6; ++a) P[a] = poses[6 * k + a];
+  adam_pose(k, P, loss[k], grad + 6 * k, S, c, true);
+#pragma unroll
+  for (int a = 0; a < 6; ++a) poses[6 * k + a] = P[a];
+}
+
+// One wave per pose after the fused kernel: reduce + combine (loss, gradient at the
+// pose the items were built from), then optionally one Adam step of the pose, then the
+// pose's two items and Jacobians for the next fused launch.  A registration step is
+// then two launches: the fused kernel and this one.
+__global__ __launch_bounds__(64) void pn_step_kernel(fvr_sncc_item* items, int n_poses, const float* __restrict__ partials,
+                                                     double* jac, double t2_ap, double t2_lat,
+                                                     double* __restrict__ moments, double* __restrict__ loss,
+                                                     double* __restrict__ grad, double* __restrict__ poses,
+                                                     int use_adam, fvr_adam_cfg c, AdamState S, fvr_sncc_item tap,
+                                                     fvr_sncc_item tlat, double pz, double py, double px,
+                                                     int lat_swap, int tile0) {
+  const int pose = blockIdx.x;
+  double L, g[6];
+  reduce_combine_pose(items, pose, partials, jac, t2_ap, t2_lat, moments, L, g);  // every lane: same L, g
+  store_loss_grad(pose, L, g, loss, grad);
+  double P[6];
+#pragma unroll
+  for (int a = 0; a < 6; ++a) P[a] = poses[6 * pose + a];
+  __syncthreads();  // every lane has read this pose's items and Jacobians before they are rebuilt
+  if (use_adam) {
+    adam_pose(pose, P, L, g, S, c, threadIdx.x == 0);  // lanes agree; lane 0 stores the state
+    if (threadIdx.x == 0)
+#pragma unroll
+      for (int a = 0; a < 6; ++a) poses[6 * pose + a] = P[a];
+  }
+  if (threadIdx.x < 2) build_item(P, pose, threadIdx.x == 1, tap, tlat, pz, py, px, lat_swap, tile0, items, jac);
 }
 
 }  // namespace fvr
@@ -265,6 +346,39 @@ int fvr_pn_reduce_combine(const fvr_sncc_item* items_dev, int32_t n_poses, const
   return FVR_OK;
 }
 
+int fvr_pn_step(fvr_sncc_item* items_dev, int32_t n_poses, const float* partials, double* jac_dev, double t2_ap,
+                double t2_lat, double* moments, double* loss, double* grad, double* poses,
+                const fvr_adam_cfg* adam_cfg_host, double* m, double* v, int32_t* t, int32_t* done, double* best_loss,
+                double* best_pose, const fvr_sncc_item* tmpl_host, double pz, double py, double px, int32_t lat_swap,
+                int32_t tile0, void* stream) {
+  FVR_CHECK_ARG(items_dev && partials && jac_dev && loss && grad && poses && tmpl_host && n_poses > 0 && tile0 >= 0,
+                "fvr_pn_step: bad arguments");
+  FVR_CHECK_ARG(!adam_cfg_host || (m && v && t && done && best_loss && best_pose), "fvr_pn_step: Adam state missing");
+  FVR_CHECK_ARG(tmpl_host[0].w == int(px) && tmpl_host[0].h == int(pz) && tmpl_host[1].h == int(pz) &&
+                    tmpl_host[1].w == int(py),
+                "fvr_pn_step: templates do not match the padded dims");
+  FVR_CHECK_ARG(sncc_tile_h_ok(tmpl_host[0].tile_h) && tmpl_host[1].tile_h == tmpl_host[0].tile_h,
+                "fvr_pn_step: bad template tile_h");
+  const int64_t t_pose =
+      int64_t(tmpl_host[0].tiles_x) * sncc_tiles_y(tmpl_host[0].h, tmpl_host[0].flags, tmpl_host[0].tile_h) +
+      int64_t(tmpl_host[1].tiles_x) * sncc_tiles_y(tmpl_host[1].h, tmpl_host[1].flags, tmpl_host[1].tile_h);
+  FVR_CHECK_ARG(t_pose > 0 && int64_t(tile0) + int64_t(n_poses) * t_pose < (int64_t(1) << 31),
+                "fvr_pn_step: tile index overflow");
+  fvr_adam_cfg c = {};
+  if (adam_cfg_host) {
+    c = *adam_cfg_host;
+    FVR_CHECK_ARG(c.beta1 >= 0.0 && c.beta1 < 1.0 && c.beta2 >= 0.0 && c.beta2 < 1.0 && c.eps >= 0.0 &&
+                      c.cosine_t >= 0,
+                  "fvr_pn_step: bad Adam hyper-parameters");
+  }
+  const AdamState S{m, v, t, done, best_loss, best_pose};
+  hipLaunchKernelGGL(pn_step_kernel, dim3(n_poses), dim3(64), 0, reinterpret_cast<hipStream_t>(stream), items_dev,
+                     n_poses, partials, jac_dev, t2_ap, t2_lat, moments, loss, grad, poses, adam_cfg_host ? 1 : 0, c,
+                     S, tmpl_host[0], tmpl_host[1], pz, py, px, int(lat_swap), int(tile0));
+  FVR_CHECK_LAUNCH("pn_step_kernel");
+  return FVR_OK;
+}
+
 int fvr_adam_step(double* poses, const double* loss, const double* grad, double* m, double* v, int32_t* t,
                   int32_t* done, double* best_loss, double* best_pose, int32_t n, const fvr_adam_cfg* cfg_host,
                   void* stream) {
@@ -274,8 +388,9 @@ int fvr_adam_step(double* poses, const double* loss, const double* grad, double*
   FVR_CHECK_ARG(c.beta1 >= 0.0 && c.beta1 < 1.0 && c.beta2 >= 0.0 && c.beta2 < 1.0 && c.eps >= 0.0 &&
                     c.cosine_t >= 0,
                 "fvr_adam_step: bad hyper-parameters");
+  const AdamState S{m, v, t, done, best_loss, best_pose};
   hipLaunchKernelGGL(adam_step_kernel, dim3((n + 63) / 64), dim3(64), 0, reinterpret_cast<hipStream_t>(stream), poses,
-                     loss, grad, m, v, t, done, best_loss, best_pose, int(n), c);
+                     loss, grad, S, int(n), c);
   FVR_CHECK_LAUNCH("adam_step_kernel");
   return FVR_OK;
 }

@@ -7,9 +7,11 @@ the reference's multistart driver
 (``experiments/exp30_gpu_multistart_registration.py:140-183``).
 
 ``AdamLoop`` runs ``steps`` x (loss + gradient of every pose, Adam step) over
-a ``BiplanarEvaluator`` or a ``RaggedBiplanarEvaluator``: eagerly, or captured
-once into a HIP graph (``torch.cuda.CUDAGraph``) and replayed, so that a
-100-step registration of hundreds of poses is one graph launch.
+a ``BiplanarEvaluator`` or a ``RaggedBiplanarEvaluator`` -- two launches per
+step: the fused kernel, then ``fvr_pn_step`` (reduce + combine, Adam, the next
+items) -- eagerly, or captured once into a HIP graph (``torch.cuda.CUDAGraph``)
+and replayed, so that a 100-step registration of hundreds of poses is one graph
+launch.
 
 ``autotune`` picks the fused kernel's launch plan (tile height, tile order,
 persistence) for an evaluator by timing each candidate on the device; choices
@@ -102,10 +104,11 @@ class AdamLoop:
             evaluator.launch_map(self.n)
 
     def _body(self, steps: int) -> None:
+        # build the first items, then per step: the fused kernel + one launch that reduces,
+        # steps Adam and builds the next items (two launches per step)
+        self.ev.begin(self.poses)
         for _ in range(steps):
-            loss, grad = self.ev.evaluate_device(self.poses) if isinstance(self.ev, RaggedBiplanarEvaluator) \
-                else self.ev.evaluate_device(self.poses, want_moments=False)
-            self.adam.step(self.poses, loss, grad)
+            self.ev.step(self.poses, self.adam)
 
     def capture(self) -> None:
         """Warm up one eager step, then capture the whole loop into a graph."""

@@ -323,6 +323,37 @@ class BiplanarEvaluator:
                                            _lib.ptr(loss), _lib.ptr(grad), st), "fvr_pn_reduce_combine")
         return loss[:n], grad[: 6 * n].view(n, 6)
 
+    # -- two-launch loop form: begin() once, then step() per iteration ----------
+    def begin(self, poses: torch.Tensor) -> int:
+        """Build the items of ``poses`` (the first launch of a loop of ``step`` calls)."""
+        n = self.build_device(poses)
+        self.prepare(n)
+        self.launch_map(n)
+        return n
+
+    def step(self, poses: torch.Tensor, adam=None, kernel_events=None):
+        """One loop iteration in two launches: the fused kernel over the items built by the
+        previous ``begin``/``step``, then ``fvr_pn_step`` (reduce + combine -> loss, grad at
+        those poses; optionally one Adam step of ``poses`` in place (``adam``: a DeviceAdam);
+        the items of the (updated) poses for the next call).  Returns (loss, grad) views."""
+        n = int(poses.shape[0])
+        _, items, jac, partials, moments, loss, grad = self.prepare(n)
+        L = _lib.lib()
+        st = _lib.stream_ptr(self.device)
+        tmap, n_slots, grid = self.launch_map(n)
+        if kernel_events is not None:
+            kernel_events[0].record()
+        _lib.check(L.fvr_sncc_eval_plan(_lib.ptr(self.packed.table), len(self.packed.descs), _lib.ptr(items), 2 * n,
+                                        n * self.tiles_per_pose, _lib.ptr(partials),
+                                        self.plan.variant(self.packed.code, self.flags),
+                                        None if tmap is None else _lib.ptr(tmap), n_slots, grid, st), "fvr_sncc_eval")
+        if kernel_events is not None:
+            kernel_events[1].record()
+        pz, py, px = self.packed.shape
+        _pn_step(items, n, partials, jac, self.t2, None, loss, grad, poses, adam, self.templates, pz, py, px,
+                 self.lat_swap, 0, st)
+        return loss[:n], grad[: 6 * n].view(n, 6)
+
     def device_items(self, n: int) -> np.ndarray:
         """The items of the last device build (for cross-checks)."""
         raw = self.ws.bufs["items"][: 2 * n * _lib.ITEM_DTYPE.itemsize].cpu().numpy()
@@ -365,6 +396,21 @@ class BiplanarEvaluator:
         launch_items(self.packed, items, n_tiles, self.flags, hws, red)
         mom = red.cpu().numpy()
         return combine_biplanar(mom, jac, self.t2, self.hw)
+
+
+def _pn_step(items, n, partials, jac, t2, moments, loss, grad, poses, adam, templates, pz, py, px, lat_swap, tile0, st,
+             item0: int = 0, pose0: int = 0):
+    """fvr_pn_step for poses [pose0, pose0 + n) whose items start at item 2 * pose0."""
+    isz = _lib.ITEM_DTYPE.itemsize
+    if adam is None:
+        a = [None] * 7
+    else:
+        a = [adam.cfg.ctypes.data, _off(adam.m, 6 * pose0), _off(adam.v, 6 * pose0), _off(adam.t, pose0),
+             _off(adam.done, pose0), _off(adam.best_loss, pose0), _off(adam.best_pose, 6 * pose0)]
+    _lib.check(_lib.lib().fvr_pn_step(_off(items, 2 * pose0 * isz), n, _lib.ptr(partials), _off(jac, 2 * pose0 * 18),
+                                      float(t2[0]), float(t2[1]), moments, _off(loss, pose0), _off(grad, 6 * pose0),
+                                      _off(poses, 6 * pose0), *a, templates.ctypes.data, float(pz), float(py),
+                                      float(px), lat_swap, tile0, st), "fvr_pn_step")
 
 
 def _off(t: torch.Tensor, elems: int) -> ctypes.c_void_p:
@@ -474,6 +520,43 @@ class RaggedBiplanarEvaluator:
                                                float(s.t2[0]), float(s.t2[1]), None, _off(loss, o),
                                                _off(grad, 6 * o), st), "fvr_pn_reduce_combine")
         return loss[: self.n], grad[: 6 * self.n].view(self.n, 6)
+
+    def begin(self, poses: torch.Tensor) -> int:
+        """Build every segment's items (the first launch of a loop of ``step`` calls)."""
+        self._check_poses(poses)
+        items, jac = self.prepare()[:2]
+        L = _lib.lib()
+        st = _lib.stream_ptr(self.device)
+        isz = _lib.ITEM_DTYPE.itemsize
+        for k, s in enumerate(self.segments):
+            o, c = self.offsets[k], self.counts[k]
+            pz, py, px = s.packed.shape
+            _lib.check(L.fvr_pn_build_items(_off(poses, 6 * o), c, self.templates[k].ctypes.data, float(pz),
+                                            float(py), float(px), s.lat_swap, self.tile0[k], _off(items, 2 * o * isz),
+                                            _off(jac, 2 * o * 18), st), "fvr_pn_build_items")
+        return self.n
+
+    def step(self, poses: torch.Tensor, adam=None):
+        """One loop iteration: one fused launch over every volume's items, then one
+        ``fvr_pn_step`` per volume (reduce + combine, optional Adam, next items)."""
+        self._check_poses(poses)
+        items, jac, partials, loss, grad = self.prepare()
+        L = _lib.lib()
+        st = _lib.stream_ptr(self.device)
+        _lib.check(L.fvr_sncc_eval_plan(_lib.ptr(self.table), len(self.descs), _lib.ptr(items), 2 * self.n,
+                                        self.n_tiles, _lib.ptr(partials), self.plan.variant(self.code, self.flags),
+                                        None if self.map is None else _lib.ptr(self.map), self.n_slots, self.grid,
+                                        st), "fvr_sncc_eval")
+        for k, s in enumerate(self.segments):
+            pz, py, px = s.packed.shape
+            _pn_step(items, self.counts[k], partials, jac, s.t2, None, loss, grad, poses, adam, self.templates[k], pz,
+                     py, px, s.lat_swap, self.tile0[k], st, pose0=self.offsets[k])
+        return loss[: self.n], grad[: 6 * self.n].view(self.n, 6)
+
+    def _check_poses(self, poses: torch.Tensor) -> None:
+        _lib.require_device(poses, "RaggedBiplanarEvaluator")
+        if poses.dtype != torch.float64 or tuple(poses.shape) != (self.n, 6) or not poses.is_contiguous():
+            raise ValueError(f"poses must be a contiguous ({self.n}, 6) float64 tensor")
 
     def device_items(self) -> np.ndarray:
         raw = self.ws.bufs["items"][: 2 * self.n * _lib.ITEM_DTYPE.itemsize].cpu().numpy()

@@ -227,6 +227,20 @@ int fvr_adam_step(double* poses, const double* loss, const double* grad, double*
                   int32_t* t, int32_t* done, double* best_loss, double* best_pose, int32_t n,
                   const fvr_adam_cfg* cfg_host, void* stream);
 
+/* One launch after the fused kernel of a registration step (no counterpart in the
+ * reference: it fuses fvr_pn_reduce_combine, fvr_adam_step and fvr_pn_build_items).
+ * Per pose: the fixed-order reduction and combine into loss[n], grad[n][6] (and
+ * moments, may be NULL) at the pose the items were built from; with adam_cfg_host,
+ * one Adam step of poses[n] (state m, v, t, done, best_loss, best_pose as in
+ * fvr_adam_step; NULL: poses unchanged); then the pose's two items and Jacobians for
+ * the next fused launch from tmpl_host (as fvr_pn_build_items).  A step is then two
+ * launches: fvr_sncc_eval_plan and this. */
+int fvr_pn_step(fvr_sncc_item* items_dev, int32_t n_poses, const float* partials, double* jac_dev, double t2_ap,
+                double t2_lat, double* moments, double* loss, double* grad, double* poses,
+                const fvr_adam_cfg* adam_cfg_host, double* m, double* v, int32_t* t, int32_t* done,
+                double* best_loss, double* best_pose, const fvr_sncc_item* tmpl_host, double pz, double py,
+                double px, int32_t lat_swap, int32_t tile0, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Generic fourier_slice_3d (grid_sample semantics, planar (B,2,D,H,W)).      */
 /* ------------------------------------------------------------------------ */

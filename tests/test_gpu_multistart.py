@@ -10,8 +10,6 @@ exactly the tiles it computes alone: bit-identical.  The Adam kernel against
 the FP64 oracle on the same loss / gradient sequence: 1e-12.
 """
 
-import math
-
 import numpy as np
 import pytest
 import torch
@@ -187,3 +185,37 @@ def test_ragged_graph_loop_and_autotune(vols):
         gp, gl, _ = (t.cpu().numpy().copy() for t in loop.run(P0))
         np.testing.assert_array_equal(gp, bp)
         np.testing.assert_array_equal(gl, bl)
+
+
+@pytest.mark.parametrize("ragged", [False, True], ids=["single", "ragged"])
+def test_two_launch_step_equals_separate_launches(vols, ragged):
+    """begin() + step(adam) (fused kernel + fvr_pn_step) reproduces, bit for bit, the
+    separate launches: set-up, fused kernel, reduce/combine, fvr_adam_step."""
+    from fvr.multistart import DeviceAdam
+    from fvr.sncc import BiplanarEvaluator, RaggedBiplanarEvaluator
+    def make():  # one evaluator (and workspace) per path
+        if ragged:
+            return RaggedBiplanarEvaluator([BiplanarEvaluator(r.packed, ta, tl) for r, (ta, tl) in vols], (4, 3, 5))
+        r, (ta, tl) = vols[1]
+        return BiplanarEvaluator(r.packed, ta, tl)
+
+    ev, ev2 = make(), make()
+    n = 12 if ragged else 9
+    P0 = _poses(n, 31)
+    dev = P0.device
+    pa, pb = P0.clone(), P0.clone()
+    aa, ab = DeviceAdam(n, dev, tol=0.0), DeviceAdam(n, dev, tol=0.0)
+    aa.reset(pa)
+    ab.reset(pb)
+    ev.begin(pa)
+    for k in range(6):
+        la, ga = (t.cpu().numpy().copy() for t in ev.step(pa, aa))
+        lb, gb = ev2.evaluate_device(pb) if ragged else ev2.evaluate_device(pb, want_moments=False)
+        lb, gb = lb.cpu().numpy().copy(), gb.cpu().numpy().copy()
+        ab.step(pb, torch.from_numpy(lb).to(dev), torch.from_numpy(gb).to(dev))
+        np.testing.assert_array_equal(la, lb, err_msg=f"step {k}")
+        np.testing.assert_array_equal(ga, gb, err_msg=f"step {k}")
+        np.testing.assert_array_equal(pa.cpu().numpy(), pb.cpu().numpy(), err_msg=f"step {k}")
+    for x, y in ((aa.best_pose, ab.best_pose), (aa.best_loss, ab.best_loss), (aa.m, ab.m), (aa.v, ab.v),
+                 (aa.t, ab.t)):
+        np.testing.assert_array_equal(x.cpu().numpy(), y.cpu().numpy())
