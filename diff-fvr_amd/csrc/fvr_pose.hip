@@ -280,6 +280,139 @@ __global__ void adam_step_kernel(double* __restrict__ poses, const double* __res
   for (int a = 0; a < 6; ++a) poses[6 * k + a] = P[a];
 }
 
+// ---------------------------------------------------------------------------
+// Convention T (torch_engine.py:138-234) on the device: one thread per pose
+// builds the item of one view from a host-validated template, and one wave per
+// pose reduces its tiles and forms NCC = A / (sqrt(E E_t) + 1e-8) and d NCC /
+// d pose (torch_engine.py:269-290).
+// ---------------------------------------------------------------------------
+struct M3 {
+  double m[3][3];
+};
+
+__device__ __forceinline__ M3 mul3(const M3& a, const M3& b) {
+  M3 r;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) r.m[i][j] = a.m[i][0] * b.m[0][j] + a.m[i][1] * b.m[1][j] + a.m[i][2] * b.m[2][j];
+  return r;
+}
+
+struct TGeom {
+  double D, H, W;    // padded dims (z, y, x)
+  double spacing0;   // spacing of the phase span R * spacing + 1e-6
+  int R, lat;        // output resolution, lateral view (R Rz(90))
+  int ord[3];        // layout axis -> xyz component
+  unsigned long long target0, target_stride;  // target of pose p: target0 + p * stride
+};
+
+__global__ void pt_build_kernel(const double* __restrict__ poses, int n, fvr_sncc_item tmpl, TGeom G,
+                                fvr_sncc_item* __restrict__ items, double* __restrict__ jac) {
+  const int pose = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pose >= n) return;
+  const double deg = 3.14159265358979323846 / 180.0;
+  const double* P = poses + 6 * pose;
+  const double cz = cos(P[0] * deg), sz = sin(P[0] * deg);  // yaw about z
+  const double cy = cos(P[1] * deg), sy = sin(P[1] * deg);  // pitch about y
+  const double cx = cos(P[2] * deg), sx = sin(P[2] * deg);  // roll about x
+  const M3 Rz = {{{cz, -sz, 0.0}, {sz, cz, 0.0}, {0.0, 0.0, 1.0}}};
+  const M3 Ry = {{{cy, 0.0, sy}, {0.0, 1.0, 0.0}, {-sy, 0.0, cy}}};
+  const M3 Rx = {{{1.0, 0.0, 0.0}, {0.0, cx, -sx}, {0.0, sx, cx}}};
+  const M3 dRz = {{{-sz, -cz, 0.0}, {cz, -sz, 0.0}, {0.0, 0.0, 0.0}}};
+  const M3 dRy = {{{-sy, 0.0, cy}, {0.0, 0.0, 0.0}, {-cy, 0.0, -sy}}};
+  const M3 dRx = {{{0.0, 0.0, 0.0}, {0.0, -sx, -cx}, {0.0, cx, -sx}}};
+  const M3 Rz90 = {{{0.0, -1.0, 0.0}, {1.0, 0.0, 0.0}, {0.0, 0.0, 1.0}}};
+  M3 R = mul3(mul3(Rz, Ry), Rx);
+  M3 dR[3] = {mul3(mul3(dRz, Ry), Rx), mul3(mul3(Rz, dRy), Rx), mul3(mul3(Rz, Ry), dRx)};
+  if (G.lat) {
+    R = mul3(R, Rz90);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) dR[k] = mul3(dR[k], Rz90);
+  }
+  // half extents (= the centre) of the padded volume in xyz
+  const double s[3] = {0.5 * fmax(G.W - 1.0, 1.0), 0.5 * fmax(G.H - 1.0, 1.0), 0.5 * fmax(G.D - 1.0, 1.0)};
+  const double f0 = -1.0, df = G.R > 1 ? 2.0 / double(G.R - 1) : 0.0;
+  fvr_sncc_item it = tmpl;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int k = G.ord[a];
+    const double u = R.m[0][k] * s[k], v = R.m[1][k] * s[k];
+    it.c[a] = to_fx(s[k] + f0 * u + f0 * v + 1.0) + kSnapFx;  // +1: per-corner coordinates (lo_int = 0)
+    it.ustep[a] = to_fx(df * v);  // along a row: next column (v)
+    it.vstep[a] = to_fx(df * u);  // down a column: next row (u)
+  }
+  const double span = double(G.R) * G.spacing0 + 1e-6;
+  it.ph_row0 = turns_fx(f0 * P[3] / span);
+  it.ph_row = turns_fx(df * P[3] / span);
+  it.ph_col0 = turns_fx(f0 * P[4] / span);
+  it.ph_col = turns_fx(df * P[4] / span);
+  it.tile_base = pose * (tmpl.tiles_x * sncc_tiles_y(tmpl.h, tmpl.flags, tmpl.tile_h));
+  it.target = G.target0 + (unsigned long long)pose * G.target_stride;
+  items[pose] = it;
+  // jac[p][0][a][theta] = d v / d theta, jac[p][1][a][theta] = d u / d theta (layout axes, per degree)
+  double* J = jac + size_t(pose) * 18;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      const int k = G.ord[a];
+      J[a * 3 + t] = dR[t].m[1][k] * s[k] * deg;
+      J[9 + a * 3 + t] = dR[t].m[0][k] * s[k] * deg;
+    }
+}
+
+// one wave per pose: ncc[p] and dgrad[p][6] = -(1/n) d NCC_p / d pose (the gradient of
+// 1 - mean NCC); e_t: target energy of pose p at e_t[p * et_stride]
+__global__ __launch_bounds__(64) void pt_combine_kernel(const fvr_sncc_item* __restrict__ items, int n,
+                                                        const float* __restrict__ partials,
+                                                        const double* __restrict__ jac, const double* __restrict__ e_t,
+                                                        int et_stride, double span, double* __restrict__ ncc_out,
+                                                        double* __restrict__ grad) {
+  const int pose = blockIdx.x, lane = threadIdx.x;
+  const fvr_sncc_item& it = items[pose];
+  const int ntile = it.tiles_x * sncc_tiles_y(it.h, it.flags, it.tile_h);
+  double m[FVR_SNCC_NMOM];
+#pragma unroll
+  for (int q = 0; q < FVR_SNCC_NMOM; ++q) m[q] = 0.0;
+  for (int t = lane; t < ntile; t += 64) {
+    const float* pp = partials + size_t(it.tile_base + t) * FVR_SNCC_NMOM;
+#pragma unroll
+    for (int q = 0; q < FVR_SNCC_NMOM; ++q) m[q] += double(pp[q]);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int q = 0; q < FVR_SNCC_NMOM; ++q) m[q] += __shfl_xor(m[q], o);
+  const double* J = jac + size_t(pose) * 18;
+  const double E = fmax(m[1], 0.0), et = e_t[size_t(pose) * et_stride];
+  const double Dn = sqrt(E * et) + 1e-8;
+  const double ncc = m[0] / Dn;
+  double d[6];
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    double Gt = 0.0, Ht = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      Gt += m[4 + a] * J[a * 3 + t] + m[7 + a] * J[9 + a * 3 + t];
+      Ht += m[10 + a] * J[a * 3 + t] + m[13 + a] * J[9 + a * 3 + t];
+    }
+    const double dD = E > 0.0 ? sqrt(et) * Ht / sqrt(E) : 0.0;
+    d[t] = Gt / Dn - (m[0] / (Dn * Dn)) * dD;
+  }
+  const double two_pi = 6.283185307179586476925286766559;
+  d[3] = two_pi / span * m[3] / Dn;  // d/dtx: row moments (rows follow u)
+  d[4] = two_pi / span * m[2] / Dn;  // d/dty: column moments
+  d[5] = 0.0;                        // tz does not enter convention T
+  if (lane == 0) ncc_out[pose] = ncc;
+  if (lane < 6) {
+    double gv = 0.0;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) gv = a == lane ? d[a] : gv;
+    grad[6 * pose + lane] = -gv / double(n);
+  }
+}
+
 // One wave per pose after the fused kernel: reduce + combine (loss, gradient at the
 // pose the items were built from), then optionally one Adam step of the pose, then the
 // pose's two items and Jacobians for the next fused launch.  A registration step is
@@ -376,6 +509,48 @@ int fvr_pn_step(fvr_sncc_item* items_dev, int32_t n_poses, const float* partials
                      n_poses, partials, jac_dev, t2_ap, t2_lat, moments, loss, grad, poses, adam_cfg_host ? 1 : 0, c,
                      S, tmpl_host[0], tmpl_host[1], pz, py, px, int(lat_swap), int(tile0));
   FVR_CHECK_LAUNCH("pn_step_kernel");
+  return FVR_OK;
+}
+
+int fvr_pt_build_items(const double* poses, int32_t n_poses, const fvr_sncc_item* tmpl_host, double D, double H,
+                       double W, double spacing0, int32_t out_res, int32_t lateral, const int32_t* layout_order,
+                       uint64_t target0, uint64_t target_stride, fvr_sncc_item* items_dev, double* jac_dev,
+                       void* stream) {
+  FVR_CHECK_ARG(poses && tmpl_host && layout_order && items_dev && jac_dev && n_poses > 0 && target0,
+                "fvr_pt_build_items: bad arguments");
+  FVR_CHECK_ARG(tmpl_host->h == out_res && tmpl_host->w == out_res && out_res > 0, "fvr_pt_build_items: bad out_res");
+  FVR_CHECK_ARG(sncc_tile_h_ok(tmpl_host->tile_h), "fvr_pt_build_items: bad template tile_h");
+  int seen = 0;
+  for (int a = 0; a < 3; ++a) {
+    FVR_CHECK_ARG(layout_order[a] >= 0 && layout_order[a] < 3, "fvr_pt_build_items: bad layout order");
+    seen |= 1 << layout_order[a];
+  }
+  FVR_CHECK_ARG(seen == 7, "fvr_pt_build_items: layout order is not a permutation");
+  const int64_t t_pose = int64_t(tmpl_host->tiles_x) * sncc_tiles_y(tmpl_host->h, tmpl_host->flags, tmpl_host->tile_h);
+  FVR_CHECK_ARG(t_pose > 0 && int64_t(n_poses) * t_pose < (int64_t(1) << 31), "fvr_pt_build_items: tile overflow");
+  TGeom G;
+  G.D = D;
+  G.H = H;
+  G.W = W;
+  G.spacing0 = spacing0;
+  G.R = out_res;
+  G.lat = lateral ? 1 : 0;
+  for (int a = 0; a < 3; ++a) G.ord[a] = layout_order[a];
+  G.target0 = target0;
+  G.target_stride = target_stride;
+  hipLaunchKernelGGL(pt_build_kernel, dim3((n_poses + 63) / 64), dim3(64), 0, reinterpret_cast<hipStream_t>(stream),
+                     poses, int(n_poses), *tmpl_host, G, items_dev, jac_dev);
+  FVR_CHECK_LAUNCH("pt_build_kernel");
+  return FVR_OK;
+}
+
+int fvr_pt_combine(const fvr_sncc_item* items_dev, int32_t n_poses, const float* partials, const double* jac,
+                   const double* e_t, int32_t et_stride, double span, double* ncc, double* grad, void* stream) {
+  FVR_CHECK_ARG(items_dev && partials && jac && e_t && ncc && grad && n_poses > 0 && et_stride >= 0,
+                "fvr_pt_combine: bad arguments");
+  hipLaunchKernelGGL(pt_combine_kernel, dim3(n_poses), dim3(64), 0, reinterpret_cast<hipStream_t>(stream), items_dev,
+                     int(n_poses), partials, jac, e_t, int(et_stride), span, ncc, grad);
+  FVR_CHECK_LAUNCH("pt_combine_kernel");
   return FVR_OK;
 }
 

@@ -107,13 +107,73 @@ class TSliceEvaluator:
         return 1.0 - float(np.mean(ncc)), -d / B, ncc
 
 
+    # -- device path: set-up, fused kernel and combine on the device, no host sync ------
+    def _template(self, view: str, R: int, flags: int, hp_sigma: Optional[float]) -> np.ndarray:
+        """Validated static item fields for (view, R, flags, hp_sigma) (cached)."""
+        key = (view, R, flags, hp_sigma)
+        cache = self.__dict__.setdefault("_tmpl", {})
+        t = cache.get(key)
+        if t is None:
+            t = _items.t_template(self.packed, view, R, 1, flags, self.rows, hp_sigma)
+            probe, _, _ = _items.t_items(t, self.pad_shape, np.zeros((1, 6)), view, R, self.spacing0, self.rows)
+            _items.validate_items(probe, len(self.packed.descs), self.packed.descs, self.rows)
+            t = _lib.as_struct(t, _lib.ITEM_DTYPE)
+            cache[key] = t
+        return t
+
+    def device_loss_and_grad(self, poses: torch.Tensor, targets: torch.Tensor, view: str,
+                             hp_sigma: Optional[float], adjoint: bool = True):
+        """(ncc (B,), d(1 - mean NCC)/d poses (B, 6)) as float64 device tensors.
+
+        poses: (B, 6) float64 device tensor; targets: (Bt, R, R) complex64 device tensor,
+        Bt in {1, B}.  Three stream-ordered launches: fvr_pt_build_items, the fused
+        kernel, fvr_pt_combine."""
+        B = int(poses.shape[0])
+        R = int(targets.shape[-1])
+        flags = _lib.ITEM_ADJOINT if adjoint else 0
+        tmpl = self._template(view, R, flags, hp_sigma)
+        perm = [0, 2, 1] if _items.T_VIEW_LAYOUT[view] == "zxy" else [0, 1, 2]
+        order = np.array([[2, 1, 0][a] for a in perm], dtype=np.int32)
+        stride = targets.stride(0) * targets.element_size() if targets.shape[0] > 1 else 0
+        ntile = int(tmpl["tiles_x"][0]) * _items.tiles_for(R, R, self.rows)[1]
+        ws = self.ws
+        isz = _lib.ITEM_DTYPE.itemsize
+        items = ws.get("t_items", B * isz, torch.uint8)
+        jac = ws.get("t_jac", B * 18, torch.float64)
+        partials = ws.get("t_partials", B * ntile * _lib.NMOM, torch.float32)
+        ncc = ws.get("t_ncc", B, torch.float64)
+        grad = ws.get("t_grad", 6 * B, torch.float64)
+        t = targets.to(torch.complex128)
+        e_t = (t.real.square() + t.imag.square()).sum(dim=(-2, -1)).reshape(-1).contiguous()
+        L = _lib.lib()
+        st = _lib.stream_ptr(self.packed.device)
+        D, H, W = self.pad_shape
+        _lib.check(L.fvr_pt_build_items(_lib.ptr(poses), B, tmpl.ctypes.data, float(D), float(H), float(W),
+                                        self.spacing0, R, int(view == "lateral"), order.ctypes.data,
+                                        targets.data_ptr(), stride, _lib.ptr(items), _lib.ptr(jac), st),
+                   "fvr_pt_build_items")
+        variant = self.packed.code | (int(tmpl["flags"][0]) << 4) | (self.rows << 8)
+        _lib.check(L.fvr_sncc_eval_v(_lib.ptr(self.packed.table), len(self.packed.descs), _lib.ptr(items), B,
+                                     B * ntile, _lib.ptr(partials), variant, st), "fvr_sncc_eval")
+        _lib.check(L.fvr_pt_combine(_lib.ptr(items), B, _lib.ptr(partials), _lib.ptr(jac), _lib.ptr(e_t),
+                                    1 if e_t.numel() > 1 else 0, _items.t_phase_span(R, self.spacing0), _lib.ptr(ncc),
+                                    _lib.ptr(grad), st), "fvr_pt_combine")
+        return ncc[:B], grad[: 6 * B].view(B, 6)
+
+    def device_items(self, B: int) -> np.ndarray:
+        raw = self.ws.bufs["t_items"][: B * _lib.ITEM_DTYPE.itemsize].cpu().numpy()
+        return raw.view(_lib.ITEM_DTYPE).copy()
+
+
 class _FusedSpectralNCC(torch.autograd.Function):
     @staticmethod
     def forward(ctx, poses, evaluator, targets, view, hp_sigma):
-        P = poses.detach().to("cpu", torch.float64).numpy()
-        loss, grad, _ = evaluator.loss_and_grad(P, targets, view, hp_sigma, adjoint=bool(ctx.needs_input_grad[0]))
-        ctx.grad = torch.from_numpy(grad)
-        return torch.tensor(loss, dtype=torch.float32, device=poses.device)
+        dev = evaluator.packed.device
+        P = poses.detach().to(dev, torch.float64).contiguous()
+        ncc, grad = evaluator.device_loss_and_grad(P, targets, view, hp_sigma, adjoint=bool(ctx.needs_input_grad[0]))
+        ctx.grad = grad.clone()
+        loss = (1.0 - ncc.mean()).to(torch.float32)
+        return loss.to(poses.device)
 
     @staticmethod
     def backward(ctx, g):

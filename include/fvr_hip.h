@@ -241,6 +241,24 @@ int fvr_pn_step(fvr_sncc_item* items_dev, int32_t n_poses, const float* partials
                 double* best_loss, double* best_pose, const fvr_sncc_item* tmpl_host, double pz, double py,
                 double px, int32_t lat_swap, int32_t tile0, void* stream);
 
+/* Convention-T set-up on the device (float64): one item per pose of one view from
+ * poses[n][6] (yaw, pitch, roll in degrees about z, y, x; tx, ty), copying the
+ * static fields of the host-validated template tmpl_host; D, H, W: padded dims;
+ * layout_order[a]: xyz component of layout axis a; target of pose p at
+ * target0 + p * target_stride bytes; jac[n][2][3][3] = dv/dtheta, du/dtheta (layout
+ * axes, per degree).  Replaces _extract_slice_spectrum's geometry and the
+ * autograd through the rotation (torch_engine.py:138-234). */
+int fvr_pt_build_items(const double* poses, int32_t n_poses, const fvr_sncc_item* tmpl_host, double D, double H,
+                       double W, double spacing0, int32_t out_res, int32_t lateral, const int32_t* layout_order,
+                       uint64_t target0, uint64_t target_stride, fvr_sncc_item* items_dev, double* jac_dev,
+                       void* stream);
+/* Per pose: fixed-order reduction of the fused kernel's tiles, NCC = A / (sqrt(E E_t)
+ * + 1e-8) into ncc[n] and grad[n][6] = -(1/n) d NCC / d pose, i.e. the gradient of
+ * 1 - mean NCC (torch_engine.py:269-290); e_t[p * et_stride]: target energies;
+ * span = out_res * spacing + 1e-6. */
+int fvr_pt_combine(const fvr_sncc_item* items_dev, int32_t n_poses, const float* partials, const double* jac,
+                   const double* e_t, int32_t et_stride, double span, double* ncc, double* grad, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Generic fourier_slice_3d (grid_sample semantics, planar (B,2,D,H,W)).      */
 /* ------------------------------------------------------------------------ */

@@ -217,3 +217,41 @@ def test_reduced_precision_storage(dtype, tol):
     L16.backward()
     assert abs(float(L32) - float(L16)) <= tol * max(float(L32), 1e-3)
     assert float((q1.grad - q2.grad).abs().max()) <= 10 * tol * float(q1.grad.abs().max())
+
+
+@pytest.mark.parametrize("view", ["ap", "lateral"])
+def test_device_t_setup_matches_host_setup(eng, view):
+    """The device convention-T set-up (fvr_pt_build_items) reproduces the host builder's
+    items and Jacobians, and its combine reproduces the host float64 combine."""
+    from fvr import _items, _lib
+    from fvr.torch_engine import combine_t
+    e, S3 = eng
+    ev = e._fused()
+    R = 24
+    ft, _ = _target(e, S3, [1.0, -0.5, 0.5, 0.4, -0.3, 0.0], view, R)
+    tg = torch.cat([ft.reshape(-1, R, R)] * 5).to(torch.complex64).contiguous()  # one target per pose
+    poses = np.array([[3.0, -2.0, 1.0, 1.5, -1.0, 0.3], [-4.0, 2.5, -1.5, -0.8, 1.2, 0.0], [0.0] * 6,
+                      [12.0, -7.0, 9.0, 2.0, 3.0, -1.0], [-0.5, 0.25, 30.0, -2.5, 0.5, 4.0]])
+    P = torch.from_numpy(poses).cuda()
+    ncc, grad = ev.device_loss_and_grad(P, tg, view, 4.0)
+    ncc, grad = ncc.cpu().numpy().copy(), grad.cpu().numpy().copy()
+    got = ev.device_items(len(poses))
+    tmpl = ev._template(view, R, _lib.ITEM_ADJOINT, 4.0).copy()
+    stride = tg.stride(0) * tg.element_size()
+    ptrs = tg.data_ptr() + stride * np.arange(len(poses), dtype=np.uint64)
+    ref, jac_ref, _ = _items.t_items(tmpl, ev.pad_shape, poses, view, R, ev.spacing0, ev.rows, ptrs)
+    for f in _lib.ITEM_DTYPE.names:
+        if f in ("c", "ustep", "vstep"):
+            assert np.max(np.abs(got[f].astype(np.int64) - ref[f].astype(np.int64))) <= 4, f
+        else:
+            np.testing.assert_array_equal(got[f], ref[f], err_msg=f)
+    jac = ev.ws.bufs["t_jac"][: 18 * len(poses)].cpu().numpy().reshape(-1, 2, 3, 3)
+    np.testing.assert_allclose(jac, jac_ref, atol=1e-13)
+    # host combine of the same moments: reduce the partials per item on the device, combine in float64
+    from fvr.sncc import run_items
+    mom = run_items(ev.packed, ref, int(ref["flags"][0])).cpu().numpy()
+    t = tg.to(torch.complex128)
+    e_t = (t.real.square() + t.imag.square()).sum(dim=(-2, -1)).cpu().numpy()
+    nr, dr = combine_t(mom, jac_ref, e_t, _items.t_phase_span(R, ev.spacing0))
+    np.testing.assert_allclose(ncc, nr, rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(grad, -dr / len(poses), rtol=1e-7, atol=1e-12 * np.abs(dr).max())
