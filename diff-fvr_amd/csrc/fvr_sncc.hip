@@ -409,21 +409,46 @@ __global__ __launch_bounds__(256) void sncc_kernel(const fvr_layout* __restrict_
 // of rows 1 .. h/2-1 is evaluated by the otherwise idle lanes 1 .. w/2-1 of
 // row h/2.
 // ---------------------------------------------------------------------------
+// One folded sample in flight: its raw loads, folded target, fractional
+// coordinates and the two box tests (lane masks).  Phase and moment weights are
+// formed when the sample is consumed, so that little state rides the pipeline.
 template <int DT>
 struct SmpH {
   Raw<DT> raw;
   f4v t;  // (T(i,j), conj T(mirror))
   float ts, tm, tf;
-  float wp, wq;  // box flags of the sample and of its mirror
-  float cw, rw;  // moment weights of the sample
-  float phr, phi;  // its phase factor
+  bool bp, bq;  // box tests of the sample and of its mirror
 };
 
-template <int DT, bool ADJ, int R>
+// Row / column of special slot s of the first wave: slot 0 is row h/2 (lanes j < w/2
+// remapped to column 0 of row j), the last slot is the unpaired row 0.
+struct SpecialSlot {
+  int ii, jj;
+  bool valid, mirror;
+};
+__device__ __forceinline__ SpecialSlot special_slot(int s, int j, int hh, int hw, bool colok) {
+  SpecialSlot r;
+  if (s == 0) {
+    const bool remap = j >= 1 && j < hw;
+    r.ii = remap ? j : hh;
+    r.jj = remap ? 0 : j;
+    r.valid = colok && (!remap || j <= hh - 1);
+    r.mirror = j > hw;
+  } else {
+    r.ii = 0;
+    r.jj = j;
+    r.valid = colok;
+    r.mirror = false;
+  }
+  return r;
+}
+
+template <int DT, bool ADJ, int R, int DEPTH>
 __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ layouts, int n_layouts,
                                                const fvr_sncc_item& it, int lt, int tile,
                                                float* __restrict__ partials) {
   static_assert(R >= 1 && R <= 64, "rows per wave: <= 64 (row phasors live in lanes)");
+  static_assert(DEPTH >= 2 && DEPTH <= 4, "samples in flight per lane");
   if (uint32_t(it.layout) >= uint32_t(n_layouts) || it.tile_h != 4 * R) {
     if (threadIdx.x < FVR_SNCC_NMOM) partials[size_t(tile) * FVR_SNCC_NMOM + threadIdx.x] = 0.f;
     return;
@@ -465,8 +490,8 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
   const float cwj = it.cw0 + float(j) * it.cws;
   // row phasors of this wave's regular rows (lane k <-> row h/2 + r0 + k)
   const cpx rowph = phasor(it.ph_row0 + uint32_t(hh + r0 + lane) * it.ph_row);
-  // regular rows are consecutive slots: their position, target offset and row weight
-  // are stepped from the wave's first regular row (slot 1 in the first wave)
+  // regular rows are consecutive slots: their position and target offset are stepped
+  // (issue side) and their row weight too (consume side), from the wave's first regular row
   const int ireg = hh + r0 + (first ? 1 : 0);
   int64_t q0 = bj0 + int64_t(ireg) * it.vstep[0];
   int64_t q1 = bj1 + int64_t(ireg) * it.vstep[1];
@@ -477,7 +502,7 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
   float rwr = it.rw0 + float(ireg) * it.rws;
   const float rws = it.rws;
 
-  // stage 1: the sample of slot s -- position, box flags, loads
+  // stage 1: the sample of slot s -- position, box tests, loads
   auto issue = [&](SmpH<DT>& sm, int s) {
     int64_t p0, p1, p2;
     bool valid, mirror;
@@ -491,37 +516,16 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
       q2 += v2;
       valid = colok;
       mirror = j >= 1;
-      const float prr = readlane_f(rowph.r, s), pri = readlane_f(rowph.i, s);
-      sm.phr = colph.r * prr - colph.i * pri;
-      sm.phi = fmaf(colph.r, pri, colph.i * prr);
-      sm.cw = cwj;
-      sm.rw = rwr;
-      rwr += rws;
       toff = toffr;
       toffr += tstep;
-    } else {  // row h/2 (remapped lanes take column 0 of rows 1 .. h/2-1) or the unpaired row 0
-      int ii, jj;
-      if (s == 0) {
-        const bool remap = j >= 1 && j < hw;
-        ii = remap ? j : hh;
-        jj = remap ? 0 : j;
-        valid = colok && (!remap || j <= hh - 1);
-        mirror = j > hw;
-      } else {
-        ii = 0;
-        jj = j;
-        valid = colok;
-        mirror = false;
-      }
-      p0 = it.c[0] + int64_t(jj) * it.ustep[0] + int64_t(ii) * it.vstep[0];
-      p1 = it.c[1] + int64_t(jj) * it.ustep[1] + int64_t(ii) * it.vstep[1];
-      p2 = it.c[2] + int64_t(jj) * it.ustep[2] + int64_t(ii) * it.vstep[2];
-      const cpx ph = phasor(it.ph_col0 + uint32_t(jj) * it.ph_col + it.ph_row0 + uint32_t(ii) * it.ph_row);
-      sm.phr = ph.r;
-      sm.phi = ph.i;
-      sm.cw = it.cw0 + float(jj) * it.cws;
-      sm.rw = it.rw0 + float(ii) * it.rws;
-      toff = uint32_t(ii * W + jj) * 16u;
+    } else {
+      const SpecialSlot sp = special_slot(s, j, hh, hw, colok);
+      p0 = it.c[0] + int64_t(sp.jj) * it.ustep[0] + int64_t(sp.ii) * it.vstep[0];
+      p1 = it.c[1] + int64_t(sp.jj) * it.ustep[1] + int64_t(sp.ii) * it.vstep[1];
+      p2 = it.c[2] + int64_t(sp.jj) * it.ustep[2] + int64_t(sp.ii) * it.vstep[2];
+      valid = sp.valid;
+      mirror = sp.mirror;
+      toff = uint32_t(sp.ii * W + sp.jj) * 16u;
     }
     const bool bp = valid & (uint64_t(p0) <= h0) & (uint64_t(p1) <= h1) & (uint64_t(p2) <= h2);
     const bool bq = valid & mirror & (uint64_t(p0) >= M0) & (uint64_t(p0) <= K0) & (uint64_t(p1) >= M1) &
@@ -536,12 +540,30 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
     const uint32_t S = fs + lo_int, M = fm + lo_int, F = ff + lo_int;
     sm.raw.load(b0, b1, inb ? Raw<DT>::offset(S, M, F, planeb, rowb) : guard, rowb, scales, S, M, F, Bm, Bf);
     sm.t = *reinterpret_cast<gf4ptr>(TT + (valid ? toff : 0u));
-    sm.wp = bp ? 1.f : 0.f;
-    sm.wq = bq ? 1.f : 0.f;
+    sm.bp = bp;
+    sm.bq = bq;
   };
 
   Acc ac = {};
-  auto consume = [&](const SmpH<DT>& sm) {
+  // stage 2: phase and weights of slot s, interpolation, the moment updates
+  auto consume = [&](const SmpH<DT>& sm, int s) {
+    float phr, phi, cw, rw;
+    if (s < nreg && !(first && s == 0)) {
+      const float prr = readlane_f(rowph.r, s), pri = readlane_f(rowph.i, s);
+      phr = colph.r * prr - colph.i * pri;
+      phi = fmaf(colph.r, pri, colph.i * prr);
+      cw = cwj;
+      rw = rwr;
+      rwr += rws;
+    } else {
+      const SpecialSlot sp = special_slot(s, j, hh, hw, colok);
+      const cpx ph = phasor(it.ph_col0 + uint32_t(sp.jj) * it.ph_col + it.ph_row0 + uint32_t(sp.ii) * it.ph_row);
+      phr = ph.r;
+      phi = ph.i;
+      cw = it.cw0 + float(sp.jj) * it.cws;
+      rw = it.rw0 + float(sp.ii) * it.rws;
+    }
+    const float wp = sm.bp ? 1.f : 0.f, wq = sm.bq ? 1.f : 0.f;
     const float ts = sm.ts, tm = sm.tm, tf = sm.tf;
     const Raw<DT>& q = sm.raw;
     float a0r, a0i, b0r, b0i, c0r, c0i, e0r_, e0i_, d00r, d00i, d01r, d01i, d10r, d10i, d11r, d11i;
@@ -563,13 +585,12 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
     // folded target T'' = bp T + bq conj T(mirror); W = phase conj(T'') sc, with I, dI
     // in units of the brick scale sc (E and the H moments carry sc^2 through J)
     const float sc = q.scale();
-    const float Tr = sc * fmaf(sm.wq, sm.t.z, sm.wp * sm.t.x);
-    const float Ti = sc * fmaf(sm.wq, sm.t.w, sm.wp * sm.t.y);
-    const float Wr = fmaf(sm.phr, Tr, sm.phi * Ti);
-    const float Wi = fmaf(sm.phi, Tr, -sm.phr * Ti);
-    const float wE = (sm.wp + sm.wq) * (sc * sc);
+    const float Tr = sc * fmaf(wq, sm.t.z, wp * sm.t.x);
+    const float Ti = sc * fmaf(wq, sm.t.w, wp * sm.t.y);
+    const float Wr = fmaf(phr, Tr, phi * Ti);
+    const float Wi = fmaf(phi, Tr, -phr * Ti);
+    const float wE = (wp + wq) * (sc * sc);
     const float Jr = Ir * wE, Ji = Ii * wE;
-    const float cw = sm.cw, rw = sm.rw;
     const float IWr = fmaf(Ir, Wr, -Ii * Wi);
     const float IWi = fmaf(Ir, Wi, Ii * Wr);
     ac.A += IWr;
@@ -598,31 +619,29 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
     }
   };
 
-  if (nslot > 0) {  // wave-uniform
-    SmpH<DT> sa, sb;
-    issue(sa, 0);
-    int s = 0;
+  // DEPTH samples in flight per lane: the loads of slot s + DEPTH - 1 are issued before
+  // slot s is consumed (ring of DEPTH buffers, indices fixed by the unrolled inner loop;
+  // every bound below is wave-uniform)
+  SmpH<DT> ring[DEPTH];
+#pragma unroll
+  for (int d = 0; d < DEPTH - 1; ++d)
+    if (d < nslot) issue(ring[d], d);
 #pragma unroll 1
-    for (; s + 2 < nslot; s += 2) {
-      issue(sb, s + 1);
-      consume(sa);
-      __builtin_amdgcn_sched_barrier(0);
-      issue(sa, s + 2);
-      consume(sb);
-      __builtin_amdgcn_sched_barrier(0);
-    }
-    if (s + 1 < nslot) {
-      issue(sb, s + 1);
-      consume(sa);
-      consume(sb);
-    } else {
-      consume(sa);
+  for (int s0 = 0; s0 < nslot; s0 += DEPTH) {
+#pragma unroll
+    for (int d = 0; d < DEPTH; ++d) {
+      const int s = s0 + d;
+      if (s < nslot) {
+        if (s + DEPTH - 1 < nslot) issue(ring[(d + DEPTH - 1) % DEPTH], s + DEPTH - 1);
+        consume(ring[d], s);
+        __builtin_amdgcn_sched_barrier(0);
+      }
     }
   }
   tile_reduce(ac, 1.f, tile, partials);
 }
 
-template <int DT, bool ADJ, int R>
+template <int DT, bool ADJ, int R, int DEPTH>
 __global__ __launch_bounds__(256) void sncc_herm_kernel(const fvr_layout* __restrict__ layouts, int n_layouts,
                                                         const fvr_sncc_item* __restrict__ items, int n_items,
                                                         int n_tiles, const int2* __restrict__ map, int n_slots,
@@ -630,7 +649,7 @@ __global__ __launch_bounds__(256) void sncc_herm_kernel(const fvr_layout* __rest
   for (int w = blockIdx.x; w < n_slots; w += gridDim.x) {
     int item, lt, tile;
     if (!resolve_tile<R>(w, map, items, n_items, n_tiles, n_slots, item, lt, tile)) continue;
-    sncc_herm_tile<DT, ADJ, R>(layouts, n_layouts, items[item], lt, tile, partials);
+    sncc_herm_tile<DT, ADJ, R, DEPTH>(layouts, n_layouts, items[item], lt, tile, partials);
   }
 }
 
@@ -673,6 +692,7 @@ struct Launch {
   const int2* map;
   int n_slots;
   float* partials;
+  int depth;  // samples in flight per lane (Hermitian-fold kernel: 2 or 3)
 };
 
 template <int DT, bool ADJ, bool HP, bool WS, int R>
@@ -683,14 +703,20 @@ static void launch_sncc(const Launch& a) {
 
 template <int DT, bool ADJ, int R>
 static void launch_herm(const Launch& a) {
-  hipLaunchKernelGGL((sncc_herm_kernel<DT, ADJ, R>), a.grid, dim3(256), 0, a.st, a.lay, a.n_lay, a.items,
-                     a.n_items, a.n_tiles, a.map, a.n_slots, a.partials);
+  if (a.depth == 3)
+    hipLaunchKernelGGL((sncc_herm_kernel<DT, ADJ, R, 3>), a.grid, dim3(256), 0, a.st, a.lay, a.n_lay, a.items,
+                       a.n_items, a.n_tiles, a.map, a.n_slots, a.partials);
+  else
+    hipLaunchKernelGGL((sncc_herm_kernel<DT, ADJ, R, 2>), a.grid, dim3(256), 0, a.st, a.lay, a.n_lay, a.items,
+                       a.n_items, a.n_tiles, a.map, a.n_slots, a.partials);
 }
 
 // Variants: every flag combination at tile_h 64; tile_h 32 / 128 for the batched
-// adjoint forms (convention N plain or folded, convention T with high-pass).
-static bool has_variant(uint32_t flags, int tile_h) {
+// adjoint forms (convention N plain or folded, convention T with high-pass); the
+// folded kernel also with 3 samples in flight per lane (depth).
+static bool has_variant(uint32_t flags, int tile_h, int depth = 2) {
   if (flags >= 16u || !sncc_tile_h_ok(tile_h)) return false;
+  if (depth != 2 && !(depth == 3 && (flags & FVR_ITEM_HERM))) return false;
   const bool adj = flags & FVR_ITEM_ADJOINT, hp = flags & FVR_ITEM_HP, ws = flags & FVR_ITEM_WRITE_SLICE;
   if (flags & FVR_ITEM_HERM) return !hp && !ws;  // the fold is convention N without slice output
   return tile_h == 64 || (adj && !ws);
@@ -742,6 +768,10 @@ int fvr_sncc_tile_rows(void) { return kSnccTileH; }
 
 int fvr_sncc_has_tile_h(uint32_t flags, int32_t tile_h) { return has_variant(flags, tile_h) ? 1 : 0; }
 
+int fvr_sncc_has_depth(uint32_t flags, int32_t tile_h, int32_t depth) {
+  return has_variant(flags, tile_h, depth) ? 1 : 0;
+}
+
 int fvr_sncc_eval_plan(const fvr_layout* layouts_dev, int32_t n_layouts, const fvr_sncc_item* items_dev,
                        int32_t n_items, int32_t n_tiles, float* partials, uint32_t variant,
                        const int32_t* tile_map, int32_t n_slots, int32_t grid, void* stream) {
@@ -749,10 +779,14 @@ int fvr_sncc_eval_plan(const fvr_layout* layouts_dev, int32_t n_layouts, const f
   FVR_CHECK_ARG(n_items > 0 && n_tiles > 0 && n_layouts > 0, "fvr_sncc_eval: empty batch");
   const int dt = int(variant & 0xF);
   const uint32_t flags = (variant >> 4) & 0xF;
-  int tile_h = int(variant >> 8);
+  int tile_h = int((variant >> 8) & 0xFF);
   if (tile_h == 0) tile_h = kSnccTileH;
+  int depth = int((variant >> 16) & 0xF);
+  if (depth == 0) depth = 2;
   FVR_CHECK_ARG(dt == FVR_C32 || dt == FVR_C16F || dt == FVR_C16B, "fvr_sncc_eval: bad dtype %d", dt);
-  FVR_CHECK_ARG(has_variant(flags, tile_h), "fvr_sncc_eval: no kernel for flags %u with tile_h %d", flags, tile_h);
+  FVR_CHECK_ARG((variant >> 20) == 0u, "fvr_sncc_eval: bad variant %u", variant);
+  FVR_CHECK_ARG(has_variant(flags, tile_h, depth), "fvr_sncc_eval: no kernel for flags %u with tile_h %d, depth %d",
+                flags, tile_h, depth);
   if (tile_map) {
     FVR_CHECK_ARG(n_slots > 0, "fvr_sncc_eval: empty tile map");
   } else {
@@ -771,6 +805,7 @@ int fvr_sncc_eval_plan(const fvr_layout* layouts_dev, int32_t n_layouts, const f
   a.map = reinterpret_cast<const int2*>(tile_map);
   a.n_slots = n_slots;
   a.partials = partials;
+  a.depth = depth;
   if (dt == FVR_C32) dispatch_dt<FVR_C32>(flags, tile_h, a);
   else if (dt == FVR_C16F) dispatch_dt<FVR_C16F>(flags, tile_h, a);
   else dispatch_dt<FVR_C16B>(flags, tile_h, a);

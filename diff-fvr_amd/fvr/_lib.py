@@ -73,6 +73,7 @@ _SIGNATURES = {
     "fvr_spec_pack": ([_P, _U64, _I, _I, _I, _I, _P], _I),
     "fvr_sncc_tile_rows": ([], _I),
     "fvr_sncc_has_tile_h": ([_U, _I], _I),
+    "fvr_sncc_has_depth": ([_U, _I, _I], _I),
     "fvr_sncc_eval_v": ([_P, _I, _P, _I, _I, _P, _U, _P], _I),
     "fvr_sncc_eval_plan": ([_P, _I, _P, _I, _I, _P, _U, _P, _I, _I, _P], _I),
     "fvr_sncc_reduce": ([_P, _I, _P, _P, _P], _I),

@@ -144,7 +144,7 @@ _LOCK = threading.Lock()
 
 
 def candidate_plans(flags: int) -> List[LaunchPlan]:
-    """Every plan the library has a kernel for: tile height x tile order x persistence."""
+    """Every plan the library has a kernel for: tile height x tile order x persistence x depth."""
     L = _lib.lib()
     out = []
     for th in (32, 64, 128):
@@ -152,7 +152,9 @@ def candidate_plans(flags: int) -> List[LaunchPlan]:
             continue
         for order in ("xcd", "target"):
             for persist in (0, 2, 4):
-                out.append(LaunchPlan(th, order, persist))
+                for depth in (2, 3):
+                    if L.fvr_sncc_has_depth(flags, th, depth):
+                        out.append(LaunchPlan(th, order, persist, depth))
     return out
 
 

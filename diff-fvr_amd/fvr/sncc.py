@@ -43,13 +43,15 @@ class LaunchPlan:
     view's target while it walks every pose.
     persist: workgroups per CU (0: one workgroup per tile slot; k: k * 256
     persistent workgroups walking the slots).
+    depth: samples whose loads are in flight per lane (2, or 3 for the fold kernel).
     """
     tile_h: int = 64
     order: str = "xcd"
     persist: int = 0
+    depth: int = 2  # samples in flight per lane (the Hermitian-fold kernel also has 3)
 
     def variant(self, code: int, flags: int) -> int:
-        return code | (flags << 4) | (self.tile_h << 8)
+        return code | (flags << 4) | (self.tile_h << 8) | (self.depth << 16)
 
 
 N_CU = 256  # MI355X compute units (8 XCDs x 32)
@@ -228,8 +230,9 @@ class BiplanarEvaluator:
         """Choose tile height / order / persistence (rebuilds the item templates)."""
         if plan.order not in ("xcd", "target") or plan.persist < 0:
             raise ValueError(f"bad launch plan {plan}")
-        if not _lib.lib().fvr_sncc_has_tile_h(self.flags, plan.tile_h):
-            raise ValueError(f"no fused-kernel variant for flags {self.flags} with tile height {plan.tile_h}")
+        if not _lib.lib().fvr_sncc_has_depth(self.flags, plan.tile_h, plan.depth):
+            raise ValueError(f"no fused-kernel variant for flags {self.flags} with tile height {plan.tile_h}, "
+                             f"depth {plan.depth}")
         self.plan = plan
         self.rows = plan.tile_h
         tgt = self.folded if self.herm else self.targets

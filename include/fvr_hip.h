@@ -156,8 +156,13 @@ int fvr_sncc_tile_rows(void);
  * tile_h = 64; 32 and 128 for the batched forms (adjoint, without slice output). */
 int fvr_sncc_has_tile_h(uint32_t flags, int32_t tile_h);
 
+/* 1 when the Hermitian-fold kernel has a variant with `depth` samples in flight per
+ * lane (2 or 3; every other kernel has depth 2 only). */
+int fvr_sncc_has_depth(uint32_t flags, int32_t tile_h, int32_t depth);
+
 /* Launch the fused kernel over n_tiles tiles (the sum of items' tile counts).
- * variant = dtype | (flags << 4) | (tile_h << 8) (tile_h 0 = 64); every item of
+ * variant = dtype | (flags << 4) | (tile_h << 8) | (depth << 16) (tile_h 0 = 64,
+ * depth 0 = 2: samples in flight per lane, fvr_sncc_has_depth); every item of
  * one launch shares flags and tile_h.  Items whose layout index is not below
  * n_layouts, or whose tile_h differs, contribute zero partials.
  * partials: float[n_tiles][FVR_SNCC_NMOM] (caller buffer).

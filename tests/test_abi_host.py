@@ -116,6 +116,10 @@ def test_kernel_variant_table():
             want = folded_ok and (bool(f & HERM) or (bool(f & A) and not f & WS))
             assert L.fvr_sncc_has_tile_h(f, th) == int(want), (f, th)
         assert L.fvr_sncc_has_tile_h(f, 48) == 0
+        for th in (32, 64, 128):
+            assert L.fvr_sncc_has_depth(f, th, 2) == L.fvr_sncc_has_tile_h(f, th)
+            assert L.fvr_sncc_has_depth(f, th, 3) == int(bool(f & HERM) and folded_ok)
+            assert L.fvr_sncc_has_depth(f, th, 4) == 0
     assert L.fvr_sncc_has_tile_h(16, 64) == 0
 
 
