@@ -366,7 +366,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-multistart", action="store_true", help="skip the ragged multistart workload")
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replay timing")
     ap.add_argument("--plan", default="auto",
-                    help="fused-kernel launch plan 'tile_h/order/persist' (e.g. 64/target/0) or 'auto' (autotune)")
+                    help="fused-kernel launch plan 'tile_h/order/persist[/depth]' (e.g. 64/target/0/3) or 'auto' "
+                         "(autotune)")
     ap.add_argument("--no-footprint", action="store_true",
                     help="skip the distinct-entry count (roofline fields null; for profiler runs)")
     args = ap.parse_args(argv)
@@ -401,8 +402,8 @@ def main(argv=None) -> int:
         if args.plan == "auto":
             plan, timings = autotune(ev, P, reps=5)
         else:
-            th, order, persist = args.plan.split("/")
-            plan = LaunchPlan(int(th), order, int(persist))
+            th, order, persist, *depth = args.plan.split("/")
+            plan = LaunchPlan(int(th), order, int(persist), int(depth[0]) if depth else 2)
             ev.set_plan(plan)
         step_ms, kern_ms, _ = time_batched(ev, P, args.steps, args.warmup)
         graph_ms = time_graph(ev, P, args.steps, args.warmup) if not args.no_graph else None

@@ -19,6 +19,7 @@
 // diff_fvr.py:184-195 (NCC), gradients.py:27-74 (translation closed form);
 // torch_engine.py:176-290 with FVR_ITEM_HP (half-bin grid, HP filter).
 #include <cmath>
+#include <type_traits>
 
 #include "fvr_common.h"
 
@@ -66,12 +67,13 @@ struct Raw<FVR_C32> {
     q[2] = *reinterpret_cast<gp>(b1 + off);
     q[3] = *reinterpret_cast<gp>(b1 + offr);
   }
-  // corner pair k: the low corner's value and the fast-axis difference (high - low)
-  __device__ __forceinline__ void pair(int k, float& ar, float& ai, float& dr, float& di) const {
-    ar = q[k].x;
-    ai = q[k].y;
-    dr = q[k].z - ar;
-    di = q[k].w - ai;
+  // corner pair k along the fast axis: the fast-axis difference d = high - low and the
+  // value e = low + t d at fraction t
+  __device__ __forceinline__ void lerp(int k, float t, float& er, float& ei, float& dr, float& di) const {
+    dr = q[k].z - q[k].x;
+    di = q[k].w - q[k].y;
+    er = fmaf(t, dr, q[k].x);
+    ei = fmaf(t, di, q[k].y);
   }
 };
 
@@ -103,20 +105,22 @@ struct RawQ {
     s = *reinterpret_cast<gfptr>(scales + (r << 2));
   }
   __device__ __forceinline__ float scale() const { return s; }
-  // corner pair k in units of the scale: the low corner's value and the fast-axis
-  // difference (high - low).  FP16: the difference is one v_fma_mix_f32 each
-  // (f16 operand converted inside the FMA: h * 1 - low).
-  __device__ __forceinline__ void pair(int k, float& ar, float& ai, float& dr, float& di) const {
+  // corner pair k along the fast axis, in units of the scale: the difference
+  // d = high - low and the value e = low + t d at fraction t.  FP16: four
+  // v_fma_mix_f32 that read the f16 halves directly (the conversion inside the FMA is
+  // exact, so d and e round as the float32 forms high - low and fma(t, d, low) do).
+  __device__ __forceinline__ void lerp(int k, float t, float& er, float& ei, float& dr, float& di) const {
     if (DT == FVR_C16F) {
-      ar = h2f(uint16_t(q[k].x & 0xFFFFu));
-      ai = h2f(uint16_t(q[k].x >> 16));
-      asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel_hi:[1,0,0]" : "=v"(dr) : "v"(q[k].y), "v"(ar));
-      asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(di) : "v"(q[k].y), "v"(ai));
+      asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel_hi:[1,0,1]" : "=v"(dr) : "v"(q[k].y), "v"(q[k].x));
+      asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel:[1,0,1] op_sel_hi:[1,0,1]" : "=v"(di) : "v"(q[k].y), "v"(q[k].x));
+      asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel_hi:[0,0,1]" : "=v"(er) : "v"(t), "v"(dr), "v"(q[k].x));
+      asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel:[0,0,1] op_sel_hi:[0,0,1]" : "=v"(ei) : "v"(t), "v"(di), "v"(q[k].x));
     } else {
-      ar = deq<DT>(q[k].x & 0xFFFFu);
-      ai = deq<DT>(q[k].x >> 16);
+      const float ar = deq<DT>(q[k].x & 0xFFFFu), ai = deq<DT>(q[k].x >> 16);
       dr = deq<DT>(q[k].y & 0xFFFFu) - ar;
       di = deq<DT>(q[k].y >> 16) - ai;
+      er = fmaf(t, dr, ar);
+      ei = fmaf(t, di, ai);
     }
   }
 };
@@ -288,15 +292,11 @@ __device__ __forceinline__ void sncc_tile(const fvr_layout* __restrict__ layouts
     const float ts = sm.ts, tm = sm.tm, tf = sm.tf;
     const Raw<DT>& q = sm.raw;
     // fast-axis lerps (d = V1 - V0, e = V0 + tf d) for the four (slow, mid) pairs
-    float a0r, a0i, b0r, b0i, c0r, c0i, e0r_, e0i_, d00r, d00i, d01r, d01i, d10r, d10i, d11r, d11i;
-    q.pair(0, a0r, a0i, d00r, d00i);
-    q.pair(1, b0r, b0i, d01r, d01i);
-    q.pair(2, c0r, c0i, d10r, d10i);
-    q.pair(3, e0r_, e0i_, d11r, d11i);
-    const float e00r = fmaf(tf, d00r, a0r), e00i = fmaf(tf, d00i, a0i);
-    const float e01r = fmaf(tf, d01r, b0r), e01i = fmaf(tf, d01i, b0i);
-    const float e10r = fmaf(tf, d10r, c0r), e10i = fmaf(tf, d10i, c0i);
-    const float e11r = fmaf(tf, d11r, e0r_), e11i = fmaf(tf, d11i, e0i_);
+    float e00r, e00i, e01r, e01i, e10r, e10i, e11r, e11i, d00r, d00i, d01r, d01i, d10r, d10i, d11r, d11i;
+    q.lerp(0, tf, e00r, e00i, d00r, d00i);
+    q.lerp(1, tf, e01r, e01i, d01r, d01i);
+    q.lerp(2, tf, e10r, e10i, d10r, d10i);
+    q.lerp(3, tf, e11r, e11i, d11r, d11i);
     // mid-axis
     const float y0r = e01r - e00r, y0i = e01i - e00i;
     const float y1r = e11r - e10r, y1i = e11i - e10i;
@@ -443,9 +443,13 @@ __device__ __forceinline__ SpecialSlot special_slot(int s, int j, int hh, int hw
   return r;
 }
 
+// opts bit kHermExactBox: every wave takes the per-sample box tests (no interior /
+// outside classes); the sums are the same bit for bit
+constexpr int kHermExactBox = 1;
+
 template <int DT, bool ADJ, int R, int DEPTH>
 __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ layouts, int n_layouts,
-                                               const fvr_sncc_item& it, int lt, int tile,
+                                               const fvr_sncc_item& it, int lt, int tile, int opts,
                                                float* __restrict__ partials) {
   static_assert(R >= 1 && R <= 64, "rows per wave: <= 64 (row phasors live in lanes)");
   static_assert(DEPTH >= 2 && DEPTH <= 4, "samples in flight per lane");
@@ -482,51 +486,37 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
   const uint64_t K1 = h1 + (1ull << 32) + (h1 & 0xFFFFFFFFull);
   const uint64_t K2 = h2 + (1ull << 32) + (h2 & 0xFFFFFFFFull);
   const uint64_t M0 = K0 - h0, M1 = K1 - h1, M2 = K2 - h2;
-  // per-lane column part of the position, phase and weight
-  const int64_t bj0 = it.c[0] + int64_t(j) * it.ustep[0];
-  const int64_t bj1 = it.c[1] + int64_t(j) * it.ustep[1];
-  const int64_t bj2 = it.c[2] + int64_t(j) * it.ustep[2];
+  // per-lane column part of the position, phase and weight; lanes past the last column
+  // take the positions and target of column W - 1 (their box weights are 0), so that
+  // their loads stay on the rows and columns the wave class below checks
+  const int jc = colok ? j : W - 1;
+  const int64_t bj0 = it.c[0] + int64_t(jc) * it.ustep[0];
+  const int64_t bj1 = it.c[1] + int64_t(jc) * it.ustep[1];
+  const int64_t bj2 = it.c[2] + int64_t(jc) * it.ustep[2];
   const cpx colph = phasor(it.ph_col0 + uint32_t(j) * it.ph_col);
   const float cwj = it.cw0 + float(j) * it.cws;
   // row phasors of this wave's regular rows (lane k <-> row h/2 + r0 + k)
   const cpx rowph = phasor(it.ph_row0 + uint32_t(hh + r0 + lane) * it.ph_row);
   // regular rows are consecutive slots: their position and target offset are stepped
   // (issue side) and their row weight too (consume side), from the wave's first regular row
+  // (stepped positions carry the storage offset lo_int in their high word: the high word
+  // is then the low corner's storage index)
   const int ireg = hh + r0 + (first ? 1 : 0);
-  int64_t q0 = bj0 + int64_t(ireg) * it.vstep[0];
-  int64_t q1 = bj1 + int64_t(ireg) * it.vstep[1];
-  int64_t q2 = bj2 + int64_t(ireg) * it.vstep[2];
+  const int64_t lob = int64_t(lo_int) << 32;
+  int64_t q0 = bj0 + int64_t(ireg) * it.vstep[0] + lob;
+  int64_t q1 = bj1 + int64_t(ireg) * it.vstep[1] + lob;
+  int64_t q2 = bj2 + int64_t(ireg) * it.vstep[2] + lob;
   const int64_t v0 = it.vstep[0], v1 = it.vstep[1], v2 = it.vstep[2];
-  uint32_t toffr = uint32_t(ireg * W + j) * 16u;
+  uint32_t toffr = uint32_t(ireg * W + jc) * 16u;
   const uint32_t tstep = uint32_t(W) * 16u;
   float rwr = it.rw0 + float(ireg) * it.rws;
   const float rws = it.rws;
 
-  // stage 1: the sample of slot s -- position, box tests, loads
-  auto issue = [&](SmpH<DT>& sm, int s) {
-    int64_t p0, p1, p2;
-    bool valid, mirror;
-    uint32_t toff;
-    if (s < nreg && !(first && s == 0)) {  // a regular row i = h/2 + r0 + s (> h/2): every column but 0 paired
-      p0 = q0;
-      p1 = q1;
-      p2 = q2;
-      q0 += v0;
-      q1 += v1;
-      q2 += v2;
-      valid = colok;
-      mirror = j >= 1;
-      toff = toffr;
-      toffr += tstep;
-    } else {
-      const SpecialSlot sp = special_slot(s, j, hh, hw, colok);
-      p0 = it.c[0] + int64_t(sp.jj) * it.ustep[0] + int64_t(sp.ii) * it.vstep[0];
-      p1 = it.c[1] + int64_t(sp.jj) * it.ustep[1] + int64_t(sp.ii) * it.vstep[1];
-      p2 = it.c[2] + int64_t(sp.jj) * it.ustep[2] + int64_t(sp.ii) * it.vstep[2];
-      valid = sp.valid;
-      mirror = sp.mirror;
-      toff = uint32_t(sp.ii * W + sp.jj) * 16u;
-    }
+  const float wEl = j >= 1 ? 2.f : 1.f;  // interior waves: the sample and, but in column 0, its mirror
+
+  // box tests, loads and fractional coordinates of a sample at (p0, p1, p2); valid /
+  // mirror: whether the sample / its mirror take part; toff: its folded-target offset
+  auto load_tested = [&](SmpH<DT>& sm, int64_t p0, int64_t p1, int64_t p2, bool valid, bool mirror, uint32_t toff) {
     const bool bp = valid & (uint64_t(p0) <= h0) & (uint64_t(p1) <= h1) & (uint64_t(p2) <= h2);
     const bool bq = valid & mirror & (uint64_t(p0) >= M0) & (uint64_t(p0) <= K0) & (uint64_t(p1) >= M1) &
                     (uint64_t(p1) <= K1) & (uint64_t(p2) >= M2) & (uint64_t(p2) <= K2);
@@ -539,42 +529,45 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
     sm.tf = float(uint32_t(p2)) * kTwoM32;
     const uint32_t S = fs + lo_int, M = fm + lo_int, F = ff + lo_int;
     sm.raw.load(b0, b1, inb ? Raw<DT>::offset(S, M, F, planeb, rowb) : guard, rowb, scales, S, M, F, Bm, Bf);
-    sm.t = *reinterpret_cast<gf4ptr>(TT + (valid ? toff : 0u));
+    sm.t = *reinterpret_cast<gf4ptr>(TT + toff);
     sm.bp = bp;
     sm.bq = bq;
   };
 
-  Acc ac = {};
-  // stage 2: phase and weights of slot s, interpolation, the moment updates
-  auto consume = [&](const SmpH<DT>& sm, int s) {
-    float phr, phi, cw, rw;
-    if (s < nreg && !(first && s == 0)) {
-      const float prr = readlane_f(rowph.r, s), pri = readlane_f(rowph.i, s);
-      phr = colph.r * prr - colph.i * pri;
-      phi = fmaf(colph.r, pri, colph.i * prr);
-      cw = cwj;
-      rw = rwr;
-      rwr += rws;
+  // stage 1 of a regular row (rows in order; position and target offset stepped):
+  // interior waves skip the box tests (every sample and its mirror are inside)
+  auto issue = [&](SmpH<DT>& sm, auto interior) {
+    if constexpr (decltype(interior)::value) {
+      const uint32_t S = uint32_t(uint64_t(q0) >> 32);
+      const uint32_t M = uint32_t(uint64_t(q1) >> 32);
+      const uint32_t F = uint32_t(uint64_t(q2) >> 32);
+      sm.ts = float(uint32_t(q0)) * kTwoM32;
+      sm.tm = float(uint32_t(q1)) * kTwoM32;
+      sm.tf = float(uint32_t(q2)) * kTwoM32;
+      sm.raw.load(b0, b1, Raw<DT>::offset(S, M, F, planeb, rowb), rowb, scales, S, M, F, Bm, Bf);
+      sm.t = *reinterpret_cast<gf4ptr>(TT + toffr);
     } else {
-      const SpecialSlot sp = special_slot(s, j, hh, hw, colok);
-      const cpx ph = phasor(it.ph_col0 + uint32_t(sp.jj) * it.ph_col + it.ph_row0 + uint32_t(sp.ii) * it.ph_row);
-      phr = ph.r;
-      phi = ph.i;
-      cw = it.cw0 + float(sp.jj) * it.cws;
-      rw = it.rw0 + float(sp.ii) * it.rws;
+      load_tested(sm, q0 - lob, q1 - lob, q2 - lob, colok, j >= 1, toffr);
     }
-    const float wp = sm.bp ? 1.f : 0.f, wq = sm.bq ? 1.f : 0.f;
+    q0 += v0;
+    q1 += v1;
+    q2 += v2;
+    toffr += tstep;
+  };
+
+  Acc ac = {};
+  // stage 2: interpolation and the moment updates of a sample with phase (phr, phi),
+  // moment weights (cw, rw) and box weights (wp, wq) of the sample and its mirror
+  // (interior: every lane valid, wp = wq = 1; column 0's mirror part of the folded target is 0)
+  auto accumulate = [&](const SmpH<DT>& sm, float phr, float phi, float cw, float rw, float wp, float wq,
+                        auto interior) {
     const float ts = sm.ts, tm = sm.tm, tf = sm.tf;
     const Raw<DT>& q = sm.raw;
-    float a0r, a0i, b0r, b0i, c0r, c0i, e0r_, e0i_, d00r, d00i, d01r, d01i, d10r, d10i, d11r, d11i;
-    q.pair(0, a0r, a0i, d00r, d00i);
-    q.pair(1, b0r, b0i, d01r, d01i);
-    q.pair(2, c0r, c0i, d10r, d10i);
-    q.pair(3, e0r_, e0i_, d11r, d11i);
-    const float e00r = fmaf(tf, d00r, a0r), e00i = fmaf(tf, d00i, a0i);
-    const float e01r = fmaf(tf, d01r, b0r), e01i = fmaf(tf, d01i, b0i);
-    const float e10r = fmaf(tf, d10r, c0r), e10i = fmaf(tf, d10i, c0i);
-    const float e11r = fmaf(tf, d11r, e0r_), e11i = fmaf(tf, d11i, e0i_);
+    float e00r, e00i, e01r, e01i, e10r, e10i, e11r, e11i, d00r, d00i, d01r, d01i, d10r, d10i, d11r, d11i;
+    q.lerp(0, tf, e00r, e00i, d00r, d00i);
+    q.lerp(1, tf, e01r, e01i, d01r, d01i);
+    q.lerp(2, tf, e10r, e10i, d10r, d10i);
+    q.lerp(3, tf, e11r, e11i, d11r, d11i);
     const float y0r = e01r - e00r, y0i = e01i - e00i;
     const float y1r = e11r - e10r, y1i = e11i - e10i;
     const float f0r = fmaf(tm, y0r, e00r), f0i = fmaf(tm, y0i, e00i);
@@ -585,11 +578,18 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
     // folded target T'' = bp T + bq conj T(mirror); W = phase conj(T'') sc, with I, dI
     // in units of the brick scale sc (E and the H moments carry sc^2 through J)
     const float sc = q.scale();
-    const float Tr = sc * fmaf(wq, sm.t.z, wp * sm.t.x);
-    const float Ti = sc * fmaf(wq, sm.t.w, wp * sm.t.y);
+    float Tr, Ti, wE;
+    if constexpr (decltype(interior)::value) {
+      Tr = sc * (sm.t.x + sm.t.z);
+      Ti = sc * (sm.t.y + sm.t.w);
+      wE = wEl * (sc * sc);
+    } else {
+      Tr = sc * fmaf(wq, sm.t.z, wp * sm.t.x);
+      Ti = sc * fmaf(wq, sm.t.w, wp * sm.t.y);
+      wE = (wp + wq) * (sc * sc);
+    }
     const float Wr = fmaf(phr, Tr, phi * Ti);
     const float Wi = fmaf(phi, Tr, -phr * Ti);
-    const float wE = (wp + wq) * (sc * sc);
     const float Jr = Ir * wE, Ji = Ii * wE;
     const float IWr = fmaf(Ir, Wr, -Ii * Wi);
     const float IWi = fmaf(Ir, Wi, Ii * Wr);
@@ -619,25 +619,77 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
     }
   };
 
-  // DEPTH samples in flight per lane: the loads of slot s + DEPTH - 1 are issued before
-  // slot s is consumed (ring of DEPTH buffers, indices fixed by the unrolled inner loop;
+  // stage 2 of a regular row whose phasor sits in lane k (rows in order: rw is stepped)
+  auto consume = [&](const SmpH<DT>& sm, int k, auto interior) {
+    const float prr = readlane_f(rowph.r, k), pri = readlane_f(rowph.i, k);
+    const float phr = colph.r * prr - colph.i * pri;
+    const float phi = fmaf(colph.r, pri, colph.i * prr);
+    const float rw = rwr;
+    rwr += rws;
+    accumulate(sm, phr, phi, cwj, rw, sm.bp ? 1.f : 0.f, sm.bq ? 1.f : 0.f, interior);
+  };
+
+  // a special slot (row h/2 with its remapped lanes, or row 0), one sample with its tests
+  auto special = [&](int s) {
+    const SpecialSlot sp = special_slot(s, j, hh, hw, colok);
+    SmpH<DT> sm;
+    load_tested(sm, it.c[0] + int64_t(sp.jj) * it.ustep[0] + int64_t(sp.ii) * it.vstep[0],
+                it.c[1] + int64_t(sp.jj) * it.ustep[1] + int64_t(sp.ii) * it.vstep[1],
+                it.c[2] + int64_t(sp.jj) * it.ustep[2] + int64_t(sp.ii) * it.vstep[2], sp.valid, sp.mirror,
+                sp.valid ? uint32_t(sp.ii * W + sp.jj) * 16u : 0u);
+    const cpx ph = phasor(it.ph_col0 + uint32_t(sp.jj) * it.ph_col + it.ph_row0 + uint32_t(sp.ii) * it.ph_row);
+    accumulate(sm, ph.r, ph.i, it.cw0 + float(sp.jj) * it.cws, it.rw0 + float(sp.ii) * it.rws, sm.bp ? 1.f : 0.f,
+               sm.bq ? 1.f : 0.f, std::false_type{});
+  };
+
+  // Class of the wave's regular rows.  A position is an exact affine integer function of
+  // (row, column), so along a lane's column its extremes are at the first and the last
+  // regular row: when every lane is a column of the slice and both lie in [M, h] on every
+  // axis for every lane, every sample and its mirror are inside the box (interior); when on
+  // some axis both lie below 0 or both above K for every lane, none is (outside: the rows
+  // contribute exactly 0 and are skipped).
+  const int kfirst = first ? 1 : 0;
+  const int n_r = nreg - kfirst;
+  int cls = 0;  // 0: per-sample tests, 1: interior, 2: outside
+  if (n_r > 0 && !(opts & kHermExactBox)) {
+    const int64_t f0 = q0 - lob, f1 = q1 - lob, f2 = q2 - lob;
+    const int64_t l0 = f0 + int64_t(n_r - 1) * v0, l1 = f1 + int64_t(n_r - 1) * v1, l2 = f2 + int64_t(n_r - 1) * v2;
+    const bool in = colok & (uint64_t(f0) >= M0) & (uint64_t(f0) <= h0) & (uint64_t(l0) >= M0) &
+                    (uint64_t(l0) <= h0) & (uint64_t(f1) >= M1) & (uint64_t(f1) <= h1) & (uint64_t(l1) >= M1) & (uint64_t(l1) <= h1) &
+                    (uint64_t(f2) >= M2) & (uint64_t(f2) <= h2) & (uint64_t(l2) >= M2) & (uint64_t(l2) <= h2);
+    const bool out = !colok || (f0 < 0 && l0 < 0) || (f0 > int64_t(K0) && l0 > int64_t(K0)) ||
+                     (f1 < 0 && l1 < 0) || (f1 > int64_t(K1) && l1 > int64_t(K1)) || (f2 < 0 && l2 < 0) ||
+                     (f2 > int64_t(K2) && l2 > int64_t(K2));
+    cls = __all(in) ? 1 : (__all(out) ? 2 : 0);
+  }
+
+  // DEPTH samples in flight per lane: the loads of row t + DEPTH - 1 are issued before
+  // row t is consumed (ring of DEPTH buffers, indices fixed by the unrolled inner loop;
   // every bound below is wave-uniform)
-  SmpH<DT> ring[DEPTH];
+  auto run = [&](auto interior) {
+    SmpH<DT> ring[DEPTH];
 #pragma unroll
-  for (int d = 0; d < DEPTH - 1; ++d)
-    if (d < nslot) issue(ring[d], d);
+    for (int d = 0; d < DEPTH - 1; ++d)
+      if (d < n_r) issue(ring[d], interior);
 #pragma unroll 1
-  for (int s0 = 0; s0 < nslot; s0 += DEPTH) {
+    for (int t0 = 0; t0 < n_r; t0 += DEPTH) {
 #pragma unroll
-    for (int d = 0; d < DEPTH; ++d) {
-      const int s = s0 + d;
-      if (s < nslot) {
-        if (s + DEPTH - 1 < nslot) issue(ring[(d + DEPTH - 1) % DEPTH], s + DEPTH - 1);
-        consume(ring[d], s);
-        __builtin_amdgcn_sched_barrier(0);
+      for (int d = 0; d < DEPTH; ++d) {
+        const int t = t0 + d;
+        if (t < n_r) {
+          if (t + DEPTH - 1 < n_r) issue(ring[(d + DEPTH - 1) % DEPTH], interior);
+          consume(ring[d], t + kfirst, interior);
+          __builtin_amdgcn_sched_barrier(0);
+        }
       }
     }
-  }
+  };
+
+  // per-lane order of the sums: row h/2 (first wave), the regular rows, row 0 (first wave)
+  if (first) special(0);
+  if (cls == 1) run(std::true_type{});
+  else if (cls == 0) run(std::false_type{});
+  if (first) special(nreg);
   tile_reduce(ac, 1.f, tile, partials);
 }
 
@@ -645,11 +697,11 @@ template <int DT, bool ADJ, int R, int DEPTH>
 __global__ __launch_bounds__(256) void sncc_herm_kernel(const fvr_layout* __restrict__ layouts, int n_layouts,
                                                         const fvr_sncc_item* __restrict__ items, int n_items,
                                                         int n_tiles, const int2* __restrict__ map, int n_slots,
-                                                        float* __restrict__ partials) {
+                                                        int opts, float* __restrict__ partials) {
   for (int w = blockIdx.x; w < n_slots; w += gridDim.x) {
     int item, lt, tile;
     if (!resolve_tile<R>(w, map, items, n_items, n_tiles, n_slots, item, lt, tile)) continue;
-    sncc_herm_tile<DT, ADJ, R, DEPTH>(layouts, n_layouts, items[item], lt, tile, partials);
+    sncc_herm_tile<DT, ADJ, R, DEPTH>(layouts, n_layouts, items[item], lt, tile, opts, partials);
   }
 }
 
@@ -693,6 +745,7 @@ struct Launch {
   int n_slots;
   float* partials;
   int depth;  // samples in flight per lane (Hermitian-fold kernel: 2 or 3)
+  int opts;   // Hermitian-fold kernel options (kHermExactBox)
 };
 
 template <int DT, bool ADJ, bool HP, bool WS, int R>
@@ -705,10 +758,10 @@ template <int DT, bool ADJ, int R>
 static void launch_herm(const Launch& a) {
   if (a.depth == 3)
     hipLaunchKernelGGL((sncc_herm_kernel<DT, ADJ, R, 3>), a.grid, dim3(256), 0, a.st, a.lay, a.n_lay, a.items,
-                       a.n_items, a.n_tiles, a.map, a.n_slots, a.partials);
+                       a.n_items, a.n_tiles, a.map, a.n_slots, a.opts, a.partials);
   else
     hipLaunchKernelGGL((sncc_herm_kernel<DT, ADJ, R, 2>), a.grid, dim3(256), 0, a.st, a.lay, a.n_lay, a.items,
-                       a.n_items, a.n_tiles, a.map, a.n_slots, a.partials);
+                       a.n_items, a.n_tiles, a.map, a.n_slots, a.opts, a.partials);
 }
 
 // Variants: every flag combination at tile_h 64; tile_h 32 / 128 for the batched
@@ -784,7 +837,9 @@ int fvr_sncc_eval_plan(const fvr_layout* layouts_dev, int32_t n_layouts, const f
   int depth = int((variant >> 16) & 0xF);
   if (depth == 0) depth = 2;
   FVR_CHECK_ARG(dt == FVR_C32 || dt == FVR_C16F || dt == FVR_C16B, "fvr_sncc_eval: bad dtype %d", dt);
-  FVR_CHECK_ARG((variant >> 20) == 0u, "fvr_sncc_eval: bad variant %u", variant);
+  const bool exact_box = (variant >> 20) & 1u;
+  FVR_CHECK_ARG((variant >> 21) == 0u, "fvr_sncc_eval: bad variant %u", variant);
+  FVR_CHECK_ARG(!exact_box || (flags & FVR_ITEM_HERM), "fvr_sncc_eval: exact-box bit without the fold flag");
   FVR_CHECK_ARG(has_variant(flags, tile_h, depth), "fvr_sncc_eval: no kernel for flags %u with tile_h %d, depth %d",
                 flags, tile_h, depth);
   if (tile_map) {
@@ -806,6 +861,7 @@ int fvr_sncc_eval_plan(const fvr_layout* layouts_dev, int32_t n_layouts, const f
   a.n_slots = n_slots;
   a.partials = partials;
   a.depth = depth;
+  a.opts = exact_box ? kHermExactBox : 0;
   if (dt == FVR_C32) dispatch_dt<FVR_C32>(flags, tile_h, a);
   else if (dt == FVR_C16F) dispatch_dt<FVR_C16F>(flags, tile_h, a);
   else dispatch_dt<FVR_C16B>(flags, tile_h, a);

@@ -44,14 +44,17 @@ class LaunchPlan:
     persist: workgroups per CU (0: one workgroup per tile slot; k: k * 256
     persistent workgroups walking the slots).
     depth: samples whose loads are in flight per lane (2, or 3 for the fold kernel).
+    exact_box: fold kernel only -- every wave takes the per-sample box tests instead of
+    the interior / outside wave classes (same sums bit for bit; for tests).
     """
     tile_h: int = 64
     order: str = "xcd"
     persist: int = 0
     depth: int = 2  # samples in flight per lane (the Hermitian-fold kernel also has 3)
+    exact_box: bool = False
 
     def variant(self, code: int, flags: int) -> int:
-        return code | (flags << 4) | (self.tile_h << 8) | (self.depth << 16)
+        return code | (flags << 4) | (self.tile_h << 8) | (self.depth << 16) | (int(self.exact_box) << 20)
 
 
 N_CU = 256  # MI355X compute units (8 XCDs x 32)
@@ -230,6 +233,8 @@ class BiplanarEvaluator:
         """Choose tile height / order / persistence (rebuilds the item templates)."""
         if plan.order not in ("xcd", "target") or plan.persist < 0:
             raise ValueError(f"bad launch plan {plan}")
+        if plan.exact_box and not self.flags & _lib.ITEM_HERM:
+            raise ValueError("exact_box applies to the Hermitian-fold kernel only")
         if not _lib.lib().fvr_sncc_has_depth(self.flags, plan.tile_h, plan.depth):
             raise ValueError(f"no fused-kernel variant for flags {self.flags} with tile height {plan.tile_h}, "
                              f"depth {plan.depth}")

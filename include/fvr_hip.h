@@ -161,9 +161,11 @@ int fvr_sncc_has_tile_h(uint32_t flags, int32_t tile_h);
 int fvr_sncc_has_depth(uint32_t flags, int32_t tile_h, int32_t depth);
 
 /* Launch the fused kernel over n_tiles tiles (the sum of items' tile counts).
- * variant = dtype | (flags << 4) | (tile_h << 8) | (depth << 16) (tile_h 0 = 64,
- * depth 0 = 2: samples in flight per lane, fvr_sncc_has_depth); every item of
- * one launch shares flags and tile_h.  Items whose layout index is not below
+ * variant = dtype | (flags << 4) | (tile_h << 8) | (depth << 16) | (exact_box << 20)
+ * (tile_h 0 = 64, depth 0 = 2: samples in flight per lane, fvr_sncc_has_depth;
+ * exact_box, fold kernel only: every wave takes the per-sample box tests instead of
+ * the interior / outside wave classes -- the same sums bit for bit, slower; for
+ * tests); every item of one launch shares flags and tile_h.  Items whose layout index is not below
  * n_layouts, or whose tile_h differs, contribute zero partials.
  * partials: float[n_tiles][FVR_SNCC_NMOM] (caller buffer).
  * Tile order: workgroup slot w (XCD w % 8) takes tile (w % 8) * ceil(n_tiles/8) + w / 8,

@@ -74,6 +74,70 @@ def test_launch_plans_agree(vols, herm):
         np.testing.assert_array_equal(G.cpu().numpy(), ref[plan.tile_h][1], err_msg=str(plan))
 
 
+def _fold_wave_classes(items: np.ndarray, tile_h: int) -> np.ndarray:
+    """Counts of (per-sample tests, interior, outside) waves of fold items: the classes
+    the fold kernel gives the regular rows of each wave, from the items' 32.32 geometry
+    (used here to show that the poses of a test reach every class)."""
+    R = tile_h // 4
+    counts = np.zeros(3, dtype=np.int64)
+    for it in items:
+        h, w = int(it["h"]), int(it["w"])
+        hh = h // 2
+        c = np.array(it["c"], dtype=np.int64)
+        u = np.array(it["ustep"], dtype=np.int64)
+        v = np.array(it["vstep"], dtype=np.int64)
+        hi = np.array(it["box_hi"], dtype=np.int64)
+        K = hi + (1 << 32) + (hi & 0xFFFFFFFF)
+        M = K - hi
+        j = np.arange(int(it["tiles_x"]) * 64)
+        colok = j < w
+        jc = np.minimum(j, w - 1)
+        for r0 in range(0, hh, R):
+            nreg = min(R, hh - r0)
+            first = int(r0 == 0)
+            n_r = nreg - first
+            if n_r <= 0:
+                continue
+            for tx in range(int(it["tiles_x"])):
+                cols = slice(tx * 64, tx * 64 + 64)
+                q = c[None, :] + jc[cols, None] * u[None, :] + (hh + r0 + first) * v[None, :]
+                last = q + (n_r - 1) * v[None, :]
+                inside = np.all(colok[cols]) and np.all((q >= M) & (q <= hi) & (last >= M) & (last <= hi))
+                away = ((q < 0) & (last < 0)) | ((q > K) & (last > K))
+                outside = np.all(~colok[cols] | np.any(away, axis=1))
+                counts[1 if inside else (2 if outside else 0)] += 1
+    return counts
+
+
+@pytest.mark.parametrize("storage", ["fp16", "bf16", "f32"])
+def test_fold_wave_classes_equal_per_sample_tests(storage):
+    """The fold kernel's interior / outside wave classes give the same sums, bit for bit,
+    as per-sample box tests on every wave, for every tile height and depth; the poses
+    (rotations up to 40 degrees on a 144 x 144 slice: its last column tile holds 16 columns)
+    reach all three classes and lanes past the last column."""
+    from fvr.multistart import candidate_plans
+    from fvr.sncc import BiplanarEvaluator, LaunchPlan
+    r = _renderer(72, 21, storage)
+    ta, tl = _targets(r, np.random.default_rng(21).uniform(-4, 4, 6))
+    ev = BiplanarEvaluator(r.packed, ta, tl, adjoint=True)
+    from fvr.phantom import random_poses
+    P = torch.from_numpy(np.concatenate([random_poses(9, 40.0, 8.0, 5), random_poses(6, 5.0, 8.0, 6)])).to("cuda")
+    seen = np.zeros(3, dtype=np.int64)
+    plans = [p for p in candidate_plans(ev.flags) if p.order == "xcd" and p.persist == 0]
+    assert len(plans) == 6
+    for plan in plans:
+        ev.set_plan(LaunchPlan(plan.tile_h, depth=plan.depth, exact_box=True))
+        L0, G0 = (x.cpu().numpy().copy() for x in ev.evaluate_device(P))
+        ev.set_plan(plan)
+        L1, G1 = (x.cpu().numpy().copy() for x in ev.evaluate_device(P))
+        np.testing.assert_array_equal(L1, L0, err_msg=str(plan))
+        np.testing.assert_array_equal(G1, G0, err_msg=str(plan))
+        assert np.all(np.isfinite(L1)) and np.all(np.isfinite(G1))
+        seen += _fold_wave_classes(ev.device_items(len(P)), plan.tile_h)
+    assert np.all(seen > 0), seen
+    assert r.packed.shape[2] % 64 != 0 or r.packed.shape[1] % 64 != 0
+
+
 @pytest.mark.parametrize("order", ["xcd", "target"])
 def test_ragged_launch_equals_separate_launches(vols, order):
     from fvr.sncc import BiplanarEvaluator, LaunchPlan, RaggedBiplanarEvaluator
