@@ -322,8 +322,9 @@ def multistart_config5(n_steps: int = 100, lat_reps: int = 100):
             "poses": N, "steps": n_steps, "loop_ms": round(loop_ms, 3), "loop_wall_ms": round(wall_ms, 3),
             "pose_evals_per_s": round(N * n_steps / (loop_ms * 1e-3), 1),
             "fused_step_latency": fused, "per_volume_step_latency_alone": per_volume,
-            "launch_plan": {"tile_h": plan.tile_h, "order": plan.order, "persist": plan.persist},
-            "autotune_ms": {f"{p.tile_h}/{p.order}/{p.persist}": round(t, 4) for p, t in
+            "launch_plan": {"tile_h": plan.tile_h, "order": plan.order, "persist": plan.persist,
+                            "depth": plan.depth},
+            "autotune_ms": {f"{p.tile_h}/{p.order}/{p.persist}/{p.depth}": round(t, 4) for p, t in
                             sorted(timings.items(), key=lambda kv: kv[1])},
             "best_start_mtre_mm": mtre, "build_s": round(build_s, 2)}
 
@@ -502,7 +503,7 @@ def main(argv=None) -> int:
                    "slice_samples_per_view": (2 * args.size) ** 2, "candidates_per_launch": args.n_cand,
                    "spectrum_storage": args.storage,
                    "launch_plan": {"tile_h": head["plan"].tile_h, "order": head["plan"].order,
-                                   "persist": head["plan"].persist,
+                                   "persist": head["plan"].persist, "depth": head["plan"].depth,
                                    "autotuned": args.plan == "auto"}},
         "roofline": {"bound": "hbm", "achieved": None if achieved is None else round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -513,7 +514,7 @@ def main(argv=None) -> int:
                      "schedule": fp.get("schedule"), "samples_per_launch": fp.get("samples"),
                      "bricks": fp["bricks"]},
         "cpu_baseline": cpu,
-        "autotune_ms": {f"{p.tile_h}/{p.order}/{p.persist}": round(t, 4)
+        "autotune_ms": {f"{p.tile_h}/{p.order}/{p.persist}/{p.depth}": round(t, 4)
                         for p, t in sorted(head["autotune"].items(), key=lambda kv: kv[1])},
         "extra": extra,
     }
