@@ -420,29 +420,6 @@ struct SmpH {
   bool bp, bq;  // box tests of the sample and of its mirror
 };
 
-// Row / column of special slot s of the first wave: slot 0 is row h/2 (lanes j < w/2
-// remapped to column 0 of row j), the last slot is the unpaired row 0.
-struct SpecialSlot {
-  int ii, jj;
-  bool valid, mirror;
-};
-__device__ __forceinline__ SpecialSlot special_slot(int s, int j, int hh, int hw, bool colok) {
-  SpecialSlot r;
-  if (s == 0) {
-    const bool remap = j >= 1 && j < hw;
-    r.ii = remap ? j : hh;
-    r.jj = remap ? 0 : j;
-    r.valid = colok && (!remap || j <= hh - 1);
-    r.mirror = j > hw;
-  } else {
-    r.ii = 0;
-    r.jj = j;
-    r.valid = colok;
-    r.mirror = false;
-  }
-  return r;
-}
-
 // opts bit kHermExactBox: every wave takes the per-sample box tests (no interior /
 // outside classes); the sums are the same bit for bit
 constexpr int kHermExactBox = 1;
@@ -451,7 +428,7 @@ template <int DT, bool ADJ, int R, int DEPTH>
 __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ layouts, int n_layouts,
                                                const fvr_sncc_item& it, int lt, int tile, int opts,
                                                float* __restrict__ partials) {
-  static_assert(R >= 1 && R <= 64, "rows per wave: <= 64 (row phasors live in lanes)");
+  static_assert(R >= 1 && 2 * R <= 64, "rows per wave: <= 64 (row phasors live in lanes)");
   static_assert(DEPTH >= 2 && DEPTH <= 4, "samples in flight per lane");
   if (uint32_t(it.layout) >= uint32_t(n_layouts) || it.tile_h != 4 * R) {
     if (threadIdx.x < FVR_SNCC_NMOM) partials[size_t(tile) * FVR_SNCC_NMOM + threadIdx.x] = 0.f;
@@ -463,13 +440,22 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int H = it.h, W = it.w, hh = H >> 1, hw = W >> 1, lo_int = it.lo_int;
   const int tyi = lt / it.tiles_x, txi = lt - tyi * it.tiles_x;
-  const int j = txi * 64 + lane;
+  // One step of a wave covers 2 rows x 32 columns (lane = 32 rl + column): a tile is 32
+  // columns x 8R rows and lane (rl, c) of wave w walks the rows h/2 + r0 + rl + 2t,
+  // t < R.  (A 2 x 32 patch of a tilted slice crosses about half as many (slow, mid)
+  // storage rows per load as a 1 x 64 row does.)
+  const int rl = lane >> 5;
+  const int j = txi * 32 + (lane & 31);
   const bool colok = uint32_t(j) < uint32_t(W);
-  const int r0 = tyi * (4 * R) + wave * R;  // first representative row (offset from h/2)
-  int nreg = hh - r0;
-  nreg = nreg < 0 ? 0 : (nreg > R ? R : nreg);
+  const int r0 = tyi * (8 * R) + wave * (2 * R);  // first row of the wave (offset from h/2)
+  int nrow = hh - r0;                               // rows of this wave
+  nrow = nrow < 0 ? 0 : (nrow > 2 * R ? 2 * R : nrow);
+  // the first wave's step 0 on lane row 0 is row h/2: the sample stands for its mirror only
+  // right of the centre and lanes 1 .. w/2 - 1 are the mirrors of the right half (box
+  // weights below; the folded target's row h/2 is stored with the same masks)
   const bool first = r0 == 0;
-  const int nslot = nreg + (first ? 1 : 0);
+  const int n_r = (nrow + 1) >> 1;  // steps
+  const bool full = nrow == 2 * R;  // every step of every lane is a row of the fold
 
   const uint32_t rowb = L.row_bytes, planeb = L.plane_bytes;
   const gcptr b0 = reinterpret_cast<gcptr>(L.base);
@@ -493,26 +479,29 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
   const int64_t bj0 = it.c[0] + int64_t(jc) * it.ustep[0];
   const int64_t bj1 = it.c[1] + int64_t(jc) * it.ustep[1];
   const int64_t bj2 = it.c[2] + int64_t(jc) * it.ustep[2];
-  const cpx colph = phasor(it.ph_col0 + uint32_t(j) * it.ph_col);
+  // the lane's column phasor carries its row offset rl (exact 32-bit turn counts): the phase
+  // of step t is then colph x rowph(lane 2t)
+  const cpx colph = phasor(it.ph_col0 + uint32_t(j) * it.ph_col + uint32_t(rl) * it.ph_row);
   const float cwj = it.cw0 + float(j) * it.cws;
-  // row phasors of this wave's regular rows (lane k <-> row h/2 + r0 + k)
+  // row phasors of this wave's rows (lane k <-> row h/2 + r0 + k, k < 2R <= 64)
   const cpx rowph = phasor(it.ph_row0 + uint32_t(hh + r0 + lane) * it.ph_row);
-  // regular rows are consecutive slots: their position and target offset are stepped
-  // (issue side) and their row weight too (consume side), from the wave's first regular row
-  // (stepped positions carry the storage offset lo_int in their high word: the high word
-  // is then the low corner's storage index)
-  const int ireg = hh + r0 + (first ? 1 : 0);
+  // a lane's rows are every other row: position, target offset and row weight are stepped
+  // by two rows from the lane's first row (stepped positions carry the storage offset
+  // lo_int in their high word: the high word is then the low corner's storage index)
+  const int irow = hh + r0 + rl;
   const int64_t lob = int64_t(lo_int) << 32;
-  int64_t q0 = bj0 + int64_t(ireg) * it.vstep[0] + lob;
-  int64_t q1 = bj1 + int64_t(ireg) * it.vstep[1] + lob;
-  int64_t q2 = bj2 + int64_t(ireg) * it.vstep[2] + lob;
-  const int64_t v0 = it.vstep[0], v1 = it.vstep[1], v2 = it.vstep[2];
-  uint32_t toffr = uint32_t(ireg * W + jc) * 16u;
-  const uint32_t tstep = uint32_t(W) * 16u;
-  float rwr = it.rw0 + float(ireg) * it.rws;
-  const float rws = it.rws;
+  int64_t q0 = bj0 + int64_t(irow) * it.vstep[0] + lob;
+  int64_t q1 = bj1 + int64_t(irow) * it.vstep[1] + lob;
+  int64_t q2 = bj2 + int64_t(irow) * it.vstep[2] + lob;
+  const int64_t v0 = 2 * it.vstep[0], v1 = 2 * it.vstep[1], v2 = 2 * it.vstep[2];
+  uint32_t toffr = uint32_t(irow * W + jc) * 16u;
+  const uint32_t tstep = uint32_t(W) * 32u;
+  float rwr = it.rw0 + float(irow) * it.rws;
+  const float rws = 2.f * it.rws;
 
   const float wEl = j >= 1 ? 2.f : 1.f;  // interior waves: the sample and, but in column 0, its mirror
+  const bool hvalid = j == 0 || j >= hw, hmirror = j > hw;  // row h/2
+  const float wEh = first && rl == 0 ? (hvalid ? (hmirror ? 2.f : 1.f) : 0.f) : wEl;  // step 0 of the wave
 
   // box tests, loads and fractional coordinates of a sample at (p0, p1, p2); valid /
   // mirror: whether the sample / its mirror take part; toff: its folded-target offset
@@ -534,9 +523,10 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
     sm.bq = bq;
   };
 
-  // stage 1 of a regular row (rows in order; position and target offset stepped):
-  // interior waves skip the box tests (every sample and its mirror are inside)
-  auto issue = [&](SmpH<DT>& sm, auto interior) {
+  // stage 1 of step t (steps in order; position and target offset stepped): interior
+  // waves skip the box tests (every sample and its mirror are inside); other waves test
+  // every sample, drop the rows past h - 1 and apply row h/2's masks
+  auto issue = [&](SmpH<DT>& sm, int t, auto interior) {
     if constexpr (decltype(interior)::value) {
       const uint32_t S = uint32_t(uint64_t(q0) >> 32);
       const uint32_t M = uint32_t(uint64_t(q1) >> 32);
@@ -547,7 +537,9 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
       sm.raw.load(b0, b1, Raw<DT>::offset(S, M, F, planeb, rowb), rowb, scales, S, M, F, Bm, Bf);
       sm.t = *reinterpret_cast<gf4ptr>(TT + toffr);
     } else {
-      load_tested(sm, q0 - lob, q1 - lob, q2 - lob, colok, j >= 1, toffr);
+      const bool h2 = first && rl == 0 && t == 0;
+      const bool valid = colok && (rl + 2 * t < nrow) && (!h2 || hvalid);
+      load_tested(sm, q0 - lob, q1 - lob, q2 - lob, valid, h2 ? hmirror : j >= 1, valid ? toffr : 0u);
     }
     q0 += v0;
     q1 += v1;
@@ -560,7 +552,7 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
   // moment weights (cw, rw) and box weights (wp, wq) of the sample and its mirror
   // (interior: every lane valid, wp = wq = 1; column 0's mirror part of the folded target is 0)
   auto accumulate = [&](const SmpH<DT>& sm, float phr, float phi, float cw, float rw, float wp, float wq,
-                        auto interior) {
+                        float wEi, auto interior) {
     const float ts = sm.ts, tm = sm.tm, tf = sm.tf;
     const Raw<DT>& q = sm.raw;
     float e00r, e00i, e01r, e01i, e10r, e10i, e11r, e11i, d00r, d00i, d01r, d01i, d10r, d10i, d11r, d11i;
@@ -582,7 +574,7 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
     if constexpr (decltype(interior)::value) {
       Tr = sc * (sm.t.x + sm.t.z);
       Ti = sc * (sm.t.y + sm.t.w);
-      wE = wEl * (sc * sc);
+      wE = wEi * (sc * sc);
     } else {
       Tr = sc * fmaf(wq, sm.t.z, wp * sm.t.x);
       Ti = sc * fmaf(wq, sm.t.w, wp * sm.t.y);
@@ -619,39 +611,43 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
     }
   };
 
-  // stage 2 of a regular row whose phasor sits in lane k (rows in order: rw is stepped)
-  auto consume = [&](const SmpH<DT>& sm, int k, auto interior) {
-    const float prr = readlane_f(rowph.r, k), pri = readlane_f(rowph.i, k);
+  // stage 2 of step t (steps in order: rw is stepped)
+  auto consume = [&](const SmpH<DT>& sm, int t, auto interior) {
+    const float prr = readlane_f(rowph.r, 2 * t), pri = readlane_f(rowph.i, 2 * t);
     const float phr = colph.r * prr - colph.i * pri;
     const float phi = fmaf(colph.r, pri, colph.i * prr);
     const float rw = rwr;
     rwr += rws;
-    accumulate(sm, phr, phi, cwj, rw, sm.bp ? 1.f : 0.f, sm.bq ? 1.f : 0.f, interior);
+    accumulate(sm, phr, phi, cwj, rw, sm.bp ? 1.f : 0.f, sm.bq ? 1.f : 0.f, t == 0 ? wEh : wEl, interior);
   };
 
-  // a special slot (row h/2 with its remapped lanes, or row 0), one sample with its tests
-  auto special = [&](int s) {
-    const SpecialSlot sp = special_slot(s, j, hh, hw, colok);
-    SmpH<DT> sm;
-    load_tested(sm, it.c[0] + int64_t(sp.jj) * it.ustep[0] + int64_t(sp.ii) * it.vstep[0],
-                it.c[1] + int64_t(sp.jj) * it.ustep[1] + int64_t(sp.ii) * it.vstep[1],
-                it.c[2] + int64_t(sp.jj) * it.ustep[2] + int64_t(sp.ii) * it.vstep[2], sp.valid, sp.mirror,
-                sp.valid ? uint32_t(sp.ii * W + sp.jj) * 16u : 0u);
-    const cpx ph = phasor(it.ph_col0 + uint32_t(sp.jj) * it.ph_col + it.ph_row0 + uint32_t(sp.ii) * it.ph_row);
-    accumulate(sm, ph.r, ph.i, it.cw0 + float(sp.jj) * it.cws, it.rw0 + float(sp.ii) * it.rws, sm.bp ? 1.f : 0.f,
-               sm.bq ? 1.f : 0.f, std::false_type{});
+  // The unpaired samples outside the rows h/2 .. h-1, one extra sample per lane of the first
+  // wave: lane row 0 takes column 0 of row j (1 <= j <= h/2 - 1, whose mirror column w is not
+  // on the grid), lane row 1 row 0 at column j (its mirror row h is not on the grid).  Its
+  // loads are issued before the row loop and it is consumed after it.
+  const int xi = rl ? 0 : j, xj = rl ? j : 0;
+  const bool xvalid = colok && (rl || (j >= 1 && j <= hh - 1));
+  SmpH<DT> xs;
+  auto extra_issue = [&]() {
+    load_tested(xs, it.c[0] + int64_t(xj) * it.ustep[0] + int64_t(xi) * it.vstep[0],
+                it.c[1] + int64_t(xj) * it.ustep[1] + int64_t(xi) * it.vstep[1],
+                it.c[2] + int64_t(xj) * it.ustep[2] + int64_t(xi) * it.vstep[2], xvalid, false,
+                xvalid ? uint32_t(xi * W + xj) * 16u : 0u);
+  };
+  auto extra_consume = [&]() {
+    const cpx ph = phasor(it.ph_col0 + uint32_t(xj) * it.ph_col + it.ph_row0 + uint32_t(xi) * it.ph_row);
+    accumulate(xs, ph.r, ph.i, it.cw0 + float(xj) * it.cws, it.rw0 + float(xi) * it.rws, xs.bp ? 1.f : 0.f,
+               0.f, 0.f, std::false_type{});
   };
 
-  // Class of the wave's regular rows.  A position is an exact affine integer function of
-  // (row, column), so along a lane's column its extremes are at the first and the last
-  // regular row: when every lane is a column of the slice and both lie in [M, h] on every
-  // axis for every lane, every sample and its mirror are inside the box (interior); when on
-  // some axis both lie below 0 or both above K for every lane, none is (outside: the rows
-  // contribute exactly 0 and are skipped).
-  const int kfirst = first ? 1 : 0;
-  const int n_r = nreg - kfirst;
+  // Class of a full wave (every step of every lane a regular row).  A position is an exact
+  // affine integer function of (row, column), so over a lane's rows its extremes are at its
+  // first and last row: when every lane is a column of the slice and both lie in [M, h] on
+  // every axis for every lane, every sample and its mirror are inside the box (interior);
+  // when on some axis both lie below 0 or both above K for every lane, none is (outside: the
+  // rows contribute exactly 0 and are skipped).  Other waves test every sample.
   int cls = 0;  // 0: per-sample tests, 1: interior, 2: outside
-  if (n_r > 0 && !(opts & kHermExactBox)) {
+  if (full && !(opts & kHermExactBox)) {
     const int64_t f0 = q0 - lob, f1 = q1 - lob, f2 = q2 - lob;
     const int64_t l0 = f0 + int64_t(n_r - 1) * v0, l1 = f1 + int64_t(n_r - 1) * v1, l2 = f2 + int64_t(n_r - 1) * v2;
     const bool in = colok & (uint64_t(f0) >= M0) & (uint64_t(f0) <= h0) & (uint64_t(l0) >= M0) &
@@ -670,26 +666,26 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
     SmpH<DT> ring[DEPTH];
 #pragma unroll
     for (int d = 0; d < DEPTH - 1; ++d)
-      if (d < n_r) issue(ring[d], interior);
+      if (d < n_r) issue(ring[d], d, interior);
 #pragma unroll 1
     for (int t0 = 0; t0 < n_r; t0 += DEPTH) {
 #pragma unroll
       for (int d = 0; d < DEPTH; ++d) {
         const int t = t0 + d;
         if (t < n_r) {
-          if (t + DEPTH - 1 < n_r) issue(ring[(d + DEPTH - 1) % DEPTH], interior);
-          consume(ring[d], t + kfirst, interior);
+          if (t + DEPTH - 1 < n_r) issue(ring[(d + DEPTH - 1) % DEPTH], t + DEPTH - 1, interior);
+          consume(ring[d], t, interior);
           __builtin_amdgcn_sched_barrier(0);
         }
       }
     }
   };
 
-  // per-lane order of the sums: row h/2 (first wave), the regular rows, row 0 (first wave)
-  if (first) special(0);
+  // per-lane order of the sums: the steps, then the extra sample (first wave)
+  if (first) extra_issue();
   if (cls == 1) run(std::true_type{});
   else if (cls == 0) run(std::false_type{});
-  if (first) special(nreg);
+  if (first) extra_consume();
   tile_reduce(ac, 1.f, tile, partials);
 }
 

@@ -92,9 +92,11 @@ def slice_dims_n(pad_shape, view) -> Tuple[int, int]:
 
 
 def tiles_for(h: int, w: int, tile_rows: int, herm: bool = False) -> Tuple[int, int]:
-    """(tiles per row, tile rows) of an item; a Hermitian-fold item covers h/2 rows."""
-    rows = h // 2 if herm else h
-    return (w + 63) // 64, (rows + tile_rows - 1) // tile_rows
+    """(tiles per row, tile rows) of an item: tiles of 64 columns x tile_rows rows; a
+    Hermitian-fold item covers h/2 rows in tiles of 32 columns x 2 tile_rows rows."""
+    if herm:
+        return (w + 31) // 32, (h // 2 + 2 * tile_rows - 1) // (2 * tile_rows)
+    return (w + 63) // 64, (h + tile_rows - 1) // tile_rows
 
 
 def physical_index(x, desc):
@@ -144,7 +146,7 @@ def n_item(packed, view: str, yaw, pitch, roll, a, b, target_ptr: int, flags: in
     it["lo_int"] = 1
     it["guard_off"] = guard_offset(desc)
     it["tile_base"] = tile_base
-    it["tiles_x"] = tiles_for(h, w, tile_rows)[0]
+    it["tiles_x"] = tiles_for(h, w, tile_rows, herm)[0]
     it["tile_h"] = tile_rows
     it["flags"] = flags
     it["target"] = target_ptr
@@ -275,7 +277,8 @@ def validate_items(items: np.ndarray, n_layouts: int, descs: np.ndarray, tile_ro
         raise ValueError("slice output requested without a buffer")
     herm = (items["flags"] & _lib.ITEM_HERM) != 0
     rows = np.where(herm, items["h"] // 2, items["h"])
-    ntile = ((items["w"] + 63) // 64) * ((rows + tile_rows - 1) // tile_rows)
+    tw, th = np.where(herm, 32, 64), np.where(herm, 2 * tile_rows, tile_rows)
+    ntile = ((items["w"] + tw - 1) // tw) * ((rows + th - 1) // th)
     if np.any(herm):
         hi_ = items[herm]
         if np.any((hi_["flags"] & (_lib.ITEM_HP | _lib.ITEM_WRITE_SLICE)) != 0) or np.any(hi_["lo_int"] != 1):
@@ -288,7 +291,7 @@ def validate_items(items: np.ndarray, n_layouts: int, descs: np.ndarray, tile_ro
         dh = descs[hi_["layout"]]
         if np.any((hi_["box_hi"] >> np.uint64(32)).astype(np.int64) + 3 > dh["sdim"].astype(np.int64) - 1):
             raise ValueError("Hermitian-fold item box exceeds the wrapped layout's storage")
-    if np.any(items["tiles_x"] != (items["w"] + 63) // 64):
+    if np.any(items["tiles_x"] != (items["w"] + tw - 1) // tw):
         raise ValueError("item tiles_x inconsistent with its width")
     base = int(tile0) + np.concatenate([[0], np.cumsum(ntile)[:-1]])
     if not np.array_equal(items["tile_base"].astype(np.int64), base):

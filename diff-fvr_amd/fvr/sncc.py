@@ -36,7 +36,8 @@ def tile_rows() -> int:
 class LaunchPlan:
     """How a batch is laid out on the fused kernel.
 
-    tile_h: rows per tile (32, 64, 128; 4 waves x 8 / 16 / 32 rows, 64 columns).
+    tile_h: rows per tile (32, 64, 128; 4 waves x 8 / 16 / 32 rows, 64 columns); a
+    Hermitian-fold tile is 32 columns x 2 tile_h rows (each wave step a 2 x 32 patch).
     order: "xcd" -- each XCD takes a contiguous range of tiles in item order
     (pose-major); "target" -- tiles sorted by (view, tile row, tile column, pose)
     before the contiguous XCD split, so that an XCD's L2 holds one band of one
@@ -110,19 +111,25 @@ class _Workspace:
 
 
 def item_tiles(items: np.ndarray, rows: int) -> np.ndarray:
-    tx = (items["w"] + 63) // 64
-    r = np.where((items["flags"] & _lib.ITEM_HERM) != 0, items["h"] // 2, items["h"])
-    ty = (r + rows - 1) // rows
-    return (tx * ty).astype(np.int64)
+    """Tile count per item (``_items.tiles_for``: 64 x rows tiles; 32 x 2 rows for the fold)."""
+    return np.array([int(np.prod(_items.tiles_for(int(it["h"]), int(it["w"]), rows,
+                                                  bool(it["flags"] & _lib.ITEM_HERM)))) for it in items],
+                    dtype=np.int64)
 
 
 def fold_target(t: torch.Tensor) -> torch.Tensor:
     """(h, w, 4) float32 (Re T, Im T, Re, Im of conj T(h-i, w-j)) for the Hermitian-fold
-    kernel; the mirror part is 0 in row 0 and column 0 (they have no mirror in the grid)."""
+    kernel; the mirror part is 0 in row 0 and column 0 (they have no mirror in the grid).
+    Row h/2 carries the fold's masks: columns 1 .. w/2 - 1 are 0 (they are the mirrors of
+    the right half of the same row) and column w/2 has no mirror part (it is its own)."""
     h, w = t.shape
     tc = t.to(torch.complex64)
     m = torch.zeros_like(tc)
     m[1:, 1:] = torch.conj(torch.flip(tc[1:, 1:], dims=(0, 1)))
+    tc = tc.clone()
+    hh, hw = h // 2, w // 2
+    tc[hh, 1:hw] = 0
+    m[hh, 1:hw + 1] = 0
     return torch.stack([tc.real, tc.imag, m.real, m.imag], dim=-1).contiguous()
 
 

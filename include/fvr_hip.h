@@ -115,8 +115,10 @@ int fvr_spec_pack(const fvr_layout* lay, uint64_t src, int32_t nz, int32_t ny, i
 /* Hermitian fold (convention N, wrapped layout, even h <= w): the item covers the
  * rows h/2 .. h-1, each sample also standing for its mirror (h - i, w - j), plus
  * the unpaired row 0 and column 0; `target` then points at float4 (T(i,j),
- * conj T(h-i, w-j)) per sample (zero mirror part in row 0 / column 0).  Sums are
- * those of the full slice; the tile count covers h/2 rows. */
+ * conj T(h-i, w-j)) per sample, with a zero mirror part in row 0 and column 0 and,
+ * in row h/2, zeros in columns 1 .. w/2 - 1 and a zero mirror part in column w/2.
+ * Sums are those of the full slice.  Tiles are 32 columns x 2 tile_h rows over the
+ * h/2 rows (tiles_x = ceil(w / 32)). */
 #define FVR_ITEM_HERM 8u
 
 /* One (pose, view) evaluation.  Sample (row i, col j) sits at
@@ -141,7 +143,7 @@ typedef struct fvr_sncc_item {
   int32_t lo_int;       /* storage offset of floor(p) (1 = scipy box, 0 = per-corner) */
   uint32_t guard_off;   /* byte offset (from layout base) of a zero region */
   int32_t tile_base;    /* global index of this item's first tile */
-  int32_t tiles_x;      /* tiles per slice row */
+  int32_t tiles_x;      /* tiles per slice row (ceil(w / 64); ceil(w / 32) with FVR_ITEM_HERM) */
   uint32_t flags;
   int32_t tile_h;       /* rows per tile: 32, 64 or 128 (4 waves x 8, 16 or 32 rows); one value per launch */
   uint64_t target;      /* device address of the complex64 target (h x w, row-major) */

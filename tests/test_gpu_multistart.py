@@ -76,8 +76,11 @@ def test_launch_plans_agree(vols, herm):
 
 def _fold_wave_classes(items: np.ndarray, tile_h: int) -> np.ndarray:
     """Counts of (per-sample tests, interior, outside) waves of fold items: the classes
-    the fold kernel gives the regular rows of each wave, from the items' 32.32 geometry
-    (used here to show that the poses of a test reach every class)."""
+    the fold kernel gives each wave, from the items' 32.32 geometry (used here to show that
+    the poses of a test reach every class).  A wave step is a 2 x 32 patch: lane (rl, c)
+    of the wave whose first row is h/2 + r0 walks rows h/2 + r0 + rl + 2t, t < R; only
+    waves whose every step is a regular row (not the first wave, no row past h - 1) are
+    classified."""
     R = tile_h // 4
     counts = np.zeros(3, dtype=np.int64)
     for it in items:
@@ -89,22 +92,21 @@ def _fold_wave_classes(items: np.ndarray, tile_h: int) -> np.ndarray:
         hi = np.array(it["box_hi"], dtype=np.int64)
         K = hi + (1 << 32) + (hi & 0xFFFFFFFF)
         M = K - hi
-        j = np.arange(int(it["tiles_x"]) * 64)
-        colok = j < w
-        jc = np.minimum(j, w - 1)
-        for r0 in range(0, hh, R):
-            nreg = min(R, hh - r0)
-            first = int(r0 == 0)
-            n_r = nreg - first
-            if n_r <= 0:
-                continue
+        for r0 in range(0, hh, 2 * R):
+            full = r0 > 0 and hh - r0 >= 2 * R
             for tx in range(int(it["tiles_x"])):
-                cols = slice(tx * 64, tx * 64 + 64)
-                q = c[None, :] + jc[cols, None] * u[None, :] + (hh + r0 + first) * v[None, :]
-                last = q + (n_r - 1) * v[None, :]
-                inside = np.all(colok[cols]) and np.all((q >= M) & (q <= hi) & (last >= M) & (last <= hi))
+                if not full:
+                    counts[0] += 1
+                    continue
+                j = tx * 32 + np.arange(64) % 32
+                rl = np.arange(64) // 32
+                colok = j < w
+                jc = np.minimum(j, w - 1)
+                q = c[None, :] + jc[:, None] * u[None, :] + (hh + r0 + rl)[:, None] * v[None, :]
+                last = q + 2 * (R - 1) * v[None, :]
+                inside = np.all(colok) and np.all((q >= M) & (q <= hi) & (last >= M) & (last <= hi))
                 away = ((q < 0) & (last < 0)) | ((q > K) & (last > K))
-                outside = np.all(~colok[cols] | np.any(away, axis=1))
+                outside = np.all(~colok | np.any(away, axis=1))
                 counts[1 if inside else (2 if outside else 0)] += 1
     return counts
 
@@ -113,11 +115,11 @@ def _fold_wave_classes(items: np.ndarray, tile_h: int) -> np.ndarray:
 def test_fold_wave_classes_equal_per_sample_tests(storage):
     """The fold kernel's interior / outside wave classes give the same sums, bit for bit,
     as per-sample box tests on every wave, for every tile height and depth; the poses
-    (rotations up to 40 degrees on a 144 x 144 slice: its last column tile holds 16 columns)
-    reach all three classes and lanes past the last column."""
+    (rotations up to 40 degrees on a 176 x 176 slice: its last 32-column tile holds 16
+    columns) reach all three classes and lanes past the last column."""
     from fvr.multistart import candidate_plans
     from fvr.sncc import BiplanarEvaluator, LaunchPlan
-    r = _renderer(72, 21, storage)
+    r = _renderer(88, 21, storage)
     ta, tl = _targets(r, np.random.default_rng(21).uniform(-4, 4, 6))
     ev = BiplanarEvaluator(r.packed, ta, tl, adjoint=True)
     from fvr.phantom import random_poses
@@ -135,7 +137,7 @@ def test_fold_wave_classes_equal_per_sample_tests(storage):
         assert np.all(np.isfinite(L1)) and np.all(np.isfinite(G1))
         seen += _fold_wave_classes(ev.device_items(len(P)), plan.tile_h)
     assert np.all(seen > 0), seen
-    assert r.packed.shape[2] % 64 != 0 or r.packed.shape[1] % 64 != 0
+    assert r.packed.shape[2] % 32 != 0 or r.packed.shape[1] % 32 != 0
 
 
 @pytest.mark.parametrize("order", ["xcd", "target"])
