@@ -623,8 +623,8 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
 
   // The unpaired samples outside the rows h/2 .. h-1, one extra sample per lane of the first
   // wave: lane row 0 takes column 0 of row j (1 <= j <= h/2 - 1, whose mirror column w is not
-  // on the grid), lane row 1 row 0 at column j (its mirror row h is not on the grid).  Its
-  // loads are issued before the row loop and it is consumed after it.
+  // on the grid), lane row 1 row 0 at column j (its mirror row h is not on the grid).  It
+  // runs after the row loop (kept out of the loop, its registers would cost a wave per SIMD).
   const int xi = rl ? 0 : j, xj = rl ? j : 0;
   const bool xvalid = colok && (rl || (j >= 1 && j <= hh - 1));
   SmpH<DT> xs;
@@ -682,10 +682,12 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
   };
 
   // per-lane order of the sums: the steps, then the extra sample (first wave)
-  if (first) extra_issue();
   if (cls == 1) run(std::true_type{});
   else if (cls == 0) run(std::false_type{});
-  if (first) extra_consume();
+  if (first) {
+    extra_issue();
+    extra_consume();
+  }
   tile_reduce(ac, 1.f, tile, partials);
 }
 
