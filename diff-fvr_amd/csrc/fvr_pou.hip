@@ -177,6 +177,85 @@ __device__ void fft2d_dif(float2* a, int P, int logP, int pitch, const float2* t
   }
 }
 
+// The same transform with pairs of radix-2 stages fused into radix-4 passes: stages s and
+// s - 1 of a block of 2^s elements act on the four elements k, k + q, k + 2q, k + 3q
+// (q = 2^(s-2)) -- stage s pairs (k, k + 2q) and (k + q, k + 3q) with twiddles k ts and
+// (k + q) ts (ts = P >> s), stage s - 1 pairs the results within each half with twiddle
+// 2k ts -- so each element is read and written once per two stages (an odd logP ends with one
+// radix-2 pass).  Same arithmetic as fft2d_dif, half the LDS passes and barriers.
+template <int SIGN>
+__device__ void fft2d_dif4(float2* a, int P, int logP, int pitch, const float2* tw) {
+  const int hp = P / 2;
+  auto twid = [&](int e) {
+    float2 w = tw[e];
+    if (SIGN > 0) w.y = -w.y;
+    return w;
+  };
+  auto bfly = [&](float2& x0, float2& x1, float2 w) {  // x0 <- x0 + x1, x1 <- (x0 - x1) w
+    const float dr = x0.x - x1.x, di = x0.y - x1.y;
+    x0 = make_float2(x0.x + x1.x, x0.y + x1.y);
+    x1 = make_float2(dr * w.x - di * w.y, dr * w.y + di * w.x);
+  };
+#pragma unroll 1
+  for (int dim = 0; dim < 2; ++dim) {
+    int s = logP;
+#pragma unroll 1
+    for (; s >= 2; s -= 2) {
+      const int q = 1 << (s - 2), ts = P >> s;
+      const int ng = P * P / 4;  // groups of four per pass
+      for (int idx = threadIdx.x; idx < ng; idx += blockDim.x) {
+        int line, g;
+        if (dim == 0) {  // along rows: consecutive threads walk one row
+          line = idx / (P / 4);
+          g = idx - line * (P / 4);
+        } else {  // along columns: consecutive threads take consecutive columns
+          line = idx & (P - 1);
+          g = idx >> logP;
+        }
+        const int k = g & (q - 1);
+        const int i0 = ((g >> (s - 2)) << s) + k;
+        int e[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = i0 + r * q;
+          e[r] = dim == 0 ? line * pitch + i : i * pitch + line;
+        }
+        float2 x0 = a[e[0]], x1 = a[e[1]], x2 = a[e[2]], x3 = a[e[3]];
+        bfly(x0, x2, twid(k * ts));
+        bfly(x1, x3, twid((k + q) * ts));
+        const float2 w2 = twid(2 * k * ts);
+        bfly(x0, x1, w2);
+        bfly(x2, x3, w2);
+        a[e[0]] = x0;
+        a[e[1]] = x1;
+        a[e[2]] = x2;
+        a[e[3]] = x3;
+      }
+      __syncthreads();
+    }
+    if (s == 1) {  // the last radix-2 stage (half = 1, twiddle 1)
+      for (int idx = threadIdx.x; idx < P * hp; idx += blockDim.x) {
+        int line, k;
+        if (dim == 0) {
+          line = idx >> (logP - 1);
+          k = idx & (hp - 1);
+        } else {
+          line = idx & (P - 1);
+          k = idx >> logP;
+        }
+        const int i0 = 2 * k;
+        const int a0 = dim == 0 ? line * pitch + i0 : i0 * pitch + line;
+        const int a1 = dim == 0 ? a0 + 1 : a0 + pitch;
+        float2 x0 = a[a0], x1 = a[a1];
+        bfly(x0, x1, twid(0));
+        a[a0] = x0;
+        a[a1] = x1;
+      }
+      __syncthreads();
+    }
+  }
+}
+
 __device__ __forceinline__ void make_twiddles(float2* tw, int P) {
   for (int k = threadIdx.x; k < P / 2; k += blockDim.x) {
     float s, c;
@@ -263,7 +342,7 @@ __global__ __launch_bounds__(kPouSliceThreads) void pou_slice_ifft_kernel(const 
     }
   }
   __syncthreads();
-  fft2d_dif<+1>(a, P, logP, pitch, tw);
+  fft2d_dif4<+1>(a, P, logP, pitch, tw);
   const float inv = 1.f / float(P * P);
   float* out = imgs + size_t(bn) * P * P;
   for (int q = threadIdx.x; q < P * P; q += blockDim.x) {
@@ -470,7 +549,7 @@ __global__ __launch_bounds__(kPouSliceThreads) void pou_render_bwd_kernel(
   }
   __syncthreads();
   // 3) adjoint of fftshift(ifft2(ifftshift(.))).real: fftshift(fft2(ifftshift(g))) / P^2
-  fft2d_dif<-1>(a, P, logP, pitch, tw);
+  fft2d_dif4<-1>(a, P, logP, pitch, tw);
   // 4) recompute every slice sample and contract its spatial gradient with the slice gradient
   const float ux = u[3 * n], uy = u[3 * n + 1], uz = u[3 * n + 2];
   const float vx = v[3 * n], vy = v[3 * n + 1], vz = v[3 * n + 2];
