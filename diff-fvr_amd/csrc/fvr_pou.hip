@@ -293,7 +293,21 @@ struct PouSlice {
 // so the poses of one block run on one XCD and share its L2 copy of the block's slices
 __device__ __forceinline__ int pou_bn(int w, int per_xcd) { return (w & 7) * per_xcd + (w >> 3); }
 
-// issue kPouBatch samples (rows i = q >> logP, columns j) of one slice: columns follow u,
+// Slice sample (row i, column j) of linear sample index q: every 64 consecutive q (one
+// wave's lanes) form a 2-row x 32-column patch (P >= 32; row-major below), which crosses
+// about half as many storage rows of a tilted slice per load as a 1 x 64 row does.
+__device__ __forceinline__ void pou_ij(int q, int logP, int& i, int& j) {
+  if (logP >= 5) {
+    const int w = q >> 6, lane = q & 63, sh = logP - 5;  // 2^sh patches per 2-row band
+    i = 2 * (w >> sh) + (lane >> 5);
+    j = 32 * (w & ((1 << sh) - 1)) + (lane & 31);
+  } else {
+    i = q >> logP;
+    j = q & ((1 << logP) - 1);
+  }
+}
+
+// issue kPouBatch samples (pou_ij(q): row i, column j) of one slice: columns follow u,
 // rows follow v, sample at c + (j - P/2) u + (i - P/2) v
 template <int VT>
 __device__ __forceinline__ void pk_issue_batch(PkSmp<VT> (&s)[kPouBatch], unsigned long long blk, const PouPk& k,
@@ -303,7 +317,8 @@ __device__ __forceinline__ void pk_issue_batch(PkSmp<VT> (&s)[kPouBatch], unsign
 #pragma unroll
   for (int t = 0; t < kPouBatch; ++t) {
     const int q = min(q0 + t * stride, nq - 1);
-    const int i = q >> g.logP, j = q & (P - 1);
+    int i, j;
+    pou_ij(q, g.logP, i, j);
     const float kx = float(j - hp), ky = float(i - hp);
     pk_issue<VT>(s[t], blk, k, P, fmaf(ky, vx, fmaf(kx, ux, g.c)), fmaf(ky, vy, fmaf(kx, uy, g.c)),
                  fmaf(ky, vz, fmaf(kx, uz, g.c)));
@@ -336,7 +351,8 @@ __global__ __launch_bounds__(kPouSliceThreads) void pou_slice_ifft_kernel(const 
       float cr[8], ci[8], re, im;
       pk_corners<VT>(smp[t], cr, ci);
       interp8<false>(cr, ci, smp[t].tx, smp[t].ty, smp[t].tz, re, im, nullptr, nullptr);
-      const int i = q >> logP, j = q & (P - 1);
+      int i, j;
+      pou_ij(q, logP, i, j);
       // ifftshift: frequency row i goes to natural index (i + P/2) mod P
       if (q < nq) a[((i + hp) & (P - 1)) * pitch + ((j + hp) & (P - 1))] = make_float2(re, im);
     }
@@ -564,7 +580,8 @@ __global__ __launch_bounds__(kPouSliceThreads) void pou_render_bwd_kernel(
     for (int t = 0; t < kPouBatch; ++t) {
       const int q = q0 + t * stride;
       if (q >= nq) continue;
-      const int i = q >> logP, j = q & (P - 1);
+      int i, j;
+      pou_ij(q, logP, i, j);
       const float kx = float(j - hp), ky = float(i - hp);
       const float2 gs = a[brev((i + hp) & (P - 1), logP) * pitch + brev((j + hp) & (P - 1), logP)];
       const float gr = gs.x * inv, gi = gs.y * inv;
