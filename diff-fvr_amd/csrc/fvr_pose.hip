@@ -120,25 +120,41 @@ __global__ void pn_build_kernel(const double* __restrict__ poses, int n_poses, f
   build_item(poses + 6 * (idx >> 1), idx >> 1, idx & 1, tap, tlat, pz, py, px, lat_swap, tile0, items, jac);
 }
 
+// What the reduction of one pose reads from its two items and Jacobians, staged in LDS
+// by the workgroup (so that the next items can be rebuilt while the sums are formed).
+struct PoseStage {
+  double J[2][18];
+  int tile_base[2], ntile[2], w[2], h[2];
+};
+__device__ __forceinline__ void stage_pose(const fvr_sncc_item* items, const double* jac, int pose, PoseStage& st) {
+  const int tid = threadIdx.x;
+  if (tid < 36) st.J[tid / 18][tid % 18] = jac[size_t(2 * pose) * 18 + tid];
+  if (tid < 2) {
+    const fvr_sncc_item& it = items[2 * pose + tid];
+    st.tile_base[tid] = it.tile_base;
+    st.ntile[tid] = it.tiles_x * sncc_tiles_y(it.h, it.flags, it.tile_h);
+    st.w[tid] = it.w;
+    st.h[tid] = it.h;
+  }
+}
+
 // One wave per pose: fixed-order float64 reduction of each view's tile
 // partials (lane-strided partial sums, then an XOR butterfly that every lane
 // evaluates in the same order), then the biplanar loss and exact gradient.
-__device__ void reduce_combine_pose(const fvr_sncc_item* items, int pose, const float* __restrict__ partials,
-                                    const double* jac, double t2_ap, double t2_lat, double* moments, double& L_out,
-                                    double (&g_out)[6]) {
-  const int lane = threadIdx.x;
+__device__ void reduce_combine_pose(const PoseStage& st, int pose, const float* __restrict__ partials, double t2_ap,
+                                    double t2_lat, double* moments, double& L_out, double (&g_out)[6]) {
+  const int lane = threadIdx.x & 63;
   const double two_pi = 6.283185307179586476925286766559;
   double g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   double L = 0.0;
 #pragma unroll
   for (int view = 0; view < 2; ++view) {
-    const fvr_sncc_item& it = items[2 * pose + view];
-    const int ntile = it.tiles_x * sncc_tiles_y(it.h, it.flags, it.tile_h);
+    const int ntile = st.ntile[view], tile_base = st.tile_base[view];
     double m[FVR_SNCC_NMOM];
 #pragma unroll
     for (int q = 0; q < FVR_SNCC_NMOM; ++q) m[q] = 0.0;
     for (int t = lane; t < ntile; t += 64) {
-      const float* pp = partials + size_t(it.tile_base + t) * FVR_SNCC_NMOM;
+      const float* pp = partials + size_t(tile_base + t) * FVR_SNCC_NMOM;
 #pragma unroll
       for (int q = 0; q < FVR_SNCC_NMOM; ++q) m[q] += double(pp[q]);
     }
@@ -152,7 +168,7 @@ __device__ void reduce_combine_pose(const fvr_sncc_item* items, int pose, const 
       for (int q = 0; q < FVR_SNCC_NMOM; ++q) mv = q == lane ? m[q] : mv;
       moments[size_t(2 * pose + view) * FVR_SNCC_NMOM + lane] = mv;
     }
-    const double* J = jac + size_t(2 * pose + view) * 18;
+    const double* J = st.J[view];
     const double nt = sqrt(view ? t2_lat : t2_ap), ns = sqrt(fmax(m[1], 0.0));
     double ncc = 0.0, d0 = 0.0, d1 = 0.0, d2 = 0.0, da = 0.0, db = 0.0;
     if (ns * nt != 0.0) {
@@ -171,8 +187,8 @@ __device__ void reduce_combine_pose(const fvr_sncc_item* items, int pose, const 
       d0 = G0 * inv - ncc * H0 / m[1];
       d1 = G1 * inv - ncc * H1 / m[1];
       d2 = G2 * inv - ncc * H2 / m[1];
-      da = two_pi / double(it.w) * m[2] * inv;  // d/da: 2 pi / W sum uu Im(S T*)
-      db = two_pi / double(it.h) * m[3] * inv;  // d/db: 2 pi / H sum vv Im(S T*)
+      da = two_pi / double(st.w[view]) * m[2] * inv;  // d/da: 2 pi / W sum uu Im(S T*)
+      db = two_pi / double(st.h[view]) * m[3] * inv;  // d/db: 2 pi / H sum vv Im(S T*)
     }
     L += 0.5 * (1.0 - ncc);
     g[0] -= 0.5 * d0;
@@ -188,7 +204,7 @@ __device__ void reduce_combine_pose(const fvr_sncc_item* items, int pose, const 
 }
 
 __device__ __forceinline__ void store_loss_grad(int pose, double L, const double (&g)[6], double* loss, double* grad) {
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   if (lane == 0) loss[pose] = L;
   if (lane < 6) {
     double gv = 0.0;
@@ -204,8 +220,11 @@ __global__ __launch_bounds__(64) void pn_reduce_combine_kernel(const fvr_sncc_it
                                                                double t2_lat, double* __restrict__ moments,
                                                                double* __restrict__ loss, double* __restrict__ grad) {
   const int pose = blockIdx.x;
+  __shared__ PoseStage st;
+  stage_pose(items, jac, pose, st);
+  __syncthreads();
   double L, g[6];
-  reduce_combine_pose(items, pose, partials, jac, t2_ap, t2_lat, moments, L, g);
+  reduce_combine_pose(st, pose, partials, t2_ap, t2_lat, moments, L, g);
   store_loss_grad(pose, L, g, loss, grad);
 }
 
@@ -413,32 +432,41 @@ __global__ __launch_bounds__(64) void pt_combine_kernel(const fvr_sncc_item* __r
   }
 }
 
-// One wave per pose after the fused kernel: reduce + combine (loss, gradient at the
-// pose the items were built from), then optionally one Adam step of the pose, then the
-// pose's two items and Jacobians for the next fused launch.  A registration step is
-// then two launches: the fused kernel and this one.
-__global__ __launch_bounds__(64) void pn_step_kernel(fvr_sncc_item* items, int n_poses, const float* __restrict__ partials,
+// One workgroup of two waves per pose after the fused kernel: reduce + combine (loss,
+// gradient at the pose the items were built from), then optionally one Adam step of the
+// pose, then the pose's two items and Jacobians for the next fused launch.  The item
+// fields and Jacobians the reduction reads are staged in LDS first; without Adam the
+// second wave rebuilds the items while the first forms the sums (with Adam the rebuild
+// needs the stepped pose and follows on the first wave).  A registration step is then two
+// launches: the fused kernel and this one.
+__global__ __launch_bounds__(128) void pn_step_kernel(fvr_sncc_item* items, int n_poses, const float* __restrict__ partials,
                                                      double* jac, double t2_ap, double t2_lat,
                                                      double* __restrict__ moments, double* __restrict__ loss,
                                                      double* __restrict__ grad, double* __restrict__ poses,
                                                      int use_adam, fvr_adam_cfg c, AdamState S, fvr_sncc_item tap,
                                                      fvr_sncc_item tlat, double pz, double py, double px,
                                                      int lat_swap, int tile0) {
-  const int pose = blockIdx.x;
-  double L, g[6];
-  reduce_combine_pose(items, pose, partials, jac, t2_ap, t2_lat, moments, L, g);  // every lane: same L, g
-  store_loss_grad(pose, L, g, loss, grad);
+  const int pose = blockIdx.x, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __shared__ PoseStage st;
+  stage_pose(items, jac, pose, st);
   double P[6];
 #pragma unroll
   for (int a = 0; a < 6; ++a) P[a] = poses[6 * pose + a];
-  __syncthreads();  // every lane has read this pose's items and Jacobians before they are rebuilt
-  if (use_adam) {
-    adam_pose(pose, P, L, g, S, c, threadIdx.x == 0);  // lanes agree; lane 0 stores the state
-    if (threadIdx.x == 0)
+  __syncthreads();  // this pose's items and Jacobians are staged before they are rebuilt
+  if (wave == 0) {
+    double L, g[6];
+    reduce_combine_pose(st, pose, partials, t2_ap, t2_lat, moments, L, g);  // every lane: same L, g
+    store_loss_grad(pose, L, g, loss, grad);
+    if (use_adam) {
+      adam_pose(pose, P, L, g, S, c, lane == 0);  // lanes agree; lane 0 stores the state
+      if (lane == 0)
 #pragma unroll
-      for (int a = 0; a < 6; ++a) poses[6 * pose + a] = P[a];
+        for (int a = 0; a < 6; ++a) poses[6 * pose + a] = P[a];
+      if (lane < 2) build_item(P, pose, lane == 1, tap, tlat, pz, py, px, lat_swap, tile0, items, jac);
+    }
+  } else if (!use_adam && lane < 2) {
+    build_item(P, pose, lane == 1, tap, tlat, pz, py, px, lat_swap, tile0, items, jac);
   }
-  if (threadIdx.x < 2) build_item(P, pose, threadIdx.x == 1, tap, tlat, pz, py, px, lat_swap, tile0, items, jac);
 }
 
 }  // namespace fvr
@@ -505,7 +533,7 @@ int fvr_pn_step(fvr_sncc_item* items_dev, int32_t n_poses, const float* partials
                   "fvr_pn_step: bad Adam hyper-parameters");
   }
   const AdamState S{m, v, t, done, best_loss, best_pose};
-  hipLaunchKernelGGL(pn_step_kernel, dim3(n_poses), dim3(64), 0, reinterpret_cast<hipStream_t>(stream), items_dev,
+  hipLaunchKernelGGL(pn_step_kernel, dim3(n_poses), dim3(128), 0, reinterpret_cast<hipStream_t>(stream), items_dev,
                      n_poses, partials, jac_dev, t2_ap, t2_lat, moments, loss, grad, poses, adam_cfg_host ? 1 : 0, c,
                      S, tmpl_host[0], tmpl_host[1], pz, py, px, int(lat_swap), int(tile0));
   FVR_CHECK_LAUNCH("pn_step_kernel");
