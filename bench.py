@@ -247,7 +247,7 @@ def pou_config4(steps: int, warmup: int, size: int = 256, n_pose: int = 64):
 
 def multistart_config5(n_steps: int = 100, lat_reps: int = 100):
     """Ragged batched multistart: seeded ellipsoid phantoms of 128^3, 192^3 and 256^3
-    (spectra 256^3 / 384^3 / 512^3, FP16 with per-brick scales), AP + LAT targets of
+    (spectra 256^3 / 384^3 / 512^3, FP16 with one scale each), AP + LAT targets of
     128^2 / 192^2 / 256^2 DRRs at a seeded pose each, 512 starting poses split 96 / 160 /
     256 (the truth plus rotations U(-8, 8) deg, translations U(-8, 8) voxels).  All
     volumes go into ONE fused launch per step (autotuned launch plan); the 100-step Adam
@@ -434,7 +434,7 @@ def main(argv=None) -> int:
     head = results[args.storage]
     elem = 8 if args.storage == "f32" else 4
     if args.no_footprint:
-        fp = {"bytes": None, "unique_entries": None, "bricks": None,
+        fp = {"bytes": None, "unique_entries": None,
               "schedule": "hermitian-fold" if head["herm"] else "full-slice"}
         achieved = None
     else:
@@ -511,8 +511,7 @@ def main(argv=None) -> int:
                      "traffic": None if traffic is None else traffic["bytes"], "traffic_detail": traffic,
                      "kernel": kname, "kernel_ms": round(head["kernel_ms"], 4),
                      "algorithmic_bytes_per_launch": fp["bytes"], "unique_entries": fp["unique_entries"],
-                     "schedule": fp.get("schedule"), "samples_per_launch": fp.get("samples"),
-                     "bricks": fp["bricks"]},
+                     "schedule": fp.get("schedule"), "samples_per_launch": fp.get("samples")},
         "cpu_baseline": cpu,
         "autotune_ms": {f"{p.tile_h}/{p.order}/{p.persist}/{p.depth}": round(t, 4)
                         for p, t in sorted(head["autotune"].items(), key=lambda kv: kv[1])},

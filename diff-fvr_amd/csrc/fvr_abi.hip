@@ -1,5 +1,6 @@
 // ABI plumbing: version, thread-local error text, layout descriptors and the
 // spectrum pack kernels (centred complex spectrum -> padded device layout).
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -18,11 +19,8 @@ void set_error(const char* fmt, ...) {
 }
 
 // ---------------------------------------------------------------------------
-// Pack: one thread per physical storage element.  Logical index = storage
-// index - 1; in the apron layout physical index g of an axis holds storage
-// index 8 (g / 9) + g % 9 (g % 9 == 8: the next brick's first element, stored
-// under this brick's scale).  transpose_xy swaps the mid and fast layout axes
-// relative to (z, y, x).
+// Pack: one thread per storage element; logical index = storage index - 1.
+// transpose_xy swaps the mid and fast layout axes relative to (z, y, x).
 // ---------------------------------------------------------------------------
 struct PackArgs {
   fvr_layout lay;
@@ -43,55 +41,45 @@ __device__ __forceinline__ float2 pack_src(const PackArgs& a, int s, int m, int 
   return a.src[(size_t(z) * a.ny + y) * a.nx + x];
 }
 
-// per-brick power-of-two scale: max|component| / scale in [2^14, 2^15), the max
-// taken over the brick's 9^3 stored elements (its 8^3 own and the apron copies)
-__global__ __launch_bounds__(768) void pack_scales_kernel(PackArgs a, float* scales) {
-  const int bf = blockIdx.x, bm = blockIdx.y, bs = blockIdx.z;
-  const int t = threadIdx.x;
+// The spectrum's power-of-two scale sc (scaled dtypes): max|component| / sc in
+// [2^14, 2^15), the maximum taken over the source (every stored element is a source
+// element or 0).  Non-negative floats order as their bit patterns, so the workgroups
+// combine their maxima with an integer atomicMax on the scale slot, which
+// scale_clear_kernel zeroes first; scale_final_kernel turns the maximum into sc.
+__global__ void scale_clear_kernel(uint32_t* slot) { *slot = 0u; }
+
+__global__ __launch_bounds__(256) void absmax_kernel(const float2* __restrict__ src, size_t n,
+                                                     uint32_t* __restrict__ slot) {
   float mx = 0.f;
-  if (t < 729) {
-    const int s = bs * 8 + t / 81, m = bm * 8 + (t / 9) % 9, f = bf * 8 + t % 9;
-    if (s < a.lay.sdim[0] && m < a.lay.sdim[1] && f < a.lay.sdim[2]) {
-      float2 v = pack_src(a, s, m, f);
-      mx = fmaxf(fabsf(v.x), fabsf(v.y));
-    }
+  for (size_t k = size_t(blockIdx.x) * 256 + threadIdx.x; k < n; k += size_t(gridDim.x) * 256) {
+    const float2 v = src[k];
+    mx = fmaxf(mx, fmaxf(fabsf(v.x), fabsf(v.y)));
   }
   for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
-  __shared__ float red[12];
-  if ((t & 63) == 0) red[t >> 6] = mx;
+  __shared__ float red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
   __syncthreads();
-  if (t == 0) {
-    float m8 = red[0];
-    for (int i = 1; i < 12; ++i) m8 = fmaxf(m8, red[i]);
-    float sc = 1.f;
-    if (m8 > 0.f && isfinite(m8)) sc = ldexpf(1.f, ilogbf(m8) - 14);
-    scales[(size_t(bs) * a.lay.bricks[1] + bm) * a.lay.bricks[2] + bf] = sc;
-  }
+  if (threadIdx.x == 0) atomicMax(slot, __float_as_uint(fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]))));
+}
+
+__global__ void scale_final_kernel(float* slot) {
+  const float m = __uint_as_float(*reinterpret_cast<const uint32_t*>(slot));
+  *slot = (m > 0.f && isfinite(m)) ? ldexpf(1.f, ilogbf(m) - 14) : 1.f;
 }
 
 __global__ __launch_bounds__(256) void pack_kernel(PackArgs a) {
-  // (gs, gm, gf): physical index; (s, m, f): the storage index it holds; (bs, bm, bf): its brick
-  const int gf = blockIdx.x * 256 + threadIdx.x;
-  const int gm = blockIdx.y, gs = blockIdx.z;
-  const bool apron = a.lay.flags & FVR_LAYOUT_APRON;
-  const int pf = int(a.lay.row_bytes / a.lay.elem_bytes);
-  if (gf >= pf) return;
-  const int bs = apron ? gs / 9 : gs >> 3, bm = apron ? gm / 9 : gm >> 3, bf = apron ? gf / 9 : gf >> 3;
-  const int s = apron ? 8 * bs + (gs - 9 * bs) : gs;
-  const int m = apron ? 8 * bm + (gm - 9 * bm) : gm;
-  const int f = apron ? 8 * bf + (gf - 9 * bf) : gf;
-  float2 v = (s < a.lay.sdim[0] && m < a.lay.sdim[1] && f < a.lay.sdim[2]) ? pack_src(a, s, m, f)
-                                                                            : make_float2(0.f, 0.f);
-  char* row = reinterpret_cast<char*>(a.lay.base) + size_t(gs) * a.lay.plane_bytes + size_t(gm) * a.lay.row_bytes;
+  const int f = blockIdx.x * 256 + threadIdx.x, m = blockIdx.y, s = blockIdx.z;  // storage index
+  if (f >= a.lay.sdim[2]) return;
+  const float2 v = pack_src(a, s, m, f);
+  char* row = reinterpret_cast<char*>(a.lay.base) + size_t(s) * a.lay.plane_bytes + size_t(m) * a.lay.row_bytes;
   if (a.lay.dtype == FVR_C32) {
-    reinterpret_cast<float2*>(row)[gf] = v;
+    reinterpret_cast<float2*>(row)[f] = v;
   } else {
-    const float* scales = reinterpret_cast<const float*>(a.lay.scales);
-    float inv = 1.f / scales[(size_t(bs) * a.lay.bricks[1] + bm) * a.lay.bricks[2] + bf];
+    const float inv = 1.f / *reinterpret_cast<const float*>(a.lay.scales);  // a power of two: exact
     uint16_t re, im;
     if (a.lay.dtype == FVR_C16F) { re = f2h(v.x * inv); im = f2h(v.y * inv); }
     else { re = f2b(v.x * inv); im = f2b(v.y * inv); }
-    reinterpret_cast<uint32_t*>(row)[gf] = uint32_t(re) | (uint32_t(im) << 16);
+    reinterpret_cast<uint32_t*>(row)[f] = uint32_t(re) | (uint32_t(im) << 16);
   }
 }
 
@@ -164,7 +152,7 @@ int fvr_layout_init(fvr_layout* lay, const int32_t n[3], int32_t dtype, uint64_t
   dtype &= 0xFF;
   FVR_CHECK_ARG(dtype == FVR_C32 || dtype == FVR_C16F || dtype == FVR_C16B, "fvr_layout_init: bad dtype %d",
                 dtype);
-  FVR_CHECK_ARG((lflags & ~FVR_LAYOUT_WRAP) == 0, "fvr_layout_init: bad layout flags %u", lflags);  // APRON is derived
+  FVR_CHECK_ARG((lflags & ~FVR_LAYOUT_WRAP) == 0, "fvr_layout_init: bad layout flags %u", lflags);
   for (int a = 0; a < 3; ++a) FVR_CHECK_ARG(n[a] >= 2 && n[a] < (1 << 14), "fvr_layout_init: bad dim %d", n[a]);
   fvr_layout L = {};
   for (int a = 0; a < 3; ++a) L.n[a] = n[a];
@@ -174,14 +162,10 @@ int fvr_layout_init(fvr_layout* lay, const int32_t n[3], int32_t dtype, uint64_t
   L.sdim[0] = n[0] + 3 + extra;
   L.sdim[1] = n[1] + 2 + extra;
   L.sdim[2] = n[2] + 2 + extra;
-  for (int a = 0; a < 3; ++a) L.bricks[a] = (L.sdim[a] + 7) / 8;
   L.elem_bytes = dtype == FVR_C32 ? 8u : 4u;
-  if (dtype != FVR_C32) L.flags |= FVR_LAYOUT_APRON;  // scaled storage: 9^3 physical per 8^3 brick
-  int pdim[3];
-  for (int a = 0; a < 3; ++a) pdim[a] = dtype == FVR_C32 ? L.sdim[a] : 9 * L.bricks[a];
-  uint64_t row = uint64_t(pdim[2]) * L.elem_bytes;
-  uint64_t plane = row * uint64_t(pdim[1]);
-  uint64_t total = plane * uint64_t(pdim[0]);
+  const uint64_t row = uint64_t(L.sdim[2]) * L.elem_bytes;
+  const uint64_t plane = row * uint64_t(L.sdim[1]);
+  const uint64_t total = plane * uint64_t(L.sdim[0]);
   // kernels form 32-bit byte offsets with 24-bit multiplies
   FVR_CHECK_ARG(plane < (1ull << 24) && total < (1ull << 32),
                 "fvr_layout_init: spectrum %dx%dx%d too large for 32-bit offsets", n[0], n[1], n[2]);
@@ -190,7 +174,7 @@ int fvr_layout_init(fvr_layout* lay, const int32_t n[3], int32_t dtype, uint64_t
   L.total_bytes = total;
   *lay = L;
   if (storage_bytes) *storage_bytes = total;
-  if (n_scales) *n_scales = dtype == FVR_C32 ? 0 : uint64_t(L.bricks[0]) * L.bricks[1] * L.bricks[2];
+  if (n_scales) *n_scales = dtype == FVR_C32 ? 0 : 1;
   return FVR_OK;
 }
 
@@ -210,16 +194,20 @@ int fvr_spec_pack(const fvr_layout* lay, uint64_t src, int32_t nz, int32_t ny, i
   a.ny = ny;
   a.nx = nx;
   a.transpose = transpose_xy;
-  const bool apron = lay->flags & FVR_LAYOUT_APRON;
-  FVR_CHECK_ARG(apron == (lay->dtype != FVR_C32), "fvr_spec_pack: scaled dtypes use the apron layout");
+  FVR_CHECK_ARG(lay->flags == (lay->flags & FVR_LAYOUT_WRAP) && lay->row_bytes == uint32_t(lay->sdim[2]) * lay->elem_bytes,
+                "fvr_spec_pack: layout descriptor not from fvr_layout_init");
   if (lay->dtype != FVR_C32) {
-    dim3 g(lay->bricks[2], lay->bricks[1], lay->bricks[0]);
-    hipLaunchKernelGGL(pack_scales_kernel, g, dim3(768), 0, st, a, reinterpret_cast<float*>(lay->scales));
-    FVR_CHECK_LAUNCH("pack_scales_kernel");
+    uint32_t* slot = reinterpret_cast<uint32_t*>(lay->scales);
+    const size_t n = size_t(nz) * ny * nx;
+    const int blocks = int(std::min<size_t>((n + 255) / 256, 4096));
+    hipLaunchKernelGGL(scale_clear_kernel, dim3(1), dim3(1), 0, st, slot);
+    FVR_CHECK_LAUNCH("scale_clear_kernel");
+    hipLaunchKernelGGL(absmax_kernel, dim3(blocks), dim3(256), 0, st, a.src, n, slot);
+    FVR_CHECK_LAUNCH("absmax_kernel");
+    hipLaunchKernelGGL(scale_final_kernel, dim3(1), dim3(1), 0, st, reinterpret_cast<float*>(slot));
+    FVR_CHECK_LAUNCH("scale_final_kernel");
   }
-  const int pm = apron ? 9 * lay->bricks[1] : lay->sdim[1], ps = apron ? 9 * lay->bricks[0] : lay->sdim[0];
-  const int pf = int(lay->row_bytes / lay->elem_bytes);
-  dim3 g((pf + 255) / 256, pm, ps);
+  dim3 g((lay->sdim[2] + 255) / 256, lay->sdim[1], lay->sdim[0]);
   hipLaunchKernelGGL(pack_kernel, g, dim3(256), 0, st, a);
   FVR_CHECK_LAUNCH("pack_kernel");
   return FVR_OK;

@@ -42,9 +42,9 @@ struct cpx {
 };
 
 // Raw loads of one sample: the four fast-axis corner pairs (slow, mid) =
-// (0,0), (0,1), (1,0), (1,1) and, for scaled storage, the one scale of the
-// brick that holds all eight corners (apron layout).  They are issued one
-// sample ahead of their use (see the row loop).
+// (0,0), (0,1), (1,0), (1,1).  They are issued one sample ahead of their use
+// (see the row loop).  Scaled storage holds value / sc with one power-of-two
+// scale sc per spectrum: samples are interpolated in units of sc.
 template <int DT>
 struct Raw;
 
@@ -57,9 +57,7 @@ struct Raw<FVR_C32> {
                                                     uint32_t rowb) {
     return __umul24(S, planeb) + __umul24(M, rowb) + F * 8u;
   }
-  __device__ __forceinline__ float scale() const { return 1.f; }
-  __device__ __forceinline__ void load(gcptr b0, gcptr b1, uint32_t off, uint32_t row, gcptr, uint32_t, uint32_t,
-                                       uint32_t, uint32_t, uint32_t) {
+  __device__ __forceinline__ void load(gcptr b0, gcptr b1, uint32_t off, uint32_t row) {
     typedef const f4a8 __attribute__((address_space(1)))* gp;
     const uint32_t offr = off + row;
     q[0] = *reinterpret_cast<gp>(b0 + off);
@@ -86,25 +84,18 @@ template <int DT>
 struct RawQ {
   static constexpr uint32_t kElem = 4;
   u2a4 q[4];
-  float s;  // scale of the brick holding the cell (every corner was quantised with it)
-  // apron layout: storage index x of an axis sits at physical index x + (x >> 3)
   static __device__ __forceinline__ uint32_t offset(uint32_t S, uint32_t M, uint32_t F, uint32_t planeb,
                                                     uint32_t rowb) {
-    return __umul24(S + (S >> 3), planeb) + __umul24(M + (M >> 3), rowb) + (F + (F >> 3)) * 4u;
+    return __umul24(S, planeb) + __umul24(M, rowb) + F * 4u;
   }
-  // S, M, F: storage indices of the low corner; Bm, Bf: brick counts along mid / fast
-  __device__ __forceinline__ void load(gcptr b0, gcptr b1, uint32_t off, uint32_t row, gcptr scales, uint32_t S,
-                                       uint32_t M, uint32_t F, uint32_t Bm, uint32_t Bf) {
+  __device__ __forceinline__ void load(gcptr b0, gcptr b1, uint32_t off, uint32_t row) {
     typedef const u2a4 __attribute__((address_space(1)))* gp;
     const uint32_t offr = off + row;
     q[0] = *reinterpret_cast<gp>(b0 + off);
     q[1] = *reinterpret_cast<gp>(b0 + offr);
     q[2] = *reinterpret_cast<gp>(b1 + off);
     q[3] = *reinterpret_cast<gp>(b1 + offr);
-    const uint32_t r = __umul24(__umul24(S >> 3, Bm) + (M >> 3), Bf) + (F >> 3);
-    s = *reinterpret_cast<gfptr>(scales + (r << 2));
   }
-  __device__ __forceinline__ float scale() const { return s; }
   // corner pair k along the fast axis, in units of the scale: the difference
   // d = high - low and the value e = low + t d at fraction t.  FP16: four
   // v_fma_mix_f32 that read the f16 halves directly (the conversion inside the FMA is
@@ -144,6 +135,34 @@ struct Acc {
   float A, E, sim, mrim;
   float gs[3], gr[3], hs[3], hr[3];
 };
+
+// the spectrum's scale sc (1 for complex64 storage)
+template <int DT>
+__device__ __forceinline__ float layout_scale(const fvr_layout& L) {
+  if constexpr (DT == FVR_C32) return 1.f;
+  else return *reinterpret_cast<gfptr>(reinterpret_cast<gcptr>(L.scales));
+}
+
+// moments accumulated in units of sc -> spectrum units: the sums linear in the slice
+// take sc, the quadratic ones (E and the H moments) sc^2.  sc is a power of two, so the
+// result is the one of scaling every sample.
+template <int DT>
+__device__ __forceinline__ void acc_scale(Acc& ac, float sc) {
+  if constexpr (DT != FVR_C32) {
+    const float s2 = sc * sc;
+    ac.A *= sc;
+    ac.sim *= sc;
+    ac.mrim *= sc;
+    ac.E *= s2;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      ac.gs[a] *= sc;
+      ac.gr[a] *= sc;
+      ac.hs[a] *= s2;
+      ac.hr[a] *= s2;
+    }
+  }
+}
 
 __device__ __forceinline__ float readlane_f(float v, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
@@ -240,8 +259,7 @@ __device__ __forceinline__ void sncc_tile(const fvr_layout* __restrict__ layouts
   const gcptr b0 = reinterpret_cast<gcptr>(L.base);
   const gcptr b1 = b0 + planeb;
   const uint32_t guard = it.guard_off;
-  const gcptr scales = reinterpret_cast<gcptr>(L.scales);
-  const uint32_t Bm = uint32_t(L.bricks[1]), Bf = uint32_t(L.bricks[2]);
+  const float gsc = layout_scale<DT>(L);
   const gcptr Tb = reinterpret_cast<gcptr>(it.target);
   float2* __restrict__ Sout = reinterpret_cast<float2*>(it.slice_out);
 
@@ -276,7 +294,7 @@ __device__ __forceinline__ void sncc_tile(const fvr_layout* __restrict__ layouts
     sm.tm = float(uint32_t(p1)) * kTwoM32;
     sm.tf = float(uint32_t(p2)) * kTwoM32;
     const uint32_t S = fs + lo_int, M = fm + lo_int, F = ff + lo_int;
-    sm.raw.load(b0, b1, inb ? Raw<DT>::offset(S, M, F, planeb, rowb) : guard, rowb, scales, S, M, F, Bm, Bf);
+    sm.raw.load(b0, b1, inb ? Raw<DT>::offset(S, M, F, planeb, rowb) : guard, rowb);
     sm.t = *reinterpret_cast<gf2ptr>(Tb + (ok ? toff : 0u));
     sm.ok = ok;
     p0 += v0;
@@ -311,20 +329,19 @@ __device__ __forceinline__ void sncc_tile(const fvr_layout* __restrict__ layouts
     const float prr = readlane_f(rowph.r, k), pri = readlane_f(rowph.i, k);
     const float phr = colph.r * prr - colph.i * pri;
     const float phi = fmaf(colph.r, pri, colph.i * prr);
-    // I, dI above are in units of the brick scale sc: the scale is carried by W (sc) and by
-    // the factor J of E and of the H moments (sc^2)
-    const float sc = q.scale();
-    if (WS && sm.ok) Sout[i * iw + j] = make_float2(sc * (Ir * phr - Ii * phi), sc * fmaf(Ir, phi, Ii * phr));
-    float wsc = sc, esc = sc * sc;
+    // I, dI above are in units of the spectrum's scale sc, applied to the sums at the end
+    if (WS && sm.ok) Sout[i * iw + j] = make_float2(gsc * (Ir * phr - Ii * phi), gsc * fmaf(Ir, phi, Ii * phr));
+    // W = phase * conj(T) (times the high-pass weight hw); J = I (times hw^2)
+    float Wr = fmaf(phr, t.x, phi * t.y);
+    float Wi = fmaf(phi, t.x, -phr * t.y);
+    float Jr = Ir, Ji = Ii;
     if (HP) {
-      const float hw = fmaf(-hx, readlane_f(hy, k), 1.f);
-      wsc *= hw;
-      esc *= hw * hw;
+      const float hw = fmaf(-hx, readlane_f(hy, k), 1.f), hw2 = hw * hw;
+      Wr *= hw;
+      Wi *= hw;
+      Jr *= hw2;
+      Ji *= hw2;
     }
-    // W = phase * conj(T)
-    const float Wr = wsc * fmaf(phr, t.x, phi * t.y);
-    const float Wi = wsc * fmaf(phi, t.x, -phr * t.y);
-    const float Jr = Ir * esc, Ji = Ii * esc;
     const float rw = rw0 + float(i) * rws;
     const float IWr = fmaf(Ir, Wr, -Ii * Wi);
     const float IWi = fmaf(Ir, Wi, Ii * Wr);
@@ -374,6 +391,7 @@ __device__ __forceinline__ void sncc_tile(const fvr_layout* __restrict__ layouts
   consume(sa, R - 2);
   consume(sb, R - 1);
 
+  acc_scale<DT>(ac, gsc);
   tile_reduce(ac, cw, tile, partials);
 }
 
@@ -461,8 +479,7 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
   const gcptr b0 = reinterpret_cast<gcptr>(L.base);
   const gcptr b1 = b0 + planeb;
   const uint32_t guard = it.guard_off;
-  const gcptr scales = reinterpret_cast<gcptr>(L.scales);
-  const uint32_t Bm = uint32_t(L.bricks[1]), Bf = uint32_t(L.bricks[2]);
+  const float gsc = layout_scale<DT>(L);
   const gcptr TT = reinterpret_cast<gcptr>(it.target);
 
   const uint64_t h0 = it.box_hi[0], h1 = it.box_hi[1], h2 = it.box_hi[2];
@@ -517,7 +534,7 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
     sm.tm = float(uint32_t(p1)) * kTwoM32;
     sm.tf = float(uint32_t(p2)) * kTwoM32;
     const uint32_t S = fs + lo_int, M = fm + lo_int, F = ff + lo_int;
-    sm.raw.load(b0, b1, inb ? Raw<DT>::offset(S, M, F, planeb, rowb) : guard, rowb, scales, S, M, F, Bm, Bf);
+    sm.raw.load(b0, b1, inb ? Raw<DT>::offset(S, M, F, planeb, rowb) : guard, rowb);
     sm.t = *reinterpret_cast<gf4ptr>(TT + toff);
     sm.bp = bp;
     sm.bq = bq;
@@ -534,7 +551,7 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
       sm.ts = float(uint32_t(q0)) * kTwoM32;
       sm.tm = float(uint32_t(q1)) * kTwoM32;
       sm.tf = float(uint32_t(q2)) * kTwoM32;
-      sm.raw.load(b0, b1, Raw<DT>::offset(S, M, F, planeb, rowb), rowb, scales, S, M, F, Bm, Bf);
+      sm.raw.load(b0, b1, Raw<DT>::offset(S, M, F, planeb, rowb), rowb);
       sm.t = *reinterpret_cast<gf4ptr>(TT + toffr);
     } else {
       const bool h2 = first && rl == 0 && t == 0;
@@ -567,18 +584,17 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
     const float zr = f1r - f0r, zi = f1i - f0i;
     const float Ir = fmaf(ts, zr, f0r), Ii = fmaf(ts, zi, f0i);
 
-    // folded target T'' = bp T + bq conj T(mirror); W = phase conj(T'') sc, with I, dI
-    // in units of the brick scale sc (E and the H moments carry sc^2 through J)
-    const float sc = q.scale();
+    // folded target T'' = bp T + bq conj T(mirror); W = phase conj(T''), with I, dI in
+    // units of the spectrum's scale (applied to the sums at the end)
     float Tr, Ti, wE;
     if constexpr (decltype(interior)::value) {
-      Tr = sc * (sm.t.x + sm.t.z);
-      Ti = sc * (sm.t.y + sm.t.w);
-      wE = wEi * (sc * sc);
+      Tr = sm.t.x + sm.t.z;
+      Ti = sm.t.y + sm.t.w;
+      wE = wEi;
     } else {
-      Tr = sc * fmaf(wq, sm.t.z, wp * sm.t.x);
-      Ti = sc * fmaf(wq, sm.t.w, wp * sm.t.y);
-      wE = (wp + wq) * (sc * sc);
+      Tr = fmaf(wq, sm.t.z, wp * sm.t.x);
+      Ti = fmaf(wq, sm.t.w, wp * sm.t.y);
+      wE = wp + wq;
     }
     const float Wr = fmaf(phr, Tr, phi * Ti);
     const float Wi = fmaf(phi, Tr, -phr * Ti);
@@ -688,6 +704,7 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
     extra_issue();
     extra_consume();
   }
+  acc_scale<DT>(ac, gsc);
   tile_reduce(ac, 1.f, tile, partials);
 }
 

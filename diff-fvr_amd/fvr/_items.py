@@ -99,15 +99,9 @@ def tiles_for(h: int, w: int, tile_rows: int, herm: bool = False) -> Tuple[int, 
     return (w + 63) // 64, (h + tile_rows - 1) // tile_rows
 
 
-def physical_index(x, desc):
-    """Physical slow/mid/fast index of storage index ``x`` in a layout: x + (x >> 3) in the
-    apron layout of scaled storage (every 8-element brick run followed by an apron copy)."""
-    return x + (x >> 3) if int(desc["flags"]) & _lib.LAYOUT_APRON else x
-
-
 def guard_offset(desc) -> int:
     """Byte offset of the all-zero guard plane (storage plane sdim0 - 2) of a layout."""
-    return int(physical_index(int(desc["sdim"][0]) - 2, desc)) * int(desc["plane_bytes"])
+    return (int(desc["sdim"][0]) - 2) * int(desc["plane_bytes"])
 
 
 def herm_ok(h: int, w: int) -> bool:
@@ -303,7 +297,6 @@ def validate_items(items: np.ndarray, n_layouts: int, descs: np.ndarray, tile_ro
     if np.any(hi + lo[:, None] + 1 > d["sdim"].astype(np.int64) - 1):
         raise ValueError("item box exceeds the layout's storage")
     s_g = d["sdim"][:, 0].astype(np.int64) - 2
-    s_g = np.where((d["flags"] & _lib.LAYOUT_APRON) != 0, s_g + (s_g >> 3), s_g)
     if np.any(items["guard_off"].astype(np.int64) != s_g * d["plane_bytes"]):
         raise ValueError("item guard offset is not the layout's zero plane")
     return int(ntile.sum())

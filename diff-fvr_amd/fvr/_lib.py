@@ -15,18 +15,18 @@ import torch  # noqa: F401  (must be loaded before the HIP library)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libfvr_hip.so")
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 FVR_C32, FVR_C16F, FVR_C16B = 0, 1, 2
 DTYPE_CODES = {"f32": FVR_C32, "fp16": FVR_C16F, "bf16": FVR_C16B}
 ITEM_ADJOINT, ITEM_HP, ITEM_WRITE_SLICE, ITEM_HERM = 1, 2, 4, 8
-LAYOUT_WRAP, LAYOUT_APRON = 1, 2
+LAYOUT_WRAP = 1
 NMOM = 16
 
 # numpy mirrors of the C structs (explicit offsets; sizes are checked against
 # the library at load time)
 LAYOUT_DTYPE = np.dtype({
-    "names": ["base", "scales", "n", "dtype", "sdim", "bricks", "elem_bytes", "row_bytes", "plane_bytes",
+    "names": ["base", "scales", "n", "dtype", "sdim", "reserved", "elem_bytes", "row_bytes", "plane_bytes",
               "flags", "total_bytes"],
     "formats": ["<u8", "<u8", ("<i4", 3), "<i4", ("<i4", 3), ("<i4", 3), "<u4", "<u4", "<u4", "<u4", "<u8"],
     "offsets": [0, 8, 16, 28, 32, 44, 56, 60, 64, 68, 72],

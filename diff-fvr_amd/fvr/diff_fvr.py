@@ -54,7 +54,7 @@ class DifferentiableFourierVolumeRenderer:
         return self._spec.cpu().numpy()
 
     def repack(self, storage: str) -> "DifferentiableFourierVolumeRenderer":
-        """Re-store the spectrum as "f32", "fp16" or "bf16" (per-8^3-brick scale)."""
+        """Re-store the spectrum as "f32", "fp16" or "bf16" (one power-of-two scale)."""
         self.storage = storage
         self.packed = PackedSpectrum(self._spec, dtype=storage, layouts=PACK_LAYOUTS)
         return self

@@ -1,10 +1,10 @@
 """Algorithmic byte count of the fused slice kernel (for roofline reporting).
 
 For one convention-N slice, U is the number of distinct spectrum entries with
-a non-zero trilinear weight (scipy box rule); for scaled storage the number
-of distinct 8^3 scale bricks is counted too.  Bytes per slice are
-``U * elem_bytes + bricks * 4``; the target spectrum (complex64, shared by all
-candidate poses of a batch) is added once per launch by the caller.
+a non-zero trilinear weight (scipy box rule).  Bytes per slice are
+``U * elem_bytes`` (plus the 4-byte scale of scaled storage once per launch); the
+target spectrum (complex64, shared by all candidate poses of a batch) is added once
+per launch.
 Runs on whatever device the tensors live on (torch ops; a measurement helper,
 not part of the rendering path).
 """
@@ -34,8 +34,8 @@ def _herm_slots(h: int, w: int, device):
     return torch.cat(ii), torch.cat(jj), torch.cat(mm)
 
 
-def slice_footprint(pad_shape, pose, view: str, device="cpu", herm: bool = False) -> Tuple[int, int, int]:
-    """(U, bricks, samples) of one convention-N slice at pose (yaw, pitch, roll in degrees).
+def slice_footprint(pad_shape, pose, view: str, device="cpu", herm: bool = False) -> Tuple[int, int]:
+    """(U, samples) of one convention-N slice at pose (yaw, pitch, roll in degrees).
 
     ``herm``: the samples the Hermitian-fold kernel evaluates (about half of the slice,
     each inside if it or its mirror is inside the scipy box) on the wrapped lattice.
@@ -62,7 +62,6 @@ def slice_footprint(pad_shape, pose, view: str, device="cpu", herm: bool = False
     i0 = torch.floor(p).to(torch.int64)
     t = p - i0
     keys = []
-    bkeys = []
     for dz in (0, 1):
         for dy in (0, 1):
             for dx in (0, 1):
@@ -71,12 +70,8 @@ def slice_footprint(pad_shape, pose, view: str, device="cpu", herm: bool = False
                 idx = i0 + torch.tensor([dz, dy, dx], device=device)
                 idx = idx[wgt > 0]
                 keys.append((idx[:, 0] * (py + 2) + idx[:, 1]) * (px + 2) + idx[:, 2])
-                # storage bricks: storage index = logical + 1, 8^3 bricks
-                b = (idx + 1) // 8
-                bkeys.append((b[:, 0] * 4096 + b[:, 1]) * 4096 + b[:, 2])
     U = int(torch.unique(torch.cat(keys)).numel())
-    B = int(torch.unique(torch.cat(bkeys)).numel())
-    return U, B, samples
+    return U, samples
 
 
 def batch_algorithmic_bytes(pad_shape, poses: np.ndarray, elem_bytes: int, scaled: bool, device="cpu",
@@ -85,16 +80,15 @@ def batch_algorithmic_bytes(pad_shape, poses: np.ndarray, elem_bytes: int, scale
 
     ``herm``: counted for the Hermitian-fold schedule (its half-plane entries and its
     16-byte folded target entries)."""
-    total_u = total_b = total_s = 0
+    total_u = total_s = 0
     for p in poses:
         for view in ("ap", "lateral"):
-            U, B, S = slice_footprint(pad_shape, p, view, device, herm)
+            U, S = slice_footprint(pad_shape, p, view, device, herm)
             total_u += U
-            total_b += B
             total_s += S
     h_ap, w_ap = slice_dims_n(pad_shape, "ap")
     h_la, w_la = slice_dims_n(pad_shape, "lateral")
     targ = (total_s // max(len(poses), 1)) * 16 if herm else (h_ap * w_ap + h_la * w_la) * 8
-    spec = total_u * elem_bytes + (total_b * 4 if scaled else 0)
-    return {"unique_entries": total_u, "bricks": total_b, "spectrum_bytes": spec, "target_bytes": targ,
+    spec = total_u * elem_bytes + (4 if scaled else 0)
+    return {"unique_entries": total_u, "spectrum_bytes": spec, "target_bytes": targ,
             "bytes": spec + targ, "samples": total_s, "schedule": "hermitian-fold" if herm else "full-slice"}

@@ -6,7 +6,7 @@
 torch.fft).  ``PackedSpectrum`` stores it in the layouts the fused kernel
 gathers from (``fvr_spec_pack``): one copy whose fast axis is x (AP and
 axial views) and one whose fast axis is y (lateral view), each with zero
-margins, in complex64 or FP16/BF16 with a power-of-two scale per 8^3 brick.
+margins, in complex64 or FP16/BF16 with one power-of-two scale per spectrum.
 """
 
 from __future__ import annotations

@@ -215,7 +215,7 @@ class TorchDiffFVREngine(nn.Module):
         """Store the spectrum in ``dtype`` (float32, float16, bfloat16; torch_engine.py:90-95).
 
         The fused path then gathers from FP16/BF16 storage with one power-of-two
-        scale per 8^3 brick (no overflow for any spectrum magnitude).
+        scale per spectrum (no overflow for any spectrum magnitude).
         """
         if dtype not in _STORAGE_OF:
             raise TypeError(f"dtype must be float32, float16 or bfloat16, got {dtype}")

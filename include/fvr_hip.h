@@ -18,7 +18,8 @@
  *                       triton_slice.py:146-339 provides to autograd.
  *   fvr_spec_pack       the storage step of diff_fvr.py:22-54 /
  *                       torch_engine.py:76-88 (centred spectrum -> device
- *                       layout, optional FP16/BF16 with per-brick scale).
+ *                       layout, optional FP16/BF16 with one power-of-two
+ *                       scale per spectrum).
  *   fvr_slice_fwd/bwd   triton_slice.py:648-661 fourier_slice_3d (forward and
  *                       the u/v gradient of triton_slice.py:306-339).
  *   fvr_pou_warp_fwd/bwd  triton_slice.py:787-808 pou_pinhole_warp_sum (forward
@@ -42,7 +43,7 @@
 extern "C" {
 #endif
 
-#define FVR_ABI_VERSION 3
+#define FVR_ABI_VERSION 4
 
 typedef enum fvr_status {
   FVR_OK = 0,
@@ -54,8 +55,8 @@ typedef enum fvr_status {
 /* element type of a packed spectrum */
 typedef enum fvr_dtype {
   FVR_C32 = 0,  /* complex64 interleaved (re, im) */
-  FVR_C16F = 1, /* fp16 (re, im) pairs, one fp32 power-of-two scale per 8^3 brick */
-  FVR_C16B = 2  /* bf16 (re, im) pairs, one fp32 power-of-two scale per 8^3 brick */
+  FVR_C16F = 1, /* fp16 (re, im) pairs, one fp32 power-of-two scale per spectrum */
+  FVR_C16B = 2  /* bf16 (re, im) pairs, one fp32 power-of-two scale per spectrum */
 } fvr_dtype;
 
 /* A packed spectrum: logical dims (slow, mid, fast) stored with a one-voxel
@@ -67,23 +68,20 @@ typedef enum fvr_dtype {
  * Hermitian-symmetric about n/2 for the spectrum of a real volume) and the
  * storage dims are slow n0+4, mid n1+3, fast n2+3.
  *
- * Scaled dtypes (FVR_C16F / FVR_C16B) use the apron layout (FVR_LAYOUT_APRON,
- * set by fvr_layout_init): storage index s of an axis lives at physical index
- * s + (s >> 3), i.e. every 8-element brick run is followed by a copy of the
- * next brick's first element, and every stored element -- apron copies
- * included -- is quantised with the scale of the brick it is stored in.  The
- * 2x2x2 corners of a trilinear cell whose low corner is s then sit at physical
- * indices s + (s >> 3) and that + 1 in one brick, under one scale.  Physical
- * dims are 9 * bricks; row_bytes and plane_bytes are physical strides. */
+ * Scaled dtypes (FVR_C16F / FVR_C16B) store every element as value / sc with one
+ * power-of-two scale sc per spectrum: max|component| / sc in [2^14, 2^15).  A
+ * power-of-two scale does not change a float's rounding, so an element rounds as
+ * it would under any other power-of-two scale that keeps it a normal fp16 number;
+ * the kernels work in units of sc and apply it once per tile.  Physical dims are
+ * the storage dims; row_bytes and plane_bytes are the strides. */
 #define FVR_LAYOUT_WRAP 1u
-#define FVR_LAYOUT_APRON 2u
 typedef struct fvr_layout {
   uint64_t base;        /* device address of storage element (0,0,0) */
-  uint64_t scales;      /* device address of float brick scales (0 for FVR_C32) */
+  uint64_t scales;      /* device address of the float scale sc (one element; 0 for FVR_C32) */
   int32_t n[3];         /* logical dims (slow, mid, fast) */
   int32_t dtype;        /* fvr_dtype */
   int32_t sdim[3];      /* storage dims */
-  int32_t bricks[3];    /* brick counts over storage dims (8^3 bricks) */
+  int32_t reserved[3];  /* 0 */
   uint32_t elem_bytes;  /* 8 (C32) or 4 (C16F, C16B) */
   uint32_t row_bytes;   /* physical fast-axis length * elem_bytes */
   uint32_t plane_bytes; /* physical mid-axis length * row_bytes */
@@ -92,8 +90,8 @@ typedef struct fvr_layout {
 } fvr_layout;
 
 /* Fill a layout descriptor for dims n (slow, mid, fast) and dtype (| FVR_LAYOUT_WRAP << 8
- * for the wrapped layout); returns the storage bytes and, for scaled dtypes, the
- * scale-array element count. */
+ * for the wrapped layout); returns the storage bytes and the scale-array element
+ * count (1 for scaled dtypes, 0 for FVR_C32). */
 int fvr_layout_init(fvr_layout* lay, const int32_t n[3], int32_t dtype,
                     uint64_t* storage_bytes, uint64_t* n_scales);
 

@@ -55,14 +55,18 @@ def test_layout_init_and_error_reporting():
     assert lay["row_bytes"][0] == 514 * 8 and lay["plane_bytes"][0] == 514 * 514 * 8
     assert nb.value == 515 * 514 * 514 * 8 and ns.value == 0
     assert L.fvr_layout_init(lay.ctypes.data, n.ctypes.data, 1, ctypes.byref(nb), ctypes.byref(ns)) == 0
-    assert ns.value == 65 * 65 * 65 and lay["elem_bytes"][0] == 4
-    # scaled storage: apron layout, 9 physical elements per 8-element brick run on every axis
-    assert lay["flags"][0] == _lib.LAYOUT_APRON and tuple(lay["bricks"][0]) == (65, 65, 65)
-    assert lay["row_bytes"][0] == 585 * 4 and lay["plane_bytes"][0] == 585 * 585 * 4
-    assert nb.value == 585 ** 3 * 4
-    from fvr._items import guard_offset, physical_index
-    assert physical_index(7, lay[0]) == 7 and physical_index(8, lay[0]) == 9 and physical_index(513, lay[0]) == 577
-    assert guard_offset(lay[0]) == 577 * 585 * 585 * 4
+    # scaled storage: the same storage dims with 4-byte elements and one scale per spectrum
+    assert ns.value == 1 and lay["elem_bytes"][0] == 4 and lay["flags"][0] == 0
+    assert tuple(lay["reserved"][0]) == (0, 0, 0)
+    assert lay["row_bytes"][0] == 514 * 4 and lay["plane_bytes"][0] == 514 * 514 * 4
+    assert nb.value == 515 * 514 * 514 * 4
+    from fvr._items import guard_offset
+    assert guard_offset(lay[0]) == 513 * 514 * 514 * 4
+    # the wrapped layout: one more index per axis
+    assert L.fvr_layout_init(lay.ctypes.data, n.ctypes.data, 1 | (_lib.LAYOUT_WRAP << 8), ctypes.byref(nb),
+                             ctypes.byref(ns)) == 0
+    assert tuple(lay["sdim"][0]) == (516, 515, 515) and lay["flags"][0] == _lib.LAYOUT_WRAP
+    assert lay["row_bytes"][0] == 515 * 4 and nb.value == 516 * 515 * 515 * 4
     # too large for 32-bit offsets -> error with a message, no abort
     big = np.array([1024, 1024, 1024], dtype=np.int32)
     assert L.fvr_layout_init(lay.ctypes.data, big.ctypes.data, 0, ctypes.byref(nb), ctypes.byref(ns)) == 1

@@ -157,7 +157,7 @@ def test_pair_moment_helpers_match_oracle(setup):
 
 @pytest.mark.parametrize("storage,tol_slice,tol_grad", [("fp16", 2e-3, 2e-2), ("bf16", 2e-2, 1e-1)])
 def test_reduced_precision_storage(storage, tol_slice, tol_grad):
-    """FP16 / BF16 spectrum storage with per-8^3-brick power-of-two scale, FP32 interpolation.
+    """FP16 / BF16 spectrum storage with one power-of-two scale, FP32 interpolation.
 
     Bounds (relative to the largest magnitude): slices 2e-3 (fp16, 11-bit significand)
     / 2e-2 (bf16, 8-bit); loss 1e-3 / 1e-2 absolute; gradient 2e-2 / 1e-1 of max|grad|.
@@ -243,45 +243,41 @@ def test_hermitian_fold_equals_full_slice_kernel(setup, storage):
 
 
 @pytest.mark.parametrize("storage", ["fp16", "bf16"])
-def test_apron_layout_storage_image(storage):
+def test_scaled_storage_image(storage):
     """The packed FP16/BF16 storage, read back and decoded by the header's definition:
-    physical index g of an axis holds storage index 8 (g // 9) + g % 9 (logical index - 1,
-    zero outside the spectrum), quantised with the scale of brick g // 9 -- a power of two
-    with max|component| / scale in [2^14, 2^15) over the brick's 9^3 stored elements."""
-    from fvr import DifferentiableFourierVolumeRenderer
+    storage index s of an axis holds logical index s - 1 (zero outside the spectrum; with
+    the wrap flag, logical index n holds index 0), quantised with the spectrum's one
+    scale -- a power of two with max|component| / scale in [2^14, 2^15)."""
+    from fvr import DifferentiableFourierVolumeRenderer, _lib
     vol = _vol((20, 22, 18))
     r = DifferentiableFourierVolumeRenderer(vol, storage=storage)
     f3 = O.make_spectrum(vol)
+    cmax = max(np.abs(f3.real).max(), np.abs(f3.imag).max())
     for name, perm in (("zyx", (0, 1, 2)), ("zxy", (0, 2, 1))):
         k = r.packed.index[name]
         d = r.packed.descs[k]
-        n = np.array(d["n"])
-        B = np.array(d["bricks"])
-        raw = r.packed.storage[name].cpu().numpy().view(np.uint16).reshape(9 * B[0], 9 * B[1], 9 * B[2], 2)
-        sc = r.packed.scales[name].cpu().numpy().reshape(tuple(B))
+        n, sd = np.array(d["n"]), np.array(d["sdim"])
+        assert d["row_bytes"] == sd[2] * 4 and d["plane_bytes"] == sd[1] * sd[2] * 4
+        raw = r.packed.storage[name].cpu().numpy().view(np.uint16).reshape(sd[0], sd[1], sd[2], 2)
+        sc = r.packed.scales[name].cpu().numpy()
+        assert sc.shape == (1,)
+        sc = float(sc[0])
         spec = np.transpose(f3, perm)
-        g = [np.arange(9 * b) for b in B]
-        s = [8 * (x // 9) + x % 9 for x in g]  # storage index held by each physical index
-        # expected values: logical index = storage - 1, zero outside [0, n)
-        ok = [(x - 1 >= 0) & (x - 1 < m) for x, m in zip(s, n)]
-        idx = np.ix_(*[np.clip(x - 1, 0, m - 1) for x, m in zip(s, n)])
+        lg = [np.arange(m) - 1 for m in sd]  # logical index of each storage index
+        if int(d["flags"]) & _lib.LAYOUT_WRAP:
+            lg = [np.where(x == m, 0, x) for x, m in zip(lg, n)]
+        ok = [(x >= 0) & (x < m) for x, m in zip(lg, n)]
+        idx = np.ix_(*[np.clip(x, 0, m - 1) for x, m in zip(lg, n)])
         exp = spec[idx] * (ok[0][:, None, None] & ok[1][None, :, None] & ok[2][None, None, :])
         if storage == "fp16":
             dec = raw.view(np.float16).astype(np.float64)
         else:
             dec = (raw.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
-        scale = sc[np.ix_(*[x // 9 for x in g])]
-        got = (dec[..., 0] + 1j * dec[..., 1]) * scale
+        got = (dec[..., 0] + 1j * dec[..., 1]) * sc
         bits = 11 if storage == "fp16" else 8
         err = np.abs(got - exp)
         # round to nearest: half an ulp of the significand, plus fp16's subnormal step (2^-24 x scale)
-        assert np.all(err <= 2.0 ** -bits * np.abs(exp) * (1 + 1e-12) + 2.0 ** -24 * scale)
-        # scales: powers of two with the brick maximum in [2^14, 2^15) x scale
-        comp = np.maximum(np.abs(exp.real), np.abs(exp.imag))
-        bmax = np.zeros(tuple(B))
-        np.maximum.at(bmax, tuple(np.meshgrid(*[x // 9 for x in g], indexing="ij")), comp)
-        nz = bmax > 0
-        assert np.all(np.log2(sc) == np.round(np.log2(sc)))
-        assert np.all(bmax[nz] / sc[nz] >= 2.0 ** 14 * (1 - 1e-6)) and np.all(bmax[nz] / sc[nz] < 2.0 ** 15)
-        # the apron copy at physical 9b + 8 equals storage index 8b + 8 (the next brick's first element)
-        assert np.array_equal(s[2][8::9], 8 * np.arange(B[2]) + 8)
+        assert np.all(err <= 2.0 ** -bits * np.abs(exp) * (1 + 1e-12) + 2.0 ** -24 * sc)
+        # the scale: a power of two with the spectrum's maximum in [2^14, 2^15) x scale
+        assert np.log2(sc) == np.round(np.log2(sc))
+        assert 2.0 ** 14 * (1 - 1e-6) <= cmax / sc < 2.0 ** 15

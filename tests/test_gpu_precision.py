@@ -1,8 +1,8 @@
 """Precision of reduced spectrum storage against the FP64 oracle (96^3 phantom,
 192^3 spectrum, AP + LAT 96^2 -> 192^2 target spectra rendered at a seeded pose).
 
-For each storage type (complex64; FP16 / BF16 pairs with a power-of-two FP32
-scale per 8^3 brick; interpolation and sums in FP32) this states:
+For each storage type (complex64; FP16 / BF16 pairs with one power-of-two FP32
+scale per spectrum; interpolation and sums in FP32) this states:
 
 * loss and gradient at 16 seeded poses against the oracle's exact FP64 loss and
   gradient (``biplanar_loss_and_grad_analytic``);
