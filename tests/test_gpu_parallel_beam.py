@@ -242,6 +242,53 @@ def test_hermitian_fold_equals_full_slice_kernel(setup, storage):
             r.repack("f32")
 
 
+def _bf16_rne(x32: np.ndarray) -> np.ndarray:
+    """float32 -> bf16 bits, round to nearest even (finite inputs)."""
+    u = x32.view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+@pytest.mark.parametrize("storage", ["fp16", "bf16"])
+def test_scaled_storage_equals_per_brick_quantisation(storage):
+    """One scale per spectrum stores the numbers that a power-of-two scale per 8^3 brick
+    stores (the FP16 / BF16 precision variant of the benchmark configurations): the
+    device's own complex64 spectrum, quantised on the host per 8^3 brick of the storage
+    lattice (max|component| / scale in [2^14, 2^15), round to nearest even), decodes to
+    the same value, bit for bit, wherever the one-scale code is a normal number; the rest
+    (fp16 subnormals under the one scale) differ by at most the subnormal step."""
+    from fvr import DifferentiableFourierVolumeRenderer
+    vol = _vol((20, 22, 18))
+    r = DifferentiableFourierVolumeRenderer(vol, storage="f32")
+    name = "zyx"
+    d = r.packed.descs[r.packed.index[name]]
+    sd = tuple(int(v) for v in d["sdim"])
+    x = r.packed.storage[name].cpu().numpy().view(np.float32).reshape(sd + (2,))  # the packed complex64 source
+    r.repack(storage)
+    raw = r.packed.storage[name].cpu().numpy().view(np.uint16).reshape(sd + (2,))
+    sc = float(r.packed.scales[name].cpu().numpy()[0])
+    if storage == "fp16":
+        got = raw.view(np.float16).astype(np.float64) * sc
+    else:
+        got = (raw.astype(np.uint32) << 16).view(np.float32).astype(np.float64) * sc
+    # per-brick power-of-two scales over the storage lattice
+    B = [(n + 7) // 8 for n in sd]
+    pad = np.zeros((8 * B[0], 8 * B[1], 8 * B[2]), np.float64)
+    pad[:sd[0], :sd[1], :sd[2]] = np.abs(x).max(axis=-1)
+    bmax = pad.reshape(B[0], 8, B[1], 8, B[2], 8).max(axis=(1, 3, 5))
+    e = np.where(bmax > 0, np.floor(np.log2(np.where(bmax > 0, bmax, 1.0))) - 14, 0)
+    s_b = np.exp2(e)[np.ix_(*[np.arange(n) // 8 for n in sd])][..., None].astype(np.float32)
+    q = (x / s_b).astype(np.float32)  # exact: power-of-two divisor, no underflow at these magnitudes
+    if storage == "fp16":
+        ref = q.astype(np.float16).astype(np.float64) * s_b
+        normal = np.abs(x) / sc >= 2.0 ** -14
+    else:
+        ref = (_bf16_rne(q).astype(np.uint32) << 16).view(np.float32).astype(np.float64) * s_b
+        normal = np.ones(x.shape, bool)
+    assert normal.mean() > 0.5
+    assert np.array_equal(got[normal], ref[normal])
+    assert np.all(np.abs(got - ref)[~normal] <= 2.0 ** -24 * sc)
+
+
 @pytest.mark.parametrize("storage", ["fp16", "bf16"])
 def test_scaled_storage_image(storage):
     """The packed FP16/BF16 storage, read back and decoded by the header's definition:
