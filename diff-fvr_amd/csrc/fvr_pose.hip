@@ -84,7 +84,16 @@ __device__ __forceinline__ void build_item(const double* P, int pose, bool lat, 
   const V3 cg = v3(pz / 2.0 + uu0 * u.a0 + vv0 * v.a0, py / 2.0 + uu0 * u.a1 + vv0 * v.a1,
                    px / 2.0 + uu0 * u.a2 + vv0 * v.a2);
   const bool swap = lat && lat_swap;
-  fvr_sncc_item it = lat ? tlat : tap;
+  // the view's template, copied word by word with a per-lane select (no private copy)
+  static_assert(sizeof(fvr_sncc_item) % 4 == 0, "item words");
+  fvr_sncc_item it;
+  {
+    const uint32_t* wa = reinterpret_cast<const uint32_t*>(&tap);
+    const uint32_t* wb = reinterpret_cast<const uint32_t*>(&tlat);
+    uint32_t* wd = reinterpret_cast<uint32_t*>(&it);
+#pragma unroll
+    for (int k = 0; k < int(sizeof(fvr_sncc_item) / 4); ++k) wd[k] = lat ? wb[k] : wa[k];
+  }
   put_fx(it.c, cg, swap);
   for (int a = 0; a < 3; ++a) it.c[a] += kSnapFx;  // lattice-plane bias, as the host builder
   put_fx(it.ustep, u, swap);
@@ -326,6 +335,9 @@ struct TGeom {
   unsigned long long target0, target_stride;  // target of pose p: target0 + p * stride
 };
 
+// v[k] for a runtime k in {0, 1, 2} without indexing private memory
+__device__ __forceinline__ double pick3(const double (&v)[3], int k) { return k == 0 ? v[0] : (k == 1 ? v[1] : v[2]); }
+
 __global__ void pt_build_kernel(const double* __restrict__ poses, int n, fvr_sncc_item tmpl, TGeom G,
                                 fvr_sncc_item* __restrict__ items, double* __restrict__ jac) {
   const int pose = blockIdx.x * blockDim.x + threadIdx.x;
@@ -356,8 +368,9 @@ __global__ void pt_build_kernel(const double* __restrict__ poses, int n, fvr_snc
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const int k = G.ord[a];
-    const double u = R.m[0][k] * s[k], v = R.m[1][k] * s[k];
-    it.c[a] = to_fx(s[k] + f0 * u + f0 * v + 1.0) + kSnapFx;  // +1: per-corner coordinates (lo_int = 0)
+    const double sk = pick3(s, k);
+    const double u = pick3(R.m[0], k) * sk, v = pick3(R.m[1], k) * sk;
+    it.c[a] = to_fx(sk + f0 * u + f0 * v + 1.0) + kSnapFx;  // +1: per-corner coordinates (lo_int = 0)
     it.ustep[a] = to_fx(df * v);  // along a row: next column (v)
     it.vstep[a] = to_fx(df * u);  // down a column: next row (u)
   }
@@ -376,8 +389,8 @@ __global__ void pt_build_kernel(const double* __restrict__ poses, int n, fvr_snc
 #pragma unroll
     for (int t = 0; t < 3; ++t) {
       const int k = G.ord[a];
-      J[a * 3 + t] = dR[t].m[1][k] * s[k] * deg;
-      J[9 + a * 3 + t] = dR[t].m[0][k] * s[k] * deg;
+      J[a * 3 + t] = pick3(dR[t].m[1], k) * pick3(s, k) * deg;
+      J[9 + a * 3 + t] = pick3(dR[t].m[0], k) * pick3(s, k) * deg;
     }
 }
 
