@@ -136,11 +136,12 @@ struct Acc {
   float gs[3], gr[3], hs[3], hr[3];
 };
 
-// the spectrum's scale sc (1 for complex64 storage)
+// the spectrum's scale sc (1 for complex64 storage); a scaled layout without a scale
+// buffer is malformed: 0, so that its items contribute nothing
 template <int DT>
 __device__ __forceinline__ float layout_scale(const fvr_layout& L) {
   if constexpr (DT == FVR_C32) return 1.f;
-  else return *reinterpret_cast<gfptr>(reinterpret_cast<gcptr>(L.scales));
+  else return L.scales ? *reinterpret_cast<gfptr>(reinterpret_cast<gcptr>(L.scales)) : 0.f;
 }
 
 // moments accumulated in units of sc -> spectrum units: the sums linear in the slice

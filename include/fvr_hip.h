@@ -77,7 +77,8 @@ typedef enum fvr_dtype {
 #define FVR_LAYOUT_WRAP 1u
 typedef struct fvr_layout {
   uint64_t base;        /* device address of storage element (0,0,0) */
-  uint64_t scales;      /* device address of the float scale sc (one element; 0 for FVR_C32) */
+  uint64_t scales;      /* device address of the float scale sc (one element; 0 for FVR_C32;
+                           a scaled layout with 0 here contributes zero sums) */
   int32_t n[3];         /* logical dims (slow, mid, fast) */
   int32_t dtype;        /* fvr_dtype */
   int32_t sdim[3];      /* storage dims */
