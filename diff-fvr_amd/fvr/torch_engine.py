@@ -75,6 +75,7 @@ class TSliceEvaluator:
         self.spacing0 = float(spacing0)
         self.rows = tile_rows()
         self.ws = _Workspace(packed.device)
+        self._energy = None  # (target tensor, its version, key, sum |T|^2 per target)
 
     def moments(self, poses: np.ndarray, targets: torch.Tensor, view: str, hp_sigma: Optional[float],
                 adjoint: bool):
@@ -142,9 +143,8 @@ class TSliceEvaluator:
         jac = ws.get("t_jac", B * 18, torch.float64)
         partials = ws.get("t_partials", B * ntile * _lib.NMOM, torch.float32)
         ncc = ws.get("t_ncc", B, torch.float64)
-        grad = ws.get("t_grad", 6 * B, torch.float64)
-        t = targets.to(torch.complex128)
-        e_t = (t.real.square() + t.imag.square()).sum(dim=(-2, -1)).reshape(-1).contiguous()
+        grad = torch.empty(6 * B, dtype=torch.float64, device=self.packed.device)  # owned by the caller
+        e_t = self.target_energy(targets)
         L = _lib.lib()
         st = _lib.stream_ptr(self.packed.device)
         D, H, W = self.pad_shape
@@ -160,6 +160,20 @@ class TSliceEvaluator:
                                     _lib.ptr(grad), st), "fvr_pt_combine")
         return ncc[:B], grad[: 6 * B].view(B, 6)
 
+    def target_energy(self, targets: torch.Tensor) -> torch.Tensor:
+        """sum |T|^2 (float64) of each (R, R) target.  Kept for the target tensor last seen
+        (same memory, shape and strides, and no in-place change since: torch's version
+        counter, shared with every view of the tensor); the cache holds the tensor, so its
+        memory cannot be reused by another one while cached."""
+        key = (targets.data_ptr(), tuple(targets.shape), tuple(targets.stride()), targets.dtype, targets.device)
+        c = self._energy
+        if c is not None and c[2] == key and c[1] == targets._version:
+            return c[3]
+        t = targets.to(torch.complex128)
+        e_t = (t.real.square() + t.imag.square()).sum(dim=(-2, -1)).reshape(-1).contiguous()
+        self._energy = (targets, targets._version, key, e_t)
+        return e_t
+
     def device_items(self, B: int) -> np.ndarray:
         raw = self.ws.bufs["t_items"][: B * _lib.ITEM_DTYPE.itemsize].cpu().numpy()
         return raw.view(_lib.ITEM_DTYPE).copy()
@@ -171,7 +185,7 @@ class _FusedSpectralNCC(torch.autograd.Function):
         dev = evaluator.packed.device
         P = poses.detach().to(dev, torch.float64).contiguous()
         ncc, grad = evaluator.device_loss_and_grad(P, targets, view, hp_sigma, adjoint=bool(ctx.needs_input_grad[0]))
-        ctx.grad = grad.clone()
+        ctx.grad = grad  # a fresh tensor per call
         loss = (1.0 - ncc.mean()).to(torch.float32)
         return loss.to(poses.device)
 

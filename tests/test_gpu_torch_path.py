@@ -255,3 +255,21 @@ def test_device_t_setup_matches_host_setup(eng, view):
     nr, dr = combine_t(mom, jac_ref, e_t, _items.t_phase_span(R, ev.spacing0))
     np.testing.assert_allclose(ncc, nr, rtol=1e-9, atol=1e-12)
     np.testing.assert_allclose(grad, -dr / len(poses), rtol=1e-7, atol=1e-12 * np.abs(dr).max())
+
+
+def test_target_energy_cache_follows_in_place_changes(eng):
+    """The fused path keeps sum |T|^2 of the target tensor it last saw: a second call with
+    the same tensor gives the same loss bit for bit, and an in-place change of that tensor
+    is seen (the loss equals that of a fresh copy of the changed tensor, bit for bit)."""
+    e, S3 = eng
+    R = 24
+    ft, _ = _target(e, S3, [1.0, -0.5, 0.5, 0.4, -0.3, 0.0], "ap", R)
+    ft = ft.to(e.device, torch.complex64).contiguous()
+    P = torch.tensor([[3.0, -2.0, 1.0, 1.5, -1.0, 0.3], [-4.0, 2.5, -1.5, -0.8, 1.2, 0.0]],
+                     dtype=torch.float64, device=e.device)
+    l0 = float(e.forward_spectral_ncc(P, ft))
+    assert float(e.forward_spectral_ncc(P, ft)) == l0
+    ft[..., : R // 3, :] = 0  # in place: the version counter moves
+    l1 = float(e.forward_spectral_ncc(P, ft))
+    l2 = float(e.forward_spectral_ncc(P, ft.clone()))
+    assert l1 == l2 and l1 != l0
