@@ -2,9 +2,11 @@
 
 Input validation, target-spectrum preparation (``_prepare_spectrum``,
 core_engine.py:71-85) and the Spectral-NCC biplanar objective
-(core_engine.py:195-205) run on the GPU through the HIP slice kernel.  The
-Powell driver (``scipy.optimize.minimize``) and the metal-masked spatial loss
-are host logic around GPU-rendered slices/DRRs, as in the reference.
+(core_engine.py:195-205) run on the GPU: each Powell evaluation of the clean
+objective is one fused loss-only launch over both views (``BiplanarEvaluator``
+without the adjoint).  The Powell driver (``scipy.optimize.minimize``, one pose
+per call, as in the reference) and the metal-masked spatial loss are host logic
+around GPU-rendered DRRs.
 """
 
 from __future__ import annotations
@@ -17,6 +19,7 @@ import torch
 from scipy.optimize import minimize
 
 from .diff_fvr import DifferentiableFourierVolumeRenderer
+from .sncc import BiplanarEvaluator
 
 
 def detect_radio_opaque_metal(img: np.ndarray, saturation_threshold: float = 0.98) -> np.ndarray:
@@ -120,19 +123,20 @@ class RobustDiffFVREngine:
                 ncc_lat = np.mean((d_lat - d_lat.mean()) / (d_lat.std() + 1e-8) * t_lat)
                 return float(0.5 * (1.0 - ncc_ap) + 0.5 * (1.0 - ncc_lat))
         else:
-            f_ap = self._prepare_spectrum_device(target_ap, "ap")
-            f_lat = self._prepare_spectrum_device(target_lat, "lateral")
+            f_ap = self._prepare_spectrum_device(target_ap, "ap").to(torch.complex64)
+            f_lat = self._prepare_spectrum_device(target_lat, "lateral").to(torch.complex64)
+            # one fused loss-only launch per evaluation: both views' slices, phase ramps and
+            # Spectral-NCC sums (the slices themselves are never stored)
+            ev = BiplanarEvaluator(r.packed, f_ap, f_lat, adjoint=False)
+            pose = np.zeros((1, 6))
 
             def objective(p):
                 nonlocal eval_count
                 eval_count += 1
-                yaw, pitch, roll, tx, ty, tz = p
-                tx, ty, tz = np.array([tx, ty, tz]) / sp
-                s_ap = r.extract_frequency_slice_device(yaw, pitch, roll, tx=tx, ty=tz, view="ap")
-                s_lat = r.extract_frequency_slice_device(yaw, pitch, roll, tx=ty, ty=tz, view="lateral")
-                ncc_ap = r.compute_spectral_ncc(s_ap, f_ap)
-                ncc_lat = r.compute_spectral_ncc(s_lat, f_lat)
-                return float(0.5 * (1.0 - ncc_ap) + 0.5 * (1.0 - ncc_lat))
+                pose[0, :3] = p[:3]
+                pose[0, 3:] = np.asarray(p[3:], dtype=float) / sp  # (tx, ty, tz) mm -> voxels
+                loss, _ = ev.evaluate(pose)
+                return float(loss[0])
 
         res = minimize(objective, x0=np.array(initial_pose), method="Powell", bounds=bounds,
                        options={"maxiter": max_iter, "ftol": 1e-4})
