@@ -383,3 +383,39 @@ def test_robust_engine_registration_clean_and_metal():
     assert res_m["metal_detected"] and m_metal < 2.0, (m_metal, res_m)
     print(f"clean: {res['evaluations']} evals {res['time_sec']:.2f} s mTRE {m_clean:.4f} mm; "
           f"metal: {res_m['evaluations']} evals {res_m['time_sec']:.2f} s mTRE {m_metal:.4f} mm")
+
+
+def test_reference_numerical_gradient_case():
+    """The reference's numerical-gradient case (tests/test_gradient_numerical.py:12-57): a
+    32^3 cube holding a 12^3 unit box, targets at the identity pose, the biplanar loss and
+    gradient at (1, -1, 0.5 deg, 1.5, -1, 2 voxels) against central differences (eps 1e-4)
+    of the same loss.  The differences are taken in float64 with the oracle (the device's
+    float32 loss is too coarse for eps 1e-4): the exact gradient (rotation_grad="analytic")
+    against them, and the default gradient (the reference's scheme: closed-form translations,
+    rotations differenced at +-0.2 deg) against the oracle's restatement of that scheme."""
+    from fvr import DifferentiableFourierVolumeRenderer, SpectralGradientEngine
+    cube = np.zeros((32, 32, 32), dtype=np.float64)
+    cube[10:22, 10:22, 10:22] = 1.0
+    r = DifferentiableFourierVolumeRenderer(cube, pad_factor=2.0)
+    eng = SpectralGradientEngine(r)
+    f3 = O.make_spectrum(cube)
+    T_ap = O.extract_frequency_slice(f3, 0, 0, 0, tx=0, ty=0, view="ap")
+    T_la = O.extract_frequency_slice(f3, 0, 0, 0, tx=0, ty=0, view="lateral")
+    pose = np.array([1.0, -1.0, 0.5, 1.5, -1.0, 2.0])
+
+    def loss64(p):
+        yaw, pitch, roll, tx, ty, tz = p
+        a = O.extract_frequency_slice(f3, yaw, pitch, roll, tx=tx, ty=tz, view="ap")
+        b = O.extract_frequency_slice(f3, yaw, pitch, roll, tx=ty, ty=tz, view="lateral")
+        return 0.5 * (1.0 - O.compute_spectral_ncc(a, T_ap)) + 0.5 * (1.0 - O.compute_spectral_ncc(b, T_la))
+
+    eps = 1e-4
+    g_fd = np.array([(loss64(pose + eps * e) - loss64(pose - eps * e)) / (2 * eps) for e in np.eye(6)])
+    L, g = eng.compute_biplanar_loss_and_gradients(pose, T_ap, T_la, rotation_grad="analytic")
+    assert abs(L - loss64(pose)) <= 1e-6
+    err = np.abs(g - g_fd).max() / np.abs(g_fd).max()
+    assert err <= 2e-3, (g, g_fd, err)
+    L, g = eng.compute_biplanar_loss_and_gradients(pose, T_ap, T_la)
+    Lr, gr = O.biplanar_loss_and_gradients(f3, pose, T_ap, T_la, delta_rot_deg=0.2)
+    assert abs(L - Lr) <= 1e-6
+    assert np.abs(g - gr).max() <= 1e-4 * np.abs(gr).max(), (g, gr)
