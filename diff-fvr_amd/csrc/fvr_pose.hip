@@ -482,6 +482,17 @@ __global__ __launch_bounds__(128) void pn_step_kernel(fvr_sncc_item* items, int 
   }
 }
 
+// loss = 1 - mean(ncc[0 .. n)) as float32: one wave, lane-strided float64 sums in a fixed
+// order, then an XOR butterfly
+__global__ __launch_bounds__(64) void mean_loss_kernel(const double* __restrict__ ncc, int n, float* __restrict__ loss) {
+  const int lane = threadIdx.x;
+  double s = 0.0;
+  for (int k = lane; k < n; k += 64) s += ncc[k];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (lane == 0) *loss = float(1.0 - s / double(n));
+}
+
 }  // namespace fvr
 
 using namespace fvr;
@@ -592,6 +603,13 @@ int fvr_pt_combine(const fvr_sncc_item* items_dev, int32_t n_poses, const float*
   hipLaunchKernelGGL(pt_combine_kernel, dim3(n_poses), dim3(64), 0, reinterpret_cast<hipStream_t>(stream), items_dev,
                      int(n_poses), partials, jac, e_t, int(et_stride), span, ncc, grad);
   FVR_CHECK_LAUNCH("pt_combine_kernel");
+  return FVR_OK;
+}
+
+int fvr_mean_loss(const double* ncc, int32_t n, float* loss, void* stream) {
+  FVR_CHECK_ARG(ncc && loss && n > 0, "fvr_mean_loss: bad arguments");
+  hipLaunchKernelGGL(mean_loss_kernel, dim3(1), dim3(64), 0, reinterpret_cast<hipStream_t>(stream), ncc, int(n), loss);
+  FVR_CHECK_LAUNCH("mean_loss_kernel");
   return FVR_OK;
 }
 

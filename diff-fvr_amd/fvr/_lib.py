@@ -83,6 +83,7 @@ _SIGNATURES = {
     "fvr_pt_build_items": ([_P, _I, _P, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _I, _I,
                             _P, _U64, _U64, _P, _P, _P], _I),
     "fvr_pt_combine": ([_P, _I, _P, _P, _P, _I, ctypes.c_double, _P, _P, _P], _I),
+    "fvr_mean_loss": ([_P, _I, _P, _P], _I),
     "fvr_pn_step": ([_P, _I, _P, _P, ctypes.c_double, ctypes.c_double, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                      _P, ctypes.c_double, ctypes.c_double, ctypes.c_double, _I, _I, _P], _I),
     "fvr_pn_reduce_combine": ([_P, _I, _P, _P, ctypes.c_double, ctypes.c_double, _P, _P, _P, _P], _I),

@@ -186,7 +186,9 @@ class _FusedSpectralNCC(torch.autograd.Function):
         P = poses.detach().to(dev, torch.float64).contiguous()
         ncc, grad = evaluator.device_loss_and_grad(P, targets, view, hp_sigma, adjoint=bool(ctx.needs_input_grad[0]))
         ctx.grad = grad  # a fresh tensor per call
-        loss = (1.0 - ncc.mean()).to(torch.float32)
+        loss = torch.empty((), dtype=torch.float32, device=dev)  # 1 - mean NCC (torch_engine.py:290)
+        _lib.check(_lib.lib().fvr_mean_loss(_lib.ptr(ncc), int(ncc.numel()), _lib.ptr(loss), _lib.stream_ptr(dev)),
+                   "fvr_mean_loss")
         return loss.to(poses.device)
 
     @staticmethod

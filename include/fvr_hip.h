@@ -266,6 +266,9 @@ int fvr_pt_build_items(const double* poses, int32_t n_poses, const fvr_sncc_item
  * span = out_res * spacing + 1e-6. */
 int fvr_pt_combine(const fvr_sncc_item* items_dev, int32_t n_poses, const float* partials, const double* jac,
                    const double* e_t, int32_t et_stride, double span, double* ncc, double* grad, void* stream);
+/* loss[0] = 1 - mean(ncc[0 .. n)) as float32, summed in float64 in a fixed order (one
+ * wave): the batch loss of torch_engine.py:290 (`1.0 - torch.mean(ncc)`) in one launch. */
+int fvr_mean_loss(const double* ncc, int32_t n, float* loss, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Generic fourier_slice_3d (grid_sample semantics, planar (B,2,D,H,W)).      */
