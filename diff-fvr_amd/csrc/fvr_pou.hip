@@ -642,20 +642,29 @@ __global__ __launch_bounds__(256) void pou_ray_grads_kernel(const float* __restr
 }
 
 // per pose: scale = 2^(40 - e) with max |go * ic| = m * 2^e (m in [0.5, 1)), 1 when all zero
-__global__ __launch_bounds__(256) void pou_gscale_kernel(const float* __restrict__ go, const float* __restrict__ ic,
-                                                         long long ray_stride, int K, float* __restrict__ scale) {
-  __shared__ float red[256];
+// (a maximum is exact in any order: 1024 threads, four independent loads in flight each)
+__global__ __launch_bounds__(1024) void pou_gscale_kernel(const float* __restrict__ go, const float* __restrict__ ic,
+                                                          long long ray_stride, int K, float* __restrict__ scale) {
+  __shared__ float red[16];
   const int n = blockIdx.x;
-  float m = 0.f;
-  for (int q = threadIdx.x; q < K; q += 256) m = fmaxf(m, fabsf(go[size_t(n) * K + q] * ic[size_t(n) * ray_stride + q]));
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + st]);
-    __syncthreads();
+  const float* g = go + size_t(n) * K;
+  const float* c = ic + size_t(n) * ray_stride;
+  float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f;
+  int q = threadIdx.x;
+  for (; q + 3 * 1024 < K; q += 4 * 1024) {
+    m0 = fmaxf(m0, fabsf(g[q] * c[q]));
+    m1 = fmaxf(m1, fabsf(g[q + 1024] * c[q + 1024]));
+    m2 = fmaxf(m2, fabsf(g[q + 2048] * c[q + 2048]));
+    m3 = fmaxf(m3, fabsf(g[q + 3072] * c[q + 3072]));
   }
+  for (; q < K; q += 1024) m0 = fmaxf(m0, fabsf(g[q] * c[q]));
+  float m = fmaxf(fmaxf(m0, m1), fmaxf(m2, m3));
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
   if (threadIdx.x == 0) {
-    const float mx = red[0];
+    float mx = red[0];
+    for (int w = 1; w < 16; ++w) mx = fmaxf(mx, red[w]);
     int e = 0;
     if (mx > 0.f && isfinite(mx)) frexpf(mx, &e);
     scale[n] = mx > 0.f && isfinite(mx) ? ldexpf(1.f, 40 - e) : 1.f;
@@ -795,7 +804,7 @@ int fvr_pou_warp_fwd(const float* imgs, int32_t B, int32_t N, int32_t P, const f
 int fvr_pou_gscale(const float* go, const float* ic, int32_t N, int32_t K, int32_t per_pose_rays, float* scale,
                    void* stream) {
   FVR_CHECK_ARG(go && ic && scale && N > 0 && K > 0, "fvr_pou_gscale: bad arguments");
-  hipLaunchKernelGGL(pou_gscale_kernel, dim3(N), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), go, ic,
+  hipLaunchKernelGGL(pou_gscale_kernel, dim3(N), dim3(1024), 0, reinterpret_cast<hipStream_t>(stream), go, ic,
                      per_pose_rays ? (long long)K : 0ll, K, scale);
   FVR_CHECK_LAUNCH("pou_gscale_kernel");
   return FVR_OK;
