@@ -5,7 +5,7 @@
 //   fvr_pou_slice_ifft   one workgroup per (block, pose): the P x P central slice
 //                        of the block spectrum (per-corner trilinear, packed
 //                        storage) is gathered straight into LDS,
-//                        inverse-transformed there (2D radix-2, unnormalised,
+//                        inverse-transformed there (2D, radix-4 passes, unnormalised,
 //                        centred: fftshift(ifft2(ifftshift(.)))) and its real part
 //                        / P^2 written as the slab image (B, N, P, P) float.
 //   fvr_pou_warp_fwd     detector pixel (n, q): sum_b bilinear(img_bn, p_bn(q)) / cos,
@@ -138,51 +138,15 @@ __device__ __forceinline__ int brev(int x, int logp) {
   return int(__builtin_bitreverse32(uint32_t(x)) >> (32 - logp));
 }
 
-// 2D radix-2 decimation-in-frequency FFT of a P x P complex array in LDS (row
-// pitch P + 1): natural-order input, bit-reversed output in both dimensions,
-// unnormalised; SIGN = +1 inverse, -1 forward.  tw[k] = exp(-2 pi i k / P).
-template <int SIGN>
-__device__ void fft2d_dif(float2* a, int P, int logP, int pitch, const float2* tw) {
-  const int nb = P * P / 2;
-  const int hp = P / 2;
-#pragma unroll 1
-  for (int dim = 0; dim < 2; ++dim) {
-#pragma unroll 1
-    for (int s = logP; s >= 1; --s) {
-      const int half = 1 << (s - 1);
-      const int tstep = P >> s;
-      for (int idx = threadIdx.x; idx < nb; idx += blockDim.x) {
-        int line, k;
-        if (dim == 0) {  // along rows: consecutive threads walk one row
-          line = idx >> (logP - 1);
-          k = idx & (hp - 1);
-        } else {  // along columns: consecutive threads take consecutive columns
-          line = idx & (P - 1);
-          k = idx >> logP;
-        }
-        const int pos = k & (half - 1);
-        const int i0 = ((k >> (s - 1)) << s) + pos;
-        const int i1 = i0 + half;
-        const int a0 = dim == 0 ? line * pitch + i0 : i0 * pitch + line;
-        const int a1 = dim == 0 ? line * pitch + i1 : i1 * pitch + line;
-        float2 w = tw[pos * tstep];
-        if (SIGN > 0) w.y = -w.y;
-        const float2 x0 = a[a0], x1 = a[a1];
-        const float dr = x0.x - x1.x, di = x0.y - x1.y;
-        a[a0] = make_float2(x0.x + x1.x, x0.y + x1.y);
-        a[a1] = make_float2(dr * w.x - di * w.y, dr * w.y + di * w.x);
-      }
-      __syncthreads();
-    }
-  }
-}
-
-// The same transform with pairs of radix-2 stages fused into radix-4 passes: stages s and
-// s - 1 of a block of 2^s elements act on the four elements k, k + q, k + 2q, k + 3q
-// (q = 2^(s-2)) -- stage s pairs (k, k + 2q) and (k + q, k + 3q) with twiddles k ts and
-// (k + q) ts (ts = P >> s), stage s - 1 pairs the results within each half with twiddle
-// 2k ts -- so each element is read and written once per two stages (an odd logP ends with one
-// radix-2 pass).  Same arithmetic as fft2d_dif, half the LDS passes and barriers.
+// 2D decimation-in-frequency FFT of a P x P complex array in LDS (row pitch P + 1):
+// natural-order input, bit-reversed output in both dimensions, unnormalised; SIGN = +1
+// inverse, -1 forward; tw[k] = exp(-2 pi i k / P).  Pairs of radix-2 stages run as radix-4
+// passes: stages s and s - 1 of a block of 2^s elements act on the four elements k, k + q,
+// k + 2q, k + 3q (q = 2^(s-2)) -- stage s pairs (k, k + 2q) and (k + q, k + 3q) with
+// twiddles k ts and (k + q) ts (ts = P >> s), stage s - 1 pairs the results within each half
+// with twiddle 2k ts -- so each element is read and written once per two stages (an odd logP
+// ends with one radix-2 pass): the arithmetic of the radix-2 stages with half the LDS passes
+// and barriers.
 template <int SIGN>
 __device__ void fft2d_dif4(float2* a, int P, int logP, int pitch, const float2* tw) {
   const int hp = P / 2;
