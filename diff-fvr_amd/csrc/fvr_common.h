@@ -41,8 +41,6 @@ __host__ __device__ __forceinline__ bool sncc_tile_h_ok(int tile_h) {
 
 // Tile geometry: a tile is 64 columns x tile_h rows; a Hermitian-fold tile is 32 columns
 // x 2 tile_h rows (the same 4096 samples at tile_h 64) and the fold covers h/2 rows.
-__host__ __device__ __forceinline__ int sncc_tile_w(uint32_t flags) { return (flags & FVR_ITEM_HERM) ? 32 : 64; }
-
 // tile rows of one item; 0 for a bad tile_h
 __host__ __device__ __forceinline__ int sncc_tiles_y(int h, uint32_t flags, int tile_h) {
   if (!sncc_tile_h_ok(tile_h)) return 0;
