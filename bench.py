@@ -69,15 +69,18 @@ def make_targets(renderer, true_pose):
 
 
 KERNEL_EVENT_EVERY = 8
+KERNEL_SAMPLES = 64
 
 
 def time_batched(ev, P, steps: int, warmup: int):
     """Wall time per step -- the fused kernel over the batch's items, then fvr_pn_step
     (per-pose reduction + combine -> loss and gradient, and the float64 set-up of the
     items for the next step); poses resident in HBM, loss and gradient left in HBM --
-    and the fused kernel's mean duration from HIP events on its launch stream, recorded
-    around the kernel of every KERNEL_EVENT_EVERY-th step of the timed region (an event
-    pair on every step adds about 6 us to a 77 us step on MI355X)."""
+    and the fused kernel's mean duration from HIP events on its launch stream.  The
+    kernel duration comes from KERNEL_SAMPLES further steps after the timed region, each
+    with an event pair around its fused kernel (an event pair on every step adds about
+    6 us to a 77 us step on MI355X, so the timed region itself carries a pair only on
+    every KERNEL_EVENT_EVERY-th step; their mean is reported beside it)."""
     import torch
     ev.begin(P)
     for _ in range(warmup):
@@ -90,8 +93,13 @@ def time_batched(ev, P, steps: int, warmup: int):
         ev.step(P, kernel_events=ks.get(k))
     torch.cuda.synchronize()
     total_ms = (time.perf_counter() - t0) * 1e3
-    kern_ms = [a.elapsed_time(b) for a, b in ks.values()]
-    return total_ms / steps, statistics.mean(kern_ms), kern_ms
+    in_region = [a.elapsed_time(b) for a, b in ks.values()]
+    es = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(KERNEL_SAMPLES)]
+    for e in es:
+        ev.step(P, kernel_events=e)
+    torch.cuda.synchronize()
+    kern_ms = [a.elapsed_time(b) for a, b in es]
+    return total_ms / steps, statistics.mean(kern_ms), kern_ms, statistics.mean(in_region)
 
 
 def time_three(ev, P, steps: int, warmup: int):
@@ -335,27 +343,108 @@ def multistart_config5(n_steps: int = 100, lat_reps: int = 100):
             "best_start_mtre_mm": mtre, "build_s": round(build_s, 2)}
 
 
-def cpu_baseline(renderer, T_ap, T_lat, cands, budget_s: float):
-    """The NumPy/SciPy oracle (reference algorithm: 14 map_coordinates slices per
-    pose eval) on one host core, on a bounded sample of the candidate poses."""
+_POOL_SPEC = None
+
+
+def _pool_init(path: str, ta, tl) -> None:
+    """Worker set-up of the all-cores CPU baseline: map the spectrum read-only."""
+    global _POOL_SPEC
+    import numpy as np
+    _POOL_SPEC = (np.load(path, mmap_mode="r"), ta, tl)
+
+
+def _pool_eval(pose):
+    from oracle import diff_fvr_np as O
+    f3, ta, tl = _POOL_SPEC
+    t = time.perf_counter()
+    O.biplanar_loss_and_gradients(f3, pose, ta, tl)
+    return time.perf_counter() - t
+
+
+def host_cores() -> int:
+    """Worker processes for the all-cores CPU baseline: the cores this process may run on,
+    at most 16 (a GPU box's host share per GPU)."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def cpu_baseline(renderer, T_ap, T_lat, cands, budget_s: float, gpu=None):
+    """The NumPy/SciPy oracle (reference algorithm: 14 map_coordinates slices per pose
+    eval) on bounded samples of the candidate poses: (a) one host core, in this process;
+    (b) a process pool over independent candidates on every host core this process may
+    use (the spectrum shared read-only through a memory-mapped file).  ``gpu``: {storage:
+    (loss (N,), grad (N, 6))} of the GPU launch -- the sampled oracle losses, and the exact
+    oracle gradient of two of them, are compared with it (max |dloss|, max |dgrad| / max|grad|)."""
+    import multiprocessing as mp
+    import tempfile
+
     import numpy as np
     from oracle import diff_fvr_np as O
     f3 = renderer.f_3d
     ta, tl = T_ap.cpu().numpy().astype(np.complex128), T_lat.cpu().numpy().astype(np.complex128)
-    times = []
+    spec_shape = "x".join(str(n) for n in f3.shape)
+    times, ref = [], {}
     t_start = time.perf_counter()
     k = 0
     while k < len(cands) and (k < 1 or time.perf_counter() - t_start < budget_s):
         t = time.perf_counter()
-        O.biplanar_loss_and_gradients(f3, cands[k], ta, tl)
+        ref[k] = O.biplanar_loss_and_gradients(f3, cands[k], ta, tl)
         times.append(time.perf_counter() - t)
         k += 1
-        log(f"cpu baseline: pose {k} took {times[-1]:.2f} s")
-    del f3
+        log(f"cpu baseline (1 core): pose {k} took {times[-1]:.2f} s")
     med = statistics.median(times)
-    return {"value": 1.0 / med, "unit": "pose evals/s", "cores": 1, "kind": "port",
-            "sample": f"{k} of the {len(cands)} candidate poses at 256^3 (512^3 complex128 spectrum), reference "
-                      f"FD scheme (14 scipy map_coordinates slices per pose eval), median of {k} evals, 1 core"}
+    parity = {}
+    if gpu:
+        exact = {j: O.biplanar_loss_and_grad_analytic(f3, cands[j], ta, tl, fold=True) for j in (0, k - 1)}
+        for st, (L, G) in gpu.items():
+            dl = max(abs(float(L[j]) - ref[j][0]) for j in ref)
+            dg = max(float(np.abs(G[j] - g).max() / np.abs(g).max()) for j, (_, g) in exact.items())
+            parity[st] = {"max_abs_dloss": dl, "max_rel_dgrad": dg}
+        parity["candidates_loss"] = sorted(ref)
+        parity["candidates_grad"] = sorted(exact)
+        parity["oracle"] = ("loss: the oracle's reference scheme; gradient: the oracle's exact adjoint "
+                            "(biplanar_loss_and_grad_analytic), both FP64 on the same spectrum and targets")
+        log(f"oracle parity at full size: {parity}")
+    # (b) every host core: a process pool over candidate poses
+    n_work = host_cores()
+    tmp = tempfile.mkdtemp(dir="/dev/shm" if os.path.isdir("/dev/shm") else None)
+    path = os.path.join(tmp, "spectrum.npy")
+    pool_value, pool_sample = None, None
+    try:
+        np.save(path, f3)
+        del f3
+        old = os.environ.get("OMP_NUM_THREADS")
+        os.environ["OMP_NUM_THREADS"] = "1"
+        try:
+            ctx = mp.get_context("spawn")
+            with ctx.Pool(n_work, initializer=_pool_init, initargs=(path, ta, tl)) as pool:
+                pool.map(_pool_eval, [cands[j % len(cands)] for j in range(n_work)], chunksize=1)  # warm-up
+                n_eval = 0
+                t0 = time.perf_counter()
+                while n_eval == 0 or time.perf_counter() - t0 < budget_s:
+                    batch = [cands[(n_eval + j) % len(cands)] for j in range(n_work)]
+                    pool.map(_pool_eval, batch, chunksize=1)
+                    n_eval += len(batch)
+                wall = time.perf_counter() - t0
+            pool_value = n_eval / wall
+            pool_sample = f"{n_eval} pose evals in {wall:.1f} s of wall time on {n_work} worker processes"
+            log(f"cpu baseline ({n_work} cores): {pool_value:.2f} pose evals/s")
+        finally:
+            if old is None:
+                os.environ.pop("OMP_NUM_THREADS", None)
+            else:
+                os.environ["OMP_NUM_THREADS"] = old
+    finally:
+        if os.path.exists(path):
+            os.remove(path)
+        os.rmdir(tmp)
+    one = {"value": 1.0 / med, "cores": 1,
+           "sample": f"{k} of the {len(cands)} candidate poses, median of {k} evals"}
+    base = (f"candidate poses on the {spec_shape} complex128 spectrum, reference FD scheme (14 scipy "
+            "map_coordinates slices per pose eval)")
+    out = {"value": pool_value, "unit": "pose evals/s", "cores": n_work, "kind": "port",
+           "sample": f"{pool_sample}; {base}", "one_core": one, "host_cpu_count": os.cpu_count(),
+           "host_cores_available": len(os.sched_getaffinity(0))}
+    return out, parity
 
 
 def main(argv=None) -> int:
@@ -412,7 +501,7 @@ def main(argv=None) -> int:
             th, order, persist, *depth = args.plan.split("/")
             plan = LaunchPlan(int(th), order, int(persist), int(depth[0]) if depth else 2)
             ev.set_plan(plan)
-        step_ms, kern_ms, _ = time_batched(ev, P, args.steps, args.warmup)
+        step_ms, kern_ms, kern_all, kern_region = time_batched(ev, P, args.steps, args.warmup)
         graph_ms = time_graph(ev, P, args.steps, args.warmup) if not args.no_graph else None
         three_ms = time_three(ev, P, args.steps, args.warmup)
         loss, grad = ev.evaluate(cands)
@@ -422,7 +511,9 @@ def main(argv=None) -> int:
         for _ in range(10):
             ev.evaluate(cands)
         host_ms = (time.perf_counter() - t_h) * 1e3 / 10
-        results[st] = {"ms_per_step": step_ms, "kernel_ms": kern_ms, "value": args.n_cand / (step_ms * 1e-3),
+        results[st] = {"ms_per_step": step_ms, "kernel_ms": kern_ms, "kernel_ms_samples": len(kern_all),
+                       "kernel_ms_p50": statistics.median(kern_all), "kernel_ms_min": min(kern_all),
+                       "kernel_ms_in_timed_region": kern_region, "value": args.n_cand / (step_ms * 1e-3),
                        "loss_mean": float(loss.mean()), "loss": loss, "grad": grad, "herm": ev.herm,
                        "plan": plan, "autotune": timings, "host_numpy_ms_per_step": host_ms,
                        "three_launch_ms_per_step": three_ms, "graph_ms_per_step": graph_ms}
@@ -456,10 +547,11 @@ def main(argv=None) -> int:
         traffic = {"bytes": pmc["traffic_bytes_per_launch"], "l2_hit_rate": pmc["l2_hit_rate"],
                    "source": os.path.relpath(pmc_path, ROOT), "pmc_avg_duration_us": pmc["avg_duration_us"]}
     cpu = None
-    if not args.no_cpu:
-        log("CPU baseline (oracle port, one core)")
-        cpu = cpu_baseline(r, T_ap, T_lat, cands, args.cpu_seconds)
     extra = {}
+    if not args.no_cpu:
+        log("CPU baseline (oracle port: one core, then every host core) + oracle parity of the sampled poses")
+        cpu, extra["oracle_parity"] = cpu_baseline(r, T_ap, T_lat, cands, args.cpu_seconds,
+                                                   {st: (v["loss"], v["grad"]) for st, v in results.items()})
     if "f32" in results and args.storage != "f32":
         extra["f32_batched_value"] = results["f32"]["value"]
         extra["f32_batched_ms_per_step"] = results["f32"]["ms_per_step"]
@@ -516,6 +608,9 @@ def main(argv=None) -> int:
                      "frac": None if achieved is None else round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None if traffic is None else traffic["bytes"], "traffic_detail": traffic,
                      "kernel": kname, "kernel_ms": round(head["kernel_ms"], 4),
+                     "kernel_ms_samples": head["kernel_ms_samples"], "kernel_ms_p50": round(head["kernel_ms_p50"], 4),
+                     "kernel_ms_min": round(head["kernel_ms_min"], 4),
+                     "kernel_ms_in_timed_region": round(head["kernel_ms_in_timed_region"], 4),
                      "algorithmic_bytes_per_launch": fp["bytes"], "unique_entries": fp["unique_entries"],
                      "schedule": fp.get("schedule"), "samples_per_launch": fp.get("samples")},
         "cpu_baseline": cpu,

@@ -410,6 +410,9 @@ __device__ __forceinline__ void warp_adjoint_scan(const float* img, const float*
                                                   int acc_pitch, bool roll, float (&s)[3]) {
   const int P = g.P, hp = P / 2, K = g.K, stride = blockDim.x;
   const int pp1 = P * P - 1;
+  // a NaN scale marks a pose whose output gradient holds NaN or inf: no fixed-point scatter
+  // (the conversion back multiplies by 1 / NaN, so that pose's image gradient is NaN)
+  const bool scatter = IMG && scale == scale;
   for (int q0 = threadIdx.x; q0 < K; q0 += kScanBatch * stride) {
     // 1) the pixels' rays and output gradients
     float r_x[kScanBatch], r_y[kScanBatch], gg[kScanBatch];
@@ -451,7 +454,7 @@ __device__ __forceinline__ void warp_adjoint_scan(const float* img, const float*
       s[0] = fmaf(gg[t], dsx * r_x[t] + dsy * r_y[t], s[0]);
       s[1] = fmaf(-gg[t], dsx, s[1]);
       s[2] = fmaf(-gg[t], dsy, s[2]);
-      if (IMG && gg[t] != 0.f) {
+      if (scatter && gg[t] != 0.f) {
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
           const int x = x0[t] + (c & 1), y = y0[t] + (c >> 1);
@@ -605,8 +608,23 @@ __global__ __launch_bounds__(256) void pou_ray_grads_kernel(const float* __restr
   gic[oq] = o * acc;
 }
 
-// per pose: scale = 2^(40 - e) with max |go * ic| = m * 2^e (m in [0.5, 1)), 1 when all zero
-// (a maximum is exact in any order: 1024 threads, four independent loads in flight each)
+// the fixed-point scale of a pose from its maximum |go * ic| (host and device: one function)
+__host__ __device__ inline float pou_scale_of_max(float mx) {
+  if (!(mx <= 3.40282347e38f)) return __builtin_nanf("");  // NaN or +inf in the output gradient
+  if (!(mx > 0.f)) return 1.f;
+  int e = 0;
+  frexpf(mx, &e);
+  const int k = 40 - e;
+  return ldexpf(1.f, k < 126 ? k : 126);
+}
+
+// max of |x| that propagates NaN (fmaxf drops it): once NaN, the result stays NaN
+__device__ __forceinline__ float nan_max(float m, float x) { return (x > m || x != x) ? x : m; }
+
+// per pose: scale = 2^min(40 - e, 126) with max |go * ic| = m * 2^e (m in [0.5, 1)), 1 when all
+// zero, NaN when any go * ic is NaN or infinite (a maximum is exact in any order: 1024 threads,
+// four independent loads in flight each).  The cap keeps the scale a finite power of two for
+// tiny maxima (below 2^-86 the fixed-point values simply use fewer than 40 bits).
 __global__ __launch_bounds__(1024) void pou_gscale_kernel(const float* __restrict__ go, const float* __restrict__ ic,
                                                           long long ray_stride, int K, float* __restrict__ scale) {
   __shared__ float red[16];
@@ -616,22 +634,20 @@ __global__ __launch_bounds__(1024) void pou_gscale_kernel(const float* __restric
   float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f;
   int q = threadIdx.x;
   for (; q + 3 * 1024 < K; q += 4 * 1024) {
-    m0 = fmaxf(m0, fabsf(g[q] * c[q]));
-    m1 = fmaxf(m1, fabsf(g[q + 1024] * c[q + 1024]));
-    m2 = fmaxf(m2, fabsf(g[q + 2048] * c[q + 2048]));
-    m3 = fmaxf(m3, fabsf(g[q + 3072] * c[q + 3072]));
+    m0 = nan_max(m0, fabsf(g[q] * c[q]));
+    m1 = nan_max(m1, fabsf(g[q + 1024] * c[q + 1024]));
+    m2 = nan_max(m2, fabsf(g[q + 2048] * c[q + 2048]));
+    m3 = nan_max(m3, fabsf(g[q + 3072] * c[q + 3072]));
   }
-  for (; q < K; q += 1024) m0 = fmaxf(m0, fabsf(g[q] * c[q]));
-  float m = fmaxf(fmaxf(m0, m1), fmaxf(m2, m3));
-  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+  for (; q < K; q += 1024) m0 = nan_max(m0, fabsf(g[q] * c[q]));
+  float m = nan_max(nan_max(m0, m1), nan_max(m2, m3));
+  for (int o = 32; o > 0; o >>= 1) m = nan_max(m, __shfl_xor(m, o));
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
     float mx = red[0];
-    for (int w = 1; w < 16; ++w) mx = fmaxf(mx, red[w]);
-    int e = 0;
-    if (mx > 0.f && isfinite(mx)) frexpf(mx, &e);
-    scale[n] = mx > 0.f && isfinite(mx) ? ldexpf(1.f, 40 - e) : 1.f;
+    for (int w = 1; w < 16; ++w) mx = nan_max(mx, red[w]);
+    scale[n] = pou_scale_of_max(mx);
   }
 }
 
@@ -773,6 +789,8 @@ int fvr_pou_gscale(const float* go, const float* ic, int32_t N, int32_t K, int32
   FVR_CHECK_LAUNCH("pou_gscale_kernel");
   return FVR_OK;
 }
+
+float fvr_pou_scale_of_max(float max_abs) { return pou_scale_of_max(max_abs); }
 
 int64_t fvr_pou_warp_bwd_workspace(int32_t B, int32_t N, int32_t P) { return int64_t(B) * N * P * P * 8; }
 

@@ -97,6 +97,7 @@ _SIGNATURES = {
     "fvr_pou_slice_ifft": ([_U64, _I, _I, _I, _P, _P, _I, _P, _P], _I),
     "fvr_pou_warp_fwd": ([_P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _I, _I, _F, _F, _P, _P], _I),
     "fvr_pou_gscale": ([_P, _P, _I, _I, _I, _P, _P], _I),
+    "fvr_pou_scale_of_max": ([_F], _F),
     "fvr_pou_warp_bwd_workspace": ([_I, _I, _I], ctypes.c_int64),
     "fvr_pou_warp_bwd": ([_P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _I, _I, _F, _F, _P, _P, _P, _P, _P], _I),
     "fvr_pou_render_bwd": ([_U64, _I, _I, _I, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _F, _F, _P, _P,

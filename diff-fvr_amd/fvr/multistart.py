@@ -87,7 +87,10 @@ class AdamLoop:
     ``run(init_poses)`` returns (best_pose (N, 6), best_loss (N,), last_pose (N, 6))
     as device tensors.  With ``graph=True`` the loop is captured on the first
     run (after one eager warm-up step that sizes every buffer) and replayed on
-    later runs; the graph keeps the evaluator's targets, plan and pose count.
+    later runs.  The graph holds the evaluator's buffers by address and its target
+    energies by value: whenever the evaluator's ``state_key()`` has changed since
+    the capture (a new launch plan or tile map, new targets, a workspace that grew
+    for a larger batch), ``run`` captures the loop again instead of replaying.
     """
 
     def __init__(self, evaluator, n: int, steps: int, graph: bool = True, **adam_kw):
@@ -99,6 +102,9 @@ class AdamLoop:
         self.poses = torch.zeros((self.n, 6), dtype=torch.float64, device=self.device)
         self.adam = DeviceAdam(self.n, self.device, **adam_kw)
         self.graph: Optional[torch.cuda.CUDAGraph] = None
+        self.captured_key: Optional[Tuple] = None
+        self.captures = 0
+        self._held: List[torch.Tensor] = []  # device buffers the current graph reads or writes
         if isinstance(evaluator, BiplanarEvaluator):
             evaluator.prepare(self.n)
             evaluator.launch_map(self.n)
@@ -110,22 +116,44 @@ class AdamLoop:
         for _ in range(steps):
             self.ev.step(self.poses, self.adam)
 
+    def _buffers(self) -> List[torch.Tensor]:
+        ev = self.ev
+        held = list(ev.ws.bufs.values())
+        if isinstance(ev, RaggedBiplanarEvaluator):
+            held += [] if ev.map is None else [ev.map]
+            for s in ev.segments:
+                held += list(s.folded or s.targets)
+        else:
+            held += [m for m, _, _ in ev._maps.values() if m is not None]
+            held += list(ev.folded or ev.targets)
+        return held
+
     def capture(self) -> None:
         """Warm up one eager step, then capture the whole loop into a graph."""
+        self.graph = None
+        if isinstance(self.ev, BiplanarEvaluator):
+            self.ev.prepare(self.n)
+            self.ev.launch_map(self.n)
         self.adam.reset(self.poses)
         self._body(1)
         torch.cuda.synchronize(self.device)
+        key = self.ev.state_key()
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
             self._body(self.steps)
         torch.cuda.synchronize(self.device)
+        if self.ev.state_key() != key:
+            raise RuntimeError("AdamLoop: the evaluator changed its buffers during capture")
         self.graph = g
+        self.captured_key = key
+        self._held = self._buffers()
+        self.captures += 1
 
     def run(self, init_poses: torch.Tensor):
         _lib.require_device(init_poses, "AdamLoop.run")
         if tuple(init_poses.shape) != (self.n, 6):
             raise ValueError(f"init_poses must be ({self.n}, 6)")
-        if self.use_graph and self.graph is None:
+        if self.use_graph and (self.graph is None or self.ev.state_key() != self.captured_key):
             self.capture()
         self.poses.copy_(init_poses.to(torch.float64))
         self.adam.reset(self.poses)

@@ -96,17 +96,21 @@ def item_major_entries(n_poses: int, tiles: Sequence[Tuple[int, int]], item0: in
 
 
 class _Workspace:
-    """Grow-only device buffers keyed by name (no allocation once sized)."""
+    """Grow-only device buffers keyed by name (no allocation once sized).  ``version``
+    counts (re)allocations: a HIP graph captured over these buffers is valid only while
+    it is unchanged."""
 
     def __init__(self, device):
         self.device = device
         self.bufs: Dict[str, torch.Tensor] = {}
+        self.version = 0
 
     def get(self, name: str, numel: int, dtype) -> torch.Tensor:
         t = self.bufs.get(name)
         if t is None or t.numel() < numel or t.dtype != dtype:
             t = torch.empty(max(numel, 1), dtype=dtype, device=self.device)
             self.bufs[name] = t
+            self.version += 1
         return t
 
 
@@ -237,7 +241,10 @@ class BiplanarEvaluator:
         self.set_plan(plan or LaunchPlan())
 
     def set_plan(self, plan: LaunchPlan) -> None:
-        """Choose tile height / order / persistence (rebuilds the item templates)."""
+        """Choose tile height / order / persistence (rebuilds the item templates; a no-op
+        for the plan already set)."""
+        if getattr(self, "plan", None) == plan:
+            return
         if plan.order not in ("xcd", "target") or plan.persist < 0:
             raise ValueError(f"bad launch plan {plan}")
         if plan.exact_box and not self.flags & _lib.ITEM_HERM:
@@ -259,6 +266,12 @@ class BiplanarEvaluator:
         ta, tl = self.view_tiles
         self.tiles_per_pose = ta[0] * ta[1] + tl[0] * tl[1]
         self._maps: Dict[int, Tuple[Optional[torch.Tensor], int, int]] = {}
+        self.plan_gen = getattr(self, "plan_gen", 0) + 1
+
+    def state_key(self) -> Tuple:
+        """Changes whenever something a captured HIP graph holds by address or by value
+        changes: the plan (templates, tile maps), the targets' energies, the workspace."""
+        return (self.plan_gen, self.target_gen, self.ws.version)
 
     def set_targets(self, target_ap: torch.Tensor, target_lat: torch.Tensor) -> None:
         """Copy new AP / LAT target spectra into the evaluator's buffers (a captured graph
@@ -271,6 +284,7 @@ class BiplanarEvaluator:
             self.t2[k] = target_energy(self.targets[k])
             if self.folded is not None:
                 self.folded[k].copy_(fold_target(self.targets[k]))
+        self.target_gen = getattr(self, "target_gen", 0) + 1
 
     # -- device path ---------------------------------------------------------
     def prepare(self, n: int):
@@ -468,9 +482,14 @@ class RaggedBiplanarEvaluator:
         self.set_plan(plan or s0.plan)
 
     def set_plan(self, plan: LaunchPlan) -> None:
+        """Set the plan of every segment and rebuild the combined tile map (a no-op for the
+        plan already set)."""
+        if getattr(self, "plan", None) == plan:
+            return
         for s in self.segments:
             s.set_plan(plan)
         self.plan = plan
+        self.plan_gen = getattr(self, "plan_gen", 0) + 1
         self.rows = plan.tile_h
         self.templates = []
         self.tile0 = []
@@ -567,6 +586,11 @@ class RaggedBiplanarEvaluator:
             _pn_step(items, self.counts[k], partials, jac, s.t2, None, loss, grad, poses, adam, self.templates[k], pz,
                      py, px, s.lat_swap, self.tile0[k], st, pose0=self.offsets[k])
         return loss[: self.n], grad[: 6 * self.n].view(self.n, 6)
+
+    def state_key(self) -> Tuple:
+        """Changes whenever something a captured HIP graph holds changes (the plan, the tile
+        map, any segment's targets, the combined workspace)."""
+        return (self.plan_gen, self.ws.version, tuple(s.target_gen for s in self.segments))
 
     def _check_poses(self, poses: torch.Tensor) -> None:
         _lib.require_device(poses, "RaggedBiplanarEvaluator")

@@ -66,6 +66,38 @@ def combine_t(mom: np.ndarray, jac: np.ndarray, e_t: np.ndarray, span: float):
     return ncc, d
 
 
+def check_t_targets(targets, B: int, device) -> int:
+    """Validate the targets of a fused convention-T launch before any address is formed from
+    them: a complex64 tensor on ``device`` of shape (Bt, R, R) with Bt in {1, B} and rows of
+    each target contiguous (stride(-1) == 1, stride(-2) == R); returns R.  The kernel reads
+    R * R * 8 bytes per target from ``data_ptr() + k * stride(0)``, so anything else would
+    read outside the tensor."""
+    if not isinstance(targets, torch.Tensor):
+        raise ValueError("targets must be a torch tensor")
+    if targets.dtype != torch.complex64:
+        raise ValueError(f"targets must be complex64, got {targets.dtype}")
+    if targets.device != torch.device(device):
+        raise ValueError(f"targets on {targets.device}, expected {device}")
+    if targets.dim() != 3 or targets.shape[-1] != targets.shape[-2] or targets.shape[-1] < 2:
+        raise ValueError(f"targets must be (Bt, R, R), got {tuple(targets.shape)}")
+    if int(targets.shape[0]) not in (1, int(B)):
+        raise ValueError(f"targets hold {targets.shape[0]} spectra for {B} poses (expected 1 or {B})")
+    R = int(targets.shape[-1])
+    if targets.stride(-1) != 1 or targets.stride(-2) != R or (targets.shape[0] > 1 and targets.stride(0) < R * R):
+        raise ValueError(f"targets must have contiguous (R, R) rows, got strides {targets.stride()}")
+    return R
+
+
+def check_t_poses(poses: torch.Tensor, device) -> int:
+    """(B, 6) contiguous float64 poses on ``device``; returns B."""
+    if (not isinstance(poses, torch.Tensor) or poses.dtype != torch.float64 or poses.dim() != 2
+            or poses.shape[1] != 6 or not poses.is_contiguous() or poses.shape[0] < 1):
+        raise ValueError("poses must be a non-empty contiguous (B, 6) float64 tensor")
+    if poses.device != torch.device(device):
+        raise ValueError(f"poses on {poses.device}, expected {device}")
+    return int(poses.shape[0])
+
+
 class TSliceEvaluator:
     """Fused convention-T Spectral-NCC (+ exact adjoint) over a batch of poses for one view."""
 
@@ -82,7 +114,7 @@ class TSliceEvaluator:
         """(B, 16) float64 moments and (B, 2, 3, 3) Jacobians; ``targets`` (Bt, R, R) complex64, Bt in {1, B}."""
         poses = np.asarray(poses, dtype=np.float64).reshape(-1, 6)
         B = len(poses)
-        R = int(targets.shape[-1])
+        R = check_t_targets(targets, B, self.packed.device)
         flags = _lib.ITEM_ADJOINT if adjoint else 0
         tmpl = _items.t_template(self.packed, view, R, targets.data_ptr(), flags, self.rows, hp_sigma)
         stride = targets.stride(0) * targets.element_size() if targets.shape[0] > 1 else 0
@@ -129,8 +161,8 @@ class TSliceEvaluator:
         poses: (B, 6) float64 device tensor; targets: (Bt, R, R) complex64 device tensor,
         Bt in {1, B}.  Three stream-ordered launches: fvr_pt_build_items, the fused
         kernel, fvr_pt_combine."""
-        B = int(poses.shape[0])
-        R = int(targets.shape[-1])
+        B = check_t_poses(poses, self.packed.device)
+        R = check_t_targets(targets, B, self.packed.device)
         flags = _lib.ITEM_ADJOINT if adjoint else 0
         tmpl = self._template(view, R, flags, hp_sigma)
         perm = [0, 2, 1] if _items.T_VIEW_LAYOUT[view] == "zxy" else [0, 1, 2]

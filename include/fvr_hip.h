@@ -316,9 +316,14 @@ int fvr_pou_warp_fwd(const float* imgs, int32_t B, int32_t N, int32_t P, const f
                      const float* cu, const float* cv, const float* rx, const float* ry,
                      const float* ic, int32_t per_pose_rays, int32_t K, float inv_ds, float c_idx,
                      float* out, void* stream);
-/* scale[n]: power of two with max_q |go[n][q] ic[q]| * scale <= 2^40 (1 when zero). */
+/* scale[n] = fvr_pou_scale_of_max(max_q |go[n][q] ic[q]|): the fixed-point scale of the
+ * deterministic image-gradient scatter of fvr_pou_warp_bwd / fvr_pou_render_bwd. */
 int fvr_pou_gscale(const float* go, const float* ic, int32_t N, int32_t K, int32_t per_pose_rays,
                    float* scale, void* stream);
+/* Host-callable: the power of two 2^min(40 - e, 126) for max_abs = m 2^e (m in [0.5, 1)),
+ * 1 for 0, NaN for NaN or +inf (a pose whose output gradient is not finite gets a NaN image
+ * gradient).  The same function the gscale kernel evaluates on the device. */
+float fvr_pou_scale_of_max(float max_abs);
 /* Bytes of the 64-bit accumulator workspace of fvr_pou_warp_bwd. */
 int64_t fvr_pou_warp_bwd_workspace(int32_t B, int32_t N, int32_t P);
 /* part[b][n][3] = d/d(d_b, cu, cv); grad_imgs (B, N, P, P) when non-NULL (needs

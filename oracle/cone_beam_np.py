@@ -11,12 +11,20 @@ How each function is pinned (``tests/test_oracle_pou.py``):
                           ``test_pou_windows_sum_to_unity``, atol 1e-5; here 1e-12).
 * ``resample_linear``  -- equals ``torch.nn.functional.interpolate``
                           (trilinear, align_corners=False) in float64.
-* ``build_blocks``     -- the reference's known answers: centre offset 1.25 mm
-                          between pixel- and grid-centre origins at 2.5 mm
+* ``build_blocks``     -- every block cube equals the PoU-weighted volume
+                          sampled (scipy ``map_coordinates``, trilinear) at the
+                          world points its samples stand for (centre + (q - P//2)
+                          ds), times the sinc^2 window and ds, to 1e-12 inside
+                          the block support, with and without resampling, on a
+                          non-cubic volume and block grid (a half-voxel centre
+                          error fails); the cubes, their window and ds divided
+                          out, re-sum to the volume; the reference's known
+                          answers: centre offset 1.25 mm between pixel- and
+                          grid-centre origins at 2.5 mm
                           (``test_grid_center_vs_pixel_center_origin``), 64
                           blocks for the isotropic preset
-                          (``test_slab_presets_and_parity_offsets``); the block
-                          weights re-sum to the volume (partition of unity).
+                          (``test_slab_presets_and_parity_offsets``).
+* ``sinc2_precomp_1d`` -- the closed form from sin(pi c) / (pi c), the 0.05 clamp.
 * ``slab_images``      -- Fourier-slice theorem: at an axis-aligned detector the
                           slab image is the ray sum of the padded, weighted cube
                           (samples hit lattice points exactly).
@@ -27,7 +35,11 @@ How each function is pinned (``tests/test_oracle_pou.py``):
                           equals the explicit-ray path
                           (``test_invariant_pinhole_matches_explicit_ray_tracer``);
                           batched equals single; front/back magnification
-                          (``test_perspective_differential_magnification_front_vs_back``).
+                          (``test_perspective_differential_magnification_front_vs_back``);
+                          brute-force line integrals of a smooth phantom along the
+                          pinhole rays (source 3 m away): centroids within 0.3 px,
+                          values within 12 % of the peak (the method's own
+                          approximation), a 1 mm centre error fails.
 """
 
 from __future__ import annotations

@@ -32,6 +32,13 @@ How each function is pinned (``tests/test_oracle_t.py``):
                                   high-passed copy is 1 (up to 1e-8 eps);
                                   gradient by FP64 central differences
                                   (h = 1e-3 deg / voxel).
+* ``spectral_ncc_loss`` (+grad) -- image-space loss: without the high-pass
+                                  weight, Parseval (NCC = sum p t / (|p| |t|) in
+                                  the image domain); with it, the same identity
+                                  for the high-passed images formed by an inverse
+                                  FFT; NCC(p, p) = 1, NCC(p, -p) = -1, scale
+                                  invariance, batch mean; gradient w.r.t. the
+                                  predicted image by FP64 central differences.
 * ``forward_drr``              -- the inverse FFT of the slice at the identity
                                   pose equals the ray sum along z of the padded
                                   volume sampled on the detector grid when the
@@ -300,3 +307,35 @@ def spectral_ncc_core_and_grad(spec, poses, f_targ, view="ap", high_pass=True, c
         grad[b, 4] = np.sum(np.real(-2j * math.pi * fj / span * S * np.conj(Tb))) / D
     loss = 1.0 - float(np.mean(nccs))
     return loss, -grad / B, nccs
+
+
+# ---------------------------------------------------------------------------
+# image-space Spectral-NCC loss (torch_engine.py:309-337)
+# ---------------------------------------------------------------------------
+def spectral_ncc_loss(pred, target, high_pass=True, cutoff_sigma=4.0, with_grad=False):
+    """1 - mean_b NCC_b of (B, 1, H, W) images, NCC = sum Re(Fp Ft*) / (sqrt(Ep Et) + 1e-8) with
+    F = fftshift(fft2(img)) [* hp] (torch_engine.py:309-337).  ``with_grad``: also d loss /
+    d pred (B, 1, H, W): dA/dp = H W Re(ifft2(ifftshift(hp Ft))), dEp/dp = 2 H W
+    Re(ifft2(ifftshift(hp Fp))) (F real-linear in p, fft2's adjoint is H W ifft2)."""
+    p = np.asarray(pred, np.float64)
+    t = np.asarray(target, np.float64)
+    H, W = p.shape[-2:]
+    hp = hp_filter(H, W, cutoff_sigma) if high_pass else np.ones((H, W))
+    Fp = np.fft.fftshift(np.fft.fft2(p), axes=(-2, -1)) * hp
+    Ft = np.fft.fftshift(np.fft.fft2(np.broadcast_to(t, p.shape)), axes=(-2, -1)) * hp
+    A = np.sum(np.real(Fp * np.conj(Ft)), axis=(-2, -1))
+    Ep = np.sum(np.abs(Fp) ** 2, axis=(-2, -1))
+    Et = np.sum(np.abs(Ft) ** 2, axis=(-2, -1))
+    D = np.sqrt(Ep * Et) + 1e-8
+    ncc = A / D
+    loss = 1.0 - float(np.mean(ncc))
+    if not with_grad:
+        return loss
+    adj = lambda X: H * W * np.real(np.fft.ifft2(np.fft.ifftshift(X, axes=(-2, -1))))
+    dA = adj(hp * Ft)
+    dEp = 2.0 * adj(hp * Fp)
+    sEp = np.sqrt(Ep)
+    dD = np.where(sEp[..., None, None] > 0, np.sqrt(Et)[..., None, None] * dEp / (2.0 * np.where(sEp > 0, sEp, 1.0))
+                  [..., None, None], 0.0)
+    dncc = dA / D[..., None, None] - (A / D ** 2)[..., None, None] * dD
+    return loss, -dncc / ncc.size

@@ -108,6 +108,26 @@ def test_pou_packed_size_and_argument_checks():
     assert L.fvr_pou_pack_blocks(16, 1, 600, 128, 16, None) == 1 and b"grid" in L.fvr_last_error()
 
 
+def test_pou_gradient_scale_formula():
+    """The fixed-point scale of the PoU image-gradient scatter (the function the gscale
+    kernel evaluates): 2^(40 - e) for a maximum m 2^e, m in [0.5, 1); capped at 2^126 so
+    that tiny and subnormal maxima keep a finite power of two (their fixed-point values stay
+    below 2^40); 1 for an all-zero gradient; NaN for NaN or infinite maxima."""
+    import math
+    from fvr import _lib
+    f = _lib.lib().fvr_pou_scale_of_max
+    for mx in (1.0, 0.75, 3.0, 1e-3, 1e6, 2.0 ** -80):
+        m, e = math.frexp(mx)
+        assert f(mx) == 2.0 ** (40 - e), mx
+        assert 2.0 ** 39 <= mx * f(mx) < 2.0 ** 40
+    for mx in (2.0 ** -90, 1e-30, 2.0 ** -130, 1e-45):  # tiny and subnormal: the cap
+        s = f(mx)
+        assert s == 2.0 ** 126 and math.isfinite(s) and mx * s < 2.0 ** 40, mx
+    assert f(0.0) == 1.0
+    assert math.isnan(f(float("nan"))) and math.isnan(f(float("inf")))
+    assert f(3.4e38) == 2.0 ** (40 - 128)
+
+
 def test_kernel_variant_table():
     """Tile heights 32 / 128 exist for the batched adjoint forms; every flag set has 64."""
     from fvr import _lib
@@ -250,3 +270,25 @@ def test_every_struct_field_offset_matches_the_header(tmp_path):
         for f in dt.names:
             assert int(out[f"{cname}.{f}"]) == dt.fields[f][1], (cname, f)
         assert int(out[f"{cname}.__size__"]) == dt.itemsize
+
+
+def test_t_target_and_pose_checks_reject_bad_inputs_before_any_launch():
+    """The fused convention-T launch forms a target address per item from data_ptr() and
+    stride(0): anything but (Bt, R, R) complex64 with contiguous rows, Bt in {1, B}, on the
+    evaluator's device, is refused first (checked here against the CPU device)."""
+    from fvr.torch_engine import check_t_poses, check_t_targets
+    cpu = torch.device("cpu")
+    good = torch.zeros((3, 8, 8), dtype=torch.complex64)
+    assert check_t_targets(good, 3, cpu) == 8 and check_t_targets(good[:1], 3, cpu) == 8
+    bad = [good.to(torch.complex128), good.real.contiguous(), good[:2], good[0], good[:, :, :4],
+           good.transpose(1, 2), good[:, ::2, ::2], torch.zeros((3, 8, 16), dtype=torch.complex64)]
+    for t in bad:
+        with pytest.raises(ValueError):
+            check_t_targets(t, 3, cpu)
+    with pytest.raises(ValueError):
+        check_t_targets(good, 3, torch.device("cuda", 0))
+    P = torch.zeros((3, 6), dtype=torch.float64)
+    assert check_t_poses(P, cpu) == 3
+    for p in (P.float(), P[:, :5], P.t().contiguous(), torch.zeros((0, 6), dtype=torch.float64), P[::2]):
+        with pytest.raises(ValueError):
+            check_t_poses(p, cpu)

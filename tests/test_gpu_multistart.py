@@ -285,3 +285,38 @@ def test_two_launch_step_equals_separate_launches(vols, ragged):
     for x, y in ((aa.best_pose, ab.best_pose), (aa.best_loss, ab.best_loss), (aa.m, ab.m), (aa.v, ab.v),
                  (aa.t, ab.t)):
         np.testing.assert_array_equal(x.cpu().numpy(), y.cpu().numpy())
+
+
+def test_graph_loop_recaptures_after_plan_targets_or_workspace_change(vols):
+    """A captured loop is replayed only while the evaluator's plan, targets and workspace are
+    the ones it was captured over: a new plan, new targets or a larger batch that regrows the
+    workspace make the next run capture again, and every run equals the eager loop."""
+    from fvr.multistart import AdamLoop
+    from fvr.sncc import BiplanarEvaluator, LaunchPlan
+    r, (ta, tl) = vols[1]
+    ev = BiplanarEvaluator(r.packed, ta, tl)
+    P0 = _poses(8, 23)
+
+    def eager_ref():
+        e = BiplanarEvaluator(r.packed, *(t.clone() for t in ev.targets), plan=ev.plan)
+        return [t.cpu().numpy().copy() for t in AdamLoop(e, 8, 7, graph=False, tol=0.0).run(P0)]
+
+    loop = AdamLoop(ev, 8, 7, graph=True, tol=0.0)
+
+    def check(expect_captures):
+        got = [t.cpu().numpy().copy() for t in loop.run(P0)]
+        for a, b in zip(got, eager_ref()):
+            np.testing.assert_array_equal(a, b)
+        assert loop.captures == expect_captures, (loop.captures, expect_captures)
+
+    check(1)
+    check(1)  # plain replay
+    ev.set_plan(ev.plan)  # the same plan: nothing changes
+    check(1)
+    ev.set_plan(LaunchPlan(32, "target", 0, 2))
+    check(2)
+    ev.set_targets(tl.flip(0).contiguous() if ta.shape == tl.shape else ta, tl)
+    check(3)
+    ev.evaluate_device(_poses(40, 3))  # regrows the workspace the graph was captured over
+    check(4)
+    check(4)

@@ -106,6 +106,10 @@ def test_fused_spectral_ncc_loss_and_gradient(eng, view, high_pass):
     e, S3 = eng
     R = 24
     ft, ft_np = _target(e, S3, [1.0, -0.5, 0.5, 0.4, -0.3, 0.0], view, R, high_pass)
+    # prepare_spectral_target (torch_engine.py:255-267: the image cast to float32, FFT on the
+    # device) against the oracle's float64 preparation of the same image
+    assert tuple(ft.shape) == (1, 1, R, R)
+    assert np.abs(ft[0, 0].cpu().numpy() - ft_np).max() <= 1e-5 * np.abs(ft_np).max()
     poses = np.array([[3.0, -2.0, 1.0, 1.5, -1.0, 0.3], [-4.0, 2.5, -1.5, -0.8, 1.2, 0.0],
                       [0.0, 0.0, 0.0, 0.0, 0.0, 0.0]])
     P = torch.from_numpy(poses).float().requires_grad_(True)
@@ -139,6 +143,29 @@ def test_fused_gradient_matches_torch_autograd_of_generic_path(eng):
     loss2.backward()
     g1, g2 = P1.grad.numpy(), P2.grad.cpu().numpy()
     assert np.max(np.abs(g1 - g2)) <= 2e-3 * np.abs(g2).max(), (g1, g2)
+
+
+@pytest.mark.parametrize("dtype,tol_l,tol_g", [(torch.float32, 1e-5, 1e-4), (torch.float64, 1e-10, 1e-9)])
+@pytest.mark.parametrize("high_pass", [True, False])
+def test_image_spectral_ncc_loss_matches_oracle(eng, dtype, tol_l, tol_g, high_pass):
+    """spectral_ncc_loss (torch_engine.py:309-337) on device DRRs of three poses against
+    the oracle's FP64 loss and its exact gradient w.r.t. the predicted images (autograd
+    through rocFFT on the device): float32 inputs within 1e-5 (loss) and 1e-4 of
+    max|grad|, float64 inputs within 1e-10 and 1e-9."""
+    e, _ = eng
+    poses = torch.tensor([[3.0, -2.0, 1.0, 1.5, -1.0, 0.3], [-4.0, 2.5, -1.5, -0.8, 1.2, 0.0],
+                          [0.5, 0.5, -0.5, 0.2, 0.1, 0.0]])
+    with torch.no_grad():
+        pred = e(poses, view="ap", out_res=32).to(dtype)
+        tgt = e(torch.tensor([[1.0, -0.5, 0.5, 0.4, -0.3, 0.0]]), view="ap", out_res=32).to(dtype)
+    x = pred.clone().requires_grad_(True)
+    loss = e.spectral_ncc_loss(x, tgt, high_pass=high_pass)
+    loss.backward()
+    L, g = T.spectral_ncc_loss(pred.double().cpu().numpy(), tgt.double().cpu().numpy(), high_pass=high_pass,
+                               with_grad=True)
+    assert abs(float(loss) - L) <= tol_l, (float(loss), L)
+    gg = x.grad.double().cpu().numpy()
+    assert np.abs(gg - g).max() <= tol_g * np.abs(g).max(), np.abs(gg - g).max()
 
 
 def test_per_pose_targets_and_batched_equals_single(eng):

@@ -192,3 +192,33 @@ def test_hermitian_footprint_is_about_half():
     h, w = 40, 40
     assert Sh == (h // 2 - 1) * w + (1 + w // 2) + (min(h, w) // 2 - 1) + w
     assert 0.45 < Uh / U < 0.6 and 0.45 < Sh / S < 0.6
+
+
+def test_product_mtre_reference_closed_forms_and_oracle():
+    """fvr.registration_3d2d.compute_mtre_6dof (registration_3d2d.py:112-175): the
+    reference's known answers (tests/test_physical_units.py:52-59: 6.0 for a 3-voxel shift
+    at 2 mm, 3.0 for 3 mm, 16.0 for a 90 degree yaw of the 8 landmarks at r = 4 voxels,
+    2 mm), a 180 degree roll (every landmark moves by 2 r in its two rotated coordinates:
+    2 * sqrt(2) * 8 mm), and the oracle's restatement on random poses and shapes."""
+    from fvr.registration_3d2d import compute_mtre_6dof
+    o = (0.0,) * 6
+    assert abs(compute_mtre_6dof(o, (0, 0, 0, 3, 0, 0), (16,) * 3, spacing=(2.0,) * 3) - 6.0) < 1e-12
+    assert abs(compute_mtre_6dof(o, (0, 0, 0, 3, 0, 0), (16,) * 3, spacing=(2.0,) * 3,
+                                 translation_units="mm") - 3.0) < 1e-12
+    assert abs(compute_mtre_6dof(o, (90, 0, 0, 0, 0, 0), (16,) * 3, spacing=(2.0,) * 3) - 16.0) < 1e-12
+    assert abs(compute_mtre_6dof(o, (0, 0, 180, 0, 0, 0), (16,) * 3, spacing=(2.0,) * 3)
+               - 2.0 * np.sqrt(2.0) * 8.0) < 1e-12
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        a, b = rng.uniform(-20, 20, 6), rng.uniform(-20, 20, 6)
+        shape = tuple(int(x) for x in rng.integers(8, 300, 3))
+        sp = tuple(rng.uniform(0.3, 3.0, 3))
+        for units in ("voxels", "mm"):
+            got = compute_mtre_6dof(a, b, shape, spacing=sp, translation_units=units)
+            ref = O.compute_mtre_6dof(a, b, shape, spacing=sp, translation_units=units)
+            assert abs(got - ref) <= 1e-12 * max(1.0, ref)
+    for sp in ((0, 1, 1), (1.0, float("nan"), 1.0), (1.0, 1.0)):
+        with pytest.raises(ValueError):
+            compute_mtre_6dof(o, o, (16,) * 3, spacing=sp)
+    with pytest.raises(ValueError):
+        compute_mtre_6dof(o, o, (16,) * 3, translation_units="cm")

@@ -128,3 +128,180 @@ def test_perspective_magnification_front_vs_back():
     x_front = float((top.sum(0) * u).sum() / top.sum())
     x_back = float((bot.sum(0) * u).sum() / bot.sum())
     assert x_front > x_back + 7.0
+
+
+# ---------------------------------------------------------------------------
+# block geometry: every block cube, sampled from world coordinates
+# ---------------------------------------------------------------------------
+def _blob_volume(shape, vox, spec):
+    """Sum of Gaussians (centre mm, sigma mm, amplitude) on a pixel-centred grid."""
+    axes = [(np.arange(n) - (n - 1) / 2.0) * vox for n in shape]
+    zz, yy, xx = np.meshgrid(*axes, indexing="ij")
+    v = np.zeros(shape)
+    for (z, y, x), s, a in spec:
+        v += a * np.exp(-((zz - z) ** 2 + (yy - y) ** 2 + (xx - x) ** 2) / (2 * s * s))
+    return v
+
+
+BLOBS = [((6.0, -8.0, 4.0), 6.0, 0.02), ((-10.0, 12.0, -6.0), 5.0, 0.03), ((14.0, 10.0, 16.0), 7.0, 0.015)]
+
+
+def _expected_cube(vol, windows, center, ds, P, block_res, vox, origin, trilinear):
+    """The padded cube a block should hold, formed from world coordinates alone: sample
+    q of the P^3 cube sits at world point center + (q - P//2) ds; inside the block_res^3
+    core it holds ds x sinc^2 pre-compensation x (PoU-weighted volume, trilinear at that
+    point); outside the core it is 0.  Returns (expected, volume index coordinates)."""
+    pad_b = (P - block_res) // 2
+    q = np.arange(pad_b, pad_b + block_res)
+    idx = [(center[k] + (q - P // 2) * ds - origin[k]) / vox for k in range(3)]
+    g = vol * windows
+    pts = np.stack(np.meshgrid(*idx, indexing="ij"), axis=0).reshape(3, -1)
+    core = trilinear(g, pts).reshape((block_res,) * 3)
+    w = C.sinc2_precomp_1d(block_res, P)
+    out = np.zeros((P, P, P))
+    sl = slice(pad_b, pad_b + block_res)
+    out[sl, sl, sl] = core * ds * (w[:, None, None] * w[None, :, None] * w[None, None, :])
+    return out, idx
+
+
+@pytest.mark.parametrize("block_res", [None, 16, 40], ids=["no-resample", "down", "up"])
+def test_block_cubes_equal_world_space_sampling_of_the_weighted_volume(block_res):
+    """Pins the block bookkeeping of cone_beam_fvr.py:208-284 (block centres with the
+    half-voxel offset to the FFT centre, ds = M vox / block_res, the cube size M, the
+    resampling rule, the sinc^2 window's centring, the ds factor, the padding) without
+    retyping it: each cube equals the PoU-weighted volume sampled (scipy
+    map_coordinates, trilinear) at the world points its samples stand for, wherever the
+    trilinear stencil lies inside the block's support (the cube is cut to the support
+    where the window exceeds 1e-6); on a non-cubic volume with a non-cubic block grid, so
+    a transposed axis fails.  Without resampling every sample lands on a voxel centre.
+    The same comparison with the centres moved by half a voxel fails (the check is
+    sensitive to the plausible mistake)."""
+    from scipy.ndimage import map_coordinates
+
+    def trilinear(g, pts):
+        return map_coordinates(g, pts, order=1, mode="constant", cval=0.0)
+
+    shape, vox, grid = (36, 40, 44), 2.0, (2, 3, 2)
+    vol = _blob_volume(shape, vox, BLOBS)
+    wl = [C.make_pou_1d(n, k, 0.25)[0] for n, k in zip(shape, grid)]
+    probe = C.from_volume(vol, vox, grid, 8, energy_thresh=-1.0)
+    M = int(round(probe["ds"] * 8 / vox))  # the cube size the bookkeeping chose
+    br = M if block_res is None else block_res
+    m = C.from_volume(vol, vox, grid, br, energy_thresh=-1.0)
+    assert m["B"] == 12 and m["P"] == 2 * br
+    assert abs(m["ds"] - M * vox / br) < 1e-15
+    origin = [-0.5 * (n - 1) * vox for n in shape]
+    b = 0
+    worst, worst_shift = 0.0, 0.0
+    for k0 in range(grid[0]):
+        for k1 in range(grid[1]):
+            for k2 in range(grid[2]):
+                win = wl[0][k0][:, None, None] * wl[1][k1][None, :, None] * wl[2][k2][None, None, :]
+                sup = [np.nonzero(w[k] > 1e-6)[0][[0, -1]] for w, k in zip(wl, (k0, k1, k2))]
+                exp, idx = _expected_cube(vol, win, m["centers"][b], m["ds"], m["P"], br, vox, origin, trilinear)
+                pad_b = (m["P"] - br) // 2
+                inside = [(x >= lo + 1) & (x <= hi - 1) for x, (lo, hi) in zip(idx, sup)]
+                mask = np.zeros((m["P"],) * 3, bool)
+                sl = slice(pad_b, pad_b + br)
+                mask[sl, sl, sl] = inside[0][:, None, None] & inside[1][None, :, None] & inside[2][None, None, :]
+                assert mask.sum() > 0.05 * br ** 3  # not vacuous: corner blocks have short supports
+                worst = max(worst, np.abs(m["cubes"][b] - exp)[mask].max())
+                exp_s, _ = _expected_cube(vol, win, m["centers"][b] + 0.5 * vox, m["ds"], m["P"], br, vox, origin,
+                                          trilinear)
+                worst_shift = max(worst_shift, np.abs(m["cubes"][b] - exp_s)[mask].max())
+                b += 1
+    peak = np.abs(m["cubes"]).max()
+    assert worst <= 1e-12 * peak, worst / peak
+    assert worst_shift >= 1e-2 * peak, worst_shift / peak
+
+
+def test_block_cubes_re_sum_to_the_volume():
+    """Without resampling, the cubes with their sinc^2 window and ds factor divided out,
+    placed back at the voxels their world coordinates name, sum to the volume (the
+    partition of unity, up to the 1e-6 support cut)."""
+    shape, vox, grid = (36, 40, 44), 2.0, (2, 3, 2)
+    vol = _blob_volume(shape, vox, BLOBS)
+    probe = C.from_volume(vol, vox, grid, 8, energy_thresh=-1.0)
+    M = int(round(probe["ds"] * 8 / vox))
+    m = C.from_volume(vol, vox, grid, M, energy_thresh=-1.0)
+    P, ds = m["P"], m["ds"]
+    assert abs(ds - vox) < 1e-15
+    pad_b = (P - M) // 2
+    w = C.sinc2_precomp_1d(M, P)
+    w3 = w[:, None, None] * w[None, :, None] * w[None, None, :]
+    origin = np.array([-0.5 * (n - 1) * vox for n in shape])
+    acc = np.zeros(shape)
+    for b in range(m["B"]):
+        core = m["cubes"][b][pad_b:pad_b + M, pad_b:pad_b + M, pad_b:pad_b + M] / (w3 * ds)
+        start = (m["centers"][b] + (pad_b - P // 2) * ds - origin) / vox  # volume index of core sample 0
+        assert np.allclose(start, np.round(start), atol=1e-9)
+        s = np.round(start).astype(int)
+        lo = np.maximum(s, 0)
+        hi = np.minimum(s + M, shape)
+        acc[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] += core[lo[0] - s[0]:hi[0] - s[0], lo[1] - s[1]:hi[1] - s[1],
+                                                          lo[2] - s[2]:hi[2] - s[2]]
+    assert np.abs(acc - vol).max() <= 1e-5 * np.abs(vol).max()
+
+
+def test_sinc2_precomp_closed_form():
+    """1 / max(sinc^2((i - (n-1)/2) / p), 0.05) (cone_beam_fvr.py:64-68) from sin(pi c)/(pi c)."""
+    import math
+    for n, p in ((5, 10), (8, 16), (64, 128), (7, 7)):
+        got = C.sinc2_precomp_1d(n, p)
+        for i in range(n):
+            c = (i - (n - 1) / 2.0) / p
+            s = 1.0 if c == 0 else math.sin(math.pi * c) / (math.pi * c)
+            assert abs(got[i] - 1.0 / max(s * s, 0.05)) <= 1e-13 * got[i]
+        np.testing.assert_allclose(got, got[::-1], rtol=1e-15)
+    assert C.sinc2_precomp_1d(5, 10)[2] == 1.0  # odd n: the centre sample is c = 0
+    assert C.sinc2_precomp_1d(3, 1)[0] == 20.0  # |c| = 1: sinc = 0, the clamp at 0.05
+
+
+def test_pinhole_projection_against_brute_force_ray_sums():
+    """End-to-end geometry: the PoU projection of a smooth phantom against brute-force
+    line integrals (trilinear, 0.1 mm steps) along the true pinhole rays, source 3 m from
+    the isocentre so that each block's single magnification is close to exact.  What is
+    left is the method's own approximation (the sinc^2 pre-compensation over-weights the
+    low frequencies by a few percent): image centroids agree within 0.3 px (measured
+    0.07-0.19 px) and values within 12 % of the peak.  A block centre moved by 1 mm (half
+    a voxel) moves the centroid by 0.4-0.65 px and fails."""
+    from scipy.ndimage import map_coordinates
+    from scipy.spatial.transform import Rotation
+    n, vox = 40, 2.0
+    vol = _blob_volume((n, n, n), vox, BLOBS)
+    sad, sdd, H, pix = 3000.0, 6000.0, 64, 3.0
+    f = sdd / pix
+    k_inv = np.array([[1 / f, 0, -H / (2 * f)], [0, 1 / f, -H / (2 * f)], [0, 0, 1]])
+    lat = C.make_pinhole_lattice(k_inv, sdd, H, H)
+    rots = Rotation.from_euler("zyx", [[0, 0, 0], [20, -15, 10], [-40, 30, 60]], degrees=True).as_matrix()
+    mats = np.tile(np.eye(4), (3, 1, 1))
+    mats[:, :3, :3] = rots
+    mats[:, :3, 3] = np.einsum("nij,j->ni", rots, [0, 0, -sad]) + [[3.0, -2.0, 0], [-4.0, 5.0, 0], [1.0, 1.0, 2.0]]
+    origin = -0.5 * (n - 1) * vox
+    brute = []
+    half = 0.5 * n * vox * np.sqrt(3) + 2 * vox
+    ss = np.arange(-half, half, 0.1)
+    for M in mats:
+        src = M[:3, :3] @ lat["src0"] + M[:3, 3]
+        trg = np.einsum("ij,hwj->hwi", M[:3, :3], lat["trg0"]) + M[:3, 3]
+        u = (trg - src) / np.linalg.norm(trg - src, axis=-1, keepdims=True)
+        s0 = np.einsum("hwi,i->hw", u, -src)  # closest approach to the isocentre
+        pts = src + (s0[..., None] + ss)[..., None] * u[:, :, None, :]
+        val = map_coordinates(vol, ((pts - origin) / vox).reshape(-1, 3).T, order=1, mode="constant", cval=0.0)
+        brute.append(val.reshape(pts.shape[:3]).sum(-1) * 0.1)
+    brute = np.array(brute)
+    yy, xx = np.mgrid[:H, :H]
+
+    def centroid(a):
+        return np.array([(a * yy).sum() / a.sum(), (a * xx).sum() / a.sum()])
+
+    for br in (31, 24, 48):
+        m = C.from_volume(vol, vox, (2, 2, 2), br)
+        out = C.forward_pinhole(m, mats, lat)[:, 0]
+        for k in range(3):
+            assert np.abs(centroid(out[k]) - centroid(brute[k])).max() <= 0.3, (br, k)
+            assert np.abs(out[k] - brute[k]).max() <= 0.12 * np.abs(brute[k]).max(), (br, k)
+    m = C.from_volume(vol, vox, (2, 2, 2), 31)
+    moved = dict(m, centers=m["centers"] + np.array([1.0, 0.0, 0.0]))
+    out = C.forward_pinhole(moved, mats, lat)[:, 0]
+    assert max(np.abs(centroid(out[k]) - centroid(brute[k])).max() for k in range(3)) > 0.3

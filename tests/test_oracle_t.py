@@ -236,3 +236,48 @@ def test_t_items_geometry_and_combine_reproduce_oracle(view):
         L1, g1, ncc1 = T.spectral_ncc_core_and_grad(S3, pose[None], tgt, view, True, sigma)
         assert abs(ncc[0] - ncc1[0]) < 1e-12
         np.testing.assert_allclose(-d[0], g1[0], atol=1e-12 * max(1.0, np.abs(g1).max()))
+
+
+@pytest.mark.parametrize("shape", [(12, 12), (10, 14)])
+def test_image_spectral_ncc_loss_parseval_invariants_and_fd(shape):
+    """Oracle spectral_ncc_loss (torch_engine.py:309-337) pinned without retyping it:
+    Parseval in the image domain (high-pass off: NCC = sum p t / (|p| |t|); high-pass on:
+    the same for the images filtered by an inverse FFT of hp * F), NCC(p, p) = 1,
+    NCC(p, -p) = -1, scale invariance, the batch mean, and the gradient w.r.t. the
+    predicted image against FP64 central differences."""
+    H, W = shape
+    rng = np.random.default_rng(7)
+    p = rng.standard_normal((3, 1, H, W))
+    t = rng.standard_normal((3, 1, H, W))
+    hw = H * W
+    # high-pass off: Parseval, with the reference's 1e-8 added to sqrt(Ep Et) = H W |p| |t|
+    ncc = np.sum(p * t, axis=(-2, -1)) / (np.sqrt(np.sum(p * p, axis=(-2, -1)) * np.sum(t * t, axis=(-2, -1)))
+                                         + 1e-8 / hw)
+    assert abs(T.spectral_ncc_loss(p, t, high_pass=False) - (1.0 - ncc.mean())) < 1e-12
+    # high-pass on: images filtered in the Fourier domain, inner products in the image domain
+
+    def filt(x):
+        F = np.fft.fftshift(np.fft.fft2(x), axes=(-2, -1)) * T.hp_filter(H, W, 4.0)
+        return np.fft.ifft2(np.fft.ifftshift(F, axes=(-2, -1)))
+
+    pf, tf = filt(p), filt(t)
+    ncc_hp = np.sum(np.real(pf * np.conj(tf)), axis=(-2, -1)) / (
+        np.sqrt(np.sum(np.abs(pf) ** 2, axis=(-2, -1)) * np.sum(np.abs(tf) ** 2, axis=(-2, -1))) + 1e-8 / hw)
+    assert abs(T.spectral_ncc_loss(p, t) - (1.0 - ncc_hp.mean())) < 1e-12
+    # invariants (up to the 1e-8 in the denominator: relative 1e-8 / (H W |p|^2) here)
+    assert abs(T.spectral_ncc_loss(p, p)) < 1e-10
+    assert abs(T.spectral_ncc_loss(p, -p) - 2.0) < 1e-10
+    assert abs(T.spectral_ncc_loss(3.5 * p, 0.25 * t) - T.spectral_ncc_loss(p, t)) < 1e-12
+    singles = [T.spectral_ncc_loss(p[b:b + 1], t[b:b + 1]) for b in range(3)]
+    assert abs(T.spectral_ncc_loss(p, t) - np.mean(singles)) < 1e-12
+    # gradient w.r.t. the predicted image
+    for hpass in (True, False):
+        L, g = T.spectral_ncc_loss(p, t, high_pass=hpass, with_grad=True)
+        h = 1e-6
+        fd = np.zeros_like(p)
+        for idx in np.ndindex(p.shape):
+            pp, pm = p.copy(), p.copy()
+            pp[idx] += h
+            pm[idx] -= h
+            fd[idx] = (T.spectral_ncc_loss(pp, t, high_pass=hpass) - T.spectral_ncc_loss(pm, t, high_pass=hpass)) / (2 * h)
+        assert np.abs(g - fd).max() <= 1e-7 * np.abs(fd).max(), np.abs(g - fd).max()
