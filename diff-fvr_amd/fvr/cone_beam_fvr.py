@@ -154,6 +154,7 @@ class _PouWarpSum(torch.autograd.Function):
                    "fvr_pou_warp_fwd")
         ctx.save_for_backward(im, db, u_, v_, rxf, ryf, icf)
         ctx.meta = (B, N, P, K, per_pose, float(inv_ds), float(c_idx), imgs.dtype, (out_h, out_w), rx.shape)
+        ctx.dts = tuple(t.dtype for t in (d_b, cu, cv, rx, ry, ic))
         return out
 
     @staticmethod
@@ -186,8 +187,8 @@ class _PouWarpSum(torch.autograd.Function):
                                            _lib.ptr(v_), _lib.ptr(rxf), _lib.ptr(ryf), _lib.ptr(icf), per_pose, K,
                                            inv_ds, c_idx, _lib.ptr(grx), _lib.ptr(gry), _lib.ptr(gic), st),
                        "fvr_pou_ray_grads")
-            grx, gry, gic = (t.reshape(rshape) for t in (grx, gry, gic))
-        return (gimg.to(idt) if gimg is not None else None, part[..., 0], part[..., 1], part[..., 2],
+            grx, gry, gic = (t.reshape(rshape).to(dt) for t, dt in zip((grx, gry, gic), ctx.dts[3:]))
+        return (gimg.to(idt) if gimg is not None else None, *(part[..., k].to(ctx.dts[k]) for k in range(3)),
                 grx, gry, gic, None, None, None, None)
 
 
@@ -217,6 +218,7 @@ class _PouRender(torch.autograd.Function):
         ctx.save_for_backward(u, v, db, c_u, c_v, rxf, ryf, icf, imgs)
         ctx.model = model
         ctx.meta = (B, N, P, K, per_pose, inv_ds, c_idx, rx.shape, u_xyz.dtype)
+        ctx.dts = tuple(t.dtype for t in (d_b, cu, cv, rx, ry, ic))
         return out
 
     @staticmethod
@@ -246,8 +248,8 @@ class _PouRender(torch.autograd.Function):
                                            _lib.ptr(c_v), _lib.ptr(rxf), _lib.ptr(ryf), _lib.ptr(icf), per_pose, K,
                                            inv_ds, c_idx, _lib.ptr(grx), _lib.ptr(gry), _lib.ptr(gic), st),
                        "fvr_pou_ray_grads")
-            grx, gry, gic = (t.reshape(rshape) for t in (grx, gry, gic))
-        return (guv[:, 0:3].to(udt), guv[:, 3:6].to(udt), part[..., 0], part[..., 1], part[..., 2], grx, gry, gic,
+            grx, gry, gic = (t.reshape(rshape).to(dt) for t, dt in zip((grx, gry, gic), ctx.dts[3:]))
+        return (guv[:, 0:3].to(udt), guv[:, 3:6].to(udt), *(part[..., k].to(ctx.dts[k]) for k in range(3)), grx, gry, gic,
                 None, None, None)
 
 
@@ -557,28 +559,36 @@ class PerspectiveSlabFVR(nn.Module):
 
     def forward_carm_view(self, yaw_deg, pitch_deg, roll_deg, tx_vox, ty_vox, tz_vox, sad_mm: float, sdd_mm: float,
                           det_px: int, det_fov_mm: float, view: str = "ap") -> torch.Tensor:
-        """(det_px, det_px) C-arm projection, AP or lateral geometry (cone_beam_fvr.py:761-832)."""
+        """(det_px, det_px) C-arm projection, AP or lateral geometry (cone_beam_fvr.py:761-832).
+
+        The geometry (rotation, source, lattice, rays) is formed in float64 and cast to
+        float32 only where the kernels read it: a translation moves the source and every
+        lattice point together, so its gradient is the difference of two large sums over
+        the pixels' ray gradients, which float32 autograd would round to ~1e-3."""
         dev = self.F_blk_ri.device
-        a = [torch.as_tensor(x, dtype=torch.float32).to(dev) for x in (yaw_deg, pitch_deg, roll_deg, tx_vox, ty_vox,
-                                                                        tz_vox)]
+        f64 = torch.float64
+        a = [torch.as_tensor(x).to(device=dev, dtype=f64) for x in (yaw_deg, pitch_deg, roll_deg, tx_vox, ty_vox,
+                                                                    tz_vox)]
         yr, pr, rr = (torch.deg2rad(x) for x in a[:3])
         cy, sy, cp, sp, cr, sr = torch.cos(yr), torch.sin(yr), torch.cos(pr), torch.sin(pr), torch.cos(rr), torch.sin(rr)
-        z, o = torch.zeros((), device=dev), torch.ones((), device=dev)
+        z, o = torch.zeros((), device=dev, dtype=f64), torch.ones((), device=dev, dtype=f64)
         rz = torch.stack([torch.stack([o, z, z]), torch.stack([z, cy, -sy]), torch.stack([z, sy, cy])])
         ry = torch.stack([torch.stack([cr, z, sr]), torch.stack([z, o, z]), torch.stack([-sr, z, cr])])
         rx = torch.stack([torch.stack([cp, -sp, z]), torch.stack([sp, cp, z]), torch.stack([z, z, o])])
         rot = rz @ ry @ rx
-        lin = torch.linspace(-0.5, 0.5, det_px, device=dev) * float(det_fov_mm)
+        lin = torch.linspace(-0.5, 0.5, det_px, device=dev, dtype=f64) * float(det_fov_mm)
         vv, uu = torch.meshgrid(lin, lin, indexing="ij")
         det_d = float(sdd_mm - sad_mm)
         if view == "ap":
-            src0 = torch.tensor([0.0, -float(sad_mm), 0.0], device=dev)
+            src0 = torch.tensor([0.0, -float(sad_mm), 0.0], device=dev, dtype=f64)
             trg0 = torch.stack([vv, torch.full_like(uu, det_d), uu], dim=-1)
-            u0, v0, n0 = (torch.tensor(x, device=dev) for x in ([0.0, 0.0, 1.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0]))
+            u0, v0, n0 = (torch.tensor(x, device=dev, dtype=f64) for x in ([0.0, 0.0, 1.0], [1.0, 0.0, 0.0],
+                                                                          [0.0, 1.0, 0.0]))
         else:
-            src0 = torch.tensor([0.0, 0.0, -float(sad_mm)], device=dev)
+            src0 = torch.tensor([0.0, 0.0, -float(sad_mm)], device=dev, dtype=f64)
             trg0 = torch.stack([vv, uu, torch.full_like(uu, det_d)], dim=-1)
-            u0, v0, n0 = (torch.tensor(x, device=dev) for x in ([0.0, 1.0, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, 1.0]))
+            u0, v0, n0 = (torch.tensor(x, device=dev, dtype=f64) for x in ([0.0, 1.0, 0.0], [1.0, 0.0, 0.0],
+                                                                          [0.0, 0.0, 1.0]))
         shift = torch.stack([a[5], a[4], a[3]]) * self.vox_mm
         src = rot @ src0 - shift
         trg = torch.einsum("ij,hwj->hwi", rot, trg0) - shift

@@ -286,8 +286,10 @@ def test_carm_view_and_pose_gradient_match_oracle(view, block_res):
     """forward_carm_view (cone_beam_fvr.py:761-832: Euler pose in degrees and voxels, AP or
     lateral C-arm geometry) against the oracle's restatement, and its 6-DoF gradient
     (autograd through the explicit-ray path: the fused or the generic kernels and the ray
-    gradients) against FP64 central differences of the oracle (h = 1e-3 deg / voxel):
-    images within 1e-4 of max, gradient within 2e-3 of max|grad|."""
+    gradients) against FP64 central differences of the oracle (h = 1e-5 deg / voxel: the
+    bilinear warp has kinks at pixel boundaries, and differences over h = 1e-3 straddle
+    enough of them to be off by 2e-3; at 1e-5 they have converged to 1e-7):
+    images within 1e-4 of max, gradient within 1e-4 of max|grad|."""
     from fvr import PerspectiveSlabFVR
     n, vox = 32, 2.0
     vol = np.zeros((n, n, n), dtype=np.float32)
@@ -306,10 +308,10 @@ def test_carm_view_and_pose_gradient_match_oracle(view, block_res):
     w = np.random.default_rng(4).standard_normal((40, 40))
     (img * torch.from_numpy(w).float().to(img.device)).sum().backward()
     g = np.array([float(p.grad) for p in params])
-    h = 1e-3
+    h = 1e-5
     fd = np.array([(np.sum(C.forward_carm_view(o, *(x0 + h * e), **geo) * w)
                     - np.sum(C.forward_carm_view(o, *(x0 - h * e), **geo) * w)) / (2 * h) for e in np.eye(6)])
-    assert np.abs(g - fd).max() <= 2e-3 * np.abs(fd).max(), (g, fd)
+    assert np.abs(g - fd).max() <= 1e-4 * np.abs(fd).max(), (g, fd)
 
 
 def test_reference_presets_and_offsets():
