@@ -531,6 +531,100 @@ int fvr_pn_reduce_combine(const fvr_sncc_item* items_dev, int32_t n_poses, const
   return FVR_OK;
 }
 
+}  // extern "C"
+
+namespace fvr {
+
+static inline int64_t align256(int64_t b) { return (b + 255) & ~int64_t(255); }
+
+// tiles of one pose (AP + LAT) for two validated templates
+static inline int64_t pn_tiles_per_pose(const fvr_sncc_item* t) {
+  return int64_t(t[0].tiles_x) * sncc_tiles_y(t[0].h, t[0].flags, t[0].tile_h) +
+         int64_t(t[1].tiles_x) * sncc_tiles_y(t[1].h, t[1].flags, t[1].tile_h);
+}
+
+// workspace of fvr_pn_evaluate: items | Jacobians | tile partials, each 256-byte aligned
+struct PnWs {
+  int64_t items, jac, partials, total;
+};
+static inline PnWs pn_ws_layout(const fvr_sncc_item* t, int32_t n) {
+  PnWs w;
+  w.items = 0;
+  w.jac = align256(int64_t(2) * n * int64_t(sizeof(fvr_sncc_item)));
+  w.partials = w.jac + align256(int64_t(2) * n * 18 * int64_t(sizeof(double)));
+  w.total = w.partials + align256(int64_t(n) * pn_tiles_per_pose(t) * FVR_SNCC_NMOM * int64_t(sizeof(float)));
+  return w;
+}
+
+// argument checks shared by fvr_pn_evaluate and its allocation-free variant
+static int pn_evaluate_check(const char* who, const fvr_layout* layouts_dev, int32_t n_layouts,
+                             const fvr_sncc_item* tmpl_host, double pz, double py, double px, uint32_t variant,
+                             const double* poses, int32_t n_poses, const double* loss, const double* grad) {
+  FVR_CHECK_ARG(layouts_dev && n_layouts > 0 && tmpl_host && poses && loss && grad && n_poses > 0,
+                "%s: bad arguments", who);
+  FVR_CHECK_ARG(tmpl_host[0].w == int(px) && tmpl_host[0].h == int(pz) && tmpl_host[1].h == int(pz) &&
+                    tmpl_host[1].w == int(py),
+                "%s: templates do not match the padded dims", who);
+  int tile_h = int((variant >> 8) & 0xFF);
+  if (tile_h == 0) tile_h = 64;
+  const uint32_t flags = (variant >> 4) & 0xF;
+  FVR_CHECK_ARG(tmpl_host[0].tile_h == tile_h && tmpl_host[1].tile_h == tile_h,
+                "%s: template tile_h %d / %d differs from the variant's %d", who, tmpl_host[0].tile_h,
+                tmpl_host[1].tile_h, tile_h);
+  FVR_CHECK_ARG(tmpl_host[0].flags == flags && tmpl_host[1].flags == flags,
+                "%s: template flags %u / %u differ from the variant's %u", who, tmpl_host[0].flags,
+                tmpl_host[1].flags, flags);
+  FVR_CHECK_ARG(flags & FVR_ITEM_ADJOINT, "%s: the biplanar gradient needs the adjoint flag", who);
+  const int64_t tp = pn_tiles_per_pose(tmpl_host);
+  FVR_CHECK_ARG(tp > 0 && int64_t(n_poses) * tp < (int64_t(1) << 30), "%s: %d poses x %lld tiles too many", who,
+                n_poses, (long long)tp);
+  return FVR_OK;
+}
+
+// build -> fused kernel (XCD-contiguous tile order) -> reduce/combine over a workspace at ws
+static int pn_evaluate_run(const fvr_layout* layouts_dev, int32_t n_layouts, const fvr_sncc_item* tmpl_host,
+                           double pz, double py, double px, int32_t lat_swap, uint32_t variant, const double* poses,
+                           int32_t n_poses, double t2_ap, double t2_lat, char* ws, double* moments, double* loss,
+                           double* grad, void* stream) {
+  const PnWs w = pn_ws_layout(tmpl_host, n_poses);
+  fvr_sncc_item* items = reinterpret_cast<fvr_sncc_item*>(ws + w.items);
+  double* jac = reinterpret_cast<double*>(ws + w.jac);
+  float* partials = reinterpret_cast<float*>(ws + w.partials);
+  const int32_t n_tiles = int32_t(int64_t(n_poses) * pn_tiles_per_pose(tmpl_host));
+  int rc = fvr_pn_build_items(poses, n_poses, tmpl_host, pz, py, px, lat_swap, 0, items, jac, stream);
+  if (rc != FVR_OK) return rc;
+  rc = fvr_sncc_eval_plan(layouts_dev, n_layouts, items, 2 * n_poses, n_tiles, partials, variant, nullptr,
+                          8 * ((n_tiles + 7) / 8), 0, stream);
+  if (rc != FVR_OK) return rc;
+  return fvr_pn_reduce_combine(items, n_poses, partials, jac, t2_ap, t2_lat, moments, loss, grad, stream);
+}
+
+}  // namespace fvr
+
+extern "C" {
+
+int64_t fvr_pn_evaluate_workspace(const fvr_sncc_item* tmpl_host, int32_t n_poses) {
+  if (!tmpl_host || n_poses <= 0 || !sncc_tile_h_ok(tmpl_host[0].tile_h) ||
+      tmpl_host[1].tile_h != tmpl_host[0].tile_h || tmpl_host[0].tiles_x <= 0 || tmpl_host[1].tiles_x <= 0)
+    return -1;
+  return pn_ws_layout(tmpl_host, n_poses).total;
+}
+
+int fvr_pn_evaluate(const fvr_layout* layouts_dev, int32_t n_layouts, const fvr_sncc_item* tmpl_host, double pz,
+                    double py, double px, int32_t lat_swap, uint32_t variant, const double* poses, int32_t n_poses,
+                    double t2_ap, double t2_lat, void* workspace, int64_t ws_bytes, double* moments, double* loss,
+                    double* grad, void* stream) {
+  if (pn_evaluate_check("fvr_pn_evaluate", layouts_dev, n_layouts, tmpl_host, pz, py, px, variant, poses, n_poses,
+                        loss, grad) != FVR_OK)
+    return FVR_ERR_INVALID;
+  const int64_t need = pn_ws_layout(tmpl_host, n_poses).total;
+  FVR_CHECK_ARG(workspace && ws_bytes >= need, "fvr_pn_evaluate: workspace of %lld bytes, %lld needed",
+                (long long)ws_bytes, (long long)need);
+  FVR_CHECK_ARG((reinterpret_cast<uintptr_t>(workspace) & 255) == 0, "fvr_pn_evaluate: workspace not 256-byte aligned");
+  return pn_evaluate_run(layouts_dev, n_layouts, tmpl_host, pz, py, px, lat_swap, variant, poses, n_poses, t2_ap,
+                         t2_lat, static_cast<char*>(workspace), moments, loss, grad, stream);
+}
+
 int fvr_pn_step(fvr_sncc_item* items_dev, int32_t n_poses, const float* partials, double* jac_dev, double t2_ap,
                 double t2_lat, double* moments, double* loss, double* grad, double* poses,
                 const fvr_adam_cfg* adam_cfg_host, double* m, double* v, int32_t* t, int32_t* done, double* best_loss,

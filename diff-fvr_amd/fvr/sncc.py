@@ -352,6 +352,31 @@ class BiplanarEvaluator:
                                            _lib.ptr(loss), _lib.ptr(grad), st), "fvr_pn_reduce_combine")
         return loss[:n], grad[: 6 * n].view(n, 6)
 
+    def evaluate_call(self, poses: torch.Tensor):
+        """The same evaluation through the one-call C entry point ``fvr_pn_evaluate`` (set-up,
+        fused kernel in XCD-contiguous order with the plan's tile height and depth, reduce +
+        combine; items, Jacobians and partials in a workspace sized by
+        ``fvr_pn_evaluate_workspace``): (loss (N,), grad (N, 6)) device views."""
+        _lib.require_device(poses, "BiplanarEvaluator.evaluate_call")
+        if poses.dtype != torch.float64 or poses.dim() != 2 or poses.shape[1] != 6 or not poses.is_contiguous():
+            raise ValueError("poses must be a contiguous (N, 6) float64 tensor")
+        n = int(poses.shape[0])
+        L = _lib.lib()
+        nb = int(L.fvr_pn_evaluate_workspace(self.templates.ctypes.data, n))
+        if nb < 0:
+            raise ValueError("fvr_pn_evaluate_workspace rejected the templates")
+        ws = self.ws.get("call_ws", nb, torch.uint8)
+        loss = self.ws.get("call_loss", n, torch.float64)
+        grad = self.ws.get("call_grad", 6 * n, torch.float64)
+        pz, py, px = self.packed.shape
+        plan = LaunchPlan(self.plan.tile_h, "xcd", 0, self.plan.depth, self.plan.exact_box)
+        _lib.check(L.fvr_pn_evaluate(_lib.ptr(self.packed.table), len(self.packed.descs), self.templates.ctypes.data,
+                                     float(pz), float(py), float(px), self.lat_swap,
+                                     plan.variant(self.packed.code, self.flags), _lib.ptr(poses), n, float(self.t2[0]),
+                                     float(self.t2[1]), _lib.ptr(ws), nb, None, _lib.ptr(loss), _lib.ptr(grad),
+                                     _lib.stream_ptr(self.device)), "fvr_pn_evaluate")
+        return loss[:n], grad[: 6 * n].view(n, 6)
+
     # -- two-launch loop form: begin() once, then step() per iteration ----------
     def begin(self, poses: torch.Tensor) -> int:
         """Build the items of ``poses`` (the first launch of a loop of ``step`` calls)."""

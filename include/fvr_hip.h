@@ -212,6 +212,24 @@ int fvr_pn_reduce_combine(const fvr_sncc_item* items_dev, int32_t n_poses, const
                           const double* jac, double t2_ap, double t2_lat, double* moments, double* loss,
                           double* grad, void* stream);
 
+/* Bytes of the workspace fvr_pn_evaluate needs for n_poses poses with these two
+ * templates (items, Jacobians and tile partials, each 256-byte aligned); -1 for bad
+ * templates. */
+int64_t fvr_pn_evaluate_workspace(const fvr_sncc_item* tmpl_host, int32_t n_poses);
+
+/* The batched biplanar loss and exact 6-DoF gradient of n_poses poses in one call
+ * (replaces the per-pose SpectralGradientEngine.compute_biplanar_loss_and_gradients
+ * of gradients.py:76-135 for a batch): fvr_pn_build_items (tile0 = 0), the fused
+ * kernel (variant as in fvr_sncc_eval_plan, XCD-contiguous tile order, no map) and
+ * fvr_pn_reduce_combine, stream-ordered, with the items, Jacobians and tile partials
+ * in the caller's workspace (device, 256-byte aligned, ws_bytes >=
+ * fvr_pn_evaluate_workspace).  The templates' flags and tile_h must be the variant's
+ * (with FVR_ITEM_ADJOINT).  No allocation, no host sync. */
+int fvr_pn_evaluate(const fvr_layout* layouts_dev, int32_t n_layouts, const fvr_sncc_item* tmpl_host,
+                    double pz, double py, double px, int32_t lat_swap, uint32_t variant,
+                    const double* poses, int32_t n_poses, double t2_ap, double t2_lat, void* workspace,
+                    int64_t ws_bytes, double* moments, double* loss, double* grad, void* stream);
+
 /* Batched Adam over 6-DoF poses on the device, one thread per pose, so that a
  * registration loop is a fixed sequence of launches (graph-capturable).  Per
  * pose, with loss / grad evaluated at the current pose (gradient_optimizer.py:45-83;

@@ -320,3 +320,23 @@ def test_graph_loop_recaptures_after_plan_targets_or_workspace_change(vols):
     ev.evaluate_device(_poses(40, 3))  # regrows the workspace the graph was captured over
     check(4)
     check(4)
+
+
+@pytest.mark.parametrize("storage", ["fp16", "f32"])
+def test_one_call_entry_point_equals_three_launches(storage):
+    """fvr_pn_evaluate (one C call: set-up, fused kernel, reduce + combine in a caller
+    workspace) gives the three-launch evaluation's losses and gradients bit for bit, for
+    every tile height and depth, and a second call with fewer poses reuses the workspace."""
+    from fvr.multistart import candidate_plans
+    from fvr.sncc import BiplanarEvaluator
+    r = _renderer(40, 13, storage)
+    ta, tl = _targets(r, np.random.default_rng(13).uniform(-4, 4, 6))
+    ev = BiplanarEvaluator(r.packed, ta, tl, adjoint=True)
+    for plan in [p for p in candidate_plans(ev.flags) if p.order == "xcd" and p.persist == 0]:
+        ev.set_plan(plan)
+        for n, seed in ((11, 3), (5, 4)):
+            P = _poses(n, seed)
+            L0, G0 = (x.cpu().numpy().copy() for x in ev.evaluate_device(P))
+            L1, G1 = (x.cpu().numpy().copy() for x in ev.evaluate_call(P))
+            np.testing.assert_array_equal(L1, L0, err_msg=str(plan))
+            np.testing.assert_array_equal(G1, G0, err_msg=str(plan))

@@ -222,3 +222,38 @@ def test_product_mtre_reference_closed_forms_and_oracle():
             compute_mtre_6dof(o, o, (16,) * 3, spacing=sp)
     with pytest.raises(ValueError):
         compute_mtre_6dof(o, o, (16,) * 3, translation_units="cm")
+
+
+def test_pn_evaluate_workspace_size_and_argument_checks():
+    """fvr_pn_evaluate's workspace query (items | Jacobians | tile partials, 256-byte
+    aligned) and its argument checks, which reject before any launch (no GPU here)."""
+    from fvr import _items, _lib
+    L = _lib.lib()
+    pk = _Packed((48, 48, 48))
+    flags = _lib.ITEM_ADJOINT
+    tm = _lib.as_struct(np.concatenate([_items.n_template(pk, "ap", 1234, flags, 64),
+                                        _items.n_template(pk, "lateral", 5678, flags, 64)]), _lib.ITEM_DTYPE)
+    tiles = sum(int(np.prod(_items.tiles_for(int(t["h"]), int(t["w"]), 64))) for t in tm)
+
+    def a256(b):
+        return (b + 255) // 256 * 256
+
+    for n in (1, 7, 64):
+        exp = a256(2 * n * _lib.ITEM_DTYPE.itemsize) + a256(2 * n * 18 * 8) + a256(n * tiles * 16 * 4)
+        assert L.fvr_pn_evaluate_workspace(tm.ctypes.data, n) == exp
+    assert L.fvr_pn_evaluate_workspace(tm.ctypes.data, 0) == -1
+    assert L.fvr_pn_evaluate_workspace(None, 4) == -1
+    fake = ctypes.c_void_p(1 << 20)  # never dereferenced: every call below fails its checks first
+    var = 1 << 4 | 64 << 8  # f32 storage, adjoint, tile_h 64
+    need = L.fvr_pn_evaluate_workspace(tm.ctypes.data, 4)
+
+    def call(tmpl=tm, variant=var, ws=fake, nbytes=need, n=4, dims=(48.0, 48.0, 48.0)):
+        return L.fvr_pn_evaluate(fake, 2, tmpl.ctypes.data, *dims, 1, variant, fake, n, 1.0, 1.0, ws, nbytes,
+                                 None, fake, fake, None)
+
+    assert call(nbytes=need - 1) == 1 and b"workspace" in L.fvr_last_error()
+    assert call(ws=ctypes.c_void_p((1 << 20) + 64)) == 1 and b"aligned" in L.fvr_last_error()
+    assert call(variant=1 << 4 | 32 << 8) == 1 and b"tile_h" in L.fvr_last_error()
+    assert call(variant=0 | 64 << 8) == 1 and b"flags" in L.fvr_last_error()
+    assert call(n=0) == 1
+    assert call(dims=(48.0, 48.0, 46.0)) == 1 and b"padded dims" in L.fvr_last_error()
