@@ -10,6 +10,7 @@
 // sample (i, j) at c + (j - W/2) u + (i - H/2) v with c = (pz, py, px)/2.
 // Written as straight-line code: no dynamically indexed local arrays.
 #include <cmath>
+#include <mutex>
 
 #include "fvr_common.h"
 
@@ -599,6 +600,40 @@ static int pn_evaluate_run(const fvr_layout* layouts_dev, int32_t n_layouts, con
   return fvr_pn_reduce_combine(items, n_poses, partials, jac, t2_ap, t2_lat, moments, loss, grad, stream);
 }
 
+// The allocation-free variant's workspace: one static array per GPU (each device that
+// loads the library holds its own copy until the process ends).  Host code never uses the
+// array's name as an address: the device address comes from hipGetSymbolAddress, once per
+// device, and is kept in scratch_dev[].
+static __device__ __align__(256) char scratch[FVR_PN_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+static std::mutex scratch_mu;
+static void* scratch_dev[kMaxDevices] = {};
+
+// device address of the current device's scratch array (looked up on its first use)
+static int scratch_address(void** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    set_error("fvr_pn_evaluate_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+    return FVR_ERR_HIP;
+  }
+  FVR_CHECK_ARG(dev >= 0 && dev < kMaxDevices, "fvr_pn_evaluate_noalloc: device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_dev[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("fvr_pn_evaluate_noalloc: hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                hipGetErrorString(e));
+      return FVR_ERR_HIP;
+    }
+    scratch_dev[dev] = p;
+  }
+  *out = scratch_dev[dev];
+  return FVR_OK;
+}
+
 }  // namespace fvr
 
 extern "C" {
@@ -623,6 +658,25 @@ int fvr_pn_evaluate(const fvr_layout* layouts_dev, int32_t n_layouts, const fvr_
   FVR_CHECK_ARG((reinterpret_cast<uintptr_t>(workspace) & 255) == 0, "fvr_pn_evaluate: workspace not 256-byte aligned");
   return pn_evaluate_run(layouts_dev, n_layouts, tmpl_host, pz, py, px, lat_swap, variant, poses, n_poses, t2_ap,
                          t2_lat, static_cast<char*>(workspace), moments, loss, grad, stream);
+}
+
+int fvr_pn_evaluate_noalloc(const fvr_layout* layouts_dev, int32_t n_layouts, const fvr_sncc_item* tmpl_host,
+                            double pz, double py, double px, int32_t lat_swap, uint32_t variant, const double* poses,
+                            int32_t n_poses, double t2_ap, double t2_lat, double* moments, double* loss,
+                            double* grad, void* stream) {
+  if (pn_evaluate_check("fvr_pn_evaluate_noalloc", layouts_dev, n_layouts, tmpl_host, pz, py, px, variant, poses,
+                        n_poses, loss, grad) != FVR_OK)
+    return FVR_ERR_INVALID;
+  const int64_t need = pn_ws_layout(tmpl_host, n_poses).total;
+  FVR_CHECK_ARG(need <= int64_t(FVR_PN_SCRATCH_BYTES),
+                "fvr_pn_evaluate_noalloc: %d poses need %lld bytes of scratch, the static array holds %lld "
+                "(use fvr_pn_evaluate with a workspace, or fewer poses per call)",
+                n_poses, (long long)need, (long long)FVR_PN_SCRATCH_BYTES);
+  void* ws = nullptr;
+  const int rc = scratch_address(&ws);
+  if (rc != FVR_OK) return rc;
+  return pn_evaluate_run(layouts_dev, n_layouts, tmpl_host, pz, py, px, lat_swap, variant, poses, n_poses, t2_ap,
+                         t2_lat, static_cast<char*>(ws), moments, loss, grad, stream);
 }
 
 int fvr_pn_step(fvr_sncc_item* items_dev, int32_t n_poses, const float* partials, double* jac_dev, double t2_ap,

@@ -230,6 +230,22 @@ int fvr_pn_evaluate(const fvr_layout* layouts_dev, int32_t n_layouts, const fvr_
                     const double* poses, int32_t n_poses, double t2_ap, double t2_lat, void* workspace,
                     int64_t ws_bytes, double* moments, double* loss, double* grad, void* stream);
 
+/* fvr_pn_evaluate without a workspace: no counterpart in the reference; it is for
+ * callers that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The items, Jacobians and tile partials live in a
+ * static device array of FVR_PN_SCRATCH_BYTES per GPU (held by every GPU the library
+ * runs on until the process ends); a call whose fvr_pn_evaluate_workspace exceeds it is
+ * rejected (FVR_ERR_INVALID): about 2,400 poses with 512^2-sample views and the
+ * Hermitian fold.  The stream must belong to the current device.  Every call on one GPU
+ * shares that GPU's array (it keeps the last call's contents; each call overwrites all
+ * it reads): calls that overlap on one GPU -- on two streams, or from two threads --
+ * race.  No allocation, no host sync. */
+#define FVR_PN_SCRATCH_BYTES (16 << 20)
+int fvr_pn_evaluate_noalloc(const fvr_layout* layouts_dev, int32_t n_layouts, const fvr_sncc_item* tmpl_host,
+                            double pz, double py, double px, int32_t lat_swap, uint32_t variant,
+                            const double* poses, int32_t n_poses, double t2_ap, double t2_lat,
+                            double* moments, double* loss, double* grad, void* stream);
+
 /* Batched Adam over 6-DoF poses on the device, one thread per pose, so that a
  * registration loop is a fixed sequence of launches (graph-capturable).  Per
  * pose, with loss / grad evaluated at the current pose (gradient_optimizer.py:45-83;

@@ -257,3 +257,22 @@ def test_pn_evaluate_workspace_size_and_argument_checks():
     assert call(variant=0 | 64 << 8) == 1 and b"flags" in L.fvr_last_error()
     assert call(n=0) == 1
     assert call(dims=(48.0, 48.0, 46.0)) == 1 and b"padded dims" in L.fvr_last_error()
+
+
+def test_pn_evaluate_noalloc_rejects_batches_beyond_its_scratch_before_any_device_call():
+    """fvr_pn_evaluate_noalloc's size limit (the static scratch array of
+    FVR_PN_SCRATCH_BYTES): a batch whose fvr_pn_evaluate_workspace exceeds it is refused
+    with a message naming the limit, before any HIP call (no GPU here)."""
+    from fvr import _items, _lib
+    L = _lib.lib()
+    pk = _Packed((48, 48, 48))
+    flags = _lib.ITEM_ADJOINT
+    tm = _lib.as_struct(np.concatenate([_items.n_template(pk, "ap", 1234, flags, 64),
+                                        _items.n_template(pk, "lateral", 5678, flags, 64)]), _lib.ITEM_DTYPE)
+    n = 64
+    while L.fvr_pn_evaluate_workspace(tm.ctypes.data, n) <= (16 << 20):
+        n *= 2
+    fake = ctypes.c_void_p(1 << 20)
+    rc = L.fvr_pn_evaluate_noalloc(fake, 2, tm.ctypes.data, 48.0, 48.0, 48.0, 1, 1 << 4 | 64 << 8, fake, n, 1.0, 1.0,
+                                   None, fake, fake, None)
+    assert rc == 1 and b"scratch" in L.fvr_last_error() and str(16 << 20).encode() in L.fvr_last_error()
