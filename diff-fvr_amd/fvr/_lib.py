@@ -109,6 +109,11 @@ _SIGNATURES = {
                             _P], _I),
     "fvr_pou_ray_grads": ([_P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _I, _I, _F, _F, _P, _P, _P, _P], _I),
     "fvr_pou_sum_blocks": ([_P, _I, _I, _I, _I, _I, _P, _P], _I),
+    "fvr_edge_tiles": ([_I, _I, _I], _I),
+    "fvr_edge_channels": ([_P, _I, _I, _I, _I, _I, _P, _P], _I),
+    "fvr_edge_stats": ([_P, _I, _I, _I, _I, _I, _P, _I, _P, _P], _I),
+    "fvr_edge_combine": ([_P, _I, _P, _I, _I, _I, _I, _I, ctypes.c_double, _P, _P, _P], _I),
+    "fvr_edge_bwd": ([_P, _I, _I, _I, _I, _I, _P, _I, _P, _P, _P, _P], _I),
 }
 
 

@@ -63,43 +63,93 @@ def _sinc2_precomp_1d(n: int, p: int, device=None, dtype=torch.float64) -> torch
     return 1.0 / torch.clamp(torch.sinc(c) ** 2, min=0.05)
 
 
-_SOBEL: Dict = {}
+class _EdgeNCC(torch.autograd.Function):
+    """Per-image edge-dominant gradient NCC (B,) of (B, H, W) float32 images against
+    (Bt, H, W) targets, Bt in {1, B}, on the HIP kernels (``fvr_edge_*``); gradients for
+    both arguments (the value is symmetric in them)."""
+
+    @staticmethod
+    def _moments(img, ref, crop, multi, weight):
+        """(value (B,), coef, channel images of ref) of img against ref."""
+        L = _lib.lib()
+        dev = img.device
+        st = _lib.stream_ptr(dev)
+        B, H, W = img.shape
+        Bt = ref.shape[0]
+        nch = 4 if multi else 2
+        tiles = int(L.fvr_edge_tiles(H, W, crop))
+        ch = torch.empty((Bt, nch, H - 2 * crop, W - 2 * crop), device=dev, dtype=torch.float32)
+        _lib.check(L.fvr_edge_channels(_lib.ptr(ref), Bt, H, W, crop, int(multi), _lib.ptr(ch), st), "fvr_edge_channels")
+        tpart = torch.empty((Bt, tiles, 12), device=dev, dtype=torch.float64)
+        _lib.check(L.fvr_edge_stats(_lib.ptr(ref), Bt, H, W, crop, int(multi), _lib.ptr(ch), Bt, _lib.ptr(tpart), st),
+                   "fvr_edge_stats")
+        part = torch.empty((B, tiles, 12), device=dev, dtype=torch.float64)
+        _lib.check(L.fvr_edge_stats(_lib.ptr(img), B, H, W, crop, int(multi), _lib.ptr(ch), Bt, _lib.ptr(part), st),
+                   "fvr_edge_stats")
+        val = torch.empty(B, device=dev, dtype=torch.float64)
+        coef = torch.empty((B, 4, 4), device=dev, dtype=torch.float32)
+        _lib.check(L.fvr_edge_combine(_lib.ptr(part), B, _lib.ptr(tpart), Bt, H, W, crop, int(multi), float(weight),
+                                      _lib.ptr(val), _lib.ptr(coef), st), "fvr_edge_combine")
+        return val, coef, ch
+
+    @staticmethod
+    def _grad(img, ch, Bt, coef, gval, crop, multi):
+        L = _lib.lib()
+        B, H, W = img.shape
+        g = torch.empty((B, H, W), device=img.device, dtype=torch.float32)
+        _lib.check(L.fvr_edge_bwd(_lib.ptr(img), B, H, W, crop, int(multi), _lib.ptr(ch), Bt, _lib.ptr(coef),
+                                  _lib.ptr(gval), _lib.ptr(g), _lib.stream_ptr(img.device)), "fvr_edge_bwd")
+        return g
+
+    @staticmethod
+    def forward(ctx, pred, target, crop, multi, weight):
+        val, coef, ch = _EdgeNCC._moments(pred, target, crop, multi, weight)
+        ctx.save_for_backward(pred, target, coef, ch)
+        ctx.args = (crop, multi, weight)
+        return val
+
+    @staticmethod
+    def backward(ctx, gval):
+        pred, target, coef, ch = ctx.saved_tensors
+        crop, multi, weight = ctx.args
+        gv = gval.to(torch.float64).contiguous()
+        gp = gt = None
+        if ctx.needs_input_grad[0]:
+            gp = _EdgeNCC._grad(pred, ch, target.shape[0], coef, gv, crop, multi)
+        if ctx.needs_input_grad[1]:
+            # the value is symmetric: d / d target is the same adjoint with the roles swapped,
+            # per pose against that pose's prediction, summed over poses for a shared target
+            tt = target.expand(pred.shape[0], -1, -1).contiguous()
+            _, coef_t, ch_p = _EdgeNCC._moments(tt, pred, crop, multi, weight)
+            gt = _EdgeNCC._grad(tt, ch_p, pred.shape[0], coef_t, gv, crop, multi)
+            if target.shape[0] == 1 and pred.shape[0] > 1:
+                gt = gt.sum(0, keepdim=True)
+        return gp, gt, None, None, None
 
 
 def edge_dominant_ncc(pred: torch.Tensor, target: torch.Tensor, border_crop: Optional[int] = None,
                       multiscale_weight: float = 0.25, reduction: str = "mean") -> torch.Tensor:
-    """Gradient-NCC over interior Sobel gradients (+ 3x3-smoothed scale) (cone_beam_fvr.py:71-112)."""
-    H = pred.shape[-1]
-    crop = max(4, H // 8) if border_crop is None else int(border_crop)
-
-    def kern(dev, dt):
-        key = (str(dev), dt)
-        if key not in _SOBEL:
-            kx = torch.tensor([[-1.0, 0.0, 1.0], [-2.0, 0.0, 2.0], [-1.0, 0.0, 1.0]], dtype=dt, device=dev)
-            _SOBEL[key] = (kx.view(1, 1, 3, 3), kx.t().contiguous().view(1, 1, 3, 3))
-        return _SOBEL[key]
-
-    def grad_ncc(a, b):
-        kx, ky = kern(a.device, a.dtype)
-        sl = slice(crop, -crop if crop > 0 else None)
-        ax, ay = F.conv2d(a, kx, padding=1)[..., sl, sl], F.conv2d(a, ky, padding=1)[..., sl, sl]
-        bx, by = F.conv2d(b, kx, padding=1)[..., sl, sl], F.conv2d(b, ky, padding=1)[..., sl, sl]
-
-        def ncc(u, v):
-            u0 = u - u.mean(dim=(-2, -1), keepdim=True)
-            v0 = v - v.mean(dim=(-2, -1), keepdim=True)
-            num = (u0 * v0).sum(dim=(-2, -1))
-            den = (u0.square().sum(dim=(-2, -1)) * v0.square().sum(dim=(-2, -1))).clamp(min=1e-12).sqrt()
-            val = (num / den).mean(dim=1)
-            return val.mean() if reduction == "mean" else val
-
-        return 0.5 * (ncc(ax, bx) + ncc(ay, by))
-
-    fine = grad_ncc(pred, target)
-    if multiscale_weight <= 0.0 or H < 96:
-        return fine
-    coarse = grad_ncc(F.avg_pool2d(pred, 3, 1, 1), F.avg_pool2d(target, 3, 1, 1))
-    return (1.0 - multiscale_weight) * fine + multiscale_weight * coarse
+    """Gradient NCC over interior Sobel gradients, plus the 3 x 3-smoothed scale
+    (cone_beam_fvr.py:71-112), on the HIP kernels ``fvr_edge_*``: pred (B, 1, H, W), target
+    (B or 1, 1, H, W) on a ROCm device; crop = max(4, W // 8) unless given (W = the last
+    axis, as the reference reads it); the blurred term for W >= 96 and a positive weight.
+    reduction "mean" gives a scalar, anything else the (B,) per-image values.  Computed in
+    float32 with float64 moments; the result and the gradients are in pred's dtype."""
+    if pred.dim() != 4 or target.dim() != 4 or pred.shape[1] != 1 or target.shape[1] != 1:
+        raise ValueError(f"edge_dominant_ncc: (B, 1, H, W) images expected, got {tuple(pred.shape)} and "
+                         f"{tuple(target.shape)}")
+    if target.shape[-2:] != pred.shape[-2:] or target.shape[0] not in (1, pred.shape[0]):
+        raise ValueError(f"edge_dominant_ncc: target {tuple(target.shape)} does not match pred {tuple(pred.shape)}")
+    _lib.require_device(pred, "edge_dominant_ncc")
+    B, _, H, W = pred.shape
+    crop = max(4, W // 8) if border_crop is None else int(border_crop)
+    if crop < 0 or H - 2 * crop <= 0 or W - 2 * crop <= 0:
+        raise ValueError(f"edge_dominant_ncc: crop {crop} leaves nothing of a {H} x {W} image")
+    multi = multiscale_weight > 0.0 and W >= 96
+    p = pred.to(torch.float32).reshape(B, H, W).contiguous()
+    t = target.to(device=pred.device, dtype=torch.float32).reshape(target.shape[0], H, W).contiguous()
+    val = _EdgeNCC.apply(p, t, crop, bool(multi), float(multiscale_weight)).to(pred.dtype)
+    return val.mean() if reduction == "mean" else val
 
 
 def sample_isocenter_parity_offsets(vox_spacing) -> torch.Tensor:

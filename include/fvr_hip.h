@@ -32,6 +32,8 @@
  *                       u_det, v_det, d_b, cu, cv, fused per (block, pose).
  *   fvr_pou_ray_grads   the ray-array gradients grid_sample gives autograd in
  *                       cone_beam_fvr.py:423-478 _project_core.
+ *   fvr_edge_*          cone_beam_fvr.py:71-112 edge_dominant_ncc (forward moments,
+ *                       combine, and the gradient w.r.t. the images).
  */
 #ifndef FVR_HIP_H
 #define FVR_HIP_H
@@ -384,6 +386,33 @@ int fvr_pou_ray_grads(const float* imgs, int32_t B, int32_t N, int32_t P, const 
 /* out[n][c] = sum_b part[b][n][c0 + c] in float64, fixed order (c < nc). */
 int fvr_pou_sum_blocks(const float* part, int32_t B, int32_t N, int32_t C, int32_t c0, int32_t nc,
                        float* out, void* stream);
+
+/* Edge-dominant gradient NCC (cone_beam_fvr.py:71-112 edge_dominant_ncc) of n_img float32
+ * images (H x W, row-major, one channel): Sobel x / y (zero padded) over the crop
+ * [crop, H - crop) x [crop, W - crop), NCC per channel with the crop mean removed and
+ * denominator sqrt(max(Eu Ev, 1e-12)); with multi = 1 also Sobel x / y of the 3 x 3
+ * box-blurred image (zero padding, divisor 9), weighted (1 - w, w) against the plain pair.
+ * Tiles of the crop: 32 x 32 (fvr_edge_tiles; -1 for a shape the crop leaves empty).
+ *   fvr_edge_channels: channels[n][2 or 4][H - 2 crop][W - 2 crop] (the target's)
+ *   fvr_edge_stats:    partials[n][tiles][12] float64 = (sum u, sum u^2, sum u v) per
+ *                      channel against the channel images of image n (n_ch_img = n_img)
+ *                      or of one shared image (n_ch_img = 1)
+ *   fvr_edge_combine:  value[n] and coef[n][4][4] = (alpha, beta, mean u, mean v) per
+ *                      channel: d value / d u = alpha (v - mean v) - beta (u - mean u);
+ *                      target_partials: fvr_edge_stats of the targets against themselves
+ *   fvr_edge_bwd:      grad[n][H][W] = grad_value[n] x d value[n] / d image n
+ * Deterministic (fixed-order sums, no atomics). */
+int32_t fvr_edge_tiles(int32_t H, int32_t W, int32_t crop);
+int fvr_edge_channels(const float* img, int32_t n_img, int32_t H, int32_t W, int32_t crop, int32_t multi,
+                      float* channels, void* stream);
+int fvr_edge_stats(const float* img, int32_t n_img, int32_t H, int32_t W, int32_t crop, int32_t multi,
+                   const float* channels, int32_t n_ch_img, double* partials, void* stream);
+int fvr_edge_combine(const double* partials, int32_t n_img, const double* target_partials, int32_t n_target,
+                     int32_t H, int32_t W, int32_t crop, int32_t multi, double weight, double* value,
+                     float* coef, void* stream);
+int fvr_edge_bwd(const float* img, int32_t n_img, int32_t H, int32_t W, int32_t crop, int32_t multi,
+                 const float* channels, int32_t n_ch_img, const float* coef, const double* grad_value,
+                 float* grad, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* misc                                                                      */

@@ -31,6 +31,15 @@ How each function is pinned (``tests/test_oracle_pou.py``):
 * ``warp_sum``         -- the explicit 4-neighbour stencil equals
                           ``grid_sample`` (float64, align_corners=True, zeros),
                           the relation of ``test_pou_pinhole_warp_sum_analytical_parity``.
+* ``correlate3`` (+adjoint) -- ``scipy.ndimage.correlate`` (zero padding) and
+                          torch ``conv2d`` / ``avg_pool2d`` in float64; adjoints by
+                          <K p, g> = <p, K^T g>.
+* ``edge_dominant_ncc`` -- the reference's vignetting known answers
+                          (``test_edge_dominant_ncc_vignetting_invariance``);
+                          NCC(p, p) = 1, symmetry, invariance to a t + b (a > 0),
+                          sign flip, batch mean, the blurred term's weighting and
+                          its W >= 96 switch, rows inside the crop not counting;
+                          gradient by FP64 directional differences.
 * ``forward_pinhole`` / ``project_rays_batched`` -- SE(3)-invariant lattice path
                           equals the explicit-ray path
                           (``test_invariant_pinhole_matches_explicit_ray_tracer``);
@@ -343,3 +352,87 @@ def ncc_loss(a: np.ndarray, b: np.ndarray) -> float:
     """1 - NCC of zero-mean images (the reference test's loss, tests/test_cone_beam_fvr.py:24-27)."""
     a0, b0 = a - a.mean(), b - b.mean()
     return 1.0 - float(np.sum(a0 * b0) / (np.linalg.norm(a0) * np.linalg.norm(b0) + 1e-8))
+
+
+# ---------------------------------------------------------------------------
+# edge-dominant gradient NCC (cone_beam_fvr.py:23-39, 71-112)
+# ---------------------------------------------------------------------------
+SOBEL_X = np.array([[-1.0, 0.0, 1.0], [-2.0, 0.0, 2.0], [-1.0, 0.0, 1.0]])
+SOBEL_Y = SOBEL_X.T.copy()
+
+
+def correlate3(img: np.ndarray, k: np.ndarray) -> np.ndarray:
+    """3 x 3 cross-correlation with zero padding over the last two axes (F.conv2d, padding=1)."""
+    H, W = img.shape[-2:]
+    p = np.pad(img, [(0, 0)] * (img.ndim - 2) + [(1, 1), (1, 1)])
+    out = np.zeros(img.shape)
+    for dy in (-1, 0, 1):
+        for dx in (-1, 0, 1):
+            if k[dy + 1, dx + 1] != 0.0:
+                out += k[dy + 1, dx + 1] * p[..., 1 + dy:1 + dy + H, 1 + dx:1 + dx + W]
+    return out
+
+
+def correlate3_adjoint(g: np.ndarray, k: np.ndarray) -> np.ndarray:
+    """Adjoint of ``correlate3``: p_bar(q) = sum_k K(k) g(q - k) over q - k in the image."""
+    H, W = g.shape[-2:]
+    p = np.pad(g, [(0, 0)] * (g.ndim - 2) + [(1, 1), (1, 1)])
+    out = np.zeros(g.shape)
+    for dy in (-1, 0, 1):
+        for dx in (-1, 0, 1):
+            if k[dy + 1, dx + 1] != 0.0:
+                out += k[dy + 1, dx + 1] * p[..., 1 - dy:1 - dy + H, 1 - dx:1 - dx + W]
+    return out
+
+
+BOX3 = np.full((3, 3), 1.0 / 9.0)
+
+
+def edge_dominant_ncc(pred, target, border_crop=None, multiscale_weight=0.25, reduction="mean", with_grad=False):
+    """Gradient NCC of (B, C, H, W) images (cone_beam_fvr.py:71-112): Sobel x / y (zero padded),
+    cropped by ``crop = max(4, W // 8)`` on every side (W = the last axis, as the reference
+    reads it), NCC per channel with the mean over the crop removed and the denominator
+    sqrt(max(Eu Ev, 1e-12)), the mean of x and y; for W >= 96 and a positive weight, (1 - w)
+    times that plus w times the same on the 3 x 3 box-blurred images (zero padding, divisor
+    9).  Channel mean, then the batch mean for reduction "mean".  ``with_grad``: also d(sum of
+    the per-pose values) / d pred, or d(mean) / d pred for "mean" (exact adjoint)."""
+    p = np.asarray(pred, np.float64)
+    t = np.broadcast_to(np.asarray(target, np.float64), p.shape)
+    W = p.shape[-1]
+    crop = max(4, W // 8) if border_crop is None else int(border_crop)
+    sl = (Ellipsis, slice(crop, -crop if crop > 0 else None), slice(crop, -crop if crop > 0 else None))
+
+    def grad_ncc(a, b):
+        val = 0.0
+        grads = []
+        for k in (SOBEL_X, SOBEL_Y):
+            u, v = correlate3(a, k)[sl], correlate3(b, k)[sl]
+            u0 = u - u.mean(axis=(-2, -1), keepdims=True)
+            v0 = v - v.mean(axis=(-2, -1), keepdims=True)
+            A = (u0 * v0).sum(axis=(-2, -1))
+            Eu, Ev = (u0 * u0).sum(axis=(-2, -1)), (v0 * v0).sum(axis=(-2, -1))
+            EE = Eu * Ev
+            D = np.sqrt(np.maximum(EE, 1e-12))
+            ncc = A / D
+            val = val + 0.5 * ncc.mean(axis=1)
+            if with_grad:  # d ncc / d u = v0 / D - (A / (D Eu)) u0 where Eu Ev > 1e-12, else v0 / D
+                beta = np.where(EE > 1e-12, A / (D * np.where(Eu > 0, Eu, 1.0)), 0.0)
+                gu = v0 / D[..., None, None] - beta[..., None, None] * u0
+                full = np.zeros(a.shape)
+                full[sl] = 0.5 * gu / a.shape[1]  # channel mean
+                grads.append(correlate3_adjoint(full, k))
+        return val, (sum(grads) if with_grad else None)
+
+    fine, g_fine = grad_ncc(p, t)
+    if multiscale_weight <= 0.0 or W < 96:
+        val, g = fine, g_fine
+    else:
+        w = float(multiscale_weight)
+        coarse, g_c = grad_ncc(correlate3(p, BOX3), correlate3(t, BOX3))
+        val = (1.0 - w) * fine + w * coarse
+        if with_grad:
+            g = (1.0 - w) * g_fine + w * correlate3_adjoint(g_c, BOX3)
+    out = float(np.mean(val)) if reduction == "mean" else val
+    if not with_grad:
+        return out
+    return out, (g / p.shape[0] if reduction == "mean" else g)

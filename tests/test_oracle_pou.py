@@ -305,3 +305,86 @@ def test_pinhole_projection_against_brute_force_ray_sums():
     moved = dict(m, centers=m["centers"] + np.array([1.0, 0.0, 0.0]))
     out = C.forward_pinhole(moved, mats, lat)[:, 0]
     assert max(np.abs(centroid(out[k]) - centroid(brute[k])).max() for k in range(3)) > 0.3
+
+
+# ---------------------------------------------------------------------------
+# edge-dominant gradient NCC
+# ---------------------------------------------------------------------------
+def test_edge_stencils_equal_library_routines_and_adjoints():
+    """Sobel x / y and the 3 x 3 box as scipy.ndimage.correlate (zero padding) and
+    torch's conv2d / avg_pool2d (padding 1, count_include_pad) in float64; the adjoints
+    by <K p, g> = <p, K^T g>."""
+    from scipy.ndimage import correlate
+    rng = np.random.default_rng(3)
+    img = rng.standard_normal((2, 1, 13, 17))
+    for k in (C.SOBEL_X, C.SOBEL_Y, C.BOX3):
+        ref = np.stack([correlate(img[b, 0], k, mode="constant", cval=0.0) for b in range(2)])[:, None]
+        np.testing.assert_allclose(C.correlate3(img, k), ref, atol=1e-13)
+        g = rng.standard_normal(img.shape)
+        assert abs(np.sum(C.correlate3(img, k) * g) - np.sum(img * C.correlate3_adjoint(g, k))) < 1e-10
+    t = torch.from_numpy(img)
+    kx = torch.tensor(C.SOBEL_X).view(1, 1, 3, 3)
+    np.testing.assert_allclose(C.correlate3(img, C.SOBEL_X), torch.nn.functional.conv2d(t, kx, padding=1).numpy(),
+                               atol=1e-13)
+    np.testing.assert_allclose(C.correlate3(img, C.BOX3), torch.nn.functional.avg_pool2d(t, 3, 1, 1).numpy(),
+                               atol=1e-13)
+
+
+def _vignette_case(H=96):
+    """The reference's vignetting case (tests/test_cone_beam_fvr.py:95-107), float64."""
+    yy, xx = np.meshgrid(np.linspace(-1, 1, H), np.linspace(-1, 1, H), indexing="ij")
+    img = np.exp(-((xx - 0.1) ** 2 + (yy + 0.05) ** 2) / 0.04) + 0.5 * ((xx ** 2 + yy ** 2) < 0.25)
+    ref = img[None, None]
+    vig = ref + 0.8 * xx[None, None]
+    vig[:, :, :6, :] = -10.0
+    vig[:, :, -6:, :] = -10.0
+    shifted = np.roll(ref, (4, 4), axis=(-2, -1))
+    return ref, vig, shifted
+
+
+def test_edge_ncc_reference_known_answers_and_invariants():
+    ref, vig, shifted = _vignette_case()
+    assert C.edge_dominant_ncc(ref, ref) > 0.999
+    assert C.edge_dominant_ncc(ref, vig) > 0.98
+    assert C.edge_dominant_ncc(shifted, vig) < 0.50
+    rng = np.random.default_rng(5)
+    p = rng.standard_normal((3, 1, 100, 104)).cumsum(-1)  # rough images with edges
+    t = rng.standard_normal((3, 1, 100, 104)).cumsum(-2)
+    both = C.edge_dominant_ncc(p, t, reduction="none")
+    assert both.shape == (3,)
+    assert abs(C.edge_dominant_ncc(p, p) - 1.0) < 1e-12
+    np.testing.assert_allclose(C.edge_dominant_ncc(p, 2.5 * t + 7.0, reduction="none"), both, atol=1e-12)  # affine
+    np.testing.assert_allclose(C.edge_dominant_ncc(t, p, reduction="none"), both, atol=1e-12)  # symmetric
+    np.testing.assert_allclose(C.edge_dominant_ncc(p, -t, reduction="none"), -both, atol=1e-12)
+    assert abs(C.edge_dominant_ncc(p, t) - both.mean()) < 1e-15
+    # the blurred term: (1 - w) fine + w coarse for W >= 96, the fine term alone below
+    fine = C.edge_dominant_ncc(p, t, multiscale_weight=0.0, reduction="none")
+    w1 = C.edge_dominant_ncc(p, t, multiscale_weight=1.0, reduction="none")
+    np.testing.assert_allclose(both, 0.75 * fine + 0.25 * w1, atol=1e-12)
+    assert not np.allclose(fine, w1)
+    q, s = p[..., :95], t[..., :95]
+    np.testing.assert_allclose(C.edge_dominant_ncc(q, s, reduction="none"),
+                               C.edge_dominant_ncc(q, s, multiscale_weight=0.0, reduction="none"), atol=0)
+    # the crop: rows / columns within crop of the border do not count (W // 8 = 13 here)
+    p2 = p.copy()
+    p2[..., :11, :] = rng.standard_normal(p2[..., :11, :].shape) * 50  # 11 = 13 - 2 (the blur + Sobel reach)
+    np.testing.assert_allclose(C.edge_dominant_ncc(p2, t, reduction="none"), both, atol=1e-12)
+
+
+@pytest.mark.parametrize("W,crop", [(40, None), (104, None), (100, 0)])
+def test_edge_ncc_gradient_matches_directional_differences(W, crop):
+    """The oracle's exact gradient w.r.t. the predicted images against FP64 central
+    differences along random directions (both reductions; crop 0 reaches the zero
+    padding at the border; W = 104 includes the blurred term)."""
+    rng = np.random.default_rng(W)
+    p = rng.standard_normal((2, 1, 36, W)).cumsum(-1)
+    t = rng.standard_normal((2, 1, 36, W)).cumsum(-2)
+    for red in ("none", "mean"):
+        val, g = C.edge_dominant_ncc(p, t, border_crop=crop, reduction=red, with_grad=True)
+        for _ in range(3):
+            d = rng.standard_normal(p.shape)
+            h = 1e-5
+            fp = np.sum(C.edge_dominant_ncc(p + h * d, t, border_crop=crop, reduction=red))
+            fm = np.sum(C.edge_dominant_ncc(p - h * d, t, border_crop=crop, reduction=red))
+            fd = (fp - fm) / (2 * h)
+            assert abs(fd - np.sum(g * d)) <= 1e-7 * max(1.0, abs(fd)), (red, fd, np.sum(g * d))
