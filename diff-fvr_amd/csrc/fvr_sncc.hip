@@ -570,7 +570,7 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
   // moment weights (cw, rw) and box weights (wp, wq) of the sample and its mirror
   // (interior: every lane valid, wp = wq = 1; column 0's mirror part of the folded target is 0)
   auto accumulate = [&](const SmpH<DT>& sm, float phr, float phi, float cw, float rw, float wp, float wq,
-                        float wEi, auto interior) {
+                        float wEi, auto interior, auto unit_we) {
     const float ts = sm.ts, tm = sm.tm, tf = sm.tf;
     const Raw<DT>& q = sm.raw;
     float e00r, e00i, e01r, e01i, e10r, e10i, e11r, e11i, d00r, d00i, d01r, d01i, d10r, d10i, d11r, d11i;
@@ -599,7 +599,10 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
     }
     const float Wr = fmaf(phr, Tr, phi * Ti);
     const float Wi = fmaf(phi, Tr, -phr * Ti);
-    const float Jr = Ir * wE, Ji = Ii * wE;
+    // unit_we: the E weight is a per-lane power of two applied to E and the H moments after
+    // the row loop (bit-identical: scaling by 1 or 2 commutes with every rounding here)
+    const float Jr = decltype(unit_we)::value ? Ir : Ir * wE;
+    const float Ji = decltype(unit_we)::value ? Ii : Ii * wE;
     const float IWr = fmaf(Ir, Wr, -Ii * Wi);
     const float IWi = fmaf(Ir, Wi, Ii * Wr);
     ac.A += IWr;
@@ -629,13 +632,14 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
   };
 
   // stage 2 of step t (steps in order: rw is stepped)
-  auto consume = [&](const SmpH<DT>& sm, int t, auto interior) {
+  auto consume = [&](const SmpH<DT>& sm, int t, auto interior, auto unit_we) {
     const float prr = readlane_f(rowph.r, 2 * t), pri = readlane_f(rowph.i, 2 * t);
     const float phr = colph.r * prr - colph.i * pri;
     const float phi = fmaf(colph.r, pri, colph.i * prr);
     const float rw = rwr;
     rwr += rws;
-    accumulate(sm, phr, phi, cwj, rw, sm.bp ? 1.f : 0.f, sm.bq ? 1.f : 0.f, t == 0 ? wEh : wEl, interior);
+    accumulate(sm, phr, phi, cwj, rw, sm.bp ? 1.f : 0.f, sm.bq ? 1.f : 0.f,
+               decltype(unit_we)::value ? 1.f : (t == 0 ? wEh : wEl), interior, unit_we);
   };
 
   // The unpaired samples outside the rows h/2 .. h-1, one extra sample per lane of the first
@@ -654,7 +658,7 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
   auto extra_consume = [&]() {
     const cpx ph = phasor(it.ph_col0 + uint32_t(xj) * it.ph_col + it.ph_row0 + uint32_t(xi) * it.ph_row);
     accumulate(xs, ph.r, ph.i, it.cw0 + float(xj) * it.cws, it.rw0 + float(xi) * it.rws, xs.bp ? 1.f : 0.f,
-               0.f, 0.f, std::false_type{});
+               0.f, 0.f, std::false_type{}, std::false_type{});
   };
 
   // Class of a full wave (every step of every lane a regular row).  A position is an exact
@@ -679,7 +683,7 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
   // DEPTH samples in flight per lane: the loads of row t + DEPTH - 1 are issued before
   // row t is consumed (ring of DEPTH buffers, indices fixed by the unrolled inner loop;
   // every bound below is wave-uniform)
-  auto run = [&](auto interior) {
+  auto run = [&](auto interior, auto unit_we) {
     SmpH<DT> ring[DEPTH];
 #pragma unroll
     for (int d = 0; d < DEPTH - 1; ++d)
@@ -691,16 +695,29 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
         const int t = t0 + d;
         if (t < n_r) {
           if (t + DEPTH - 1 < n_r) issue(ring[(d + DEPTH - 1) % DEPTH], t + DEPTH - 1, interior);
-          consume(ring[d], t, interior);
+          consume(ring[d], t, interior, unit_we);
           __builtin_amdgcn_sched_barrier(0);
         }
       }
     }
   };
 
-  // per-lane order of the sums: the steps, then the extra sample (first wave)
-  if (cls == 1) run(std::true_type{});
-  else if (cls == 0) run(std::false_type{});
+  // per-lane order of the sums: the steps, then the extra sample (first wave).  An interior
+  // wave other than the first has one E weight per lane (wEl: 2, or 1 in column 0), applied
+  // to E and the H moments once after its rows.
+  if (cls == 1 && !first) {
+    run(std::true_type{}, std::true_type{});
+    ac.E *= wEl;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      ac.hs[a] *= wEl;
+      ac.hr[a] *= wEl;
+    }
+  } else if (cls == 1) {
+    run(std::true_type{}, std::false_type{});
+  } else if (cls == 0) {
+    run(std::false_type{}, std::false_type{});
+  }
   if (first) {
     extra_issue();
     extra_consume();

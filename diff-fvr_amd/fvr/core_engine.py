@@ -19,6 +19,7 @@ import torch
 from scipy.optimize import minimize
 
 from .diff_fvr import DifferentiableFourierVolumeRenderer
+from .powell import batched_powell
 from .sncc import BiplanarEvaluator
 
 
@@ -83,8 +84,15 @@ class RobustDiffFVREngine:
     def register_biplanar_robust(self, target_ap: np.ndarray, target_lat: np.ndarray,
                                  initial_pose=(0.0, 0.0, 0.0, 0.0, 0.0, 0.0), handle_metal: bool = True,
                                  max_iter: int = 50, bounds_trans_mm: float = 25.0,
-                                 bounds_rot_deg: float = 20.0) -> Dict[str, Any]:
-        """Powell over Spectral-NCC (clean) or masked spatial NCC (metal), poses in (deg, mm)."""
+                                 bounds_rot_deg: float = 20.0, optimizer: str = "powell") -> Dict[str, Any]:
+        """Powell over Spectral-NCC (clean) or masked spatial NCC (metal), poses in (deg, mm).
+
+        ``optimizer="powell"`` is the reference's scipy Powell (one pose per evaluation);
+        ``"batched_powell"`` runs ``fvr.powell.batched_powell`` on the clean objective: every
+        line-search round evaluates a grid of poses along the direction in one fused
+        loss-only launch (the metal-masked objective always uses scipy's Powell)."""
+        if optimizer not in ("powell", "batched_powell"):
+            raise ValueError(f"optimizer must be 'powell' or 'batched_powell', got {optimizer!r}")
         for name, target in (("AP", target_ap), ("lateral", target_lat)):
             if target.shape != self.volume.shape[:2] or not np.all(np.isfinite(target)):
                 raise ValueError(f"{name} target must be finite with shape {self.volume.shape[:2]}; "
@@ -138,6 +146,25 @@ class RobustDiffFVREngine:
                 loss, _ = ev.evaluate(pose)
                 return float(loss[0])
 
+            if optimizer == "batched_powell":
+                def loss_batch(P):
+                    nonlocal eval_count
+                    Q = np.array(P, dtype=np.float64).reshape(-1, 6)
+                    eval_count += len(Q)
+                    Q[:, 3:] /= sp
+                    return ev.evaluate(Q)[0]
+
+                res = batched_powell(loss_batch, np.array(initial_pose), bounds, max_iter=max_iter, ftol=1e-4)
+                return {
+                    "success": bool(res["success"]),
+                    "final_pose": res["x"],
+                    "final_loss": float(res["fun"]),
+                    "evaluations": eval_count,
+                    "launches": res["calls"],
+                    "metal_detected": metal_present,
+                    "time_sec": time.time() - t0,
+                }
+
         res = minimize(objective, x0=np.array(initial_pose), method="Powell", bounds=bounds,
                        options={"maxiter": max_iter, "ftol": 1e-4})
         return {
@@ -145,6 +172,7 @@ class RobustDiffFVREngine:
             "final_pose": res.x,
             "final_loss": float(res.fun),
             "evaluations": eval_count,
+            "launches": eval_count,
             "metal_detected": metal_present,
             "time_sec": time.time() - t0,
         }

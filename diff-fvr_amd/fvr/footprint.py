@@ -92,3 +92,49 @@ def batch_algorithmic_bytes(pad_shape, poses: np.ndarray, elem_bytes: int, scale
     spec = total_u * elem_bytes + (4 if scaled else 0)
     return {"unique_entries": total_u, "spectrum_bytes": spec, "target_bytes": targ,
             "bytes": spec + targ, "samples": total_s, "schedule": "hermitian-fold" if herm else "full-slice"}
+
+
+def pou_slice_footprint(P: int, u_xyz, v_xyz, device="cpu") -> int:
+    """Distinct block-spectrum entries one PoU slab-image slice reads (the same for every
+    block of a pose): the P x P samples P/2 + (j - P/2) u + (i - P/2) v of the per-corner
+    trilinear stencil (cone_beam_fvr.py:480-505; u, v in (x, y, z) storage order), corners
+    inside [0, P - 1]^3 with a non-zero weight."""
+    k = torch.arange(P, dtype=torch.float64, device=device) - P // 2
+    I, J = torch.meshgrid(k, k, indexing="ij")
+    u = torch.as_tensor(np.ascontiguousarray(u_xyz, np.float64), device=device)
+    v = torch.as_tensor(np.ascontiguousarray(v_xyz, np.float64), device=device)
+    p = (P // 2) + J.reshape(-1, 1) * u + I.reshape(-1, 1) * v  # (K, 3) in (x, y, z)
+    i0 = torch.floor(p).to(torch.int64)
+    t = p - i0
+    keys = []
+    for dz in (0, 1):
+        for dy in (0, 1):
+            for dx in (0, 1):
+                d = torch.tensor([dx, dy, dz], device=device)
+                wgt = ((t[:, 0] if dx else 1 - t[:, 0]) * (t[:, 1] if dy else 1 - t[:, 1])
+                       * (t[:, 2] if dz else 1 - t[:, 2]))
+                idx = i0 + d
+                ok = (wgt > 0) & ((idx >= 0) & (idx <= P - 1)).all(dim=1)
+                idx = idx[ok]
+                keys.append((idx[:, 2] * P + idx[:, 1]) * P + idx[:, 0])
+    return int(torch.unique(torch.cat(keys)).numel())
+
+
+def pou_step_bytes(B: int, P: int, K: int, u_xyz: np.ndarray, v_xyz: np.ndarray, elem_bytes: int,
+                   device="cpu") -> dict:
+    """Algorithmic bytes of one PoU forward + adjoint step over N poses (u, v: (N, 3) in
+    (x, y, z)), each byte counted once per kernel that must move it:
+      slice + inverse FFT   U elements per (block, pose) read, the P^2 float32 slab image written;
+      warp                  slab images read, the K-pixel float32 projection written;
+      fused adjoint         U elements re-read, slab images read, the K-pixel output gradient read.
+    The ray cache (3 K floats) is shared by every pose and counted once per kernel."""
+    U = [pou_slice_footprint(P, u, v, device) for u, v in zip(u_xyz, v_xyz)]
+    N = len(U)
+    gather = sum(U) * B * elem_bytes
+    img = N * B * P * P * 4
+    rays = 3 * K * 4
+    fwd = gather + img
+    warp = img + N * K * 4 + rays
+    adj = gather + img + N * K * 4 + rays
+    return {"unique_entries_per_pose_mean": float(np.mean(U)), "slice_ifft": fwd, "warp_fwd": warp,
+            "render_bwd": adj, "step": fwd + warp + adj}
