@@ -29,7 +29,9 @@
 namespace fvr {
 
 constexpr int kPouThreads = 512;
-constexpr int kPouSliceThreads = 1024;  // the forward slice + iFFT (no reduction buffer in LDS)
+constexpr int kPouSliceThreads = 1024;  // the fused adjoint (the full-size LDS image: one workgroup per CU)
+constexpr int kPouHalfThreads = 512;    // the forward slice + iFFT (half-spectrum LDS: two workgroups per CU)
+constexpr int kPouPairBatch = 2;        // sample pairs in flight per thread in the forward gather
 #ifndef FVR_POU_BATCH
 #define FVR_POU_BATCH 4
 #endif
@@ -228,6 +230,82 @@ __device__ __forceinline__ void make_twiddles(float2* tw, int P) {
   }
 }
 
+// Batched 1D decimation-in-frequency FFTs of length L = 2^logL over nlines lines of an LDS
+// complex array: element i of line l at a[l * lpitch + i * epitch]; natural-order input,
+// bit-reversed output, unnormalised; SIGN = +1 inverse, -1 forward.  Twiddles of length L
+// are tw[e * twstep] with tw[k] = exp(-2 pi i k / P) (twstep = P / L).  Radix-4 passes as in
+// fft2d_dif4 (an odd logL ends with one radix-2 pass).  COLS: consecutive threads take
+// consecutive lines (lines adjacent in memory); otherwise consecutive groups of one line.
+template <int SIGN, bool COLS>
+__device__ void fft_lines_dif4(float2* a, int nlines, int L, int logL, int lpitch, int epitch, const float2* tw,
+                               int twstep) {
+  auto twid = [&](int e) {
+    float2 w = tw[e * twstep];
+    if (SIGN > 0) w.y = -w.y;
+    return w;
+  };
+  auto bfly = [&](float2& x0, float2& x1, float2 w) {  // x0 <- x0 + x1, x1 <- (x0 - x1) w
+    const float dr = x0.x - x1.x, di = x0.y - x1.y;
+    x0 = make_float2(x0.x + x1.x, x0.y + x1.y);
+    x1 = make_float2(dr * w.x - di * w.y, dr * w.y + di * w.x);
+  };
+  int s = logL;
+#pragma unroll 1
+  for (; s >= 2; s -= 2) {
+    const int q = 1 << (s - 2), ts = L >> s, gpl = L >> 2;
+    const int ng = nlines * gpl;
+    for (int idx = threadIdx.x; idx < ng; idx += blockDim.x) {
+      int line, g;
+      if (COLS) {
+        g = idx / nlines;
+        line = idx - g * nlines;
+      } else {
+        line = idx >> (logL - 2);
+        g = idx & (gpl - 1);
+      }
+      const int k = g & (q - 1);
+      const int i0 = ((g >> (s - 2)) << s) + k;
+      float2* base = a + line * lpitch;
+      float2& r0 = base[i0 * epitch];
+      float2& r1 = base[(i0 + q) * epitch];
+      float2& r2 = base[(i0 + 2 * q) * epitch];
+      float2& r3 = base[(i0 + 3 * q) * epitch];
+      float2 x0 = r0, x1 = r1, x2 = r2, x3 = r3;
+      bfly(x0, x2, twid(k * ts));
+      bfly(x1, x3, twid((k + q) * ts));
+      const float2 w2 = twid(2 * k * ts);
+      bfly(x0, x1, w2);
+      bfly(x2, x3, w2);
+      r0 = x0;
+      r1 = x1;
+      r2 = x2;
+      r3 = x3;
+    }
+    __syncthreads();
+  }
+  if (s == 1) {  // the last radix-2 stage (twiddle 1)
+    const int hl = L >> 1, ng = nlines * hl;
+    for (int idx = threadIdx.x; idx < ng; idx += blockDim.x) {
+      int line, k;
+      if (COLS) {
+        k = idx / nlines;
+        line = idx - k * nlines;
+      } else {
+        line = idx >> (logL - 1);
+        k = idx & (hl - 1);
+      }
+      float2* base = a + line * lpitch;
+      float2& r0 = base[(2 * k) * epitch];
+      float2& r1 = base[(2 * k + 1) * epitch];
+      float2 x0 = r0, x1 = r1;
+      bfly(x0, x1, make_float2(1.f, 0.f));
+      r0 = x0;
+      r1 = x1;
+    }
+    __syncthreads();
+  }
+}
+
 // trilinear value (and gradient) from the 8 corners (c = x + 2 y + 4 z) and the fractions
 template <bool GRAD>
 __device__ __forceinline__ void interp8(const float (&cr)[8], const float (&ci)[8], float tx, float ty, float tz,
@@ -289,15 +367,37 @@ __device__ __forceinline__ void pk_issue_batch(PkSmp<VT> (&s)[kPouBatch], unsign
   }
 }
 
+// slice sample (row i, column jj) of the half plane, jj in [0, W) with W = 2^logW: every 64
+// consecutive q form a 2-row x 32-column patch (W >= 32), else row-major
+__device__ __forceinline__ void pou_ij_w(int q, int logW, int& i, int& jj) {
+  if (logW >= 5) {
+    const int w = q >> 6, lane = q & 63, sh = logW - 5;
+    i = 2 * (w >> sh) + (lane >> 5);
+    jj = 32 * (w & ((1 << sh) - 1)) + (lane & 31);
+  } else {
+    i = q >> logW;
+    jj = q & ((1 << logW) - 1);
+  }
+}
+
+// Real slab image of one (block, pose) through a Hermitian half spectrum.  The image is
+// Re(fftshift(ifft2(ifftshift(S)))) = fftshift(ifft2(H)) with H(k) = (S(k) + conj S(-k)) / 2
+// (indices mod P), a Hermitian array, so it is kept for the columns v in [0, P/2] only: each
+// thread gathers a sample and its mirror and writes H into LDS (P x (P/2 + 1) complex); the
+// columns take inverse FFTs of length P; each row of the result, Y(v), is Hermitian in v and
+// its real transform comes from one complex FFT of length P/2 of
+//   Z(k) = (Y(k) + conj Y(P/2 - k)) + i (Y(k) - conj Y(P/2 - k)) e^{2 pi i k / P},
+// whose outputs z(n) = x(2n) + i x(2n + 1) are the even and odd image samples.  Half the
+// LDS and half the transform work of the full complex 2D transform.
 template <int VT>
-__global__ __launch_bounds__(kPouSliceThreads) void pou_slice_ifft_kernel(const char* __restrict__ Fp, PouSlice g,
-                                                                          PouPk k, const float* __restrict__ u,
-                                                                          const float* __restrict__ v, int per_xcd,
-                                                                          float* __restrict__ imgs) {
+__global__ __launch_bounds__(kPouHalfThreads) void pou_slice_ifft_kernel(const char* __restrict__ Fp, PouSlice g,
+                                                                         PouPk k, const float* __restrict__ u,
+                                                                         const float* __restrict__ v, int per_xcd,
+                                                                         float* __restrict__ imgs) {
   extern __shared__ float2 smem[];
-  const int P = g.P, logP = g.logP, pitch = P + 1, hp = P / 2;
-  float2* a = smem;
-  float2* tw = smem + P * pitch;
+  const int P = g.P, logP = g.logP, hp = P / 2, pitch = hp + 1;
+  float2* a = smem;               // H, then Y, Z, z: P rows x (P/2 + 1) columns
+  float2* tw = smem + P * pitch;  // exp(-2 pi i k / P), k < P/2
   const int bn = pou_bn(blockIdx.x, per_xcd);
   if (bn >= g.B * g.N) return;  // whole workgroup: padding slot of the XCD split
   const int b = bn / g.N, n = bn - b * g.N;
@@ -305,30 +405,74 @@ __global__ __launch_bounds__(kPouSliceThreads) void pou_slice_ifft_kernel(const 
   const float ux = u[3 * n], uy = u[3 * n + 1], uz = u[3 * n + 2];
   const float vx = v[3 * n], vy = v[3 * n + 1], vz = v[3 * n + 2];
   const unsigned long long blk = reinterpret_cast<unsigned long long>(Fp) + (unsigned long long)b * k.block;
-  const int nq = P * P, stride = blockDim.x;
-  for (int q0 = threadIdx.x; q0 < nq; q0 += kPouBatch * stride) {
-    PkSmp<VT> smp[kPouBatch];
-    pk_issue_batch<VT>(smp, blk, k, g, q0, stride, ux, uy, uz, vx, vy, vz);
+  const int stride = blockDim.x;
+  // sample (i, j) sits at c + (j - P/2) u + (i - P/2) v; its mirror (-k) is ((P - i) mod P, (P - j) mod P)
+  auto issue_pair = [&](PkSmp<VT>& s0, PkSmp<VT>& s1, int i, int j) {
+    const float kx = float(j - hp), ky = float(i - hp);
+    const int im = (P - i) & (P - 1), jm = (P - j) & (P - 1);
+    const float mx = float(jm - hp), my = float(im - hp);
+    pk_issue<VT>(s0, blk, k, P, fmaf(ky, vx, fmaf(kx, ux, g.c)), fmaf(ky, vy, fmaf(kx, uy, g.c)),
+                 fmaf(ky, vz, fmaf(kx, uz, g.c)));
+    pk_issue<VT>(s1, blk, k, P, fmaf(my, vx, fmaf(mx, ux, g.c)), fmaf(my, vy, fmaf(mx, uy, g.c)),
+                 fmaf(my, vz, fmaf(mx, uz, g.c)));
+  };
+  auto store_pair = [&](const PkSmp<VT>& s0, const PkSmp<VT>& s1, int i, int vcol) {
+    float cr[8], ci[8], r0, i0, r1, i1;
+    pk_corners<VT>(s0, cr, ci);
+    interp8<false>(cr, ci, s0.tx, s0.ty, s0.tz, r0, i0, nullptr, nullptr);
+    pk_corners<VT>(s1, cr, ci);
+    interp8<false>(cr, ci, s1.tx, s1.ty, s1.tz, r1, i1, nullptr, nullptr);
+    a[((i + hp) & (P - 1)) * pitch + vcol] = make_float2(0.5f * (r0 + r1), 0.5f * (i0 - i1));
+  };
+  // columns v = jj in [0, P/2): samples j = P/2 + jj and their mirrors
+  const int nq = P * hp, logH = logP - 1;
+  for (int q0 = threadIdx.x; q0 < nq; q0 += kPouPairBatch * stride) {
+    PkSmp<VT> s0[kPouPairBatch], s1[kPouPairBatch];
+    int ii[kPouPairBatch], jj[kPouPairBatch];
 #pragma unroll
-    for (int t = 0; t < kPouBatch; ++t) {
-      const int q = q0 + t * stride;
-      float cr[8], ci[8], re, im;
-      pk_corners<VT>(smp[t], cr, ci);
-      interp8<false>(cr, ci, smp[t].tx, smp[t].ty, smp[t].tz, re, im, nullptr, nullptr);
-      int i, j;
-      pou_ij(q, logP, i, j);
-      // ifftshift: frequency row i goes to natural index (i + P/2) mod P
-      if (q < nq) a[((i + hp) & (P - 1)) * pitch + ((j + hp) & (P - 1))] = make_float2(re, im);
+    for (int t = 0; t < kPouPairBatch; ++t) {
+      pou_ij_w(min(q0 + t * stride, nq - 1), logH, ii[t], jj[t]);
+      issue_pair(s0[t], s1[t], ii[t], hp + jj[t]);
     }
+#pragma unroll
+    for (int t = 0; t < kPouPairBatch; ++t)
+      if (q0 + t * stride < nq) store_pair(s0[t], s1[t], ii[t], jj[t]);
+  }
+  // column v = P/2: samples j = 0 (frequency -P/2) and their mirrors
+  for (int i = threadIdx.x; i < P; i += stride) {
+    PkSmp<VT> s0, s1;
+    issue_pair(s0, s1, i, 0);
+    store_pair(s0, s1, i, hp);
   }
   __syncthreads();
-  fft2d_dif4<+1>(a, P, logP, pitch, tw);
+  // inverse FFTs of the P/2 + 1 columns (length P): Y at rows brev(m)
+  fft_lines_dif4<+1, true>(a, pitch, P, logP, 1, pitch, tw, 1);
+  // even / odd pre-pass per row: pairs (kk, P/2 - kk), kk in [0, P/4]
+  const int qp = P >> 2, npair = P * (qp + 1);
+  for (int idx = threadIdx.x; idx < npair; idx += stride) {
+    const int r = idx / (qp + 1), kk = idx - r * (qp + 1);
+    float2* row = a + r * pitch;
+    const float2 y0 = row[kk], y1 = row[hp - kk];
+    auto zk = [&](float2 ya, float2 yb, int e) {  // Z(e) from Y(e), Y(P/2 - e)
+      const float er = ya.x + yb.x, ei = ya.y - yb.y;  // E = Y(e) + conj Y(P/2 - e)
+      const float dr = ya.x - yb.x, di = ya.y + yb.y;  // Y(e) - conj Y(P/2 - e)
+      const float2 w = tw[e];                          // exp(-2 pi i e / P); its conjugate is used
+      const float orr = dr * w.x + di * w.y, oi = di * w.x - dr * w.y;
+      return make_float2(er - oi, ei + orr);  // E + i O
+    };
+    row[kk] = zk(y0, y1, kk);
+    if (kk != 0 && kk != hp - kk) row[hp - kk] = zk(y1, y0, hp - kk);
+  }
+  __syncthreads();
+  // inverse FFTs of the rows (length P/2): z(n) = x(2n) + i x(2n + 1) at column brev(n)
+  fft_lines_dif4<+1, false>(a, P, hp, logH, pitch, 1, tw, 2);
   const float inv = 1.f / float(P * P);
   float* out = imgs + size_t(bn) * P * P;
-  for (int q = threadIdx.x; q < P * P; q += blockDim.x) {
+  for (int q = threadIdx.x; q < P * P; q += stride) {
     const int i = q >> logP, j = q & (P - 1);
-    // fftshift of the natural-order result, read at its bit-reversed position
-    out[q] = a[brev((i + hp) & (P - 1), logP) * pitch + brev((j + hp) & (P - 1), logP)].x * inv;
+    const int m = (i + hp) & (P - 1), nn = (j + hp) & (P - 1);  // fftshift
+    const float2 z = a[brev(m, logP) * pitch + brev(nn >> 1, logH)];
+    out[q] = ((nn & 1) ? z.y : z.x) * inv;
   }
 }
 
@@ -683,6 +827,7 @@ __global__ __launch_bounds__(256) void pou_pack_kernel(const void* __restrict__ 
 }
 
 static size_t slice_lds_bytes(int P) { return (size_t(P) * (P + 1) + P / 2) * sizeof(float2); }
+static size_t half_lds_bytes(int P) { return (size_t(P) * (P / 2 + 1) + P / 2) * sizeof(float2); }
 
 __global__ void pou_zero_u64_kernel(unsigned long long* __restrict__ p, long long n) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -738,13 +883,13 @@ int fvr_pou_slice_ifft(uint64_t F, int32_t dtype, int32_t B, int32_t P, const fl
   FVR_CHECK_ARG(int64_t(B) * N < (int64_t(1) << 30), "fvr_pou_slice_ifft: too many (block, pose) pairs");
   PouSlice g{B, N, P, 31 - __builtin_clz(uint32_t(P)), float(P / 2)};
   const PouPk k = pou_pk(P, dtype);
-  const size_t lds = slice_lds_bytes(P);
+  const size_t lds = half_lds_bytes(P);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   const char* fp = reinterpret_cast<const char*>(F);
   const int per_xcd = int((int64_t(B) * N + 7) / 8);
 #define FVR_SI(VT)                                                                                            \
   FVR_CHECK_ARG(allow_lds(pou_slice_ifft_kernel<VT>, lds), "fvr_pou_slice_ifft: %zu B of LDS refused", lds); \
-  hipLaunchKernelGGL(pou_slice_ifft_kernel<VT>, dim3(8 * per_xcd), dim3(kPouSliceThreads), lds, st, fp, g, k, \
+  hipLaunchKernelGGL(pou_slice_ifft_kernel<VT>, dim3(8 * per_xcd), dim3(kPouHalfThreads), lds, st, fp, g, k, \
                      u_xyz, v_xyz, per_xcd, imgs)
   switch (dtype) {
     case 0: FVR_SI(0); break;

@@ -82,6 +82,33 @@ def test_fused_slab_images_match_oracle(model):
     assert np.max(np.abs(gen - ref)) <= 1e-4 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("P", [4, 8, 16, 32, 64])
+def test_half_spectrum_slab_images_match_oracle_at_every_lds_size(P):
+    """The fused slice + inverse FFT (Hermitian half spectrum: a sample and its mirror per
+    stored entry, column inverse FFTs, the even / odd pre-pass, row inverse FFTs of length
+    P/2) against the oracle's full complex transform, for every power-of-two P the LDS kernel
+    takes below 128 (128: the full-size test), float32 and bfloat16 block spectra."""
+    from fvr import PerspectiveSlabFVR
+    n = max(8, P)
+    vol = np.zeros((n, n, n), dtype=np.float32)
+    zz, yy, xx = np.ogrid[:n, :n, :n]
+    vol[((zz - n * 0.45) ** 2 + (yy - n * 0.55) ** 2 + (xx - n * 0.5) ** 2) < (0.3 * n) ** 2] = 0.08
+    vol[n // 3:n // 2, n // 5:n // 3, n // 2:2 * n // 3] += 0.05
+    o = C.from_volume(vol.astype(np.float64), 2.0, (2, 2, 2), P // 2)
+    rng = np.random.default_rng(P)
+    Q = np.linalg.qr(rng.standard_normal((4, 3, 3)))[0]
+    u, v = Q[:, :, 0], Q[:, :, 1]
+    u[0], v[0] = [0.0, 0.0, 1.0], [0.0, 1.0, 0.0]  # axis-aligned: samples on lattice points
+    ref = C.slab_images(o, u, v)
+    for dt, tol in ((torch.float32, 1e-4), (torch.bfloat16, 2e-2)):
+        m = PerspectiveSlabFVR.from_volume(vol, vox_mm=2.0, grid_blocks=(2, 2, 2), block_res=P // 2, dtype=dt)
+        assert m.P == P and m.fused_ok()
+        imgs = m._slice_and_ifft(torch.from_numpy(u).float(), torch.from_numpy(v).float()).cpu().numpy()
+        assert imgs.shape == ref.shape
+        err = np.abs(imgs - ref).max() / np.abs(ref).max()
+        assert err <= tol, (P, dt, err)
+
+
 def test_warp_forward_and_gradients_match_grid_sample_autograd():
     from fvr import pou_pinhole_warp_sum
     from test_oracle_pou import _warp_torch_f64, warp_case
