@@ -668,15 +668,70 @@ __global__ __launch_bounds__(kPouSliceThreads) void pou_render_bwd_kernel(
   warp_adjoint_scan<true>(imgs + size_t(bn) * P * P, go, rx, ry, ic, g, n, d_b[bn], cu[bn], cv[bn], scale, acc,
                           pitch, true, s3);
   __syncthreads();
-  // 2) fixed point -> complex (real part), in place
+  // 2) fixed point -> float, repacked in place for a real-input transform: row y holds the
+  // P / 2 complex values z(n) = g(y, 2n) + i g(y, 2n + 1) at pitch P / 2 + 1 (every value
+  // is read into registers before any is written: the packed rows overlap the 64-bit ones)
   const float inv_scale = 1.f / scale;
-  for (int i = threadIdx.x; i < P * pitch; i += blockDim.x) {
-    const long long x = (long long)acc[i];
-    a[i] = make_float2(float(double(x)) * inv_scale, 0.f);
+  const int hpitch = hp + 1, logH = logP - 1;
+  constexpr int kConv = (128 * 128) / (2 * kPouSliceThreads);  // pairs per thread at P = 128
+  float2 zv[kConv];
+#pragma unroll
+  for (int t = 0; t < kConv; ++t) {
+    const int idx = threadIdx.x + t * blockDim.x;  // pair (y, n)
+    if (idx < P * hp) {
+      const int y = idx >> logH, n2 = idx & (hp - 1);
+      zv[t] = make_float2(float(double((long long)acc[y * pitch + 2 * n2])) * inv_scale,
+                          float(double((long long)acc[y * pitch + 2 * n2 + 1])) * inv_scale);
+    }
   }
   __syncthreads();
-  // 3) adjoint of fftshift(ifft2(ifftshift(.))).real: fftshift(fft2(ifftshift(g))) / P^2
-  fft2d_dif4<-1>(a, P, logP, pitch, tw);
+#pragma unroll
+  for (int t = 0; t < kConv; ++t) {
+    const int idx = threadIdx.x + t * blockDim.x;
+    if (idx < P * hp) a[(idx >> logH) * hpitch + (idx & (hp - 1))] = zv[t];
+  }
+  __syncthreads();
+  // 3) adjoint of fftshift(ifft2(ifftshift(.))).real: fftshift(fft2(ifftshift(g))) / P^2, of a
+  // real g: forward FFTs of length P / 2 of the rows' z, then per row
+  //   X(k) = (Z(k) + conj Z(P/2 - k)) / 2 - (i / 2) e^{-2 pi i k / P} (Z(k) - conj Z(P/2 - k)),
+  // k in [0, P/2] (Z periodic in P / 2), the row transform of g for the half spectrum; then
+  // forward FFTs of length P of the P / 2 + 1 columns (the other half is conj X(-k))
+  fft_lines_dif4<-1, false>(a, P, hp, logH, hpitch, 1, tw, 2);
+  {
+    const int qp = P >> 2, npair = P * (qp + 1);
+    constexpr int kPost = (128 * (128 / 4 + 1) + kPouSliceThreads - 1) / kPouSliceThreads;
+    float2 xa[kPost], xb[kPost];
+#pragma unroll
+    for (int t = 0; t < kPost; ++t) {
+      const int idx = threadIdx.x + t * blockDim.x;
+      if (idx < npair) {
+        const int r = idx / (qp + 1), kk = idx - r * (qp + 1);
+        const float2* row = a + r * hpitch;
+        const float2 z0 = row[brev(kk, logH)], z1 = row[brev((hp - kk) & (hp - 1), logH)];
+        auto xk = [&](float2 za, float2 zb, float2 w) {  // X from Z(k), Z(P/2 - k), w = e^{-2 pi i k / P}
+          const float er = 0.5f * (za.x + zb.x), ei = 0.5f * (za.y - zb.y);  // (Z + conj Z') / 2
+          const float dr = 0.5f * (za.x - zb.x), di = 0.5f * (za.y + zb.y);  // (Z - conj Z') / 2
+          const float pr = dr * w.x - di * w.y, pi = dr * w.y + di * w.x;    // w (Z - conj Z') / 2
+          return make_float2(er + pi, ei - pr);                             // E - i p
+        };
+        xa[t] = xk(z0, z1, tw[kk]);
+        xb[t] = kk == 0 ? xk(z0, z1, make_float2(-1.f, 0.f)) : xk(z1, z0, tw[hp - kk]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < kPost; ++t) {
+      const int idx = threadIdx.x + t * blockDim.x;
+      if (idx < npair) {
+        const int r = idx / (qp + 1), kk = idx - r * (qp + 1);
+        float2* row = a + r * hpitch;
+        row[kk] = xa[t];
+        row[hp - kk] = xb[t];  // kk = 0: X(P/2); kk = P/4: the same entry as X(kk)
+      }
+    }
+    __syncthreads();
+  }
+  fft_lines_dif4<-1, true>(a, hpitch, P, logP, 1, hpitch, tw, 1);
   // 4) recompute every slice sample and contract its spatial gradient with the slice gradient
   const float ux = u[3 * n], uy = u[3 * n + 1], uz = u[3 * n + 2];
   const float vx = v[3 * n], vy = v[3 * n + 1], vz = v[3 * n + 2];
@@ -694,8 +749,11 @@ __global__ __launch_bounds__(kPouSliceThreads) void pou_render_bwd_kernel(
       int i, j;
       pou_ij(q, logP, i, j);
       const float kx = float(j - hp), ky = float(i - hp);
-      const float2 gs = a[brev((i + hp) & (P - 1), logP) * pitch + brev((j + hp) & (P - 1), logP)];
-      const float gr = gs.x * inv, gi = gs.y * inv;
+      // G at frequency (uu, vv) (natural order): stored for vv <= P/2, else conj G(-uu, -vv)
+      const int uu = (i + hp) & (P - 1), vv = (j + hp) & (P - 1);
+      const bool lo = vv <= hp;
+      const float2 gs = a[brev(lo ? uu : (P - uu) & (P - 1), logP) * hpitch + (lo ? vv : P - vv)];
+      const float gr = gs.x * inv, gi = (lo ? gs.y : -gs.y) * inv;
       float cr[8], ci[8], re, im, dre[3], dim[3];
       pk_corners<VT>(smp[t], cr, ci);
       interp8<true>(cr, ci, smp[t].tx, smp[t].ty, smp[t].tz, re, im, dre, dim);
