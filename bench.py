@@ -253,11 +253,12 @@ def pou_config4(steps: int, warmup: int, size: int = 256, n_pose: int = 64):
             model.forward_pinhole(cands, lattice)
         torch.cuda.synchronize()
         ms_fwd = (time.perf_counter() - t1) * 1e3 / steps
-    from fvr.footprint import pou_step_bytes
+    from fvr.footprint import pou_step_bytes, pou_texels_read
     R = cands[:, :3, :3].double().cpu().numpy()
     u_det = np.einsum("nij,j->ni", R, lattice[2].double().cpu().numpy())[:, ::-1]  # (x, y, z) storage order
     v_det = np.einsum("nij,j->ni", R, lattice[3].double().cpu().numpy())[:, ::-1]
-    fp = pou_step_bytes(model.B, model.P, det * det, u_det, v_det, 4 if model.F_blk_ri.dtype != torch.float32 else 8)
+    fp = pou_step_bytes(model.B, model.P, det * det, u_det, v_det, 4 if model.F_blk_ri.dtype != torch.float32 else 8,
+                        texels_read=pou_texels_read(model, cands, lattice))
     gbs = fp["step"] / (ms * 1e-3) / 1e9
     return {"workload": "PoU-FVR 256^3, 4x4x4 blocks @64 (P=128, bf16), pinhole SDD 1020/SAD 510, 256^2 det @0.8mm",
             "poses": n_pose, "blocks": model.B, "P": model.P, "ms_fwd_adj": round(ms, 3), "ms_fwd": round(ms_fwd, 3),
@@ -267,6 +268,7 @@ def pou_config4(steps: int, warmup: int, size: int = 256, n_pose: int = 64):
                          "frac": round(gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes_per_step": fp["step"],
                          "bytes_by_kernel": {k: fp[k] for k in ("slice_ifft", "warp_fwd", "render_bwd")},
                          "unique_entries_per_pose_slice": fp["unique_entries_per_pose_mean"],
+                         "slab_texels_read": fp["slab_texels_read"],
                          "note": "whole forward + adjoint step (slice + LDS iFFT, warp, fused adjoint) over 64 poses"}}
 
 

@@ -120,21 +120,50 @@ def pou_slice_footprint(P: int, u_xyz, v_xyz, device="cpu") -> int:
     return int(torch.unique(torch.cat(keys)).numel())
 
 
+def pou_texels_mask(model, pose_mats, lattice):
+    """(B, N, P, P) bool: the slab-image texels that the pinhole warp reads with a non-zero
+    bilinear weight, and the warp's arguments (d_b, cu, cv, rx, ry, ic).  The image gradient
+    of ``pou_pinhole_warp_sum`` for an all-ones output gradient is positive exactly there
+    (every bilinear weight and 1/cos is >= 0).  With a 256^2 detector at 0.4 mm per pixel at
+    the isocentre only part of each block's P x P image falls inside the field of view."""
+    from .cone_beam_fvr import pou_pinhole_warp_sum
+    dev = model.F_blk_ri.device
+    R, t = pose_mats[:, :3, :3].to(dev), pose_mats[:, :3, 3].to(dev)
+    src0, _, u0, v0, n0 = (x.to(dev) for x in lattice[:5])
+    srcs = torch.einsum("nij,j->ni", R, src0) + t
+    u_det, v_det, n_0 = (torch.einsum("nij,j->ni", R, a) for a in (u0, v0, n0))
+    d_b, cu, cv = (x.detach() for x in model._compute_block_offsets(srcs, u_det, v_det, n_0))
+    rx, ry, ic = (x.to(dev) for x in lattice[5:8])
+    N, P = int(R.shape[0]), int(model.P)
+    imgs = torch.ones((model.B, N, P, P), device=dev, dtype=torch.float32, requires_grad=True)
+    out = pou_pinhole_warp_sum(imgs, d_b, cu, cv, rx, ry, ic, inv_ds=1.0 / model.ds_block, c_idx=float(P // 2))
+    out.backward(torch.ones_like(out))
+    return imgs.grad > 0, (d_b, cu, cv, rx, ry, ic)
+
+
+def pou_texels_read(model, pose_mats, lattice) -> int:
+    """Number of texels in ``pou_texels_mask``, summed over (block, pose)."""
+    return int(pou_texels_mask(model, pose_mats, lattice)[0].sum())
+
+
 def pou_step_bytes(B: int, P: int, K: int, u_xyz: np.ndarray, v_xyz: np.ndarray, elem_bytes: int,
-                   device="cpu") -> dict:
+                   texels_read: int = None, device="cpu") -> dict:
     """Algorithmic bytes of one PoU forward + adjoint step over N poses (u, v: (N, 3) in
     (x, y, z)), each byte counted once per kernel that must move it:
-      slice + inverse FFT   U elements per (block, pose) read, the P^2 float32 slab image written;
-      warp                  slab images read, the K-pixel float32 projection written;
-      fused adjoint         U elements re-read, slab images read, the K-pixel output gradient read.
-    The ray cache (3 K floats) is shared by every pose and counted once per kernel."""
+      slice + inverse FFT   U elements per (block, pose) read (the inverse FFT needs the whole
+                            slice), the slab-image texels the warp reads written;
+      warp                  those texels read, the K-pixel float32 projection written;
+      fused adjoint         U elements re-read, those texels read, the K-pixel output gradient read.
+    ``texels_read`` (float32 texels over all (block, pose), from ``pou_texels_read``); None
+    counts every texel of the P x P images.  The ray cache (3 K floats) is shared by every
+    pose and counted once per kernel."""
     U = [pou_slice_footprint(P, u, v, device) for u, v in zip(u_xyz, v_xyz)]
     N = len(U)
     gather = sum(U) * B * elem_bytes
-    img = N * B * P * P * 4
+    img = (N * B * P * P if texels_read is None else int(texels_read)) * 4
     rays = 3 * K * 4
     fwd = gather + img
     warp = img + N * K * 4 + rays
     adj = gather + img + N * K * 4 + rays
-    return {"unique_entries_per_pose_mean": float(np.mean(U)), "slice_ifft": fwd, "warp_fwd": warp,
-            "render_bwd": adj, "step": fwd + warp + adj}
+    return {"unique_entries_per_pose_mean": float(np.mean(U)), "slab_texels_read": img // 4,
+            "slice_ifft": fwd, "warp_fwd": warp, "render_bwd": adj, "step": fwd + warp + adj}

@@ -147,6 +147,47 @@ def test_256cube_gradient_matches_fp64_central_differences(headline):
     assert err <= 2e-3, (G[k], g_fd, err)
 
 
+@pytest.mark.parametrize("storage", ["fp16", "bf16"])
+def test_512cube_one_scale_storage_equals_per_brick_quantisation(headline, storage):
+    """The 512^3 spectrum stored with one power-of-two scale holds the numbers that a
+    power-of-two scale per 8^3 brick of the storage lattice holds (max|component| / scale
+    in [2^14, 2^15), round to nearest even, quantised on the device from the same
+    complex64 values): bit for bit wherever the one-scale FP16 code is a normal number,
+    and within FP16's subnormal step (2^-24 x scale) elsewhere.  BF16 has FP32's exponent
+    range, so every element is equal.  Prints the subnormal fraction and its energy share."""
+    r = headline["r"]
+    if r.storage != "f32":
+        r.repack("f32")
+    name = "zyx"
+    d = r.packed.descs[r.packed.index[name]]
+    sd = tuple(int(v) for v in d["sdim"])
+    x = r.packed.storage[name].view(torch.float32).reshape(sd + (2,)).clone()
+    r.repack(storage)
+    sc = float(r.packed.scales[name][0])
+    raw = r.packed.storage[name].view(torch.float16 if storage == "fp16" else torch.bfloat16).reshape(sd + (2,))
+    got = raw.double() * sc
+    nb = [(n + 7) // 8 for n in sd]
+    pad = torch.zeros((8 * nb[0], 8 * nb[1], 8 * nb[2]), dtype=torch.float32, device=DEV)
+    pad[:sd[0], :sd[1], :sd[2]] = x.abs().amax(-1)
+    bmax = pad.view(nb[0], 8, nb[1], 8, nb[2], 8).amax(dim=(1, 3, 5)).double()
+    e = torch.where(bmax > 0, torch.floor(torch.log2(torch.where(bmax > 0, bmax, 1.0))) - 14, 0.0)
+    del pad, bmax
+    ix = [torch.arange(n, device=DEV) // 8 for n in sd]
+    s_b = torch.exp2(e)[ix[0]][:, ix[1]][:, :, ix[2]].float()[..., None]
+    q = x / s_b  # exact: a power-of-two divisor, no underflow at these magnitudes
+    ref = (q.half() if storage == "fp16" else q.bfloat16()).double() * s_b
+    del q
+    normal = (x.abs() / sc >= 2.0 ** -14) if storage == "fp16" else torch.ones_like(x, dtype=torch.bool)
+    frac = 1.0 - float(normal.double().mean())
+    x2 = x.double() ** 2
+    energy = float(x2[~normal].sum() / x2.sum())
+    print(f"[full-size] 512^3 {storage}: scale 2^{int(np.log2(sc))}, subnormal fraction {frac:.3e}, "
+          f"their energy share {energy:.3e}")
+    assert torch.equal(got[normal], ref[normal])
+    sub = (got - ref).abs()[~normal]
+    assert sub.numel() == 0 or float(sub.max()) <= 2.0 ** -24 * sc
+
+
 # ---------------------------------------------------------------------------
 # PoU-FVR: 256^3, 4 x 4 x 4 blocks, P = 128, 256^2 pinhole detector, 64 poses
 # ---------------------------------------------------------------------------

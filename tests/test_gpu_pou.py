@@ -135,6 +135,28 @@ def test_warp_forward_and_gradients_match_grid_sample_autograd():
         assert torch.equal(a.grad, b.grad)
 
 
+def test_pou_texels_mask_covers_every_weighted_texel(model):
+    """fvr.footprint.pou_texels_mask (the slab-image texels the PoU step's algorithmic byte
+    count charges to the warp and the adjoint): zeroing every texel outside it leaves the
+    warp's output unchanged bit for bit, so it covers every texel the warp weights; here the
+    detector sees part of the volume, so it is a strict, non-empty subset of the images."""
+    from fvr import pou_pinhole_warp_sum
+    from fvr.footprint import pou_texels_mask, pou_texels_read
+    m, _ = model
+    lat, _ = _lattice()
+    mats = torch.from_numpy(MATS).float().to(DEV)
+    mask, (d_b, cu, cv, rx, ry, ic) = pou_texels_mask(m, mats, lat)
+    assert mask.shape == (m.B, len(MATS), m.P, m.P)
+    assert 0 < int(mask.sum()) < mask.numel()
+    assert pou_texels_read(m, mats, lat) == int(mask.sum())
+    imgs = torch.randn(mask.shape, device=DEV, generator=torch.Generator(device=DEV).manual_seed(5))
+    kw = dict(inv_ds=1.0 / m.ds_block, c_idx=float(m.P // 2))
+    full = pou_pinhole_warp_sum(imgs, d_b, cu, cv, rx, ry, ic, **kw)
+    kept = pou_pinhole_warp_sum(imgs * mask, d_b, cu, cv, rx, ry, ic, **kw)
+    assert torch.equal(full, kept)
+    assert not torch.equal(full, pou_pinhole_warp_sum(imgs * ~mask, d_b, cu, cv, rx, ry, ic, **kw))
+
+
 def test_warp_image_gradient_tiny_and_non_finite_upstream_gradients():
     """The fixed-point scatter of the image gradient keeps its relative precision for tiny
     upstream gradients (1e-30: the scale is capped at 2^126, not infinite) and passes NaN

@@ -182,6 +182,37 @@ def test_item_byte_image_keeps_the_c_layout():
     assert _lib.struct_bytes(packed, _lib.ITEM_DTYPE).size == 400
 
 
+def test_pou_slice_footprint_closed_form_and_brute_force():
+    """fvr.footprint.pou_slice_footprint: an axis-aligned slab-image slice samples lattice
+    points only (one corner with weight 1 each: U = P^2); at a tilted pose the count equals
+    a sample-by-sample loop over the trilinear corners inside [0, P - 1]^3; and
+    pou_step_bytes adds the kernels' bytes as its docstring states."""
+    from fvr.footprint import pou_slice_footprint, pou_step_bytes
+    P = 8
+    assert pou_slice_footprint(P, [1.0, 0.0, 0.0], [0.0, 1.0, 0.0]) == P * P
+    Q = np.linalg.qr(np.random.default_rng(3).standard_normal((3, 3)))[0]
+    u, v = Q[:, 0], Q[:, 1]
+    keys = set()
+    for i in range(P):
+        for j in range(P):
+            p = P // 2 + (j - P // 2) * u + (i - P // 2) * v
+            i0 = np.floor(p).astype(int)
+            t = p - i0
+            for d in np.ndindex(2, 2, 2):
+                w = np.prod([t[a] if d[a] else 1.0 - t[a] for a in range(3)])
+                c = i0 + np.array(d)
+                if w > 0 and np.all((c >= 0) & (c <= P - 1)):
+                    keys.add(tuple(c))
+    U = pou_slice_footprint(P, u, v)
+    assert U == len(keys) and P * P < U < 8 * P * P
+    B, K, tex = 3, 100, 1234
+    fb = pou_step_bytes(B, P, K, np.stack([u, u]), np.stack([v, v]), 4, texels_read=tex)
+    g, img, rays, out = 2 * U * B * 4, tex * 4, 3 * K * 4, 2 * K * 4
+    assert (fb["slice_ifft"], fb["warp_fwd"], fb["render_bwd"]) == (g + img, img + out + rays, g + img + out + rays)
+    assert fb["step"] == 2 * g + 3 * img + 2 * out + 2 * rays and fb["slab_texels_read"] == tex
+    assert pou_step_bytes(B, P, K, np.stack([u]), np.stack([v]), 4)["slab_texels_read"] == B * P * P
+
+
 def test_hermitian_footprint_is_about_half():
     """The fold schedule touches about half of the full slice's distinct entries and
     evaluates about half of its samples."""
