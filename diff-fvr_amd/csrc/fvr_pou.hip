@@ -547,30 +547,39 @@ __device__ __forceinline__ void block_sum(float (&v)[NV], float* red) {
 #endif
 constexpr int kScanBatch = FVR_POU_SCAN_BATCH;  // detector pixels in flight per thread
 
+// llrint(x) for |x| < 2^51 (round to nearest even): 1.5 * 2^52 + x, rounded in double, holds
+// the integer two's complement in the bits above the constant's own bit pattern
+__device__ __forceinline__ unsigned long long fix64(float x) {
+  return (unsigned long long)__double_as_longlong(double(x) + 6755399441055744.0) - 0x4338000000000000ull;
+}
+
 template <bool IMG>
 __device__ __forceinline__ void warp_adjoint_scan(const float* img, const float* go, const float* rx,
                                                   const float* ry, const float* ic, const PouWarp& g, int n, float db,
                                                   float c_u, float c_v, float scale, unsigned long long* acc,
-                                                  int acc_pitch, bool roll, float (&s)[3]) {
-  const int P = g.P, hp = P / 2, K = g.K, stride = blockDim.x;
-  const int pp1 = P * P - 1;
+                                                  int acc_pitch, float (&s)[3]) {
+  const int P = g.P, K = g.K, stride = blockDim.x;
   // a NaN scale marks a pose whose output gradient holds NaN or inf: no fixed-point scatter
   // (the conversion back multiplies by 1 / NaN, so that pose's image gradient is NaN)
   const bool scatter = IMG && scale == scale;
+  const float* rxn = rx + size_t(n) * g.ray_stride;
+  const float* ryn = ry + size_t(n) * g.ray_stride;
+  const float* icn = ic + size_t(n) * g.ray_stride;
+  const float* gon = go + size_t(n) * K;
   for (int q0 = threadIdx.x; q0 < K; q0 += kScanBatch * stride) {
     // 1) the pixels' rays and output gradients
     float r_x[kScanBatch], r_y[kScanBatch], gg[kScanBatch];
 #pragma unroll
     for (int t = 0; t < kScanBatch; ++t) {
-      const int q = min(q0 + t * stride, K - 1);
-      const size_t ro = size_t(n) * g.ray_stride + q;
-      r_x[t] = rx[ro];
-      r_y[t] = ry[ro];
-      gg[t] = q0 + t * stride < K ? go[size_t(n) * K + q] * ic[ro] : 0.f;
+      const unsigned q = unsigned(min(q0 + t * stride, K - 1));
+      r_x[t] = rxn[q];
+      r_y[t] = ryn[q];
+      gg[t] = q0 + t * stride < K ? gon[q] * icn[q] : 0.f;
     }
-    // 2) their four image corners (clamped addresses; out-of-range corners count 0)
+    // 2) their four image corners: clamped coordinates, out-of-range corners count 0
     float tx[kScanBatch], ty[kScanBatch], vv[kScanBatch][4];
-    int x0[kScanBatch], y0[kScanBatch];
+    int xa[kScanBatch], xb[kScanBatch], ya[kScanBatch], yb[kScanBatch];
+    bool in[kScanBatch][4];
 #pragma unroll
     for (int t = 0; t < kScanBatch; ++t) {
       const float px = (db * r_x[t] - c_u) * g.inv_ds + g.c_idx;
@@ -578,17 +587,25 @@ __device__ __forceinline__ void warp_adjoint_scan(const float* img, const float*
       const float x0f = floorf(px), y0f = floorf(py);
       tx[t] = px - x0f;
       ty[t] = py - y0f;
-      x0[t] = int(x0f);
-      y0[t] = int(y0f);
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const int x = x0[t] + (c & 1), y = y0[t] + (c >> 1);
-        const bool in = (unsigned)x < (unsigned)P && (unsigned)y < (unsigned)P;
-        const float val = img[min(max(y * P + x, 0), pp1)];
-        vv[t][c] = in ? val : 0.f;
-      }
+      const int x0 = int(x0f), y0 = int(y0f);
+      const bool ix0 = unsigned(x0) < unsigned(P), ix1 = unsigned(x0 + 1) < unsigned(P);
+      const bool iy0 = unsigned(y0) < unsigned(P), iy1 = unsigned(y0 + 1) < unsigned(P);
+      xa[t] = min(max(x0, 0), P - 1);
+      xb[t] = min(max(x0 + 1, 0), P - 1);
+      ya[t] = min(max(y0, 0), P - 1);
+      yb[t] = min(max(y0 + 1, 0), P - 1);
+      in[t][0] = ix0 && iy0;
+      in[t][1] = ix1 && iy0;
+      in[t][2] = ix0 && iy1;
+      in[t][3] = ix1 && iy1;
+      const unsigned ra = __umul24(ya[t], P), rb = __umul24(yb[t], P);
+      const float f00 = img[ra + xa[t]], f01 = img[ra + xb[t]], f10 = img[rb + xa[t]], f11 = img[rb + xb[t]];
+      vv[t][0] = in[t][0] ? f00 : 0.f;
+      vv[t][1] = in[t][1] ? f01 : 0.f;
+      vv[t][2] = in[t][2] ? f10 : 0.f;
+      vv[t][3] = in[t][3] ? f11 : 0.f;
     }
-    // 3) geometry sums and the image-gradient scatter
+    // 3) geometry sums and the image-gradient scatter (a zero added where a corner is out of range)
 #pragma unroll
     for (int t = 0; t < kScanBatch; ++t) {
       const float ox = 1.f - tx[t], oy = 1.f - ty[t];
@@ -598,17 +615,13 @@ __device__ __forceinline__ void warp_adjoint_scan(const float* img, const float*
       s[0] = fmaf(gg[t], dsx * r_x[t] + dsy * r_y[t], s[0]);
       s[1] = fmaf(-gg[t], dsx, s[1]);
       s[2] = fmaf(-gg[t], dsy, s[2]);
-      if (scatter && gg[t] != 0.f) {
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          const int x = x0[t] + (c & 1), y = y0[t] + (c >> 1);
-          if ((unsigned)x < (unsigned)P && (unsigned)y < (unsigned)P) {
-            const float w = ((c & 1) ? tx[t] : ox) * ((c >> 1) ? ty[t] : oy);
-            const long long fx = llrintf(gg[t] * w * scale);
-            const int yy = roll ? ((y + hp) & (P - 1)) : y, xx = roll ? ((x + hp) & (P - 1)) : x;
-            atomicAdd(acc + size_t(yy) * acc_pitch + xx, (unsigned long long)fx);
-          }
-        }
+      if (scatter && gg[t] != 0.f && (in[t][0] || in[t][1] || in[t][2] || in[t][3])) {
+        const float gs = gg[t] * scale;  // a power of two: the product rounds as gg * w would
+        const unsigned ra = __umul24(ya[t], acc_pitch), rb = __umul24(yb[t], acc_pitch);
+        atomicAdd(acc + ra + xa[t], in[t][0] ? fix64(gs * (ox * oy)) : 0ull);
+        atomicAdd(acc + ra + xb[t], in[t][1] ? fix64(gs * (tx[t] * oy)) : 0ull);
+        atomicAdd(acc + rb + xa[t], in[t][2] ? fix64(gs * (ox * ty[t])) : 0ull);
+        atomicAdd(acc + rb + xb[t], in[t][3] ? fix64(gs * (tx[t] * ty[t])) : 0ull);
       }
     }
   }
@@ -626,10 +639,9 @@ __global__ __launch_bounds__(kPouThreads) void pou_warp_bwd_kernel(
   float s[3] = {0.f, 0.f, 0.f};
   if (gacc)
     warp_adjoint_scan<true>(imgs + bn * pp, go, rx, ry, ic, g, n, d_b[bn], cu[bn], cv[bn], gscale[n],
-                            gacc + bn * pp, g.P, false, s);
+                            gacc + bn * pp, g.P, s);
   else
-    warp_adjoint_scan<false>(imgs + bn * pp, go, rx, ry, ic, g, n, d_b[bn], cu[bn], cv[bn], 1.f, nullptr, g.P,
-                             false, s);
+    warp_adjoint_scan<false>(imgs + bn * pp, go, rx, ry, ic, g, n, d_b[bn], cu[bn], cv[bn], 1.f, nullptr, g.P, s);
   block_sum<3>(s, red);
   if (threadIdx.x < 3) part[size_t(bn) * 3 + threadIdx.x] = s[threadIdx.x];
 }
@@ -662,15 +674,15 @@ __global__ __launch_bounds__(kPouSliceThreads) void pou_render_bwd_kernel(
   for (int i = threadIdx.x; i < P * pitch; i += blockDim.x) acc[i] = 0ull;
   make_twiddles(tw, P);
   __syncthreads();
-  // 1) warp adjoint: geometry sums + image gradient (ifftshifted) in LDS fixed point
+  // 1) warp adjoint: geometry sums + image gradient in LDS fixed point
   float s3[3] = {0.f, 0.f, 0.f};
   const float scale = gscale[n];
   warp_adjoint_scan<true>(imgs + size_t(bn) * P * P, go, rx, ry, ic, g, n, d_b[bn], cu[bn], cv[bn], scale, acc,
-                          pitch, true, s3);
+                          pitch, s3);
   __syncthreads();
-  // 2) fixed point -> float, repacked in place for a real-input transform: row y holds the
-  // P / 2 complex values z(n) = g(y, 2n) + i g(y, 2n + 1) at pitch P / 2 + 1 (every value
-  // is read into registers before any is written: the packed rows overlap the 64-bit ones)
+  // 2) fixed point -> float, ifftshifted and repacked in place for a real-input transform: row
+  // y holds the P / 2 complex values z(n) = g(y, 2n) + i g(y, 2n + 1) at pitch P / 2 + 1 (every
+  // value is read into registers before any is written: the packed rows overlap the 64-bit ones)
   const float inv_scale = 1.f / scale;
   const int hpitch = hp + 1, logH = logP - 1;
   constexpr int kConv = (128 * 128) / (2 * kPouSliceThreads);  // pairs per thread at P = 128
@@ -680,8 +692,9 @@ __global__ __launch_bounds__(kPouSliceThreads) void pou_render_bwd_kernel(
     const int idx = threadIdx.x + t * blockDim.x;  // pair (y, n)
     if (idx < P * hp) {
       const int y = idx >> logH, n2 = idx & (hp - 1);
-      zv[t] = make_float2(float(double((long long)acc[y * pitch + 2 * n2])) * inv_scale,
-                          float(double((long long)acc[y * pitch + 2 * n2 + 1])) * inv_scale);
+      const int src = ((y + hp) & (P - 1)) * pitch + ((2 * n2 + hp) & (P - 1));  // ifftshift
+      zv[t] = make_float2(float(double((long long)acc[src])) * inv_scale,
+                          float(double((long long)acc[src + 1])) * inv_scale);
     }
   }
   __syncthreads();
