@@ -253,6 +253,35 @@ def pou_config4(steps: int, warmup: int, size: int = 256, n_pose: int = 64):
             model.forward_pinhole(cands, lattice)
         torch.cuda.synchronize()
         ms_fwd = (time.perf_counter() - t1) * 1e3 / steps
+    # edge_dominant_ncc (cone_beam_fvr.py:71-112, the loss of the reference's PoU registration)
+    # on the 64 projections against the target: forward, and forward + backward to the images
+    from fvr import edge_dominant_ncc
+    with torch.no_grad():
+        proj = model.forward_pinhole(cands, lattice)
+    pimg = proj.detach().clone().requires_grad_(True)
+    for _ in range(warmup):
+        edge_dominant_ncc(pimg, target).backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    with torch.no_grad():
+        for _ in range(steps):
+            edge_dominant_ncc(pimg, target)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    for _ in range(steps):
+        pimg.grad = None
+        edge_dominant_ncc(pimg, target).backward()
+    torch.cuda.synchronize()
+    e_fwd = (t3 - t2) * 1e3 / steps
+    e_fb = (time.perf_counter() - t3) * 1e3 / steps
+    img_b = n_pose * det * det * 4
+    e_bytes = 3 * img_b + det * det * 4  # images read (forward, backward), gradient written; the target once
+    edge = {"workload": f"edge_dominant_ncc, {n_pose} x {det}^2 projections vs one target (multiscale term on)",
+            "ms_fwd": round(e_fwd, 4), "ms_fwd_bwd": round(e_fb, 4),
+            "images_per_s_fwd_bwd": round(n_pose / (e_fb * 1e-3), 1),
+            "algorithmic_bytes_fwd_bwd": e_bytes,
+            "hbm_frac_fwd_bwd": round(e_bytes / (e_fb * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "grad_finite": bool(torch.isfinite(pimg.grad).all())}
     from fvr.footprint import pou_step_bytes, pou_texels_read
     R = cands[:, :3, :3].double().cpu().numpy()
     u_det = np.einsum("nij,j->ni", R, lattice[2].double().cpu().numpy())[:, ::-1]  # (x, y, z) storage order
@@ -263,7 +292,7 @@ def pou_config4(steps: int, warmup: int, size: int = 256, n_pose: int = 64):
     return {"workload": "PoU-FVR 256^3, 4x4x4 blocks @64 (P=128, bf16), pinhole SDD 1020/SAD 510, 256^2 det @0.8mm",
             "poses": n_pose, "blocks": model.B, "P": model.P, "ms_fwd_adj": round(ms, 3), "ms_fwd": round(ms_fwd, 3),
             "pose_evals_per_s": round(n_pose / (ms * 1e-3), 1), "build_s": round(build_s, 2),
-            "loss": float(loss.detach()), "grad_finite": bool(torch.isfinite(g).all()),
+            "loss": float(loss.detach()), "grad_finite": bool(torch.isfinite(g).all()), "edge_ncc": edge,
             "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes_per_step": fp["step"],
                          "bytes_by_kernel": {k: fp[k] for k in ("slice_ifft", "warp_fwd", "render_bwd")},

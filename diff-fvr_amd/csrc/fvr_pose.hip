@@ -148,30 +148,49 @@ __device__ __forceinline__ void stage_pose(const fvr_sncc_item* items, const dou
   }
 }
 
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(int(b & 0xFFFFFFFFll), l), hi = __builtin_amdgcn_readlane(int(b >> 32), l);
+  return __longlong_as_double((long long)(unsigned(lo)) | ((long long)hi << 32));
+}
+
 // One wave per pose: fixed-order float64 reduction of each view's tile
-// partials (lane-strided partial sums, then an XOR butterfly that every lane
-// evaluates in the same order), then the biplanar loss and exact gradient.
+// partials (half a wave per view: lane-strided partial sums, then an XOR butterfly
+// inside the half), then the biplanar loss and exact gradient.
 __device__ void reduce_combine_pose(const PoseStage& st, int pose, const float* __restrict__ partials, double t2_ap,
                                     double t2_lat, double* moments, double& L_out, double (&g_out)[6]) {
   const int lane = threadIdx.x & 63;
   const double two_pi = 6.283185307179586476925286766559;
   double g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   double L = 0.0;
+  // lanes 0-31 sum the AP view's tiles and lanes 32-63 the LAT view's, each half in a fixed
+  // order: lane-strided (stride 32) over its view's tiles, then an XOR butterfly inside the
+  // half (every lane of a half ends with the same bits); then both sums are read from lanes
+  // 0 and 32
+  const int half = lane >> 5;
+  double mh[FVR_SNCC_NMOM];
 #pragma unroll
-  for (int view = 0; view < 2; ++view) {
-    const int ntile = st.ntile[view], tile_base = st.tile_base[view];
-    double m[FVR_SNCC_NMOM];
-#pragma unroll
-    for (int q = 0; q < FVR_SNCC_NMOM; ++q) m[q] = 0.0;
-    for (int t = lane; t < ntile; t += 64) {
+  for (int q = 0; q < FVR_SNCC_NMOM; ++q) mh[q] = 0.0;
+  {
+    const int ntile = st.ntile[half], tile_base = st.tile_base[half];
+    for (int t = lane & 31; t < ntile; t += 32) {
       const float* pp = partials + size_t(tile_base + t) * FVR_SNCC_NMOM;
 #pragma unroll
-      for (int q = 0; q < FVR_SNCC_NMOM; ++q) m[q] += double(pp[q]);
+      for (int q = 0; q < FVR_SNCC_NMOM; ++q) mh[q] += double(pp[q]);
     }
+  }
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1)
+  for (int o = 16; o > 0; o >>= 1)
 #pragma unroll
-      for (int q = 0; q < FVR_SNCC_NMOM; ++q) m[q] += __shfl_xor(m[q], o);
+    for (int q = 0; q < FVR_SNCC_NMOM; ++q) mh[q] += __shfl_xor(mh[q], o);
+  double mm[2][FVR_SNCC_NMOM];
+#pragma unroll
+  for (int view = 0; view < 2; ++view)
+#pragma unroll
+    for (int q = 0; q < FVR_SNCC_NMOM; ++q) mm[view][q] = readlane_d(mh[q], 32 * view);
+#pragma unroll
+  for (int view = 0; view < 2; ++view) {
+    const double* m = mm[view];
     if (moments && lane < FVR_SNCC_NMOM) {
       double mv = 0.0;
 #pragma unroll

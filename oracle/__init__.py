@@ -12,11 +12,13 @@ Modules
                      ``gradient_optimizer.py``, ``core_engine.py``,
                      ``registration_3d2d.py``) plus the exact analytic pose
                      adjoint the HIP kernel implements.
-``torch_engine_ref`` float64 torch-CPU restatement of convention T
-                     (``src/fvr/torch_engine.py`` + the stencils of
-                     ``src/fvr/triton_slice.py``).
-``cone_beam_ref``    float64 torch-CPU restatement of the PoU-FVR pinhole path
-                     (``src/fvr/cone_beam_fvr.py``).
+``torch_engine_np``  FP64 NumPy restatement of convention T
+                     (``src/fvr/torch_engine.py``: geometry, slice, image-space
+                     loss) and of the generic per-corner slice operator of
+                     ``src/fvr/triton_slice.py``.
+``cone_beam_np``     FP64 NumPy restatement of the PoU-FVR pinhole path
+                     (``src/fvr/cone_beam_fvr.py``: blocks, slab images, warp,
+                     pinhole / C-arm projections, ``edge_dominant_ncc``).
 
 How each function is pinned (golden vectors, known answers, closed forms,
 brute force) is listed in each module's header; functions without such a
