@@ -14,7 +14,7 @@ from .gradient_optimizer import GradientRegistrationOptimizer
 from .core_engine import RobustDiffFVREngine, detect_radio_opaque_metal
 from .registration_3d2d import compute_mtre_6dof
 from .phantom import ellipsoid_phantom
-from .slice_ops import fourier_slice_3d
+from .triton_slice import fourier_slice_3d
 from .torch_engine import TorchDiffFVREngine
 from .cone_beam_fvr import (PerspectiveSlabFVR, SLAB_PRESETS, edge_dominant_ncc, pou_pinhole_warp_sum,
                             sample_isocenter_parity_offsets)

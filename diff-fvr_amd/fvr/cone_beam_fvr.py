@@ -28,7 +28,7 @@ import torch.nn as nn
 import torch.nn.functional as F
 
 from . import _lib
-from .slice_ops import _VOL_DTYPES, fourier_slice_3d
+from .triton_slice import _VOL_DTYPES, _PouWarpSum, _f32, _rays, fourier_slice_3d, pou_pinhole_warp_sum
 
 SLAB_PRESETS: Dict[str, Dict[str, object]] = {
     "fast_coarse": {"grid_blocks": (2, 2, 2), "block_res": 64, "dtype": torch.bfloat16},
@@ -162,86 +162,8 @@ def sample_isocenter_parity_offsets(vox_spacing) -> torch.Tensor:
 
 
 # ---------------------------------------------------------------------------
-# autograd operators over the HIP kernels
+# fused autograd operator over the PoU kernels
 # ---------------------------------------------------------------------------
-def _f32(t: torch.Tensor, dev) -> torch.Tensor:
-    return t.detach().to(device=dev, dtype=torch.float32).contiguous()
-
-
-def _rays(rx, ry, ic, N: int, dev):
-    """Flattened float32 ray arrays and the per-pose flag: (K,) shared or (N, K) per pose."""
-    per_pose = rx.dim() == 3
-    r = [_f32(a, dev) for a in (rx, ry, ic)]
-    if per_pose and r[0].shape[0] != N:
-        raise ValueError(f"per-pose ray arrays hold {r[0].shape[0]} poses, expected {N}")
-    if per_pose:
-        K = r[0].shape[-2] * r[0].shape[-1]
-        r = [a.reshape(N, K) for a in r]
-    else:
-        K = r[0].numel()
-        r = [a.reshape(K) for a in r]
-    return r, int(per_pose), K
-
-
-class _PouWarpSum(torch.autograd.Function):
-    """pou_pinhole_warp_sum (triton_slice.py:787-808): grads for imgs, d_b, cu, cv and (per-pose) rays."""
-
-    @staticmethod
-    def forward(ctx, imgs, d_b, cu, cv, rx, ry, ic, inv_ds, c_idx, out_h, out_w):
-        dev = imgs.device
-        B, N, P, P2 = imgs.shape
-        if P2 != P:
-            raise ValueError("slab images must be square (B, N, P, P)")
-        im = _f32(imgs, dev)
-        db, u_, v_ = _f32(d_b, dev), _f32(cu, dev), _f32(cv, dev)
-        (rxf, ryf, icf), per_pose, K = _rays(rx, ry, ic, N, dev)
-        if K != out_h * out_w or tuple(db.shape) != (B, N) or tuple(u_.shape) != (B, N) or tuple(v_.shape) != (B, N):
-            raise ValueError("pou_pinhole_warp_sum: inconsistent shapes")
-        out = torch.empty((N, out_h, out_w), device=dev, dtype=torch.float32)
-        _lib.check(_lib.lib().fvr_pou_warp_fwd(_lib.ptr(im), B, N, P, _lib.ptr(db), _lib.ptr(u_), _lib.ptr(v_),
-                                               _lib.ptr(rxf), _lib.ptr(ryf), _lib.ptr(icf), per_pose, K,
-                                               float(inv_ds), float(c_idx), _lib.ptr(out), _lib.stream_ptr(dev)),
-                   "fvr_pou_warp_fwd")
-        ctx.save_for_backward(im, db, u_, v_, rxf, ryf, icf)
-        ctx.meta = (B, N, P, K, per_pose, float(inv_ds), float(c_idx), imgs.dtype, (out_h, out_w), rx.shape)
-        ctx.dts = tuple(t.dtype for t in (d_b, cu, cv, rx, ry, ic))
-        return out
-
-    @staticmethod
-    def backward(ctx, g):
-        im, db, u_, v_, rxf, ryf, icf = ctx.saved_tensors
-        B, N, P, K, per_pose, inv_ds, c_idx, idt, hw, rshape = ctx.meta
-        dev = im.device
-        L = _lib.lib()
-        st = _lib.stream_ptr(dev)
-        go = _f32(g, dev).reshape(N, K)
-        want_img = ctx.needs_input_grad[0]
-        part = torch.empty((B, N, 3), device=dev, dtype=torch.float32)
-        scale = torch.empty(N, device=dev, dtype=torch.float32)
-        gimg = None
-        ws = None
-        if want_img:
-            gimg = torch.empty((B, N, P, P), device=dev, dtype=torch.float32)
-            ws = torch.empty(int(L.fvr_pou_warp_bwd_workspace(B, N, P)), device=dev, dtype=torch.uint8)
-            _lib.check(L.fvr_pou_gscale(_lib.ptr(go), _lib.ptr(icf), N, K, per_pose, _lib.ptr(scale), st),
-                       "fvr_pou_gscale")
-        _lib.check(L.fvr_pou_warp_bwd(_lib.ptr(im), B, N, P, _lib.ptr(go), _lib.ptr(db), _lib.ptr(u_), _lib.ptr(v_),
-                                      _lib.ptr(rxf), _lib.ptr(ryf), _lib.ptr(icf), per_pose, K, inv_ds, c_idx,
-                                      _lib.ptr(scale), _lib.ptr(ws) if ws is not None else None,
-                                      _lib.ptr(gimg) if gimg is not None else None, _lib.ptr(part), st),
-                   "fvr_pou_warp_bwd")
-        grx = gry = gic = None
-        if per_pose and any(ctx.needs_input_grad[4:7]):
-            grx, gry, gic = (torch.empty((N, K), device=dev, dtype=torch.float32) for _ in range(3))
-            _lib.check(L.fvr_pou_ray_grads(_lib.ptr(im), B, N, P, _lib.ptr(go), _lib.ptr(db), _lib.ptr(u_),
-                                           _lib.ptr(v_), _lib.ptr(rxf), _lib.ptr(ryf), _lib.ptr(icf), per_pose, K,
-                                           inv_ds, c_idx, _lib.ptr(grx), _lib.ptr(gry), _lib.ptr(gic), st),
-                       "fvr_pou_ray_grads")
-            grx, gry, gic = (t.reshape(rshape).to(dt) for t, dt in zip((grx, gry, gic), ctx.dts[3:]))
-        return (gimg.to(idt) if gimg is not None else None, *(part[..., k].to(ctx.dts[k]) for k in range(3)),
-                grx, gry, gic, None, None, None, None)
-
-
 class _PouRender(torch.autograd.Function):
     """Fused slice -> LDS iFFT -> warp for every (block, pose); backward in one fused launch."""
 
@@ -301,18 +223,6 @@ class _PouRender(torch.autograd.Function):
             grx, gry, gic = (t.reshape(rshape).to(dt) for t, dt in zip((grx, gry, gic), ctx.dts[3:]))
         return (guv[:, 0:3].to(udt), guv[:, 3:6].to(udt), *(part[..., k].to(ctx.dts[k]) for k in range(3)), grx, gry, gic,
                 None, None, None)
-
-
-def pou_pinhole_warp_sum(imgs, d_b, cu, cv, rx_hw, ry_hw, inv_cos_hw, rx_flat=None, ry_flat=None, ic_flat=None,
-                         inv_ds: float = 1.0, c_idx: float = 0.0, use_triton: bool = True) -> torch.Tensor:
-    """(N, H, W) = sum_b bilinear(imgs[b, n], p_bn) * inv_cos (triton_slice.py:787-808), on HIP.
-
-    Ray arrays are (H, W) shared by all poses (the reference's signature) or
-    (N, H, W) per pose; ``use_triton`` is accepted and ignored.
-    """
-    _lib.require_device(imgs, "pou_pinhole_warp_sum")
-    H, W = int(rx_hw.shape[-2]), int(rx_hw.shape[-1])
-    return _PouWarpSum.apply(imgs, d_b, cu, cv, rx_hw, ry_hw, inv_cos_hw, float(inv_ds), float(c_idx), H, W)
 
 
 # ---------------------------------------------------------------------------

@@ -126,7 +126,7 @@ def pou_texels_mask(model, pose_mats, lattice):
     of ``pou_pinhole_warp_sum`` for an all-ones output gradient is positive exactly there
     (every bilinear weight and 1/cos is >= 0).  With a 256^2 detector at 0.4 mm per pixel at
     the isocentre only part of each block's P x P image falls inside the field of view."""
-    from .cone_beam_fvr import pou_pinhole_warp_sum
+    from .triton_slice import pou_pinhole_warp_sum
     dev = model.F_blk_ri.device
     R, t = pose_mats[:, :3, :3].to(dev), pose_mats[:, :3, 3].to(dev)
     src0, _, u0, v0, n0 = (x.to(dev) for x in lattice[:5])

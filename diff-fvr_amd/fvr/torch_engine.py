@@ -6,7 +6,7 @@ Restates ``src/fvr/torch_engine.py`` of pareelamre/diff-fvr:
   sinc^2), stored as ``f_vol_5d`` = (1, 2, D, H, W) real/imag planes
   (:28-88); computed here in float64 on the device, then cast;
 * ``_extract_slice_spectrum`` / ``forward`` (:176-253): the slice through the
-  generic HIP operator (``slice_ops.fourier_slice_3d``), the phase ramp, and for
+  generic HIP operator (``triton_slice.fourier_slice_3d``), the phase ramp, and for
   ``forward`` the inverse FFT + ReLU, differentiable w.r.t. the poses;
 * ``forward_spectral_ncc`` (:269-307), the registration hot path: one fused HIP
   launch per call (slice + phase ramp + high-pass + NCC sums + exact pose
@@ -26,7 +26,7 @@ import torch
 import torch.nn as nn
 
 from . import _items, _lib
-from .slice_ops import fourier_slice_3d
+from .triton_slice import fourier_slice_3d
 from .sncc import _Workspace, launch_items, tile_rows
 from .spectrum import PackedSpectrum
 
