@@ -585,7 +585,22 @@ def main(argv=None) -> int:
         for _ in range(10):
             ev.evaluate(cands)
         host_ms = (time.perf_counter() - t_h) * 1e3 / 10
-        results[st] = {"ms_per_step": step_ms, "kernel_ms": kern_ms, "kernel_ms_samples": len(kern_all),
+        c_abi = None
+        if st == args.storage and not args.no_extra:
+            # the one-call C entry point (fvr_pn_evaluate) with the plan its own C autotuner
+            # picks (fvr_autotune_pn_evaluate): what a caller outside Python gets per batch
+            c_plan, c_tuned_ms = ev.autotune_call(P, reps=5)
+            for _ in range(args.warmup):
+                ev.evaluate_call(P)
+            torch.cuda.synchronize()
+            t_c = time.perf_counter()
+            for _ in range(args.steps):
+                ev.evaluate_call(P)
+            torch.cuda.synchronize()
+            c_ms = (time.perf_counter() - t_c) * 1e3 / args.steps
+            c_abi = {"plan": f"{c_plan.tile_h}/{c_plan.order}/{c_plan.depth}", "autotune_ms": round(c_tuned_ms, 4),
+                     "ms_per_call": round(c_ms, 4), "pose_evals_per_s": round(args.n_cand / (c_ms * 1e-3), 1)}
+        results[st] = {"c_abi": c_abi, "ms_per_step": step_ms, "kernel_ms": kern_ms, "kernel_ms_samples": len(kern_all),
                        "kernel_ms_p50": statistics.median(kern_all), "kernel_ms_min": min(kern_all),
                        "kernel_ms_in_timed_region": kern_region, "value": args.n_cand / (step_ms * 1e-3),
                        "loss_mean": float(loss.mean()), "loss": loss, "grad": grad, "herm": ev.herm,
@@ -654,6 +669,8 @@ def main(argv=None) -> int:
         extra["multistart"] = multistart_config5()
         log(f"multistart: {extra['multistart']}")
     extra["three_launch_ms_per_step"] = round(head["three_launch_ms_per_step"], 4)
+    if head["c_abi"] is not None:
+        extra["c_abi_one_call"] = head["c_abi"]
     if head["graph_ms_per_step"] is not None:
         extra["graph_replay_ms_per_step"] = round(head["graph_ms_per_step"], 4)
     extra["host_numpy_ms_per_step"] = round(head["host_numpy_ms_per_step"], 4)

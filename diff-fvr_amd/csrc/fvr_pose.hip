@@ -10,7 +10,9 @@
 // sample (i, j) at c + (j - W/2) u + (i - H/2) v with c = (pz, py, px)/2.
 // Written as straight-line code: no dynamically indexed local arrays.
 #include <cmath>
+#include <map>
 #include <mutex>
+#include <tuple>
 
 #include "fvr_common.h"
 
@@ -696,6 +698,106 @@ int fvr_pn_evaluate_noalloc(const fvr_layout* layouts_dev, int32_t n_layouts, co
   if (rc != FVR_OK) return rc;
   return pn_evaluate_run(layouts_dev, n_layouts, tmpl_host, pz, py, px, lat_swap, variant, poses, n_poses, t2_ap,
                          t2_lat, static_cast<char*>(ws), moments, loss, grad, stream);
+}
+
+// Launch-plan autotuner of fvr_pn_evaluate: the (tile height, samples in flight, tile order)
+// variants the library has for the templates' flags, timed with HIP events on the stream;
+// results cached per (device, dtype / flags / exact_box bits, padded dims, slice sizes,
+// lateral axis order, pose count) under a mutex.
+static std::mutex tune_mu;
+static std::map<std::tuple<int, uint32_t, int, int, int, int, int, int, int, int>, std::pair<uint32_t, float>>
+    tune_cache;
+
+int fvr_autotune_pn_evaluate(const fvr_layout* layouts_dev, int32_t n_layouts, const fvr_sncc_item* tmpl_host,
+                             double pz, double py, double px, int32_t lat_swap, uint32_t variant, const double* poses,
+                             int32_t n_poses, double t2_ap, double t2_lat, void* workspace, int64_t ws_bytes,
+                             double* loss, double* grad, int32_t reps, int32_t use_cache, uint32_t* best_variant,
+                             float* best_ms, void* stream) {
+  if (pn_evaluate_check("fvr_autotune_pn_evaluate", layouts_dev, n_layouts, tmpl_host, pz, py, px, variant, poses,
+                        n_poses, loss, grad) != FVR_OK)
+    return FVR_ERR_INVALID;
+  FVR_CHECK_ARG(best_variant && best_ms && reps > 0 && reps <= 1000, "fvr_autotune_pn_evaluate: bad arguments");
+  FVR_CHECK_ARG(workspace && (reinterpret_cast<uintptr_t>(workspace) & 255) == 0,
+                "fvr_autotune_pn_evaluate: workspace missing or not 256-byte aligned");
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    set_error("fvr_autotune_pn_evaluate: hipGetDevice failed");
+    return FVR_ERR_HIP;
+  }
+  const uint32_t base = variant & (0xFu | (0xFu << 4) | (1u << 20));  // dtype, flags, exact_box
+  const auto key = std::make_tuple(dev, base, n_poses, int(pz), int(py), int(px), int(lat_swap), tmpl_host[0].w,
+                                   tmpl_host[1].w, tmpl_host[0].h);
+  if (use_cache) {
+    std::lock_guard<std::mutex> g(tune_mu);
+    const auto hit = tune_cache.find(key);
+    if (hit != tune_cache.end()) {
+      *best_variant = hit->second.first;
+      *best_ms = hit->second.second;
+      return FVR_OK;
+    }
+  }
+  const uint32_t flags = (variant >> 4) & 0xF;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess) {
+    set_error("fvr_autotune_pn_evaluate: hipEventCreate failed");
+    return FVR_ERR_HIP;
+  }
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    set_error("fvr_autotune_pn_evaluate: hipEventCreate failed");
+    return FVR_ERR_HIP;
+  }
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  uint32_t best = 0;
+  float best_t = 0.f;
+  int rc = FVR_OK;
+  const int tile_hs[3] = {32, 64, 128};
+  for (int a = 0; a < 3 && rc == FVR_OK; ++a) {
+    for (int depth = 2; depth <= 3 && rc == FVR_OK; ++depth) {
+      if (!fvr_sncc_has_depth(flags, tile_hs[a], depth)) continue;
+      fvr_sncc_item tm[2] = {tmpl_host[0], tmpl_host[1]};
+      tm[0].tile_h = tm[1].tile_h = tile_hs[a];
+      if (pn_ws_layout(tm, n_poses).total > ws_bytes) continue;  // this tile height needs more workspace
+      for (int order = 0; order < 2 && rc == FVR_OK; ++order) {
+        const uint32_t v = base | (uint32_t(tile_hs[a]) << 8) | (uint32_t(depth) << 16) | (uint32_t(order) << 21);
+        char* ws = static_cast<char*>(workspace);
+        rc = pn_evaluate_run(layouts_dev, n_layouts, tm, pz, py, px, lat_swap, v, poses, n_poses, t2_ap, t2_lat, ws,
+                             nullptr, loss, grad, stream);  // warm-up
+        if (rc == FVR_OK && hipEventRecord(e0, st) != hipSuccess) rc = FVR_ERR_HIP;
+        for (int k = 0; k < reps && rc == FVR_OK; ++k)
+          rc = pn_evaluate_run(layouts_dev, n_layouts, tm, pz, py, px, lat_swap, v, poses, n_poses, t2_ap, t2_lat,
+                               ws, nullptr, loss, grad, stream);
+        float ms = 0.f;
+        if (rc == FVR_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                             hipEventElapsedTime(&ms, e0, e1) != hipSuccess)) {
+          set_error("fvr_autotune_pn_evaluate: event timing failed");
+          rc = FVR_ERR_HIP;
+        }
+        if (rc == FVR_OK && (best == 0 || ms / reps < best_t)) {
+          best = v;
+          best_t = ms / reps;
+        }
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != FVR_OK) return rc;
+  FVR_CHECK_ARG(best != 0, "fvr_autotune_pn_evaluate: no variant fits a workspace of %lld bytes",
+                (long long)ws_bytes);
+  {
+    std::lock_guard<std::mutex> g(tune_mu);
+    tune_cache[key] = std::make_pair(best, best_t);
+  }
+  *best_variant = best;
+  *best_ms = best_t;
+  return FVR_OK;
+}
+
+int fvr_autotune_clear(void) {
+  std::lock_guard<std::mutex> g(tune_mu);
+  tune_cache.clear();
+  return FVR_OK;
 }
 
 int fvr_pn_step(fvr_sncc_item* items_dev, int32_t n_poses, const float* partials, double* jac_dev, double t2_ap,

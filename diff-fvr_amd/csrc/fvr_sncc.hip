@@ -204,14 +204,21 @@ __device__ __forceinline__ void tile_reduce(const Acc& ac, float cw, int tile, f
   __syncthreads();  // red is reused by the workgroup's next tile
 }
 
+// Launch options (both kernels): kOrderBiplanar -- no map; the items are (AP, LAT) pairs
+// (item 2 n + v, every item of view v with the tile count of items[v]) and the slots walk
+// them in target-major order (view, tile, pose), split contiguously over the XCDs, the
+// order a host-built 'target' map gives such a batch.
+constexpr int kOrderBiplanar = 2;
+
 // Workgroup slot w -> (item, tile within the item, global tile index).  With a
 // map the host chose the order (entries with an out-of-range item are idle
-// slots); without one, each XCD (slot % 8) takes a contiguous range of tiles
-// and the item is found by binary search over the items' tile_base.
+// slots); with kOrderBiplanar the order is computed as above; otherwise each XCD
+// (slot % 8) takes a contiguous range of tiles and the item is found by binary
+// search over the items' tile_base.
 template <int R>
 __device__ __forceinline__ bool resolve_tile(int w, const int2* __restrict__ map,
                                              const fvr_sncc_item* __restrict__ items, int n_items, int n_tiles,
-                                             int n_slots, int& item, int& lt, int& tile) {
+                                             int n_slots, int opts, int& item, int& lt, int& tile) {
   if (map) {
     const int2 e = map[w];
     if (uint32_t(e.x) >= uint32_t(n_items)) return false;
@@ -223,8 +230,27 @@ __device__ __forceinline__ bool resolve_tile(int w, const int2* __restrict__ map
     lt = e.y;
     return true;
   }
-  tile = (w & 7) * (n_slots >> 3) + (w >> 3);
-  if (tile >= n_tiles) return false;
+  const int g = (w & 7) * (n_slots >> 3) + (w >> 3);  // position in the order
+  if (g >= n_tiles) return false;
+  if (opts & kOrderBiplanar) {
+    const int np = n_items >> 1;
+    if (np < 1) return false;
+    const int t0 = items[0].tiles_x * sncc_tiles_y(items[0].h, items[0].flags, 4 * R);
+    int v = 0, r = g;
+    if (r >= np * t0) {
+      v = 1;
+      r -= np * t0;
+    }
+    const int t = r / np, pose = r - t * np;
+    const fvr_sncc_item& it = items[2 * pose + v];
+    const int nt = it.tiles_x * sncc_tiles_y(it.h, it.flags, 4 * R);
+    tile = it.tile_base + t;
+    if (uint32_t(t) >= uint32_t(nt) || uint32_t(tile) >= uint32_t(n_tiles)) return false;
+    item = 2 * pose + v;
+    lt = t;
+    return true;
+  }
+  tile = g;
   int lo = 0, hi = n_items - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -402,10 +428,10 @@ template <int DT, bool ADJ, bool HP, bool WS, int R>
 __global__ __launch_bounds__(256) void sncc_kernel(const fvr_layout* __restrict__ layouts, int n_layouts,
                                                    const fvr_sncc_item* __restrict__ items, int n_items,
                                                    int n_tiles, const int2* __restrict__ map, int n_slots,
-                                                   float* __restrict__ partials) {
+                                                   int opts, float* __restrict__ partials) {
   for (int w = blockIdx.x; w < n_slots; w += gridDim.x) {
     int item, lt, tile;
-    if (!resolve_tile<R>(w, map, items, n_items, n_tiles, n_slots, item, lt, tile)) continue;
+    if (!resolve_tile<R>(w, map, items, n_items, n_tiles, n_slots, opts, item, lt, tile)) continue;
     sncc_tile<DT, ADJ, HP, WS, R>(layouts, n_layouts, items[item], lt, tile, partials);
   }
 }
@@ -733,7 +759,7 @@ __global__ __launch_bounds__(256) void sncc_herm_kernel(const fvr_layout* __rest
                                                         int opts, float* __restrict__ partials) {
   for (int w = blockIdx.x; w < n_slots; w += gridDim.x) {
     int item, lt, tile;
-    if (!resolve_tile<R>(w, map, items, n_items, n_tiles, n_slots, item, lt, tile)) continue;
+    if (!resolve_tile<R>(w, map, items, n_items, n_tiles, n_slots, opts, item, lt, tile)) continue;
     sncc_herm_tile<DT, ADJ, R, DEPTH>(layouts, n_layouts, items[item], lt, tile, opts, partials);
   }
 }
@@ -778,13 +804,13 @@ struct Launch {
   int n_slots;
   float* partials;
   int depth;  // samples in flight per lane (Hermitian-fold kernel: 2 or 3)
-  int opts;   // Hermitian-fold kernel options (kHermExactBox)
+  int opts;   // kHermExactBox (fold kernel), kOrderBiplanar
 };
 
 template <int DT, bool ADJ, bool HP, bool WS, int R>
 static void launch_sncc(const Launch& a) {
   hipLaunchKernelGGL((sncc_kernel<DT, ADJ, HP, WS, R>), a.grid, dim3(256), 0, a.st, a.lay, a.n_lay, a.items,
-                     a.n_items, a.n_tiles, a.map, a.n_slots, a.partials);
+                     a.n_items, a.n_tiles, a.map, a.n_slots, a.opts, a.partials);
 }
 
 template <int DT, bool ADJ, int R>
@@ -871,7 +897,10 @@ int fvr_sncc_eval_plan(const fvr_layout* layouts_dev, int32_t n_layouts, const f
   if (depth == 0) depth = 2;
   FVR_CHECK_ARG(dt == FVR_C32 || dt == FVR_C16F || dt == FVR_C16B, "fvr_sncc_eval: bad dtype %d", dt);
   const bool exact_box = (variant >> 20) & 1u;
-  FVR_CHECK_ARG((variant >> 21) == 0u, "fvr_sncc_eval: bad variant %u", variant);
+  const bool biplanar = (variant >> 21) & 1u;
+  FVR_CHECK_ARG((variant >> 22) == 0u, "fvr_sncc_eval: bad variant %u", variant);
+  FVR_CHECK_ARG(!biplanar || (!tile_map && n_items % 2 == 0),
+                "fvr_sncc_eval: the biplanar order takes (AP, LAT) item pairs and no map");
   FVR_CHECK_ARG(!exact_box || (flags & FVR_ITEM_HERM), "fvr_sncc_eval: exact-box bit without the fold flag");
   FVR_CHECK_ARG(has_variant(flags, tile_h, depth), "fvr_sncc_eval: no kernel for flags %u with tile_h %d, depth %d",
                 flags, tile_h, depth);
@@ -894,7 +923,7 @@ int fvr_sncc_eval_plan(const fvr_layout* layouts_dev, int32_t n_layouts, const f
   a.n_slots = n_slots;
   a.partials = partials;
   a.depth = depth;
-  a.opts = exact_box ? kHermExactBox : 0;
+  a.opts = (exact_box ? kHermExactBox : 0) | (biplanar ? kOrderBiplanar : 0);
   if (dt == FVR_C32) dispatch_dt<FVR_C32>(flags, tile_h, a);
   else if (dt == FVR_C16F) dispatch_dt<FVR_C16F>(flags, tile_h, a);
   else dispatch_dt<FVR_C16B>(flags, tile_h, a);

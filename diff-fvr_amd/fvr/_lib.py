@@ -92,6 +92,10 @@ _SIGNATURES = {
                          ctypes.c_double, ctypes.c_double, _P, ctypes.c_int64, _P, _P, _P, _P], _I),
     "fvr_pn_evaluate_noalloc": ([_P, _I, _P, ctypes.c_double, ctypes.c_double, ctypes.c_double, _I, _U, _P, _I,
                                  ctypes.c_double, ctypes.c_double, _P, _P, _P, _P], _I),
+    "fvr_autotune_pn_evaluate": ([_P, _I, _P, ctypes.c_double, ctypes.c_double, ctypes.c_double, _I, _U, _P, _I,
+                                  ctypes.c_double, ctypes.c_double, _P, ctypes.c_int64, _P, _P, _I, _I, _P, _P, _P],
+                                 _I),
+    "fvr_autotune_clear": ([], _I),
     "fvr_c64_pair_moments": ([_U64, _U64, _I, _I, _P, _P], _I),
     "fvr_slice_fwd": ([_U64, _I, _I, _I, _I, _I, _P, _P, _I, _P, _P, _I, _F, _F, _F, _U64, _P], _I),
     "fvr_slice_bwd_workspace": ([_I, _I], ctypes.c_int64),

@@ -58,6 +58,10 @@ class LaunchPlan:
         return code | (flags << 4) | (self.tile_h << 8) | (self.depth << 16) | (int(self.exact_box) << 20)
 
 
+# variant bit of the fused kernel's map-free target-major order for (AP, LAT) item pairs
+VARIANT_BIPLANAR_ORDER = 1 << 21
+
+
 N_CU = 256  # MI355X compute units (8 XCDs x 32)
 
 
@@ -354,7 +358,8 @@ class BiplanarEvaluator:
 
     def evaluate_call(self, poses: torch.Tensor):
         """The same evaluation through the one-call C entry point ``fvr_pn_evaluate`` (set-up,
-        fused kernel in XCD-contiguous order with the plan's tile height and depth, reduce +
+        fused kernel with the plan's tile height, depth and order -- "target" through the
+        kernel's map-free target-major order of item pairs, persistence ignored -- reduce +
         combine; items, Jacobians and partials in a workspace sized by
         ``fvr_pn_evaluate_workspace``): (loss (N,), grad (N, 6)) device views."""
         _lib.require_device(poses, "BiplanarEvaluator.evaluate_call")
@@ -370,12 +375,49 @@ class BiplanarEvaluator:
         grad = self.ws.get("call_grad", 6 * n, torch.float64)
         pz, py, px = self.packed.shape
         plan = LaunchPlan(self.plan.tile_h, "xcd", 0, self.plan.depth, self.plan.exact_box)
+        variant = plan.variant(self.packed.code, self.flags)
+        if self.plan.order == "target":
+            variant |= VARIANT_BIPLANAR_ORDER
         _lib.check(L.fvr_pn_evaluate(_lib.ptr(self.packed.table), len(self.packed.descs), self.templates.ctypes.data,
                                      float(pz), float(py), float(px), self.lat_swap,
-                                     plan.variant(self.packed.code, self.flags), _lib.ptr(poses), n, float(self.t2[0]),
+                                     variant, _lib.ptr(poses), n, float(self.t2[0]),
                                      float(self.t2[1]), _lib.ptr(ws), nb, None, _lib.ptr(loss), _lib.ptr(grad),
                                      _lib.stream_ptr(self.device)), "fvr_pn_evaluate")
         return loss[:n], grad[: 6 * n].view(n, 6)
+
+    def autotune_call(self, poses: torch.Tensor, reps: int = 5, use_cache: bool = True):
+        """Autotune ``fvr_pn_evaluate`` on ``poses`` in the library (``fvr_autotune_pn_evaluate``)
+        and set the matching plan: returns (plan, mean ms of the best variant).  The
+        workspace is sized for the smallest tile height, so every candidate fits."""
+        _lib.require_device(poses, "BiplanarEvaluator.autotune_call")
+        if poses.dtype != torch.float64 or poses.dim() != 2 or poses.shape[1] != 6 or not poses.is_contiguous():
+            raise ValueError("poses must be a contiguous (N, 6) float64 tensor")
+        n = int(poses.shape[0])
+        L = _lib.lib()
+        t32 = self.templates.copy()
+        t32["tile_h"] = 32
+        nb = int(L.fvr_pn_evaluate_workspace(t32.ctypes.data, n))
+        if nb < 0:
+            raise ValueError("fvr_pn_evaluate_workspace rejected the templates")
+        ws = self.ws.get("call_ws", nb, torch.uint8)
+        loss = self.ws.get("call_loss", n, torch.float64)
+        grad = self.ws.get("call_grad", 6 * n, torch.float64)
+        pz, py, px = self.packed.shape
+        best = ctypes.c_uint32(0)
+        ms = ctypes.c_float(0.0)
+        base = LaunchPlan(self.plan.tile_h, "xcd", 0, self.plan.depth, self.plan.exact_box)
+        _lib.check(L.fvr_autotune_pn_evaluate(_lib.ptr(self.packed.table), len(self.packed.descs),
+                                              self.templates.ctypes.data, float(pz), float(py), float(px),
+                                              self.lat_swap, base.variant(self.packed.code, self.flags),
+                                              _lib.ptr(poses), n, float(self.t2[0]), float(self.t2[1]), _lib.ptr(ws),
+                                              nb, _lib.ptr(loss), _lib.ptr(grad), int(reps), int(bool(use_cache)),
+                                              ctypes.byref(best), ctypes.byref(ms), _lib.stream_ptr(self.device)),
+                   "fvr_autotune_pn_evaluate")
+        v = int(best.value)
+        plan = LaunchPlan((v >> 8) & 0xFF, "target" if v & VARIANT_BIPLANAR_ORDER else "xcd", 0, (v >> 16) & 0xF,
+                          bool((v >> 20) & 1))
+        self.set_plan(plan)
+        return plan, float(ms.value)
 
     # -- two-launch loop form: begin() once, then step() per iteration ----------
     def begin(self, poses: torch.Tensor) -> int:

@@ -165,10 +165,16 @@ int fvr_sncc_has_depth(uint32_t flags, int32_t tile_h, int32_t depth);
 
 /* Launch the fused kernel over n_tiles tiles (the sum of items' tile counts).
  * variant = dtype | (flags << 4) | (tile_h << 8) | (depth << 16) | (exact_box << 20)
+ *           | (biplanar_order << 21)
  * (tile_h 0 = 64, depth 0 = 2: samples in flight per lane, fvr_sncc_has_depth;
  * exact_box, fold kernel only: every wave takes the per-sample box tests instead of
  * the interior / outside wave classes -- the same sums bit for bit, slower; for
- * tests); every item of one launch shares flags and tile_h.  Items whose layout index is not below
+ * tests; biplanar_order, no tile map: the items are (AP, LAT) pairs, item 2n + v,
+ * every item of view v with the tile count of item v (as fvr_pn_build_items makes
+ * them), and the workgroup slots take the tiles in target-major order -- view, tile
+ * within the item, pose -- split contiguously over the XCDs, so that an XCD's L2 holds
+ * one band of one view's target while it walks every pose; the sums are the same bit
+ * for bit as in the default order); every item of one launch shares flags and tile_h.  Items whose layout index is not below
  * n_layouts, or whose tile_h differs, contribute zero partials.
  * partials: float[n_tiles][FVR_SNCC_NMOM] (caller buffer).
  * Tile order: workgroup slot w (XCD w % 8) takes tile (w % 8) * ceil(n_tiles/8) + w / 8,
@@ -231,6 +237,30 @@ int fvr_pn_evaluate(const fvr_layout* layouts_dev, int32_t n_layouts, const fvr_
                     double pz, double py, double px, int32_t lat_swap, uint32_t variant,
                     const double* poses, int32_t n_poses, double t2_ap, double t2_lat, void* workspace,
                     int64_t ws_bytes, double* moments, double* loss, double* grad, void* stream);
+
+/* Launch-plan autotuner of fvr_pn_evaluate.  No counterpart in the reference, whose
+ * Triton launches use a fixed BLOCK = 256 (triton_slice.py:281, 312, 524, 553); it is
+ * the C form of the plan search fvr.multistart.autotune makes in Python: runs fvr_pn_evaluate
+ * on these poses with every (tile height 32 / 64 / 128, samples in flight 2 / 3,
+ * default or biplanar tile order) variant the library has for the variant's dtype,
+ * flags and exact_box bits -- each once to warm up, then `reps` times between two HIP
+ * events on the stream -- and returns the fastest variant and its mean time (ms).
+ * The templates may carry any valid tile height: each candidate runs on copies whose
+ * tile_h is the candidate's (a caller then builds its templates with
+ * (*best_variant >> 8) & 0xFF).  Candidates whose fvr_pn_evaluate_workspace exceeds
+ * ws_bytes are skipped; loss / grad receive the evaluations (n_poses and 6 n_poses
+ * doubles).  Results are cached per (device, dtype / flags / exact_box, padded dims,
+ * slice sizes, lateral axis order, pose count) behind a mutex; with use_cache a cached
+ * workload returns at once without a launch.  A set-up call: it synchronises the
+ * stream and is not for graph capture. */
+int fvr_autotune_pn_evaluate(const fvr_layout* layouts_dev, int32_t n_layouts, const fvr_sncc_item* tmpl_host,
+                             double pz, double py, double px, int32_t lat_swap, uint32_t variant, const double* poses,
+                             int32_t n_poses, double t2_ap, double t2_lat, void* workspace, int64_t ws_bytes,
+                             double* loss, double* grad, int32_t reps, int32_t use_cache, uint32_t* best_variant,
+                             float* best_ms, void* stream);
+
+/* Empty the autotuner's cache (every device). */
+int fvr_autotune_clear(void);
 
 /* fvr_pn_evaluate without a workspace: no counterpart in the reference; it is for
  * callers that cannot allocate (code under HIP graph capture, tight serving loops,

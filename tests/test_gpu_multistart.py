@@ -326,13 +326,15 @@ def test_graph_loop_recaptures_after_plan_targets_or_workspace_change(vols):
 def test_one_call_entry_point_equals_three_launches(storage):
     """fvr_pn_evaluate (one C call: set-up, fused kernel, reduce + combine in a caller
     workspace) gives the three-launch evaluation's losses and gradients bit for bit, for
-    every tile height and depth, and a second call with fewer poses reuses the workspace."""
+    every tile height and depth, in the default order and in the kernel's map-free
+    target-major order (against the host-built target map), and a second call with fewer
+    poses reuses the workspace."""
     from fvr.multistart import candidate_plans
     from fvr.sncc import BiplanarEvaluator
     r = _renderer(40, 13, storage)
     ta, tl = _targets(r, np.random.default_rng(13).uniform(-4, 4, 6))
     ev = BiplanarEvaluator(r.packed, ta, tl, adjoint=True)
-    for plan in [p for p in candidate_plans(ev.flags) if p.order == "xcd" and p.persist == 0]:
+    for plan in [p for p in candidate_plans(ev.flags) if p.persist == 0]:
         ev.set_plan(plan)
         for n, seed in ((11, 3), (5, 4)):
             P = _poses(n, seed)
@@ -340,3 +342,28 @@ def test_one_call_entry_point_equals_three_launches(storage):
             L1, G1 = (x.cpu().numpy().copy() for x in ev.evaluate_call(P))
             np.testing.assert_array_equal(L1, L0, err_msg=str(plan))
             np.testing.assert_array_equal(G1, G0, err_msg=str(plan))
+
+
+def test_autotune_entry_point_picks_a_runnable_plan_and_caches_it():
+    """fvr_autotune_pn_evaluate times the (tile height, depth, order) variants of
+    fvr_pn_evaluate in the library and returns one the evaluator runs: the C call with it
+    gives the three-launch result of that plan bit for bit; a second call is a cache hit
+    (the same variant and time); after fvr_autotune_clear the search runs again and still
+    returns a valid plan."""
+    from fvr import _lib
+    from fvr.sncc import BiplanarEvaluator
+    r = _renderer(40, 13, "fp16")
+    ta, tl = _targets(r, np.random.default_rng(14).uniform(-4, 4, 6))
+    ev = BiplanarEvaluator(r.packed, ta, tl, adjoint=True)
+    P = _poses(9, 5)
+    _lib.check(_lib.lib().fvr_autotune_clear(), "fvr_autotune_clear")
+    plan, ms = ev.autotune_call(P, reps=2)
+    assert ms > 0 and plan.tile_h in (32, 64, 128) and plan.order in ("xcd", "target") and ev.plan == plan
+    L0, G0 = (x.cpu().numpy().copy() for x in ev.evaluate_device(P))
+    L1, G1 = (x.cpu().numpy().copy() for x in ev.evaluate_call(P))
+    np.testing.assert_array_equal(L1, L0)
+    np.testing.assert_array_equal(G1, G0)
+    assert ev.autotune_call(P, reps=2) == (plan, ms)
+    _lib.check(_lib.lib().fvr_autotune_clear(), "fvr_autotune_clear")
+    plan3, ms3 = ev.autotune_call(P, reps=2)
+    assert ms3 > 0 and plan3.tile_h in (32, 64, 128)

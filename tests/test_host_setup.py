@@ -290,6 +290,44 @@ def test_pn_evaluate_workspace_size_and_argument_checks():
     assert call(dims=(48.0, 48.0, 46.0)) == 1 and b"padded dims" in L.fvr_last_error()
 
 
+def test_autotune_and_biplanar_order_argument_checks():
+    """fvr_autotune_pn_evaluate rejects bad arguments before any HIP call (as
+    fvr_pn_evaluate does, plus its own: repetitions, an unaligned workspace, missing
+    outputs); fvr_autotune_clear needs no device; the fused kernel's biplanar tile order
+    (variant bit 21) refuses a tile map and an odd item count, and bits above 21 are
+    refused (no GPU here)."""
+    from fvr import _items, _lib
+    L = _lib.lib()
+    pk = _Packed((48, 48, 48))
+    flags = _lib.ITEM_ADJOINT
+    tm = _lib.as_struct(np.concatenate([_items.n_template(pk, "ap", 1234, flags, 64),
+                                        _items.n_template(pk, "lateral", 5678, flags, 64)]), _lib.ITEM_DTYPE)
+    fake = ctypes.c_void_p(1 << 20)  # never dereferenced: every call below fails its checks first
+    var = 1 << 4 | 64 << 8
+    need = L.fvr_pn_evaluate_workspace(tm.ctypes.data, 4)
+    best, ms = ctypes.c_uint32(0), ctypes.c_float(0.0)
+
+    def tune(variant=var, ws=fake, reps=3, out=True, n=4):
+        return L.fvr_autotune_pn_evaluate(fake, 2, tm.ctypes.data, 48.0, 48.0, 48.0, 1, variant, fake, n, 1.0, 1.0,
+                                          ws, need, fake, fake, reps, 1, ctypes.byref(best) if out else None,
+                                          ctypes.byref(ms), None)
+
+    assert tune(reps=0) == 1 and b"bad arguments" in L.fvr_last_error()
+    assert tune(out=False) == 1
+    assert tune(ws=ctypes.c_void_p((1 << 20) + 8)) == 1 and b"aligned" in L.fvr_last_error()
+    assert tune(variant=1 << 4 | 128 << 8) == 1 and b"tile_h" in L.fvr_last_error()
+    assert tune(n=0) == 1
+    assert best.value == 0
+    assert L.fvr_autotune_clear() == 0
+
+    def plan(variant, n_items=2, tile_map=None):
+        return L.fvr_sncc_eval_plan(fake, 2, fake, n_items, 8, fake, variant, tile_map, 8, 0, None)
+
+    assert plan(var | 1 << 21, tile_map=fake) == 1 and b"biplanar" in L.fvr_last_error()
+    assert plan(var | 1 << 21, n_items=3) == 1 and b"biplanar" in L.fvr_last_error()
+    assert plan(var | 1 << 22) == 1 and b"bad variant" in L.fvr_last_error()
+
+
 def test_pn_evaluate_noalloc_rejects_batches_beyond_its_scratch_before_any_device_call():
     """fvr_pn_evaluate_noalloc's size limit (the static scratch array of
     FVR_PN_SCRATCH_BYTES): a batch whose fvr_pn_evaluate_workspace exceeds it is refused
