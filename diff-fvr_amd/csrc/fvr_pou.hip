@@ -553,7 +553,19 @@ __device__ __forceinline__ unsigned long long fix64(float x) {
   return (unsigned long long)__double_as_longlong(double(x) + 6755399441055744.0) - 0x4338000000000000ull;
 }
 
-template <bool IMG>
+// global-memory view of a uniform base pointer: loads at base + a 32-bit byte offset use the
+// scalar base register and one offset register
+typedef const char __attribute__((address_space(1)))* pou_gcptr;
+__device__ __forceinline__ pou_gcptr pou_global(const void* p) {
+  return reinterpret_cast<pou_gcptr>(reinterpret_cast<uintptr_t>(p));
+}
+__device__ __forceinline__ float pou_ld(pou_gcptr base, uint32_t byte_off) {
+  return *reinterpret_cast<const float __attribute__((address_space(1)))*>(base + byte_off);
+}
+
+// POW2: P is a power of two, so a corner's column / row is wrapped into the image with a mask
+// (an out-of-range corner then reads some texel of the image and counts 0) instead of clamped
+template <bool IMG, bool POW2>
 __device__ __forceinline__ void warp_adjoint_scan(const float* img, const float* go, const float* rx,
                                                   const float* ry, const float* ic, const PouWarp& g, int n, float db,
                                                   float c_u, float c_v, float scale, unsigned long long* acc,
@@ -562,21 +574,23 @@ __device__ __forceinline__ void warp_adjoint_scan(const float* img, const float*
   // a NaN scale marks a pose whose output gradient holds NaN or inf: no fixed-point scatter
   // (the conversion back multiplies by 1 / NaN, so that pose's image gradient is NaN)
   const bool scatter = IMG && scale == scale;
-  const float* rxn = rx + size_t(n) * g.ray_stride;
-  const float* ryn = ry + size_t(n) * g.ray_stride;
-  const float* icn = ic + size_t(n) * g.ray_stride;
-  const float* gon = go + size_t(n) * K;
+  const pou_gcptr imgb = pou_global(img);
+  const pou_gcptr rxn = pou_global(rx + size_t(n) * g.ray_stride);
+  const pou_gcptr ryn = pou_global(ry + size_t(n) * g.ray_stride);
+  const pou_gcptr icn = pou_global(ic + size_t(n) * g.ray_stride);
+  const pou_gcptr gon = pou_global(go + size_t(n) * K);
+  const int mask = P - 1;
   for (int q0 = threadIdx.x; q0 < K; q0 += kScanBatch * stride) {
     // 1) the pixels' rays and output gradients
     float r_x[kScanBatch], r_y[kScanBatch], gg[kScanBatch];
 #pragma unroll
     for (int t = 0; t < kScanBatch; ++t) {
-      const unsigned q = unsigned(min(q0 + t * stride, K - 1));
-      r_x[t] = rxn[q];
-      r_y[t] = ryn[q];
-      gg[t] = q0 + t * stride < K ? gon[q] * icn[q] : 0.f;
+      const uint32_t qb = uint32_t(min(q0 + t * stride, K - 1)) * 4u;
+      r_x[t] = pou_ld(rxn, qb);
+      r_y[t] = pou_ld(ryn, qb);
+      gg[t] = q0 + t * stride < K ? pou_ld(gon, qb) * pou_ld(icn, qb) : 0.f;
     }
-    // 2) their four image corners: clamped coordinates, out-of-range corners count 0
+    // 2) their four image corners: addresses inside the image, out-of-range corners count 0
     float tx[kScanBatch], ty[kScanBatch], vv[kScanBatch][4];
     int xa[kScanBatch], xb[kScanBatch], ya[kScanBatch], yb[kScanBatch];
     bool in[kScanBatch][4];
@@ -590,16 +604,24 @@ __device__ __forceinline__ void warp_adjoint_scan(const float* img, const float*
       const int x0 = int(x0f), y0 = int(y0f);
       const bool ix0 = unsigned(x0) < unsigned(P), ix1 = unsigned(x0 + 1) < unsigned(P);
       const bool iy0 = unsigned(y0) < unsigned(P), iy1 = unsigned(y0 + 1) < unsigned(P);
-      xa[t] = min(max(x0, 0), P - 1);
-      xb[t] = min(max(x0 + 1, 0), P - 1);
-      ya[t] = min(max(y0, 0), P - 1);
-      yb[t] = min(max(y0 + 1, 0), P - 1);
+      if (POW2) {
+        xa[t] = x0 & mask;
+        xb[t] = (x0 + 1) & mask;
+        ya[t] = y0 & mask;
+        yb[t] = (y0 + 1) & mask;
+      } else {
+        xa[t] = min(max(x0, 0), mask);
+        xb[t] = min(max(x0 + 1, 0), mask);
+        ya[t] = min(max(y0, 0), mask);
+        yb[t] = min(max(y0 + 1, 0), mask);
+      }
       in[t][0] = ix0 && iy0;
       in[t][1] = ix1 && iy0;
       in[t][2] = ix0 && iy1;
       in[t][3] = ix1 && iy1;
-      const unsigned ra = __umul24(ya[t], P), rb = __umul24(yb[t], P);
-      const float f00 = img[ra + xa[t]], f01 = img[ra + xb[t]], f10 = img[rb + xa[t]], f11 = img[rb + xb[t]];
+      const uint32_t ra = __umul24(ya[t], P), rb = __umul24(yb[t], P);
+      const float f00 = pou_ld(imgb, (ra + xa[t]) * 4u), f01 = pou_ld(imgb, (ra + xb[t]) * 4u);
+      const float f10 = pou_ld(imgb, (rb + xa[t]) * 4u), f11 = pou_ld(imgb, (rb + xb[t]) * 4u);
       vv[t][0] = in[t][0] ? f00 : 0.f;
       vv[t][1] = in[t][1] ? f01 : 0.f;
       vv[t][2] = in[t][2] ? f10 : 0.f;
@@ -616,12 +638,13 @@ __device__ __forceinline__ void warp_adjoint_scan(const float* img, const float*
       s[1] = fmaf(-gg[t], dsx, s[1]);
       s[2] = fmaf(-gg[t], dsy, s[2]);
       if (scatter && gg[t] != 0.f && (in[t][0] || in[t][1] || in[t][2] || in[t][3])) {
-        const float gs = gg[t] * scale;  // a power of two: the product rounds as gg * w would
+        const float gs = gg[t] * scale;  // a power of two: exact
+        const float gox = gs * ox, gtx = gs * tx[t];
         const unsigned ra = __umul24(ya[t], acc_pitch), rb = __umul24(yb[t], acc_pitch);
-        atomicAdd(acc + ra + xa[t], in[t][0] ? fix64(gs * (ox * oy)) : 0ull);
-        atomicAdd(acc + ra + xb[t], in[t][1] ? fix64(gs * (tx[t] * oy)) : 0ull);
-        atomicAdd(acc + rb + xa[t], in[t][2] ? fix64(gs * (ox * ty[t])) : 0ull);
-        atomicAdd(acc + rb + xb[t], in[t][3] ? fix64(gs * (tx[t] * ty[t])) : 0ull);
+        atomicAdd(acc + ra + xa[t], fix64(in[t][0] ? gox * oy : 0.f));
+        atomicAdd(acc + ra + xb[t], fix64(in[t][1] ? gtx * oy : 0.f));
+        atomicAdd(acc + rb + xa[t], fix64(in[t][2] ? gox * ty[t] : 0.f));
+        atomicAdd(acc + rb + xb[t], fix64(in[t][3] ? gtx * ty[t] : 0.f));
       }
     }
   }
@@ -638,10 +661,11 @@ __global__ __launch_bounds__(kPouThreads) void pou_warp_bwd_kernel(
   const size_t pp = size_t(g.P) * g.P;
   float s[3] = {0.f, 0.f, 0.f};
   if (gacc)
-    warp_adjoint_scan<true>(imgs + bn * pp, go, rx, ry, ic, g, n, d_b[bn], cu[bn], cv[bn], gscale[n],
-                            gacc + bn * pp, g.P, s);
+    warp_adjoint_scan<true, false>(imgs + bn * pp, go, rx, ry, ic, g, n, d_b[bn], cu[bn], cv[bn], gscale[n],
+                                   gacc + bn * pp, g.P, s);
   else
-    warp_adjoint_scan<false>(imgs + bn * pp, go, rx, ry, ic, g, n, d_b[bn], cu[bn], cv[bn], 1.f, nullptr, g.P, s);
+    warp_adjoint_scan<false, false>(imgs + bn * pp, go, rx, ry, ic, g, n, d_b[bn], cu[bn], cv[bn], 1.f, nullptr, g.P,
+                                    s);
   block_sum<3>(s, red);
   if (threadIdx.x < 3) part[size_t(bn) * 3 + threadIdx.x] = s[threadIdx.x];
 }
@@ -677,8 +701,8 @@ __global__ __launch_bounds__(kPouSliceThreads) void pou_render_bwd_kernel(
   // 1) warp adjoint: geometry sums + image gradient in LDS fixed point
   float s3[3] = {0.f, 0.f, 0.f};
   const float scale = gscale[n];
-  warp_adjoint_scan<true>(imgs + size_t(bn) * P * P, go, rx, ry, ic, g, n, d_b[bn], cu[bn], cv[bn], scale, acc,
-                          pitch, s3);
+  warp_adjoint_scan<true, true>(imgs + size_t(bn) * P * P, go, rx, ry, ic, g, n, d_b[bn], cu[bn], cv[bn], scale,
+                                acc, pitch, s3);  // P: a power of two here
   __syncthreads();
   // 2) fixed point -> float, ifftshifted and repacked in place for a real-input transform: row
   // y holds the P / 2 complex values z(n) = g(y, 2n) + i g(y, 2n + 1) at pitch P / 2 + 1 (every
