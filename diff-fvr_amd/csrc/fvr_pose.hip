@@ -99,6 +99,12 @@ __device__ __forceinline__ void build_item(const double* P, int pose, bool lat, 
   }
   put_fx(it.c, cg, swap);
   for (int a = 0; a < 3; ++a) it.c[a] += kSnapFx;  // lattice-plane bias, as the host builder
+  // a pose with a NaN or infinite entry: the item reads nothing (a layout index out of range
+  // gives zero partials) and its Jacobians are NaN, which the combine turns into a NaN loss
+  // and gradient for the pose
+  bool finite = true;
+  for (int k = 0; k < 6; ++k) finite = finite && isfinite(P[k]);
+  if (!finite) it.layout = -1;
   put_fx(it.ustep, u, swap);
   put_fx(it.vstep, v, swap);
   const double a = lat ? P[4] : P[3];  // AP: tx, LAT: ty (gradients.py:90-94)
@@ -112,7 +118,7 @@ __device__ __forceinline__ void build_item(const double* P, int pose, bool lat, 
   it.tile_base = tile0 + pose * (t_ap + t_lat) + (lat ? t_ap : 0);
   items[idx] = it;
   double* J = jac + size_t(idx) * 18;
-  const double k = deg;
+  const double k = finite ? deg : __builtin_nan("");
   if (lat) {
     put_jac(J, v3(dy1.a0 * k, dy1.a1 * k, dy1.a2 * k), v3(dp1.a0 * k, dp1.a1 * k, dp1.a2 * k),
             v3(dr1.a0 * k, dr1.a1 * k, dr1.a2 * k), swap);
@@ -200,6 +206,11 @@ __device__ void reduce_combine_pose(const PoseStage& st, int pose, const float* 
       moments[size_t(2 * pose + view) * FVR_SNCC_NMOM + lane] = mv;
     }
     const double* J = st.J[view];
+    if (J[0] != J[0]) {  // a pose with a non-finite entry (set-up marks its Jacobians NaN)
+      L = J[0];
+      for (int a = 0; a < 6; ++a) g[a] = J[0];
+      continue;
+    }
     const double nt = sqrt(view ? t2_lat : t2_ap), ns = sqrt(fmax(m[1], 0.0));
     double ncc = 0.0, d0 = 0.0, d1 = 0.0, d2 = 0.0, da = 0.0, db = 0.0;
     if (ns * nt != 0.0) {
@@ -403,16 +414,22 @@ __global__ void pt_build_kernel(const double* __restrict__ poses, int n, fvr_snc
   it.ph_col = turns_fx(df * P[4] / span);
   it.tile_base = pose * (tmpl.tiles_x * sncc_tiles_y(tmpl.h, tmpl.flags, tmpl.tile_h));
   it.target = G.target0 + (unsigned long long)pose * G.target_stride;
+  // a non-finite angle or in-plane translation (tz does not enter convention T): the item reads
+  // nothing and its Jacobians are NaN, which the combine turns into a NaN NCC and gradient
+  bool finite = true;
+  for (int k = 0; k < 5; ++k) finite = finite && isfinite(P[k]);
+  if (!finite) it.layout = -1;
   items[pose] = it;
   // jac[p][0][a][theta] = d v / d theta, jac[p][1][a][theta] = d u / d theta (layout axes, per degree)
   double* J = jac + size_t(pose) * 18;
+  const double jdeg = finite ? deg : __builtin_nan("");
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
     for (int t = 0; t < 3; ++t) {
       const int k = G.ord[a];
-      J[a * 3 + t] = pick3(dR[t].m[1], k) * pick3(s, k) * deg;
-      J[9 + a * 3 + t] = pick3(dR[t].m[0], k) * pick3(s, k) * deg;
+      J[a * 3 + t] = pick3(dR[t].m[1], k) * pick3(s, k) * jdeg;
+      J[9 + a * 3 + t] = pick3(dR[t].m[0], k) * pick3(s, k) * jdeg;
     }
 }
 
@@ -458,7 +475,13 @@ __global__ __launch_bounds__(64) void pt_combine_kernel(const fvr_sncc_item* __r
   d[3] = two_pi / span * m[3] / Dn;  // d/dtx: row moments (rows follow u)
   d[4] = two_pi / span * m[2] / Dn;  // d/dty: column moments
   d[5] = 0.0;                        // tz does not enter convention T
-  if (lane == 0) ncc_out[pose] = ncc;
+  double ncc_p = ncc;
+  if (J[0] != J[0]) {  // a pose with a non-finite entry (set-up marks its Jacobians NaN)
+    ncc_p = J[0];
+#pragma unroll
+    for (int a = 0; a < 6; ++a) d[a] = J[0];
+  }
+  if (lane == 0) ncc_out[pose] = ncc_p;
   if (lane < 6) {
     double gv = 0.0;
 #pragma unroll

@@ -367,3 +367,27 @@ def test_autotune_entry_point_picks_a_runnable_plan_and_caches_it():
     _lib.check(_lib.lib().fvr_autotune_clear(), "fvr_autotune_clear")
     plan3, ms3 = ev.autotune_call(P, reps=2)
     assert ms3 > 0 and plan3.tile_h in (32, 64, 128)
+
+
+def test_non_finite_poses_give_nan_and_leave_the_other_poses_unchanged():
+    """A pose with a NaN or infinite entry (any of the six, tz included: it enters both views'
+    phase ramps) gets a NaN loss and gradient -- its items read nothing and its Jacobians
+    are NaN -- through the three-launch and the one-call paths, and every other pose of the
+    batch keeps its result bit for bit."""
+    from fvr.sncc import BiplanarEvaluator
+    r = _renderer(40, 13, "fp16")
+    ta, tl = _targets(r, np.random.default_rng(15).uniform(-4, 4, 6))
+    ev = BiplanarEvaluator(r.packed, ta, tl, adjoint=True)
+    P = _poses(7, 21)
+    L0, G0 = (x.cpu().numpy().copy() for x in ev.evaluate_device(P))
+    Q = P.clone()
+    Q[1, 0] = float("nan")
+    Q[3, 4] = float("inf")
+    Q[5, 5] = -float("inf")
+    bad = [1, 3, 5]
+    good = [0, 2, 4, 6]
+    for run in (ev.evaluate_device, ev.evaluate_call):
+        L1, G1 = (x.cpu().numpy().copy() for x in run(Q))
+        assert np.all(np.isnan(L1[bad])) and np.all(np.isnan(G1[bad]))
+        np.testing.assert_array_equal(L1[good], L0[good])
+        np.testing.assert_array_equal(G1[good], G0[good])

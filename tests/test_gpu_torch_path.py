@@ -181,6 +181,36 @@ def test_per_pose_targets_and_batched_equals_single(eng):
     assert abs(Lb - 0.5 * (L0 + L1)) < 1e-6
 
 
+def test_non_finite_pose_gives_nan(eng):
+    """forward_spectral_ncc with a NaN angle or in-plane translation in one pose: the loss
+    (1 - mean NCC) is NaN and that pose's gradient is NaN; the other poses' gradients are the
+    ones of the same batch with that pose finite, bit for bit (tz does not enter
+    convention T, so a NaN there changes nothing)."""
+    e, S3 = eng
+    R = 24
+    ft, _ = _target(e, S3, [1.0, -0.5, 0.5, 0.4, -0.3, 0.0], "ap", R)
+    base = torch.tensor([[3.0, -2.0, 1.0, 1.5, -1.0, 0.3], [-4.0, 2.5, -1.5, -0.8, 1.2, 0.0],
+                         [0.5, 0.5, -0.5, 0.2, 0.1, 0.0]])
+
+    def run(p):
+        x = p.clone().requires_grad_(True)
+        loss = e.forward_spectral_ncc(x, ft, view="ap")
+        loss.backward()
+        return float(loss), x.grad.double().numpy()
+
+    L0, g0 = run(base)
+    for col in (0, 3):
+        bad = base.clone()
+        bad[1, col] = float("nan")
+        L1, g1 = run(bad)
+        assert np.isnan(L1) and np.all(np.isnan(g1[1]))
+        np.testing.assert_array_equal(g1[[0, 2]], g0[[0, 2]])
+    tz = base.clone()
+    tz[1, 5] = float("nan")
+    L2, g2 = run(tz)
+    assert L2 == L0 and np.array_equal(g2, g0)
+
+
 # --- the reference's engine checks (tests/test_torch_engine.py) on the HIP path --------------
 def test_reference_forward_batch_and_autograd(eng):
     e, _ = eng
