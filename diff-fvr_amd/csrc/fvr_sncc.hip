@@ -144,27 +144,6 @@ __device__ __forceinline__ float layout_scale(const fvr_layout& L) {
   else return L.scales ? *reinterpret_cast<gfptr>(reinterpret_cast<gcptr>(L.scales)) : 0.f;
 }
 
-// moments accumulated in units of sc -> spectrum units: the sums linear in the slice
-// take sc, the quadratic ones (E and the H moments) sc^2.  sc is a power of two, so the
-// result is the one of scaling every sample.
-template <int DT>
-__device__ __forceinline__ void acc_scale(Acc& ac, float sc) {
-  if constexpr (DT != FVR_C32) {
-    const float s2 = sc * sc;
-    ac.A *= sc;
-    ac.sim *= sc;
-    ac.mrim *= sc;
-    ac.E *= s2;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      ac.gs[a] *= sc;
-      ac.gr[a] *= sc;
-      ac.hs[a] *= s2;
-      ac.hr[a] *= s2;
-    }
-  }
-}
-
 __device__ __forceinline__ float readlane_f(float v, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
@@ -178,8 +157,12 @@ __device__ __forceinline__ cpx phasor(uint32_t t) {
 }
 
 // deterministic workgroup reduction of the 16 moments into partials[tile][16];
-// the column-weighted sums are scaled by cw (1 when accumulated per sample)
-__device__ __forceinline__ void tile_reduce(const Acc& ac, float cw, int tile, float* __restrict__ partials) {
+// the column-weighted sums are scaled by cw (1 when accumulated per sample) and the
+// tile sums, accumulated in units of the spectrum's scale sc, are brought to spectrum
+// units: sc for the sums linear in the slice, sc^2 for E and the H moments.  sc is a
+// power of two (1 for complex64 storage), so this equals scaling every sample.
+__device__ __forceinline__ void tile_reduce(const Acc& ac, float cw, float sc, int tile,
+                                            float* __restrict__ partials) {
   __shared__ float red[FVR_SNCC_NMOM][257];
   const int tid = threadIdx.x;
   red[0][tid] = ac.A;
@@ -200,6 +183,7 @@ __device__ __forceinline__ void tile_reduce(const Acc& ac, float cw, int tile, f
   for (int jj = 0; jj < 16; ++jj) part += red[qn][s * 16 + jj];
 #pragma unroll
   for (int o = 8; o > 0; o >>= 1) part += __shfl_xor(part, o, 16);
+  part *= (qn == 1 || qn >= 10) ? sc * sc : sc;
   if (s == 0) partials[size_t(tile) * FVR_SNCC_NMOM + qn] = part;
   __syncthreads();  // red is reused by the workgroup's next tile
 }
@@ -418,8 +402,7 @@ __device__ __forceinline__ void sncc_tile(const fvr_layout* __restrict__ layouts
   consume(sa, R - 2);
   consume(sb, R - 1);
 
-  acc_scale<DT>(ac, gsc);
-  tile_reduce(ac, cw, tile, partials);
+  tile_reduce(ac, cw, gsc, tile, partials);
 }
 
 // Slot loop: one tile per slot, slots walked with stride gridDim.x (a grid of
@@ -501,6 +484,7 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
   const bool first = r0 == 0;
   const int n_r = (nrow + 1) >> 1;  // steps
   const bool full = nrow == 2 * R;  // every step of every lane is a row of the fold
+  const int vlim = colok ? nrow - rl : 0;  // step t of the lane is a row of the fold iff 2t < vlim
 
   const uint32_t rowb = L.row_bytes, planeb = L.plane_bytes;
   const gcptr b0 = reinterpret_cast<gcptr>(L.base);
@@ -523,12 +507,19 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
   const int64_t bj0 = it.c[0] + int64_t(jc) * it.ustep[0];
   const int64_t bj1 = it.c[1] + int64_t(jc) * it.ustep[1];
   const int64_t bj2 = it.c[2] + int64_t(jc) * it.ustep[2];
-  // the lane's column phasor carries its row offset rl (exact 32-bit turn counts): the phase
-  // of step t is then colph x rowph(lane 2t)
-  const cpx colph = phasor(it.ph_col0 + uint32_t(j) * it.ph_col + uint32_t(rl) * it.ph_row);
+  // Phasors of the tile (exact 32-bit turn counts), computed once per workgroup and shared
+  // through LDS: the lanes' column phasors, the same in every wave (each carries its lane's
+  // row offset rl), by wave 0; the phasors of the tile's even rows h/2 + tyi 8R + 2m, m < 4R,
+  // by waves 1 .. 4R/64.  The phase of step t of wave w is colph x rowph(m = w R + t).
+  __shared__ cpx ph_tile[64 + 4 * R];
+  if (wave == 0) {
+    ph_tile[lane] = phasor(it.ph_col0 + uint32_t(j) * it.ph_col + uint32_t(rl) * it.ph_row);
+  } else {
+    const int m = (wave - 1) * 64 + lane;
+    if (m < 4 * R) ph_tile[64 + m] = phasor(it.ph_row0 + uint32_t(hh + tyi * (8 * R) + 2 * m) * it.ph_row);
+  }
+  cpx colph, rowph;  // read after the barrier below (rowph: lane t <-> step t)
   const float cwj = it.cw0 + float(j) * it.cws;
-  // row phasors of this wave's rows (lane k <-> row h/2 + r0 + k, k < 2R <= 64)
-  const cpx rowph = phasor(it.ph_row0 + uint32_t(hh + r0 + lane) * it.ph_row);
   // a lane's rows are every other row: position, target offset and row weight are stepped
   // by two rows from the lane's first row (stepped positions carry the storage offset
   // lo_int in their high word: the high word is then the low corner's storage index)
@@ -581,9 +572,11 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
       sm.raw.load(b0, b1, Raw<DT>::offset(S, M, F, planeb, rowb), rowb);
       sm.t = *reinterpret_cast<gf4ptr>(TT + toffr);
     } else {
+      // (tested waves step their positions without the storage offset: it is taken out once
+      // before their rows)
       const bool h2 = first && rl == 0 && t == 0;
-      const bool valid = colok && (rl + 2 * t < nrow) && (!h2 || hvalid);
-      load_tested(sm, q0 - lob, q1 - lob, q2 - lob, valid, h2 ? hmirror : j >= 1, valid ? toffr : 0u);
+      const bool valid = (2 * t < vlim) && (!h2 || hvalid);
+      load_tested(sm, q0, q1, q2, valid, h2 ? hmirror : j >= 1, valid ? toffr : 0u);
     }
     q0 += v0;
     q1 += v1;
@@ -659,7 +652,7 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
 
   // stage 2 of step t (steps in order: rw is stepped)
   auto consume = [&](const SmpH<DT>& sm, int t, auto interior, auto unit_we) {
-    const float prr = readlane_f(rowph.r, 2 * t), pri = readlane_f(rowph.i, 2 * t);
+    const float prr = readlane_f(rowph.r, t), pri = readlane_f(rowph.i, t);
     const float phr = colph.r * prr - colph.i * pri;
     const float phi = fmaf(colph.r, pri, colph.i * prr);
     const float rw = rwr;
@@ -693,6 +686,10 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
   // every axis for every lane, every sample and its mirror are inside the box (interior);
   // when on some axis both lie below 0 or both above K for every lane, none is (outside: the
   // rows contribute exactly 0 and are skipped).  Other waves test every sample.
+  __syncthreads();
+  colph = ph_tile[lane];
+  rowph = ph_tile[64 + wave * R + (lane < R ? lane : 0)];
+
   int cls = 0;  // 0: per-sample tests, 1: interior, 2: outside
   if (full && !(opts & kHermExactBox)) {
     const int64_t f0 = q0 - lob, f1 = q1 - lob, f2 = q2 - lob;
@@ -742,14 +739,16 @@ __device__ __forceinline__ void sncc_herm_tile(const fvr_layout* __restrict__ la
   } else if (cls == 1) {
     run(std::true_type{}, std::false_type{});
   } else if (cls == 0) {
+    q0 -= lob;
+    q1 -= lob;
+    q2 -= lob;
     run(std::false_type{}, std::false_type{});
   }
   if (first) {
     extra_issue();
     extra_consume();
   }
-  acc_scale<DT>(ac, gsc);
-  tile_reduce(ac, 1.f, tile, partials);
+  tile_reduce(ac, 1.f, gsc, tile, partials);
 }
 
 template <int DT, bool ADJ, int R, int DEPTH>
