@@ -301,14 +301,16 @@ def pou_config4(steps: int, warmup: int, size: int = 256, n_pose: int = 64):
                          "note": "whole forward + adjoint step (slice + LDS iFFT, warp, fused adjoint) over 64 poses"}}
 
 
-def pou_reference_workload(runs: int = 60, warmup: int = 15, n_pose: int = 64):
-    """The PoU-FVR workload of the reference's published GPU figure (exp29:259-270,
-    296-311, 395-405: 2x2x2 blocks @ 64 (P = 128), BF16 block spectra, 256^2 pinhole
-    detector at 1.0 mm, SDD 1020 mm, source 850 mm from the isocentre, B = 64 poses with
-    rotations (5, -3, 4) + N(0, 2^2) deg; forward + backward of forward_pinhole(...).sum() to
-    the pose matrices, and the forward alone).  The reference ran it on a DeepFluoro CT,
-    which is not available here: a seeded 256^3 ellipsoid phantom at 1 mm stands in (the
-    cost follows the block count, P, the detector and the batch, not the volume's content)."""
+def pou_reference_workload(runs: int = 60, warmup: int = 15):
+    """The PoU-FVR workloads of the reference's published GPU figures (exp29:259-270,
+    296-311, 395-405; exp29_gpu_benchmark.csv:81, 92, 98-99, 103, 136): 2x2x2 blocks @ 64
+    (P = 128), BF16 block spectra, pinhole detector of H^2 pixels at 2 * 128 / H mm, SDD
+    1020 mm, source 850 mm from the isocentre, B poses with rotations (5, -3, 4) + N(0, 2^2)
+    deg; forward + backward of forward_pinhole(...).sum() to the pose matrices, and the
+    forward alone, at (B, H) = (64, 256) -- the headline point -- (16, 256), (1, 256) and
+    (64, 512).  The reference ran them on a DeepFluoro CT, which is not available here: a
+    seeded 256^3 ellipsoid phantom at 1 mm stands in (the cost follows the block count, P,
+    the detector and the batch, not the volume's content)."""
     import numpy as np
     import torch
     from fvr import PerspectiveSlabFVR
@@ -318,21 +320,7 @@ def pou_reference_workload(runs: int = 60, warmup: int = 15, n_pose: int = 64):
     vol = ellipsoid_phantom(256, n_ellipsoids=12, seed=1234).astype(np.float32)
     model = PerspectiveSlabFVR.from_volume(vol, vox_mm=1.0, grid_blocks=(2, 2, 2), block_res=64, pad_mult=2,
                                            dtype=torch.bfloat16, device=dev)
-    det, pix, sdd, sid = 256, 1.0, 1020.0, 850.0
-    f = sdd / pix
-    k_inv = torch.tensor([[1.0 / f, 0.0, -det / (2 * f)], [0.0, 1.0 / f, -det / (2 * f)], [0.0, 0.0, 1.0]], device=dev)
-    lattice = model.make_pinhole_lattice(k_inv, sdd, det, det)
-    rng = np.random.default_rng(29)
-    rot = np.array([5.0, -3.0, 4.0]) + rng.normal(0.0, 2.0, (n_pose, 3))
-    M = np.tile(np.eye(4), (n_pose, 1, 1))
-    M[:, :3, :3] = Rotation.from_euler("zyx", rot, degrees=True).as_matrix()
-    M[:, :3, 3] = (0.0, 0.0, -sid)
-    poses = torch.from_numpy(M).float().to(dev)
-
-    def fwd_bwd():
-        rp = poses.clone().requires_grad_(True)
-        model.forward_pinhole(rp, lattice).sum().backward()
-        return rp.grad
+    sdd, sid = 1020.0, 850.0
 
     def timed(fn):
         for _ in range(warmup):
@@ -344,17 +332,102 @@ def pou_reference_workload(runs: int = 60, warmup: int = 15, n_pose: int = 64):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) * 1e3 / runs, out
 
-    ms_fb, g = timed(fwd_bwd)
-    with torch.no_grad():
-        ms_f, _ = timed(lambda: model.forward_pinhole(poses, lattice))
+    def point(n_pose, det):
+        pix = 2.0 * 128.0 / det
+        f = sdd / pix
+        k_inv = torch.tensor([[1.0 / f, 0.0, -det / (2 * f)], [0.0, 1.0 / f, -det / (2 * f)], [0.0, 0.0, 1.0]],
+                             device=dev)
+        lattice = model.make_pinhole_lattice(k_inv, sdd, det, det)
+        rng = np.random.default_rng(29)
+        rot = np.array([5.0, -3.0, 4.0]) + rng.normal(0.0, 2.0, (n_pose, 3))
+        M = np.tile(np.eye(4), (n_pose, 1, 1))
+        M[:, :3, :3] = Rotation.from_euler("zyx", rot, degrees=True).as_matrix()
+        M[:, :3, 3] = (0.0, 0.0, -sid)
+        poses = torch.from_numpy(M).float().to(dev)
+
+        def fwd_bwd():
+            rp = poses.clone().requires_grad_(True)
+            model.forward_pinhole(rp, lattice).sum().backward()
+            return rp.grad
+
+        ms_fb, g = timed(fwd_bwd)
+        with torch.no_grad():
+            ms_f, _ = timed(lambda: model.forward_pinhole(poses, lattice))
+        return {"B": n_pose, "det": det, "ms_fwd_bwd": round(ms_fb, 4), "ms_fwd": round(ms_f, 4),
+                "poses_per_s_fwd_bwd": round(n_pose / (ms_fb * 1e-3), 1), "grad_finite": bool(torch.isfinite(g).all())}
+
+    head = point(64, 256)
+    ref = {(16, 256): 4.270, (1, 256): 4.210, (64, 512): 21.843}
+    others = []
+    for (b, h), ref_ms in ref.items():
+        r = point(b, h)
+        r["reference_a100_ms_fwd_bwd"] = ref_ms
+        others.append(r)
     return {"workload": "PoU-FVR 2x2x2 blocks @64 (P=128, bf16), 256^2 pinhole det @1.0mm, SDD 1020 / source 850 mm, "
                         "B=64, forward_pinhole(...).sum() fwd+bwd (the reference's published PoU GPU workload; a 256^3 "
                         "phantom stands in for its CT)",
-            "blocks": model.B, "P": model.P, "runs": runs, "ms_fwd_bwd": round(ms_fb, 4), "ms_fwd": round(ms_f, 4),
-            "poses_per_s_fwd_bwd": round(n_pose / (ms_fb * 1e-3), 1), "poses_per_s_fwd": round(n_pose / (ms_f * 1e-3), 1),
-            "grad_finite": bool(torch.isfinite(g).all()),
+            "blocks": model.B, "P": model.P, "runs": runs, "ms_fwd_bwd": head["ms_fwd_bwd"], "ms_fwd": head["ms_fwd"],
+            "poses_per_s_fwd_bwd": head["poses_per_s_fwd_bwd"],
+            "poses_per_s_fwd": round(64 / (head["ms_fwd"] * 1e-3), 1), "grad_finite": head["grad_finite"],
             "reference_a100": {"ms_fwd_bwd": 7.074, "poses_per_s_fwd_bwd": 9047, "ms_fwd_eager": 2.703,
-                               "ms_fwd_cuda_graph": 1.969}}
+                               "ms_fwd_cuda_graph": 1.969},
+            "other_points": others}
+
+
+def reference_drr_workloads(reps: int = 10):
+    """Two more of the reference's published timings, on this build's GPU path.
+    (1) The convention-T `forward` DRR of exp24 (exp24_torch_gpu_benchmark.py:44-105,
+    table24_torch_gpu_benchmark.csv:6): 96^3 volume, AP, out_res 96, B = 64 poses U(-10, 10)
+    (degrees / voxels), forward per DRR and the backward of sum(drr^2); the reference's row
+    has no device recorded (0.418 ms per DRR, backward 34.96 ms).  (2) The parallel-beam
+    spectrum precompute and `render_drr` at 256^3 (exp1_complexity_scaling.py:33-49,
+    table1_complexity_benchmark.csv:5: 6.38 s and 123.9 ms on the author's CPU).  Seeded
+    ellipsoid phantoms stand in for the reference's clinical volume."""
+    import numpy as np
+    import torch
+    from fvr import DifferentiableFourierVolumeRenderer, TorchDiffFVREngine
+    from fvr.phantom import ellipsoid_phantom
+    dev = torch.device("cuda", 0)
+    out = {}
+    eng = TorchDiffFVREngine(ellipsoid_phantom(96, n_ellipsoids=12, seed=1234).astype(np.float32),
+                             spacing=(1.0, 1.0, 1.0), pad_factor=2.0, device=dev)
+    rng = np.random.default_rng(24)
+    poses = torch.tensor(rng.uniform(-10.0, 10.0, (64, 6)).astype(np.float32), device=dev, requires_grad=True)
+    eng(poses, view="ap", out_res=96)
+    torch.cuda.synchronize()
+    fw, bw = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        drrs = eng(poses, view="ap", out_res=96)
+        torch.cuda.synchronize()
+        fw.append(time.perf_counter() - t0)
+        loss = torch.sum(drrs ** 2)
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        bw.append(time.perf_counter() - t1)
+        poses.grad = None
+    fwd_ms = float(np.mean(fw)) * 1e3
+    out["torch_forward_drr_96"] = {"B": 64, "out_res": 96, "forward_ms": round(fwd_ms, 4),
+                                   "per_drr_ms": round(fwd_ms / 64, 5), "backward_ms": round(float(np.mean(bw)) * 1e3, 4),
+                                   "reference": {"per_drr_ms": 0.418, "backward_ms": 34.96, "device": "not recorded"}}
+    vol = ellipsoid_phantom(256, n_ellipsoids=12, seed=1234).astype(np.float32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = DifferentiableFourierVolumeRenderer(vol, pad_factor=2.0, device=dev)
+    torch.cuda.synchronize()
+    prep_s = time.perf_counter() - t0
+    r.render_drr(0.0, 0.0, 0.0, 0.0, 0.0, "ap")
+    ts = []
+    for k in range(reps):
+        t0 = time.perf_counter()
+        img = r.render_drr(2.0 * k, -1.0, 3.0, 1.5, -2.0, "ap")
+        ts.append(time.perf_counter() - t0)
+    out["render_drr_256"] = {"prep_s": round(prep_s, 3), "render_drr_ms": round(float(np.median(ts)) * 1e3, 4),
+                             "image": list(np.asarray(img).shape),
+                             "note": "render_drr returns a host array: the time includes its device-to-host copy",
+                             "reference_author_cpu": {"prep_s": 6.38, "render_drr_ms": 123.9}}
+    return out
 
 
 def multistart_config5(n_steps: int = 100, lat_reps: int = 100):
@@ -711,6 +784,8 @@ def main(argv=None) -> int:
         log(f"PoU: {extra['pou']}")
         extra["pou_reference_workload"] = pou_reference_workload()
         log(f"PoU (the reference's published workload): {extra['pou_reference_workload']}")
+        extra["reference_drr_workloads"] = reference_drr_workloads()
+        log(f"DRR workloads of the reference's published timings: {extra['reference_drr_workloads']}")
     if not args.no_extra:
         log("convention-T engine on the reference's own GPU benchmark workload")
         torch.cuda.empty_cache()
