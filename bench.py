@@ -393,7 +393,9 @@ def reference_drr_workloads(reps: int = 10):
                              spacing=(1.0, 1.0, 1.0), pad_factor=2.0, device=dev)
     rng = np.random.default_rng(24)
     poses = torch.tensor(rng.uniform(-10.0, 10.0, (64, 6)).astype(np.float32), device=dev, requires_grad=True)
-    eng(poses, view="ap", out_res=96)
+    for _ in range(2):  # warm-up of both passes (FFT plans, autograd)
+        torch.sum(eng(poses, view="ap", out_res=96) ** 2).backward()
+        poses.grad = None
     torch.cuda.synchronize()
     fw, bw = [], []
     for _ in range(reps):
@@ -407,9 +409,10 @@ def reference_drr_workloads(reps: int = 10):
         torch.cuda.synchronize()
         bw.append(time.perf_counter() - t1)
         poses.grad = None
-    fwd_ms = float(np.mean(fw)) * 1e3
+    fwd_ms = float(np.median(fw)) * 1e3
     out["torch_forward_drr_96"] = {"B": 64, "out_res": 96, "forward_ms": round(fwd_ms, 4),
-                                   "per_drr_ms": round(fwd_ms / 64, 5), "backward_ms": round(float(np.mean(bw)) * 1e3, 4),
+                                   "per_drr_ms": round(fwd_ms / 64, 5), "backward_ms": round(float(np.median(bw)) * 1e3, 4),
+                                   "timing": f"medians of {reps} calls after two warm-up forward + backward passes",
                                    "reference": {"per_drr_ms": 0.418, "backward_ms": 34.96, "device": "not recorded"}}
     vol = ellipsoid_phantom(256, n_ellipsoids=12, seed=1234).astype(np.float32)
     torch.cuda.synchronize()
