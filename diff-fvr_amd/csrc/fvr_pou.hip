@@ -482,6 +482,12 @@ struct PouWarp {
   float inv_ds, c_idx;
 };
 
+// image position of a detector ray: (d_b r - c) inv_ds + c_idx, with the roundings fixed
+// (two FMAs) so that every kernel of the warp and of its adjoints forms the same coordinate
+__device__ __forceinline__ float pou_img_pos(float db, float r, float c, float inv_ds, float c_idx) {
+  return fmaf(fmaf(db, r, -c), inv_ds, c_idx);
+}
+
 __device__ __forceinline__ float img_at(const float* img, int P, int x, int y) {
   return ((unsigned)x < (unsigned)P && (unsigned)y < (unsigned)P) ? img[y * P + x] : 0.f;
 }
@@ -505,8 +511,8 @@ __global__ __launch_bounds__(256) void pou_warp_fwd_kernel(const float* __restri
   float acc = 0.f;
   for (int b = 0; b < g.B; ++b) {
     const int bn = b * g.N + n;
-    const float px = (d_b[bn] * r_x - cu[bn]) * g.inv_ds + g.c_idx;
-    const float py = (d_b[bn] * r_y - cv[bn]) * g.inv_ds + g.c_idx;
+    const float px = pou_img_pos(d_b[bn], r_x, cu[bn], g.inv_ds, g.c_idx);
+    const float py = pou_img_pos(d_b[bn], r_y, cv[bn], g.inv_ds, g.c_idx);
     const float x0f = floorf(px), y0f = floorf(py);
     const float tx = px - x0f, ty = py - y0f;
     const int x0 = int(x0f), y0 = int(y0f);
@@ -596,8 +602,8 @@ __device__ __forceinline__ void warp_adjoint_scan(const float* img, const float*
     bool in[kScanBatch][4];
 #pragma unroll
     for (int t = 0; t < kScanBatch; ++t) {
-      const float px = (db * r_x[t] - c_u) * g.inv_ds + g.c_idx;
-      const float py = (db * r_y[t] - c_v) * g.inv_ds + g.c_idx;
+      const float px = pou_img_pos(db, r_x[t], c_u, g.inv_ds, g.c_idx);
+      const float py = pou_img_pos(db, r_y[t], c_v, g.inv_ds, g.c_idx);
       const float x0f = floorf(px), y0f = floorf(py);
       tx[t] = px - x0f;
       ty[t] = py - y0f;
@@ -679,6 +685,101 @@ __global__ void pou_fix_to_float_kernel(unsigned long long* __restrict__ acc, co
   out[i] = float(double((long long)acc[i]) / double(gscale[n]));
 }
 
+// Steps of the fused adjoint after the warp adjoint, shared by both warp adjoints: the image
+// gradient g, ifftshifted and packed for a real-input transform in `a` (row y holds the P / 2
+// complex values z(n) = g'(y, 2n) + i g'(y, 2n + 1) at pitch P / 2 + 1, g' = ifftshift(g)), is
+// taken through the adjoint of fftshift(ifft2(ifftshift(.))).real, i.e.
+// fftshift(fft2(ifftshift(g))) / P^2: forward FFTs of length P / 2 of the rows' z, then per row
+//   X(k) = (Z(k) + conj Z(P/2 - k)) / 2 - (i / 2) e^{-2 pi i k / P} (Z(k) - conj Z(P/2 - k)),
+// k in [0, P/2] (Z periodic in P / 2), the row transform of g for the half spectrum; then
+// forward FFTs of length P of the P / 2 + 1 columns (the other half is conj X(-k)).  Every
+// slice sample is then recomputed and its spatial gradient contracted with the slice
+// gradient; the nine sums (the warp's three, d/du and d/dv) go to part[bn] in fixed order,
+// each plus fl (0, or NaN to mark the pose's sums as undefined).  NT: threads of the workgroup.
+template <int VT, int NT>
+__device__ __forceinline__ void pou_adjoint_tail(float2* a, const float2* tw, float* red, const char* Fp,
+                                                 const PouSlice& sg, const PouPk& k, int b, int n, int bn,
+                                                 const float* __restrict__ u, const float* __restrict__ v,
+                                                 const float (&s3)[3], float fl, float* __restrict__ part) {
+  const int P = sg.P, logP = sg.logP, hp = P / 2, hpitch = hp + 1, logH = logP - 1;
+  fft_lines_dif4<-1, false>(a, P, hp, logH, hpitch, 1, tw, 2);
+  {
+    const int qp = P >> 2, npair = P * (qp + 1);
+    constexpr int kPost = (128 * (128 / 4 + 1) + NT - 1) / NT;
+    float2 xa[kPost], xb[kPost];
+#pragma unroll
+    for (int t = 0; t < kPost; ++t) {
+      const int idx = threadIdx.x + t * NT;
+      if (idx < npair) {
+        const int r = idx / (qp + 1), kk = idx - r * (qp + 1);
+        const float2* row = a + r * hpitch;
+        const float2 z0 = row[brev(kk, logH)], z1 = row[brev((hp - kk) & (hp - 1), logH)];
+        auto xk = [&](float2 za, float2 zb, float2 w) {  // X from Z(k), Z(P/2 - k), w = e^{-2 pi i k / P}
+          const float er = 0.5f * (za.x + zb.x), ei = 0.5f * (za.y - zb.y);  // (Z + conj Z') / 2
+          const float dr = 0.5f * (za.x - zb.x), di = 0.5f * (za.y + zb.y);  // (Z - conj Z') / 2
+          const float pr = dr * w.x - di * w.y, pi = dr * w.y + di * w.x;    // w (Z - conj Z') / 2
+          return make_float2(er + pi, ei - pr);                             // E - i p
+        };
+        xa[t] = xk(z0, z1, tw[kk]);
+        xb[t] = kk == 0 ? xk(z0, z1, make_float2(-1.f, 0.f)) : xk(z1, z0, tw[hp - kk]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < kPost; ++t) {
+      const int idx = threadIdx.x + t * NT;
+      if (idx < npair) {
+        const int r = idx / (qp + 1), kk = idx - r * (qp + 1);
+        float2* row = a + r * hpitch;
+        row[kk] = xa[t];
+        row[hp - kk] = xb[t];  // kk = 0: X(P/2); kk = P/4: the same entry as X(kk)
+      }
+    }
+    __syncthreads();
+  }
+  fft_lines_dif4<-1, true>(a, hpitch, P, logP, 1, hpitch, tw, 1);
+  // every slice sample recomputed and contracted with its gradient
+  const float ux = u[3 * n], uy = u[3 * n + 1], uz = u[3 * n + 2];
+  const float vx = v[3 * n], vy = v[3 * n + 1], vz = v[3 * n + 2];
+  const unsigned long long blk = reinterpret_cast<unsigned long long>(Fp) + (unsigned long long)b * k.block;
+  const float inv = 1.f / float(P * P);
+  float gu[3] = {0.f, 0.f, 0.f}, gv[3] = {0.f, 0.f, 0.f};
+  const int nq = P * P;
+  for (int q0 = threadIdx.x; q0 < nq; q0 += kPouBatch * NT) {
+    PkSmp<VT> smp[kPouBatch];
+    pk_issue_batch<VT>(smp, blk, k, sg, q0, NT, ux, uy, uz, vx, vy, vz);
+#pragma unroll
+    for (int t = 0; t < kPouBatch; ++t) {
+      const int q = q0 + t * NT;
+      if (q >= nq) continue;
+      int i, j;
+      pou_ij(q, logP, i, j);
+      const float kx = float(j - hp), ky = float(i - hp);
+      // G at frequency (uu, vv) (natural order): stored for vv <= P/2, else conj G(-uu, -vv)
+      const int uu = (i + hp) & (P - 1), vv = (j + hp) & (P - 1);
+      const bool lo = vv <= hp;
+      const float2 gs = a[brev(lo ? uu : (P - uu) & (P - 1), logP) * hpitch + (lo ? vv : P - vv)];
+      const float gr = gs.x * inv, gi = (lo ? gs.y : -gs.y) * inv;
+      float cr[8], ci[8], re, im, dre[3], dim[3];
+      pk_corners<VT>(smp[t], cr, ci);
+      interp8<true>(cr, ci, smp[t].tx, smp[t].ty, smp[t].tz, re, im, dre, dim);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const float d = gr * dre[c] + gi * dim[c];
+        gu[c] = fmaf(kx, d, gu[c]);
+        gv[c] = fmaf(ky, d, gv[c]);
+      }
+    }
+  }
+  float all[9] = {s3[0], s3[1], s3[2], gu[0], gu[1], gu[2], gv[0], gv[1], gv[2]};
+  __syncthreads();
+  block_sum<9>(all, red);
+  if (threadIdx.x < 9) part[size_t(bn) * 9 + threadIdx.x] = all[threadIdx.x] + fl;
+}
+
+// The fused adjoint for any ray arrays: the warp adjoint scattered into an LDS image of 64-bit
+// fixed point (the whole P x (P + 1) image, one 1024-thread workgroup per CU), converted to
+// float and packed for the shared steps above.
 template <int VT>
 __global__ __launch_bounds__(kPouSliceThreads) void pou_render_bwd_kernel(
     const char* __restrict__ Fp, PouSlice sg, PouPk k, const float* __restrict__ u, const float* __restrict__ v,
@@ -704,9 +805,9 @@ __global__ __launch_bounds__(kPouSliceThreads) void pou_render_bwd_kernel(
   warp_adjoint_scan<true, true>(imgs + size_t(bn) * P * P, go, rx, ry, ic, g, n, d_b[bn], cu[bn], cv[bn], scale,
                                 acc, pitch, s3);  // P: a power of two here
   __syncthreads();
-  // 2) fixed point -> float, ifftshifted and repacked in place for a real-input transform: row
-  // y holds the P / 2 complex values z(n) = g(y, 2n) + i g(y, 2n + 1) at pitch P / 2 + 1 (every
-  // value is read into registers before any is written: the packed rows overlap the 64-bit ones)
+  // 2) fixed point -> float, ifftshifted and repacked in place for the real-input transform
+  // (every value is read into registers before any is written: the packed rows overlap the
+  // 64-bit ones)
   const float inv_scale = 1.f / scale;
   const int hpitch = hp + 1, logH = logP - 1;
   constexpr int kConv = (128 * 128) / (2 * kPouSliceThreads);  // pairs per thread at P = 128
@@ -728,84 +829,376 @@ __global__ __launch_bounds__(kPouSliceThreads) void pou_render_bwd_kernel(
     if (idx < P * hp) a[(idx >> logH) * hpitch + (idx & (hp - 1))] = zv[t];
   }
   __syncthreads();
-  // 3) adjoint of fftshift(ifft2(ifftshift(.))).real: fftshift(fft2(ifftshift(g))) / P^2, of a
-  // real g: forward FFTs of length P / 2 of the rows' z, then per row
-  //   X(k) = (Z(k) + conj Z(P/2 - k)) / 2 - (i / 2) e^{-2 pi i k / P} (Z(k) - conj Z(P/2 - k)),
-  // k in [0, P/2] (Z periodic in P / 2), the row transform of g for the half spectrum; then
-  // forward FFTs of length P of the P / 2 + 1 columns (the other half is conj X(-k))
-  fft_lines_dif4<-1, false>(a, P, hp, logH, hpitch, 1, tw, 2);
-  {
-    const int qp = P >> 2, npair = P * (qp + 1);
-    constexpr int kPost = (128 * (128 / 4 + 1) + kPouSliceThreads - 1) / kPouSliceThreads;
-    float2 xa[kPost], xb[kPost];
-#pragma unroll
-    for (int t = 0; t < kPost; ++t) {
-      const int idx = threadIdx.x + t * blockDim.x;
-      if (idx < npair) {
-        const int r = idx / (qp + 1), kk = idx - r * (qp + 1);
-        const float2* row = a + r * hpitch;
-        const float2 z0 = row[brev(kk, logH)], z1 = row[brev((hp - kk) & (hp - 1), logH)];
-        auto xk = [&](float2 za, float2 zb, float2 w) {  // X from Z(k), Z(P/2 - k), w = e^{-2 pi i k / P}
-          const float er = 0.5f * (za.x + zb.x), ei = 0.5f * (za.y - zb.y);  // (Z + conj Z') / 2
-          const float dr = 0.5f * (za.x - zb.x), di = 0.5f * (za.y + zb.y);  // (Z - conj Z') / 2
-          const float pr = dr * w.x - di * w.y, pi = dr * w.y + di * w.x;    // w (Z - conj Z') / 2
-          return make_float2(er + pi, ei - pr);                             // E - i p
-        };
-        xa[t] = xk(z0, z1, tw[kk]);
-        xb[t] = kk == 0 ? xk(z0, z1, make_float2(-1.f, 0.f)) : xk(z1, z0, tw[hp - kk]);
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int t = 0; t < kPost; ++t) {
-      const int idx = threadIdx.x + t * blockDim.x;
-      if (idx < npair) {
-        const int r = idx / (qp + 1), kk = idx - r * (qp + 1);
-        float2* row = a + r * hpitch;
-        row[kk] = xa[t];
-        row[hp - kk] = xb[t];  // kk = 0: X(P/2); kk = P/4: the same entry as X(kk)
-      }
-    }
-    __syncthreads();
+  pou_adjoint_tail<VT, kPouSliceThreads>(a, tw, red, Fp, sg, k, b, n, bn, u, v, s3, 0.f, part);
+}
+
+// ---------------------------------------------------------------------------
+// The fused adjoint for separable rays (a zero-skew pinhole: r_x depends on the detector
+// column only, r_y on the row only, rays shared by every pose).  The image position of pixel
+// (y, x) is then (px(x), py(y)) and the bilinear warp weight factorises, so the warp adjoint
+// is a gather: texel (ty, tx) of the image gradient is
+//   G(ty, tx) = sum_y hat(py(y) - ty) A(y, tx),   A(y, tx) = sum_x hat(px(x) - tx) X(y, x),
+// X = go * ic, over the pixels whose cell touches the texel (a rectangle of the detector for
+// a monotone map), and the geometry sums of the scan factorise the same way:
+//   dsx-sums from Bd(y, tx) = sum_x hat'(px - tx) X and Br(y, tx) = sum_x hat'(px - tx) r_x X,
+//   dsy-sums from A with hat'(py - ty) = -1 (ty = floor py) / +1 (ty = floor py + 1).
+// Every texel row ty is owned by one segment of the detector rows (the segment holding its
+// first detector row; segments split the rows that reach the image evenly), and thread
+// (tx, segment) walks its segment's rows, adding to the texels (fy, tx), (fy + 1, tx) it owns:
+// every texel is written by one thread, in detector-row order -- no atomics, no fixed point,
+// deterministic -- straight into the packed half-spectrum layout of the transform that
+// follows (P x (P/2 + 1) complex, 66 KB at P = 128).
+// Staged form (P >= 64, W a multiple of 4, W <= 512): each wave takes 64 consecutive texel
+// columns from the first one the detector reaches (one or two column groups, 8 or 4 row
+// segments), and per detector row loads the window of X its lanes read as one 16-byte load
+// per lane into a per-wave LDS buffer, one row ahead of its use (the slab-image values the
+// row needs are fetched with it), so that a row costs LDS latency, not a global round trip.
+// Otherwise thread (tx, segment) reads its pixels from global memory directly.
+// ---------------------------------------------------------------------------
+constexpr int kPouSepThreads = 512;
+constexpr int kPouSepWaves = kPouSepThreads / 64;
+constexpr int kSepNX = 16;     // pixels per texel column held in registers (wider footprints: per-x path)
+constexpr int kSepChunks = 2;  // 256-float chunks of a wave's row window in the staged form (W <= 512)
+constexpr int kSepOrderPose = 1;  // launch option: workgroups in pose-major order
+
+__host__ __device__ inline bool sep_staged(int P, int W) { return P >= 64 && W % 4 == 0 && W <= 256 * kSepChunks; }
+
+// entries of the segment table: thread (tid mod P, tid / P) segments, or up to one per wave
+__host__ __device__ inline int sep_max_segments(int P) {
+  return kPouSepThreads / P > kPouSepWaves ? kPouSepThreads / P : kPouSepWaves;
+}
+
+__host__ __device__ inline size_t sep_lds_bytes(int P, int W) {
+  const int nseg = sep_max_segments(P);
+  const size_t stage = sep_staged(P, W) ? size_t(kPouSepWaves) * 256 * ((W + 255) / 256) * sizeof(float) : 0;
+  return size_t(P) * (P / 2 + 1) * sizeof(float2) + stage + (P / 2) * sizeof(float2) +
+         9 * kPouMaxWaves * sizeof(float) + (5 * size_t(P) + 2 * nseg + 4) * sizeof(int);
+}
+
+template <int VT>
+__global__ __launch_bounds__(kPouSepThreads, 4) void pou_render_bwd_sep_kernel(
+    const char* __restrict__ Fp, PouSlice sg, PouPk k, const float* __restrict__ u, const float* __restrict__ v,
+    const float* __restrict__ imgs, const float* __restrict__ X, const float* __restrict__ flag,
+    const float* __restrict__ d_b, const float* __restrict__ cu, const float* __restrict__ cv,
+    const float* __restrict__ rx_col, const float* __restrict__ ry_row, int H, int W, float inv_ds, float c_idx,
+    int per_xcd, int opts, float* __restrict__ part) {
+  constexpr int NT = kPouSepThreads;
+  extern __shared__ float2 smem[];
+  const int P = sg.P, logP = sg.logP, hp = P / 2, hpitch = hp + 1;
+  const bool staged = sep_staged(P, W);
+  const int wch = (W + 255) / 256;                      // staged: 256-float chunks per wave buffer
+  float2* a = smem;                                     // the packed gradient image: P x hpitch
+  float* xstage = reinterpret_cast<float*>(a + P * hpitch);  // staged: per-wave row windows
+  float2* tw = reinterpret_cast<float2*>(xstage + (staged ? kPouSepWaves * 256 * wch : 0));  // P/2
+  float* red = reinterpret_cast<float*>(tw + hp);       // 9 * kPouMaxWaves
+  int* xlo = reinterpret_cast<int*>(red + 9 * kPouMaxWaves);  // per texel column / row: first and
+  int* xhi = xlo + P;                                   //   last pixel column / row touching it
+  int* ylo = xhi + P;
+  int* yhi = ylo + P;
+  int* sown = yhi + P;                                  // segment owning texel row ty (-1: none)
+  int* segy = sown + P;                                 // (first, last) detector row of each segment
+  int* rng = segy + 2 * sep_max_segments(P);            // (first, last) row / texel column reached
+  const int m = pou_bn(blockIdx.x, per_xcd);            // position in the launch order
+  if (m >= sg.B * sg.N) return;  // whole workgroup: padding slot of the XCD split
+  int b, n;
+  if (opts & kSepOrderPose) {
+    n = m / sg.B;
+    b = m - n * sg.B;
+  } else {
+    b = m / sg.N;
+    n = m - b * sg.N;
   }
-  fft_lines_dif4<-1, true>(a, hpitch, P, logP, 1, hpitch, tw, 1);
-  // 4) recompute every slice sample and contract its spatial gradient with the slice gradient
-  const float ux = u[3 * n], uy = u[3 * n + 1], uz = u[3 * n + 2];
-  const float vx = v[3 * n], vy = v[3 * n + 1], vz = v[3 * n + 2];
-  const unsigned long long blk = reinterpret_cast<unsigned long long>(Fp) + (unsigned long long)b * k.block;
-  const float inv = 1.f / float(P * P);
-  float gu[3] = {0.f, 0.f, 0.f}, gv[3] = {0.f, 0.f, 0.f};
-  const int nq = P * P, stride = blockDim.x;
-  for (int q0 = threadIdx.x; q0 < nq; q0 += kPouBatch * stride) {
-    PkSmp<VT> smp[kPouBatch];
-    pk_issue_batch<VT>(smp, blk, k, sg, q0, stride, ux, uy, uz, vx, vy, vz);
-#pragma unroll
-    for (int t = 0; t < kPouBatch; ++t) {
-      const int q = q0 + t * stride;
-      if (q >= nq) continue;
-      int i, j;
-      pou_ij(q, logP, i, j);
-      const float kx = float(j - hp), ky = float(i - hp);
-      // G at frequency (uu, vv) (natural order): stored for vv <= P/2, else conj G(-uu, -vv)
-      const int uu = (i + hp) & (P - 1), vv = (j + hp) & (P - 1);
-      const bool lo = vv <= hp;
-      const float2 gs = a[brev(lo ? uu : (P - uu) & (P - 1), logP) * hpitch + (lo ? vv : P - vv)];
-      const float gr = gs.x * inv, gi = (lo ? gs.y : -gs.y) * inv;
-      float cr[8], ci[8], re, im, dre[3], dim[3];
-      pk_corners<VT>(smp[t], cr, ci);
-      interp8<true>(cr, ci, smp[t].tx, smp[t].ty, smp[t].tz, re, im, dre, dim);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const float d = gr * dre[c] + gi * dim[c];
-        gu[c] = fmaf(kx, d, gu[c]);
-        gv[c] = fmaf(ky, d, gv[c]);
-      }
+  const int bn = b * sg.N + n;
+  const int tid = threadIdx.x;
+  const float db = d_b[bn], c_u = cu[bn], c_v = cv[bn];
+  const float hi_lim = float(P) + 1.f;
+  // floor and fraction of a row's image position (fy = -2: the row reaches no texel)
+  auto rowpos = [&](int y, int& fy, float& wy, float& r) {
+    r = ry_row[y];
+    const float py = pou_img_pos(db, r, c_v, inv_ds, c_idx);
+    fy = -2;
+    wy = 0.f;
+    if (py > -2.f && py < hi_lim) {  // false for NaN
+      const float f = floorf(py);
+      fy = int(f);
+      wy = py - f;
     }
+  };
+
+  // 0) tables: the texel columns' / rows' pixel ranges, the segments, the owners
+  for (int i = tid; i < P * hpitch; i += NT) a[i] = make_float2(0.f, 0.f);
+  for (int t = tid; t < P; t += NT) {
+    xlo[t] = ylo[t] = 0x7fffffff;
+    xhi[t] = yhi[t] = -1;
   }
-  float all[9] = {s3[0], s3[1], s3[2], gu[0], gu[1], gu[2], gv[0], gv[1], gv[2]};
+  for (int s = tid; s < sep_max_segments(P); s += NT) {
+    segy[2 * s] = 0x7fffffff;
+    segy[2 * s + 1] = -1;
+  }
+  if (tid < 2) {
+    rng[2 * tid] = 0x7fffffff;
+    rng[2 * tid + 1] = -1;
+  }
+  make_twiddles(tw, P);
   __syncthreads();
-  block_sum<9>(all, red);
-  if (threadIdx.x < 9) part[size_t(bn) * 9 + threadIdx.x] = all[threadIdx.x];
+  for (int x = tid; x < W; x += NT) {
+    const float px = pou_img_pos(db, rx_col[x], c_u, inv_ds, c_idx);
+    if (px > -2.f && px < hi_lim) {
+      const int fx = int(floorf(px));
+      if (unsigned(fx) < unsigned(P)) {
+        atomicMin(&xlo[fx], x);
+        atomicMax(&xhi[fx], x);
+      }
+      if (unsigned(fx + 1) < unsigned(P)) {
+        atomicMin(&xlo[fx + 1], x);
+        atomicMax(&xhi[fx + 1], x);
+      }
+    }
+  }
+  for (int y = tid; y < H; y += NT) {
+    int fy;
+    float wy, r;
+    rowpos(y, fy, wy, r);
+    if (unsigned(fy) < unsigned(P)) {
+      atomicMin(&ylo[fy], y);
+      atomicMax(&yhi[fy], y);
+    }
+    if (unsigned(fy + 1) < unsigned(P)) {
+      atomicMin(&ylo[fy + 1], y);
+      atomicMax(&yhi[fy + 1], y);
+    }
+  }
+  __syncthreads();
+  for (int t = tid; t < P; t += NT) {
+    if (ylo[t] <= yhi[t]) {
+      atomicMin(&rng[0], ylo[t]);
+      atomicMax(&rng[1], yhi[t]);
+    }
+    if (xlo[t] <= xhi[t]) {
+      atomicMin(&rng[2], t);
+      atomicMax(&rng[3], t);
+    }
+  }
+  __syncthreads();
+  const int tx0 = rng[2], tx1 = rng[3];
+  // staged: column groups of 64 texel columns from tx0 (one or two at P <= 128), and the row
+  // segments they leave to the waves; otherwise thread (tid mod P, tid / P)
+  const int ncg = staged ? (tx1 >= tx0 ? (tx1 - tx0) / 64 + 1 : 0) : 1;
+  const int nseg = staged ? kPouSepWaves / (ncg > 0 ? ncg : 1) : (NT >> logP);
+  for (int t = tid; t < P; t += NT) {
+    int s = -1;
+    if (ylo[t] <= yhi[t]) {
+      const long long span = (long long)rng[1] - rng[0] + 1;
+      s = int(((long long)(ylo[t] - rng[0]) * nseg) / span);
+      s = s < nseg - 1 ? s : nseg - 1;
+      atomicMin(&segy[2 * s], ylo[t]);
+      atomicMax(&segy[2 * s + 1], yhi[t]);
+    }
+    sown[t] = s;
+  }
+  __syncthreads();
+
+  // 1) the gather
+  const int wave = tid >> 6, lane = tid & 63;
+  int tx, seg;
+  bool active;
+  if (staged) {
+    const int cg = ncg > 0 ? wave % ncg : 0;
+    seg = ncg > 0 ? wave / ncg : 0;
+    tx = ncg > 0 ? tx0 + 64 * cg + lane : lane;  // (tx0 is a sentinel when no column is reached)
+    active = ncg > 0 && tx <= tx1;               // tx1 < P
+  } else {
+    tx = tid & (P - 1);
+    seg = tid >> logP;
+    active = true;
+  }
+  const int xl = active ? xlo[tx] : 1, xh = active ? xhi[tx] : 0;
+  const bool colok = xl <= xh;
+  const int xb = colok ? xl : 0;
+  const int nx = colok ? xh - xl + 1 : 0;
+  int nxw = nx, wlo = colok ? xb : 0x7fffffff, whi = colok ? xb + nx - 1 : -1;  // all lanes active here
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    nxw = max(nxw, __shfl_xor(nxw, o));
+    wlo = min(wlo, __shfl_xor(wlo, o));
+    whi = max(whi, __shfl_xor(whi, o));
+  }
+  // the column weights of the lane's pixels (0 where a pixel's cell misses the texel)
+  auto coef = [&](int x, float& c, float& d, float& e) {
+    const float r = rx_col[x];
+    const float px = pou_img_pos(db, r, c_u, inv_ds, c_idx);
+    c = d = 0.f;
+    if (px > -2.f && px < hi_lim) {
+      const float f = floorf(px);
+      const int fx = int(f);
+      const float wx = px - f;
+      if (fx == tx) {
+        c = 1.f - wx;
+        d = -1.f;
+      } else if (fx == tx - 1) {
+        c = wx;
+        d = 1.f;
+      }
+    }
+    e = d * r;
+  };
+  float cc[kSepNX], dd[kSepNX], ee[kSepNX];
+#pragma unroll
+  for (int q = 0; q < kSepNX; ++q) {
+    cc[q] = dd[q] = ee[q] = 0.f;
+    if (q < nx) coef(xb + q, cc[q], dd[q], ee[q]);
+  }
+  const size_t K = size_t(H) * W;
+  const float* Xn = X + size_t(n) * K;
+  const float* img = imgs + size_t(bn) * P * P;
+  float* gimg = reinterpret_cast<float*>(a);
+  const int gcol = (tx + hp) & (P - 1);  // ifftshifted column: float index in the packed row
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+  // the texels (fy, tx), (fy + 1, tx) of detector row y that this thread owns, with the row's
+  // column sums A, Bd, Br and the slab-image values I0, I1 at those texels
+  auto accumulate = [&](int fy, float wy, float ry, float A, float Bd, float Br, float I0, float I1) {
+    if (unsigned(fy) < unsigned(P) && sown[fy] == seg) {  // texel row fy: weight 1 - wy, d/dpy -1
+      const float w0 = 1.f - wy;
+      if (colok) gimg[2 * hpitch * ((fy + hp) & (P - 1)) + gcol] += w0 * A;
+      s0 = fmaf(I0, fmaf(w0, Br, -ry * A), s0);
+      s1 = fmaf(-I0 * w0, Bd, s1);
+      s2 = fmaf(I0, A, s2);
+    }
+    if (unsigned(fy + 1) < unsigned(P) && sown[fy + 1] == seg) {  // texel row fy + 1: weight wy, d/dpy +1
+      const float w1 = wy;
+      if (colok) gimg[2 * hpitch * ((fy + 1 + hp) & (P - 1)) + gcol] += w1 * A;
+      s0 = fmaf(I1, fmaf(w1, Br, ry * A), s0);
+      s1 = fmaf(-I1 * w1, Bd, s1);
+      s2 = fmaf(-I1, A, s2);
+    }
+  };
+  auto img_at_row = [&](int ty) { return colok && unsigned(ty) < unsigned(P) ? img[ty * P + tx] : 0.f; };
+  const int x0a = wlo & ~3;
+  const int nch = whi >= 0 ? (whi - x0a) / 256 + 1 : 0;
+  if (staged && nch <= kSepChunks) {
+    if (ncg > 0 && whi >= 0) {  // wave-uniform: every lane stages its part of the window
+      const int yb = __builtin_amdgcn_readfirstlane(segy[2 * seg]);
+      const int ye = __builtin_amdgcn_readfirstlane(segy[2 * seg + 1]);
+      float* xw = xstage + wave * 256 * wch;
+      float4 pre[kSepChunks];
+      int pfy = 0;
+      float pwy = 0.f, pry = 0.f, pI0 = 0.f, pI1 = 0.f;
+      // row y's window and position, and its image values (reused when fy repeats)
+      auto fetch = [&](int y, int lastf, float lI0, float lI1) {
+        const float* rowp = Xn + size_t(y) * W;
+#pragma unroll
+        for (int c = 0; c < kSepChunks; ++c)
+          if (c < nch) {
+            const int x = x0a + 256 * c + 4 * lane;  // W % 4 == 0: a group is all in or all out
+            pre[c] = x < W ? *reinterpret_cast<const float4*>(rowp + x) : make_float4(0.f, 0.f, 0.f, 0.f);
+          }
+        rowpos(y, pfy, pwy, pry);
+        if (pfy == lastf) {
+          pI0 = lI0;
+          pI1 = lI1;
+        } else {
+          pI0 = img_at_row(pfy);
+          pI1 = img_at_row(pfy + 1);
+        }
+      };
+      if (yb <= ye) fetch(yb, -0x7fffffff, 0.f, 0.f);
+      const int qmax = 256 * nch - 1;
+      for (int y = yb; y <= ye; ++y) {
+#pragma unroll
+        for (int c = 0; c < kSepChunks; ++c)
+          if (c < nch) *reinterpret_cast<float4*>(xw + 256 * c + 4 * lane) = pre[c];
+        const int fy = pfy;
+        const float wy = pwy, ry = pry, I0 = pI0, I1 = pI1;
+        if (y < ye) fetch(y + 1, fy, I0, I1);
+        // (the wave's LDS operations complete in order: the reads below see this row's window)
+        float A = 0.f, Bd = 0.f, Br = 0.f;
+        if (nxw <= kSepNX) {
+#pragma unroll
+          for (int q = 0; q < kSepNX; ++q)
+            if (q < nxw) {
+              const int i = min(max(xb - x0a + q, 0), qmax);  // weight 0 past the lane's range
+              const float xv = xw[i];
+              A = fmaf(cc[q], xv, A);
+              Bd = fmaf(dd[q], xv, Bd);
+              Br = fmaf(ee[q], xv, Br);
+            }
+        } else {
+          for (int q = 0; q < nx; ++q) {
+            float c, d, e;
+            coef(xb + q, c, d, e);
+            const float xv = xw[xb - x0a + q];
+            A = fmaf(c, xv, A);
+            Bd = fmaf(d, xv, Bd);
+            Br = fmaf(e, xv, Br);
+          }
+        }
+        accumulate(fy, wy, ry, A, Bd, Br, I0, I1);
+      }
+    }
+  } else if (active) {
+    const int yb = segy[2 * seg], ye = segy[2 * seg + 1];
+    int lastf = -0x7fffffff;
+    float I0 = 0.f, I1 = 0.f;
+    for (int y = yb; y <= ye; ++y) {
+      int fy;
+      float wy, ry;
+      rowpos(y, fy, wy, ry);
+      const float* xr = Xn + size_t(y) * W;
+      float A = 0.f, Bd = 0.f, Br = 0.f;
+      if (nxw <= kSepNX) {
+#pragma unroll
+        for (int q = 0; q < kSepNX; ++q)
+          if (q < nxw) {
+            const float xv = xr[min(xb + q, W - 1)];  // a pixel past the lane's range has weight 0
+            A = fmaf(cc[q], xv, A);
+            Bd = fmaf(dd[q], xv, Bd);
+            Br = fmaf(ee[q], xv, Br);
+          }
+      } else {
+        for (int q = 0; q < nx; ++q) {
+          float c, d, e;
+          coef(xb + q, c, d, e);
+          const float xv = xr[xb + q];
+          A = fmaf(c, xv, A);
+          Bd = fmaf(d, xv, Bd);
+          Br = fmaf(e, xv, Br);
+        }
+      }
+      if (fy != lastf) {
+        lastf = fy;
+        I0 = img_at_row(fy);
+        I1 = img_at_row(fy + 1);
+      }
+      accumulate(fy, wy, ry, A, Bd, Br, I0, I1);
+    }
+  }
+  const float s3[3] = {s0 * inv_ds, s1 * inv_ds, s2 * inv_ds};
+  __syncthreads();
+  // a pose whose X holds a non-finite value (flag NaN) gets NaN sums
+  pou_adjoint_tail<VT, NT>(a, tw, red, Fp, sg, k, b, n, bn, u, v, s3, flag[n], part);
+}
+
+// X[n][q] = go[n][q] ic[q] (ic per pose when per_pose_rays) and flag[n] = 0, or NaN when any
+// X of pose n is not finite
+__global__ __launch_bounds__(1024) void pou_gprep_kernel(const float* __restrict__ go, const float* __restrict__ ic,
+                                                         long long ray_stride, int K, float* __restrict__ X,
+                                                         float* __restrict__ flag) {
+  __shared__ int bad;
+  const int n = blockIdx.x;
+  if (threadIdx.x == 0) bad = 0;
+  __syncthreads();
+  const float* g = go + size_t(n) * K;
+  const float* c = ic + size_t(n) * ray_stride;
+  float* xo = X + size_t(n) * K;
+  bool nf = false;
+  for (int q = threadIdx.x; q < K; q += 1024) {
+    const float x = g[q] * c[q];
+    xo[q] = x;
+    nf |= !(fabsf(x) <= 3.40282347e38f);
+  }
+  if (__any(nf) && (threadIdx.x & 63) == 0) bad = 1;
+  __syncthreads();
+  if (threadIdx.x == 0) flag[n] = bad ? __builtin_nanf("") : 0.f;
 }
 
 // d out / d(rx, ry, inv_cos) for per-pose ray arrays (explicit-ray projections)
@@ -829,8 +1222,8 @@ __global__ __launch_bounds__(256) void pou_ray_grads_kernel(const float* __restr
   for (int b = 0; b < g.B; ++b) {
     const int bn = b * g.N + n;
     const float db = d_b[bn];
-    const float px = (db * r_x - cu[bn]) * g.inv_ds + g.c_idx;
-    const float py = (db * r_y - cv[bn]) * g.inv_ds + g.c_idx;
+    const float px = pou_img_pos(db, r_x, cu[bn], g.inv_ds, g.c_idx);
+    const float py = pou_img_pos(db, r_y, cv[bn], g.inv_ds, g.c_idx);
     const float x0f = floorf(px), y0f = floorf(py);
     const float tx = px - x0f, ty = py - y0f, ox = 1.f - tx, oy = 1.f - ty;
     const int x0 = int(x0f), y0 = int(y0f);
@@ -1089,6 +1482,51 @@ int fvr_pou_render_bwd(uint64_t F, int32_t dtype, int32_t B, int32_t P, const fl
   }
 #undef FVR_RB
   FVR_CHECK_LAUNCH("pou_render_bwd_kernel");
+  return FVR_OK;
+}
+
+int fvr_pou_sep_ok(int32_t P, int32_t H, int32_t W) {
+  if (!fvr_pou_lds_ok(P) || H < 1 || W < 1 || int64_t(H) * W >= (int64_t(1) << 31)) return 0;
+  return sep_lds_bytes(P, W) <= 160 * 1024 ? 1 : 0;
+}
+
+int fvr_pou_gprep(const float* go, const float* ic, int32_t N, int32_t K, int32_t per_pose_rays, float* X,
+                  float* flag, void* stream) {
+  FVR_CHECK_ARG(go && ic && X && flag && N > 0 && K > 0, "fvr_pou_gprep: bad arguments");
+  hipLaunchKernelGGL(pou_gprep_kernel, dim3(N), dim3(1024), 0, reinterpret_cast<hipStream_t>(stream), go, ic,
+                     per_pose_rays ? (long long)K : 0ll, K, X, flag);
+  FVR_CHECK_LAUNCH("pou_gprep_kernel");
+  return FVR_OK;
+}
+
+int fvr_pou_render_bwd_sep(uint64_t F, int32_t dtype, int32_t B, int32_t P, const float* u_xyz, const float* v_xyz,
+                           int32_t N, const float* imgs, const float* X, const float* flag, const float* d_b,
+                           const float* cu, const float* cv, const float* rx_col, const float* ry_row, int32_t H,
+                           int32_t W, float inv_ds, float c_idx, int32_t opts, float* part, void* stream) {
+  FVR_CHECK_ARG(F && u_xyz && v_xyz && imgs && X && flag && d_b && cu && cv && rx_col && ry_row && part,
+                "fvr_pou_render_bwd_sep: null pointer");
+  FVR_CHECK_ARG((opts & ~kSepOrderPose) == 0, "fvr_pou_render_bwd_sep: bad options %d", opts);
+  FVR_CHECK_ARG(B > 0 && N > 0 && int64_t(B) * N < (int64_t(1) << 30), "fvr_pou_render_bwd_sep: bad batch");
+  FVR_CHECK_ARG(fvr_pou_sep_ok(P, H, W), "fvr_pou_render_bwd_sep: P = %d with a %d x %d detector does not fit", P, H,
+                W);
+  PouSlice sg{B, N, P, 31 - __builtin_clz(uint32_t(P)), float(P / 2)};
+  const PouPk k = pou_pk(P, dtype);
+  const size_t lds = sep_lds_bytes(P, W);
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const char* fp = reinterpret_cast<const char*>(F);
+  const int per_xcd = int((int64_t(B) * N + 7) / 8);
+#define FVR_RS(VT)                                                                                                 \
+  FVR_CHECK_ARG(allow_lds(pou_render_bwd_sep_kernel<VT>, lds), "fvr_pou_render_bwd_sep: %zu B of LDS refused", lds); \
+  hipLaunchKernelGGL(pou_render_bwd_sep_kernel<VT>, dim3(8 * per_xcd), dim3(kPouSepThreads), lds, st, fp, sg, k,     \
+                     u_xyz, v_xyz, imgs, X, flag, d_b, cu, cv, rx_col, ry_row, H, W, inv_ds, c_idx, per_xcd, opts, part)
+  switch (dtype) {
+    case 0: FVR_RS(0); break;
+    case 1: FVR_RS(1); break;
+    case 2: FVR_RS(2); break;
+    default: set_error("fvr_pou_render_bwd_sep: bad dtype %d", dtype); return FVR_ERR_INVALID;
+  }
+#undef FVR_RS
+  FVR_CHECK_LAUNCH("pou_render_bwd_sep_kernel");
   return FVR_OK;
 }
 

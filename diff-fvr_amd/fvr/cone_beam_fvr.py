@@ -12,14 +12,17 @@ warp that sums all blocks onto the detector with the 1/cos obliquity
 HIP kernels (``csrc/fvr_pou.hip``): for power-of-two P <= 128 one launch
 gathers each slice into LDS and inverse-transforms it there; the warp is one
 launch over (pose, pixel); the backward is one fused launch per (block, pose)
-(warp adjoint into an LDS fixed-point image, forward FFT in LDS, recomputed
-slice samples contracted with their gradient) plus fixed-order block sums.
+(the warp adjoint -- a gather over each texel's detector footprint for the
+separable rays of a zero-skew pinhole lattice, a scatter into an LDS fixed-point
+image otherwise -- forward FFT in LDS, recomputed slice samples contracted with
+their gradient) plus fixed-order block sums.
 Other P use the generic slice operator, ``torch.fft`` and the warp kernels.
 """
 
 from __future__ import annotations
 
 import math
+import weakref
 from typing import Dict, Optional, Tuple, Union
 
 import numpy as np
@@ -164,8 +167,28 @@ def sample_isocenter_parity_offsets(vox_spacing) -> torch.Tensor:
 # ---------------------------------------------------------------------------
 # fused autograd operator over the PoU kernels
 # ---------------------------------------------------------------------------
+def _separable_rays(model, rx: torch.Tensor, ry: torch.Tensor, H: int, W: int):
+    """(r_x per detector column (W,), r_y per detector row (H,)) when the shared ray arrays
+    are exactly separable (every row of r_x equal, every column of r_y equal: a zero-skew
+    pinhole lattice, ``make_pinhole_lattice``), else None.  Cached on the model for the ray
+    tensors last seen (object and version), so the check's host synchronisation happens once
+    per lattice."""
+    key = (id(rx), rx._version, id(ry), ry._version, H, W)
+    c = getattr(model, "_sep_rays", None)
+    if c is not None and c[0] == key and c[1]() is rx and c[2]() is ry:
+        return c[3]
+    res = None
+    if rx.dim() == 2 and tuple(rx.shape) == (H, W) and tuple(ry.shape) == (H, W):
+        if bool((rx == rx[:1]).all()) and bool((ry == ry[:, :1]).all()):
+            dev = model.F_blk_ri.device
+            res = (_f32(rx[0], dev).contiguous(), _f32(ry[:, 0], dev).contiguous())
+    model._sep_rays = (key, weakref.ref(rx), weakref.ref(ry), res)
+    return res
+
+
 class _PouRender(torch.autograd.Function):
-    """Fused slice -> LDS iFFT -> warp for every (block, pose); backward in one fused launch."""
+    """Fused slice -> LDS iFFT -> warp for every (block, pose); backward in one fused launch
+    (the gather form of the warp adjoint for separable rays, the fixed-point scan otherwise)."""
 
     @staticmethod
     def forward(ctx, u_xyz, v_xyz, d_b, cu, cv, rx, ry, ic, model, out_h, out_w):
@@ -187,30 +210,47 @@ class _PouRender(torch.autograd.Function):
         _lib.check(L.fvr_pou_warp_fwd(_lib.ptr(imgs), B, N, P, _lib.ptr(db), _lib.ptr(c_u), _lib.ptr(c_v),
                                       _lib.ptr(rxf), _lib.ptr(ryf), _lib.ptr(icf), per_pose, K, inv_ds, c_idx,
                                       _lib.ptr(out), st), "fvr_pou_warp_fwd")
+        sep = None
+        if not per_pose and model.separable_adjoint and L.fvr_pou_sep_ok(P, out_h, out_w):
+            sep = _separable_rays(model, rx, ry, out_h, out_w)
+        ctx.sep = sep
         ctx.save_for_backward(u, v, db, c_u, c_v, rxf, ryf, icf, imgs)
         ctx.model = model
-        ctx.meta = (B, N, P, K, per_pose, inv_ds, c_idx, rx.shape, u_xyz.dtype)
+        ctx.meta = (B, N, P, K, per_pose, inv_ds, c_idx, rx.shape, u_xyz.dtype, out_h, out_w)
         ctx.dts = tuple(t.dtype for t in (d_b, cu, cv, rx, ry, ic))
         return out
 
     @staticmethod
     def backward(ctx, g):
         u, v, db, c_u, c_v, rxf, ryf, icf, imgs = ctx.saved_tensors
-        B, N, P, K, per_pose, inv_ds, c_idx, rshape, udt = ctx.meta
+        B, N, P, K, per_pose, inv_ds, c_idx, rshape, udt, H, W = ctx.meta
         model = ctx.model
         dev = imgs.device
         L = _lib.lib()
         st = _lib.stream_ptr(dev)
         go = _f32(g, dev).reshape(N, K)
-        scale = torch.empty(N, device=dev, dtype=torch.float32)
-        _lib.check(L.fvr_pou_gscale(_lib.ptr(go), _lib.ptr(icf), N, K, per_pose, _lib.ptr(scale), st),
-                   "fvr_pou_gscale")
         part = torch.empty((B, N, 9), device=dev, dtype=torch.float32)
         Fp = model.packed_blocks()
-        _lib.check(L.fvr_pou_render_bwd(Fp.data_ptr(), _VOL_DTYPES[model.F_blk_ri.dtype], B, P, _lib.ptr(u), _lib.ptr(v), N,
-                                        _lib.ptr(imgs), _lib.ptr(go), _lib.ptr(db), _lib.ptr(c_u), _lib.ptr(c_v),
-                                        _lib.ptr(rxf), _lib.ptr(ryf), _lib.ptr(icf), per_pose, K, inv_ds, c_idx,
-                                        _lib.ptr(scale), _lib.ptr(part), st), "fvr_pou_render_bwd")
+        dt = _VOL_DTYPES[model.F_blk_ri.dtype]
+        if ctx.sep is not None:
+            X = torch.empty((N, K), device=dev, dtype=torch.float32)
+            flag = torch.empty(N, device=dev, dtype=torch.float32)
+            _lib.check(L.fvr_pou_gprep(_lib.ptr(go), _lib.ptr(icf), N, K, per_pose, _lib.ptr(X), _lib.ptr(flag), st),
+                       "fvr_pou_gprep")
+            rx_col, ry_row = ctx.sep
+            _lib.check(L.fvr_pou_render_bwd_sep(Fp.data_ptr(), dt, B, P, _lib.ptr(u), _lib.ptr(v), N, _lib.ptr(imgs),
+                                                _lib.ptr(X), _lib.ptr(flag), _lib.ptr(db), _lib.ptr(c_u),
+                                                _lib.ptr(c_v), _lib.ptr(rx_col), _lib.ptr(ry_row), H, W, inv_ds,
+                                                c_idx, int(model.sep_adjoint_order), _lib.ptr(part), st),
+                       "fvr_pou_render_bwd_sep")
+        else:
+            scale = torch.empty(N, device=dev, dtype=torch.float32)
+            _lib.check(L.fvr_pou_gscale(_lib.ptr(go), _lib.ptr(icf), N, K, per_pose, _lib.ptr(scale), st),
+                       "fvr_pou_gscale")
+            _lib.check(L.fvr_pou_render_bwd(Fp.data_ptr(), dt, B, P, _lib.ptr(u), _lib.ptr(v), N, _lib.ptr(imgs),
+                                            _lib.ptr(go), _lib.ptr(db), _lib.ptr(c_u), _lib.ptr(c_v), _lib.ptr(rxf),
+                                            _lib.ptr(ryf), _lib.ptr(icf), per_pose, K, inv_ds, c_idx,
+                                            _lib.ptr(scale), _lib.ptr(part), st), "fvr_pou_render_bwd")
         guv = torch.empty((N, 6), device=dev, dtype=torch.float32)
         _lib.check(L.fvr_pou_sum_blocks(_lib.ptr(part), B, N, 9, 3, 6, _lib.ptr(guv), st), "fvr_pou_sum_blocks")
         grx = gry = gic = None
@@ -359,6 +399,11 @@ class PerspectiveSlabFVR(nn.Module):
     # the fused LDS kernels are used whenever P allows them; False selects the
     # generic composition (slice operator + torch.fft + warp), for cross-checks
     fused_kernels = True
+    # separable shared rays take the gather form of the fused adjoint; False selects the
+    # fixed-point scan for them too, for cross-checks
+    separable_adjoint = True
+    # workgroup order of the gather form: 0 block-major, 1 pose-major
+    sep_adjoint_order = 0
 
     def fused_ok(self) -> bool:
         return bool(self.fused_kernels) and bool(_lib.lib().fvr_pou_lds_ok(self.P))
@@ -406,11 +451,13 @@ class PerspectiveSlabFVR(nn.Module):
     @staticmethod
     def _make_ray_cache(src0, trg0, u0, v0, n0):
         """SE(3)-invariant (rx, ry, inv_cos) (H, W) and their flat copies (cone_beam_fvr.py:524-545)."""
-        rays = (trg0 - src0.view(1, 1, 3)).float()
+        rays = trg0 - src0.view(1, 1, 3)
+        dt = rays.dtype if rays.dtype == torch.float64 else torch.float32
+        rays = rays.to(dt)
         r = rays * torch.rsqrt((rays * rays).sum(dim=-1, keepdim=True).clamp(min=1e-12))
-        ic = (1.0 / (r * n0.float().view(1, 1, 3)).sum(dim=-1).clamp(min=0.1)).contiguous()
-        rx = ((r * u0.float().view(1, 1, 3)).sum(dim=-1) * ic).contiguous()
-        ry = ((r * v0.float().view(1, 1, 3)).sum(dim=-1) * ic).contiguous()
+        ic = (1.0 / (r * n0.to(dt).view(1, 1, 3)).sum(dim=-1).clamp(min=0.1)).contiguous()
+        rx = ((r * u0.to(dt).view(1, 1, 3)).sum(dim=-1) * ic).contiguous()
+        ry = ((r * v0.to(dt).view(1, 1, 3)).sum(dim=-1) * ic).contiguous()
         return rx, ry, ic, rx.reshape(-1).contiguous(), ry.reshape(-1).contiguous(), ic.reshape(-1).contiguous()
 
     @staticmethod
@@ -472,9 +519,15 @@ class PerspectiveSlabFVR(nn.Module):
 
     @staticmethod
     def make_pinhole_lattice(k_inv: torch.Tensor, sdd_mm: float, height: int, width: int):
-        """Camera-frame pinhole lattice + SE(3)-invariant ray cache (11-tuple) (:706-728)."""
+        """Camera-frame pinhole lattice + SE(3)-invariant ray cache (11-tuple) (:706-728).
+
+        Formed in float64 and returned in float32.  For a zero-skew pinhole r_x depends on
+        the detector column only and r_y on the row only; when the float64 cache shows that
+        (to 1e-12 relative), every row of r_x is made row 0's and every column of r_y
+        column 0's, so that the float32 cache is exactly separable and the backward takes
+        the gather form of the warp adjoint (``fvr_pou_render_bwd_sep``)."""
         k0 = k_inv.squeeze(0) if k_inv.ndim == 3 else k_inv
-        dev, dt = k0.device, torch.float32
+        dev, dt = k0.device, torch.float64
         cols = torch.arange(width, device=dev, dtype=dt) + 0.5
         rows = torch.arange(height, device=dev, dtype=dt) + 0.5
         rr, cc = torch.meshgrid(rows, cols, indexing="ij")
@@ -484,7 +537,14 @@ class PerspectiveSlabFVR(nn.Module):
         src0 = torch.zeros(3, device=dev, dtype=dt)
         u0, v0, n0 = PerspectiveSlabFVR._extract_detector_basis(src0.unsqueeze(0), trg0.unsqueeze(0))
         u0, v0, n0 = u0.squeeze(0), v0.squeeze(0), n0.squeeze(0)
-        return (src0, trg0, u0, v0, n0, *PerspectiveSlabFVR._make_ray_cache(src0, trg0, u0, v0, n0))
+        rx, ry, ic = PerspectiveSlabFVR._make_ray_cache(src0, trg0, u0, v0, n0)[:3]
+        tol = 1e-12 * max(1.0, float(rx.abs().max()), float(ry.abs().max()))
+        if float((rx - rx[:1]).abs().max()) <= tol and float((ry - ry[:, :1]).abs().max()) <= tol:
+            rx, ry = rx[:1].expand_as(rx), ry[:, :1].expand_as(ry)
+        f32 = torch.float32
+        rx, ry, ic = (t.to(f32).contiguous() for t in (rx, ry, ic))
+        return (src0.to(f32), trg0.to(f32), u0.to(f32), v0.to(f32), n0.to(f32), rx, ry, ic,
+                rx.reshape(-1).contiguous(), ry.reshape(-1).contiguous(), ic.reshape(-1).contiguous())
 
     def _forward_from_lattice(self, pose_mats: torch.Tensor, lattice) -> torch.Tensor:
         """Transform a camera-frame lattice by SE(3) matrices (N, 4, 4) and project (:670-688)."""

@@ -30,6 +30,9 @@
  *   fvr_pou_render_bwd  the adjoint of cone_beam_fvr.py:547-573
  *                       _project_core_invariant (slice -> iFFT -> warp) w.r.t.
  *                       u_det, v_det, d_b, cu, cv, fused per (block, pose).
+ *   fvr_pou_render_bwd_sep  the same adjoint for the separable rays of a zero-skew
+ *                       pinhole lattice (cone_beam_fvr.py:706-728), the warp
+ *                       adjoint as a gather; fvr_pou_gprep forms its input.
  *   fvr_pou_ray_grads   the ray-array gradients grid_sample gives autograd in
  *                       cone_beam_fvr.py:423-478 _project_core.
  *   fvr_edge_*          cone_beam_fvr.py:71-112 edge_dominant_ncc (forward moments,
@@ -408,6 +411,23 @@ int fvr_pou_render_bwd(uint64_t F, int32_t dtype, int32_t B, int32_t P, const fl
                        const float* d_b, const float* cu, const float* cv, const float* rx,
                        const float* ry, const float* ic, int32_t per_pose_rays, int32_t K,
                        float inv_ds, float c_idx, const float* gscale, float* part, void* stream);
+/* 1 when fvr_pou_render_bwd_sep takes P with an H-row detector (P as fvr_pou_lds_ok, its
+ * LDS tables within one CU's 160 KiB, H * W < 2^31). */
+int fvr_pou_sep_ok(int32_t P, int32_t H, int32_t W);
+/* X[n][q] = go[n][q] * ic[q] (ic[n][q] when per_pose_rays) and flag[n] = 0, or NaN when
+ * any X of pose n is not finite: the input of fvr_pou_render_bwd_sep. */
+int fvr_pou_gprep(const float* go, const float* ic, int32_t N, int32_t K, int32_t per_pose_rays, float* X,
+                  float* flag, void* stream);
+/* fvr_pou_render_bwd for separable rays shared by every pose: r_x of pixel (y, x) is
+ * rx_col[x], r_y is ry_row[y] (a zero-skew pinhole lattice), X and flag from
+ * fvr_pou_gprep, K = H * W.  The warp adjoint is a gather (each texel of the image
+ * gradient sums its detector footprint, no atomics); the same part[b][n][9].  opts: 0
+ * workgroups in block-major order, 1 pose-major (the results are the same). */
+int fvr_pou_render_bwd_sep(uint64_t F, int32_t dtype, int32_t B, int32_t P, const float* u_xyz,
+                           const float* v_xyz, int32_t N, const float* imgs, const float* X,
+                           const float* flag, const float* d_b, const float* cu, const float* cv,
+                           const float* rx_col, const float* ry_row, int32_t H, int32_t W,
+                           float inv_ds, float c_idx, int32_t opts, float* part, void* stream);
 /* grx/gry/gic (N, K): d/d(rx, ry, ic) of sum go * out for per-pose rays. */
 int fvr_pou_ray_grads(const float* imgs, int32_t B, int32_t N, int32_t P, const float* go,
                       const float* d_b, const float* cu, const float* cv, const float* rx,

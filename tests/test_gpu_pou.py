@@ -405,3 +405,41 @@ def test_packed_block_layout_image(dtype):
     exp[:, 1:P + 1, 1:P + 1, 1:P + 1, 0] = Fc[:, 0]
     exp[:, 1:P + 1, 1:P + 1, 1:P + 1, 1] = Fc[:, 1]
     assert torch.equal(got, exp)
+
+
+@pytest.mark.parametrize("block_res,H,W", [(8, 20, 22), (16, 20, 22), (32, 64, 48), (64, 96, 80)])
+def test_separable_gather_adjoint_matches_the_scan(block_res, H, W):
+    """The fused backward of a zero-skew pinhole lattice takes the gather form of the warp
+    adjoint (fvr_pou_render_bwd_sep, P = 2 block_res: every LDS size); its pose gradient
+    equals the fixed-point scan's (fvr_pou_render_bwd) to float32 summation order, and is
+    bit-identical run to run."""
+    from fvr import PerspectiveSlabFVR
+    m = PerspectiveSlabFVR.from_volume(_vol(), vox_mm=2.0, grid_blocks=(2, 2, 2), block_res=block_res)
+    f = 300.0 * H / 20.0
+    k_inv = torch.tensor([[1.0 / f, 0.0, -W / (2 * f)], [0.0, 1.0 / f, -H / (2 * f)], [0.0, 0.0, 1.0]])
+    lat = [t.to(DEV) for t in PerspectiveSlabFVR.make_pinhole_lattice(k_inv, 600.0, H, W)]
+    w = torch.randn((3, 1, H, W), generator=torch.Generator().manual_seed(11)).to(DEV)
+    g_sep = _loss_grad(m, MATS, lat, w)
+    assert m._sep_rays[3] is not None  # the gather kernel ran
+    assert np.array_equal(g_sep, _loss_grad(m, MATS, lat, w))
+    m.separable_adjoint = False
+    try:
+        g_scan = _loss_grad(m, MATS, lat, w)
+    finally:
+        m.separable_adjoint = True
+    assert np.max(np.abs(g_sep - g_scan)) <= 1e-5 * np.abs(g_scan).max(), np.abs(g_sep - g_scan).max()
+
+
+def test_separable_gather_adjoint_propagates_non_finite_upstream_gradients(model):
+    """A NaN in one pose's upstream gradient makes that pose's gradient NaN (the flag of
+    fvr_pou_gprep) and leaves the other poses' gradients bit for bit unchanged."""
+    m, _ = model
+    lat = [t.to(DEV) for t in _lattice()[0]]
+    w = torch.randn((3, 1, 20, 22), generator=torch.Generator().manual_seed(12)).to(DEV)
+    g1 = _loss_grad(m, MATS, lat, w)
+    w_nan = w.clone()
+    w_nan[1, 0, 3, 4] = float("nan")
+    g2 = _loss_grad(m, MATS, lat, w_nan)
+    assert m._sep_rays[3] is not None
+    assert np.isnan(g2[1, :3, :]).all()
+    assert np.array_equal(g2[0], g1[0]) and np.array_equal(g2[2], g1[2])

@@ -345,3 +345,27 @@ def test_pn_evaluate_noalloc_rejects_batches_beyond_its_scratch_before_any_devic
     rc = L.fvr_pn_evaluate_noalloc(fake, 2, tm.ctypes.data, 48.0, 48.0, 48.0, 1, 1 << 4 | 64 << 8, fake, n, 1.0, 1.0,
                                    None, fake, fake, None)
     assert rc == 1 and b"scratch" in L.fvr_last_error() and str(16 << 20).encode() in L.fvr_last_error()
+
+
+def test_pinhole_lattice_ray_cache_is_exactly_separable_for_a_zero_skew_pinhole():
+    """make_pinhole_lattice forms the ray cache in float64: for a zero-skew pinhole every row
+    of r_x is row 0's and every column of r_y column 0's, bit for bit (the condition of the
+    gather form of the PoU warp adjoint), and the values are the oracle's float64 cache
+    rounded once; a skewed pinhole keeps its full arrays."""
+    import torch
+    from fvr import PerspectiveSlabFVR
+    from oracle import cone_beam_np as C
+    H, W, f, sdd = 20, 22, 300.0, 600.0
+    k_inv = torch.tensor([[1.0 / f, 0.0, -W / (2 * f)], [0.0, 1.0 / f, -H / (2 * f)], [0.0, 0.0, 1.0]])
+    lat = PerspectiveSlabFVR.make_pinhole_lattice(k_inv, sdd, H, W)
+    rx, ry, ic = lat[5:8]
+    assert rx.dtype == ry.dtype == ic.dtype == torch.float32 and tuple(rx.shape) == (H, W)
+    assert torch.equal(rx, rx[:1].expand(H, W)) and torch.equal(ry, ry[:, :1].expand(H, W))
+    assert torch.equal(lat[8], rx.reshape(-1)) and torch.equal(lat[9], ry.reshape(-1))
+    ref = C.make_pinhole_lattice(k_inv.double().numpy(), sdd, H, W)
+    for got, key in ((rx, "rx"), (ry, "ry"), (ic, "ic")):
+        assert np.max(np.abs(got.double().numpy() - ref[key])) <= 1e-7 * np.abs(ref[key]).max()
+    k_skew = k_inv.clone()
+    k_skew[0, 1] = 2e-4
+    rs = PerspectiveSlabFVR.make_pinhole_lattice(k_skew, sdd, H, W)[5]
+    assert not torch.equal(rs, rs[:1].expand(H, W))
