@@ -22,6 +22,8 @@
 //   fvr_pou_ray_grads    d/d(rx, ry, inv_cos) per (pose, pixel) for per-pose rays.
 //   fvr_pou_gscale       per-pose power-of-two fixed-point scale of |grad * inv_cos|.
 //   fvr_pou_sum_blocks   fixed-order sum over blocks of per-(block, pose) partials.
+//   fvr_pou_pose_geom(_bwd)  pose matrices -> detector axes and block offsets of the
+//                        lattice path, and its adjoint from the fused adjoint's sums.
 #include <cmath>
 
 #include "fvr_common.h"
@@ -1293,6 +1295,92 @@ __global__ void pou_sum_blocks_kernel(const float* __restrict__ part, int B, int
   out[idx] = float(s);
 }
 
+// Pose geometry of the lattice path (cone_beam_fvr.py:670-688 _forward_from_lattice,
+// :507-521 _compute_block_offsets): pose n is M[n] = [R t; 0 1] (row-major 4 x 4), the
+// camera-frame lattice lat = (src0, u0, v0, n0) (12 floats, world (z, y, x) components);
+// src = R src0 + t, u = R u0, v = R v0, w = R n0 and, per block centre c_b,
+//   d_b = (c_b - src) . w,  cu = (c_b - src) . u,  cv = (c_b - src) . v;
+// the detector axes go to the slice kernels in (x, y, z) voxel order: u_xyz = (u2, u1, u0).
+struct PoseGeom {
+  float src[3], u[3], v[3], w[3];
+};
+
+__device__ __forceinline__ PoseGeom pose_geom(const float* __restrict__ M, const float* __restrict__ lat, int n) {
+  const float* m = M + size_t(n) * 16;
+  PoseGeom g;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float r0 = m[4 * i], r1 = m[4 * i + 1], r2 = m[4 * i + 2];
+    g.src[i] = fmaf(r0, lat[0], fmaf(r1, lat[1], fmaf(r2, lat[2], m[4 * i + 3])));
+    g.u[i] = fmaf(r0, lat[3], fmaf(r1, lat[4], r2 * lat[5]));
+    g.v[i] = fmaf(r0, lat[6], fmaf(r1, lat[7], r2 * lat[8]));
+    g.w[i] = fmaf(r0, lat[9], fmaf(r1, lat[10], r2 * lat[11]));
+  }
+  return g;
+}
+
+__global__ __launch_bounds__(256) void pou_pose_geom_kernel(const float* __restrict__ M, int N,
+                                                            const float* __restrict__ lat,
+                                                            const float* __restrict__ centers, int B,
+                                                            float* __restrict__ u_xyz, float* __restrict__ v_xyz,
+                                                            float* __restrict__ d_b, float* __restrict__ cu,
+                                                            float* __restrict__ cv) {
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx >= B * N) return;
+  const int b = idx / N, n = idx - b * N;
+  const PoseGeom g = pose_geom(M, lat, n);
+  const float d0 = centers[3 * b] - g.src[0], d1 = centers[3 * b + 1] - g.src[1], d2 = centers[3 * b + 2] - g.src[2];
+  d_b[idx] = fmaf(d0, g.w[0], fmaf(d1, g.w[1], d2 * g.w[2]));
+  cu[idx] = fmaf(d0, g.u[0], fmaf(d1, g.u[1], d2 * g.u[2]));
+  cv[idx] = fmaf(d0, g.v[0], fmaf(d1, g.v[1], d2 * g.v[2]));
+  if (b == 0) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      u_xyz[3 * n + i] = g.u[2 - i];
+      v_xyz[3 * n + i] = g.v[2 - i];
+    }
+  }
+}
+
+// d(sum go * render)/dM from the fused adjoint's per-(block, pose) sums part[b][n][9]
+// (d/d(d_b, cu, cv), d/du_xyz, d/dv_xyz), summed over the blocks in fixed order in float64:
+//   g_src = -sum_b (g_db w + g_cu u + g_cv v),  g_w = sum_b g_db (c_b - src),
+//   g_u = flip(sum_b d/du_xyz) + sum_b g_cu (c_b - src),  g_v likewise,
+//   dM[i][j] = g_src[i] src0[j] + g_u[i] u0[j] + g_v[i] v0[j] + g_w[i] n0[j],  dM[i][3] = g_src[i],
+// and row 3 zero.  One thread per pose.
+__global__ __launch_bounds__(64) void pou_pose_geom_bwd_kernel(const float* __restrict__ M, int N,
+                                                               const float* __restrict__ lat,
+                                                               const float* __restrict__ centers, int B,
+                                                               const float* __restrict__ part,
+                                                               float* __restrict__ gM) {
+  const int n = blockIdx.x * 64 + threadIdx.x;
+  if (n >= N) return;
+  const PoseGeom g = pose_geom(M, lat, n);
+  double gs[3] = {0.0, 0.0, 0.0}, gw[3] = {0.0, 0.0, 0.0}, gu[3] = {0.0, 0.0, 0.0}, gv[3] = {0.0, 0.0, 0.0};
+  for (int b = 0; b < B; ++b) {
+    const float* p = part + (size_t(b) * N + n) * 9;
+    const double gdb = p[0], gcu = p[1], gcv = p[2];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const double d = double(centers[3 * b + i]) - double(g.src[i]);
+      gs[i] -= gdb * g.w[i] + gcu * g.u[i] + gcv * g.v[i];
+      gw[i] += gdb * d;
+      gu[i] += gcu * d + double(p[3 + 2 - i]);
+      gv[i] += gcv * d + double(p[6 + 2 - i]);
+    }
+  }
+  float* o = gM + size_t(n) * 16;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      o[4 * i + j] = float(gs[i] * lat[j] + gu[i] * lat[3 + j] + gv[i] * lat[6 + j] + gw[i] * lat[9 + j]);
+    o[4 * i + 3] = float(gs[i]);
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) o[12 + j] = 0.f;
+}
+
 // planar (B, 2, P, P, P) -> gather layout; one thread per stored element (margins written as 0)
 template <int VT>
 __global__ __launch_bounds__(256) void pou_pack_kernel(const void* __restrict__ F, int P, PouPk k,
@@ -1541,6 +1629,25 @@ int fvr_pou_ray_grads(const float* imgs, int32_t B, int32_t N, int32_t P, const 
   hipLaunchKernelGGL(pou_ray_grads_kernel, dim3((K + 255) / 256, N), dim3(256), 0,
                      reinterpret_cast<hipStream_t>(stream), imgs, go, d_b, cu, cv, rx, ry, ic, g, grx, gry, gic);
   FVR_CHECK_LAUNCH("pou_ray_grads_kernel");
+  return FVR_OK;
+}
+
+int fvr_pou_pose_geom(const float* M, int32_t N, const float* lat, const float* centers, int32_t B, float* u_xyz,
+                      float* v_xyz, float* d_b, float* cu, float* cv, void* stream) {
+  FVR_CHECK_ARG(M && lat && centers && u_xyz && v_xyz && d_b && cu && cv, "fvr_pou_pose_geom: null pointer");
+  FVR_CHECK_ARG(N > 0 && B > 0 && int64_t(B) * N < (int64_t(1) << 30), "fvr_pou_pose_geom: bad shape");
+  hipLaunchKernelGGL(pou_pose_geom_kernel, dim3(unsigned((int64_t(B) * N + 255) / 256)), dim3(256), 0,
+                     reinterpret_cast<hipStream_t>(stream), M, N, lat, centers, B, u_xyz, v_xyz, d_b, cu, cv);
+  FVR_CHECK_LAUNCH("pou_pose_geom_kernel");
+  return FVR_OK;
+}
+
+int fvr_pou_pose_geom_bwd(const float* M, int32_t N, const float* lat, const float* centers, int32_t B,
+                          const float* part, float* gM, void* stream) {
+  FVR_CHECK_ARG(M && lat && centers && part && gM && N > 0 && B > 0, "fvr_pou_pose_geom_bwd: bad arguments");
+  hipLaunchKernelGGL(pou_pose_geom_bwd_kernel, dim3(unsigned((N + 63) / 64)), dim3(64), 0,
+                     reinterpret_cast<hipStream_t>(stream), M, N, lat, centers, B, part, gM);
+  FVR_CHECK_LAUNCH("pou_pose_geom_bwd_kernel");
   return FVR_OK;
 }
 

@@ -116,6 +116,8 @@ _SIGNATURES = {
     "fvr_pou_render_bwd_sep": ([_U64, _I, _I, _I, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _F, _F, _I,
                                 _P, _P], _I),
     "fvr_pou_ray_grads": ([_P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _I, _I, _F, _F, _P, _P, _P, _P], _I),
+    "fvr_pou_pose_geom": ([_P, _I, _P, _P, _I, _P, _P, _P, _P, _P, _P], _I),
+    "fvr_pou_pose_geom_bwd": ([_P, _I, _P, _P, _I, _P, _P, _P], _I),
     "fvr_pou_sum_blocks": ([_P, _I, _I, _I, _I, _I, _P, _P], _I),
     "fvr_edge_tiles": ([_I, _I, _I], _I),
     "fvr_edge_channels": ([_P, _I, _I, _I, _I, _I, _P, _P], _I),

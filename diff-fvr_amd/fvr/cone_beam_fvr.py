@@ -186,6 +186,74 @@ def _separable_rays(model, rx: torch.Tensor, ry: torch.Tensor, H: int, W: int):
     return res
 
 
+class _PouState:
+    """What the fused forward leaves for the adjoint (device tensors and shapes)."""
+    __slots__ = ("u", "v", "db", "cu", "cv", "rays", "per_pose", "K", "H", "W", "sep", "imgs")
+
+
+def _pou_forward(model, u, v, db, c_u, c_v, rx, ry, ic, out_h, out_w):
+    """Slab images (slice + LDS iFFT) and the warp onto the detector for float32 device
+    u, v (N, 3) in (x, y, z) voxel order and d_b, cu, cv (B, N): (out (N, H, W), state)."""
+    dev = model.F_blk_ri.device
+    B, P, N = model.B, model.P, u.shape[0]
+    (rxf, ryf, icf), per_pose, K = _rays(rx, ry, ic, N, dev)
+    if K != out_h * out_w or tuple(db.shape) != (B, N):
+        raise ValueError("PoU render: inconsistent shapes")
+    L = _lib.lib()
+    st = _lib.stream_ptr(dev)
+    Fp = model.packed_blocks()
+    imgs = torch.empty((B, N, P, P), device=dev, dtype=torch.float32)
+    _lib.check(L.fvr_pou_slice_ifft(Fp.data_ptr(), _VOL_DTYPES[model.F_blk_ri.dtype], B, P, _lib.ptr(u), _lib.ptr(v),
+                                    N, _lib.ptr(imgs), st), "fvr_pou_slice_ifft")
+    out = torch.empty((N, out_h, out_w), device=dev, dtype=torch.float32)
+    _lib.check(L.fvr_pou_warp_fwd(_lib.ptr(imgs), B, N, P, _lib.ptr(db), _lib.ptr(c_u), _lib.ptr(c_v), _lib.ptr(rxf),
+                                  _lib.ptr(ryf), _lib.ptr(icf), per_pose, K, 1.0 / model.ds_block, float(P // 2),
+                                  _lib.ptr(out), st), "fvr_pou_warp_fwd")
+    sep = None
+    if not per_pose and model.separable_adjoint and L.fvr_pou_sep_ok(P, out_h, out_w):
+        sep = _separable_rays(model, rx, ry, out_h, out_w)
+    s = _PouState()
+    s.u, s.v, s.db, s.cu, s.cv, s.rays, s.per_pose, s.K, s.H, s.W, s.sep, s.imgs = (
+        u, v, db, c_u, c_v, (rxf, ryf, icf), per_pose, K, out_h, out_w, sep, imgs)
+    return out, s
+
+
+def _pou_adjoint(model, s, g):
+    """The fused adjoint of _pou_forward for the upstream gradient g (N, H, W): per (block,
+    pose) the nine sums d/d(d_b, cu, cv), d/du (3), d/dv (3) as part (B, N, 9) and the
+    output gradient go (N, K)."""
+    dev = s.imgs.device
+    B, P, N, K = model.B, model.P, s.u.shape[0], s.K
+    L = _lib.lib()
+    st = _lib.stream_ptr(dev)
+    go = _f32(g, dev).reshape(N, K)
+    part = torch.empty((B, N, 9), device=dev, dtype=torch.float32)
+    Fp = model.packed_blocks()
+    dt = _VOL_DTYPES[model.F_blk_ri.dtype]
+    inv_ds, c_idx = 1.0 / model.ds_block, float(P // 2)
+    rxf, ryf, icf = s.rays
+    if s.sep is not None:
+        X = torch.empty((N, K), device=dev, dtype=torch.float32)
+        flag = torch.empty(N, device=dev, dtype=torch.float32)
+        _lib.check(L.fvr_pou_gprep(_lib.ptr(go), _lib.ptr(icf), N, K, s.per_pose, _lib.ptr(X), _lib.ptr(flag), st),
+                   "fvr_pou_gprep")
+        rx_col, ry_row = s.sep
+        _lib.check(L.fvr_pou_render_bwd_sep(Fp.data_ptr(), dt, B, P, _lib.ptr(s.u), _lib.ptr(s.v), N, _lib.ptr(s.imgs),
+                                            _lib.ptr(X), _lib.ptr(flag), _lib.ptr(s.db), _lib.ptr(s.cu),
+                                            _lib.ptr(s.cv), _lib.ptr(rx_col), _lib.ptr(ry_row), s.H, s.W, inv_ds,
+                                            c_idx, int(model.sep_adjoint_order), _lib.ptr(part), st),
+                   "fvr_pou_render_bwd_sep")
+    else:
+        scale = torch.empty(N, device=dev, dtype=torch.float32)
+        _lib.check(L.fvr_pou_gscale(_lib.ptr(go), _lib.ptr(icf), N, K, s.per_pose, _lib.ptr(scale), st),
+                   "fvr_pou_gscale")
+        _lib.check(L.fvr_pou_render_bwd(Fp.data_ptr(), dt, B, P, _lib.ptr(s.u), _lib.ptr(s.v), N, _lib.ptr(s.imgs),
+                                        _lib.ptr(go), _lib.ptr(s.db), _lib.ptr(s.cu), _lib.ptr(s.cv), _lib.ptr(rxf),
+                                        _lib.ptr(ryf), _lib.ptr(icf), s.per_pose, K, inv_ds, c_idx,
+                                        _lib.ptr(scale), _lib.ptr(part), st), "fvr_pou_render_bwd")
+    return part, go
+
+
 class _PouRender(torch.autograd.Function):
     """Fused slice -> LDS iFFT -> warp for every (block, pose); backward in one fused launch
     (the gather form of the warp adjoint for separable rays, the fixed-point scan otherwise)."""
@@ -193,76 +261,68 @@ class _PouRender(torch.autograd.Function):
     @staticmethod
     def forward(ctx, u_xyz, v_xyz, d_b, cu, cv, rx, ry, ic, model, out_h, out_w):
         dev = model.F_blk_ri.device
-        B, P, N = model.B, model.P, u_xyz.shape[0]
-        u, v = _f32(u_xyz, dev), _f32(v_xyz, dev)
-        db, c_u, c_v = _f32(d_b, dev), _f32(cu, dev), _f32(cv, dev)
-        (rxf, ryf, icf), per_pose, K = _rays(rx, ry, ic, N, dev)
-        if K != out_h * out_w or tuple(db.shape) != (B, N):
-            raise ValueError("PoU render: inconsistent shapes")
-        L = _lib.lib()
-        st = _lib.stream_ptr(dev)
-        Fp = model.packed_blocks()
-        imgs = torch.empty((B, N, P, P), device=dev, dtype=torch.float32)
-        _lib.check(L.fvr_pou_slice_ifft(Fp.data_ptr(), _VOL_DTYPES[model.F_blk_ri.dtype], B, P, _lib.ptr(u),
-                                        _lib.ptr(v), N, _lib.ptr(imgs), st), "fvr_pou_slice_ifft")
-        out = torch.empty((N, out_h, out_w), device=dev, dtype=torch.float32)
-        inv_ds, c_idx = 1.0 / model.ds_block, float(P // 2)
-        _lib.check(L.fvr_pou_warp_fwd(_lib.ptr(imgs), B, N, P, _lib.ptr(db), _lib.ptr(c_u), _lib.ptr(c_v),
-                                      _lib.ptr(rxf), _lib.ptr(ryf), _lib.ptr(icf), per_pose, K, inv_ds, c_idx,
-                                      _lib.ptr(out), st), "fvr_pou_warp_fwd")
-        sep = None
-        if not per_pose and model.separable_adjoint and L.fvr_pou_sep_ok(P, out_h, out_w):
-            sep = _separable_rays(model, rx, ry, out_h, out_w)
-        ctx.sep = sep
-        ctx.save_for_backward(u, v, db, c_u, c_v, rxf, ryf, icf, imgs)
-        ctx.model = model
-        ctx.meta = (B, N, P, K, per_pose, inv_ds, c_idx, rx.shape, u_xyz.dtype, out_h, out_w)
+        out, s = _pou_forward(model, _f32(u_xyz, dev), _f32(v_xyz, dev), _f32(d_b, dev), _f32(cu, dev),
+                              _f32(cv, dev), rx, ry, ic, out_h, out_w)
+        ctx.state, ctx.model = s, model
+        ctx.meta = (rx.shape, u_xyz.dtype)
         ctx.dts = tuple(t.dtype for t in (d_b, cu, cv, rx, ry, ic))
         return out
 
     @staticmethod
     def backward(ctx, g):
-        u, v, db, c_u, c_v, rxf, ryf, icf, imgs = ctx.saved_tensors
-        B, N, P, K, per_pose, inv_ds, c_idx, rshape, udt, H, W = ctx.meta
-        model = ctx.model
-        dev = imgs.device
+        s, model = ctx.state, ctx.model
+        rshape, udt = ctx.meta
+        B, P, N, K = model.B, model.P, s.u.shape[0], s.K
+        dev = s.imgs.device
         L = _lib.lib()
         st = _lib.stream_ptr(dev)
-        go = _f32(g, dev).reshape(N, K)
-        part = torch.empty((B, N, 9), device=dev, dtype=torch.float32)
-        Fp = model.packed_blocks()
-        dt = _VOL_DTYPES[model.F_blk_ri.dtype]
-        if ctx.sep is not None:
-            X = torch.empty((N, K), device=dev, dtype=torch.float32)
-            flag = torch.empty(N, device=dev, dtype=torch.float32)
-            _lib.check(L.fvr_pou_gprep(_lib.ptr(go), _lib.ptr(icf), N, K, per_pose, _lib.ptr(X), _lib.ptr(flag), st),
-                       "fvr_pou_gprep")
-            rx_col, ry_row = ctx.sep
-            _lib.check(L.fvr_pou_render_bwd_sep(Fp.data_ptr(), dt, B, P, _lib.ptr(u), _lib.ptr(v), N, _lib.ptr(imgs),
-                                                _lib.ptr(X), _lib.ptr(flag), _lib.ptr(db), _lib.ptr(c_u),
-                                                _lib.ptr(c_v), _lib.ptr(rx_col), _lib.ptr(ry_row), H, W, inv_ds,
-                                                c_idx, int(model.sep_adjoint_order), _lib.ptr(part), st),
-                       "fvr_pou_render_bwd_sep")
-        else:
-            scale = torch.empty(N, device=dev, dtype=torch.float32)
-            _lib.check(L.fvr_pou_gscale(_lib.ptr(go), _lib.ptr(icf), N, K, per_pose, _lib.ptr(scale), st),
-                       "fvr_pou_gscale")
-            _lib.check(L.fvr_pou_render_bwd(Fp.data_ptr(), dt, B, P, _lib.ptr(u), _lib.ptr(v), N, _lib.ptr(imgs),
-                                            _lib.ptr(go), _lib.ptr(db), _lib.ptr(c_u), _lib.ptr(c_v), _lib.ptr(rxf),
-                                            _lib.ptr(ryf), _lib.ptr(icf), per_pose, K, inv_ds, c_idx,
-                                            _lib.ptr(scale), _lib.ptr(part), st), "fvr_pou_render_bwd")
+        part, go = _pou_adjoint(model, s, g)
         guv = torch.empty((N, 6), device=dev, dtype=torch.float32)
         _lib.check(L.fvr_pou_sum_blocks(_lib.ptr(part), B, N, 9, 3, 6, _lib.ptr(guv), st), "fvr_pou_sum_blocks")
         grx = gry = gic = None
-        if per_pose and any(ctx.needs_input_grad[5:8]):
+        if s.per_pose and any(ctx.needs_input_grad[5:8]):
+            rxf, ryf, icf = s.rays
             grx, gry, gic = (torch.empty((N, K), device=dev, dtype=torch.float32) for _ in range(3))
-            _lib.check(L.fvr_pou_ray_grads(_lib.ptr(imgs), B, N, P, _lib.ptr(go), _lib.ptr(db), _lib.ptr(c_u),
-                                           _lib.ptr(c_v), _lib.ptr(rxf), _lib.ptr(ryf), _lib.ptr(icf), per_pose, K,
-                                           inv_ds, c_idx, _lib.ptr(grx), _lib.ptr(gry), _lib.ptr(gic), st),
-                       "fvr_pou_ray_grads")
+            _lib.check(L.fvr_pou_ray_grads(_lib.ptr(s.imgs), B, N, P, _lib.ptr(go), _lib.ptr(s.db), _lib.ptr(s.cu),
+                                           _lib.ptr(s.cv), _lib.ptr(rxf), _lib.ptr(ryf), _lib.ptr(icf), s.per_pose, K,
+                                           1.0 / model.ds_block, float(P // 2), _lib.ptr(grx), _lib.ptr(gry),
+                                           _lib.ptr(gic), st), "fvr_pou_ray_grads")
             grx, gry, gic = (t.reshape(rshape).to(dt) for t, dt in zip((grx, gry, gic), ctx.dts[3:]))
         return (guv[:, 0:3].to(udt), guv[:, 3:6].to(udt), *(part[..., k].to(ctx.dts[k]) for k in range(3)), grx, gry, gic,
                 None, None, None)
+
+
+class _PouPinhole(torch.autograd.Function):
+    """forward_pinhole on a lattice with an SE(3)-invariant ray cache, fused: the pose geometry
+    (detector axes and block offsets, cone_beam_fvr.py:507-521, 670-688) in one launch, then
+    slice + LDS iFFT and the warp; backward: the fused adjoint, then the block sums and the
+    geometry's adjoint to the pose matrices in one launch."""
+
+    @staticmethod
+    def forward(ctx, pose_mats, model, lat12, rx, ry, ic, out_h, out_w):
+        dev = model.F_blk_ri.device
+        M = _f32(pose_mats, dev).contiguous()
+        N, B = M.shape[0], model.B
+        L = _lib.lib()
+        st = _lib.stream_ptr(dev)
+        u, v = (torch.empty((N, 3), device=dev, dtype=torch.float32) for _ in range(2))
+        db, c_u, c_v = (torch.empty((B, N), device=dev, dtype=torch.float32) for _ in range(3))
+        _lib.check(L.fvr_pou_pose_geom(_lib.ptr(M), N, _lib.ptr(lat12), _lib.ptr(model.centers), B, _lib.ptr(u),
+                                       _lib.ptr(v), _lib.ptr(db), _lib.ptr(c_u), _lib.ptr(c_v), st),
+                   "fvr_pou_pose_geom")
+        out, s = _pou_forward(model, u, v, db, c_u, c_v, rx, ry, ic, out_h, out_w)
+        ctx.state, ctx.model, ctx.M, ctx.lat12, ctx.dt = s, model, M, lat12, pose_mats.dtype
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        s, model, M = ctx.state, ctx.model, ctx.M
+        part, _ = _pou_adjoint(model, s, g)
+        gM = torch.empty_like(M)
+        _lib.check(_lib.lib().fvr_pou_pose_geom_bwd(_lib.ptr(M), M.shape[0], _lib.ptr(ctx.lat12),
+                                                    _lib.ptr(model.centers), model.B, _lib.ptr(part), _lib.ptr(gM),
+                                                    _lib.stream_ptr(M.device)), "fvr_pou_pose_geom_bwd")
+        return gM.to(ctx.dt), None, None, None, None, None, None, None
 
 
 # ---------------------------------------------------------------------------
@@ -546,10 +606,26 @@ class PerspectiveSlabFVR(nn.Module):
         return (src0.to(f32), trg0.to(f32), u0.to(f32), v0.to(f32), n0.to(f32), rx, ry, ic,
                 rx.reshape(-1).contiguous(), ry.reshape(-1).contiguous(), ic.reshape(-1).contiguous())
 
+    def _lattice_frame(self, lattice) -> torch.Tensor:
+        """(src0, u0, v0, n0) of a lattice as one float32 device array of 12, cached for the
+        lattice tensors last seen (object and version)."""
+        ts = lattice[0], lattice[2], lattice[3], lattice[4]
+        key = tuple((id(t), t._version) for t in ts)
+        c = getattr(self, "_lat12", None)
+        if c is not None and c[0] == key and all(r() is t for r, t in zip(c[1], ts)):
+            return c[2]
+        lat12 = torch.cat([_f32(t, self.F_blk_ri.device).reshape(3) for t in ts]).contiguous()
+        self._lat12 = (key, [weakref.ref(t) for t in ts], lat12)
+        return lat12
+
     def _forward_from_lattice(self, pose_mats: torch.Tensor, lattice) -> torch.Tensor:
         """Transform a camera-frame lattice by SE(3) matrices (N, 4, 4) and project (:670-688)."""
         dev = self.F_blk_ri.device
         pose_mats = pose_mats.to(dev)
+        if len(lattice) >= 11 and self.fused_ok() and pose_mats.dim() == 3 and tuple(pose_mats.shape[1:]) == (4, 4):
+            rx, ry, ic = (x.to(dev) for x in lattice[5:8])
+            return _PouPinhole.apply(pose_mats, self, self._lattice_frame(lattice), rx, ry, ic, int(rx.shape[-2]),
+                                     int(rx.shape[-1])).unsqueeze(1)
         src0, trg0, u0, v0, n0 = (t.to(dev) for t in lattice[:5])
         R, t = pose_mats[:, :3, :3], pose_mats[:, :3, 3]
         srcs = torch.einsum("nij,j->ni", R, src0) + t

@@ -35,6 +35,9 @@
  *                       adjoint as a gather; fvr_pou_gprep forms its input.
  *   fvr_pou_ray_grads   the ray-array gradients grid_sample gives autograd in
  *                       cone_beam_fvr.py:423-478 _project_core.
+ *   fvr_pou_pose_geom(_bwd)  cone_beam_fvr.py:670-688 _forward_from_lattice and
+ *                       :507-521 _compute_block_offsets (pose -> detector axes and
+ *                       block offsets) and their adjoint to the pose matrices.
  *   fvr_edge_*          cone_beam_fvr.py:71-112 edge_dominant_ncc (forward moments,
  *                       combine, and the gradient w.r.t. the images).
  */
@@ -433,6 +436,16 @@ int fvr_pou_ray_grads(const float* imgs, int32_t B, int32_t N, int32_t P, const 
                       const float* d_b, const float* cu, const float* cv, const float* rx,
                       const float* ry, const float* ic, int32_t per_pose_rays, int32_t K,
                       float inv_ds, float c_idx, float* grx, float* gry, float* gic, void* stream);
+/* Pose geometry of the lattice path (cone_beam_fvr.py:670-688, 507-521): M (N, 4, 4)
+ * row-major pose matrices [R t; 0 1], lat = (src0, u0, v0, n0) (12 floats), centers (B, 3);
+ * src = R src0 + t, u = R u0, v = R v0, w = R n0, d_b[b][n] = (c_b - src) . w, cu, cv the
+ * same with u, v; u_xyz / v_xyz (N, 3) = u, v reversed (the slice kernels' (x, y, z)). */
+int fvr_pou_pose_geom(const float* M, int32_t N, const float* lat, const float* centers, int32_t B,
+                      float* u_xyz, float* v_xyz, float* d_b, float* cu, float* cv, void* stream);
+/* gM (N, 4, 4): the adjoint of fvr_pou_pose_geom applied to the fused adjoint's sums part
+ * (B, N, 9) of fvr_pou_render_bwd(_sep), the blocks summed in fixed order in float64. */
+int fvr_pou_pose_geom_bwd(const float* M, int32_t N, const float* lat, const float* centers, int32_t B,
+                          const float* part, float* gM, void* stream);
 /* out[n][c] = sum_b part[b][n][c0 + c] in float64, fixed order (c < nc). */
 int fvr_pou_sum_blocks(const float* part, int32_t B, int32_t N, int32_t C, int32_t c0, int32_t nc,
                        float* out, void* stream);

@@ -443,3 +443,27 @@ def test_separable_gather_adjoint_propagates_non_finite_upstream_gradients(model
     assert m._sep_rays[3] is not None
     assert np.isnan(g2[1, :3, :]).all()
     assert np.array_equal(g2[0], g1[0]) and np.array_equal(g2[2], g1[2])
+
+
+def test_fused_pinhole_geometry_matches_the_torch_chain(model):
+    """forward_pinhole on a lattice with a ray cache computes the pose geometry (source,
+    detector axes, block offsets) in one launch and its adjoint in another
+    (fvr_pou_pose_geom / _bwd); projections and pose-matrix gradients equal those of the
+    reference's torch composition (cone_beam_fvr.py:670-688, 507-521) over the same
+    render operator, to float32 rounding."""
+    m, _ = model
+    lat = [t.to(DEV) for t in _lattice()[0]]
+    w = torch.randn((3, 1, 20, 22), generator=torch.Generator().manual_seed(13)).to(DEV)
+    M = torch.from_numpy(MATS).float().to(DEV).requires_grad_(True)
+    out = m.forward_pinhole(M, lat)
+    (out * w).sum().backward()
+    M2 = torch.from_numpy(MATS).float().to(DEV).requires_grad_(True)
+    R, t = M2[:, :3, :3], M2[:, :3, 3]
+    srcs = torch.einsum("nij,j->ni", R, lat[0]) + t
+    u_det, v_det, n_0 = (torch.einsum("nij,j->ni", R, a) for a in lat[2:5])
+    ref = m._project_core_invariant(srcs, u_det, v_det, n_0, lat[5:11]).unsqueeze(1)
+    (ref * w).sum().backward()
+    assert float((out - ref).abs().max()) <= 1e-5 * float(ref.abs().max())
+    g, gr = M.grad.double().cpu().numpy(), M2.grad.double().cpu().numpy()
+    assert np.max(np.abs(g - gr)) <= 1e-5 * np.abs(gr).max(), np.abs(g - gr).max()
+    assert np.all(g[:, 3, :] == 0.0)
