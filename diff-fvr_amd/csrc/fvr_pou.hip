@@ -33,7 +33,6 @@ namespace fvr {
 constexpr int kPouThreads = 512;
 constexpr int kPouSliceThreads = 1024;  // the fused adjoint (the full-size LDS image: one workgroup per CU)
 constexpr int kPouHalfThreads = 512;    // the forward slice + iFFT (half-spectrum LDS: two workgroups per CU)
-constexpr int kPouPairBatch = 2;        // sample pairs in flight per thread in the forward gather
 #ifndef FVR_POU_BATCH
 #define FVR_POU_BATCH 4
 #endif
@@ -337,38 +336,6 @@ struct PouSlice {
 // so the poses of one block run on one XCD and share its L2 copy of the block's slices
 __device__ __forceinline__ int pou_bn(int w, int per_xcd) { return (w & 7) * per_xcd + (w >> 3); }
 
-// Slice sample (row i, column j) of linear sample index q: every 64 consecutive q (one
-// wave's lanes) form a 2-row x 32-column patch (P >= 32; row-major below), which crosses
-// about half as many storage rows of a tilted slice per load as a 1 x 64 row does.
-__device__ __forceinline__ void pou_ij(int q, int logP, int& i, int& j) {
-  if (logP >= 5) {
-    const int w = q >> 6, lane = q & 63, sh = logP - 5;  // 2^sh patches per 2-row band
-    i = 2 * (w >> sh) + (lane >> 5);
-    j = 32 * (w & ((1 << sh) - 1)) + (lane & 31);
-  } else {
-    i = q >> logP;
-    j = q & ((1 << logP) - 1);
-  }
-}
-
-// issue kPouBatch samples (pou_ij(q): row i, column j) of one slice: columns follow u,
-// rows follow v, sample at c + (j - P/2) u + (i - P/2) v
-template <int VT>
-__device__ __forceinline__ void pk_issue_batch(PkSmp<VT> (&s)[kPouBatch], unsigned long long blk, const PouPk& k,
-                                               const PouSlice& g, int q0, int stride, float ux, float uy, float uz,
-                                               float vx, float vy, float vz) {
-  const int P = g.P, hp = P / 2, nq = P * P;
-#pragma unroll
-  for (int t = 0; t < kPouBatch; ++t) {
-    const int q = min(q0 + t * stride, nq - 1);
-    int i, j;
-    pou_ij(q, g.logP, i, j);
-    const float kx = float(j - hp), ky = float(i - hp);
-    pk_issue<VT>(s[t], blk, k, P, fmaf(ky, vx, fmaf(kx, ux, g.c)), fmaf(ky, vy, fmaf(kx, uy, g.c)),
-                 fmaf(ky, vz, fmaf(kx, uz, g.c)));
-  }
-}
-
 // slice sample (row i, column jj) of the half plane, jj in [0, W) with W = 2^logW: every 64
 // consecutive q form a 2-row x 32-column patch (W >= 32), else row-major
 __device__ __forceinline__ void pou_ij_w(int q, int logW, int& i, int& jj) {
@@ -380,6 +347,46 @@ __device__ __forceinline__ void pou_ij_w(int q, int logW, int& i, int& jj) {
     i = q >> logW;
     jj = q & ((1 << logW) - 1);
   }
+}
+
+// position (block index space) of slice sample (i, j): c + (j - P/2) u + (i - P/2) v
+__device__ __forceinline__ void pou_kpos(const PouSlice& g, float ux, float uy, float uz, float vx, float vy,
+                                         float vz, int i, int j, float& px, float& py, float& pz) {
+  const int hp = g.P / 2;
+  const float kx = float(j - hp), ky = float(i - hp);
+  px = fmaf(ky, vx, fmaf(kx, ux, g.c));
+  py = fmaf(ky, vy, fmaf(kx, uy, g.c));
+  pz = fmaf(ky, vz, fmaf(kx, uz, g.c));
+}
+
+// the four samples that are their own DFT mirror: i, j in {0, P/2}
+__device__ __forceinline__ bool pou_self_mirror(int i, int j, int hp) {
+  return (i == 0 || i == hp) && (j == 0 || j == hp);
+}
+
+// Whether the DFT mirror of sample (i, j) at p must be gathered: the stored block spectrum
+// of a real cube is Hermitian about the centre c = P/2 on the storage indices 1 .. P-1, so
+// the trilinear sample at the reflection 2c - p is conj S(p) when no corner of p (hence of
+// its reflection) is storage index 0 or P, i.e. floor(p) is not in [-1, 0] or [P-1, P] on
+// any axis (corners further out read the zero padding on both sides).  On row / column 0
+// (frequency -P/2) the DFT mirror is not the reflection, so it is always gathered.
+__device__ __forceinline__ bool pou_edge(float p, int P) {
+  return (p >= -1.f && p < 1.f) || (p >= float(P - 1) && p < float(P + 1));
+}
+__device__ __forceinline__ bool pou_mirror_explicit(int i, int j, int P, float px, float py, float pz) {
+  return i == 0 || j == 0 || pou_edge(px, P) || pou_edge(py, P) || pou_edge(pz, P);
+}
+// The same for the spatial gradient of the sample: at (or within float rounding of) a lattice
+// plane the trilinear derivative is one-sided (the cell above), and the mirror's is the cell
+// above the reflection, not the reflected cell, so those samples take the explicit mirror too
+// (1e-4 is far above the rounding of the mirror's position, about 1e-5 at P = 128).
+__device__ __forceinline__ bool pou_near_lattice(float p) {
+  const float f = p - floorf(p);
+  return !(f > 1e-4f && f < 1.f - 1e-4f);
+}
+__device__ __forceinline__ bool pou_grad_mirror_explicit(int i, int j, int P, float px, float py, float pz) {
+  return pou_mirror_explicit(i, j, P, px, py, pz) || pou_near_lattice(px) || pou_near_lattice(py) ||
+         pou_near_lattice(pz);
 }
 
 // Real slab image of one (block, pose) through a Hermitian half spectrum.  The image is
@@ -408,43 +415,70 @@ __global__ __launch_bounds__(kPouHalfThreads) void pou_slice_ifft_kernel(const c
   const float vx = v[3 * n], vy = v[3 * n + 1], vz = v[3 * n + 2];
   const unsigned long long blk = reinterpret_cast<unsigned long long>(Fp) + (unsigned long long)b * k.block;
   const int stride = blockDim.x;
-  // sample (i, j) sits at c + (j - P/2) u + (i - P/2) v; its mirror (-k) is ((P - i) mod P, (P - j) mod P)
-  auto issue_pair = [&](PkSmp<VT>& s0, PkSmp<VT>& s1, int i, int j) {
-    const float kx = float(j - hp), ky = float(i - hp);
-    const int im = (P - i) & (P - 1), jm = (P - j) & (P - 1);
-    const float mx = float(jm - hp), my = float(im - hp);
-    pk_issue<VT>(s0, blk, k, P, fmaf(ky, vx, fmaf(kx, ux, g.c)), fmaf(ky, vy, fmaf(kx, uy, g.c)),
-                 fmaf(ky, vz, fmaf(kx, uz, g.c)));
-    pk_issue<VT>(s1, blk, k, P, fmaf(my, vx, fmaf(mx, ux, g.c)), fmaf(my, vy, fmaf(mx, uy, g.c)),
-                 fmaf(my, vz, fmaf(mx, uz, g.c)));
+  // Sample (i, j) sits at c + (j - P/2) u + (i - P/2) v; its DFT mirror is ((P - i) mod P,
+  // (P - j) mod P).  Pass 1 stores H = S(k) (Re S for the four self-mirrored samples); pass 2
+  // completes H = (S(k) + conj S(mirror)) / 2 where the mirror is not conj S(k) (pou_mirror_mode).
+  const int nq = P * hp, nitems = nq + P, logH = logP - 1;
+  auto item = [&](int q, int& i, int& j, int& vcol) {  // stored entry q: columns v = jj < P/2, then v = P/2
+    if (q < nq) {
+      int jj;
+      pou_ij_w(q, logH, i, jj);
+      j = hp + jj;
+      vcol = jj;
+    } else {
+      i = q - nq;
+      j = 0;
+      vcol = hp;
+    }
   };
-  auto store_pair = [&](const PkSmp<VT>& s0, const PkSmp<VT>& s1, int i, int vcol) {
-    float cr[8], ci[8], r0, i0, r1, i1;
-    pk_corners<VT>(s0, cr, ci);
-    interp8<false>(cr, ci, s0.tx, s0.ty, s0.tz, r0, i0, nullptr, nullptr);
-    pk_corners<VT>(s1, cr, ci);
-    interp8<false>(cr, ci, s1.tx, s1.ty, s1.tz, r1, i1, nullptr, nullptr);
-    a[((i + hp) & (P - 1)) * pitch + vcol] = make_float2(0.5f * (r0 + r1), 0.5f * (i0 - i1));
+  auto sample = [&](const PkSmp<VT>& sm, float& re, float& im) {
+    float cr[8], ci[8];
+    pk_corners<VT>(sm, cr, ci);
+    interp8<false>(cr, ci, sm.tx, sm.ty, sm.tz, re, im, nullptr, nullptr);
   };
-  // columns v = jj in [0, P/2): samples j = P/2 + jj and their mirrors
-  const int nq = P * hp, logH = logP - 1;
-  for (int q0 = threadIdx.x; q0 < nq; q0 += kPouPairBatch * stride) {
-    PkSmp<VT> s0[kPouPairBatch], s1[kPouPairBatch];
-    int ii[kPouPairBatch], jj[kPouPairBatch];
+  for (int q0 = threadIdx.x; q0 < nitems; q0 += kPouBatch * stride) {
+    PkSmp<VT> sm[kPouBatch];
+    int ii[kPouBatch], jj[kPouBatch], vc[kPouBatch];
 #pragma unroll
-    for (int t = 0; t < kPouPairBatch; ++t) {
-      pou_ij_w(min(q0 + t * stride, nq - 1), logH, ii[t], jj[t]);
-      issue_pair(s0[t], s1[t], ii[t], hp + jj[t]);
+    for (int t = 0; t < kPouBatch; ++t) {
+      item(min(q0 + t * stride, nitems - 1), ii[t], jj[t], vc[t]);
+      float px, py, pz;
+      pou_kpos(g, ux, uy, uz, vx, vy, vz, ii[t], jj[t], px, py, pz);
+      pk_issue<VT>(sm[t], blk, k, P, px, py, pz);
     }
 #pragma unroll
-    for (int t = 0; t < kPouPairBatch; ++t)
-      if (q0 + t * stride < nq) store_pair(s0[t], s1[t], ii[t], jj[t]);
+    for (int t = 0; t < kPouBatch; ++t)
+      if (q0 + t * stride < nitems) {
+        float re, im;
+        sample(sm[t], re, im);
+        const bool self = pou_self_mirror(ii[t], jj[t], hp);
+        a[((ii[t] + hp) & (P - 1)) * pitch + vc[t]] = make_float2(re, self ? 0.f : im);
+      }
   }
-  // column v = P/2: samples j = 0 (frequency -P/2) and their mirrors
-  for (int i = threadIdx.x; i < P; i += stride) {
-    PkSmp<VT> s0, s1;
-    issue_pair(s0, s1, i, 0);
-    store_pair(s0, s1, i, hp);
+  for (int q0 = threadIdx.x; q0 < nitems; q0 += kPouBatch * stride) {
+    PkSmp<VT> sm[kPouBatch];
+    int ii[kPouBatch], jj[kPouBatch], vc[kPouBatch];
+    bool mir[kPouBatch];
+#pragma unroll
+    for (int t = 0; t < kPouBatch; ++t) {
+      item(min(q0 + t * stride, nitems - 1), ii[t], jj[t], vc[t]);
+      float px, py, pz;
+      pou_kpos(g, ux, uy, uz, vx, vy, vz, ii[t], jj[t], px, py, pz);
+      mir[t] = q0 + t * stride < nitems && !pou_self_mirror(ii[t], jj[t], hp) &&
+               pou_mirror_explicit(ii[t], jj[t], P, px, py, pz);
+      if (mir[t]) {
+        pou_kpos(g, ux, uy, uz, vx, vy, vz, (P - ii[t]) & (P - 1), (P - jj[t]) & (P - 1), px, py, pz);
+        pk_issue<VT>(sm[t], blk, k, P, px, py, pz);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < kPouBatch; ++t)
+      if (mir[t]) {
+        float re, im;
+        sample(sm[t], re, im);
+        float2& h = a[((ii[t] + hp) & (P - 1)) * pitch + vc[t]];
+        h = make_float2(0.5f * (h.x + re), 0.5f * (h.y - im));
+      }
   }
   __syncthreads();
   // inverse FFTs of the P/2 + 1 columns (length P): Y at rows brev(m)
@@ -740,38 +774,91 @@ __device__ __forceinline__ void pou_adjoint_tail(float2* a, const float2* tw, fl
     __syncthreads();
   }
   fft_lines_dif4<-1, true>(a, hpitch, P, logP, 1, hpitch, tw, 1);
-  // every slice sample recomputed and contracted with its gradient
+  // every slice sample recomputed and contracted with its gradient.  The sum runs over one
+  // representative of each DFT-mirror pair (the columns j >= P/2, then column 0; on the
+  // self-mirrored columns j = P/2 and 0 the rows i > P/2 with i = 0, P/2): where the mirror is
+  // the reflection of an interior sample its term equals the representative's (the slice
+  // gradient G is Hermitian, the mirror sample conj S with the gradient -conj dS and the
+  // frequencies negated), so the representative counts twice; otherwise
+  // (pou_grad_mirror_explicit) the mirror is gathered in a second pass.
   const float ux = u[3 * n], uy = u[3 * n + 1], uz = u[3 * n + 2];
   const float vx = v[3 * n], vy = v[3 * n + 1], vz = v[3 * n + 2];
   const unsigned long long blk = reinterpret_cast<unsigned long long>(Fp) + (unsigned long long)b * k.block;
   const float inv = 1.f / float(P * P);
   float gu[3] = {0.f, 0.f, 0.f}, gv[3] = {0.f, 0.f, 0.f};
-  const int nq = P * P;
-  for (int q0 = threadIdx.x; q0 < nq; q0 += kPouBatch * NT) {
+  const int nh = P * hp, nitems = nh + P;
+  auto item = [&](int q, int& i, int& j) {
+    if (q < nh) {
+      int jj;
+      pou_ij_w(q, logH, i, jj);
+      j = hp + jj;
+    } else {
+      i = q - nh;
+      j = 0;
+    }
+  };
+  // weight of a representative's own term: 0 (counted through its mirror), 1 (self-mirrored,
+  // or its mirror gathered explicitly), 2 (the mirror's term equals its own)
+  auto weight = [&](int i, int j, float px, float py, float pz) {
+    const bool colself = j == 0 || j == hp;
+    if (colself && i > 0 && i < hp) return 0.f;
+    if (pou_self_mirror(i, j, hp)) return 1.f;
+    return pou_grad_mirror_explicit(i, j, P, px, py, pz) ? 1.f : 2.f;
+  };
+  auto contract = [&](const PkSmp<VT>& sm, int i, int j, float w) {
+    const float kx = float(j - hp), ky = float(i - hp);
+    // G at frequency (uu, vv) (natural order): stored for vv <= P/2, else conj G(-uu, -vv)
+    const int uu = (i + hp) & (P - 1), vv = (j + hp) & (P - 1);
+    const bool lo = vv <= hp;
+    const float2 gs = a[brev(lo ? uu : (P - uu) & (P - 1), logP) * hpitch + (lo ? vv : P - vv)];
+    const float gr = gs.x * inv * w, gi = (lo ? gs.y : -gs.y) * inv * w;
+    float cr[8], ci[8], re, im, dre[3], dim[3];
+    pk_corners<VT>(sm, cr, ci);
+    interp8<true>(cr, ci, sm.tx, sm.ty, sm.tz, re, im, dre, dim);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float d = gr * dre[c] + gi * dim[c];
+      gu[c] = fmaf(kx, d, gu[c]);
+      gv[c] = fmaf(ky, d, gv[c]);
+    }
+  };
+  for (int q0 = threadIdx.x; q0 < nitems; q0 += kPouBatch * NT) {
     PkSmp<VT> smp[kPouBatch];
-    pk_issue_batch<VT>(smp, blk, k, sg, q0, NT, ux, uy, uz, vx, vy, vz);
+    int ii[kPouBatch], jj[kPouBatch];
+    float ww[kPouBatch];
 #pragma unroll
     for (int t = 0; t < kPouBatch; ++t) {
-      const int q = q0 + t * NT;
-      if (q >= nq) continue;
-      int i, j;
-      pou_ij(q, logP, i, j);
-      const float kx = float(j - hp), ky = float(i - hp);
-      // G at frequency (uu, vv) (natural order): stored for vv <= P/2, else conj G(-uu, -vv)
-      const int uu = (i + hp) & (P - 1), vv = (j + hp) & (P - 1);
-      const bool lo = vv <= hp;
-      const float2 gs = a[brev(lo ? uu : (P - uu) & (P - 1), logP) * hpitch + (lo ? vv : P - vv)];
-      const float gr = gs.x * inv, gi = (lo ? gs.y : -gs.y) * inv;
-      float cr[8], ci[8], re, im, dre[3], dim[3];
-      pk_corners<VT>(smp[t], cr, ci);
-      interp8<true>(cr, ci, smp[t].tx, smp[t].ty, smp[t].tz, re, im, dre, dim);
+      item(min(q0 + t * NT, nitems - 1), ii[t], jj[t]);
+      float px, py, pz;
+      pou_kpos(sg, ux, uy, uz, vx, vy, vz, ii[t], jj[t], px, py, pz);
+      ww[t] = q0 + t * NT < nitems ? weight(ii[t], jj[t], px, py, pz) : 0.f;
+      pk_issue<VT>(smp[t], blk, k, P, px, py, pz);
+    }
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const float d = gr * dre[c] + gi * dim[c];
-        gu[c] = fmaf(kx, d, gu[c]);
-        gv[c] = fmaf(ky, d, gv[c]);
+    for (int t = 0; t < kPouBatch; ++t)
+      if (ww[t] != 0.f) contract(smp[t], ii[t], jj[t], ww[t]);
+  }
+  for (int q0 = threadIdx.x; q0 < nitems; q0 += kPouBatch * NT) {
+    PkSmp<VT> smp[kPouBatch];
+    int ii[kPouBatch], jj[kPouBatch];
+    bool mir[kPouBatch];
+#pragma unroll
+    for (int t = 0; t < kPouBatch; ++t) {
+      item(min(q0 + t * NT, nitems - 1), ii[t], jj[t]);
+      float px, py, pz;
+      pou_kpos(sg, ux, uy, uz, vx, vy, vz, ii[t], jj[t], px, py, pz);
+      mir[t] = q0 + t * NT < nitems && weight(ii[t], jj[t], px, py, pz) == 1.f &&
+               !pou_self_mirror(ii[t], jj[t], hp);
+      ii[t] = (P - ii[t]) & (P - 1);
+      jj[t] = (P - jj[t]) & (P - 1);
+      if (mir[t]) {
+        pou_kpos(sg, ux, uy, uz, vx, vy, vz, ii[t], jj[t], px, py, pz);
+        pk_issue<VT>(smp[t], blk, k, P, px, py, pz);
       }
     }
+#pragma unroll
+    for (int t = 0; t < kPouBatch; ++t)
+      if (mir[t]) contract(smp[t], ii[t], jj[t], 1.f);
   }
   float all[9] = {s3[0], s3[1], s3[2], gu[0], gu[1], gu[2], gv[0], gv[1], gv[2]};
   __syncthreads();
