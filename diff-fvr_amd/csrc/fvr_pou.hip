@@ -25,6 +25,7 @@
 //   fvr_pou_pose_geom(_bwd)  pose matrices -> detector axes and block offsets of the
 //                        lattice path, and its adjoint from the fused adjoint's sums.
 #include <cmath>
+#include <type_traits>
 
 #include "fvr_common.h"
 
@@ -951,6 +952,8 @@ constexpr int kSepChunks = 2;  // 256-float chunks of a wave's row window in the
 constexpr int kSepOrderPose = 1;  // launch option: workgroups in pose-major order
 
 __host__ __device__ inline bool sep_staged(int P, int W) { return P >= 64 && W % 4 == 0 && W <= 256 * kSepChunks; }
+// floats of one wave's staging buffer: its row window (256-float chunks) and kSepNX of zeros past it
+__host__ __device__ inline int sep_stage_floats(int W) { return 256 * ((W + 255) / 256) + kSepNX; }
 
 // entries of the segment table: thread (tid mod P, tid / P) segments, or up to one per wave
 __host__ __device__ inline int sep_max_segments(int P) {
@@ -959,7 +962,7 @@ __host__ __device__ inline int sep_max_segments(int P) {
 
 __host__ __device__ inline size_t sep_lds_bytes(int P, int W) {
   const int nseg = sep_max_segments(P);
-  const size_t stage = sep_staged(P, W) ? size_t(kPouSepWaves) * 256 * ((W + 255) / 256) * sizeof(float) : 0;
+  const size_t stage = sep_staged(P, W) ? size_t(kPouSepWaves) * sep_stage_floats(W) * sizeof(float) : 0;
   return size_t(P) * (P / 2 + 1) * sizeof(float2) + stage + (P / 2) * sizeof(float2) +
          9 * kPouMaxWaves * sizeof(float) + (5 * size_t(P) + 2 * nseg + 4) * sizeof(int);
 }
@@ -975,10 +978,9 @@ __global__ __launch_bounds__(kPouSepThreads, 4) void pou_render_bwd_sep_kernel(
   extern __shared__ float2 smem[];
   const int P = sg.P, logP = sg.logP, hp = P / 2, hpitch = hp + 1;
   const bool staged = sep_staged(P, W);
-  const int wch = (W + 255) / 256;                      // staged: 256-float chunks per wave buffer
   float2* a = smem;                                     // the packed gradient image: P x hpitch
   float* xstage = reinterpret_cast<float*>(a + P * hpitch);  // staged: per-wave row windows
-  float2* tw = reinterpret_cast<float2*>(xstage + (staged ? kPouSepWaves * 256 * wch : 0));  // P/2
+  float2* tw = reinterpret_cast<float2*>(xstage + (staged ? kPouSepWaves * sep_stage_floats(W) : 0));  // P/2
   float* red = reinterpret_cast<float*>(tw + hp);       // 9 * kPouMaxWaves
   int* xlo = reinterpret_cast<int*>(red + 9 * kPouMaxWaves);  // per texel column / row: first and
   int* xhi = xlo + P;                                   //   last pixel column / row touching it
@@ -1168,7 +1170,11 @@ __global__ __launch_bounds__(kPouSepThreads, 4) void pou_render_bwd_sep_kernel(
     if (ncg > 0 && whi >= 0) {  // wave-uniform: every lane stages its part of the window
       const int yb = __builtin_amdgcn_readfirstlane(segy[2 * seg]);
       const int ye = __builtin_amdgcn_readfirstlane(segy[2 * seg + 1]);
-      float* xw = xstage + wave * 256 * wch;
+      float* xw = xstage + wave * sep_stage_floats(W);
+      // the buffer past this wave's window reads 0 (lanes whose footprint is narrower than the
+      // unrolled count read there with weight 0)
+      for (int i = 256 * nch + lane; i < sep_stage_floats(W); i += 64) xw[i] = 0.f;
+      const int ib = colok ? xb - x0a : 0;  // the lane's first pixel in the window
       float4 pre[kSepChunks];
       int pfy = 0;
       float pwy = 0.f, pry = 0.f, pI0 = 0.f, pI1 = 0.f;
@@ -1190,39 +1196,44 @@ __global__ __launch_bounds__(kPouSepThreads, 4) void pou_render_bwd_sep_kernel(
           pI1 = img_at_row(pfy + 1);
         }
       };
-      if (yb <= ye) fetch(yb, -0x7fffffff, 0.f, 0.f);
-      const int qmax = 256 * nch - 1;
-      for (int y = yb; y <= ye; ++y) {
+      // NXT: pixels per lane, unrolled (the lane's weights past its footprint are 0); 0: the
+      // per-pixel path for footprints wider than kSepNX
+      auto run = [&](auto nxt) {
+        constexpr int NXT = decltype(nxt)::value;
+        if (yb <= ye) fetch(yb, -0x7fffffff, 0.f, 0.f);
+        for (int y = yb; y <= ye; ++y) {
 #pragma unroll
-        for (int c = 0; c < kSepChunks; ++c)
-          if (c < nch) *reinterpret_cast<float4*>(xw + 256 * c + 4 * lane) = pre[c];
-        const int fy = pfy;
-        const float wy = pwy, ry = pry, I0 = pI0, I1 = pI1;
-        if (y < ye) fetch(y + 1, fy, I0, I1);
-        // (the wave's LDS operations complete in order: the reads below see this row's window)
-        float A = 0.f, Bd = 0.f, Br = 0.f;
-        if (nxw <= kSepNX) {
+          for (int c = 0; c < kSepChunks; ++c)
+            if (c < nch) *reinterpret_cast<float4*>(xw + 256 * c + 4 * lane) = pre[c];
+          const int fy = pfy;
+          const float wy = pwy, ry = pry, I0 = pI0, I1 = pI1;
+          if (y < ye) fetch(y + 1, fy, I0, I1);
+          // (the wave's LDS operations complete in order: the reads below see this row's window)
+          float A = 0.f, Bd = 0.f, Br = 0.f;
+          if constexpr (NXT > 0) {
 #pragma unroll
-          for (int q = 0; q < kSepNX; ++q)
-            if (q < nxw) {
-              const int i = min(max(xb - x0a + q, 0), qmax);  // weight 0 past the lane's range
-              const float xv = xw[i];
+            for (int q = 0; q < NXT; ++q) {
+              const float xv = xw[ib + q];
               A = fmaf(cc[q], xv, A);
               Bd = fmaf(dd[q], xv, Bd);
               Br = fmaf(ee[q], xv, Br);
             }
-        } else {
-          for (int q = 0; q < nx; ++q) {
-            float c, d, e;
-            coef(xb + q, c, d, e);
-            const float xv = xw[xb - x0a + q];
-            A = fmaf(c, xv, A);
-            Bd = fmaf(d, xv, Bd);
-            Br = fmaf(e, xv, Br);
+          } else {
+            for (int q = 0; q < nx; ++q) {
+              float c, d, e;
+              coef(xb + q, c, d, e);
+              const float xv = xw[ib + q];
+              A = fmaf(c, xv, A);
+              Bd = fmaf(d, xv, Bd);
+              Br = fmaf(e, xv, Br);
+            }
           }
+          accumulate(fy, wy, ry, A, Bd, Br, I0, I1);
         }
-        accumulate(fy, wy, ry, A, Bd, Br, I0, I1);
-      }
+      };
+      if (nxw <= 8) run(std::integral_constant<int, 8>{});
+      else if (nxw <= kSepNX) run(std::integral_constant<int, kSepNX>{});
+      else run(std::integral_constant<int, 0>{});
     }
   } else if (active) {
     const int yb = segy[2 * seg], ye = segy[2 * seg + 1];
