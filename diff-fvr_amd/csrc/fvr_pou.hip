@@ -1133,12 +1133,15 @@ __global__ __launch_bounds__(kPouSepThreads, 4) void pou_render_bwd_sep_kernel(
     }
     e = d * r;
   };
-  float cc[kSepNX], dd[kSepNX], ee[kSepNX];
+  // the weights of the lane's first NXT pixels (0 past its footprint)
+  auto coefs = [&](auto nxt, float* cc, float* dd, float* ee) {
+    constexpr int NXT = decltype(nxt)::value;
 #pragma unroll
-  for (int q = 0; q < kSepNX; ++q) {
-    cc[q] = dd[q] = ee[q] = 0.f;
-    if (q < nx) coef(xb + q, cc[q], dd[q], ee[q]);
-  }
+    for (int q = 0; q < NXT; ++q) {
+      cc[q] = dd[q] = ee[q] = 0.f;
+      if (q < nx) coef(xb + q, cc[q], dd[q], ee[q]);
+    }
+  };
   const size_t K = size_t(H) * W;
   const float* Xn = X + size_t(n) * K;
   const float* img = imgs + size_t(bn) * P * P;
@@ -1175,40 +1178,43 @@ __global__ __launch_bounds__(kPouSepThreads, 4) void pou_render_bwd_sep_kernel(
       // unrolled count read there with weight 0)
       for (int i = 256 * nch + lane; i < sep_stage_floats(W); i += 64) xw[i] = 0.f;
       const int ib = colok ? xb - x0a : 0;  // the lane's first pixel in the window
-      float4 pre[kSepChunks];
-      int pfy = 0;
-      float pwy = 0.f, pry = 0.f, pI0 = 0.f, pI1 = 0.f;
-      // row y's window and position, and its image values (reused when fy repeats)
-      auto fetch = [&](int y, int lastf, float lI0, float lI1) {
+      // a fetched row: its window, position and slab-image values; two of them alternate so that
+      // a row's loads are consumed one row after they are issued, with no register copies
+      struct Row {
+        float4 win[kSepChunks];
+        int fy;
+        float wy, ry, I0, I1;
+      };
+      // row y's window and position, and its image values (those of `prev` when fy repeats)
+      auto fetch = [&](Row& r, int y, const Row& prev) {
         const float* rowp = Xn + size_t(y) * W;
 #pragma unroll
         for (int c = 0; c < kSepChunks; ++c)
           if (c < nch) {
             const int x = x0a + 256 * c + 4 * lane;  // W % 4 == 0: a group is all in or all out
-            pre[c] = x < W ? *reinterpret_cast<const float4*>(rowp + x) : make_float4(0.f, 0.f, 0.f, 0.f);
+            r.win[c] = x < W ? *reinterpret_cast<const float4*>(rowp + x) : make_float4(0.f, 0.f, 0.f, 0.f);
           }
-        rowpos(y, pfy, pwy, pry);
-        if (pfy == lastf) {
-          pI0 = lI0;
-          pI1 = lI1;
+        rowpos(y, r.fy, r.wy, r.ry);
+        if (r.fy == prev.fy) {
+          r.I0 = prev.I0;
+          r.I1 = prev.I1;
         } else {
-          pI0 = img_at_row(pfy);
-          pI1 = img_at_row(pfy + 1);
+          r.I0 = img_at_row(r.fy);
+          r.I1 = img_at_row(r.fy + 1);
         }
       };
       // NXT: pixels per lane, unrolled (the lane's weights past its footprint are 0); 0: the
       // per-pixel path for footprints wider than kSepNX
       auto run = [&](auto nxt) {
         constexpr int NXT = decltype(nxt)::value;
-        if (yb <= ye) fetch(yb, -0x7fffffff, 0.f, 0.f);
-        for (int y = yb; y <= ye; ++y) {
+        float cc[NXT > 0 ? NXT : 1], dd[NXT > 0 ? NXT : 1], ee[NXT > 0 ? NXT : 1];
+        if constexpr (NXT > 0) coefs(nxt, cc, dd, ee);
+        // the staged window of a fetched row, then its column sums and texel updates
+        auto process = [&](const Row& r) {
 #pragma unroll
           for (int c = 0; c < kSepChunks; ++c)
-            if (c < nch) *reinterpret_cast<float4*>(xw + 256 * c + 4 * lane) = pre[c];
-          const int fy = pfy;
-          const float wy = pwy, ry = pry, I0 = pI0, I1 = pI1;
-          if (y < ye) fetch(y + 1, fy, I0, I1);
-          // (the wave's LDS operations complete in order: the reads below see this row's window)
+            if (c < nch) *reinterpret_cast<float4*>(xw + 256 * c + 4 * lane) = r.win[c];
+          // (the wave's LDS operations complete in order: the reads below see this window)
           float A = 0.f, Bd = 0.f, Br = 0.f;
           if constexpr (NXT > 0) {
 #pragma unroll
@@ -1228,7 +1234,18 @@ __global__ __launch_bounds__(kPouSepThreads, 4) void pou_render_bwd_sep_kernel(
               Br = fmaf(e, xv, Br);
             }
           }
-          accumulate(fy, wy, ry, A, Bd, Br, I0, I1);
+          accumulate(r.fy, r.wy, r.ry, A, Bd, Br, r.I0, r.I1);
+        };
+        Row ra, rb;
+        rb.fy = -0x7fffffff;
+        rb.I0 = rb.I1 = 0.f;
+        if (yb <= ye) fetch(ra, yb, rb);
+        for (int y = yb; y <= ye; y += 2) {
+          if (y + 1 <= ye) fetch(rb, y + 1, ra);
+          process(ra);
+          if (y + 1 > ye) break;
+          if (y + 2 <= ye) fetch(ra, y + 2, rb);
+          process(rb);
         }
       };
       if (nxw <= 8) run(std::integral_constant<int, 8>{});
@@ -1239,6 +1256,8 @@ __global__ __launch_bounds__(kPouSepThreads, 4) void pou_render_bwd_sep_kernel(
     const int yb = segy[2 * seg], ye = segy[2 * seg + 1];
     int lastf = -0x7fffffff;
     float I0 = 0.f, I1 = 0.f;
+    float cc[kSepNX], dd[kSepNX], ee[kSepNX];
+    coefs(std::integral_constant<int, kSepNX>{}, cc, dd, ee);
     for (int y = yb; y <= ye; ++y) {
       int fy;
       float wy, ry;
