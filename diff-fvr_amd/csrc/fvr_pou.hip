@@ -1460,42 +1460,48 @@ __global__ __launch_bounds__(256) void pou_pose_geom_kernel(const float* __restr
 }
 
 // d(sum go * render)/dM from the fused adjoint's per-(block, pose) sums part[b][n][9]
-// (d/d(d_b, cu, cv), d/du_xyz, d/dv_xyz), summed over the blocks in fixed order in float64:
+// (d/d(d_b, cu, cv), d/du_xyz, d/dv_xyz), summed over the blocks in float64:
 //   g_src = -sum_b (g_db w + g_cu u + g_cv v),  g_w = sum_b g_db (c_b - src),
 //   g_u = flip(sum_b d/du_xyz) + sum_b g_cu (c_b - src),  g_v likewise,
 //   dM[i][j] = g_src[i] src0[j] + g_u[i] u0[j] + g_v[i] v0[j] + g_w[i] n0[j],  dM[i][3] = g_src[i],
-// and row 3 zero.  One thread per pose.
+// and row 3 zero.  One wave per pose: lane l takes the blocks l, l + 64, ... in order, then a
+// fixed XOR butterfly sums the lanes (the same order every call: deterministic).
 __global__ __launch_bounds__(64) void pou_pose_geom_bwd_kernel(const float* __restrict__ M, int N,
                                                                const float* __restrict__ lat,
                                                                const float* __restrict__ centers, int B,
                                                                const float* __restrict__ part,
                                                                float* __restrict__ gM) {
-  const int n = blockIdx.x * 64 + threadIdx.x;
-  if (n >= N) return;
+  const int n = blockIdx.x, lane = threadIdx.x;
   const PoseGeom g = pose_geom(M, lat, n);
-  double gs[3] = {0.0, 0.0, 0.0}, gw[3] = {0.0, 0.0, 0.0}, gu[3] = {0.0, 0.0, 0.0}, gv[3] = {0.0, 0.0, 0.0};
-  for (int b = 0; b < B; ++b) {
+  double acc[12];  // g_src, g_w, g_u, g_v (3 each)
+#pragma unroll
+  for (int k = 0; k < 12; ++k) acc[k] = 0.0;
+  for (int b = lane; b < B; b += 64) {
     const float* p = part + (size_t(b) * N + n) * 9;
     const double gdb = p[0], gcu = p[1], gcv = p[2];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       const double d = double(centers[3 * b + i]) - double(g.src[i]);
-      gs[i] -= gdb * g.w[i] + gcu * g.u[i] + gcv * g.v[i];
-      gw[i] += gdb * d;
-      gu[i] += gcu * d + double(p[3 + 2 - i]);
-      gv[i] += gcv * d + double(p[6 + 2 - i]);
+      acc[i] -= gdb * g.w[i] + gcu * g.u[i] + gcv * g.v[i];
+      acc[3 + i] += gdb * d;
+      acc[6 + i] += gcu * d + double(p[3 + 2 - i]);
+      acc[9 + i] += gcv * d + double(p[6 + 2 - i]);
     }
   }
-  float* o = gM + size_t(n) * 16;
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
+  for (int k = 0; k < 12; ++k)
 #pragma unroll
-    for (int j = 0; j < 3; ++j)
-      o[4 * i + j] = float(gs[i] * lat[j] + gu[i] * lat[3 + j] + gv[i] * lat[6 + j] + gw[i] * lat[9 + j]);
-    o[4 * i + 3] = float(gs[i]);
-  }
+    for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o);
+  float val = 0.f;  // entry `lane` of dM (row 3: 0)
 #pragma unroll
-  for (int j = 0; j < 4; ++j) o[12 + j] = 0.f;
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (lane == 4 * i + j)
+        val = j == 3 ? float(acc[i])
+                     : float(acc[i] * lat[j] + acc[6 + i] * lat[3 + j] + acc[9 + i] * lat[6 + j] +
+                             acc[3 + i] * lat[9 + j]);
+  if (lane < 16) gM[size_t(n) * 16 + lane] = val;
 }
 
 // planar (B, 2, P, P, P) -> gather layout; one thread per stored element (margins written as 0)
@@ -1762,7 +1768,7 @@ int fvr_pou_pose_geom(const float* M, int32_t N, const float* lat, const float* 
 int fvr_pou_pose_geom_bwd(const float* M, int32_t N, const float* lat, const float* centers, int32_t B,
                           const float* part, float* gM, void* stream) {
   FVR_CHECK_ARG(M && lat && centers && part && gM && N > 0 && B > 0, "fvr_pou_pose_geom_bwd: bad arguments");
-  hipLaunchKernelGGL(pou_pose_geom_bwd_kernel, dim3(unsigned((N + 63) / 64)), dim3(64), 0,
+  hipLaunchKernelGGL(pou_pose_geom_bwd_kernel, dim3(unsigned(N)), dim3(64), 0,
                      reinterpret_cast<hipStream_t>(stream), M, N, lat, centers, B, part, gM);
   FVR_CHECK_LAUNCH("pou_pose_geom_bwd_kernel");
   return FVR_OK;
