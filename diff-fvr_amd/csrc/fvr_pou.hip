@@ -308,24 +308,34 @@ __device__ void fft_lines_dif4(float2* a, int nlines, int L, int logL, int lpitc
   }
 }
 
-// trilinear value (and gradient) from the 8 corners (c = x + 2 y + 4 z) and the fractions
+// trilinear value (and gradient) from the 8 corners (c = x + 2 y + 4 z) and the fractions:
+// lerps along x (the difference d = high - low kept), then y, then z; the x-gradient is the
+// bilinear (y, z) interpolation of the four x-differences, the y-gradient the z-lerp of the two
+// y-differences, the z-gradient the last difference (the exact derivatives of the same trilinear
+// interpolant, sharing its partial results)
 template <bool GRAD>
 __device__ __forceinline__ void interp8(const float (&cr)[8], const float (&ci)[8], float tx, float ty, float tz,
                                         float& re, float& im, float* gre, float* gim) {
-  const float ox = 1.f - tx, oy = 1.f - ty, oz = 1.f - tz;
-  // corner c = x + 2 y + 4 z
-  re = ((cr[0] * ox + cr[1] * tx) * oy + (cr[2] * ox + cr[3] * tx) * ty) * oz +
-       ((cr[4] * ox + cr[5] * tx) * oy + (cr[6] * ox + cr[7] * tx) * ty) * tz;
-  im = ((ci[0] * ox + ci[1] * tx) * oy + (ci[2] * ox + ci[3] * tx) * ty) * oz +
-       ((ci[4] * ox + ci[5] * tx) * oy + (ci[6] * ox + ci[7] * tx) * ty) * tz;
-  if (GRAD) {
-    gre[0] = ((cr[1] - cr[0]) * oy + (cr[3] - cr[2]) * ty) * oz + ((cr[5] - cr[4]) * oy + (cr[7] - cr[6]) * ty) * tz;
-    gre[1] = ((cr[2] - cr[0]) * ox + (cr[3] - cr[1]) * tx) * oz + ((cr[6] - cr[4]) * ox + (cr[7] - cr[5]) * tx) * tz;
-    gre[2] = ((cr[4] - cr[0]) * ox + (cr[5] - cr[1]) * tx) * oy + ((cr[6] - cr[2]) * ox + (cr[7] - cr[3]) * tx) * ty;
-    gim[0] = ((ci[1] - ci[0]) * oy + (ci[3] - ci[2]) * ty) * oz + ((ci[5] - ci[4]) * oy + (ci[7] - ci[6]) * ty) * tz;
-    gim[1] = ((ci[2] - ci[0]) * ox + (ci[3] - ci[1]) * tx) * oz + ((ci[6] - ci[4]) * ox + (ci[7] - ci[5]) * tx) * tz;
-    gim[2] = ((ci[4] - ci[0]) * ox + (ci[5] - ci[1]) * tx) * oy + ((ci[6] - ci[2]) * ox + (ci[7] - ci[3]) * tx) * ty;
-  }
+  auto comp = [&](const float (&c)[8], float& val, float* g) {
+    float d[4], e[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {  // k = y + 2 z
+      d[k] = c[2 * k + 1] - c[2 * k];
+      e[k] = fmaf(tx, d[k], c[2 * k]);
+    }
+    const float y0 = e[1] - e[0], y1 = e[3] - e[2];
+    const float f0 = fmaf(ty, y0, e[0]), f1 = fmaf(ty, y1, e[2]);
+    const float z = f1 - f0;
+    val = fmaf(tz, z, f0);
+    if (GRAD) {
+      const float x0 = fmaf(ty, d[1] - d[0], d[0]), x1 = fmaf(ty, d[3] - d[2], d[2]);
+      g[0] = fmaf(tz, x1 - x0, x0);
+      g[1] = fmaf(tz, y1 - y0, y0);
+      g[2] = z;
+    }
+  };
+  comp(cr, re, gre);
+  comp(ci, im, gim);
 }
 
 struct PouSlice {
