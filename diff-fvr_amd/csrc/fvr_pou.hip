@@ -535,6 +535,23 @@ __device__ __forceinline__ float pou_img_pos(float db, float r, float c, float i
   return fmaf(fmaf(db, r, -c), inv_ds, c_idx);
 }
 
+// global-memory view of a uniform base pointer: loads at base + a 32-bit byte offset use the
+// scalar base register and one offset register
+typedef const char __attribute__((address_space(1)))* pou_gcptr;
+__device__ __forceinline__ pou_gcptr pou_global(const void* p) {
+  return reinterpret_cast<pou_gcptr>(reinterpret_cast<uintptr_t>(p));
+}
+__device__ __forceinline__ float pou_ld(pou_gcptr base, uint32_t byte_off) {
+  return *reinterpret_cast<const float __attribute__((address_space(1)))*>(base + byte_off);
+}
+
+// the bilinear lerps of the warp with their roundings fixed (the x-lerps, then the y-lerp), so that
+// the per-pixel and the tiled forward kernels give the same value bit for bit
+__device__ __forceinline__ float pou_lerp(float lo, float hi, float t) { return fmaf(hi, t, lo * (1.f - t)); }
+__device__ __forceinline__ float pou_bilerp(float v00, float v01, float v10, float v11, float tx, float ty) {
+  return pou_lerp(pou_lerp(v00, v01, tx), pou_lerp(v10, v11, tx), ty);
+}
+
 __device__ __forceinline__ float img_at(const float* img, int P, int x, int y) {
   return ((unsigned)x < (unsigned)P && (unsigned)y < (unsigned)P) ? img[y * P + x] : 0.f;
 }
@@ -555,6 +572,8 @@ __global__ __launch_bounds__(256) void pou_warp_fwd_kernel(const float* __restri
   const size_t ro = size_t(n) * g.ray_stride + q;
   const float r_x = rx[ro], r_y = ry[ro];
   const size_t pp = size_t(g.P) * g.P;
+  const unsigned P = unsigned(g.P), mask = P - 1u;
+  const bool pow2 = (P & mask) == 0u;
   float acc = 0.f;
   for (int b = 0; b < g.B; ++b) {
     const int bn = b * g.N + n;
@@ -563,12 +582,135 @@ __global__ __launch_bounds__(256) void pou_warp_fwd_kernel(const float* __restri
     const float x0f = floorf(px), y0f = floorf(py);
     const float tx = px - x0f, ty = py - y0f;
     const int x0 = int(x0f), y0 = int(y0f);
-    const float* img = imgs + size_t(bn) * pp;
-    const float v00 = img_at(img, g.P, x0, y0), v01 = img_at(img, g.P, x0 + 1, y0);
-    const float v10 = img_at(img, g.P, x0, y0 + 1), v11 = img_at(img, g.P, x0 + 1, y0 + 1);
-    acc += (v00 * (1.f - tx) + v01 * tx) * (1.f - ty) + (v10 * (1.f - tx) + v11 * tx) * ty;
+    float v00, v01, v10, v11;
+    if (pow2) {
+      // every corner address wrapped into the image (loads without a branch), the corners
+      // outside it then counting 0: the same values as the guarded loads
+      const unsigned ux = unsigned(x0), uy = unsigned(y0);
+      const pou_gcptr img = pou_global(imgs + size_t(bn) * pp);
+      const uint32_t ra = (uy & mask) * P, rb = ((uy + 1u) & mask) * P;
+      const uint32_t xa = ux & mask, xb = (ux + 1u) & mask;
+      const float f00 = pou_ld(img, (ra + xa) * 4u), f01 = pou_ld(img, (ra + xb) * 4u);
+      const float f10 = pou_ld(img, (rb + xa) * 4u), f11 = pou_ld(img, (rb + xb) * 4u);
+      const bool ix0 = ux < P, ix1 = ux + 1u < P, iy0 = uy < P, iy1 = uy + 1u < P;
+      v00 = ix0 && iy0 ? f00 : 0.f;
+      v01 = ix1 && iy0 ? f01 : 0.f;
+      v10 = ix0 && iy1 ? f10 : 0.f;
+      v11 = ix1 && iy1 ? f11 : 0.f;
+    } else {
+      const float* img = imgs + size_t(bn) * pp;
+      v00 = img_at(img, g.P, x0, y0);
+      v01 = img_at(img, g.P, x0 + 1, y0);
+      v10 = img_at(img, g.P, x0, y0 + 1);
+      v11 = img_at(img, g.P, x0 + 1, y0 + 1);
+    }
+    acc += pou_bilerp(v00, v01, v10, v11, tx, ty);
   }
   out[size_t(n) * g.K + q] = acc * ic[ro];
+}
+
+// Separable forward warp (rays shared by every pose, r_x per detector column and r_y per
+// detector row, both monotone): one workgroup per (pose, 32 x 32 detector tile).  Per block the
+// tile's pixels fall in a patch of the slab image spanning its first to its last column's / row's
+// texels (at most 32 s + 2 texels a side for s texels per pixel); the patch is staged in LDS
+// (zeros outside the image), the x-lerps are formed once per (patch row, tile column), then each
+// pixel takes its y-lerp.  The arithmetic per pixel is that of pou_warp_fwd_kernel (the same
+// lerps in the same order, out-of-image corners 0, blocks summed in order), so the projections are
+// the same bit for bit; a (tile, block) whose patch exceeds kWarpPatch texels a side, or whose
+// geometry is not finite, takes the per-pixel gather instead.
+constexpr int kWarpTile = 32;
+constexpr int kWarpPatch = 16;
+
+__global__ __launch_bounds__(256) void pou_warp_fwd_sep_kernel(const float* __restrict__ imgs,
+                                                               const float* __restrict__ d_b,
+                                                               const float* __restrict__ cu,
+                                                               const float* __restrict__ cv,
+                                                               const float* __restrict__ rx_col,
+                                                               const float* __restrict__ ry_row,
+                                                               const float* __restrict__ ic, PouWarp g, int H, int W,
+                                                               int tiles_x, int ntile, int per_xcd,
+                                                               float* __restrict__ out) {
+  __shared__ float patch[kWarpPatch][kWarpPatch + 1];
+  __shared__ float arow[kWarpPatch][kWarpTile + 1];
+  const int t = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+  if (t >= g.N * ntile) return;  // whole workgroup
+  const int n = t / ntile, tile = t - n * ntile;
+  const int tyi = tile / tiles_x, txi = tile - tyi * tiles_x;
+  const int xs = txi * kWarpTile, ys = tyi * kWarpTile;
+  const int xe = min(xs + kWarpTile, W) - 1, ye = min(ys + kWarpTile, H) - 1;  // the tile's last pixel
+  const int tid = threadIdx.x, cx = tid & (kWarpTile - 1), rg = tid >> 5;     // column, row group
+  const int x = xs + cx;
+  const bool xok = x < W;
+  const float rxv = rx_col[xok ? x : xe];
+  const float rxa = rx_col[xs], rxb = rx_col[xe], rya = ry_row[ys], ryb = ry_row[ye];
+  float ryv[4];
+  bool yok[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int y = ys + rg + 8 * k;
+    yok[k] = y < H;
+    ryv[k] = ry_row[yok[k] ? y : ye];
+  }
+  const unsigned P = unsigned(g.P);
+  const size_t pp = size_t(g.P) * g.P;
+  const float lim = float(kWarpPatch - 1);
+  float acc[4] = {0.f, 0.f, 0.f, 0.f};
+  for (int b = 0; b < g.B; ++b) {
+    const int bn = b * g.N + n;
+    const float db = d_b[bn], c_u = cu[bn], c_v = cv[bn];
+    const float px = pou_img_pos(db, rxv, c_u, g.inv_ds, g.c_idx);
+    const float fxf = floorf(px), wx = px - fxf;
+    float fyf[4], wy[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const float py = pou_img_pos(db, ryv[k], c_v, g.inv_ds, g.c_idx);
+      fyf[k] = floorf(py);
+      wy[k] = py - fyf[k];
+    }
+    // the tile's texel columns [flo, fhi] and rows [glo, ghi] (monotone maps: the end pixels)
+    const float pa = pou_img_pos(db, rxa, c_u, g.inv_ds, g.c_idx), pb = pou_img_pos(db, rxb, c_u, g.inv_ds, g.c_idx);
+    const float qa = pou_img_pos(db, rya, c_v, g.inv_ds, g.c_idx), qb = pou_img_pos(db, ryb, c_v, g.inv_ds, g.c_idx);
+    const float flo = floorf(fminf(pa, pb)), fhi = floorf(fmaxf(pa, pb)) + 1.f;
+    const float glo = floorf(fminf(qa, qb)), ghi = floorf(fmaxf(qa, qb)) + 1.f;
+    const bool fit = fhi - flo <= lim && ghi - glo <= lim;  // false for NaN or huge positions
+    if (fit) {
+      // a patch outside the image adds exact zeros: skip it
+      if (fhi < 0.f || flo > float(P - 1) || ghi < 0.f || glo > float(P - 1)) continue;
+      const int tx0 = int(flo), ty0 = int(glo);
+      const float* img = imgs + size_t(bn) * pp;
+      {
+        const int pr = tid >> 4, pc = tid & (kWarpPatch - 1);  // 256 threads: one texel each
+        const int ix = tx0 + pc, iy = ty0 + pr;
+        patch[pr][pc] = (unsigned(ix) < P && unsigned(iy) < P) ? img[size_t(iy) * P + ix] : 0.f;
+      }
+      __syncthreads();
+      const int c0 = int(fxf) - tx0;  // in [0, fhi - flo - 1]
+#pragma unroll
+      for (int r = rg; r < kWarpPatch; r += 8) arow[r][cx] = pou_lerp(patch[r][c0], patch[r][c0 + 1], wx);
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int r0 = int(fyf[k]) - ty0;  // in [0, ghi - glo - 1]
+        acc[k] += pou_lerp(arow[r0][cx], arow[r0 + 1][cx], wy[k]);
+      }
+    } else {
+      const float* img = imgs + size_t(bn) * pp;
+      const int x0 = int(fxf);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int y0 = int(fyf[k]);
+        const float v00 = img_at(img, g.P, x0, y0), v01 = img_at(img, g.P, x0 + 1, y0);
+        const float v10 = img_at(img, g.P, x0, y0 + 1), v11 = img_at(img, g.P, x0 + 1, y0 + 1);
+        acc[k] += pou_bilerp(v00, v01, v10, v11, wx, wy[k]);
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (xok && yok[k]) {
+      const size_t q = size_t(ys + rg + 8 * k) * W + x;
+      out[size_t(n) * g.K + q] = acc[k] * ic[q];
+    }
 }
 
 // fixed-order workgroup sum of NV per-thread values: an XOR butterfly inside every wave
@@ -604,16 +746,6 @@ constexpr int kScanBatch = FVR_POU_SCAN_BATCH;  // detector pixels in flight per
 // the integer two's complement in the bits above the constant's own bit pattern
 __device__ __forceinline__ unsigned long long fix64(float x) {
   return (unsigned long long)__double_as_longlong(double(x) + 6755399441055744.0) - 0x4338000000000000ull;
-}
-
-// global-memory view of a uniform base pointer: loads at base + a 32-bit byte offset use the
-// scalar base register and one offset register
-typedef const char __attribute__((address_space(1)))* pou_gcptr;
-__device__ __forceinline__ pou_gcptr pou_global(const void* p) {
-  return reinterpret_cast<pou_gcptr>(reinterpret_cast<uintptr_t>(p));
-}
-__device__ __forceinline__ float pou_ld(pou_gcptr base, uint32_t byte_off) {
-  return *reinterpret_cast<const float __attribute__((address_space(1)))*>(base + byte_off);
 }
 
 // POW2: P is a power of two, so a corner's column / row is wrapped into the image with a mask
@@ -1359,7 +1491,7 @@ __global__ __launch_bounds__(256) void pou_ray_grads_kernel(const float* __restr
     const float* img = imgs + size_t(bn) * pp;
     const float v00 = img_at(img, g.P, x0, y0), v01 = img_at(img, g.P, x0 + 1, y0);
     const float v10 = img_at(img, g.P, x0, y0 + 1), v11 = img_at(img, g.P, x0 + 1, y0 + 1);
-    acc += (v00 * ox + v01 * tx) * oy + (v10 * ox + v11 * tx) * ty;
+    acc += pou_bilerp(v00, v01, v10, v11, tx, ty);
     sx = fmaf(((v01 - v00) * oy + (v11 - v10) * ty) * g.inv_ds, db, sx);
     sy = fmaf(((v10 - v00) * ox + (v11 - v01) * tx) * g.inv_ds, db, sy);
   }
@@ -1632,6 +1764,23 @@ int fvr_pou_warp_fwd(const float* imgs, int32_t B, int32_t N, int32_t P, const f
   hipLaunchKernelGGL(pou_warp_fwd_kernel, dim3(per_xcd * 8), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
                      imgs, d_b, cu, cv, rx, ry, ic, g, qtiles, per_xcd, out);
   FVR_CHECK_LAUNCH("pou_warp_fwd_kernel");
+  return FVR_OK;
+}
+
+int fvr_pou_warp_fwd_sep(const float* imgs, int32_t B, int32_t N, int32_t P, const float* d_b, const float* cu,
+                         const float* cv, const float* rx_col, const float* ry_row, const float* ic, int32_t H,
+                         int32_t W, float inv_ds, float c_idx, float* out, void* stream) {
+  FVR_CHECK_ARG(imgs && d_b && cu && cv && rx_col && ry_row && ic && out, "fvr_pou_warp_fwd_sep: null pointer");
+  FVR_CHECK_ARG(H > 0 && W > 0 && int64_t(H) * W < (int64_t(1) << 31), "fvr_pou_warp_fwd_sep: bad detector");
+  PouWarp g;
+  if (warp_geom(g, B, N, P, H * W, 0, inv_ds, c_idx) != FVR_OK) return FVR_ERR_INVALID;
+  const int tiles_x = (W + kWarpTile - 1) / kWarpTile, ntile = tiles_x * ((H + kWarpTile - 1) / kWarpTile);
+  const int64_t total = int64_t(N) * ntile;
+  FVR_CHECK_ARG(total < (int64_t(1) << 30), "fvr_pou_warp_fwd_sep: too many tiles");
+  const int per_xcd = int((total + 7) / 8);
+  hipLaunchKernelGGL(pou_warp_fwd_sep_kernel, dim3(per_xcd * 8), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
+                     imgs, d_b, cu, cv, rx_col, ry_row, ic, g, H, W, tiles_x, ntile, per_xcd, out);
+  FVR_CHECK_LAUNCH("pou_warp_fwd_sep_kernel");
   return FVR_OK;
 }
 

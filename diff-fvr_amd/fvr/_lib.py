@@ -105,6 +105,7 @@ _SIGNATURES = {
     "fvr_pou_pack_blocks": ([_U64, _I, _I, _I, _U64, _P], _I),
     "fvr_pou_slice_ifft": ([_U64, _I, _I, _I, _P, _P, _I, _P, _P], _I),
     "fvr_pou_warp_fwd": ([_P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _I, _I, _F, _F, _P, _P], _I),
+    "fvr_pou_warp_fwd_sep": ([_P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _I, _I, _F, _F, _P, _P], _I),
     "fvr_pou_gscale": ([_P, _P, _I, _I, _I, _P, _P], _I),
     "fvr_pou_scale_of_max": ([_F], _F),
     "fvr_pou_warp_bwd_workspace": ([_I, _I, _I], ctypes.c_int64),

@@ -168,9 +168,9 @@ def sample_isocenter_parity_offsets(vox_spacing) -> torch.Tensor:
 # fused autograd operator over the PoU kernels
 # ---------------------------------------------------------------------------
 def _separable_rays(model, rx: torch.Tensor, ry: torch.Tensor, H: int, W: int):
-    """(r_x per detector column (W,), r_y per detector row (H,)) when the shared ray arrays
-    are exactly separable (every row of r_x equal, every column of r_y equal: a zero-skew
-    pinhole lattice, ``make_pinhole_lattice``), else None.  Cached on the model for the ray
+    """(r_x per detector column (W,), r_y per detector row (H,), both monotone) when the shared
+    ray arrays are exactly separable (every row of r_x equal, every column of r_y equal: a
+    zero-skew pinhole lattice, ``make_pinhole_lattice``), else None.  Cached on the model for the ray
     tensors last seen (object and version), so the check's host synchronisation happens once
     per lattice."""
     key = (id(rx), rx._version, id(ry), ry._version, H, W)
@@ -181,7 +181,13 @@ def _separable_rays(model, rx: torch.Tensor, ry: torch.Tensor, H: int, W: int):
     if rx.dim() == 2 and tuple(rx.shape) == (H, W) and tuple(ry.shape) == (H, W):
         if bool((rx == rx[:1]).all()) and bool((ry == ry[:, :1]).all()):
             dev = model.F_blk_ri.device
-            res = (_f32(rx[0], dev).contiguous(), _f32(ry[:, 0], dev).contiguous())
+            rc, rr = _f32(rx[0], dev).contiguous(), _f32(ry[:, 0], dev).contiguous()
+
+            def monotone(a):
+                d = a[1:] - a[:-1]
+                return bool((d >= 0).all()) or bool((d <= 0).all())
+
+            res = (rc, rr, monotone(rc) and monotone(rr))
     model._sep_rays = (key, weakref.ref(rx), weakref.ref(ry), res)
     return res
 
@@ -206,12 +212,19 @@ def _pou_forward(model, u, v, db, c_u, c_v, rx, ry, ic, out_h, out_w):
     _lib.check(L.fvr_pou_slice_ifft(Fp.data_ptr(), _VOL_DTYPES[model.F_blk_ri.dtype], B, P, _lib.ptr(u), _lib.ptr(v),
                                     N, _lib.ptr(imgs), st), "fvr_pou_slice_ifft")
     out = torch.empty((N, out_h, out_w), device=dev, dtype=torch.float32)
-    _lib.check(L.fvr_pou_warp_fwd(_lib.ptr(imgs), B, N, P, _lib.ptr(db), _lib.ptr(c_u), _lib.ptr(c_v), _lib.ptr(rxf),
-                                  _lib.ptr(ryf), _lib.ptr(icf), per_pose, K, 1.0 / model.ds_block, float(P // 2),
-                                  _lib.ptr(out), st), "fvr_pou_warp_fwd")
+    sr = _separable_rays(model, rx, ry, out_h, out_w) if not per_pose else None
+    inv_ds, c_idx = 1.0 / model.ds_block, float(P // 2)
+    if sr is not None and sr[2] and model.separable_warp:
+        _lib.check(L.fvr_pou_warp_fwd_sep(_lib.ptr(imgs), B, N, P, _lib.ptr(db), _lib.ptr(c_u), _lib.ptr(c_v),
+                                          _lib.ptr(sr[0]), _lib.ptr(sr[1]), _lib.ptr(icf), out_h, out_w, inv_ds, c_idx,
+                                          _lib.ptr(out), st), "fvr_pou_warp_fwd_sep")
+    else:
+        _lib.check(L.fvr_pou_warp_fwd(_lib.ptr(imgs), B, N, P, _lib.ptr(db), _lib.ptr(c_u), _lib.ptr(c_v),
+                                      _lib.ptr(rxf), _lib.ptr(ryf), _lib.ptr(icf), per_pose, K, inv_ds, c_idx,
+                                      _lib.ptr(out), st), "fvr_pou_warp_fwd")
     sep = None
-    if not per_pose and model.separable_adjoint and L.fvr_pou_sep_ok(P, out_h, out_w):
-        sep = _separable_rays(model, rx, ry, out_h, out_w)
+    if sr is not None and model.separable_adjoint and L.fvr_pou_sep_ok(P, out_h, out_w):
+        sep = sr[:2]
     s = _PouState()
     s.u, s.v, s.db, s.cu, s.cv, s.rays, s.per_pose, s.K, s.H, s.W, s.sep, s.imgs = (
         u, v, db, c_u, c_v, (rxf, ryf, icf), per_pose, K, out_h, out_w, sep, imgs)
@@ -464,6 +477,9 @@ class PerspectiveSlabFVR(nn.Module):
     separable_adjoint = True
     # workgroup order of the gather form: 0 block-major, 1 pose-major
     sep_adjoint_order = 0
+    # separable monotone shared rays take the tiled forward warp (the same projections bit for
+    # bit); False selects the per-pixel warp, for cross-checks
+    separable_warp = True
 
     def fused_ok(self) -> bool:
         return bool(self.fused_kernels) and bool(_lib.lib().fvr_pou_lds_ok(self.P))

@@ -25,6 +25,8 @@
  *   fvr_pou_warp_fwd/bwd  triton_slice.py:787-808 pou_pinhole_warp_sum (forward
  *                       K3 :349-407 and the imgs/d_b/cu/cv gradients of K4
  *                       :410-496, :548-591).
+ *   fvr_pou_warp_fwd_sep  the same forward (K3) for the separable monotone rays of
+ *                       a zero-skew pinhole lattice, tiled through LDS.
  *   fvr_pou_slice_ifft  cone_beam_fvr.py:480-505 _slice_and_ifft (slice +
  *                       centred inverse 2D FFT, fused in LDS).
  *   fvr_pou_render_bwd  the adjoint of cone_beam_fvr.py:547-573
@@ -388,6 +390,13 @@ int fvr_pou_warp_fwd(const float* imgs, int32_t B, int32_t N, int32_t P, const f
                      const float* cu, const float* cv, const float* rx, const float* ry,
                      const float* ic, int32_t per_pose_rays, int32_t K, float inv_ds, float c_idx,
                      float* out, void* stream);
+/* fvr_pou_warp_fwd for separable monotone rays shared by every pose (r_x of pixel (y, x) is
+ * rx_col[x], r_y is ry_row[y], each monotone; K = H * W): per (pose, 32 x 32 tile) the slab-image
+ * patch each block contributes is staged in LDS; the same projections bit for bit. */
+int fvr_pou_warp_fwd_sep(const float* imgs, int32_t B, int32_t N, int32_t P, const float* d_b,
+                         const float* cu, const float* cv, const float* rx_col, const float* ry_row,
+                         const float* ic, int32_t H, int32_t W, float inv_ds, float c_idx, float* out,
+                         void* stream);
 /* scale[n] = fvr_pou_scale_of_max(max_q |go[n][q] ic[q]|): the fixed-point scale of the
  * deterministic image-gradient scatter of fvr_pou_warp_bwd / fvr_pou_render_bwd. */
 int fvr_pou_gscale(const float* go, const float* ic, int32_t N, int32_t K, int32_t per_pose_rays,

@@ -467,3 +467,26 @@ def test_fused_pinhole_geometry_matches_the_torch_chain(model):
     g, gr = M.grad.double().cpu().numpy(), M2.grad.double().cpu().numpy()
     assert np.max(np.abs(g - gr)) <= 1e-5 * np.abs(gr).max(), np.abs(g - gr).max()
     assert np.all(g[:, 3, :] == 0.0)
+
+
+@pytest.mark.parametrize("block_res,H,W", [(8, 20, 22), (16, 40, 36), (64, 96, 80), (64, 256, 256)])
+def test_tiled_separable_warp_equals_the_per_pixel_warp(block_res, H, W):
+    """A zero-skew pinhole lattice takes the tiled forward warp (fvr_pou_warp_fwd_sep: per
+    (pose, 32 x 32 tile) the slab-image patch of each block staged in LDS); its projections
+    equal the per-pixel warp's (fvr_pou_warp_fwd) bit for bit, ragged tiles included."""
+    from fvr import PerspectiveSlabFVR
+    m = PerspectiveSlabFVR.from_volume(_vol(), vox_mm=2.0, grid_blocks=(2, 2, 2), block_res=block_res)
+    f = 300.0 * H / 20.0
+    k_inv = torch.tensor([[1.0 / f, 0.0, -W / (2 * f)], [0.0, 1.0 / f, -H / (2 * f)], [0.0, 0.0, 1.0]])
+    lat = [t.to(DEV) for t in PerspectiveSlabFVR.make_pinhole_lattice(k_inv, 600.0, H, W)]
+    M = torch.from_numpy(MATS).float().to(DEV)
+    with torch.no_grad():
+        tiled = m.forward_pinhole(M, lat)
+        assert m._sep_rays[3] is not None and m._sep_rays[3][2]
+        m.separable_warp = False
+        try:
+            plain = m.forward_pinhole(M, lat)
+        finally:
+            m.separable_warp = True
+    assert float(plain.abs().max()) > 0
+    assert torch.equal(tiled, plain)
