@@ -447,7 +447,11 @@ __global__ __launch_bounds__(kPouHalfThreads) void pou_slice_ifft_kernel(const c
     pk_corners<VT>(sm, cr, ci);
     interp8<false>(cr, ci, sm.tx, sm.ty, sm.tz, re, im, nullptr, nullptr);
   };
-  for (int q0 = threadIdx.x; q0 < nitems; q0 += kPouBatch * stride) {
+  // (bit kPouBatch * iteration + t of mirbits: item t of the thread's iteration needs its mirror;
+  // at most 5 iterations of 4 items at P = 128)
+  uint32_t mirbits = 0u;
+  int it = 0;
+  for (int q0 = threadIdx.x; q0 < nitems; q0 += kPouBatch * stride, ++it) {
     PkSmp<VT> sm[kPouBatch];
     int ii[kPouBatch], jj[kPouBatch], vc[kPouBatch];
 #pragma unroll
@@ -456,6 +460,9 @@ __global__ __launch_bounds__(kPouHalfThreads) void pou_slice_ifft_kernel(const c
       float px, py, pz;
       pou_kpos(g, ux, uy, uz, vx, vy, vz, ii[t], jj[t], px, py, pz);
       pk_issue<VT>(sm[t], blk, k, P, px, py, pz);
+      if (q0 + t * stride < nitems && !pou_self_mirror(ii[t], jj[t], hp) &&
+          pou_mirror_explicit(ii[t], jj[t], P, px, py, pz))
+        mirbits |= 1u << (kPouBatch * it + t);
     }
 #pragma unroll
     for (int t = 0; t < kPouBatch; ++t)
@@ -466,25 +473,24 @@ __global__ __launch_bounds__(kPouHalfThreads) void pou_slice_ifft_kernel(const c
         a[((ii[t] + hp) & (P - 1)) * pitch + vc[t]] = make_float2(re, self ? 0.f : im);
       }
   }
-  for (int q0 = threadIdx.x; q0 < nitems; q0 += kPouBatch * stride) {
+  it = 0;
+  for (int q0 = threadIdx.x; q0 < nitems; q0 += kPouBatch * stride, ++it) {
+    const uint32_t bits = (mirbits >> (kPouBatch * it)) & ((1u << kPouBatch) - 1u);
+    if (!__any(bits != 0u)) continue;  // no lane of the wave needs a mirror here
     PkSmp<VT> sm[kPouBatch];
     int ii[kPouBatch], jj[kPouBatch], vc[kPouBatch];
-    bool mir[kPouBatch];
 #pragma unroll
     for (int t = 0; t < kPouBatch; ++t) {
       item(min(q0 + t * stride, nitems - 1), ii[t], jj[t], vc[t]);
-      float px, py, pz;
-      pou_kpos(g, ux, uy, uz, vx, vy, vz, ii[t], jj[t], px, py, pz);
-      mir[t] = q0 + t * stride < nitems && !pou_self_mirror(ii[t], jj[t], hp) &&
-               pou_mirror_explicit(ii[t], jj[t], P, px, py, pz);
-      if (mir[t]) {
+      if ((bits >> t) & 1u) {
+        float px, py, pz;
         pou_kpos(g, ux, uy, uz, vx, vy, vz, (P - ii[t]) & (P - 1), (P - jj[t]) & (P - 1), px, py, pz);
         pk_issue<VT>(sm[t], blk, k, P, px, py, pz);
       }
     }
 #pragma unroll
     for (int t = 0; t < kPouBatch; ++t)
-      if (mir[t]) {
+      if ((bits >> t) & 1u) {
         float re, im;
         sample(sm[t], re, im);
         float2& h = a[((ii[t] + hp) & (P - 1)) * pitch + vc[t]];
@@ -965,7 +971,11 @@ __device__ __forceinline__ void pou_adjoint_tail(float2* a, const float2* tw, fl
       gv[c] = fmaf(ky, d, gv[c]);
     }
   };
-  for (int q0 = threadIdx.x; q0 < nitems; q0 += kPouBatch * NT) {
+  // (bit kPouBatch * iteration + t of mirbits: the representative's mirror is gathered in the
+  // second pass; at most 5 iterations of 4 items at P = 128 with 512 threads)
+  uint32_t mirbits = 0u;
+  int it = 0;
+  for (int q0 = threadIdx.x; q0 < nitems; q0 += kPouBatch * NT, ++it) {
     PkSmp<VT> smp[kPouBatch];
     int ii[kPouBatch], jj[kPouBatch];
     float ww[kPouBatch];
@@ -975,33 +985,33 @@ __device__ __forceinline__ void pou_adjoint_tail(float2* a, const float2* tw, fl
       float px, py, pz;
       pou_kpos(sg, ux, uy, uz, vx, vy, vz, ii[t], jj[t], px, py, pz);
       ww[t] = q0 + t * NT < nitems ? weight(ii[t], jj[t], px, py, pz) : 0.f;
+      if (ww[t] == 1.f && !pou_self_mirror(ii[t], jj[t], hp)) mirbits |= 1u << (kPouBatch * it + t);
       pk_issue<VT>(smp[t], blk, k, P, px, py, pz);
     }
 #pragma unroll
     for (int t = 0; t < kPouBatch; ++t)
       if (ww[t] != 0.f) contract(smp[t], ii[t], jj[t], ww[t]);
   }
-  for (int q0 = threadIdx.x; q0 < nitems; q0 += kPouBatch * NT) {
+  it = 0;
+  for (int q0 = threadIdx.x; q0 < nitems; q0 += kPouBatch * NT, ++it) {
+    const uint32_t bits = (mirbits >> (kPouBatch * it)) & ((1u << kPouBatch) - 1u);
+    if (!__any(bits != 0u)) continue;  // no lane of the wave needs a mirror here
     PkSmp<VT> smp[kPouBatch];
     int ii[kPouBatch], jj[kPouBatch];
-    bool mir[kPouBatch];
 #pragma unroll
     for (int t = 0; t < kPouBatch; ++t) {
       item(min(q0 + t * NT, nitems - 1), ii[t], jj[t]);
-      float px, py, pz;
-      pou_kpos(sg, ux, uy, uz, vx, vy, vz, ii[t], jj[t], px, py, pz);
-      mir[t] = q0 + t * NT < nitems && weight(ii[t], jj[t], px, py, pz) == 1.f &&
-               !pou_self_mirror(ii[t], jj[t], hp);
       ii[t] = (P - ii[t]) & (P - 1);
       jj[t] = (P - jj[t]) & (P - 1);
-      if (mir[t]) {
+      if ((bits >> t) & 1u) {
+        float px, py, pz;
         pou_kpos(sg, ux, uy, uz, vx, vy, vz, ii[t], jj[t], px, py, pz);
         pk_issue<VT>(smp[t], blk, k, P, px, py, pz);
       }
     }
 #pragma unroll
     for (int t = 0; t < kPouBatch; ++t)
-      if (mir[t]) contract(smp[t], ii[t], jj[t], 1.f);
+      if ((bits >> t) & 1u) contract(smp[t], ii[t], jj[t], 1.f);
   }
   float all[9] = {s3[0], s3[1], s3[2], gu[0], gu[1], gu[2], gv[0], gv[1], gv[2]};
   __syncthreads();
