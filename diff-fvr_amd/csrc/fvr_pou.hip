@@ -251,6 +251,10 @@ __device__ void fft_lines_dif4(float2* a, int nlines, int L, int logL, int lpitc
     x0 = make_float2(x0.x + x1.x, x0.y + x1.y);
     x1 = make_float2(dr * w.x - di * w.y, dr * w.y + di * w.x);
   };
+  // COLS with nlines - 1 a power of two (the P/2 + 1 columns of a half spectrum): no division
+  const int nmain = nlines - 1;
+  const bool pow2m1 = nmain > 0 && (nmain & (nmain - 1)) == 0;
+  const int lmain = pow2m1 ? 31 - __builtin_clz(unsigned(nmain)) : 0;
   int s = logL;
 #pragma unroll 1
   for (; s >= 2; s -= 2) {
@@ -259,8 +263,20 @@ __device__ void fft_lines_dif4(float2* a, int nlines, int L, int logL, int lpitc
     for (int idx = threadIdx.x; idx < ng; idx += blockDim.x) {
       int line, g;
       if (COLS) {
-        g = idx / nlines;
-        line = idx - g * nlines;
+        // lines first; with nlines - 1 a power of two the first nlines - 1 take shifts and the
+        // last line the tail of the index range
+        if (pow2m1) {
+          if (idx < nmain * gpl) {
+            line = idx & (nmain - 1);
+            g = idx >> lmain;
+          } else {
+            line = nmain;
+            g = idx - nmain * gpl;
+          }
+        } else {
+          g = idx / nlines;
+          line = idx - g * nlines;
+        }
       } else {
         line = idx >> (logL - 2);
         g = idx & (gpl - 1);
@@ -290,8 +306,18 @@ __device__ void fft_lines_dif4(float2* a, int nlines, int L, int logL, int lpitc
     for (int idx = threadIdx.x; idx < ng; idx += blockDim.x) {
       int line, k;
       if (COLS) {
-        k = idx / nlines;
-        line = idx - k * nlines;
+        if (pow2m1) {
+          if (idx < nmain * hl) {
+            line = idx & (nmain - 1);
+            k = idx >> lmain;
+          } else {
+            line = nmain;
+            k = idx - nmain * hl;
+          }
+        } else {
+          k = idx / nlines;
+          line = idx - k * nlines;
+        }
       } else {
         line = idx >> (logL - 1);
         k = idx & (hl - 1);
