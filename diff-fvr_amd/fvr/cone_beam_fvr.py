@@ -193,8 +193,23 @@ def _separable_rays(model, rx: torch.Tensor, ry: torch.Tensor, H: int, W: int):
 
 
 class _PouState:
-    """What the fused forward leaves for the adjoint (device tensors and shapes)."""
+    """What the fused forward leaves for the adjoint (device tensors and shapes).  The tensors
+    go through ctx.save_for_backward (``save`` / ``restore``), so that an input changed in place
+    between forward and backward raises autograd's version error instead of giving a wrong
+    gradient."""
     __slots__ = ("u", "v", "db", "cu", "cv", "rays", "per_pose", "K", "H", "W", "sep", "imgs")
+
+    def tensors(self):
+        return (self.u, self.v, self.db, self.cu, self.cv, *self.rays, self.imgs)
+
+    def drop_tensors(self):
+        self.u = self.v = self.db = self.cu = self.cv = self.rays = self.imgs = None
+        return self
+
+    def restore(self, saved):
+        self.u, self.v, self.db, self.cu, self.cv, rx, ry, ic, self.imgs = saved[:9]
+        self.rays = (rx, ry, ic)
+        return self
 
 
 def _pou_forward(model, u, v, db, c_u, c_v, rx, ry, ic, out_h, out_w):
@@ -276,14 +291,15 @@ class _PouRender(torch.autograd.Function):
         dev = model.F_blk_ri.device
         out, s = _pou_forward(model, _f32(u_xyz, dev), _f32(v_xyz, dev), _f32(d_b, dev), _f32(cu, dev),
                               _f32(cv, dev), rx, ry, ic, out_h, out_w)
-        ctx.state, ctx.model = s, model
+        ctx.save_for_backward(*s.tensors())
+        ctx.state, ctx.model = s.drop_tensors(), model
         ctx.meta = (rx.shape, u_xyz.dtype)
         ctx.dts = tuple(t.dtype for t in (d_b, cu, cv, rx, ry, ic))
         return out
 
     @staticmethod
     def backward(ctx, g):
-        s, model = ctx.state, ctx.model
+        s, model = ctx.state.restore(ctx.saved_tensors), ctx.model
         rshape, udt = ctx.meta
         B, P, N, K = model.B, model.P, s.u.shape[0], s.K
         dev = s.imgs.device
@@ -324,15 +340,17 @@ class _PouPinhole(torch.autograd.Function):
                                        _lib.ptr(v), _lib.ptr(db), _lib.ptr(c_u), _lib.ptr(c_v), st),
                    "fvr_pou_pose_geom")
         out, s = _pou_forward(model, u, v, db, c_u, c_v, rx, ry, ic, out_h, out_w)
-        ctx.state, ctx.model, ctx.M, ctx.lat12, ctx.dt = s, model, M, lat12, pose_mats.dtype
+        ctx.save_for_backward(*s.tensors(), M, lat12)
+        ctx.state, ctx.model, ctx.dt = s.drop_tensors(), model, pose_mats.dtype
         return out
 
     @staticmethod
     def backward(ctx, g):
-        s, model, M = ctx.state, ctx.model, ctx.M
+        saved = ctx.saved_tensors
+        s, model, M, lat12 = ctx.state.restore(saved), ctx.model, saved[9], saved[10]
         part, _ = _pou_adjoint(model, s, g)
         gM = torch.empty_like(M)
-        _lib.check(_lib.lib().fvr_pou_pose_geom_bwd(_lib.ptr(M), M.shape[0], _lib.ptr(ctx.lat12),
+        _lib.check(_lib.lib().fvr_pou_pose_geom_bwd(_lib.ptr(M), M.shape[0], _lib.ptr(lat12),
                                                     _lib.ptr(model.centers), model.B, _lib.ptr(part), _lib.ptr(gM),
                                                     _lib.stream_ptr(M.device)), "fvr_pou_pose_geom_bwd")
         return gM.to(ctx.dt), None, None, None, None, None, None, None
