@@ -24,6 +24,7 @@
 //   fvr_pou_sum_blocks   fixed-order sum over blocks of per-(block, pose) partials.
 //   fvr_pou_pose_geom(_bwd)  pose matrices -> detector axes and block offsets of the
 //                        lattice path, and its adjoint from the fused adjoint's sums.
+#include <atomic>
 #include <cmath>
 #include <type_traits>
 
@@ -39,6 +40,7 @@ constexpr int kPouHalfThreads = 512;    // the forward slice + iFFT (half-spectr
 #endif
 constexpr int kPouBatch = FVR_POU_BATCH;  // slice samples in flight per thread during a gather
 constexpr int kPouMaxLogP = 7;          // LDS-resident transforms up to 128 x 128
+constexpr int kPouMaxWaves = 16;        // waves of the largest workgroup (1024 threads)
 
 typedef float pf4a8 __attribute__((ext_vector_type(4), aligned(8)));
 typedef uint32_t pu2a4 __attribute__((ext_vector_type(2), aligned(4)));
@@ -366,8 +368,46 @@ __device__ __forceinline__ void interp8(const float (&cr)[8], const float (&ci)[
 
 struct PouSlice {
   int B, N, P, logP;
-  float c;  // P // 2, the centre voxel of every axis
+  float c;   // P // 2, the centre voxel of every axis
+  int mcap;  // entries of the workgroup's explicit-mirror queue (pou_mirror_queue)
 };
+
+// Capacity of the explicit-mirror queue of one workgroup: every stored item of the half plane
+// (P x P/2 + P) up to 4096 (8 KB of 16-bit entries, so that two 512-thread workgroups of the
+// forward kernel still fit in a CU's LDS at P = 128)
+__host__ __device__ inline int pou_mirror_cap(int P) {
+  const int n = P * (P / 2) + P;
+  return n < 4096 ? n : 4096;
+}
+__host__ __device__ inline size_t pou_queue_bytes(int P) { return (size_t(pou_mirror_cap(P)) * 2 + 15) & ~size_t(15); }
+
+// Deterministic compaction of the items whose DFT mirror is gathered explicitly: bit r of
+// `bits` stands for the thread's item threadIdx.x + r * stride; the items go to
+// queue[off ...] in bit order, off the exclusive prefix sum of the threads' counts in thread
+// order (a shuffle scan per wave, the wave totals through `scan`, one int per wave).  Returns
+// the number of entries, or -1 (nothing written) when they exceed cap: the caller then takes
+// its per-thread pass over `bits`.  Contains two workgroup barriers.
+__device__ __forceinline__ int pou_mirror_queue(uint32_t bits, int stride, int* scan, uint16_t* queue, int cap) {
+  const int c = __popc(bits), lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  int x = c;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int y = __shfl_up(x, d);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) scan[w] = x;
+  __syncthreads();
+  int off = x - c, tot = 0;
+  for (int k = 0; k < nw; ++k) {
+    const int t = scan[k];
+    off += k < w ? t : 0;
+    tot += t;
+  }
+  if (tot <= cap)
+    for (uint32_t r = bits; r != 0u; r &= r - 1u) queue[off++] = uint16_t(threadIdx.x + __builtin_ctz(r) * stride);
+  __syncthreads();
+  return tot <= cap ? tot : -1;
+}
 
 // (block, pose) of workgroup slot w: each XCD (w % 8) takes a contiguous block-major range,
 // so the poses of one block run on one XCD and share its L2 copy of the block's slices
@@ -444,6 +484,8 @@ __global__ __launch_bounds__(kPouHalfThreads) void pou_slice_ifft_kernel(const c
   const int P = g.P, logP = g.logP, hp = P / 2, pitch = hp + 1;
   float2* a = smem;               // H, then Y, Z, z: P rows x (P/2 + 1) columns
   float2* tw = smem + P * pitch;  // exp(-2 pi i k / P), k < P/2
+  int* scan = reinterpret_cast<int*>(tw + hp);                   // kPouMaxWaves
+  uint16_t* queue = reinterpret_cast<uint16_t*>(scan + kPouMaxWaves);  // g.mcap entries
   const int bn = pou_bn(blockIdx.x, per_xcd);
   if (bn >= g.B * g.N) return;  // whole workgroup: padding slot of the XCD split
   const int b = bn / g.N, n = bn - b * g.N;
@@ -454,7 +496,10 @@ __global__ __launch_bounds__(kPouHalfThreads) void pou_slice_ifft_kernel(const c
   const int stride = blockDim.x;
   // Sample (i, j) sits at c + (j - P/2) u + (i - P/2) v; its DFT mirror is ((P - i) mod P,
   // (P - j) mod P).  Pass 1 stores H = S(k) (Re S for the four self-mirrored samples); pass 2
-  // completes H = (S(k) + conj S(mirror)) / 2 where the mirror is not conj S(k) (pou_mirror_mode).
+  // completes H = (S(k) + conj S(mirror)) / 2 where the mirror is not conj S(k)
+  // (pou_mirror_explicit): over the compacted queue of those items (a few hundred of the
+  // P x P/2 + P at P = 128, mostly in bands along the box faces that every patch of a slice
+  // column range crosses), or, when they overflow the queue, per thread from its bitmask.
   const int nq = P * hp, nitems = nq + P, logH = logP - 1;
   auto item = [&](int q, int& i, int& j, int& vcol) {  // stored entry q: columns v = jj < P/2, then v = P/2
     if (q < nq) {
@@ -499,29 +544,48 @@ __global__ __launch_bounds__(kPouHalfThreads) void pou_slice_ifft_kernel(const c
         a[((ii[t] + hp) & (P - 1)) * pitch + vc[t]] = make_float2(re, self ? 0.f : im);
       }
   }
-  it = 0;
-  for (int q0 = threadIdx.x; q0 < nitems; q0 += kPouBatch * stride, ++it) {
-    const uint32_t bits = (mirbits >> (kPouBatch * it)) & ((1u << kPouBatch) - 1u);
-    if (!__any(bits != 0u)) continue;  // no lane of the wave needs a mirror here
-    PkSmp<VT> sm[kPouBatch];
-    int ii[kPouBatch], jj[kPouBatch], vc[kPouBatch];
+  auto mirror_issue = [&](PkSmp<VT>& sm, int i, int j) {
+    float px, py, pz;
+    pou_kpos(g, ux, uy, uz, vx, vy, vz, (P - i) & (P - 1), (P - j) & (P - 1), px, py, pz);
+    pk_issue<VT>(sm, blk, k, P, px, py, pz);
+  };
+  auto mirror_add = [&](const PkSmp<VT>& sm, int i, int vcol) {
+    float re, im;
+    sample(sm, re, im);
+    float2& h = a[((i + hp) & (P - 1)) * pitch + vcol];
+    h = make_float2(0.5f * (h.x + re), 0.5f * (h.y - im));
+  };
+  const int nmq = pou_mirror_queue(mirbits, stride, scan, queue, g.mcap);
+  if (nmq >= 0) {
+    for (int e0 = threadIdx.x; e0 < nmq; e0 += 2 * stride) {
+      PkSmp<VT> sm[2];
+      int ii[2], jj[2], vc[2];
 #pragma unroll
-    for (int t = 0; t < kPouBatch; ++t) {
-      item(min(q0 + t * stride, nitems - 1), ii[t], jj[t], vc[t]);
-      if ((bits >> t) & 1u) {
-        float px, py, pz;
-        pou_kpos(g, ux, uy, uz, vx, vy, vz, (P - ii[t]) & (P - 1), (P - jj[t]) & (P - 1), px, py, pz);
-        pk_issue<VT>(sm[t], blk, k, P, px, py, pz);
-      }
+      for (int t = 0; t < 2; ++t)
+        if (e0 + t * stride < nmq) {
+          item(queue[e0 + t * stride], ii[t], jj[t], vc[t]);
+          mirror_issue(sm[t], ii[t], jj[t]);
+        }
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+        if (e0 + t * stride < nmq) mirror_add(sm[t], ii[t], vc[t]);
     }
+  } else {
+    it = 0;
+    for (int q0 = threadIdx.x; q0 < nitems; q0 += kPouBatch * stride, ++it) {
+      const uint32_t bits = (mirbits >> (kPouBatch * it)) & ((1u << kPouBatch) - 1u);
+      if (!__any(bits != 0u)) continue;  // no lane of the wave needs a mirror here
+      PkSmp<VT> sm[kPouBatch];
+      int ii[kPouBatch], jj[kPouBatch], vc[kPouBatch];
 #pragma unroll
-    for (int t = 0; t < kPouBatch; ++t)
-      if ((bits >> t) & 1u) {
-        float re, im;
-        sample(sm[t], re, im);
-        float2& h = a[((ii[t] + hp) & (P - 1)) * pitch + vc[t]];
-        h = make_float2(0.5f * (h.x + re), 0.5f * (h.y - im));
+      for (int t = 0; t < kPouBatch; ++t) {
+        item(min(q0 + t * stride, nitems - 1), ii[t], jj[t], vc[t]);
+        if ((bits >> t) & 1u) mirror_issue(sm[t], ii[t], jj[t]);
       }
+#pragma unroll
+      for (int t = 0; t < kPouBatch; ++t)
+        if ((bits >> t) & 1u) mirror_add(sm[t], ii[t], vc[t]);
+    }
   }
   __syncthreads();
   // inverse FFTs of the P/2 + 1 columns (length P): Y at rows brev(m)
@@ -748,7 +812,6 @@ __global__ __launch_bounds__(256) void pou_warp_fwd_sep_kernel(const float* __re
 // fixed-order workgroup sum of NV per-thread values: an XOR butterfly inside every wave
 // (lane 0's result is used), then the per-wave sums added in wave order by every thread;
 // red holds NV x (waves per workgroup) floats
-constexpr int kPouMaxWaves = 16;
 template <int NV>
 __device__ __forceinline__ void block_sum(float (&v)[NV], float* red) {
 #pragma unroll
@@ -908,7 +971,8 @@ __global__ void pou_fix_to_float_kernel(unsigned long long* __restrict__ acc, co
 // gradient; the nine sums (the warp's three, d/du and d/dv) go to part[bn] in fixed order,
 // each plus fl (0, or NaN to mark the pose's sums as undefined).  NT: threads of the workgroup.
 template <int VT, int NT>
-__device__ __forceinline__ void pou_adjoint_tail(float2* a, const float2* tw, float* red, const char* Fp,
+__device__ __forceinline__ void pou_adjoint_tail(float2* a, const float2* tw, float* red, uint16_t* queue,
+                                                 const char* Fp,
                                                  const PouSlice& sg, const PouPk& k, int b, int n, int bn,
                                                  const float* __restrict__ u, const float* __restrict__ v,
                                                  const float (&s3)[3], float fl, float* __restrict__ part) {
@@ -955,7 +1019,9 @@ __device__ __forceinline__ void pou_adjoint_tail(float2* a, const float2* tw, fl
   // the reflection of an interior sample its term equals the representative's (the slice
   // gradient G is Hermitian, the mirror sample conj S with the gradient -conj dS and the
   // frequencies negated), so the representative counts twice; otherwise
-  // (pou_grad_mirror_explicit) the mirror is gathered in a second pass.
+  // (pou_grad_mirror_explicit) the mirror is gathered in a second pass, over the compacted queue
+  // of those representatives (sg.mcap entries at `queue`), or per thread from its bitmask when
+  // they overflow it.
   const float ux = u[3 * n], uy = u[3 * n + 1], uz = u[3 * n + 2];
   const float vx = v[3 * n], vy = v[3 * n + 1], vz = v[3 * n + 2];
   const unsigned long long blk = reinterpret_cast<unsigned long long>(Fp) + (unsigned long long)b * k.block;
@@ -1018,26 +1084,44 @@ __device__ __forceinline__ void pou_adjoint_tail(float2* a, const float2* tw, fl
     for (int t = 0; t < kPouBatch; ++t)
       if (ww[t] != 0.f) contract(smp[t], ii[t], jj[t], ww[t]);
   }
-  it = 0;
-  for (int q0 = threadIdx.x; q0 < nitems; q0 += kPouBatch * NT, ++it) {
-    const uint32_t bits = (mirbits >> (kPouBatch * it)) & ((1u << kPouBatch) - 1u);
-    if (!__any(bits != 0u)) continue;  // no lane of the wave needs a mirror here
-    PkSmp<VT> smp[kPouBatch];
-    int ii[kPouBatch], jj[kPouBatch];
+  auto mirror_issue = [&](PkSmp<VT>& sm, int& i, int& j) {  // (i, j) <- the mirror's indices
+    i = (P - i) & (P - 1);
+    j = (P - j) & (P - 1);
+    float px, py, pz;
+    pou_kpos(sg, ux, uy, uz, vx, vy, vz, i, j, px, py, pz);
+    pk_issue<VT>(sm, blk, k, P, px, py, pz);
+  };
+  const int nmq = pou_mirror_queue(mirbits, NT, reinterpret_cast<int*>(red), queue, sg.mcap);
+  if (nmq >= 0) {
+    for (int e0 = threadIdx.x; e0 < nmq; e0 += 2 * NT) {
+      PkSmp<VT> smp[2];
+      int ii[2], jj[2];
 #pragma unroll
-    for (int t = 0; t < kPouBatch; ++t) {
-      item(min(q0 + t * NT, nitems - 1), ii[t], jj[t]);
-      ii[t] = (P - ii[t]) & (P - 1);
-      jj[t] = (P - jj[t]) & (P - 1);
-      if ((bits >> t) & 1u) {
-        float px, py, pz;
-        pou_kpos(sg, ux, uy, uz, vx, vy, vz, ii[t], jj[t], px, py, pz);
-        pk_issue<VT>(smp[t], blk, k, P, px, py, pz);
-      }
+      for (int t = 0; t < 2; ++t)
+        if (e0 + t * NT < nmq) {
+          item(queue[e0 + t * NT], ii[t], jj[t]);
+          mirror_issue(smp[t], ii[t], jj[t]);
+        }
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+        if (e0 + t * NT < nmq) contract(smp[t], ii[t], jj[t], 1.f);
     }
+  } else {
+    it = 0;
+    for (int q0 = threadIdx.x; q0 < nitems; q0 += kPouBatch * NT, ++it) {
+      const uint32_t bits = (mirbits >> (kPouBatch * it)) & ((1u << kPouBatch) - 1u);
+      if (!__any(bits != 0u)) continue;  // no lane of the wave needs a mirror here
+      PkSmp<VT> smp[kPouBatch];
+      int ii[kPouBatch], jj[kPouBatch];
 #pragma unroll
-    for (int t = 0; t < kPouBatch; ++t)
-      if ((bits >> t) & 1u) contract(smp[t], ii[t], jj[t], 1.f);
+      for (int t = 0; t < kPouBatch; ++t) {
+        item(min(q0 + t * NT, nitems - 1), ii[t], jj[t]);
+        if ((bits >> t) & 1u) mirror_issue(smp[t], ii[t], jj[t]);
+      }
+#pragma unroll
+      for (int t = 0; t < kPouBatch; ++t)
+        if ((bits >> t) & 1u) contract(smp[t], ii[t], jj[t], 1.f);
+    }
   }
   float all[9] = {s3[0], s3[1], s3[2], gu[0], gu[1], gu[2], gv[0], gv[1], gv[2]};
   __syncthreads();
@@ -1097,7 +1181,9 @@ __global__ __launch_bounds__(kPouSliceThreads) void pou_render_bwd_kernel(
     if (idx < P * hp) a[(idx >> logH) * hpitch + (idx & (hp - 1))] = zv[t];
   }
   __syncthreads();
-  pou_adjoint_tail<VT, kPouSliceThreads>(a, tw, red, Fp, sg, k, b, n, bn, u, v, s3, 0.f, part);
+  // the queue: the upper half of the full-size image, unused by the half-spectrum steps
+  pou_adjoint_tail<VT, kPouSliceThreads>(a, tw, red, reinterpret_cast<uint16_t*>(a + P * hpitch), Fp, sg, k, b, n,
+                                         bn, u, v, s3, 0.f, part);
 }
 
 // ---------------------------------------------------------------------------
@@ -1138,10 +1224,16 @@ __host__ __device__ inline int sep_max_segments(int P) {
   return kPouSepThreads / P > kPouSepWaves ? kPouSepThreads / P : kPouSepWaves;
 }
 
+// bytes of the region after the packed image: the staged row windows during the gather, the
+// explicit-mirror queue of the contraction afterwards
+__host__ __device__ inline size_t sep_stage_bytes(int P, int W) {
+  const size_t stage = sep_staged(P, W) ? size_t(kPouSepWaves) * sep_stage_floats(W) * sizeof(float) : 0;
+  return stage > pou_queue_bytes(P) ? stage : pou_queue_bytes(P);
+}
+
 __host__ __device__ inline size_t sep_lds_bytes(int P, int W) {
   const int nseg = sep_max_segments(P);
-  const size_t stage = sep_staged(P, W) ? size_t(kPouSepWaves) * sep_stage_floats(W) * sizeof(float) : 0;
-  return size_t(P) * (P / 2 + 1) * sizeof(float2) + stage + (P / 2) * sizeof(float2) +
+  return size_t(P) * (P / 2 + 1) * sizeof(float2) + sep_stage_bytes(P, W) + (P / 2) * sizeof(float2) +
          9 * kPouMaxWaves * sizeof(float) + (5 * size_t(P) + 2 * nseg + 4) * sizeof(int);
 }
 
@@ -1157,8 +1249,8 @@ __global__ __launch_bounds__(kPouSepThreads, 4) void pou_render_bwd_sep_kernel(
   const int P = sg.P, logP = sg.logP, hp = P / 2, hpitch = hp + 1;
   const bool staged = sep_staged(P, W);
   float2* a = smem;                                     // the packed gradient image: P x hpitch
-  float* xstage = reinterpret_cast<float*>(a + P * hpitch);  // staged: per-wave row windows
-  float2* tw = reinterpret_cast<float2*>(xstage + (staged ? kPouSepWaves * sep_stage_floats(W) : 0));  // P/2
+  float* xstage = reinterpret_cast<float*>(a + P * hpitch);  // staged: per-wave row windows; then the queue
+  float2* tw = reinterpret_cast<float2*>(reinterpret_cast<char*>(xstage) + sep_stage_bytes(P, W));  // P/2
   float* red = reinterpret_cast<float*>(tw + hp);       // 9 * kPouMaxWaves
   int* xlo = reinterpret_cast<int*>(red + 9 * kPouMaxWaves);  // per texel column / row: first and
   int* xhi = xlo + P;                                   //   last pixel column / row touching it
@@ -1472,7 +1564,8 @@ __global__ __launch_bounds__(kPouSepThreads, 4) void pou_render_bwd_sep_kernel(
   const float s3[3] = {s0 * inv_ds, s1 * inv_ds, s2 * inv_ds};
   __syncthreads();
   // a pose whose X holds a non-finite value (flag NaN) gets NaN sums
-  pou_adjoint_tail<VT, NT>(a, tw, red, Fp, sg, k, b, n, bn, u, v, s3, flag[n], part);
+  pou_adjoint_tail<VT, NT>(a, tw, red, reinterpret_cast<uint16_t*>(xstage), Fp, sg, k, b, n, bn, u, v, s3, flag[n],
+                           part);
 }
 
 // X[n][q] = go[n][q] ic[q] (ic per pose when per_pose_rays) and flag[n] = 0, or NaN when any
@@ -1704,7 +1797,19 @@ __global__ __launch_bounds__(256) void pou_pack_kernel(const void* __restrict__ 
 }
 
 static size_t slice_lds_bytes(int P) { return (size_t(P) * (P + 1) + P / 2) * sizeof(float2); }
-static size_t half_lds_bytes(int P) { return (size_t(P) * (P / 2 + 1) + P / 2) * sizeof(float2); }
+// the forward kernel: half spectrum, twiddles, the scan's wave totals, the explicit-mirror queue
+static size_t half_lds_bytes(int P) {
+  return (size_t(P) * (P / 2 + 1) + P / 2) * sizeof(float2) + kPouMaxWaves * sizeof(int) + pou_queue_bytes(P);
+}
+
+// Whether the gathers compact their explicit mirrors into a queue (1, the default) or take the
+// per-thread pass for all of them (0); fvr_pou_set_mirror_queue
+static std::atomic<int> g_pou_mirror_queue{1};
+
+static PouSlice pou_slice(int B, int N, int P) {
+  return PouSlice{B, N, P, 31 - __builtin_clz(uint32_t(P)), float(P / 2),
+                  g_pou_mirror_queue.load(std::memory_order_relaxed) ? pou_mirror_cap(P) : 0};
+}
 
 __global__ void pou_zero_u64_kernel(unsigned long long* __restrict__ p, long long n) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1723,6 +1828,8 @@ static bool allow_lds(K kernel, size_t bytes) {
 using namespace fvr;
 
 extern "C" {
+
+int fvr_pou_set_mirror_queue(int32_t enabled) { return g_pou_mirror_queue.exchange(enabled ? 1 : 0); }
 
 int fvr_pou_lds_ok(int32_t P) {
   if (P < 4 || P > (1 << kPouMaxLogP) || (P & (P - 1))) return 0;
@@ -1758,7 +1865,7 @@ int fvr_pou_slice_ifft(uint64_t F, int32_t dtype, int32_t B, int32_t P, const fl
   FVR_CHECK_ARG(F && u_xyz && v_xyz && imgs && B > 0 && N > 0, "fvr_pou_slice_ifft: bad arguments");
   FVR_CHECK_ARG(fvr_pou_lds_ok(P), "fvr_pou_slice_ifft: P = %d must be a power of two in [4, 128]", P);
   FVR_CHECK_ARG(int64_t(B) * N < (int64_t(1) << 30), "fvr_pou_slice_ifft: too many (block, pose) pairs");
-  PouSlice g{B, N, P, 31 - __builtin_clz(uint32_t(P)), float(P / 2)};
+  const PouSlice g = pou_slice(B, N, P);
   const PouPk k = pou_pk(P, dtype);
   const size_t lds = half_lds_bytes(P);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
@@ -1870,7 +1977,7 @@ int fvr_pou_render_bwd(uint64_t F, int32_t dtype, int32_t B, int32_t P, const fl
   PouWarp g;
   if (warp_geom(g, B, N, P, K, per_pose_rays, inv_ds, c_idx) != FVR_OK) return FVR_ERR_INVALID;
   FVR_CHECK_ARG(int64_t(B) * N < (int64_t(1) << 30), "fvr_pou_render_bwd: too many (block, pose) pairs");
-  PouSlice sg{B, N, P, 31 - __builtin_clz(uint32_t(P)), float(P / 2)};
+  const PouSlice sg = pou_slice(B, N, P);
   const PouPk k = pou_pk(P, dtype);
   const size_t lds = slice_lds_bytes(P) + 9 * kPouMaxWaves * sizeof(float);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
@@ -1915,7 +2022,7 @@ int fvr_pou_render_bwd_sep(uint64_t F, int32_t dtype, int32_t B, int32_t P, cons
   FVR_CHECK_ARG(B > 0 && N > 0 && int64_t(B) * N < (int64_t(1) << 30), "fvr_pou_render_bwd_sep: bad batch");
   FVR_CHECK_ARG(fvr_pou_sep_ok(P, H, W), "fvr_pou_render_bwd_sep: P = %d with a %d x %d detector does not fit", P, H,
                 W);
-  PouSlice sg{B, N, P, 31 - __builtin_clz(uint32_t(P)), float(P / 2)};
+  const PouSlice sg = pou_slice(B, N, P);
   const PouPk k = pou_pk(P, dtype);
   const size_t lds = sep_lds_bytes(P, W);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);

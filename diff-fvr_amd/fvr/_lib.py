@@ -101,6 +101,7 @@ _SIGNATURES = {
     "fvr_slice_bwd_workspace": ([_I, _I], ctypes.c_int64),
     "fvr_slice_bwd": ([_U64, _I, _I, _I, _I, _I, _U64, _P, _P, _I, _P, _P, _I, _F, _F, _F, _P, _P, _P, _P], _I),
     "fvr_pou_lds_ok": ([_I], _I),
+    "fvr_pou_set_mirror_queue": ([_I], _I),
     "fvr_pou_packed_bytes": ([_I, _I, _I], ctypes.c_int64),
     "fvr_pou_pack_blocks": ([_U64, _I, _I, _I, _U64, _P], _I),
     "fvr_pou_slice_ifft": ([_U64, _I, _I, _I, _P, _P, _I, _P, _P], _I),

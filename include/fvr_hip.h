@@ -368,6 +368,14 @@ int fvr_slice_bwd(uint64_t vol, int32_t vol_dtype, int32_t b_vol, int32_t d, int
 /* ------------------------------------------------------------------------ */
 /* 1 when P is a power of two in [4, 128] (the LDS-resident transforms). */
 int fvr_pou_lds_ok(int32_t P);
+/* Whether the PoU gathers (fvr_pou_slice_ifft, fvr_pou_render_bwd, fvr_pou_render_bwd_sep)
+ * compact the samples whose DFT mirror they gather explicitly into a per-workgroup queue
+ * (enabled != 0, the default: one dense pass over a few hundred samples) or take a pass of
+ * every thread over its own (0; also what a workgroup does when its samples overflow the
+ * queue).  Process-wide; returns the previous setting.  Forward results are the same bit for
+ * bit; the adjoint's sums differ only in float rounding (another summation order).  No
+ * counterpart in the reference (a switch for tests and timing). */
+int fvr_pou_set_mirror_queue(int32_t enabled);
 /* Gather layout of the fused PoU kernels: per block (P+3) x (P+2) x (P+2) elements
  * (z, y, x; storage index = logical + 1, zero margins, one extra zero plane on the
  * slow axis), each element the (re, im) pair -- float2 (dtype 0) or two 16-bit
