@@ -39,6 +39,11 @@ constexpr int kPouHalfThreads = 512;    // the forward slice + iFFT (half-spectr
 #define FVR_POU_BATCH 4
 #endif
 constexpr int kPouBatch = FVR_POU_BATCH;  // slice samples in flight per thread during a gather
+#ifndef FVR_POU_FWD_BATCH
+#define FVR_POU_FWD_BATCH 4
+#endif
+// the same in the forward kernel (95 VGPRs: its occupancy is set by its LDS, not its registers)
+constexpr int kFwdBatch = FVR_POU_FWD_BATCH;
 constexpr int kPouMaxLogP = 7;          // LDS-resident transforms up to 128 x 128
 constexpr int kPouMaxWaves = 16;        // waves of the largest workgroup (1024 threads)
 
@@ -83,35 +88,46 @@ struct PkSmp {
   float tx, ty, tz;
 };
 
-// issue the four pair loads of the sample at p (x, y, z block index space)
+// whether storage index X = floor(p) + 1 of the low corner is 0, 1, P or P + 1, i.e. a corner of
+// p's cell is storage index 0 or P (floor(p) in {-1, 0, P - 1, P}); P a power of two
+__device__ __forceinline__ bool pou_edge_idx(int X, int P) {
+  return (unsigned(X) <= unsigned(P + 1)) & ((X & (P - 1)) <= 1);
+}
+
+// issue the four pair loads of the sample at p (x, y, z block index space): a uniform block
+// base plus 32-bit offsets (the block is below 4 GB); returns whether p's cell has a corner at
+// storage index 0 or P on some axis (pou_edge_idx)
 template <int VT>
-__device__ __forceinline__ void pk_issue(PkSmp<VT>& s, unsigned long long blk, const PouPk& k, int P, float px,
+__device__ __forceinline__ bool pk_issue(PkSmp<VT>& s, unsigned long long blk, const PouPk& k, int P, float px,
                                          float py, float pz) {
   const float fx = floorf(px), fy = floorf(py), fz = floorf(pz);
   s.tx = px - fx;
   s.ty = py - fy;
   s.tz = pz - fz;
-  const int X = int(fx) + 1, Y = int(fy) + 1, Z = int(fz) + 1;  // storage index of the low corner
-  const bool in = unsigned(X) <= unsigned(P) && unsigned(Y) <= unsigned(P) && unsigned(Z) <= unsigned(P);
-  const uint32_t off = in ? __umul24(uint32_t(Z), k.plane) + __umul24(uint32_t(Y), k.row) + uint32_t(X) * k.esz
-                          : uint32_t(P + 1) * k.plane;
+  // storage index of the low corner (unsigned add: no overflow for a saturated conversion)
+  const int X = int(unsigned(int(fx)) + 1u), Y = int(unsigned(int(fy)) + 1u), Z = int(unsigned(int(fz)) + 1u);
+  const bool in = (unsigned(X) <= unsigned(P)) & (unsigned(Y) <= unsigned(P)) & (unsigned(Z) <= unsigned(P));
+  const uint32_t o0 = in ? __umul24(uint32_t(Z), k.plane) + __umul24(uint32_t(Y), k.row) + uint32_t(X) * k.esz
+                         : uint32_t(P + 1) * k.plane;
+  const uint32_t o1 = o0 + k.row, o2 = o0 + k.plane, o3 = o2 + k.row;
   typedef const char __attribute__((address_space(1)))* gc;
   const gc b = reinterpret_cast<gc>(blk);
   if (VT == 0) {
     typedef const pf4a8 __attribute__((address_space(1)))* gp;
     PkRaw<0>& r = reinterpret_cast<PkRaw<0>&>(s.r);
-    r.q[0] = *reinterpret_cast<gp>(b + off);
-    r.q[1] = *reinterpret_cast<gp>(b + off + k.row);
-    r.q[2] = *reinterpret_cast<gp>(b + off + k.plane);
-    r.q[3] = *reinterpret_cast<gp>(b + off + k.plane + k.row);
+    r.q[0] = *reinterpret_cast<gp>(b + o0);
+    r.q[1] = *reinterpret_cast<gp>(b + o1);
+    r.q[2] = *reinterpret_cast<gp>(b + o2);
+    r.q[3] = *reinterpret_cast<gp>(b + o3);
   } else {
     typedef const pu2a4 __attribute__((address_space(1)))* gp;
     PkRaw<1>& r = reinterpret_cast<PkRaw<1>&>(s.r);
-    r.q[0] = *reinterpret_cast<gp>(b + off);
-    r.q[1] = *reinterpret_cast<gp>(b + off + k.row);
-    r.q[2] = *reinterpret_cast<gp>(b + off + k.plane);
-    r.q[3] = *reinterpret_cast<gp>(b + off + k.plane + k.row);
+    r.q[0] = *reinterpret_cast<gp>(b + o0);
+    r.q[1] = *reinterpret_cast<gp>(b + o1);
+    r.q[2] = *reinterpret_cast<gp>(b + o2);
+    r.q[3] = *reinterpret_cast<gp>(b + o3);
   }
+  return pou_edge_idx(X, P) | pou_edge_idx(Y, P) | pou_edge_idx(Z, P);
 }
 
 template <int VT>
@@ -413,13 +429,24 @@ __device__ __forceinline__ int pou_mirror_queue(uint32_t bits, int stride, int* 
 // so the poses of one block run on one XCD and share its L2 copy of the block's slices
 __device__ __forceinline__ int pou_bn(int w, int per_xcd) { return (w & 7) * per_xcd + (w >> 3); }
 
+// Columns of a wave's slice patch in the PoU gathers: 2^kPatchLogC.  8 x 8 patches (the rows
+// walk the block's mid axis, the columns its fast axis) touch fewer cache lines per load than
+// the 2 x 32 patches of the parallel-beam kernel here: forward 0.51 -> 0.49 ms and the adjoint
+// about 5 % faster at 55 blocks x 64 poses, P = 128 (4 x 16 and 16 x 4 measured the same).
+#ifndef FVR_POU_PATCH_LOGC
+#define FVR_POU_PATCH_LOGC 3
+#endif
+constexpr int kPatchLogC = FVR_POU_PATCH_LOGC;
+
 // slice sample (row i, column jj) of the half plane, jj in [0, W) with W = 2^logW: every 64
-// consecutive q form a 2-row x 32-column patch (W >= 32), else row-major
+// consecutive q form a (64 >> kPatchLogC)-row x 2^kPatchLogC-column patch (W >= 2^kPatchLogC),
+// else row-major
 __device__ __forceinline__ void pou_ij_w(int q, int logW, int& i, int& jj) {
-  if (logW >= 5) {
-    const int w = q >> 6, lane = q & 63, sh = logW - 5;
-    i = 2 * (w >> sh) + (lane >> 5);
-    jj = 32 * (w & ((1 << sh) - 1)) + (lane & 31);
+  constexpr int C = 1 << kPatchLogC, R = 64 / C;
+  if (logW >= kPatchLogC) {  // uniform
+    const int w = q >> 6, lane = q & 63, sh = logW - kPatchLogC;
+    i = R * (w >> sh) + (lane >> kPatchLogC);
+    jj = C * (w & ((1 << sh) - 1)) + (lane & (C - 1));
   } else {
     i = q >> logW;
     jj = q & ((1 << logW) - 1);
@@ -436,34 +463,29 @@ __device__ __forceinline__ void pou_kpos(const PouSlice& g, float ux, float uy, 
   pz = fmaf(ky, vz, fmaf(kx, uz, g.c));
 }
 
-// the four samples that are their own DFT mirror: i, j in {0, P/2}
+// the four samples that are their own DFT mirror: i, j in {0, P/2} (bitwise operators: no
+// branches)
 __device__ __forceinline__ bool pou_self_mirror(int i, int j, int hp) {
-  return (i == 0 || i == hp) && (j == 0 || j == hp);
+  return ((i == 0) | (i == hp)) & ((j == 0) | (j == hp));
 }
 
 // Whether the DFT mirror of sample (i, j) at p must be gathered: the stored block spectrum
 // of a real cube is Hermitian about the centre c = P/2 on the storage indices 1 .. P-1, so
 // the trilinear sample at the reflection 2c - p is conj S(p) when no corner of p (hence of
 // its reflection) is storage index 0 or P, i.e. floor(p) is not in [-1, 0] or [P-1, P] on
-// any axis (corners further out read the zero padding on both sides).  On row / column 0
-// (frequency -P/2) the DFT mirror is not the reflection, so it is always gathered.
-__device__ __forceinline__ bool pou_edge(float p, int P) {
-  return (p >= -1.f && p < 1.f) || (p >= float(P - 1) && p < float(P + 1));
-}
-__device__ __forceinline__ bool pou_mirror_explicit(int i, int j, int P, float px, float py, float pz) {
-  return i == 0 || j == 0 || pou_edge(px, P) || pou_edge(py, P) || pou_edge(pz, P);
-}
+// any axis (corners further out read the zero padding on both sides): `edge`, from pk_issue.
+// On row / column 0 (frequency -P/2) the DFT mirror is not the reflection, so it is always
+// gathered.
+__device__ __forceinline__ bool pou_mirror_explicit(int i, int j, bool edge) { return (i == 0) | (j == 0) | edge; }
 // The same for the spatial gradient of the sample: at (or within float rounding of) a lattice
 // plane the trilinear derivative is one-sided (the cell above), and the mirror's is the cell
 // above the reflection, not the reflected cell, so those samples take the explicit mirror too
-// (1e-4 is far above the rounding of the mirror's position, about 1e-5 at P = 128).
-__device__ __forceinline__ bool pou_near_lattice(float p) {
-  const float f = p - floorf(p);
-  return !(f > 1e-4f && f < 1.f - 1e-4f);
-}
-__device__ __forceinline__ bool pou_grad_mirror_explicit(int i, int j, int P, float px, float py, float pz) {
-  return pou_mirror_explicit(i, j, P, px, py, pz) || pou_near_lattice(px) || pou_near_lattice(py) ||
-         pou_near_lattice(pz);
+// (1e-4 is far above the rounding of the mirror's position, about 1e-5 at P = 128); f is the
+// fraction p - floor(p) of an axis.
+__device__ __forceinline__ bool pou_near_lattice(float f) { return !((f > 1e-4f) & (f < 1.f - 1e-4f)); }
+template <int VT>
+__device__ __forceinline__ bool pou_grad_mirror_explicit(int i, int j, bool edge, const PkSmp<VT>& s) {
+  return pou_mirror_explicit(i, j, edge) | pou_near_lattice(s.tx) | pou_near_lattice(s.ty) | pou_near_lattice(s.tz);
 }
 
 // Real slab image of one (block, pose) through a Hermitian half spectrum.  The image is
@@ -476,7 +498,7 @@ __device__ __forceinline__ bool pou_grad_mirror_explicit(int i, int j, int P, fl
 // whose outputs z(n) = x(2n) + i x(2n + 1) are the even and odd image samples.  Half the
 // LDS and half the transform work of the full complex 2D transform.
 template <int VT>
-__global__ __launch_bounds__(kPouHalfThreads) void pou_slice_ifft_kernel(const char* __restrict__ Fp, PouSlice g,
+__global__ __launch_bounds__(kPouHalfThreads, 4) void pou_slice_ifft_kernel(const char* __restrict__ Fp, PouSlice g,
                                                                          PouPk k, const float* __restrict__ u,
                                                                          const float* __restrict__ v, int per_xcd,
                                                                          float* __restrict__ imgs) {
@@ -502,41 +524,37 @@ __global__ __launch_bounds__(kPouHalfThreads) void pou_slice_ifft_kernel(const c
   // column range crosses), or, when they overflow the queue, per thread from its bitmask.
   const int nq = P * hp, nitems = nq + P, logH = logP - 1;
   auto item = [&](int q, int& i, int& j, int& vcol) {  // stored entry q: columns v = jj < P/2, then v = P/2
-    if (q < nq) {
-      int jj;
-      pou_ij_w(q, logH, i, jj);
-      j = hp + jj;
-      vcol = jj;
-    } else {
-      i = q - nq;
-      j = 0;
-      vcol = hp;
-    }
+    int ip, jj;
+    pou_ij_w(q, logH, ip, jj);
+    const bool main = q < nq;
+    i = main ? ip : q - nq;
+    j = main ? hp + jj : 0;
+    vcol = main ? jj : hp;
   };
   auto sample = [&](const PkSmp<VT>& sm, float& re, float& im) {
     float cr[8], ci[8];
     pk_corners<VT>(sm, cr, ci);
     interp8<false>(cr, ci, sm.tx, sm.ty, sm.tz, re, im, nullptr, nullptr);
   };
-  // (bit kPouBatch * iteration + t of mirbits: item t of the thread's iteration needs its mirror;
-  // at most 5 iterations of 4 items at P = 128)
+  // (bit kFwdBatch * iteration + t of mirbits: item t of the thread's iteration needs its mirror;
+  // at most 5 iterations of 4 items at P = 128: more samples in flight per thread measured slower)
   uint32_t mirbits = 0u;
   int it = 0;
-  for (int q0 = threadIdx.x; q0 < nitems; q0 += kPouBatch * stride, ++it) {
-    PkSmp<VT> sm[kPouBatch];
-    int ii[kPouBatch], jj[kPouBatch], vc[kPouBatch];
+  for (int q0 = threadIdx.x; q0 < nitems; q0 += kFwdBatch * stride, ++it) {
+    PkSmp<VT> sm[kFwdBatch];
+    int ii[kFwdBatch], jj[kFwdBatch], vc[kFwdBatch];
 #pragma unroll
-    for (int t = 0; t < kPouBatch; ++t) {
+    for (int t = 0; t < kFwdBatch; ++t) {
       item(min(q0 + t * stride, nitems - 1), ii[t], jj[t], vc[t]);
       float px, py, pz;
       pou_kpos(g, ux, uy, uz, vx, vy, vz, ii[t], jj[t], px, py, pz);
-      pk_issue<VT>(sm[t], blk, k, P, px, py, pz);
-      if (q0 + t * stride < nitems && !pou_self_mirror(ii[t], jj[t], hp) &&
-          pou_mirror_explicit(ii[t], jj[t], P, px, py, pz))
-        mirbits |= 1u << (kPouBatch * it + t);
+      const bool edge = pk_issue<VT>(sm[t], blk, k, P, px, py, pz);
+      const bool need = (q0 + t * stride < nitems) & !pou_self_mirror(ii[t], jj[t], hp) &
+                        pou_mirror_explicit(ii[t], jj[t], edge);
+      mirbits |= uint32_t(need) << (kFwdBatch * it + t);
     }
 #pragma unroll
-    for (int t = 0; t < kPouBatch; ++t)
+    for (int t = 0; t < kFwdBatch; ++t)
       if (q0 + t * stride < nitems) {
         float re, im;
         sample(sm[t], re, im);
@@ -572,18 +590,18 @@ __global__ __launch_bounds__(kPouHalfThreads) void pou_slice_ifft_kernel(const c
     }
   } else {
     it = 0;
-    for (int q0 = threadIdx.x; q0 < nitems; q0 += kPouBatch * stride, ++it) {
-      const uint32_t bits = (mirbits >> (kPouBatch * it)) & ((1u << kPouBatch) - 1u);
+    for (int q0 = threadIdx.x; q0 < nitems; q0 += kFwdBatch * stride, ++it) {
+      const uint32_t bits = (mirbits >> (kFwdBatch * it)) & ((1u << kFwdBatch) - 1u);
       if (!__any(bits != 0u)) continue;  // no lane of the wave needs a mirror here
-      PkSmp<VT> sm[kPouBatch];
-      int ii[kPouBatch], jj[kPouBatch], vc[kPouBatch];
+      PkSmp<VT> sm[kFwdBatch];
+      int ii[kFwdBatch], jj[kFwdBatch], vc[kFwdBatch];
 #pragma unroll
-      for (int t = 0; t < kPouBatch; ++t) {
+      for (int t = 0; t < kFwdBatch; ++t) {
         item(min(q0 + t * stride, nitems - 1), ii[t], jj[t], vc[t]);
         if ((bits >> t) & 1u) mirror_issue(sm[t], ii[t], jj[t]);
       }
 #pragma unroll
-      for (int t = 0; t < kPouBatch; ++t)
+      for (int t = 0; t < kFwdBatch; ++t)
         if ((bits >> t) & 1u) mirror_add(sm[t], ii[t], vc[t]);
     }
   }
@@ -1028,7 +1046,7 @@ __device__ __forceinline__ void pou_adjoint_tail(float2* a, const float2* tw, fl
   const float inv = 1.f / float(P * P);
   float gu[3] = {0.f, 0.f, 0.f}, gv[3] = {0.f, 0.f, 0.f};
   const int nh = P * hp, nitems = nh + P;
-  auto item = [&](int q, int& i, int& j) {
+  auto item = [&](int q, int& i, int& j) {  // (a branch: the select form makes the gather form spill)
     if (q < nh) {
       int jj;
       pou_ij_w(q, logH, i, jj);
@@ -1040,11 +1058,10 @@ __device__ __forceinline__ void pou_adjoint_tail(float2* a, const float2* tw, fl
   };
   // weight of a representative's own term: 0 (counted through its mirror), 1 (self-mirrored,
   // or its mirror gathered explicitly), 2 (the mirror's term equals its own)
-  auto weight = [&](int i, int j, float px, float py, float pz) {
-    const bool colself = j == 0 || j == hp;
-    if (colself && i > 0 && i < hp) return 0.f;
-    if (pou_self_mirror(i, j, hp)) return 1.f;
-    return pou_grad_mirror_explicit(i, j, P, px, py, pz) ? 1.f : 2.f;
+  auto weight = [&](int i, int j, bool edge, const PkSmp<VT>& s) {
+    const bool colself = (j == 0) | (j == hp);
+    const bool one = pou_self_mirror(i, j, hp) | pou_grad_mirror_explicit<VT>(i, j, edge, s);
+    return (colself & (i > 0) & (i < hp)) ? 0.f : (one ? 1.f : 2.f);
   };
   auto contract = [&](const PkSmp<VT>& sm, int i, int j, float w) {
     const float kx = float(j - hp), ky = float(i - hp);
@@ -1076,9 +1093,9 @@ __device__ __forceinline__ void pou_adjoint_tail(float2* a, const float2* tw, fl
       item(min(q0 + t * NT, nitems - 1), ii[t], jj[t]);
       float px, py, pz;
       pou_kpos(sg, ux, uy, uz, vx, vy, vz, ii[t], jj[t], px, py, pz);
-      ww[t] = q0 + t * NT < nitems ? weight(ii[t], jj[t], px, py, pz) : 0.f;
-      if (ww[t] == 1.f && !pou_self_mirror(ii[t], jj[t], hp)) mirbits |= 1u << (kPouBatch * it + t);
-      pk_issue<VT>(smp[t], blk, k, P, px, py, pz);
+      const bool edge = pk_issue<VT>(smp[t], blk, k, P, px, py, pz);
+      ww[t] = q0 + t * NT < nitems ? weight(ii[t], jj[t], edge, smp[t]) : 0.f;
+      mirbits |= uint32_t((ww[t] == 1.f) & !pou_self_mirror(ii[t], jj[t], hp)) << (kPouBatch * it + t);
     }
 #pragma unroll
     for (int t = 0; t < kPouBatch; ++t)
