@@ -227,14 +227,14 @@ def pou_config4(steps: int, warmup: int, size: int = 256, n_pose: int = 64):
     cands = mats(rng.uniform(-5, 5, (n_pose, 3)), rng.uniform(-5, 5, (n_pose, 3)))
     with torch.no_grad():
         target = model.forward_pinhole(true, lattice)
-    t0c = target - target.mean()
+    from fvr import image_ncc
 
     def step():
+        # NCC loss of the reference's PoU test (tests/test_cone_beam_fvr.py:24-27), per pose,
+        # on the fused NCC kernels
         M = cands.clone().requires_grad_(True)
         out = model.forward_pinhole(M, lattice)
-        o0 = out - out.mean(dim=(-2, -1), keepdim=True)
-        ncc = (o0 * t0c).sum(dim=(-2, -1)) / (o0.norm(dim=(-2, -1)) * t0c.norm() + 1e-8)
-        loss = (1.0 - ncc).sum()
+        loss = (1.0 - image_ncc(out, target)).sum()
         loss.backward()
         return loss, M.grad
 

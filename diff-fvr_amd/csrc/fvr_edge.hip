@@ -17,6 +17,8 @@
 //   edge_stats_kernel     per tile: sum u_c, sum u_c^2, sum u_c v_c (float64)
 //   edge_combine_kernel   per image: NCCs, the value, and the adjoint coefficients
 //   edge_bwd_kernel       d value / d image (scaled by the upstream gradient)
+//
+// Also here: the zero-mean NCC of whole images (ncc_*_kernel, fvr_ncc_fwd / fvr_ncc_bwd).
 
 #include <cmath>
 
@@ -322,6 +324,123 @@ static int edge_geom(EdgeGeom& g, int32_t H, int32_t W, int32_t crop, int32_t mu
   return FVR_OK;
 }
 
+
+// ---------------------------------------------------------------------------
+// Zero-mean NCC of whole images, the NCC of the PoU path's registration loss
+// (tests/test_cone_beam_fvr.py:24-27 of the reference: 1 - NCC):
+//   val = sum(u0 v0) / (|u0| |v0| + 1e-8),  u0 = u - mean u, v0 = v - mean v.
+//   ncc_stats_kernel    per chunk of kNccChunk pixels of an image: sum u, u^2, u v, v, v^2
+//                       (float64, reduced in a fixed order)
+//   ncc_combine_kernel  one wave per image: the chunk sums in a fixed order, the value and the
+//                       adjoint coefficients
+//   ncc_bwd_kernel      d val / d self = alpha (other - mean other) - beta (self - mean self),
+//                       times the upstream gradient (elementwise)
+// ---------------------------------------------------------------------------
+constexpr int kNccChunk = 4096;
+constexpr int kNccThreads = 256;
+constexpr int kNccSums = 5;
+
+__host__ __device__ inline long long ncc_chunks(long long K) { return (K + kNccChunk - 1) / kNccChunk; }
+
+__global__ __launch_bounds__(kNccThreads) void ncc_stats_kernel(const float* __restrict__ u,
+                                                               const float* __restrict__ v, int bt, long long K,
+                                                               bool vec4, double* __restrict__ part) {
+  const int n = blockIdx.y;
+  const long long c0 = (long long)blockIdx.x * kNccChunk;
+  const long long c1 = c0 + kNccChunk < K ? c0 + kNccChunk : K;
+  const float* un = u + size_t(n) * size_t(K);
+  const float* vn = v + size_t(bt == 1 ? 0 : n) * size_t(K);
+  double s[kNccSums] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  auto add = [&](float a, float b) {
+    const double x = a, y = b;
+    s[0] += x;
+    s[1] = fma(x, x, s[1]);
+    s[2] = fma(x, y, s[2]);
+    s[3] += y;
+    s[4] = fma(y, y, s[4]);
+  };
+  if (vec4) {  // K a multiple of 4 and 16-byte aligned rows: float4 loads
+    for (long long k = c0 + 4 * threadIdx.x; k < c1; k += 4 * kNccThreads) {
+      const float4 a = *reinterpret_cast<const float4*>(un + k);
+      const float4 b = *reinterpret_cast<const float4*>(vn + k);
+      add(a.x, b.x);
+      add(a.y, b.y);
+      add(a.z, b.z);
+      add(a.w, b.w);
+    }
+  } else {
+    for (long long k = c0 + threadIdx.x; k < c1; k += kNccThreads) add(un[k], vn[k]);
+  }
+  __shared__ double wred[kNccThreads / 64][kNccSums];
+#pragma unroll
+  for (int q = 0; q < kNccSums; ++q) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s[q] += __shfl_xor(s[q], o);
+  }
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int q = 0; q < kNccSums; ++q) wred[threadIdx.x >> 6][q] = s[q];
+  __syncthreads();
+  if (threadIdx.x < kNccSums) {
+    double x = wred[0][threadIdx.x];
+    for (int w = 1; w < kNccThreads / 64; ++w) x += wred[w][threadIdx.x];
+    part[(size_t(n) * gridDim.x + blockIdx.x) * kNccSums + threadIdx.x] = x;
+  }
+}
+
+// coef[n] = (alpha, beta_u, mean u, mean v, alpha, beta_v, mean v, mean u): the adjoint with
+// respect to the image (first four) and to the target (last four)
+__global__ __launch_bounds__(64) void ncc_combine_kernel(const double* __restrict__ part, int chunks, long long K,
+                                                         double* __restrict__ val, float* __restrict__ coef) {
+  const int n = blockIdx.x, lane = threadIdx.x;
+  const double* pp = part + size_t(n) * chunks * kNccSums;
+  double s[kNccSums] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int c = lane; c < chunks; c += 64)
+#pragma unroll
+    for (int q = 0; q < kNccSums; ++q) s[q] += pp[size_t(c) * kNccSums + q];
+#pragma unroll
+  for (int q = 0; q < kNccSums; ++q) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s[q] += __shfl_xor(s[q], o);
+  }
+  if (lane != 0) return;
+  const double npx = double(K);
+  const double su = s[0], suu = s[1], suv = s[2], sv = s[3], svv = s[4];
+  const double A = suv - su * sv / npx;
+  const double nu = sqrt(fmax(suu - su * su / npx, 0.0)), nv = sqrt(fmax(svv - sv * sv / npx, 0.0));
+  const double D = nu * nv + 1e-8;
+  val[n] = A / D;
+  // d val / d u = (v0 - A nv u0 / (nu D)) / D (nothing from |u0| for a constant image)
+  float* cf = coef + size_t(n) * 8;
+  cf[0] = cf[4] = float(1.0 / D);
+  cf[1] = float(nu > 0.0 ? A * nv / (nu * D * D) : 0.0);
+  cf[5] = float(nv > 0.0 ? A * nu / (nv * D * D) : 0.0);
+  cf[2] = cf[7] = float(su / npx);
+  cf[3] = cf[6] = float(sv / npx);
+}
+
+// grad[n, k] = gval[n] (alpha (other_k - mean other) - beta (self_k - mean self)); self = the
+// image n (wrt 0) or its target (wrt 1), other the other one
+__global__ __launch_bounds__(kNccThreads) void ncc_bwd_kernel(const float* __restrict__ u,
+                                                             const float* __restrict__ v, int bt, long long K,
+                                                             const float* __restrict__ coef,
+                                                             const double* __restrict__ gval, int wrt,
+                                                             float* __restrict__ grad) {
+  const int n = blockIdx.y;
+  const float* un = u + size_t(n) * size_t(K);
+  const float* vn = v + size_t(bt == 1 ? 0 : n) * size_t(K);
+  const float* self = wrt ? vn : un;
+  const float* other = wrt ? un : vn;
+  const float* cf = coef + size_t(n) * 8 + 4 * wrt;
+  const float g = float(gval[n]);
+  const float al = g * cf[0], be = g * cf[1], ms = cf[2], mo = cf[3];
+  float* gn = grad + size_t(n) * size_t(K);
+  const long long c0 = (long long)blockIdx.x * kNccChunk;
+  const long long c1 = c0 + kNccChunk < K ? c0 + kNccChunk : K;
+  for (long long k = c0 + threadIdx.x; k < c1; k += kNccThreads)
+    gn[k] = fmaf(al, other[k] - mo, -be * (self[k] - ms));
+}
+
 }  // namespace fvr
 
 using namespace fvr;
@@ -384,6 +503,43 @@ int fvr_edge_bwd(const float* img, int32_t n_img, int32_t H, int32_t W, int32_t 
   hipLaunchKernelGGL(edge_bwd_kernel, dim3(ftx * fty, n_img), dim3(kEThreads), 0,
                      reinterpret_cast<hipStream_t>(stream), img, g, channels, n_ch_img, coef, grad_value, ftx, grad);
   FVR_CHECK_LAUNCH("edge_bwd_kernel");
+  return FVR_OK;
+}
+
+int64_t fvr_ncc_work_doubles(int32_t n_img, int64_t K) {
+  if (n_img <= 0 || K <= 0) return -1;
+  return int64_t(n_img) * ncc_chunks(K) * kNccSums;
+}
+
+int fvr_ncc_fwd(const float* img, int32_t n_img, const float* target, int32_t n_target, int64_t K, double* work,
+                double* value, float* coef, void* stream) {
+  FVR_CHECK_ARG(img && target && work && value && coef && n_img > 0 && n_img < 65536 && K > 0,
+                "fvr_ncc_fwd: bad arguments");
+  FVR_CHECK_ARG(n_target == 1 || n_target == n_img, "fvr_ncc_fwd: %d targets for %d images", n_target, n_img);
+  const long long chunks = ncc_chunks(K);
+  FVR_CHECK_ARG(chunks < (1ll << 31), "fvr_ncc_fwd: %lld pixels per image exceed the grid", (long long)K);
+  const bool vec4 = (K % 4 == 0) && (reinterpret_cast<uintptr_t>(img) % 16 == 0) &&
+                    (reinterpret_cast<uintptr_t>(target) % 16 == 0);
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(ncc_stats_kernel, dim3(unsigned(chunks), n_img), dim3(kNccThreads), 0, st, img, target, n_target,
+                     (long long)K, vec4, work);
+  FVR_CHECK_LAUNCH("ncc_stats_kernel");
+  hipLaunchKernelGGL(ncc_combine_kernel, dim3(n_img), dim3(64), 0, st, work, int(chunks), (long long)K, value, coef);
+  FVR_CHECK_LAUNCH("ncc_combine_kernel");
+  return FVR_OK;
+}
+
+int fvr_ncc_bwd(const float* img, int32_t n_img, const float* target, int32_t n_target, int64_t K, const float* coef,
+                const double* grad_value, int32_t wrt_target, float* grad, void* stream) {
+  FVR_CHECK_ARG(img && target && coef && grad_value && grad && n_img > 0 && n_img < 65536 && K > 0,
+                "fvr_ncc_bwd: bad arguments");
+  FVR_CHECK_ARG(n_target == 1 || n_target == n_img, "fvr_ncc_bwd: %d targets for %d images", n_target, n_img);
+  const long long chunks = ncc_chunks(K);
+  FVR_CHECK_ARG(chunks < (1ll << 31), "fvr_ncc_bwd: %lld pixels per image exceed the grid", (long long)K);
+  hipLaunchKernelGGL(ncc_bwd_kernel, dim3(unsigned(chunks), n_img), dim3(kNccThreads), 0,
+                     reinterpret_cast<hipStream_t>(stream), img, target, n_target, (long long)K, coef, grad_value,
+                     wrt_target ? 1 : 0, grad);
+  FVR_CHECK_LAUNCH("ncc_bwd_kernel");
   return FVR_OK;
 }
 

@@ -16,7 +16,7 @@ from .registration_3d2d import compute_mtre_6dof
 from .phantom import ellipsoid_phantom
 from .triton_slice import fourier_slice_3d
 from .torch_engine import TorchDiffFVREngine
-from .cone_beam_fvr import (PerspectiveSlabFVR, SLAB_PRESETS, edge_dominant_ncc, pou_pinhole_warp_sum,
+from .cone_beam_fvr import (PerspectiveSlabFVR, SLAB_PRESETS, edge_dominant_ncc, image_ncc, pou_pinhole_warp_sum,
                             sample_isocenter_parity_offsets)
 
 __version__ = "0.1.0"
@@ -35,5 +35,6 @@ __all__ = [
     "SLAB_PRESETS",
     "sample_isocenter_parity_offsets",
     "edge_dominant_ncc",
+    "image_ncc",
     "pou_pinhole_warp_sum",
 ]

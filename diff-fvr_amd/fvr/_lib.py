@@ -126,6 +126,9 @@ _SIGNATURES = {
     "fvr_edge_stats": ([_P, _I, _I, _I, _I, _I, _P, _I, _P, _P], _I),
     "fvr_edge_combine": ([_P, _I, _P, _I, _I, _I, _I, _I, ctypes.c_double, _P, _P, _P], _I),
     "fvr_edge_bwd": ([_P, _I, _I, _I, _I, _I, _P, _I, _P, _P, _P, _P], _I),
+    "fvr_ncc_work_doubles": ([_I, ctypes.c_int64], ctypes.c_int64),
+    "fvr_ncc_fwd": ([_P, _I, _P, _I, ctypes.c_int64, _P, _P, _P, _P], _I),
+    "fvr_ncc_bwd": ([_P, _I, _P, _I, ctypes.c_int64, _P, _P, _I, _P, _P], _I),
 }
 
 

@@ -155,6 +155,63 @@ def edge_dominant_ncc(pred: torch.Tensor, target: torch.Tensor, border_crop: Opt
     return val.mean() if reduction == "mean" else val
 
 
+class _ImageNCC(torch.autograd.Function):
+    """Per-image zero-mean NCC (B,) of (B, K) float32 images against (Bt, K) targets, Bt in
+    {1, B}, on the HIP kernels ``fvr_ncc_*``; gradients for both arguments."""
+
+    @staticmethod
+    def forward(ctx, pred, target):
+        L = _lib.lib()
+        B, K = pred.shape
+        Bt = target.shape[0]
+        dev = pred.device
+        st = _lib.stream_ptr(dev)
+        work = torch.empty(int(L.fvr_ncc_work_doubles(B, K)), device=dev, dtype=torch.float64)
+        val = torch.empty(B, device=dev, dtype=torch.float64)
+        coef = torch.empty((B, 8), device=dev, dtype=torch.float32)
+        _lib.check(L.fvr_ncc_fwd(_lib.ptr(pred), B, _lib.ptr(target), Bt, K, _lib.ptr(work), _lib.ptr(val),
+                                 _lib.ptr(coef), st), "fvr_ncc_fwd")
+        ctx.save_for_backward(pred, target, coef)
+        return val
+
+    @staticmethod
+    def backward(ctx, gval):
+        pred, target, coef = ctx.saved_tensors
+        L = _lib.lib()
+        B, K = pred.shape
+        Bt = target.shape[0]
+        st = _lib.stream_ptr(pred.device)
+        gv = gval.to(torch.float64).contiguous()
+        gp = gt = None
+        for wrt in (0, 1):
+            if not ctx.needs_input_grad[wrt]:
+                continue
+            g = torch.empty((B, K), device=pred.device, dtype=torch.float32)
+            _lib.check(L.fvr_ncc_bwd(_lib.ptr(pred), B, _lib.ptr(target), Bt, K, _lib.ptr(coef), _lib.ptr(gv), wrt,
+                                     _lib.ptr(g), st), "fvr_ncc_bwd")
+            if wrt == 0:
+                gp = g
+            else:
+                gt = g.sum(0, keepdim=True) if Bt == 1 and B > 1 else g
+        return gp, gt
+
+
+def image_ncc(pred: torch.Tensor, target: torch.Tensor) -> torch.Tensor:
+    """Zero-mean NCC of each image of pred (B, ...) against target (B or 1, ...) of the same
+    trailing shape: sum(p0 t0) / (|p0| |t0| + 1e-8) with the means removed -- the NCC of the
+    reference's PoU registration test loss (tests/test_cone_beam_fvr.py:24-27, 1 - NCC) -- as
+    (B,) values in pred's dtype, on the HIP kernels ``fvr_ncc_*`` (float32 images, float64 sums
+    in a fixed order); differentiable in both arguments."""
+    if pred.dim() < 2 or target.dim() != pred.dim() or target.shape[1:] != pred.shape[1:] or \
+            target.shape[0] not in (1, pred.shape[0]):
+        raise ValueError(f"image_ncc: target {tuple(target.shape)} does not match pred {tuple(pred.shape)}")
+    _lib.require_device(pred, "image_ncc")
+    B = pred.shape[0]
+    p = pred.to(torch.float32).reshape(B, -1).contiguous()
+    t = target.to(device=pred.device, dtype=torch.float32).reshape(target.shape[0], -1).contiguous()
+    return _ImageNCC.apply(p, t).to(pred.dtype)
+
+
 def sample_isocenter_parity_offsets(vox_spacing) -> torch.Tensor:
     """{-0.5, 0, 0.5}^3 voxel-centre parity offsets scaled by the spacing (cone_beam_fvr.py:122-137)."""
     sp = torch.as_tensor(vox_spacing, dtype=torch.float32)

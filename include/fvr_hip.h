@@ -42,6 +42,8 @@
  *                       block offsets) and their adjoint to the pose matrices.
  *   fvr_edge_*          cone_beam_fvr.py:71-112 edge_dominant_ncc (forward moments,
  *                       combine, and the gradient w.r.t. the images).
+ *   fvr_ncc_*           tests/test_cone_beam_fvr.py:24-27 _ncc_loss (the zero-mean
+ *                       NCC of whole images) and its gradient.
  */
 #ifndef FVR_HIP_H
 #define FVR_HIP_H
@@ -493,6 +495,23 @@ int fvr_edge_combine(const double* partials, int32_t n_img, const double* target
 int fvr_edge_bwd(const float* img, int32_t n_img, int32_t H, int32_t W, int32_t crop, int32_t multi,
                  const float* channels, int32_t n_ch_img, const float* coef, const double* grad_value,
                  float* grad, void* stream);
+
+/* Zero-mean NCC of whole images, the NCC of the PoU path's registration loss (the
+ * reference's tests/test_cone_beam_fvr.py:24-27 _ncc_loss = 1 - NCC, written there in torch):
+ * n_img float32 images of K pixels against n_target (1 or n_img) targets,
+ * value[n] = sum(u0 v0) / (|u0| |v0| + 1e-8), u0 / v0 less their means; float64 sums in a
+ * fixed order (deterministic).
+ *   fvr_ncc_work_doubles: size of the work array of fvr_ncc_fwd (-1 for bad sizes)
+ *   fvr_ncc_fwd:  value[n] (float64) and coef[n][8] = (alpha, beta_u, mean u, mean v,
+ *                 alpha, beta_v, mean v, mean u)
+ *   fvr_ncc_bwd:  grad[n][K] = grad_value[n] x d value[n] / d image n (wrt_target = 0) or
+ *                 d value[n] / d its target (wrt_target = 1; summed over n by the caller
+ *                 for a shared target): alpha (other - mean) - beta (self - mean) */
+int64_t fvr_ncc_work_doubles(int32_t n_img, int64_t K);
+int fvr_ncc_fwd(const float* img, int32_t n_img, const float* target, int32_t n_target, int64_t K, double* work,
+                double* value, float* coef, void* stream);
+int fvr_ncc_bwd(const float* img, int32_t n_img, const float* target, int32_t n_target, int64_t K,
+                const float* coef, const double* grad_value, int32_t wrt_target, float* grad, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* misc                                                                      */

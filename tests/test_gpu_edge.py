@@ -131,3 +131,58 @@ def test_edge_ncc_through_the_pinhole_projection_matches_oracle_fd():
             mm[k, i, 3] -= h
             fd[k, i] = (loss64(mp) - loss64(mm)) / (2 * h)
     assert np.abs(g[:, :3, 3] - fd).max() <= 2e-3 * np.abs(fd).max(), (g[:, :3, 3], fd)
+
+
+def _ncc_torch64(p, t):
+    p0 = p - p.mean(dim=tuple(range(1, p.dim())), keepdim=True)
+    t0 = t - t.mean(dim=tuple(range(1, t.dim())), keepdim=True)
+    dims = tuple(range(1, p.dim()))
+    return (p0 * t0).sum(dim=dims) / (p0.square().sum(dim=dims).sqrt() * t0.square().sum(dim=dims).sqrt() + 1e-8)
+
+
+@pytest.mark.parametrize("B,shape,Bt", [(3, (1, 50, 61), 3), (3, (1, 50, 61), 1), (2, (1, 100, 104), 1),
+                                        (4, (7, 9), 4), (64, (1, 256, 256), 1)])
+def test_image_ncc_values_and_gradients(B, shape, Bt):
+    """image_ncc (the reference test's NCC, tests/test_cone_beam_fvr.py:24-27) against the
+    FP64 oracle's ncc_loss per image and float64 torch autograd of the same formula for both
+    arguments: float32 images (rows of K % 4 != 0 take the scalar loads, K > 4096 several
+    chunks); values within 1e-6, gradients within 1e-5 of max|grad|."""
+    from fvr import image_ncc
+    rng = np.random.default_rng(B + Bt + shape[-1])
+    p = (rng.standard_normal((B,) + shape).cumsum(-1) * 0.1).astype(np.float32)
+    t = (rng.standard_normal((Bt,) + shape).cumsum(-2) * 0.1 + 0.3 * p[:Bt]).astype(np.float32)
+    P = torch.from_numpy(p).to(DEV).requires_grad_(True)
+    T = torch.from_numpy(t).to(DEV).requires_grad_(True)
+    val = image_ncc(P, T)
+    assert val.shape == (B,) and val.dtype == torch.float32
+    w = torch.from_numpy(rng.uniform(-1.0, 1.0, B).astype(np.float32)).to(DEV)
+    (val * w).sum().backward()
+    ref = np.array([1.0 - C.ncc_loss(p[n].astype(np.float64), t[n if Bt > 1 else 0].astype(np.float64))
+                    for n in range(B)])
+    assert np.abs(val.detach().double().cpu().numpy() - ref).max() <= 1e-6
+    P64 = torch.from_numpy(p.astype(np.float64)).requires_grad_(True)
+    T64 = torch.from_numpy(t.astype(np.float64)).requires_grad_(True)
+    (_ncc_torch64(P64, T64.expand_as(P64)) * w.double().cpu()).sum().backward()
+    for got, exp in ((P.grad, P64.grad), (T.grad, T64.grad)):
+        e = exp.numpy()
+        assert got.shape == exp.shape
+        assert np.abs(got.double().cpu().numpy() - e).max() <= 1e-5 * np.abs(e).max()
+
+
+def test_image_ncc_constant_image_and_shape_checks():
+    """A constant image has NCC 0 (the 1e-8 of the denominator) and a finite gradient (no
+    |p0| term); mismatched shapes raise before any launch."""
+    from fvr import image_ncc
+    p = torch.full((2, 1, 16, 16), 0.5, device=DEV, requires_grad=True)
+    t = torch.randn((1, 1, 16, 16), generator=torch.Generator().manual_seed(2)).to(DEV)
+    val = image_ncc(p, t)
+    assert torch.all(val == 0)
+    val.sum().backward()
+    g = p.grad.double().cpu().numpy()
+    td = t.double().cpu().numpy()
+    e = (td - td.mean()) / 1e-8
+    assert np.all(np.isfinite(g)) and np.abs(g - e).max() <= 1e-5 * np.abs(e).max()
+    with pytest.raises(ValueError):
+        image_ncc(p, torch.zeros((1, 1, 16, 15), device=DEV))
+    with pytest.raises(ValueError):
+        image_ncc(p, torch.zeros((3, 1, 16, 16), device=DEV))
