@@ -1598,11 +1598,39 @@ __global__ __launch_bounds__(1024) void pou_gprep_kernel(const float* __restrict
   const float* c = ic + size_t(n) * ray_stride;
   float* xo = X + size_t(n) * K;
   bool nf = false;
-  for (int q = threadIdx.x; q < K; q += 1024) {
-    const float x = g[q] * c[q];
-    xo[q] = x;
+  auto one = [&](float x) {
     nf |= !(fabsf(x) <= 3.40282347e38f);
+    return x;
+  };
+  // one workgroup streams a pose: 16-byte loads, four in flight per thread, when the rows allow
+  const bool vec = (K % 4 == 0) && ((reinterpret_cast<uintptr_t>(g) | reinterpret_cast<uintptr_t>(c) |
+                                     reinterpret_cast<uintptr_t>(xo)) % 16 == 0);
+  int q0 = 0;
+  if (vec) {
+    const int K4 = K / 4;
+    const float4* g4 = reinterpret_cast<const float4*>(g);
+    const float4* c4 = reinterpret_cast<const float4*>(c);
+    float4* x4 = reinterpret_cast<float4*>(xo);
+    int q = threadIdx.x;
+    for (; q + 3 * 1024 < K4; q += 4 * 1024) {
+      float4 a[4], b[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        a[t] = g4[q + t * 1024];
+        b[t] = c4[q + t * 1024];
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        x4[q + t * 1024] = make_float4(one(a[t].x * b[t].x), one(a[t].y * b[t].y), one(a[t].z * b[t].z),
+                                       one(a[t].w * b[t].w));
+    }
+    for (; q < K4; q += 1024) {
+      const float4 a = g4[q], b = c4[q];
+      x4[q] = make_float4(one(a.x * b.x), one(a.y * b.y), one(a.z * b.z), one(a.w * b.w));
+    }
+    q0 = K;
   }
+  for (int q = q0 + threadIdx.x; q < K; q += 1024) xo[q] = one(g[q] * c[q]);
   if (__any(nf) && (threadIdx.x & 63) == 0) bad = 1;
   __syncthreads();
   if (threadIdx.x == 0) flag[n] = bad ? __builtin_nanf("") : 0.f;
