@@ -352,6 +352,171 @@ __device__ void fft_lines_dif4(float2* a, int nlines, int L, int logL, int lpitc
   }
 }
 
+// ---------------------------------------------------------------------------
+// P = 128: the 2D transforms with register DFTs, two LDS passes per dimension
+// (the LDS array: 128 rows x 65 columns, pitch 65).  SIGN = +1: exponent +2 pi i n k / n
+// (inverse), -1: forward; unnormalised.
+//   columns (length 128 = 16 x 8): pass A, task (column, n1 < 8), a 16-point DFT over
+//   n2 of x[n1 + 8 n2] times W128^(n1 k2) back to row n1 + 8 k2; pass B, task (column,
+//   k2 < 16), an 8-point DFT over the rows 8 k2 .. 8 k2 + 7: X[k2 + 16 k1] at row
+//   8 k2 + k1 (pou_row128).
+//   rows (length 64 = 8 x 8): the same with 8-point DFTs: X[k2 + 8 k1] at column 8 k2 + k1
+//   (pou_col64).
+// Columns 0 and 64 of a half spectrum are both real after the transform (inverse) or before
+// it (forward), so they share one complex transform: column 0 carries col0 + i col64.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 w) {
+  return make_float2(fmaf(a.x, w.x, -a.y * w.y), fmaf(a.x, w.y, a.y * w.x));
+}
+// a * (SIGN i)
+template <int SIGN>
+__device__ __forceinline__ float2 cmul_i(float2 a) {
+  return SIGN > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+// a * (c + SIGN i s)
+template <int SIGN>
+__device__ __forceinline__ float2 cmul_cs(float2 a, float c, float s) {
+  const float ss = SIGN > 0 ? s : -s;
+  return make_float2(fmaf(a.x, c, -a.y * ss), fmaf(a.x, ss, a.y * c));
+}
+
+template <int SIGN>
+__device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
+  const float2 s02 = cadd(a0, a2), d02 = csub(a0, a2), s13 = cadd(a1, a3), d13 = cmul_i<SIGN>(csub(a1, a3));
+  a0 = cadd(s02, s13);
+  a2 = csub(s02, s13);
+  a1 = cadd(d02, d13);
+  a3 = csub(d02, d13);
+}
+
+constexpr float kR2 = 0.70710678118654752f;  // 1 / sqrt 2
+constexpr float kC8 = 0.92387953251128676f;  // cos(pi / 8)
+constexpr float kS8 = 0.38268343236508977f;  // sin(pi / 8)
+
+// natural order in and out: even / odd 4-point DFTs, twiddles w8^k
+template <int SIGN>
+__device__ __forceinline__ void dft8(float2 (&v)[8]) {
+  float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6], o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+  dft4<SIGN>(e0, e1, e2, e3);
+  dft4<SIGN>(o0, o1, o2, o3);
+  o1 = cmul_cs<SIGN>(o1, kR2, kR2);
+  o2 = cmul_i<SIGN>(o2);
+  o3 = cmul_cs<SIGN>(o3, -kR2, kR2);
+  v[0] = cadd(e0, o0);
+  v[4] = csub(e0, o0);
+  v[1] = cadd(e1, o1);
+  v[5] = csub(e1, o1);
+  v[2] = cadd(e2, o2);
+  v[6] = csub(e2, o2);
+  v[3] = cadd(e3, o3);
+  v[7] = csub(e3, o3);
+}
+
+// natural order in and out: n = n1 + 4 n2, k = k2 + 4 k1; 4-point DFTs over n2, twiddles
+// w16^(n1 k2), 4-point DFTs over n1
+template <int SIGN>
+__device__ __forceinline__ void dft16(float2 (&v)[16]) {
+#pragma unroll
+  for (int n1 = 0; n1 < 4; ++n1) dft4<SIGN>(v[n1], v[n1 + 4], v[n1 + 8], v[n1 + 12]);
+  // v[n1 + 4 k2] now holds A[n1][k2]
+  v[1 + 4 * 1] = cmul_cs<SIGN>(v[5], kC8, kS8);     // w16^1
+  v[1 + 4 * 2] = cmul_cs<SIGN>(v[9], kR2, kR2);     // w16^2
+  v[1 + 4 * 3] = cmul_cs<SIGN>(v[13], kS8, kC8);    // w16^3
+  v[2 + 4 * 1] = cmul_cs<SIGN>(v[6], kR2, kR2);     // w16^2
+  v[2 + 4 * 2] = cmul_i<SIGN>(v[10]);               // w16^4
+  v[2 + 4 * 3] = cmul_cs<SIGN>(v[14], -kR2, kR2);   // w16^6
+  v[3 + 4 * 1] = cmul_cs<SIGN>(v[7], kS8, kC8);     // w16^3
+  v[3 + 4 * 2] = cmul_cs<SIGN>(v[11], -kR2, kR2);   // w16^6
+  v[3 + 4 * 3] = cmul_cs<SIGN>(v[15], -kC8, -kS8);  // w16^9
+#pragma unroll
+  for (int k2 = 0; k2 < 4; ++k2) dft4<SIGN>(v[4 * k2], v[4 * k2 + 1], v[4 * k2 + 2], v[4 * k2 + 3]);
+  // v[4 k2 + k1] holds X[k2 + 4 k1]: transpose to natural order
+  float2 t[16];
+#pragma unroll
+  for (int k2 = 0; k2 < 4; ++k2)
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) t[k2 + 4 * k1] = v[4 * k2 + k1];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) v[k] = t[k];
+}
+
+// W128^e with the sign of the transform, e in [0, 128), from tw[k] = exp(-2 pi i k / 128), k < 64
+template <int SIGN>
+__device__ __forceinline__ float2 tw128(const float2* tw, int e) {
+  float2 w = tw[e & 63];
+  if (e & 64) w = make_float2(-w.x, -w.y);
+  if (SIGN > 0) w.y = -w.y;
+  return w;
+}
+
+// storage row of frequency m after the column transform / column of n after the row transform
+__device__ __forceinline__ int pou_row128(int m) { return 8 * (m & 15) + (m >> 4); }
+__device__ __forceinline__ int pou_col64(int n) { return 8 * (n & 7) + (n >> 3); }
+
+// the 64 column lines of a 128 x 65 half spectrum (column 0 packing col0 + i col64 when pack)
+template <int SIGN, int NT>
+__device__ void pou_cols128(float2* a, const float2* tw, bool pack) {
+  constexpr int kP = 65;
+  for (int task = threadIdx.x; task < 512; task += NT) {  // pass A: (v, n1)
+    const int v = task & 63, n1 = task >> 6;
+    float2 x[16];
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) {
+      x[n2] = a[(n1 + 8 * n2) * kP + v];
+      if (pack && v == 0) {
+        const float2 h = a[(n1 + 8 * n2) * kP + 64];
+        x[n2] = make_float2(x[n2].x - h.y, x[n2].y + h.x);
+      }
+    }
+    dft16<SIGN>(x);
+#pragma unroll
+    for (int k2 = 1; k2 < 16; ++k2) x[k2] = cmul(x[k2], tw128<SIGN>(tw, n1 * k2));
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) a[(n1 + 8 * k2) * kP + v] = x[k2];
+  }
+  __syncthreads();
+  for (int task = threadIdx.x; task < 1024; task += NT) {  // pass B: (v, k2)
+    const int v = task & 63, k2 = task >> 6;
+    float2 y[8];
+#pragma unroll
+    for (int n1 = 0; n1 < 8; ++n1) y[n1] = a[(8 * k2 + n1) * kP + v];
+    dft8<SIGN>(y);
+#pragma unroll
+    for (int k1 = 0; k1 < 8; ++k1) a[(8 * k2 + k1) * kP + v] = y[k1];
+  }
+  __syncthreads();
+}
+
+// the 128 row lines (length 64) of a 128 x 65 array
+template <int SIGN, int NT>
+__device__ void pou_rows64(float2* a, const float2* tw) {
+  constexpr int kP = 65;
+  for (int task = threadIdx.x; task < 1024; task += NT) {  // pass A: (row, n1)
+    const int r = task & 127, n1 = task >> 7;
+    float2 x[8];
+#pragma unroll
+    for (int n2 = 0; n2 < 8; ++n2) x[n2] = a[r * kP + n1 + 8 * n2];
+    dft8<SIGN>(x);
+#pragma unroll
+    for (int k2 = 1; k2 < 8; ++k2) x[k2] = cmul(x[k2], tw128<SIGN>(tw, 2 * n1 * k2));
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) a[r * kP + n1 + 8 * k2] = x[k2];
+  }
+  __syncthreads();
+  for (int task = threadIdx.x; task < 1024; task += NT) {  // pass B: (row, k2)
+    const int r = task & 127, k2 = task >> 7;
+    float2 y[8];
+#pragma unroll
+    for (int n1 = 0; n1 < 8; ++n1) y[n1] = a[r * kP + 8 * k2 + n1];
+    dft8<SIGN>(y);
+#pragma unroll
+    for (int k1 = 0; k1 < 8; ++k1) a[r * kP + 8 * k2 + k1] = y[k1];
+  }
+  __syncthreads();
+}
+
 // trilinear value (and gradient) from the 8 corners (c = x + 2 y + 4 z) and the fractions:
 // lerps along x (the difference d = high - low kept), then y, then z; the x-gradient is the
 // bilinear (y, z) interpolation of the four x-differences, the y-gradient the z-lerp of the two
@@ -606,14 +771,23 @@ __global__ __launch_bounds__(kPouHalfThreads, 4) void pou_slice_ifft_kernel(cons
     }
   }
   __syncthreads();
-  // inverse FFTs of the P/2 + 1 columns (length P): Y at rows brev(m)
-  fft_lines_dif4<+1, true>(a, pitch, P, logP, 1, pitch, tw, 1);
+  // inverse FFTs of the P/2 + 1 columns (length P): Y at rows brev(m); at P = 128 the register
+  // transforms, Y at rows pou_row128(m), column 0 holding Y(0) + i Y(P/2) (both real)
+  const bool r128 = P == 128;
+  if (r128)
+    pou_cols128<+1, kPouHalfThreads>(a, tw, true);
+  else
+    fft_lines_dif4<+1, true>(a, pitch, P, logP, 1, pitch, tw, 1);
   // even / odd pre-pass per row: pairs (kk, P/2 - kk), kk in [0, P/4]
   const int qp = P >> 2, npair = P * (qp + 1);
   for (int idx = threadIdx.x; idx < npair; idx += stride) {
     const int r = idx / (qp + 1), kk = idx - r * (qp + 1);
     float2* row = a + r * pitch;
-    const float2 y0 = row[kk], y1 = row[hp - kk];
+    float2 y0 = row[kk], y1 = row[hp - kk];
+    if (r128 && kk == 0) {  // the packed pair
+      y1 = make_float2(y0.y, 0.f);
+      y0 = make_float2(y0.x, 0.f);
+    }
     auto zk = [&](float2 ya, float2 yb, int e) {  // Z(e) from Y(e), Y(P/2 - e)
       const float er = ya.x + yb.x, ei = ya.y - yb.y;  // E = Y(e) + conj Y(P/2 - e)
       const float dr = ya.x - yb.x, di = ya.y + yb.y;  // Y(e) - conj Y(P/2 - e)
@@ -626,13 +800,18 @@ __global__ __launch_bounds__(kPouHalfThreads, 4) void pou_slice_ifft_kernel(cons
   }
   __syncthreads();
   // inverse FFTs of the rows (length P/2): z(n) = x(2n) + i x(2n + 1) at column brev(n)
-  fft_lines_dif4<+1, false>(a, P, hp, logH, pitch, 1, tw, 2);
+  // (pou_col64(n) at P = 128)
+  if (r128)
+    pou_rows64<+1, kPouHalfThreads>(a, tw);
+  else
+    fft_lines_dif4<+1, false>(a, P, hp, logH, pitch, 1, tw, 2);
   const float inv = 1.f / float(P * P);
   float* out = imgs + size_t(bn) * P * P;
   for (int q = threadIdx.x; q < P * P; q += stride) {
     const int i = q >> logP, j = q & (P - 1);
     const int m = (i + hp) & (P - 1), nn = (j + hp) & (P - 1);  // fftshift
-    const float2 z = a[brev(m, logP) * pitch + brev(nn >> 1, logH)];
+    const float2 z = r128 ? a[pou_row128(m) * pitch + pou_col64(nn >> 1)]
+                          : a[brev(m, logP) * pitch + brev(nn >> 1, logH)];
     out[q] = ((nn & 1) ? z.y : z.x) * inv;
   }
 }
@@ -995,7 +1174,11 @@ __device__ __forceinline__ void pou_adjoint_tail(float2* a, const float2* tw, fl
                                                  const float* __restrict__ u, const float* __restrict__ v,
                                                  const float (&s3)[3], float fl, float* __restrict__ part) {
   const int P = sg.P, logP = sg.logP, hp = P / 2, hpitch = hp + 1, logH = logP - 1;
-  fft_lines_dif4<-1, false>(a, P, hp, logH, hpitch, 1, tw, 2);
+  const bool r128 = P == 128;  // the register transforms: outputs at pou_col64 / pou_row128
+  if (r128)
+    pou_rows64<-1, NT>(a, tw);
+  else
+    fft_lines_dif4<-1, false>(a, P, hp, logH, hpitch, 1, tw, 2);
   {
     const int qp = P >> 2, npair = P * (qp + 1);
     constexpr int kPost = (128 * (128 / 4 + 1) + NT - 1) / NT;
@@ -1006,7 +1189,9 @@ __device__ __forceinline__ void pou_adjoint_tail(float2* a, const float2* tw, fl
       if (idx < npair) {
         const int r = idx / (qp + 1), kk = idx - r * (qp + 1);
         const float2* row = a + r * hpitch;
-        const float2 z0 = row[brev(kk, logH)], z1 = row[brev((hp - kk) & (hp - 1), logH)];
+        const int c0 = r128 ? pou_col64(kk) : brev(kk, logH);
+        const int c1 = r128 ? pou_col64((hp - kk) & (hp - 1)) : brev((hp - kk) & (hp - 1), logH);
+        const float2 z0 = row[c0], z1 = row[c1];
         auto xk = [&](float2 za, float2 zb, float2 w) {  // X from Z(k), Z(P/2 - k), w = e^{-2 pi i k / P}
           const float er = 0.5f * (za.x + zb.x), ei = 0.5f * (za.y - zb.y);  // (Z + conj Z') / 2
           const float dr = 0.5f * (za.x - zb.x), di = 0.5f * (za.y + zb.y);  // (Z - conj Z') / 2
@@ -1030,7 +1215,27 @@ __device__ __forceinline__ void pou_adjoint_tail(float2* a, const float2* tw, fl
     }
     __syncthreads();
   }
-  fft_lines_dif4<-1, true>(a, hpitch, P, logP, 1, hpitch, tw, 1);
+  if (r128) {
+    // columns 0 and P/2 are real (the row transform's X(0), X(P/2) of real rows): one complex
+    // transform of col0 + i col64, F = G0 + i G64, then G0 = (F(u) + conj F(-u)) / 2 and
+    // G64 = -i (F(u) - conj F(-u)) / 2
+    pou_cols128<-1, NT>(a, tw, true);
+    const int u = threadIdx.x;
+    float2 g0 = make_float2(0.f, 0.f), g64 = g0;
+    if (u < 128) {
+      const float2 f = a[pou_row128(u) * hpitch], fm = a[pou_row128((128 - u) & 127) * hpitch];
+      g0 = make_float2(0.5f * (f.x + fm.x), 0.5f * (f.y - fm.y));
+      g64 = make_float2(0.5f * (f.y + fm.y), -0.5f * (f.x - fm.x));
+    }
+    __syncthreads();
+    if (u < 128) {
+      a[pou_row128(u) * hpitch] = g0;
+      a[pou_row128(u) * hpitch + 64] = g64;
+    }
+    __syncthreads();
+  } else {
+    fft_lines_dif4<-1, true>(a, hpitch, P, logP, 1, hpitch, tw, 1);
+  }
   // every slice sample recomputed and contracted with its gradient.  The sum runs over one
   // representative of each DFT-mirror pair (the columns j >= P/2, then column 0; on the
   // self-mirrored columns j = P/2 and 0 the rows i > P/2 with i = 0, P/2): where the mirror is
@@ -1068,7 +1273,8 @@ __device__ __forceinline__ void pou_adjoint_tail(float2* a, const float2* tw, fl
     // G at frequency (uu, vv) (natural order): stored for vv <= P/2, else conj G(-uu, -vv)
     const int uu = (i + hp) & (P - 1), vv = (j + hp) & (P - 1);
     const bool lo = vv <= hp;
-    const float2 gs = a[brev(lo ? uu : (P - uu) & (P - 1), logP) * hpitch + (lo ? vv : P - vv)];
+    const int gu_ = lo ? uu : (P - uu) & (P - 1);
+    const float2 gs = a[(r128 ? pou_row128(gu_) : brev(gu_, logP)) * hpitch + (lo ? vv : P - vv)];
     const float gr = gs.x * inv * w, gi = (lo ? gs.y : -gs.y) * inv * w;
     float cr[8], ci[8], re, im, dre[3], dim[3];
     pk_corners<VT>(sm, cr, ci);
