@@ -1703,6 +1703,27 @@ __global__ __launch_bounds__(kPouSepThreads, 4) void pou_render_bwd_sep_kernel(
         float cc[NXT > 0 ? NXT : 1], dd[NXT > 0 ? NXT : 1], ee[NXT > 0 ? NXT : 1];
         if constexpr (NXT > 0) coefs(nxt, cc, dd, ee);
         // the staged window of a fetched row, then its column sums and texel updates
+        // the current run of rows with one fy (wave-uniform): its texel rows' sums acc0, acc1,
+        // whether the wave's segment owns them, their slab-image values, and the geometry sums
+        // t (d/dpx, d/dpy terms), d (d/dsx), sa (d/dsy) that the run's texel rows multiply
+        int rf = -0x7fffffff;
+        bool own0 = false, own1 = false;
+        float rI0 = 0.f, rI1 = 0.f, acc0 = 0.f, acc1 = 0.f, t0 = 0.f, t1 = 0.f, d0 = 0.f, d1 = 0.f, sa = 0.f;
+        auto flush = [&]() {
+          if (own0) {  // texel row rf: weight 1 - wy, d/dpy -1
+            if (colok) gimg[2 * hpitch * ((rf + hp) & (P - 1)) + gcol] += acc0;
+            s0 = fmaf(rI0, t0, s0);
+            s1 = fmaf(-rI0, d0, s1);
+            s2 = fmaf(rI0, sa, s2);
+          }
+          if (own1) {  // texel row rf + 1: weight wy, d/dpy +1
+            if (colok) gimg[2 * hpitch * ((rf + 1 + hp) & (P - 1)) + gcol] += acc1;
+            s0 = fmaf(rI1, t1, s0);
+            s1 = fmaf(-rI1, d1, s1);
+            s2 = fmaf(-rI1, sa, s2);
+          }
+          acc0 = acc1 = t0 = t1 = d0 = d1 = sa = 0.f;
+        };
         auto process = [&](const Row& r) {
 #pragma unroll
           for (int c = 0; c < kSepChunks; ++c)
@@ -1727,7 +1748,26 @@ __global__ __launch_bounds__(kPouSepThreads, 4) void pou_render_bwd_sep_kernel(
               Br = fmaf(e, xv, Br);
             }
           }
-          accumulate(r.fy, r.wy, r.ry, A, Bd, Br, r.I0, r.I1);
+          // the row's texel rows fy, fy + 1 are the wave's (one detector row per wave step):
+          // consecutive rows of one fy form a run whose texel and geometry terms are summed in
+          // registers and added once, at the run's end
+          const int fy = __builtin_amdgcn_readfirstlane(r.fy);
+          if (fy != rf) {
+            flush();
+            rf = fy;
+            own0 = unsigned(fy) < unsigned(P) && __builtin_amdgcn_readfirstlane(sown[fy]) == seg;
+            own1 = unsigned(fy + 1) < unsigned(P) && __builtin_amdgcn_readfirstlane(sown[fy + 1]) == seg;
+            rI0 = r.I0;
+            rI1 = r.I1;
+          }
+          const float w0 = 1.f - r.wy, w1 = r.wy;
+          acc0 = fmaf(w0, A, acc0);
+          acc1 = fmaf(w1, A, acc1);
+          t0 = fmaf(w0, Br, fmaf(-r.ry, A, t0));
+          t1 = fmaf(w1, Br, fmaf(r.ry, A, t1));
+          d0 = fmaf(w0, Bd, d0);
+          d1 = fmaf(w1, Bd, d1);
+          sa += A;
         };
         Row ra, rb;
         rb.fy = -0x7fffffff;
@@ -1740,6 +1780,7 @@ __global__ __launch_bounds__(kPouSepThreads, 4) void pou_render_bwd_sep_kernel(
           if (y + 2 <= ye) fetch(ra, y + 2, rb);
           process(rb);
         }
+        flush();
       };
       if (nxw <= 8) run(std::integral_constant<int, 8>{});
       else if (nxw <= kSepNX) run(std::integral_constant<int, kSepNX>{});
