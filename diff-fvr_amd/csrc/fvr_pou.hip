@@ -837,6 +837,11 @@ __device__ __forceinline__ pou_gcptr pou_global(const void* p) {
 __device__ __forceinline__ float pou_ld(pou_gcptr base, uint32_t byte_off) {
   return *reinterpret_cast<const float __attribute__((address_space(1)))*>(base + byte_off);
 }
+typedef float pou_f4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 pou_ld4(pou_gcptr base, uint32_t byte_off) {
+  const pou_f4 v = *reinterpret_cast<const pou_f4 __attribute__((address_space(1)))*>(base + byte_off);
+  return make_float4(v.x, v.y, v.z, v.w);
+}
 
 // the bilinear lerps of the warp with their roundings fixed (the x-lerps, then the y-lerp), so that
 // the per-pixel and the tiled forward kernels give the same value bit for bit
@@ -1659,7 +1664,10 @@ __global__ __launch_bounds__(kPouSepThreads, 4) void pou_render_bwd_sep_kernel(
       s2 = fmaf(-I1, A, s2);
     }
   };
-  auto img_at_row = [&](int ty) { return colok && unsigned(ty) < unsigned(P) ? img[ty * P + tx] : 0.f; };
+  const pou_gcptr imgg = pou_global(img);  // a uniform base and 32-bit offsets (no 64-bit lane pointers)
+  auto img_at_row = [&](int ty) {
+    return colok && unsigned(ty) < unsigned(P) ? pou_ld(imgg, uint32_t(ty * P + tx) * 4u) : 0.f;
+  };
   const int x0a = wlo & ~3;
   const int nch = whi >= 0 ? (whi - x0a) / 256 + 1 : 0;
   if (staged && nch <= kSepChunks) {
@@ -1680,12 +1688,12 @@ __global__ __launch_bounds__(kPouSepThreads, 4) void pou_render_bwd_sep_kernel(
       };
       // row y's window and position, and its image values (those of `prev` when fy repeats)
       auto fetch = [&](Row& r, int y, const Row& prev) {
-        const float* rowp = Xn + size_t(y) * W;
+        const pou_gcptr rowp = pou_global(Xn + size_t(y) * W);  // uniform
 #pragma unroll
         for (int c = 0; c < kSepChunks; ++c)
           if (c < nch) {
             const int x = x0a + 256 * c + 4 * lane;  // W % 4 == 0: a group is all in or all out
-            r.win[c] = x < W ? *reinterpret_cast<const float4*>(rowp + x) : make_float4(0.f, 0.f, 0.f, 0.f);
+            r.win[c] = x < W ? pou_ld4(rowp, uint32_t(x) * 4u) : make_float4(0.f, 0.f, 0.f, 0.f);
           }
         rowpos(y, r.fy, r.wy, r.ry);
         if (r.fy == prev.fy) {
