@@ -912,12 +912,19 @@ __global__ __launch_bounds__(256) void pou_warp_fwd_kernel(const float* __restri
 // tile's pixels fall in a patch of the slab image spanning its first to its last column's / row's
 // texels (at most 32 s + 2 texels a side for s texels per pixel); the patch is staged in LDS
 // (zeros outside the image), the x-lerps are formed once per (patch row, tile column), then each
-// pixel takes its y-lerp.  The arithmetic per pixel is that of pou_warp_fwd_kernel (the same
-// lerps in the same order, out-of-image corners 0, blocks summed in order), so the projections are
-// the same bit for bit; a (tile, block) whose patch exceeds kWarpPatch texels a side, or whose
-// geometry is not finite, takes the per-pixel gather instead.
+// pixel takes its y-lerp.  The image positions of the tile's rows and the patch of every block
+// are formed once per workgroup (kWarpBlocks blocks at a time, one thread per (block, row) and
+// per block) into LDS, not by each of the 32 threads of a row for every block.  The
+// arithmetic per pixel is that of pou_warp_fwd_kernel (the same positions, the same lerps in the
+// same order, out-of-image corners 0, blocks summed in order), so the projections are the same
+// bit for bit; a (tile, block) whose patch exceeds kWarpPatch texels a side, or whose geometry is
+// not finite, takes the per-pixel gather instead.
 constexpr int kWarpTile = 32;
 constexpr int kWarpPatch = 16;
+#ifndef FVR_WARP_BLOCKS
+#define FVR_WARP_BLOCKS 16
+#endif
+constexpr int kWarpBlocks = FVR_WARP_BLOCKS;  // blocks whose row positions are in LDS at a time
 
 __global__ __launch_bounds__(256) void pou_warp_fwd_sep_kernel(const float* __restrict__ imgs,
                                                                const float* __restrict__ d_b,
@@ -930,6 +937,8 @@ __global__ __launch_bounds__(256) void pou_warp_fwd_sep_kernel(const float* __re
                                                                float* __restrict__ out) {
   __shared__ float patch[kWarpPatch][kWarpPatch + 1];
   __shared__ float arow[kWarpPatch][kWarpTile + 1];
+  __shared__ float2 rpos[kWarpBlocks][kWarpTile];  // (floor, fraction) of py per (block, tile row)
+  __shared__ int3 tinfo[kWarpBlocks];               // the patch's first texel column, row; 0 skip, 1 patch, 2 per pixel
   const int t = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
   if (t >= g.N * ntile) return;  // whole workgroup
   const int n = t / ntile, tile = t - n * ntile;
@@ -940,66 +949,73 @@ __global__ __launch_bounds__(256) void pou_warp_fwd_sep_kernel(const float* __re
   const int x = xs + cx;
   const bool xok = x < W;
   const float rxv = rx_col[xok ? x : xe];
-  const float rxa = rx_col[xs], rxb = rx_col[xe], rya = ry_row[ys], ryb = ry_row[ye];
-  float ryv[4];
   bool yok[4];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int y = ys + rg + 8 * k;
-    yok[k] = y < H;
-    ryv[k] = ry_row[yok[k] ? y : ye];
-  }
+  for (int k = 0; k < 4; ++k) yok[k] = ys + rg + 8 * k < H;
+  const float rxa = rx_col[xs], rxb = rx_col[xe], rya = ry_row[ys], ryb = ry_row[ye];
   const unsigned P = unsigned(g.P);
   const size_t pp = size_t(g.P) * g.P;
   const float lim = float(kWarpPatch - 1);
   float acc[4] = {0.f, 0.f, 0.f, 0.f};
-  for (int b = 0; b < g.B; ++b) {
-    const int bn = b * g.N + n;
-    const float db = d_b[bn], c_u = cu[bn], c_v = cv[bn];
-    const float px = pou_img_pos(db, rxv, c_u, g.inv_ds, g.c_idx);
-    const float fxf = floorf(px), wx = px - fxf;
-    float fyf[4], wy[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const float py = pou_img_pos(db, ryv[k], c_v, g.inv_ds, g.c_idx);
-      fyf[k] = floorf(py);
-      wy[k] = py - fyf[k];
+  for (int b0 = 0; b0 < g.B; b0 += kWarpBlocks) {
+    const int nb = min(kWarpBlocks, g.B - b0);
+    __syncthreads();  // the previous group's positions are no longer read
+    for (int i = tid; i < nb * kWarpTile; i += 256) {
+      const int bb = i >> 5, c = i & (kWarpTile - 1);
+      const int bn = (b0 + bb) * g.N + n;
+      const float py = pou_img_pos(d_b[bn], ry_row[min(ys + c, ye)], cv[bn], g.inv_ds, g.c_idx);
+      const float fy = floorf(py);
+      rpos[bb][c] = make_float2(fy, py - fy);
     }
-    // the tile's texel columns [flo, fhi] and rows [glo, ghi] (monotone maps: the end pixels)
-    const float pa = pou_img_pos(db, rxa, c_u, g.inv_ds, g.c_idx), pb = pou_img_pos(db, rxb, c_u, g.inv_ds, g.c_idx);
-    const float qa = pou_img_pos(db, rya, c_v, g.inv_ds, g.c_idx), qb = pou_img_pos(db, ryb, c_v, g.inv_ds, g.c_idx);
-    const float flo = floorf(fminf(pa, pb)), fhi = floorf(fmaxf(pa, pb)) + 1.f;
-    const float glo = floorf(fminf(qa, qb)), ghi = floorf(fmaxf(qa, qb)) + 1.f;
-    const bool fit = fhi - flo <= lim && ghi - glo <= lim;  // false for NaN or huge positions
-    if (fit) {
-      // a patch outside the image adds exact zeros: skip it
-      if (fhi < 0.f || flo > float(P - 1) || ghi < 0.f || glo > float(P - 1)) continue;
-      const int tx0 = int(flo), ty0 = int(glo);
+    for (int bb = tid; bb < nb; bb += 256) {
+      // the tile's texel columns [flo, fhi] and rows [glo, ghi] (monotone maps: the end pixels)
+      const int bn = (b0 + bb) * g.N + n;
+      const float db = d_b[bn], c_u = cu[bn], c_v = cv[bn];
+      const float pa = pou_img_pos(db, rxa, c_u, g.inv_ds, g.c_idx), pb = pou_img_pos(db, rxb, c_u, g.inv_ds, g.c_idx);
+      const float qa = pou_img_pos(db, rya, c_v, g.inv_ds, g.c_idx), qb = pou_img_pos(db, ryb, c_v, g.inv_ds, g.c_idx);
+      const float flo = floorf(fminf(pa, pb)), fhi = floorf(fmaxf(pa, pb)) + 1.f;
+      const float glo = floorf(fminf(qa, qb)), ghi = floorf(fmaxf(qa, qb)) + 1.f;
+      const bool fit = fhi - flo <= lim && ghi - glo <= lim;  // false for NaN or huge positions
+      // a patch outside the image adds exact zeros: skipped
+      const bool out_img = fhi < 0.f || flo > float(P - 1) || ghi < 0.f || glo > float(P - 1);
+      tinfo[bb] = make_int3(fit ? int(flo) : 0, fit ? int(glo) : 0, fit ? (out_img ? 0 : 1) : 2);
+    }
+    __syncthreads();
+    for (int bb = 0; bb < nb; ++bb) {
+      const int3 ti = tinfo[bb];  // the same for every thread
+      if (ti.z == 0) continue;
+      const int bn = (b0 + bb) * g.N + n;
       const float* img = imgs + size_t(bn) * pp;
-      {
-        const int pr = tid >> 4, pc = tid & (kWarpPatch - 1);  // 256 threads: one texel each
-        const int ix = tx0 + pc, iy = ty0 + pr;
-        patch[pr][pc] = (unsigned(ix) < P && unsigned(iy) < P) ? img[size_t(iy) * P + ix] : 0.f;
-      }
-      __syncthreads();
-      const int c0 = int(fxf) - tx0;  // in [0, fhi - flo - 1]
+      const float px = pou_img_pos(d_b[bn], rxv, cu[bn], g.inv_ds, g.c_idx);
+      const float fxf = floorf(px), wx = px - fxf;
+      if (ti.z == 1) {
+        const int tx0 = ti.x, ty0 = ti.y;
+        {
+          const int pr = tid >> 4, pc = tid & (kWarpPatch - 1);  // 256 threads: one texel each
+          const int ix = tx0 + pc, iy = ty0 + pr;
+          patch[pr][pc] = (unsigned(ix) < P && unsigned(iy) < P) ? img[size_t(iy) * P + ix] : 0.f;
+        }
+        __syncthreads();
+        const int c0 = int(fxf) - tx0;  // in [0, fhi - flo - 1]
 #pragma unroll
-      for (int r = rg; r < kWarpPatch; r += 8) arow[r][cx] = pou_lerp(patch[r][c0], patch[r][c0 + 1], wx);
-      __syncthreads();
+        for (int r = rg; r < kWarpPatch; r += 8) arow[r][cx] = pou_lerp(patch[r][c0], patch[r][c0 + 1], wx);
+        __syncthreads();
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int r0 = int(fyf[k]) - ty0;  // in [0, ghi - glo - 1]
-        acc[k] += pou_lerp(arow[r0][cx], arow[r0 + 1][cx], wy[k]);
-      }
-    } else {
-      const float* img = imgs + size_t(bn) * pp;
-      const int x0 = int(fxf);
+        for (int k = 0; k < 4; ++k) {
+          const float2 rp = rpos[bb][rg + 8 * k];
+          const int r0 = int(rp.x) - ty0;  // in [0, ghi - glo - 1]
+          acc[k] += pou_lerp(arow[r0][cx], arow[r0 + 1][cx], rp.y);
+        }
+      } else {
+        const int x0 = int(fxf);
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int y0 = int(fyf[k]);
-        const float v00 = img_at(img, g.P, x0, y0), v01 = img_at(img, g.P, x0 + 1, y0);
-        const float v10 = img_at(img, g.P, x0, y0 + 1), v11 = img_at(img, g.P, x0 + 1, y0 + 1);
-        acc[k] += pou_bilerp(v00, v01, v10, v11, wx, wy[k]);
+        for (int k = 0; k < 4; ++k) {
+          const float2 rp = rpos[bb][rg + 8 * k];
+          const int y0 = int(rp.x);
+          const float v00 = img_at(img, g.P, x0, y0), v01 = img_at(img, g.P, x0 + 1, y0);
+          const float v10 = img_at(img, g.P, x0, y0 + 1), v11 = img_at(img, g.P, x0 + 1, y0 + 1);
+          acc[k] += pou_bilerp(v00, v01, v10, v11, wx, rp.y);
+        }
       }
     }
   }
