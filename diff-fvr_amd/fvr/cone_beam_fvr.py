@@ -71,9 +71,14 @@ class _EdgeNCC(torch.autograd.Function):
     (Bt, H, W) targets, Bt in {1, B}, on the HIP kernels (``fvr_edge_*``); gradients for
     both arguments (the value is symmetric in them)."""
 
+    # the reference image's channel images and moments, kept for the reference tensor last seen
+    # (object, version, storage, crop, term): a registration loop against one target forms them once
+    _ref_cache = None
+
     @staticmethod
-    def _moments(img, ref, crop, multi, weight):
-        """(value (B,), coef, channel images of ref) of img against ref."""
+    def _moments(img, ref, crop, multi, weight, ref_key=None):
+        """(value (B,), coef, channel images of ref) of img against ref; ref_key (the caller's
+        target tensor) lets the reference's channels and moments come from the cache."""
         L = _lib.lib()
         dev = img.device
         st = _lib.stream_ptr(dev)
@@ -81,11 +86,25 @@ class _EdgeNCC(torch.autograd.Function):
         Bt = ref.shape[0]
         nch = 4 if multi else 2
         tiles = int(L.fvr_edge_tiles(H, W, crop))
-        ch = torch.empty((Bt, nch, H - 2 * crop, W - 2 * crop), device=dev, dtype=torch.float32)
-        _lib.check(L.fvr_edge_channels(_lib.ptr(ref), Bt, H, W, crop, int(multi), _lib.ptr(ch), st), "fvr_edge_channels")
-        tpart = torch.empty((Bt, tiles, 12), device=dev, dtype=torch.float64)
-        _lib.check(L.fvr_edge_stats(_lib.ptr(ref), Bt, H, W, crop, int(multi), _lib.ptr(ch), Bt, _lib.ptr(tpart), st),
-                   "fvr_edge_stats")
+        key = None
+        if ref_key is not None:
+            key = (id(ref_key), ref_key._version, ref_key.data_ptr(), tuple(ref_key.shape), ref_key.dtype, crop,
+                   bool(multi), H, W)
+            c = _EdgeNCC._ref_cache
+            if c is not None and c[0] == key and c[1]() is ref_key:
+                ch, tpart = c[2], c[3]
+                key = None  # a hit: nothing to store
+            else:
+                c = None
+        if ref_key is None or key is not None:
+            ch = torch.empty((Bt, nch, H - 2 * crop, W - 2 * crop), device=dev, dtype=torch.float32)
+            _lib.check(L.fvr_edge_channels(_lib.ptr(ref), Bt, H, W, crop, int(multi), _lib.ptr(ch), st),
+                       "fvr_edge_channels")
+            tpart = torch.empty((Bt, tiles, 12), device=dev, dtype=torch.float64)
+            _lib.check(L.fvr_edge_stats(_lib.ptr(ref), Bt, H, W, crop, int(multi), _lib.ptr(ch), Bt,
+                                        _lib.ptr(tpart), st), "fvr_edge_stats")
+            if key is not None:
+                _EdgeNCC._ref_cache = (key, weakref.ref(ref_key), ch, tpart)
         part = torch.empty((B, tiles, 12), device=dev, dtype=torch.float64)
         _lib.check(L.fvr_edge_stats(_lib.ptr(img), B, H, W, crop, int(multi), _lib.ptr(ch), Bt, _lib.ptr(part), st),
                    "fvr_edge_stats")
@@ -105,8 +124,8 @@ class _EdgeNCC(torch.autograd.Function):
         return g
 
     @staticmethod
-    def forward(ctx, pred, target, crop, multi, weight):
-        val, coef, ch = _EdgeNCC._moments(pred, target, crop, multi, weight)
+    def forward(ctx, pred, target, crop, multi, weight, target_key=None):
+        val, coef, ch = _EdgeNCC._moments(pred, target, crop, multi, weight, target_key)
         ctx.save_for_backward(pred, target, coef, ch)
         ctx.args = (crop, multi, weight)
         return val
@@ -127,7 +146,7 @@ class _EdgeNCC(torch.autograd.Function):
             gt = _EdgeNCC._grad(tt, ch_p, pred.shape[0], coef_t, gv, crop, multi)
             if target.shape[0] == 1 and pred.shape[0] > 1:
                 gt = gt.sum(0, keepdim=True)
-        return gp, gt, None, None, None
+        return gp, gt, None, None, None, None
 
 
 def edge_dominant_ncc(pred: torch.Tensor, target: torch.Tensor, border_crop: Optional[int] = None,
@@ -151,7 +170,10 @@ def edge_dominant_ncc(pred: torch.Tensor, target: torch.Tensor, border_crop: Opt
     multi = multiscale_weight > 0.0 and W >= 96
     p = pred.to(torch.float32).reshape(B, H, W).contiguous()
     t = target.to(device=pred.device, dtype=torch.float32).reshape(target.shape[0], H, W).contiguous()
-    val = _EdgeNCC.apply(p, t, crop, bool(multi), float(multiscale_weight)).to(pred.dtype)
+    # the target's channels and moments are cached for the target tensor object (its version
+    # counter invalidates the entry when it is changed in place)
+    key = target if not target.requires_grad else None
+    val = _EdgeNCC.apply(p, t, crop, bool(multi), float(multiscale_weight), key).to(pred.dtype)
     return val.mean() if reduction == "mean" else val
 
 

@@ -186,3 +186,20 @@ def test_image_ncc_constant_image_and_shape_checks():
         image_ncc(p, torch.zeros((1, 1, 16, 15), device=DEV))
     with pytest.raises(ValueError):
         image_ncc(p, torch.zeros((3, 1, 16, 16), device=DEV))
+
+
+def test_edge_ncc_target_cache_follows_the_target_tensor():
+    """The target's channel images and moments are kept for the target tensor last seen: a
+    second call equals the first bit for bit, and after the target is changed in place (its
+    version counter moves) the value equals that of a fresh copy of the changed target."""
+    from fvr import edge_dominant_ncc
+    p, t = _images(4, 100, 104, 5)
+    P = torch.from_numpy(p).to(DEV)
+    T = torch.from_numpy(t[:1]).to(DEV)
+    v1 = edge_dominant_ncc(P, T, reduction="none")
+    v2 = edge_dominant_ncc(P, T, reduction="none")
+    assert torch.equal(v1, v2)
+    T.mul_(0.5).add_(0.01 * P[:1])
+    v3 = edge_dominant_ncc(P, T, reduction="none")
+    v4 = edge_dominant_ncc(P, T.clone(), reduction="none")
+    assert torch.equal(v3, v4) and not torch.equal(v1, v3)
