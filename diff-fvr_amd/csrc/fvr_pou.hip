@@ -160,88 +160,6 @@ __device__ __forceinline__ int brev(int x, int logp) {
   return int(__builtin_bitreverse32(uint32_t(x)) >> (32 - logp));
 }
 
-// 2D decimation-in-frequency FFT of a P x P complex array in LDS (row pitch P + 1):
-// natural-order input, bit-reversed output in both dimensions, unnormalised; SIGN = +1
-// inverse, -1 forward; tw[k] = exp(-2 pi i k / P).  Pairs of radix-2 stages run as radix-4
-// passes: stages s and s - 1 of a block of 2^s elements act on the four elements k, k + q,
-// k + 2q, k + 3q (q = 2^(s-2)) -- stage s pairs (k, k + 2q) and (k + q, k + 3q) with
-// twiddles k ts and (k + q) ts (ts = P >> s), stage s - 1 pairs the results within each half
-// with twiddle 2k ts -- so each element is read and written once per two stages (an odd logP
-// ends with one radix-2 pass): the arithmetic of the radix-2 stages with half the LDS passes
-// and barriers.
-template <int SIGN>
-__device__ void fft2d_dif4(float2* a, int P, int logP, int pitch, const float2* tw) {
-  const int hp = P / 2;
-  auto twid = [&](int e) {
-    float2 w = tw[e];
-    if (SIGN > 0) w.y = -w.y;
-    return w;
-  };
-  auto bfly = [&](float2& x0, float2& x1, float2 w) {  // x0 <- x0 + x1, x1 <- (x0 - x1) w
-    const float dr = x0.x - x1.x, di = x0.y - x1.y;
-    x0 = make_float2(x0.x + x1.x, x0.y + x1.y);
-    x1 = make_float2(dr * w.x - di * w.y, dr * w.y + di * w.x);
-  };
-#pragma unroll 1
-  for (int dim = 0; dim < 2; ++dim) {
-    int s = logP;
-#pragma unroll 1
-    for (; s >= 2; s -= 2) {
-      const int q = 1 << (s - 2), ts = P >> s;
-      const int ng = P * P / 4;  // groups of four per pass
-      for (int idx = threadIdx.x; idx < ng; idx += blockDim.x) {
-        int line, g;
-        if (dim == 0) {  // along rows: consecutive threads walk one row
-          line = idx / (P / 4);
-          g = idx - line * (P / 4);
-        } else {  // along columns: consecutive threads take consecutive columns
-          line = idx & (P - 1);
-          g = idx >> logP;
-        }
-        const int k = g & (q - 1);
-        const int i0 = ((g >> (s - 2)) << s) + k;
-        int e[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int i = i0 + r * q;
-          e[r] = dim == 0 ? line * pitch + i : i * pitch + line;
-        }
-        float2 x0 = a[e[0]], x1 = a[e[1]], x2 = a[e[2]], x3 = a[e[3]];
-        bfly(x0, x2, twid(k * ts));
-        bfly(x1, x3, twid((k + q) * ts));
-        const float2 w2 = twid(2 * k * ts);
-        bfly(x0, x1, w2);
-        bfly(x2, x3, w2);
-        a[e[0]] = x0;
-        a[e[1]] = x1;
-        a[e[2]] = x2;
-        a[e[3]] = x3;
-      }
-      __syncthreads();
-    }
-    if (s == 1) {  // the last radix-2 stage (half = 1, twiddle 1)
-      for (int idx = threadIdx.x; idx < P * hp; idx += blockDim.x) {
-        int line, k;
-        if (dim == 0) {
-          line = idx >> (logP - 1);
-          k = idx & (hp - 1);
-        } else {
-          line = idx & (P - 1);
-          k = idx >> logP;
-        }
-        const int i0 = 2 * k;
-        const int a0 = dim == 0 ? line * pitch + i0 : i0 * pitch + line;
-        const int a1 = dim == 0 ? a0 + 1 : a0 + pitch;
-        float2 x0 = a[a0], x1 = a[a1];
-        bfly(x0, x1, twid(0));
-        a[a0] = x0;
-        a[a1] = x1;
-      }
-      __syncthreads();
-    }
-  }
-}
-
 __device__ __forceinline__ void make_twiddles(float2* tw, int P) {
   for (int k = threadIdx.x; k < P / 2; k += blockDim.x) {
     float s, c;
@@ -253,8 +171,12 @@ __device__ __forceinline__ void make_twiddles(float2* tw, int P) {
 // Batched 1D decimation-in-frequency FFTs of length L = 2^logL over nlines lines of an LDS
 // complex array: element i of line l at a[l * lpitch + i * epitch]; natural-order input,
 // bit-reversed output, unnormalised; SIGN = +1 inverse, -1 forward.  Twiddles of length L
-// are tw[e * twstep] with tw[k] = exp(-2 pi i k / P) (twstep = P / L).  Radix-4 passes as in
-// fft2d_dif4 (an odd logL ends with one radix-2 pass).  COLS: consecutive threads take
+// are tw[e * twstep] with tw[k] = exp(-2 pi i k / P) (twstep = P / L).  Pairs of radix-2 stages
+// run as radix-4 passes: stages s and s - 1 of a block of 2^s elements act on the four elements
+// k, k + q, k + 2q, k + 3q (q = 2^(s-2)) -- stage s pairs (k, k + 2q) and (k + q, k + 3q) with
+// twiddles k ts and (k + q) ts (ts = L >> s), stage s - 1 pairs the results within each half with
+// twiddle 2k ts -- so each element is read and written once per two stages (an odd logL ends
+// with one radix-2 pass).  COLS: consecutive threads take
 // consecutive lines (lines adjacent in memory); otherwise consecutive groups of one line.
 template <int SIGN, bool COLS>
 __device__ void fft_lines_dif4(float2* a, int nlines, int L, int logL, int lpitch, int epitch, const float2* tw,
