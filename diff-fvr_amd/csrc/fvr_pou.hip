@@ -52,8 +52,11 @@ typedef uint32_t pu2a4 __attribute__((ext_vector_type(2), aligned(4)));
 
 // Gather layout of the fused kernels (fvr_pou_pack_blocks): per block (P+3) x (P+2) x (P+2)
 // elements (z, y, x; storage index = logical index + 1 with zero margins, the slow axis one
-// zero plane longer: the guard), each element the (re, im) pair -- float2 for f32, two
-// 16-bit values in one 32-bit word for f16 / bf16.  Every trilinear corner of a sample with
+// zero plane longer: the guard), 8 bytes each: the (re, im) pair as float2 for f32, or for
+// f16 / bf16 the (re | im << 16) words of rows y and y + 1 (the row pair), so that a 16-byte
+// load returns a cell's four (y, x) corners in one plane and a sample takes two loads instead
+// of four pair loads (twice the footprint of one word per element: config 4's 55 blocks take
+// 0.97 GB).  Every trilinear corner of a sample with
 // floor(p) in [-1, P-1] is a valid address (out-of-range corners read the zero margins,
 // i.e. grid_sample's per-corner zero padding); a sample with no corner inside reads the
 // guard planes.
@@ -64,7 +67,7 @@ struct PouPk {
 
 __host__ __device__ inline PouPk pou_pk(int P, int vt) {
   PouPk k;
-  k.esz = vt == 0 ? 8u : 4u;
+  k.esz = 8u;  // float2 (f32), or the 16-bit pairs of two mid-axis rows (f16 / bf16)
   k.row = uint32_t(P + 2) * k.esz;
   k.plane = k.row * uint32_t(P + 2);
   k.block = (unsigned long long)k.plane * (unsigned long long)(P + 3);
@@ -77,9 +80,10 @@ template <>
 struct PkRaw<0> {
   pf4a8 q[4];  // corner pairs (z, y) = (0,0), (0,1), (1,0), (1,1): (re x0, im x0, re x1, im x1)
 };
+typedef uint32_t pu4a8 __attribute__((ext_vector_type(4), aligned(8)));
 template <int VT>
 struct PkRaw {
-  pu2a4 q[4];  // the same pairs, 16-bit (re | im << 16) at x0, x1
+  pu4a8 q[2];  // z0, z1: the 16-bit (re | im << 16) entries (y0, x0), (y1, x0), (y0, x1), (y1, x1)
 };
 
 template <int VT>
@@ -119,13 +123,11 @@ __device__ __forceinline__ bool pk_issue(PkSmp<VT>& s, unsigned long long blk, c
     r.q[1] = *reinterpret_cast<gp>(b + o1);
     r.q[2] = *reinterpret_cast<gp>(b + o2);
     r.q[3] = *reinterpret_cast<gp>(b + o3);
-  } else {
-    typedef const pu2a4 __attribute__((address_space(1)))* gp;
+  } else {  // row-pair layout: one 16-byte load per plane holds the cell's four (y, x) corners
+    typedef const pu4a8 __attribute__((address_space(1)))* gp;
     PkRaw<1>& r = reinterpret_cast<PkRaw<1>&>(s.r);
     r.q[0] = *reinterpret_cast<gp>(b + o0);
-    r.q[1] = *reinterpret_cast<gp>(b + o1);
-    r.q[2] = *reinterpret_cast<gp>(b + o2);
-    r.q[3] = *reinterpret_cast<gp>(b + o3);
+    r.q[1] = *reinterpret_cast<gp>(b + o2);
   }
   return pou_edge_idx(X, P) | pou_edge_idx(Y, P) | pou_edge_idx(Z, P);
 }
@@ -147,11 +149,15 @@ __device__ __forceinline__ void pk_corners(const PkSmp<VT>& s, float (&cr)[8], f
       cr[2 * p + 1] = r.q[p].z;
       ci[2 * p + 1] = r.q[p].w;
     } else {
+      // pair p = (z, y): plane z = p >> 1, row y = p & 1; entries (y, x0) at component y, (y, x1)
+      // at component 2 + y of the plane's load
       const PkRaw<1>& r = reinterpret_cast<const PkRaw<1>&>(s.r);
-      cr[2 * p] = pk_deq<VT>(r.q[p].x & 0xFFFFu);
-      ci[2 * p] = pk_deq<VT>(r.q[p].x >> 16);
-      cr[2 * p + 1] = pk_deq<VT>(r.q[p].y & 0xFFFFu);
-      ci[2 * p + 1] = pk_deq<VT>(r.q[p].y >> 16);
+      const pu4a8 v = r.q[p >> 1];
+      const uint32_t lo = (p & 1) ? v.y : v.x, hi = (p & 1) ? v.w : v.z;
+      cr[2 * p] = pk_deq<VT>(lo & 0xFFFFu);
+      ci[2 * p] = pk_deq<VT>(lo >> 16);
+      cr[2 * p + 1] = pk_deq<VT>(hi & 0xFFFFu);
+      ci[2 * p + 1] = pk_deq<VT>(hi >> 16);
     }
   }
 }
@@ -2028,9 +2034,12 @@ __global__ __launch_bounds__(256) void pou_pack_kernel(const void* __restrict__ 
   if (VT == 0) {
     const float* f = reinterpret_cast<const float*>(F);
     *reinterpret_cast<float2*>(dst) = in ? make_float2(f[o], f[o + plane]) : make_float2(0.f, 0.f);
-  } else {
+  } else {  // element (z, y, x): the entries of rows y and y + 1 (row P + 2 is zero)
     const uint16_t* h = reinterpret_cast<const uint16_t*>(F);
-    *reinterpret_cast<uint32_t*>(dst) = in ? uint32_t(h[o]) | (uint32_t(h[o + plane]) << 16) : 0u;
+    const bool in1 = unsigned(x) < unsigned(P) && unsigned(y + 1) < unsigned(P) && unsigned(z) < unsigned(P);
+    const size_t o1 = size_t(b) * 2 * plane + (size_t(in1 ? z : 0) * P + (in1 ? y + 1 : 0)) * P + (in1 ? x : 0);
+    *reinterpret_cast<uint2*>(dst) = make_uint2(in ? uint32_t(h[o]) | (uint32_t(h[o + plane]) << 16) : 0u,
+                                                in1 ? uint32_t(h[o1]) | (uint32_t(h[o1 + plane]) << 16) : 0u);
   }
 }
 

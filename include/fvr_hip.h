@@ -380,8 +380,10 @@ int fvr_pou_lds_ok(int32_t P);
 int fvr_pou_set_mirror_queue(int32_t enabled);
 /* Gather layout of the fused PoU kernels: per block (P+3) x (P+2) x (P+2) elements
  * (z, y, x; storage index = logical + 1, zero margins, one extra zero plane on the
- * slow axis), each element the (re, im) pair -- float2 (dtype 0) or two 16-bit
- * values in a 32-bit word, re in the low half (dtype 1 f16 / 2 bf16).  No
+ * slow axis), 8 bytes each: the (re, im) pair as float2 (dtype 0), or (dtype 1 f16 /
+ * 2 bf16) the (re, im) words of rows y and y + 1 (16-bit re in the low half of a 32-bit
+ * word, im in the high half; row P + 2 reads zero), so that one 16-byte load returns the
+ * four (y, x) corners of a cell in one plane.  No
  * counterpart in the reference (its kernel reads the planar (B, 2, P, P, P) array
  * of cone_beam_fvr.py:291-294 directly); bytes for B blocks, -1 on bad arguments: */
 int64_t fvr_pou_packed_bytes(int32_t dtype, int32_t B, int32_t P);

@@ -380,8 +380,9 @@ def test_reference_presets_and_offsets():
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float16, torch.bfloat16])
 def test_packed_block_layout_image(dtype):
     """The fused kernels' gather layout, read back and decoded by the header's definition:
-    per block (P+3) x (P+2) x (P+2) elements, storage index = logical + 1, zero margins,
-    element = (re, im) (float2, or 16-bit re in the low half and im in the high half)."""
+    per block (P+3) x (P+2) x (P+2) elements, storage index = logical + 1, zero margins;
+    element (z, y, x) = (re, im) as float2, or for 16-bit spectra the (re, im) words (16-bit re
+    in the low half, im in the high half) of rows y and y + 1 (row P + 2 zero)."""
     from fvr import _lib
     L = _lib.lib()
     dev = torch.device("cuda", 0)
@@ -390,21 +391,22 @@ def test_packed_block_layout_image(dtype):
     F = torch.randn((B, 2, P, P, P), generator=g).to(dtype).to(dev)
     code = {torch.float32: 0, torch.float16: 1, torch.bfloat16: 2}[dtype]
     nb = int(L.fvr_pou_packed_bytes(code, B, P))
-    esz = 8 if code == 0 else 4
-    assert nb == B * (P + 3) * (P + 2) * (P + 2) * esz
+    assert nb == B * (P + 3) * (P + 2) * (P + 2) * 8
     buf = torch.full((nb,), 0x5A, dtype=torch.uint8, device=dev)  # every byte must be written
     _lib.check(L.fvr_pou_pack_blocks(F.data_ptr(), code, B, P, buf.data_ptr(), _lib.stream_ptr(dev)), "pack")
     torch.cuda.synchronize()
     raw = buf.cpu()
+    ent = torch.zeros((B, P + 3, P + 3, P + 2, 2), dtype=dtype)  # one extra zero row
+    Fc = F.cpu()
+    ent[:, 1:P + 1, 1:P + 1, 1:P + 1, 0] = Fc[:, 0]
+    ent[:, 1:P + 1, 1:P + 1, 1:P + 1, 1] = Fc[:, 1]
     if code == 0:
         got = raw.view(torch.float32).view(B, P + 3, P + 2, P + 2, 2)
+        assert torch.equal(got, ent[:, :, :P + 2])
     else:
-        got = raw.view(dtype).view(B, P + 3, P + 2, P + 2, 2)
-    exp = torch.zeros((B, P + 3, P + 2, P + 2, 2), dtype=dtype)
-    Fc = F.cpu()
-    exp[:, 1:P + 1, 1:P + 1, 1:P + 1, 0] = Fc[:, 0]
-    exp[:, 1:P + 1, 1:P + 1, 1:P + 1, 1] = Fc[:, 1]
-    assert torch.equal(got, exp)
+        got = raw.view(dtype).view(B, P + 3, P + 2, P + 2, 2, 2)
+        assert torch.equal(got[..., 0, :], ent[:, :, :P + 2])
+        assert torch.equal(got[..., 1, :], ent[:, :, 1:P + 3])
 
 
 @pytest.mark.parametrize("block_res,H,W", [(8, 20, 22), (16, 20, 22), (32, 64, 48), (64, 96, 80)])
