@@ -1383,8 +1383,7 @@ __global__ __launch_bounds__(kPouSliceThreads) void pou_render_bwd_kernel(
 // ---------------------------------------------------------------------------
 constexpr int kPouSepThreads = 512;
 constexpr int kPouSepWaves = kPouSepThreads / 64;
-constexpr int kSepNX = 16;     // pixels per texel column held in registers by the unstaged form, and the
-                               // staged form's zero pad (its unrolled widths go to 12; wider: per-x path)
+constexpr int kSepNX = 16;     // pixels per texel column held in registers (wider footprints: per-x path)
 constexpr int kSepChunks = 2;  // 256-float chunks of a wave's row window in the staged form (W <= 512)
 constexpr int kSepOrderPose = 1;  // launch option: workgroups in pose-major order
 
@@ -1735,12 +1734,8 @@ __global__ __launch_bounds__(kPouSepThreads, 4) void pou_render_bwd_sep_kernel(
         }
         flush();
       };
-      // (a texel column's footprint is 2 texel widths: 2-3 pixels where a pixel spans about one
-      // texel, the usual case; the set 4 / 8 / 12 keeps the kernel within 128 VGPRs without
-      // spills, which 4 / 8 / 16 does not)
-      if (nxw <= 4) run(std::integral_constant<int, 4>{});
-      else if (nxw <= 8) run(std::integral_constant<int, 8>{});
-      else if (nxw <= 12) run(std::integral_constant<int, 12>{});
+      if (nxw <= 8) run(std::integral_constant<int, 8>{});
+      else if (nxw <= kSepNX) run(std::integral_constant<int, kSepNX>{});
       else run(std::integral_constant<int, 0>{});
     }
   } else if (active) {
