@@ -100,7 +100,10 @@ def test_entry_points_reject_bad_arguments_without_a_gpu():
 def test_pou_packed_size_and_argument_checks():
     from fvr import _lib
     L = _lib.lib()
-    assert L.fvr_pou_packed_bytes(2, 55, 128) == 55 * 131 * 130 * 130 * 4
+    # (P + 3) x (P + 2)^2 elements of 8 bytes for every dtype: a float2, or the 16-bit
+    # (re, im) words of two mid-axis rows
+    assert L.fvr_pou_packed_bytes(2, 55, 128) == 55 * 131 * 130 * 130 * 8
+    assert L.fvr_pou_packed_bytes(1, 3, 32) == 3 * 35 * 34 * 34 * 8
     assert L.fvr_pou_packed_bytes(0, 1, 16) == 19 * 18 * 18 * 8
     assert L.fvr_pou_packed_bytes(3, 1, 16) == -1 and L.fvr_pou_packed_bytes(1, 0, 16) == -1
     assert L.fvr_pou_pack_blocks(0, 1, 1, 16, 0, None) == 1
